@@ -1,0 +1,548 @@
+// C ABI of libgtoric.so (declared in include/gtoric.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/gtoric.h"
+#include "gt_launch.h"
+
+using namespace gt;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return -1;
+}
+
+#define GT_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) return fail("%s: %s", #call, hipGetErrorString(e_));    \
+  } while (0)
+
+#define GT_OK(call)            \
+  do {                         \
+    int r_ = (call);           \
+    if (r_ != 0) return r_;    \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return 0;
+    if (p) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      cap = 0;
+      if (e != hipSuccess) return fail("hipFree: %s", hipGetErrorString(e));
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    want = (want + 255) & ~size_t(255);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) { p = nullptr; return fail("hipMalloc(%zu): %s", want, hipGetErrorString(e)); }
+    cap = want;
+    return 0;
+  }
+  template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+};
+
+// ------------------------------------------------------------- GF(8) ----
+// Field GF(2)[x]/(x^3 + x + 1) with elements 0..7 in the polynomial basis.
+int gf8_mul(int a, int b) {
+  int r = 0;
+  for (int i = 0; i < 3; ++i) {
+    if (b & 1) r ^= a;
+    a <<= 1;
+    if (a & 8) a ^= 0xB;
+    b >>= 1;
+  }
+  return r;
+}
+int gf8_pow(int a, int e) {
+  int r = 1;
+  for (int i = 0; i < e; ++i) r = gf8_mul(r, a);
+  return r;
+}
+
+// Bit planes of the 49 evaluation rows: row i = (a, b) = divmod(i, 7)
+// evaluated at torus point j = (x, y) = (j / 7 + 1, j % 7 + 1).
+void build_mplanes(uint32_t out[kNpts * kRowWords]) {
+  std::memset(out, 0, sizeof(uint32_t) * kNpts * kRowWords);
+  for (int i = 0; i < kNpts; ++i) {
+    const int a = i / 7, b = i % 7;
+    uint64_t pl[3] = {0, 0, 0};
+    for (int j = 0; j < kNpts; ++j) {
+      const int x = j / 7 + 1, y = j % 7 + 1;
+      const int v = gf8_mul(gf8_pow(x, a), gf8_pow(y, b));
+      for (int p = 0; p < 3; ++p)
+        if ((v >> p) & 1) pl[p] |= 1ull << j;
+    }
+    for (int p = 0; p < 3; ++p) {
+      out[i * kRowWords + 2 * p] = (uint32_t)pl[p];
+      out[i * kRowWords + 2 * p + 1] = (uint32_t)(pl[p] >> 32);
+    }
+  }
+}
+
+// GL2(F7) as permutations of the 49 points, 52-byte stride, matrices
+// (a, b, c, d) with a*d - b*c != 0 mod 7 in lexicographic order; point
+// (x, y) = divmod(p, 7) maps to ((a x + b y) % 7) * 7 + (c x + d y) % 7.
+std::vector<uint8_t> build_gl2_perm() {
+  std::vector<uint8_t> t(2016 * 52, 0);
+  int n = 0;
+  for (int a = 0; a < 7; ++a)
+    for (int b = 0; b < 7; ++b)
+      for (int c = 0; c < 7; ++c)
+        for (int d = 0; d < 7; ++d) {
+          if (((a * d - b * c) % 7 + 7) % 7 == 0) continue;
+          for (int p = 0; p < kNpts; ++p) {
+            const int x = p / 7, y = p % 7;
+            t[n * 52 + p] = (uint8_t)(((a * x + b * y) % 7) * 7 + (c * x + d * y) % 7);
+          }
+          ++n;
+        }
+  return t;
+}
+
+// Default Gray walk bits per k: big enough that per-chunk set-up is a few
+// percent of the walk, small enough for a balanced tail.
+int default_L(int k) {
+  static const uint8_t tab[kMaxKExact + 1] = {4, 4, 4, 4, 4, 4, 7, 8, 10, 11, 12, 13,
+                                              14, 15, 16, 17, 18, 19, 20, 20, 20, 20};
+  return tab[k];
+}
+
+struct ProfRec {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+
+}  // namespace
+
+struct gt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  int cu_count = 0;
+  int clock_khz = 0;
+  int lds_per_block = 0;
+  std::string name;
+  int md_grid[2] = {0, 0};
+  MdParams params{};
+  bool params_dirty = true;
+
+  DevBuf mplanes, params_buf, perm, ctrl;
+  DevBuf idx, offs, tmax, rows, cum, best, out, zmax, stopf;  // min distance
+  DevBuf sbuf;                                                // single code
+  DevBuf masks, order, hist, klo, khi, canon, stab;            // canonical
+
+  bool prof = false;
+  std::map<std::string, ProfRec> prof_rec;
+
+  int activate() {
+    GT_HIP(hipSetDevice(device));
+    return 0;
+  }
+  hipStream_t pick(void* s) const { return s ? reinterpret_cast<hipStream_t>(s) : stream; }
+
+  int upload_params(hipStream_t s) {
+    if (!params_dirty) return 0;
+    GT_HIP(hipMemcpyAsync(params_buf.p, &params, sizeof(MdParams), hipMemcpyHostToDevice, s));
+    GT_HIP(hipStreamSynchronize(s));
+    params_dirty = false;
+    return 0;
+  }
+
+  int prof_begin(const char* k, hipStream_t s, hipEvent_t* e0) {
+    if (!prof) return 0;
+    GT_HIP(hipEventCreate(e0));
+    GT_HIP(hipEventRecord(*e0, s));
+    (void)k;
+    return 0;
+  }
+  int prof_end(const char* k, hipStream_t s, hipEvent_t e0) {
+    if (!prof) return 0;
+    hipEvent_t e1;
+    GT_HIP(hipEventCreate(&e1));
+    GT_HIP(hipEventRecord(e1, s));
+    prof_rec[k].ev.emplace_back(e0, e1);
+    return 0;
+  }
+
+  int check_ctrl(hipStream_t s) {
+    uint32_t h[kCtrlWords];
+    GT_HIP(hipMemcpyAsync(h, ctrl.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+    if (h[2] != 0) {
+      const uint32_t zero = 0;
+      GT_HIP(hipMemcpyAsync(ctrl.as<uint32_t>() + 2, &zero, 4, hipMemcpyHostToDevice, s));
+      GT_HIP(hipStreamSynchronize(s));
+      if (h[2] & kErrBadIndex) return fail("input error: index outside 0..48 or repeated in a set");
+      return fail("input error: set size outside the supported range, or too much work for one call");
+    }
+    return 0;
+  }
+
+  // ---- min distance on device pointers (stream-ordered, no host sync) ----
+  int md_enqueue(const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets, int32_t n,
+                 const int32_t* d_tmax, int32_t tmax_s, bool thresholded,
+                 int64_t sample_count, uint64_t seed, int32_t* d_out, hipStream_t s) {
+    if (n <= 0) return 0;
+    GT_OK(rows.ensure((size_t)std::max<int64_t>(n_idx, 1) * 3 * kRowWords * 4));
+    GT_OK(upload_params(s));
+    GT_HIP(launch_md_rows(d_idx, n_idx, mplanes.as<uint32_t>(), rows.as<uint32_t>(),
+                          ctrl.as<uint32_t>(), s));
+    if (sample_count > 0) {
+      GT_OK(zmax.ensure((size_t)n * 4));
+      GT_OK(stopf.ensure((size_t)n * 4));
+      hipEvent_t e0 = nullptr;
+      GT_OK(prof_begin("md_sampled", s, &e0));
+      GT_HIP(launch_md_sampled(rows.as<uint32_t>(), d_offsets, n, d_tmax, tmax_s, sample_count,
+                               seed, zmax.as<int32_t>(), stopf.as<uint32_t>(), d_out,
+                               ctrl.as<uint32_t>(), s));
+      GT_OK(prof_end("md_sampled", s, e0));
+      return 0;
+    }
+    GT_OK(cum.ensure((size_t)(n + 1) * 4));
+    GT_OK(best.ensure((size_t)n * 4));
+    GT_HIP(launch_md_plan(d_offsets, n, params_buf.as<MdParams>(), cum.as<uint32_t>(),
+                          best.as<uint32_t>(), ctrl.as<uint32_t>(), s));
+    hipEvent_t e0 = nullptr;
+    GT_OK(prof_begin("md_main", s, &e0));
+    GT_HIP(launch_md_main(thresholded, md_grid[thresholded ? 1 : 0], rows.as<uint32_t>(),
+                          d_offsets, cum.as<uint32_t>(), n, best.as<uint32_t>(), d_tmax, tmax_s,
+                          ctrl.as<uint32_t>(), params_buf.as<MdParams>(), s));
+    GT_OK(prof_end("md_main", s, e0));
+    GT_HIP(launch_md_finalize(d_offsets, n, best.as<uint32_t>(), d_tmax, tmax_s, d_out, s));
+    return 0;
+  }
+};
+
+extern "C" {
+
+const char* gt_last_error(void) { return g_err.c_str(); }
+
+int gt_device_count(int32_t* n) {
+  if (!n) return fail("null output pointer");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  *n = (e == hipSuccess) ? c : 0;
+  return 0;
+}
+
+int gt_open(int32_t device, gt_ctx** out) {
+  if (!out) return fail("gt_open: null output pointer");
+  *out = nullptr;
+  int ndev = 0;
+  GT_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail("gt_open: device %d not present (%d devices)", device, ndev);
+  gt_ctx* c = new gt_ctx();
+  c->device = device;
+  auto bail = [&](int r) { delete c; return r; };
+  if (c->activate()) return bail(-1);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return bail(fail("hipGetDeviceProperties failed"));
+  c->cu_count = prop.multiProcessorCount;
+  c->clock_khz = prop.clockRate;
+  c->lds_per_block = (int)prop.sharedMemPerBlock;
+  c->name = prop.gcnArchName;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail("hipStreamCreate failed"));
+  for (int k = 0; k <= kMaxKExact; ++k) {
+    c->params.L[k] = (uint8_t)default_L(k);
+    c->params.check_log2[k] = 2;
+  }
+  c->md_grid[0] = c->cu_count * md_main_blocks_per_cu(false);
+  c->md_grid[1] = c->cu_count * md_main_blocks_per_cu(true);
+  uint32_t mp[kNpts * kRowWords];
+  build_mplanes(mp);
+  std::vector<uint8_t> perm = build_gl2_perm();
+  if (c->mplanes.ensure(sizeof(mp)) || c->params_buf.ensure(sizeof(MdParams)) ||
+      c->perm.ensure(perm.size()) || c->ctrl.ensure(kCtrlWords * 4))
+    return bail(-1);
+  uint32_t zeros[kCtrlWords] = {0};
+  if (hipMemcpy(c->mplanes.p, mp, sizeof(mp), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->perm.p, perm.data(), perm.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->ctrl.p, zeros, sizeof(zeros), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->params_buf.p, &c->params, sizeof(MdParams), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail("gt_open: upload of constant tables failed"));
+  c->params_dirty = false;
+  *out = c;
+  return 0;
+}
+
+int gt_close(gt_ctx* c) {
+  if (!c) return 0;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->prof_rec)
+      for (auto& pr : kv.second.ev) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    DevBuf* bufs[] = {&c->mplanes, &c->params_buf, &c->perm, &c->ctrl, &c->idx, &c->offs,
+                      &c->tmax, &c->rows, &c->cum, &c->best, &c->out, &c->zmax, &c->stopf,
+                      &c->sbuf, &c->masks, &c->order, &c->hist, &c->klo, &c->khi,
+                      &c->canon, &c->stab};
+    for (DevBuf* b : bufs) b->release();
+    (void)hipStreamDestroy(c->stream);
+  }
+  delete c;
+  return 0;
+}
+
+int gt_device_info(gt_ctx* c, int32_t* cu_count, int32_t* clock_khz, int32_t* lds_per_block,
+                   char* name, int32_t name_len) {
+  if (!c) return fail("null context");
+  if (cu_count) *cu_count = c->cu_count;
+  if (clock_khz) *clock_khz = c->clock_khz;
+  if (lds_per_block) *lds_per_block = c->lds_per_block;
+  if (name && name_len > 0) {
+    std::strncpy(name, c->name.c_str(), (size_t)name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  return 0;
+}
+
+int gt_set_md_params(gt_ctx* c, int32_t k, int32_t L, int32_t check_log2) {
+  if (!c) return fail("null context");
+  if (k < 0 || k > kMaxKExact) return fail("k=%d outside 0..%d", k, kMaxKExact);
+  if (L < kU || L > 30) return fail("L=%d outside %d..30", L, kU);
+  if (check_log2 < 0 || check_log2 > 16) return fail("check_log2=%d outside 0..16", check_log2);
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->params.L[k] = (uint8_t)L;
+  c->params.check_log2[k] = (uint8_t)check_log2;
+  c->params_dirty = true;
+  return 0;
+}
+
+int gt_get_md_params(gt_ctx* c, int32_t k, int32_t* L, int32_t* check_log2) {
+  if (!c) return fail("null context");
+  if (k < 0 || k > kMaxKExact) return fail("k=%d outside 0..%d", k, kMaxKExact);
+  if (L) *L = c->params.L[k];
+  if (check_log2) *check_log2 = c->params.check_log2[k];
+  return 0;
+}
+
+int gt_md_batch(gt_ctx* c, const int32_t* idx, const int32_t* offsets, int32_t n,
+                const int32_t* tmax_per_set, int32_t tmax, int64_t sample_count,
+                uint64_t sample_seed, int32_t* out) {
+  if (!c) return fail("null context");
+  if (n < 0) return fail("negative batch size");
+  if (n == 0) return 0;
+  if (!offsets || !out) return fail("null offsets/out");
+  const int64_t n_idx = offsets[n];
+  if (offsets[0] != 0) return fail("offsets[0] must be 0");
+  bool thresholded = false;
+  uint64_t chunks = 0;
+  for (int32_t s = 0; s < n; ++s) {
+    const int k = offsets[s + 1] - offsets[s];
+    if (k < 0) return fail("offsets must be non-decreasing");
+    if (sample_count > 0 ? k > kMaxK : k > kMaxKExact)
+      return fail(sample_count > 0 ? "sampled evaluation supports k <= 49 (set %d has k=%d)"
+                                   : "exact evaluation supports k <= 21 (set %d has k=%d)", s, k);
+    const int t = tmax_per_set ? tmax_per_set[s] : tmax;
+    if (t < 48) thresholded = true;
+    if (sample_count <= 0 && k > 0) chunks += code_chunks(k, c->params.L[k]);
+  }
+  if (chunks >= (1ull << 31)) return fail("exact search too large for one call (%llu chunks)", (unsigned long long)chunks);
+  for (int64_t i = 0; i < n_idx; ++i)
+    if (idx[i] < 0 || idx[i] >= kNpts) return fail("index %d outside 0..48", idx[i]);
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  hipStream_t s = c->stream;
+  GT_OK(c->idx.ensure((size_t)std::max<int64_t>(n_idx, 1) * 4));
+  GT_OK(c->offs.ensure((size_t)(n + 1) * 4));
+  GT_OK(c->out.ensure((size_t)n * 4));
+  if (tmax_per_set) GT_OK(c->tmax.ensure((size_t)n * 4));
+  if (n_idx > 0) GT_HIP(hipMemcpyAsync(c->idx.p, idx, (size_t)n_idx * 4, hipMemcpyHostToDevice, s));
+  GT_HIP(hipMemcpyAsync(c->offs.p, offsets, (size_t)(n + 1) * 4, hipMemcpyHostToDevice, s));
+  if (tmax_per_set)
+    GT_HIP(hipMemcpyAsync(c->tmax.p, tmax_per_set, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  GT_OK(c->md_enqueue(c->idx.as<int32_t>(), n_idx, c->offs.as<int32_t>(), n,
+                      tmax_per_set ? c->tmax.as<int32_t>() : nullptr, tmax, thresholded,
+                      sample_count, sample_seed, c->out.as<int32_t>(), s));
+  GT_HIP(hipMemcpyAsync(out, c->out.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  GT_OK(c->check_ctrl(s));
+  return 0;
+}
+
+int gt_md_batch_dev(gt_ctx* c, const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets,
+                    int32_t n, const int32_t* d_tmax, int32_t tmax, int32_t thresholded,
+                    int64_t sample_count, uint64_t sample_seed, int32_t* d_out, void* stream) {
+  if (!c) return fail("null context");
+  if (n < 0 || n_idx < 0) return fail("negative sizes");
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  return c->md_enqueue(d_idx, n_idx, d_offsets, n, d_tmax, tmax, thresholded != 0, sample_count,
+                       sample_seed, d_out, c->pick(stream));
+}
+
+int gt_md_single(gt_ctx* c, const int32_t* idx, int32_t k, int32_t tmax, int32_t* out) {
+  if (!c) return fail("null context");
+  if (!out) return fail("null out");
+  if (k <= 0) { *out = 0; return 0; }
+  if (k > kMaxKExact) return fail("exact evaluation supports k <= 21 (k=%d)", k);
+  for (int i = 0; i < k; ++i)
+    if (idx[i] < 0 || idx[i] >= kNpts) return fail("index %d outside 0..48", idx[i]);
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  hipStream_t s = c->stream;
+  // Host-built bit rows + the running minimum in one upload.
+  uint32_t mp[kNpts * kRowWords];
+  build_mplanes(mp);
+  std::vector<uint32_t> h((size_t)3 * k * kRowWords + kRowWords, 0u);
+  for (int i = 0; i < k; ++i) {
+    uint32_t w[6];
+    for (int q = 0; q < 6; ++q) w[q] = mp[idx[i] * kRowWords + q];
+    for (int b = 0; b < 3; ++b) {
+      for (int q = 0; q < 6; ++q) h[(size_t)(3 * i + b) * kRowWords + q] = w[q];
+      mul_x(w);
+    }
+  }
+  const size_t best_off = (size_t)3 * k * kRowWords;
+  h[best_off] = 64u;
+  // One-code launches want many waves: about 2^11 chunks for the largest piece.
+  const int L = std::max<int>(kU, std::min<int>(c->params.L[k], 3 * (k - 1) - kLaneBits - 11));
+  const int thr = 48 - tmax;
+  GT_OK(c->sbuf.ensure(h.size() * 4));
+  GT_HIP(hipMemcpyAsync(c->sbuf.p, h.data(), h.size() * 4, hipMemcpyHostToDevice, s));
+  uint32_t* d_best = c->sbuf.as<uint32_t>() + best_off;
+  hipEvent_t e0 = nullptr;
+  GT_OK(c->prof_begin("md_single", s, &e0));
+  GT_HIP(launch_md_single(tmax < 48, c->sbuf.as<uint32_t>(), k, L, c->params.check_log2[k], thr,
+                          d_best, s));
+  GT_OK(c->prof_end("md_single", s, e0));
+  uint32_t bw = 0;
+  GT_HIP(hipMemcpyAsync(&bw, d_best, 4, hipMemcpyDeviceToHost, s));
+  GT_HIP(hipStreamSynchronize(s));
+  *out = ((int)bw <= thr) ? tmax + 1 : 49 - (int)bw;
+  return 0;
+}
+
+static int canon_enqueue(gt_ctx* c, const uint64_t* d_masks, int32_t n, uint64_t* d_lo,
+                         uint64_t* d_hi, uint64_t* d_canon, int32_t* d_stab, hipStream_t s) {
+  if (n <= 0) return 0;
+  GT_OK(c->order.ensure(((size_t)n + (kMaxKCanon + 1) * kWave) * 4));
+  GT_OK(c->hist.ensure(64 * 4));
+  hipEvent_t e0 = nullptr;
+  GT_OK(c->prof_begin("canon", s, &e0));
+  GT_HIP(launch_canon(d_masks, n, c->perm.as<uint8_t>(), c->order.as<uint32_t>(),
+                      c->hist.as<uint32_t>(), d_lo, d_hi, d_canon, d_stab, c->ctrl.as<uint32_t>(), s));
+  GT_OK(c->prof_end("canon", s, e0));
+  return 0;
+}
+
+static int canon_host(gt_ctx* c, const uint64_t* masks, const int32_t* sets, int32_t n, int32_t k,
+                      uint64_t* key_lo, uint64_t* key_hi, uint64_t* canon_mask, int32_t* stab) {
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  hipStream_t s = c->stream;
+  GT_OK(c->masks.ensure((size_t)n * 8));
+  GT_OK(c->klo.ensure((size_t)n * 8));
+  GT_OK(c->khi.ensure((size_t)n * 8));
+  GT_OK(c->canon.ensure((size_t)n * 8));
+  GT_OK(c->stab.ensure((size_t)n * 4));
+  if (sets) {
+    GT_OK(c->idx.ensure((size_t)n * k * 4));
+    GT_HIP(hipMemcpyAsync(c->idx.p, sets, (size_t)n * k * 4, hipMemcpyHostToDevice, s));
+    GT_HIP(launch_sets_to_masks(c->idx.as<int32_t>(), n, k, c->masks.as<uint64_t>(),
+                                c->ctrl.as<uint32_t>(), s));
+  } else {
+    GT_HIP(hipMemcpyAsync(c->masks.p, masks, (size_t)n * 8, hipMemcpyHostToDevice, s));
+  }
+  GT_OK(canon_enqueue(c, c->masks.as<uint64_t>(), n, c->klo.as<uint64_t>(), c->khi.as<uint64_t>(),
+                      c->canon.as<uint64_t>(), c->stab.as<int32_t>(), s));
+  if (key_lo) GT_HIP(hipMemcpyAsync(key_lo, c->klo.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+  if (key_hi) GT_HIP(hipMemcpyAsync(key_hi, c->khi.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+  if (canon_mask) GT_HIP(hipMemcpyAsync(canon_mask, c->canon.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+  if (stab) GT_HIP(hipMemcpyAsync(stab, c->stab.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  GT_OK(c->check_ctrl(s));
+  return 0;
+}
+
+int gt_canon_batch(gt_ctx* c, const int32_t* sets, int32_t n, int32_t k, uint64_t* key_lo,
+                   uint64_t* key_hi, uint64_t* canon_mask, int32_t* stab) {
+  if (!c) return fail("null context");
+  if (k < 1 || k > kMaxKCanon) return fail("canonical form supports 1 <= k <= 16 (k=%d)", k);
+  if (n < 0) return fail("negative batch size");
+  if (n == 0) return 0;
+  if (!sets) return fail("null sets");
+  return canon_host(c, nullptr, sets, n, k, key_lo, key_hi, canon_mask, stab);
+}
+
+int gt_canon_masks(gt_ctx* c, const uint64_t* masks, int32_t n, uint64_t* key_lo, uint64_t* key_hi,
+                   uint64_t* canon_mask, int32_t* stab) {
+  if (!c) return fail("null context");
+  if (n < 0) return fail("negative batch size");
+  if (n == 0) return 0;
+  if (!masks) return fail("null masks");
+  return canon_host(c, masks, nullptr, n, 0, key_lo, key_hi, canon_mask, stab);
+}
+
+int gt_canon_masks_dev(gt_ctx* c, const uint64_t* d_masks, int32_t n, uint64_t* d_key_lo,
+                       uint64_t* d_key_hi, uint64_t* d_canon_mask, int32_t* d_stab, void* stream) {
+  if (!c) return fail("null context");
+  if (!d_key_lo || !d_key_hi || !d_canon_mask || !d_stab) return fail("null device output");
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  return canon_enqueue(c, d_masks, n, d_key_lo, d_key_hi, d_canon_mask, d_stab, c->pick(stream));
+}
+
+int gt_sync(gt_ctx* c, void* stream) {
+  if (!c) return fail("null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  hipStream_t s = c->pick(stream);
+  GT_HIP(hipStreamSynchronize(s));
+  return c->check_ctrl(s);
+}
+
+int gt_prof_enable(gt_ctx* c, int32_t on) {
+  if (!c) return fail("null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->prof = on != 0;
+  return 0;
+}
+
+int gt_prof_read(gt_ctx* c, const char* kernel, int32_t* count, double* total_ms) {
+  if (!c || !kernel) return fail("null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  auto it = c->prof_rec.find(kernel);
+  int32_t n = 0;
+  double tot = 0.0;
+  if (it != c->prof_rec.end()) {
+    for (auto& pr : it->second.ev) {
+      GT_HIP(hipEventSynchronize(pr.second));
+      float ms = 0.f;
+      GT_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+      tot += ms;
+      ++n;
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+    it->second.ev.clear();
+  }
+  if (count) *count = n;
+  if (total_ms) *total_ms = tot;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+// Shared definitions for the gfx950 exact-search core (host and device).
+//
+// Codes.  A code is given by k exponent points S_0..S_{k-1} in (Z/7)^2, index
+// i <-> (a, b) = divmod(i, 7); its generator rows are rows S_i of the 49x49
+// evaluation matrix M[i, j] = x_j^a * y_j^b over GF(8) = GF(2)[x]/(x^3+x+1)
+// (precompute.py:17-43, gf8.py:9-38 in the reference).
+//
+// Bit planes.  A GF(8) vector of length 49 is three 49-bit planes (bit p of
+// every coordinate).  It is held as six 32-bit words
+//   w0 = plane0[0:32]  w1 = plane0[32:49]  w2 = plane1[0:32]
+//   w3 = plane1[32:49] w4 = plane2[0:32]   w5 = plane2[32:49]
+// so the Hamming weight is popc(w0|w2|w4) + popc(w1|w3|w5).  In memory a
+// vector ("row") takes kRowWords = 8 words (32 bytes, two pad words) so one
+// scalar load fetches it.
+//
+// Bit rows.  Row r = 3*i + b of a code is x^b * g_i (b = 0, 1, 2).  Multiplying
+// by x is the fixed plane mix (p0, p1, p2) -> (p2, p0 ^ p2, p1), because
+// x * (c0 + c1 x + c2 x^2) = c2 + (c0 + c2) x + c1 x^2 modulo x^3 + x + 1.
+//
+// Projective enumeration.  Scalar multiples have the same zero set, so only
+// codewords whose last nonzero coefficient is 1 are visited: piece j (0 <= j <
+// k) is g_j + span_GF(2){rows 0 .. 3j-1}, 8^j codewords, walked in reflected
+// Gray order so that each step adds one bit row.  In total (8^k - 1) / 7
+// codewords instead of the reference's 8^k - 1 (kernel.py:128-139).
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define GT_HD __host__ __device__ __forceinline__
+#else
+#define GT_HD inline
+#endif
+
+namespace gt {
+
+constexpr int kNpts = 49;
+constexpr int kRowWords = 8;
+constexpr int kMaxKExact = 21;   // kernel.py:246-247, 290-291
+constexpr int kMaxK = 49;        // kernel.py:288-289 (sampled)
+constexpr int kMaxKCanon = 16;   // kernel_canon.py:161-162
+constexpr int kWave = 64;
+constexpr int kLaneBits = 6;     // 64 lanes own 6 Gray index bits
+
+// Per-k launch parameters chosen by the autotuner.
+struct MdParams {
+  uint8_t L[kMaxKExact + 1];      // Gray walk bits per lane for full chunks
+  uint8_t check_log2[kMaxKExact + 1];  // early-exit check every 2^x blocks
+};
+
+// Static unroll of the inner Gray walk: 2^kU steps per block with rows
+// 0..kU-1 held in scalar registers.
+constexpr int kU = 4;
+// Pieces j with 3j < kLaneBits + kU are enumerated directly by the "tiny" chunk.
+constexpr int kTinyTop = (kLaneBits + kU - 1) / 3;   // largest tiny j
+
+GT_HD int tiny_top(int k) { return (k - 1 < kTinyTop) ? (k - 1) : kTinyTop; }
+
+// Number of chunks of Gray piece j with walk length L (L >= kU).
+GT_HD uint64_t piece_chunks(int j, int L) {
+  int B = 3 * j;
+  return (B >= kLaneBits + L) ? (1ull << (B - kLaneBits - L)) : 1ull;
+}
+GT_HD int piece_L(int j, int L) {
+  int B = 3 * j;
+  return (B >= kLaneBits + L) ? L : (B - kLaneBits);
+}
+
+// Chunks of one code: Gray pieces j = k-1 .. tiny_top(k)+1, then one tiny chunk.
+GT_HD uint64_t code_chunks(int k, int L) {
+  if (k <= 0) return 0;
+  uint64_t n = 1;  // tiny chunk
+  for (int j = k - 1; j > tiny_top(k); --j) n += piece_chunks(j, L);
+  return n;
+}
+
+// Projective codewords of a k-dimensional code: (8^k - 1) / 7.
+GT_HD uint64_t proj_codewords(int k) {
+  return (k <= 0) ? 0 : (((1ull << (3 * k)) - 1ull) / 7ull);
+}
+
+// x * v for a bit-plane vector (six words).
+GT_HD void mul_x(uint32_t w[6]) {
+  uint32_t p0lo = w[0], p0hi = w[1], p1lo = w[2], p1hi = w[3], p2lo = w[4], p2hi = w[5];
+  w[0] = p2lo;        w[1] = p2hi;
+  w[2] = p0lo ^ p2lo; w[3] = p0hi ^ p2hi;
+  w[4] = p1lo;        w[5] = p1hi;
+}
+
+// Error bits reported through the control block.
+enum : uint32_t {
+  kErrBadIndex = 1u,
+  kErrBadK = 2u,
+};
+
+}  // namespace gt

@@ -1,0 +1,44 @@
+// Host-side launch wrappers for the gfx950 kernels (one translation unit per
+// kernel family; the C-ABI in gt_api.hip calls only these).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "gt_common.h"
+
+namespace gt {
+
+// ---- min distance -------------------------------------------------------
+// ctrl layout (uint32): [0] chunk counter, [1] unused, [2] error bits,
+// [3] total chunks of the last plan, [4..15] reserved.
+constexpr int kCtrlWords = 16;
+
+hipError_t launch_md_rows(const int32_t* d_idx, int64_t n_idx, const uint32_t* d_mplanes,
+                          uint32_t* d_rows, uint32_t* d_ctrl, hipStream_t s);
+hipError_t launch_md_plan(const int32_t* d_offsets, int32_t n, const MdParams* d_params,
+                          uint32_t* d_cum, uint32_t* d_best, uint32_t* d_ctrl, hipStream_t s);
+hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
+                          const int32_t* d_offsets, const uint32_t* d_cum, int32_t n,
+                          uint32_t* d_best, const int32_t* d_tmax, int32_t tmax,
+                          uint32_t* d_ctrl, const MdParams* d_params, hipStream_t s);
+hipError_t launch_md_finalize(const int32_t* d_offsets, int32_t n, const uint32_t* d_best,
+                              const int32_t* d_tmax, int32_t tmax, int32_t* d_out,
+                              hipStream_t s);
+// One code, one launch: grid = code_chunks(k, L) single-wave workgroups.
+hipError_t launch_md_single(bool thresholded, const uint32_t* d_rows, int k, int L,
+                            int check_log2, int thr, uint32_t* d_best, hipStream_t s);
+hipError_t launch_md_sampled(const uint32_t* d_rows, const int32_t* d_offsets, int32_t n,
+                             const int32_t* d_tmax, int32_t tmax, int64_t sample_count,
+                             uint64_t sample_seed, int32_t* d_zmax, uint32_t* d_stop,
+                             int32_t* d_out, uint32_t* d_ctrl, hipStream_t s);
+int md_main_blocks_per_cu(bool thresholded);
+
+// ---- canonical form ------------------------------------------------------
+hipError_t launch_canon(const uint64_t* d_masks, int32_t n, const uint8_t* d_perm,
+                        uint32_t* d_order, uint32_t* d_hist, uint64_t* d_key_lo,
+                        uint64_t* d_key_hi, uint64_t* d_canon, int32_t* d_stab,
+                        uint32_t* d_ctrl, hipStream_t s);
+hipError_t launch_sets_to_masks(const int32_t* d_sets, int32_t n, int32_t k, uint64_t* d_masks,
+                                uint32_t* d_ctrl, hipStream_t s);
+int canon_perm_bytes();
+
+}  // namespace gt

@@ -1,0 +1,486 @@
+// Exact / thresholded / sampled minimum distance of GF(8) generalized toric
+// codes on gfx950.
+//
+// Replaces eval_min_distance_batch_bp (kernel.py:97-203, kernel_cuda_bp.py:146-238)
+// and eval_min_distance / eval_min_distance_single_bp (kernel.py:63-94,
+// kernel_cuda_bp.py:99-127).  Work decomposition (see gt_common.h):
+//
+//   md_rows     one thread per generator: the three bit rows x^b * g_i.
+//   md_plan     one workgroup: chunk count per code, exclusive scan -> cum[],
+//               reset of the per-code minimum and of the chunk counter.
+//   md_main     persistent: each wave claims a chunk id with one atomic,
+//               finds its code by binary search in cum[] (scalar loads),
+//               walks 64 lanes x 2^L Gray steps, and folds its wave minimum
+//               into the code's minimum with one atomicMin.
+//   md_finalize minimum weight -> max_zeros, clamped at tmax + 1.
+//
+// Inner step (per projective codeword, all VALU, rows in SGPRs):
+//   6 v_xor_b32 (row add), 2 v_or3_b32 (plane OR), 2 v_bcnt_u32_b32,
+//   and one v_min3_u32 per two codewords.
+#include "gt_launch.h"
+
+namespace gt {
+
+struct RowV {
+  uint32_t w[6];
+};
+
+__device__ __forceinline__ RowV load_row(const uint32_t* __restrict__ R, uint32_t r) {
+  const uint32_t* p = R + (size_t)r * kRowWords;
+  const uint4 a = *reinterpret_cast<const uint4*>(p);
+  const uint2 b = *reinterpret_cast<const uint2*>(p + 4);
+  RowV v;
+  v.w[0] = a.x; v.w[1] = a.y; v.w[2] = a.z; v.w[3] = a.w; v.w[4] = b.x; v.w[5] = b.y;
+  return v;
+}
+
+__device__ __forceinline__ void xor_row(RowV& v, const RowV& r) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) v.w[i] ^= r.w[i];
+}
+
+__device__ __forceinline__ void xor_row_masked(RowV& v, const RowV& r, uint32_t m) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) v.w[i] ^= r.w[i] & m;
+}
+
+__device__ __forceinline__ uint32_t weight(const RowV& v) {
+  return __builtin_popcount(v.w[0] | v.w[2] | v.w[4]) +
+         __builtin_popcount(v.w[1] | v.w[3] | v.w[5]);
+}
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off, kWave));
+  return x;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Two Gray steps: add row a, evaluate, add row b, evaluate.
+#define GT_STEP2(A, B)                              \
+  {                                                 \
+    xor_row(v, A);                                  \
+    const uint32_t wa = weight(v);                  \
+    xor_row(v, B);                                  \
+    const uint32_t wb = weight(v);                  \
+    best = min(best, min(wa, wb));                  \
+  }
+
+// One chunk of Gray piece j: lanes own Gray indices (c*64 + lane) * 2^L and
+// walk 2^L consecutive indices.  The walk also evaluates one codeword past the
+// chunk end (row L added at the last step); it is a nonzero codeword of the
+// same piece, so the minimum is unchanged.
+template <bool THR>
+__device__ __forceinline__ uint32_t gray_chunk(const uint32_t* __restrict__ R, uint32_t j,
+                                               uint32_t L, uint64_t c, uint32_t lane, int thr,
+                                               uint32_t check_mask, const uint32_t* best_g) {
+  RowV base = load_row(R, 3 * j);
+  // Chunk-uniform Gray bits above the lane bits.
+  uint64_t gc = c ^ (c >> 1);
+  for (uint32_t r = L + kLaneBits; gc != 0; gc >>= 1, ++r)
+    if (gc & 1) xor_row(base, load_row(R, r));
+  // Lane bits: gray(c*64 + lane) contributes lane ^ (lane >> 1) ^ (c & 1) << 5
+  // at bit L, and gray() of the shifted index sets bit L-1 when lane is odd.
+  const uint32_t lp = lane ^ (lane >> 1) ^ ((uint32_t)(c & 1) << 5);
+  RowV v = base;
+#pragma unroll
+  for (int t = 0; t < kLaneBits; ++t)
+    xor_row_masked(v, load_row(R, L + t), 0u - ((lp >> t) & 1u));
+  xor_row_masked(v, load_row(R, L - 1), 0u - (lane & 1u));
+
+  const RowV s0 = load_row(R, 0), s1 = load_row(R, 1), s2 = load_row(R, 2), s3 = load_row(R, 3);
+  uint32_t best = weight(v);
+  const uint32_t nb = 1u << (L - kU);
+  for (uint32_t m = 0; m < nb; ++m) {
+    const RowV d = load_row(R, kU + __builtin_ctz(m + 1));
+    GT_STEP2(s0, s1)
+    GT_STEP2(s0, s2)
+    GT_STEP2(s0, s1)
+    GT_STEP2(s0, s3)
+    GT_STEP2(s0, s1)
+    GT_STEP2(s0, s2)
+    GT_STEP2(s0, s1)
+    GT_STEP2(s0, d)
+    if (THR && (m & check_mask) == check_mask) {
+      if (__any((int)best <= thr)) break;
+      if ((int)__hip_atomic_load(best_g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= thr) break;
+    }
+  }
+  return best;
+}
+static_assert(kU == 4, "gray_chunk unrolls 16 steps");
+
+// Pieces 0..jt enumerated directly: codeword index t in [0, (8^(jt+1)-1)/7).
+__device__ __forceinline__ uint32_t tiny_chunk(const uint32_t* __restrict__ R, int jt,
+                                               uint32_t lane) {
+  uint32_t best = 64;
+  const uint32_t T = (uint32_t)proj_codewords(jt + 1);
+  for (uint32_t t = lane; t < T; t += kWave) {
+    uint32_t j = 0;
+    while ((int)j < jt && t >= (uint32_t)proj_codewords(j + 1)) ++j;
+    const uint32_t a = t - (uint32_t)proj_codewords(j);
+    RowV v = load_row(R, 3 * j);
+    for (int r = 0; r < 3 * jt; ++r) xor_row_masked(v, load_row(R, r), 0u - ((a >> r) & 1u));
+    best = min(best, weight(v));
+  }
+  return best;
+}
+
+// Map a code-local chunk id to (piece j, chunk c); j == tiny_top(k) means tiny.
+__device__ __forceinline__ void locate_chunk(int k, int L, uint64_t local, int& j, uint64_t& c) {
+  const int jt = tiny_top(k);
+  j = k - 1;
+  for (; j > jt; --j) {
+    const uint64_t pc = piece_chunks(j, L);
+    if (local < pc) break;
+    local -= pc;
+  }
+  c = local;
+}
+
+template <bool THR>
+__device__ __forceinline__ uint32_t run_chunk(const uint32_t* __restrict__ R, int k, int L,
+                                              uint64_t local, uint32_t lane, int thr,
+                                              uint32_t check_mask, const uint32_t* best_g) {
+  int j;
+  uint64_t c;
+  locate_chunk(k, L, local, j, c);
+  if (j > tiny_top(k))
+    return gray_chunk<THR>(R, (uint32_t)j, (uint32_t)piece_L(j, L), c, lane, thr, check_mask,
+                           best_g);
+  return tiny_chunk(R, tiny_top(k), lane);
+}
+
+// --------------------------------------------------------------- kernels --
+
+__global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict__ idx,
+                                                      int64_t n_idx,
+                                                      const uint32_t* __restrict__ mplanes,
+                                                      uint32_t* __restrict__ rows,
+                                                      uint32_t* ctrl) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_idx) return;
+  int g = idx[t];
+  uint32_t w[6] = {0, 0, 0, 0, 0, 0};
+  if (g < 0 || g >= kNpts) {
+    atomicOr(&ctrl[2], kErrBadIndex);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) w[i] = mplanes[g * kRowWords + i];
+  }
+  uint32_t* out = rows + (size_t)t * 3 * kRowWords;
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    uint4* o = reinterpret_cast<uint4*>(out + b * kRowWords);
+    o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    o[1] = make_uint4(w[4], w[5], 0u, 0u);
+    mul_x(w);
+  }
+}
+
+__global__ __launch_bounds__(1024) void md_plan_kernel(const int32_t* __restrict__ offsets,
+                                                       int32_t n,
+                                                       const MdParams* __restrict__ p,
+                                                       uint32_t* __restrict__ cum,
+                                                       uint32_t* __restrict__ best,
+                                                       uint32_t* ctrl) {
+  __shared__ uint64_t s[1024];
+  const int tid = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b = min(n, tid * per), e = min(n, b + per);
+  uint64_t sum = 0;
+  uint32_t err = 0;
+  for (int i = b; i < e; ++i) {
+    const int k = offsets[i + 1] - offsets[i];
+    if (k < 0 || k > kMaxKExact) {
+      err |= kErrBadK;
+    } else if (k > 0) {
+      sum += code_chunks(k, p->L[k]);
+    }
+    best[i] = 64u;
+  }
+  s[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint64_t t = (tid >= off) ? s[tid - off] : 0;
+    __syncthreads();
+    s[tid] += t;
+    __syncthreads();
+  }
+  const uint64_t total = s[1023];
+  const bool overflow = total >= (1ull << 31);
+  uint64_t run = s[tid] - sum;
+  for (int i = b; i < e; ++i) {
+    cum[i] = overflow ? 0u : (uint32_t)run;
+    const int k = offsets[i + 1] - offsets[i];
+    if (k > 0 && k <= kMaxKExact) run += code_chunks(k, p->L[k]);
+  }
+  if (err) atomicOr(&ctrl[2], err);
+  if (tid == 1023) {
+    cum[n] = overflow ? 0u : (uint32_t)total;
+    ctrl[0] = 0u;
+    ctrl[3] = overflow ? 0xFFFFFFFFu : (uint32_t)total;
+    if (overflow) atomicOr(&ctrl[2], kErrBadK);
+  }
+}
+
+template <bool THR>
+__global__ __launch_bounds__(256) void md_main_kernel(
+    const uint32_t* __restrict__ rows, const int32_t* __restrict__ offsets,
+    const uint32_t* __restrict__ cum, int32_t n, uint32_t* best_w,
+    const int32_t* __restrict__ tmax_arr, int32_t tmax_s, uint32_t* ctrl,
+    const MdParams* __restrict__ p) {
+  const uint32_t lane = lane_id();
+  const uint32_t total = cum[n];
+  for (;;) {
+    uint32_t g = 0;
+    if (lane == 0) g = atomicAdd(&ctrl[0], 1u);
+    g = __builtin_amdgcn_readfirstlane(g);
+    if (g >= total) break;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (cum[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const int code = lo;
+    const int k = offsets[code + 1] - offsets[code];
+    const int L = p->L[k];
+    const uint32_t cmask = (1u << p->check_log2[k]) - 1u;
+    const uint32_t* R = rows + (size_t)offsets[code] * 3 * kRowWords;
+    int thr = -1;
+    if (THR) {
+      thr = 48 - (tmax_arr ? tmax_arr[code] : tmax_s);
+      if ((int)__hip_atomic_load(&best_w[code], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= thr)
+        continue;
+    }
+    uint32_t b = run_chunk<THR>(R, k, L, g - cum[code], lane, thr, cmask, &best_w[code]);
+    b = wave_min(b);
+    if (lane == 0) atomicMin(&best_w[code], b);
+  }
+}
+
+__global__ __launch_bounds__(256) void md_finalize_kernel(const int32_t* __restrict__ offsets,
+                                                          int32_t n,
+                                                          const uint32_t* __restrict__ best,
+                                                          const int32_t* __restrict__ tmax_arr,
+                                                          int32_t tmax_s,
+                                                          int32_t* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = offsets[i + 1] - offsets[i];
+  if (k <= 0 || k > kMaxKExact) { out[i] = 0; return; }
+  const int tmax = tmax_arr ? tmax_arr[i] : tmax_s;
+  const int b = (int)best[i];
+  out[i] = (b <= 48 - tmax) ? tmax + 1 : 49 - b;
+}
+
+template <bool THR>
+__global__ __launch_bounds__(64) void md_single_kernel(const uint32_t* __restrict__ R, int k, int L,
+                                                       uint32_t check_mask, int thr,
+                                                       uint32_t* best_out) {
+  const uint32_t lane = lane_id();
+  if (THR && (int)__hip_atomic_load(best_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= thr)
+    return;
+  uint32_t b = run_chunk<THR>(R, k, L, blockIdx.x, lane, thr, check_mask, best_out);
+  b = wave_min(b);
+  if (lane == 0) atomicMin(best_out, b);
+}
+
+// ------------------------------------------------------------- sampled ----
+
+// kernel.py:10-15 (SplitMix64 finalizer, Vigna).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// x mod (2^m - 1), 3 <= m <= 63: 2^m == 1 modulo 2^m - 1, so fold m-bit digits.
+__device__ __forceinline__ uint64_t mod_mersenne(uint64_t x, int m) {
+  const uint64_t M = (1ull << m) - 1ull;
+  while (x > M) x = (x & M) + (x >> m);
+  return (x == M) ? 0ull : x;
+}
+
+constexpr int kSampledBlock = 256;
+
+// Sampled screening (kernel.py:136-178): work_id = 1..sample_count, set_id =
+// index of the code in this call.  Several workgroups share one code; the
+// result min(max_zeros, tmax + 1) does not depend on their schedule.
+__global__ __launch_bounds__(kSampledBlock) void md_sampled_kernel(
+    const uint32_t* __restrict__ rows, const int32_t* __restrict__ offsets, int32_t n,
+    const int32_t* __restrict__ tmax_arr, int32_t tmax_s, uint64_t sample_count,
+    uint64_t seed, uint32_t bpc, int32_t* zmax, uint32_t* stopf) {
+  __shared__ uint32_t bp[kMaxK * 8 * kRowWords];  // [i][v] -> v * g_i planes
+  __shared__ int32_t red[kSampledBlock / kWave];
+  const uint32_t code = blockIdx.x / bpc, part = blockIdx.x % bpc;
+  const int k = offsets[code + 1] - offsets[code];
+  if (k <= 0 || k > kMaxK) return;
+  const uint32_t* R = rows + (size_t)offsets[code] * 3 * kRowWords;
+  for (int e = threadIdx.x; e < k * 8; e += blockDim.x) {
+    const int i = e >> 3, v = e & 7;
+    RowV w = {{0, 0, 0, 0, 0, 0}};
+    for (int b = 0; b < 3; ++b)
+      if ((v >> b) & 1) xor_row(w, load_row(R, 3 * i + b));
+#pragma unroll
+    for (int q = 0; q < 6; ++q) bp[e * kRowWords + q] = w.w[q];
+  }
+  __syncthreads();
+  const int tmax = tmax_arr ? tmax_arr[code] : tmax_s;
+  const bool use_pid = (3 * k <= 63);
+  const int m = 3 * k;
+  int tz = 0;
+  uint32_t it = 0;
+  const uint64_t stride = (uint64_t)bpc * blockDim.x;
+  for (uint64_t wid = 1ull + (uint64_t)part * blockDim.x + threadIdx.x; wid <= sample_count;
+       wid += stride, ++it) {
+    if ((it & 31u) == 0 &&
+        __hip_atomic_load(&stopf[code], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      break;
+    RowV v = {{0, 0, 0, 0, 0, 0}};
+    if (use_pid) {
+      const uint64_t x = seed ^ ((uint64_t)code * 0x9E3779B97F4A7C15ull) ^
+                         (wid * 0xBF58476D1CE4E5B9ull);
+      uint64_t pid = mod_mersenne(splitmix64(x), m) + 1ull;
+      for (int i = 0; i < k; ++i, pid >>= 3) {
+        const uint32_t* src = bp + ((i << 3) + (int)(pid & 7ull)) * kRowWords;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) v.w[q] ^= src[q];
+      }
+    } else {
+      const uint64_t xf = seed ^ ((uint64_t)code * 0xD6E8FEB86659FD93ull) ^
+                          (wid * 0xA5A3564E27F8861Full);
+      const int forced = (int)(splitmix64(xf) % (uint64_t)k);
+      for (int i = 0; i < k; ++i) {
+        const uint64_t x = seed ^ ((uint64_t)code * 0x9E3779B97F4A7C15ull) ^
+                           (wid * 0xBF58476D1CE4E5B9ull) ^
+                           ((uint64_t)(i + 1) * 0x94D049BB133111EBull);
+        int c = (int)(splitmix64(x) & 7ull);
+        if (i == forced && c == 0) c = 1;
+        const uint32_t* src = bp + ((i << 3) + c) * kRowWords;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) v.w[q] ^= src[q];
+      }
+    }
+    const int zeros = kNpts - (int)weight(v);
+    if (zeros > tmax) {
+      atomicMax(&zmax[code], zeros);
+      __hip_atomic_store(&stopf[code], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    tz = max(tz, zeros);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) tz = max(tz, __shfl_xor(tz, off, kWave));
+  if (lane_id() == 0) red[threadIdx.x / kWave] = tz;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int r = red[0];
+    for (int w = 1; w < (int)(blockDim.x / kWave); ++w) r = max(r, red[w]);
+    atomicMax(&zmax[code], r);
+  }
+}
+
+__global__ void md_sampled_init_kernel(int32_t n, int32_t* zmax, uint32_t* stopf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) { zmax[i] = 0; stopf[i] = 0u; }
+}
+
+__global__ void md_sampled_finalize_kernel(const int32_t* __restrict__ offsets, int32_t n,
+                                           const int32_t* __restrict__ zmax,
+                                           const int32_t* __restrict__ tmax_arr, int32_t tmax_s,
+                                           int32_t* __restrict__ out, uint32_t* ctrl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = offsets[i + 1] - offsets[i];
+  if (k < 0 || k > kMaxK) { atomicOr(&ctrl[2], kErrBadK); out[i] = 0; return; }
+  if (k == 0) { out[i] = 0; return; }
+  const int tmax = tmax_arr ? tmax_arr[i] : tmax_s;
+  out[i] = min(zmax[i], tmax + 1);
+}
+
+// ------------------------------------------------------------- launchers --
+
+hipError_t launch_md_rows(const int32_t* d_idx, int64_t n_idx, const uint32_t* d_mplanes,
+                          uint32_t* d_rows, uint32_t* d_ctrl, hipStream_t s) {
+  if (n_idx <= 0) return hipSuccess;
+  const int64_t blocks = (n_idx + 255) / 256;
+  hipLaunchKernelGGL(md_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d_idx, n_idx,
+                     d_mplanes, d_rows, d_ctrl);
+  return hipGetLastError();
+}
+
+hipError_t launch_md_plan(const int32_t* d_offsets, int32_t n, const MdParams* d_params,
+                          uint32_t* d_cum, uint32_t* d_best, uint32_t* d_ctrl, hipStream_t s) {
+  hipLaunchKernelGGL(md_plan_kernel, dim3(1), dim3(1024), 0, s, d_offsets, n, d_params, d_cum,
+                     d_best, d_ctrl);
+  return hipGetLastError();
+}
+
+int md_main_blocks_per_cu(bool thresholded) {
+  int nb = 0;
+  hipError_t e = thresholded
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, md_main_kernel<true>, 256, 0)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, md_main_kernel<false>, 256, 0);
+  if (e != hipSuccess || nb <= 0) nb = 4;
+  return nb;
+}
+
+hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
+                          const int32_t* d_offsets, const uint32_t* d_cum, int32_t n,
+                          uint32_t* d_best, const int32_t* d_tmax, int32_t tmax,
+                          uint32_t* d_ctrl, const MdParams* d_params, hipStream_t s) {
+  if (thresholded)
+    hipLaunchKernelGGL(md_main_kernel<true>, dim3(grid), dim3(256), 0, s, d_rows, d_offsets,
+                       d_cum, n, d_best, d_tmax, tmax, d_ctrl, d_params);
+  else
+    hipLaunchKernelGGL(md_main_kernel<false>, dim3(grid), dim3(256), 0, s, d_rows, d_offsets,
+                       d_cum, n, d_best, d_tmax, tmax, d_ctrl, d_params);
+  return hipGetLastError();
+}
+
+hipError_t launch_md_finalize(const int32_t* d_offsets, int32_t n, const uint32_t* d_best,
+                              const int32_t* d_tmax, int32_t tmax, int32_t* d_out,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(md_finalize_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_offsets, n,
+                     d_best, d_tmax, tmax, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_md_single(bool thresholded, const uint32_t* d_rows, int k, int L,
+                            int check_log2, int thr, uint32_t* d_best, hipStream_t s) {
+  const uint64_t grid = code_chunks(k, L);
+  if (grid == 0) return hipSuccess;
+  if (grid >= (1ull << 31)) return hipErrorInvalidValue;
+  const uint32_t cmask = (1u << check_log2) - 1u;
+  if (thresholded)
+    hipLaunchKernelGGL(md_single_kernel<true>, dim3((unsigned)grid), dim3(64), 0, s, d_rows, k,
+                       L, cmask, thr, d_best);
+  else
+    hipLaunchKernelGGL(md_single_kernel<false>, dim3((unsigned)grid), dim3(64), 0, s, d_rows, k,
+                       L, cmask, thr, d_best);
+  return hipGetLastError();
+}
+
+hipError_t launch_md_sampled(const uint32_t* d_rows, const int32_t* d_offsets, int32_t n,
+                             const int32_t* d_tmax, int32_t tmax, int64_t sample_count,
+                             uint64_t sample_seed, int32_t* d_zmax, uint32_t* d_stop,
+                             int32_t* d_out, uint32_t* d_ctrl, hipStream_t s) {
+  hipLaunchKernelGGL(md_sampled_init_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, d_zmax,
+                     d_stop);
+  // Workgroups per code: about 16 samples per thread, at most 64 groups.
+  uint64_t bpc = (uint64_t)sample_count / (kSampledBlock * 16ull);
+  if (bpc < 1) bpc = 1;
+  if (bpc > 64) bpc = 64;
+  while (bpc > 1 && (uint64_t)n * bpc > (1ull << 30)) bpc >>= 1;
+  hipLaunchKernelGGL(md_sampled_kernel, dim3((unsigned)((uint64_t)n * bpc)), dim3(kSampledBlock),
+                     0, s, d_rows, d_offsets, n, d_tmax, tmax, (uint64_t)sample_count,
+                     sample_seed, (uint32_t)bpc, d_zmax, d_stop);
+  hipLaunchKernelGGL(md_sampled_finalize_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
+                     d_offsets, n, d_zmax, d_tmax, tmax, d_out, d_ctrl);
+  return hipGetLastError();
+}
+
+}  // namespace gt

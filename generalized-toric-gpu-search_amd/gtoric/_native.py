@@ -1,0 +1,220 @@
+"""ctypes binding of libgtoric.so (C ABI declared in include/gtoric.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (``make -C csrc``).
+There is no fallback: if the library is missing or no HIP device is present,
+opening a context raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libgtoric.so"
+
+_lib = None
+_lock = threading.Lock()
+
+P = ctypes.POINTER
+i32, i64, u64, u8 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint8
+vp = ctypes.c_void_p
+
+
+class GtError(RuntimeError):
+    """Error reported by libgtoric."""
+
+
+_SIGS = {
+    "gt_last_error": ([], ctypes.c_char_p),
+    "gt_device_count": ([P(i32)], ctypes.c_int),
+    "gt_open": ([i32, P(vp)], ctypes.c_int),
+    "gt_close": ([vp], ctypes.c_int),
+    "gt_device_info": ([vp, P(i32), P(i32), P(i32), ctypes.c_char_p, i32], ctypes.c_int),
+    "gt_set_md_params": ([vp, i32, i32, i32], ctypes.c_int),
+    "gt_get_md_params": ([vp, i32, P(i32), P(i32)], ctypes.c_int),
+    "gt_md_batch": ([vp, vp, vp, i32, vp, i32, i64, u64, vp], ctypes.c_int),
+    "gt_md_batch_dev": ([vp, vp, i64, vp, i32, vp, i32, i32, i64, u64, vp, vp], ctypes.c_int),
+    "gt_md_single": ([vp, vp, i32, i32, P(i32)], ctypes.c_int),
+    "gt_canon_batch": ([vp, vp, i32, i32, vp, vp, vp, vp], ctypes.c_int),
+    "gt_canon_masks": ([vp, vp, i32, vp, vp, vp, vp], ctypes.c_int),
+    "gt_canon_masks_dev": ([vp, vp, i32, vp, vp, vp, vp, vp], ctypes.c_int),
+    "gt_sync": ([vp, vp], ctypes.c_int),
+    "gt_prof_enable": ([vp, i32], ctypes.c_int),
+    "gt_prof_read": ([vp, ctypes.c_char_p, P(i32), P(ctypes.c_double)], ctypes.c_int),
+}
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def lib():
+    """Load libgtoric.so once (raises if it was not built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise GtError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+            h = ctypes.CDLL(str(LIB_PATH))
+            for name, (args, res) in _SIGS.items():
+                fn = getattr(h, name)
+                fn.argtypes = args
+                fn.restype = res
+            _lib = h
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().gt_last_error()
+        raise GtError(msg.decode() if msg else f"libgtoric error {rc}")
+
+
+def ptr(a: np.ndarray | None):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """One HIP device context: a stream plus grow-only device buffers."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        check(lib().gt_open(int(device), ctypes.byref(h)))
+        self.handle = h
+        self.device = int(device)
+        cu, clk, lds = i32(), i32(), i32()
+        name = ctypes.create_string_buffer(64)
+        check(lib().gt_device_info(h, ctypes.byref(cu), ctypes.byref(clk), ctypes.byref(lds),
+                                   name, 64))
+        self.cu_count = cu.value
+        self.clock_khz = clk.value
+        self.lds_per_block = lds.value
+        self.arch = name.value.decode()
+
+    def close(self) -> None:
+        if self.handle:
+            lib().gt_close(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- parameters ---------------------------------------------------
+    def set_md_params(self, k: int, L: int, check_log2: int) -> None:
+        check(lib().gt_set_md_params(self.handle, int(k), int(L), int(check_log2)))
+
+    def get_md_params(self, k: int) -> tuple[int, int]:
+        L, c = i32(), i32()
+        check(lib().gt_get_md_params(self.handle, int(k), ctypes.byref(L), ctypes.byref(c)))
+        return L.value, c.value
+
+    # ---- min distance ------------------------------------------------------
+    def md_batch(self, flat: np.ndarray, offsets: np.ndarray, tmax, sample_count: int = 0,
+                 sample_seed: int = 0) -> np.ndarray:
+        n = len(offsets) - 1
+        out = np.zeros(n, dtype=np.int32)
+        if n == 0:
+            return out
+        flat = np.ascontiguousarray(flat, dtype=np.int32)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+        if np.ndim(tmax) == 0:
+            tm, ts = None, int(tmax)
+        else:
+            tm, ts = np.ascontiguousarray(tmax, dtype=np.int32), 0
+        if flat.size == 0:
+            flat = np.zeros(1, dtype=np.int32)
+        check(lib().gt_md_batch(self.handle, ptr(flat), ptr(offsets), n, ptr(tm), ts,
+                                int(sample_count),
+                                ctypes.c_uint64(int(sample_seed) & 0xFFFF_FFFF_FFFF_FFFF),
+                                ptr(out)))
+        return out
+
+    def md_batch_dev(self, d_idx: int, n_idx: int, d_offsets: int, n: int, d_out: int,
+                     tmax: int = 48, d_tmax: int | None = None, thresholded: bool = False,
+                     sample_count: int = 0, sample_seed: int = 0, stream: int | None = None):
+        check(lib().gt_md_batch_dev(self.handle, vp(d_idx), int(n_idx), vp(d_offsets), int(n),
+                                    vp(d_tmax) if d_tmax else None, int(tmax),
+                                    int(bool(thresholded)), int(sample_count),
+                                    ctypes.c_uint64(int(sample_seed) & 0xFFFF_FFFF_FFFF_FFFF),
+                                    vp(d_out), vp(stream) if stream else None))
+
+    def md_single(self, S, tmax: int) -> int:
+        s = np.ascontiguousarray(S, dtype=np.int32)
+        out = i32()
+        if s.size == 0:
+            s = np.zeros(1, dtype=np.int32)
+            k = 0
+        else:
+            k = len(s)
+        check(lib().gt_md_single(self.handle, ptr(s), k, int(tmax), ctypes.byref(out)))
+        return out.value
+
+    # ---- canonical form --------------------------------------------------
+    def canon_sets(self, sets: np.ndarray):
+        sets = np.ascontiguousarray(sets, dtype=np.int32)
+        n, k = sets.shape
+        lo = np.zeros(n, dtype=np.uint64)
+        hi = np.zeros(n, dtype=np.uint64)
+        cm = np.zeros(n, dtype=np.uint64)
+        st = np.zeros(n, dtype=np.int32)
+        if n:
+            check(lib().gt_canon_batch(self.handle, ptr(sets), n, k, ptr(lo), ptr(hi), ptr(cm),
+                                       ptr(st)))
+        return lo, hi, cm, st
+
+    def canon_masks(self, masks: np.ndarray):
+        masks = np.ascontiguousarray(masks, dtype=np.uint64)
+        n = len(masks)
+        lo = np.zeros(n, dtype=np.uint64)
+        hi = np.zeros(n, dtype=np.uint64)
+        cm = np.zeros(n, dtype=np.uint64)
+        st = np.zeros(n, dtype=np.int32)
+        if n:
+            check(lib().gt_canon_masks(self.handle, ptr(masks), n, ptr(lo), ptr(hi), ptr(cm),
+                                       ptr(st)))
+        return lo, hi, cm, st
+
+    def canon_masks_dev(self, d_masks: int, n: int, d_lo: int, d_hi: int, d_canon: int,
+                        d_stab: int, stream: int | None = None) -> None:
+        check(lib().gt_canon_masks_dev(self.handle, vp(d_masks), int(n), vp(d_lo), vp(d_hi),
+                                       vp(d_canon), vp(d_stab), vp(stream) if stream else None))
+
+    # ---- sync / profiling ------------------------------------------------
+    def sync(self, stream: int | None = None) -> None:
+        check(lib().gt_sync(self.handle, vp(stream) if stream else None))
+
+    def prof_enable(self, on: bool = True) -> None:
+        check(lib().gt_prof_enable(self.handle, int(bool(on))))
+
+    def prof_read(self, kernel: str) -> tuple[int, float]:
+        n, ms = i32(), ctypes.c_double()
+        check(lib().gt_prof_read(self.handle, kernel.encode(), ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+
+_contexts: dict[int, Context] = {}
+
+
+def context(device: int = 0) -> Context:
+    """Shared per-device context (created on first use)."""
+    with _lock:
+        ctx = _contexts.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        with _lock:
+            _contexts.setdefault(device, ctx)
+            ctx = _contexts[device]
+    return ctx
+
+
+def device_count() -> int:
+    """Number of visible HIP devices (0 when none)."""
+    n = i32()
+    check(lib().gt_device_count(ctypes.byref(n)))
+    return n.value

@@ -1,0 +1,151 @@
+"""GPU parity of the min-distance kernels against the CPU oracle and fixtures.
+
+Every comparison is bit-exact on max_zeros values (thresholded values compared
+as min(max_zeros, tmax + 1), the oracle's deterministic clamp).
+"""
+
+from __future__ import annotations
+
+import itertools
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module")
+def dist():
+    from gtoric.distance import DistanceOracle
+    return DistanceOracle(0)
+
+
+@pytest.fixture(scope="module")
+def md_gold():
+    return np.load(GOLD / "md_golden.npz")
+
+
+def _rows(flat, offs):
+    return [flat[offs[i]:offs[i + 1]].tolist() for i in range(len(offs) - 1)]
+
+
+def test_kernel_backup_kats(dist):
+    # "kernel backup.py":183-245
+    assert dist.max_zeros([0], 49) == 0
+    assert dist.max_zeros([0, 1, 7], 1) == 7
+    assert dist.max_zeros([0, 1, 7], 42) == 7
+    assert dist.max_zeros([0, 1, 7], 49) > 0
+    assert dist.max_zeros_batch([[0, 1, 7]], 1) == [7]
+    assert dist.max_zeros_batch([[0]], 49) == [0]
+    assert dist.max_zeros_batch([[0, 1, 7]], 49) == [1]
+
+
+def test_config1_single_k5(dist, md_gold):
+    sets = _rows(md_gold["c1_flat"], md_gold["c1_offsets"])
+    exp = md_gold["c1_mz"].tolist()
+    assert dist.max_zeros_batch(sets, 1) == exp
+    assert [dist.max_zeros(s, 1) for s in sets] == exp
+
+
+def test_config2_mixed_k_4096(dist, md_gold):
+    sets = _rows(md_gold["c2_flat"], md_gold["c2_offsets"])
+    got = dist.max_zeros_batch(sets, 1)
+    assert got == md_gold["c2_mz"].tolist()
+
+
+def test_thresholded_per_set(dist, md_gold):
+    sets = _rows(md_gold["thr_flat"], md_gold["thr_offsets"])
+    got = dist.max_zeros_batch(sets, md_gold["thr_td"])
+    assert got == md_gold["thr_mz"].tolist()
+
+
+def test_big_k9_k10(dist, md_gold):
+    sets = _rows(md_gold["big_flat"], md_gold["big_offsets"])
+    assert dist.max_zeros_batch(sets, 1) == md_gold["big_mz"].tolist()
+
+
+def test_sampled_stream(dist, md_gold):
+    sets = _rows(md_gold["samp_flat"], md_gold["samp_offsets"])
+    got = dist.max_zeros_batch(sets, 1, sample_count=5000, sample_seed=123456789)
+    assert got == md_gold["samp_mz"].tolist()
+    got = dist.max_zeros_batch(sets, 29, sample_count=5000, sample_seed=987654321)
+    assert got == md_gold["samp_thr_mz"].tolist()
+
+
+def test_random_mixed_vs_oracle(dist):
+    rng = np.random.default_rng(101)
+    sets = [sorted(rng.choice(49, int(k), replace=False).tolist())
+            for k in rng.integers(1, 8, size=300)]
+    sets += [[], [5], [48], [0, 48]]
+    exp = oracle.max_zeros_batch(sets, 48).tolist()
+    assert dist.max_zeros_batch(sets, 1) == exp
+    td = rng.integers(-2, 52, size=len(sets)).astype(np.int32)
+    exp = oracle.max_zeros_batch(sets, (49 - td)).tolist()
+    assert dist.max_zeros_batch(sets, td) == exp
+
+
+def test_single_launch_matches_batch(dist):
+    rng = np.random.default_rng(102)
+    for k in range(1, 9):
+        S = sorted(rng.choice(49, k, replace=False).tolist())
+        exact = oracle.max_zeros(S, 48)
+        assert dist.max_zeros(S, 1) == exact
+        for td in (1, 49 - exact, 50 - exact, 49):
+            assert dist.max_zeros(S, td) == oracle.max_zeros(S, 49 - td)
+
+
+def test_dependent_generators(dist):
+    # Repeated rows give the zero codeword: 49 zeros (reference enumerates it too).
+    assert dist.max_zeros_batch([[3, 3], [0, 5, 5, 9]], 1) == [49, 49]
+    assert dist.max_zeros([7, 7, 7], 1) == 49
+
+
+def test_params_do_not_change_results(dist, md_gold):
+    ctx = dist.context
+    sets = _rows(md_gold["c2_flat"], md_gold["c2_offsets"])[:512]
+    exp = md_gold["c2_mz"][:512].tolist()
+    saved = {k: ctx.get_md_params(k) for k in range(3, 9)}
+    try:
+        for L, c in itertools.product((4, 5, 9, 13), (0, 3)):
+            for k in range(3, 9):
+                ctx.set_md_params(k, L, c)
+            assert dist.max_zeros_batch(sets, 1) == exp
+    finally:
+        for k, (L, c) in saved.items():
+            ctx.set_md_params(k, L, c)
+
+
+def test_agl2_invariance_on_gpu(dist):
+    # d(S) = d(A S + b): a size-independent property at k = 8.
+    from gtoric.canonical import precompute_gl2_perms
+    perms = precompute_gl2_perms()
+    rng = np.random.default_rng(103)
+    base = [sorted(rng.choice(49, 8, replace=False).tolist()) for _ in range(64)]
+    imgs = []
+    for S in base:
+        A = perms[rng.integers(0, len(perms))]
+        b = divmod(int(rng.integers(0, 49)), 7)
+        img = []
+        for p in S:
+            u, v = divmod(int(A[p]), 7)
+            img.append(((u + b[0]) % 7) * 7 + (v + b[1]) % 7)
+        imgs.append(sorted(img))
+    assert dist.max_zeros_batch(base, 1) == dist.max_zeros_batch(imgs, 1)
+
+
+def test_errors(dist):
+    from gtoric._native import GtError
+    with pytest.raises(ValueError):
+        dist.max_zeros_batch([list(range(22))], 1)
+    with pytest.raises(ValueError):
+        dist.max_zeros(list(range(22)), 1)
+    with pytest.raises(GtError):
+        dist.max_zeros_batch([[0, 49]], 1)
+    assert dist.max_zeros_batch([], 1) == []
+    # The context still works after an error.
+    assert dist.max_zeros_batch([[0, 1, 7]], 1) == [7]
