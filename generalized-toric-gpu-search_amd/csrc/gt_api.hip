@@ -42,11 +42,25 @@ int fail(const char* fmt, ...) {
     if (r_ != 0) return r_;    \
   } while (0)
 
+// Reallocation state shared by a context's buffers: captured graphs hold raw
+// buffer addresses, so growth is refused while capturing and counted so that
+// a graph built before a reallocation refuses to launch.
+struct BufState {
+  uint64_t gen = 0;
+  bool frozen = false;
+};
+
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  BufState* st = nullptr;
   int ensure(size_t bytes) {
     if (bytes <= cap) return 0;
+    if (st && st->frozen)
+      return fail("buffer growth to %zu bytes during graph capture; run the same call once "
+                  "before capturing", bytes);
+    // Only moving an existing allocation can invalidate a captured graph.
+    if (st && p) ++st->gen;
     if (p) {
       hipError_t e = hipFree(p);
       p = nullptr;
@@ -136,6 +150,11 @@ struct ProfRec {
 
 }  // namespace
 
+int gt::set_error(const char* msg) {
+  g_err = msg;
+  return -1;
+}
+
 struct gt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -155,6 +174,13 @@ struct gt_ctx {
 
   bool prof = false;
   std::map<std::string, ProfRec> prof_rec;
+  BufState bstate;
+  hipStream_t capture_stream = nullptr;
+
+  std::vector<DevBuf*> all_bufs() {
+    return {&mplanes, &params_buf, &perm, &ctrl, &idx, &offs, &tmax, &rows, &cum, &best,
+            &out, &zmax, &stopf, &sbuf, &masks, &order, &hist, &klo, &khi, &canon, &stab};
+  }
 
   int activate() {
     GT_HIP(hipSetDevice(device));
@@ -164,6 +190,7 @@ struct gt_ctx {
 
   int upload_params(hipStream_t s) {
     if (!params_dirty) return 0;
+    if (bstate.frozen) return fail("launch parameters changed during graph capture");
     GT_HIP(hipMemcpyAsync(params_buf.p, &params, sizeof(MdParams), hipMemcpyHostToDevice, s));
     GT_HIP(hipStreamSynchronize(s));
     params_dirty = false;
@@ -171,14 +198,14 @@ struct gt_ctx {
   }
 
   int prof_begin(const char* k, hipStream_t s, hipEvent_t* e0) {
-    if (!prof) return 0;
+    if (!prof || bstate.frozen) return 0;
     GT_HIP(hipEventCreate(e0));
     GT_HIP(hipEventRecord(*e0, s));
     (void)k;
     return 0;
   }
   int prof_end(const char* k, hipStream_t s, hipEvent_t e0) {
-    if (!prof) return 0;
+    if (!prof || bstate.frozen) return 0;
     hipEvent_t e1;
     GT_HIP(hipEventCreate(&e1));
     GT_HIP(hipEventRecord(e1, s));
@@ -284,6 +311,7 @@ int gt_open(int32_t device, gt_ctx** out) {
       hipMemcpy(c->params_buf.p, &c->params, sizeof(MdParams), hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail("gt_open: upload of constant tables failed"));
   c->params_dirty = false;
+  for (DevBuf* b : c->all_bufs()) b->st = &c->bstate;
   *out = c;
   return 0;
 }
@@ -296,11 +324,7 @@ int gt_close(gt_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     for (auto& kv : c->prof_rec)
       for (auto& pr : kv.second.ev) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
-    DevBuf* bufs[] = {&c->mplanes, &c->params_buf, &c->perm, &c->ctrl, &c->idx, &c->offs,
-                      &c->tmax, &c->rows, &c->cum, &c->best, &c->out, &c->zmax, &c->stopf,
-                      &c->sbuf, &c->masks, &c->order, &c->hist, &c->klo, &c->khi,
-                      &c->canon, &c->stab};
-    for (DevBuf* b : bufs) b->release();
+    for (DevBuf* b : c->all_bufs()) b->release();
     (void)hipStreamDestroy(c->stream);
   }
   delete c;
@@ -503,6 +527,66 @@ int gt_canon_masks_dev(gt_ctx* c, const uint64_t* d_masks, int32_t n, uint64_t* 
   std::lock_guard<std::mutex> lk(c->mu);
   GT_OK(c->activate());
   return canon_enqueue(c, d_masks, n, d_key_lo, d_key_hi, d_canon_mask, d_stab, c->pick(stream));
+}
+
+struct gt_graph {
+  gt_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t gen = 0;
+};
+
+int gt_capture_begin(gt_ctx* c) {
+  if (!c) return fail("null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (c->bstate.frozen) return fail("capture already in progress");
+  GT_OK(c->activate());
+  GT_OK(c->upload_params(c->stream));
+  GT_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->bstate.frozen = true;
+  return 0;
+}
+
+int gt_capture_end(gt_ctx* c, gt_graph** out) {
+  if (!c || !out) return fail("null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!c->bstate.frozen) return fail("no capture in progress");
+  c->bstate.frozen = false;
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  GT_HIP(hipStreamEndCapture(c->stream, &g));
+  hipGraphExec_t ex = nullptr;
+  hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail("hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  gt_graph* gr = new gt_graph();
+  gr->ctx = c;
+  gr->graph = g;
+  gr->exec = ex;
+  gr->gen = c->bstate.gen;
+  *out = gr;
+  return 0;
+}
+
+int gt_graph_launch(gt_graph* g, void* stream) {
+  if (!g || !g->ctx) return fail("null graph");
+  gt_ctx* c = g->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (g->gen != c->bstate.gen)
+    return fail("context buffers were reallocated after this graph was captured; recapture");
+  GT_OK(c->activate());
+  GT_HIP(hipGraphLaunch(g->exec, c->pick(stream)));
+  return 0;
+}
+
+int gt_graph_destroy(gt_graph* g) {
+  if (!g) return 0;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return 0;
 }
 
 int gt_sync(gt_ctx* c, void* stream) {

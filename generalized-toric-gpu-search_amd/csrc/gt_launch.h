@@ -7,6 +7,9 @@
 
 namespace gt {
 
+// Record `msg` as the calling thread's gt_last_error() message; returns -1.
+int set_error(const char* msg);
+
 // ---- min distance -------------------------------------------------------
 // ctrl layout (uint32): [0] chunk counter, [1] unused, [2] error bits,
 // [3] total chunks of the last plan, [4..15] reserved.

@@ -1,0 +1,42 @@
+// Layout of the surrogate MLP (shared by its kernels and the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "gt_common.h"
+
+namespace gt {
+
+constexpr int kMlpIn = 128;    // 49 mask bits, |S|/16, 48 difference bins, padding
+constexpr int kMlpHid = 128;
+constexpr int kMlpTile = 32;   // rows per workgroup
+// Flat fp32 parameter vector: W1[hid][in], b1, W2[hid][hid], b2, w3[hid], b3.
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kMlpHid * kMlpIn;
+constexpr int kOffW2 = kOffB1 + kMlpHid;
+constexpr int kOffB2 = kOffW2 + kMlpHid * kMlpHid;
+constexpr int kOffW3 = kOffB2 + kMlpHid;
+constexpr int kOffB3 = kOffW3 + kMlpHid;
+constexpr int kMlpParams = kOffB3 + 1;
+constexpr int kMlpPartialStride = (kMlpParams + 1 + 3) & ~3;   // params + loss slot
+
+struct MlpBufs {
+  float* theta = nullptr;   // kMlpParams
+  float* grads = nullptr;   // kMlpParams + 1 (last = loss)
+  float* m1 = nullptr;
+  float* m2 = nullptr;
+  uint16_t* w1b = nullptr;
+  uint16_t* w2b = nullptr;
+  uint16_t* w3b = nullptr;
+};
+
+hipError_t launch_mlp_features(const uint64_t* d_masks, int32_t n, uint16_t* d_feat, hipStream_t s);
+hipError_t launch_mlp_predict(const MlpBufs& b, const uint16_t* d_feat, int32_t n, float* d_out,
+                              hipStream_t s);
+hipError_t launch_mlp_grads(const MlpBufs& b, const uint16_t* d_feat, const float* d_target,
+                            int32_t n, float* d_partial, hipStream_t s);
+hipError_t launch_mlp_adam(const MlpBufs& b, float lr, int step, hipStream_t s);
+hipError_t launch_mlp_cast(const MlpBufs& b, hipStream_t s);
+hipError_t launch_bf16_gemm(const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K,
+                            hipStream_t s);
+
+}  // namespace gt

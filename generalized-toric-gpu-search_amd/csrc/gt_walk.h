@@ -1,0 +1,160 @@
+// Device side of the projective Gray walk (shared by the batch, single-code
+// and serving kernels).  See gt_common.h for the row layout and the piece /
+// chunk decomposition.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "gt_common.h"
+
+namespace gt {
+
+struct RowV {
+  uint32_t w[6];
+};
+
+// Rows that no thread writes while the kernel runs: plain loads, which the
+// compiler turns into scalar loads for wave-uniform addresses.
+struct ConstRows {
+  const uint32_t* __restrict__ R;
+  __device__ __forceinline__ RowV operator()(uint32_t r) const {
+    const uint32_t* p = R + (size_t)r * kRowWords;
+    const uint4 a = *reinterpret_cast<const uint4*>(p);
+    const uint2 b = *reinterpret_cast<const uint2*>(p + 4);
+    RowV v;
+    v.w[0] = a.x; v.w[1] = a.y; v.w[2] = a.z; v.w[3] = a.w; v.w[4] = b.x; v.w[5] = b.y;
+    return v;
+  }
+};
+
+// Rows written by another wave of the same (persistent) kernel: relaxed
+// agent-scope atomic loads, which are vector loads that bypass the
+// non-coherent scalar cache.  Ordering comes from the acquire that published
+// the request.
+struct CoherentRows {
+  const uint32_t* R;
+  __device__ __forceinline__ RowV operator()(uint32_t r) const {
+    const uint32_t* p = R + (size_t)r * kRowWords;
+    RowV v;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+      v.w[i] = __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return v;
+  }
+};
+
+__device__ __forceinline__ void xor_row(RowV& v, const RowV& r) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) v.w[i] ^= r.w[i];
+}
+
+__device__ __forceinline__ void xor_row_masked(RowV& v, const RowV& r, uint32_t m) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) v.w[i] ^= r.w[i] & m;
+}
+
+__device__ __forceinline__ uint32_t weight(const RowV& v) {
+  return __builtin_popcount(v.w[0] | v.w[2] | v.w[4]) +
+         __builtin_popcount(v.w[1] | v.w[3] | v.w[5]);
+}
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off, kWave));
+  return x;
+}
+
+// Two Gray steps: add row a, evaluate, add row b, evaluate.
+#define GT_STEP2(A, B)                              \
+  {                                                 \
+    xor_row(v, A);                                  \
+    const uint32_t wa = weight(v);                  \
+    xor_row(v, B);                                  \
+    const uint32_t wb = weight(v);                  \
+    best = min(best, min(wa, wb));                  \
+  }
+
+// One chunk of Gray piece j: lanes own Gray indices (c*64 + lane) * 2^L and
+// walk 2^L consecutive indices.  The walk also evaluates one codeword past the
+// chunk end (row L added at the last step); it is a nonzero codeword of the
+// same piece, so the minimum is unchanged.  With THR the walk stops once a
+// lane of this wave, or any wave (through *best_g), reaches weight <= thr.
+template <bool THR, class Rows>
+__device__ __forceinline__ uint32_t gray_chunk(const Rows& row, uint32_t j, uint32_t L,
+                                               uint64_t c, uint32_t lane, int thr,
+                                               uint32_t check_mask, const uint32_t* best_g) {
+  RowV base = row(3 * j);
+  // Chunk-uniform Gray bits above the lane bits.
+  uint64_t gc = c ^ (c >> 1);
+  for (uint32_t r = L + kLaneBits; gc != 0; gc >>= 1, ++r)
+    if (gc & 1) xor_row(base, row(r));
+  // Lane bits: gray(c*64 + lane) contributes lane ^ (lane >> 1) ^ (c & 1) << 5
+  // at bit L, and gray() of the shifted index sets bit L-1 when lane is odd.
+  const uint32_t lp = lane ^ (lane >> 1) ^ ((uint32_t)(c & 1) << 5);
+  RowV v = base;
+#pragma unroll
+  for (int t = 0; t < kLaneBits; ++t) xor_row_masked(v, row(L + t), 0u - ((lp >> t) & 1u));
+  xor_row_masked(v, row(L - 1), 0u - (lane & 1u));
+
+  const RowV s0 = row(0), s1 = row(1), s2 = row(2), s3 = row(3);
+  uint32_t best = weight(v);
+  const uint32_t nb = 1u << (L - kU);
+  for (uint32_t m = 0; m < nb; ++m) {
+    const RowV d = row(kU + __builtin_ctz(m + 1));
+    GT_STEP2(s0, s1)
+    GT_STEP2(s0, s2)
+    GT_STEP2(s0, s1)
+    GT_STEP2(s0, s3)
+    GT_STEP2(s0, s1)
+    GT_STEP2(s0, s2)
+    GT_STEP2(s0, s1)
+    GT_STEP2(s0, d)
+    if (THR && (m & check_mask) == check_mask) {
+      if (__any((int)best <= thr)) break;
+      if ((int)__hip_atomic_load(best_g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= thr) break;
+    }
+  }
+  return best;
+}
+static_assert(kU == 4, "gray_chunk unrolls 16 steps");
+
+// Pieces 0..jt enumerated directly: codeword index t in [0, (8^(jt+1)-1)/7).
+template <class Rows>
+__device__ __forceinline__ uint32_t tiny_chunk(const Rows& row, int jt, uint32_t lane) {
+  uint32_t best = 64;
+  const uint32_t T = (uint32_t)proj_codewords(jt + 1);
+  for (uint32_t t = lane; t < T; t += kWave) {
+    uint32_t j = 0;
+    while ((int)j < jt && t >= (uint32_t)proj_codewords(j + 1)) ++j;
+    const uint32_t a = t - (uint32_t)proj_codewords(j);
+    RowV v = row(3 * j);
+    for (int r = 0; r < 3 * jt; ++r) xor_row_masked(v, row(r), 0u - ((a >> r) & 1u));
+    best = min(best, weight(v));
+  }
+  return best;
+}
+
+// Map a code-local chunk id to (piece j, chunk c); j == tiny_top(k) means tiny.
+__device__ __forceinline__ void locate_chunk(int k, int L, uint64_t local, int& j, uint64_t& c) {
+  const int jt = tiny_top(k);
+  j = k - 1;
+  for (; j > jt; --j) {
+    const uint64_t pc = piece_chunks(j, L);
+    if (local < pc) break;
+    local -= pc;
+  }
+  c = local;
+}
+
+template <bool THR, class Rows>
+__device__ __forceinline__ uint32_t run_chunk(const Rows& row, int k, int L, uint64_t local,
+                                              uint32_t lane, int thr, uint32_t check_mask,
+                                              const uint32_t* best_g) {
+  int j;
+  uint64_t c;
+  locate_chunk(k, L, local, j, c);
+  if (j > tiny_top(k))
+    return gray_chunk<THR>(row, (uint32_t)j, (uint32_t)piece_L(j, L), c, lane, thr, check_mask,
+                           best_g);
+  return tiny_chunk(row, tiny_top(k), lane);
+}
+
+}  // namespace gt

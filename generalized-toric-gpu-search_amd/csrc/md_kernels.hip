@@ -18,139 +18,11 @@
 //   6 v_xor_b32 (row add), 2 v_or3_b32 (plane OR), 2 v_bcnt_u32_b32,
 //   and one v_min3_u32 per two codewords.
 #include "gt_launch.h"
+#include "gt_walk.h"
 
 namespace gt {
 
-struct RowV {
-  uint32_t w[6];
-};
-
-__device__ __forceinline__ RowV load_row(const uint32_t* __restrict__ R, uint32_t r) {
-  const uint32_t* p = R + (size_t)r * kRowWords;
-  const uint4 a = *reinterpret_cast<const uint4*>(p);
-  const uint2 b = *reinterpret_cast<const uint2*>(p + 4);
-  RowV v;
-  v.w[0] = a.x; v.w[1] = a.y; v.w[2] = a.z; v.w[3] = a.w; v.w[4] = b.x; v.w[5] = b.y;
-  return v;
-}
-
-__device__ __forceinline__ void xor_row(RowV& v, const RowV& r) {
-#pragma unroll
-  for (int i = 0; i < 6; ++i) v.w[i] ^= r.w[i];
-}
-
-__device__ __forceinline__ void xor_row_masked(RowV& v, const RowV& r, uint32_t m) {
-#pragma unroll
-  for (int i = 0; i < 6; ++i) v.w[i] ^= r.w[i] & m;
-}
-
-__device__ __forceinline__ uint32_t weight(const RowV& v) {
-  return __builtin_popcount(v.w[0] | v.w[2] | v.w[4]) +
-         __builtin_popcount(v.w[1] | v.w[3] | v.w[5]);
-}
-
-__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off, kWave));
-  return x;
-}
-
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-
-// Two Gray steps: add row a, evaluate, add row b, evaluate.
-#define GT_STEP2(A, B)                              \
-  {                                                 \
-    xor_row(v, A);                                  \
-    const uint32_t wa = weight(v);                  \
-    xor_row(v, B);                                  \
-    const uint32_t wb = weight(v);                  \
-    best = min(best, min(wa, wb));                  \
-  }
-
-// One chunk of Gray piece j: lanes own Gray indices (c*64 + lane) * 2^L and
-// walk 2^L consecutive indices.  The walk also evaluates one codeword past the
-// chunk end (row L added at the last step); it is a nonzero codeword of the
-// same piece, so the minimum is unchanged.
-template <bool THR>
-__device__ __forceinline__ uint32_t gray_chunk(const uint32_t* __restrict__ R, uint32_t j,
-                                               uint32_t L, uint64_t c, uint32_t lane, int thr,
-                                               uint32_t check_mask, const uint32_t* best_g) {
-  RowV base = load_row(R, 3 * j);
-  // Chunk-uniform Gray bits above the lane bits.
-  uint64_t gc = c ^ (c >> 1);
-  for (uint32_t r = L + kLaneBits; gc != 0; gc >>= 1, ++r)
-    if (gc & 1) xor_row(base, load_row(R, r));
-  // Lane bits: gray(c*64 + lane) contributes lane ^ (lane >> 1) ^ (c & 1) << 5
-  // at bit L, and gray() of the shifted index sets bit L-1 when lane is odd.
-  const uint32_t lp = lane ^ (lane >> 1) ^ ((uint32_t)(c & 1) << 5);
-  RowV v = base;
-#pragma unroll
-  for (int t = 0; t < kLaneBits; ++t)
-    xor_row_masked(v, load_row(R, L + t), 0u - ((lp >> t) & 1u));
-  xor_row_masked(v, load_row(R, L - 1), 0u - (lane & 1u));
-
-  const RowV s0 = load_row(R, 0), s1 = load_row(R, 1), s2 = load_row(R, 2), s3 = load_row(R, 3);
-  uint32_t best = weight(v);
-  const uint32_t nb = 1u << (L - kU);
-  for (uint32_t m = 0; m < nb; ++m) {
-    const RowV d = load_row(R, kU + __builtin_ctz(m + 1));
-    GT_STEP2(s0, s1)
-    GT_STEP2(s0, s2)
-    GT_STEP2(s0, s1)
-    GT_STEP2(s0, s3)
-    GT_STEP2(s0, s1)
-    GT_STEP2(s0, s2)
-    GT_STEP2(s0, s1)
-    GT_STEP2(s0, d)
-    if (THR && (m & check_mask) == check_mask) {
-      if (__any((int)best <= thr)) break;
-      if ((int)__hip_atomic_load(best_g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= thr) break;
-    }
-  }
-  return best;
-}
-static_assert(kU == 4, "gray_chunk unrolls 16 steps");
-
-// Pieces 0..jt enumerated directly: codeword index t in [0, (8^(jt+1)-1)/7).
-__device__ __forceinline__ uint32_t tiny_chunk(const uint32_t* __restrict__ R, int jt,
-                                               uint32_t lane) {
-  uint32_t best = 64;
-  const uint32_t T = (uint32_t)proj_codewords(jt + 1);
-  for (uint32_t t = lane; t < T; t += kWave) {
-    uint32_t j = 0;
-    while ((int)j < jt && t >= (uint32_t)proj_codewords(j + 1)) ++j;
-    const uint32_t a = t - (uint32_t)proj_codewords(j);
-    RowV v = load_row(R, 3 * j);
-    for (int r = 0; r < 3 * jt; ++r) xor_row_masked(v, load_row(R, r), 0u - ((a >> r) & 1u));
-    best = min(best, weight(v));
-  }
-  return best;
-}
-
-// Map a code-local chunk id to (piece j, chunk c); j == tiny_top(k) means tiny.
-__device__ __forceinline__ void locate_chunk(int k, int L, uint64_t local, int& j, uint64_t& c) {
-  const int jt = tiny_top(k);
-  j = k - 1;
-  for (; j > jt; --j) {
-    const uint64_t pc = piece_chunks(j, L);
-    if (local < pc) break;
-    local -= pc;
-  }
-  c = local;
-}
-
-template <bool THR>
-__device__ __forceinline__ uint32_t run_chunk(const uint32_t* __restrict__ R, int k, int L,
-                                              uint64_t local, uint32_t lane, int thr,
-                                              uint32_t check_mask, const uint32_t* best_g) {
-  int j;
-  uint64_t c;
-  locate_chunk(k, L, local, j, c);
-  if (j > tiny_top(k))
-    return gray_chunk<THR>(R, (uint32_t)j, (uint32_t)piece_L(j, L), c, lane, thr, check_mask,
-                           best_g);
-  return tiny_chunk(R, tiny_top(k), lane);
-}
 
 // --------------------------------------------------------------- kernels --
 
@@ -254,7 +126,7 @@ __global__ __launch_bounds__(256) void md_main_kernel(
       if ((int)__hip_atomic_load(&best_w[code], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= thr)
         continue;
     }
-    uint32_t b = run_chunk<THR>(R, k, L, g - cum[code], lane, thr, cmask, &best_w[code]);
+    uint32_t b = run_chunk<THR>(ConstRows{R}, k, L, g - cum[code], lane, thr, cmask, &best_w[code]);
     b = wave_min(b);
     if (lane == 0) atomicMin(&best_w[code], b);
   }
@@ -282,7 +154,7 @@ __global__ __launch_bounds__(64) void md_single_kernel(const uint32_t* __restric
   const uint32_t lane = lane_id();
   if (THR && (int)__hip_atomic_load(best_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= thr)
     return;
-  uint32_t b = run_chunk<THR>(R, k, L, blockIdx.x, lane, thr, check_mask, best_out);
+  uint32_t b = run_chunk<THR>(ConstRows{R}, k, L, blockIdx.x, lane, thr, check_mask, best_out);
   b = wave_min(b);
   if (lane == 0) atomicMin(best_out, b);
 }
@@ -323,7 +195,7 @@ __global__ __launch_bounds__(kSampledBlock) void md_sampled_kernel(
     const int i = e >> 3, v = e & 7;
     RowV w = {{0, 0, 0, 0, 0, 0}};
     for (int b = 0; b < 3; ++b)
-      if ((v >> b) & 1) xor_row(w, load_row(R, 3 * i + b));
+      if ((v >> b) & 1) xor_row(w, ConstRows{R}(3 * i + b));
 #pragma unroll
     for (int q = 0; q < 6; ++q) bp[e * kRowWords + q] = w.w[q];
   }

@@ -1,0 +1,300 @@
+// Surrogate MLP that ranks exponent sets S before exact evaluation.
+//
+// Network: x (128 inputs: 49 mask bits, |S|/16, 48 difference-multiset bins,
+// zero padding) -> 128 ReLU -> 128 ReLU -> 1.  Weights are kept in fp32
+// (Adam state) with bf16 copies that feed the matrix cores; every product
+// accumulates in fp32 (v_mfma_f32_16x16x16_bf16).  Activations and
+// back-propagated errors are rounded to bf16 where they become MFMA operands;
+// the CPU reference (oracle/mlp_ref.py) applies the same roundings.
+//
+// Training step on n rows: one workgroup per 32-row tile runs the forward
+// pass, the loss derivative and the backward pass with all activations in LDS
+// and writes its gradient contribution to its own row of a partial buffer;
+// a second kernel sums the partials in a fixed order (deterministic), and a
+// third applies Adam and refreshes the bf16 copies.
+#include "gt_mlp.h"
+
+namespace gt {
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;   // NaN
+  u += 0x7fffu + ((u >> 16) & 1u);                       // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// C[M][N] = A[M][K] * B[K][N] over 16x16 output tiles shared by the waves of
+// the workgroup; A(m, k) = A[m * sam + k * sak], B(k, n) = B[k * sbk + n * sbn]
+// (bf16, LDS or global).  MFMA 16x16x16 lane layout: lane l supplies
+// A(l % 16, 4 * (l / 16) + e) and B(4 * (l / 16) + e, l % 16), e = 0..3, and
+// receives C(4 * (l / 16) + e, l % 16).  epi(m, n, value) consumes each result.
+template <class Epi>
+__device__ __forceinline__ void wg_gemm(const uint16_t* A, int sam, int sak, const uint16_t* B,
+                                        int sbk, int sbn, int M, int N, int K, Epi epi) {
+  const int nw = blockDim.x / kWave;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int tn = N / 16, tiles = (M / 16) * tn;
+  const int r = lane & 15, q = lane >> 4;
+  for (int t = wave; t < tiles; t += nw) {
+    const int m0 = (t / tn) * 16, n0 = (t % tn) * 16;
+    f4v acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+      s4v a, b;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int kk = k0 + 4 * q + e;
+        a[e] = (short)A[(m0 + r) * sam + kk * sak];
+        b[e] = (short)B[kk * sbk + (n0 + r) * sbn];
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) epi(m0 + 4 * q + e, n0 + r, acc[e]);
+  }
+}
+
+// ------------------------------------------------------------- features --
+
+// Thread per set: mask bits, |S| / 16, and for every ordered pair s != t the
+// bin of (s - t) mod 7 (index 1..48) counted and divided by |S|.
+__global__ __launch_bounds__(256) void mlp_features_kernel(const uint64_t* __restrict__ masks,
+                                                           int32_t n,
+                                                           uint16_t* __restrict__ feat) {
+  __shared__ uint8_t cnt[256][kNpts];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint8_t* c = cnt[threadIdx.x];
+  for (int d = 0; d < kNpts; ++d) c[d] = 0;
+  const uint64_t m = masks[i] & ((1ull << 49) - 1ull);
+  int pts[kNpts];
+  int k = 0;
+  for (uint64_t w = m; w; w &= w - 1) pts[k++] = __builtin_ctzll(w);
+  for (int a = 0; a < k; ++a)
+    for (int b = 0; b < k; ++b) {
+      if (a == b) continue;
+      const int du = (pts[a] / 7 - pts[b] / 7 + 7) % 7, dv = (pts[a] % 7 - pts[b] % 7 + 7) % 7;
+      ++c[du * 7 + dv];
+    }
+  uint16_t* x = feat + (size_t)i * kMlpIn;
+  const float invk = (k > 0) ? 1.0f / (float)k : 0.0f;
+  for (int p = 0; p < kNpts; ++p) x[p] = f2bf(((m >> p) & 1ull) ? 1.0f : 0.0f);
+  x[kNpts] = f2bf((float)k / 16.0f);
+  for (int d = 1; d < kNpts; ++d) x[kNpts + d] = f2bf((float)c[d] * invk);
+  for (int p = 2 * kNpts; p < kMlpIn; ++p) x[p] = 0;
+}
+
+// ------------------------------------------------------------ train step --
+
+struct MlpView {
+  const float* theta;      // fp32 master parameters
+  const uint16_t* w1b;     // [128][128] bf16
+  const uint16_t* w2b;     // [128][128] bf16
+  const uint16_t* w3b;     // [128] bf16
+};
+
+// Forward pass of one tile into LDS: X (loaded), H1, H2 (bf16), y (fp32).
+__device__ __forceinline__ void tile_forward(const MlpView& W, const uint16_t* __restrict__ feat,
+                                             int row0, int n, uint16_t* X, uint16_t* H1,
+                                             uint16_t* H2, float* y) {
+  for (int e = threadIdx.x; e < kMlpTile * kMlpIn; e += blockDim.x) {
+    const int m = e / kMlpIn, i = e % kMlpIn;
+    X[e] = (row0 + m < n) ? feat[(size_t)(row0 + m) * kMlpIn + i] : (uint16_t)0;
+  }
+  __syncthreads();
+  const float* b1 = W.theta + kOffB1;
+  const float* b2 = W.theta + kOffB2;
+  wg_gemm(X, kMlpIn, 1, W.w1b, 1, kMlpIn, kMlpTile, kMlpHid, kMlpIn,
+          [&](int m, int h, float acc) { H1[m * kMlpHid + h] = f2bf(fmaxf(acc + b1[h], 0.f)); });
+  __syncthreads();
+  wg_gemm(H1, kMlpHid, 1, W.w2b, 1, kMlpHid, kMlpTile, kMlpHid, kMlpHid,
+          [&](int m, int h, float acc) { H2[m * kMlpHid + h] = f2bf(fmaxf(acc + b2[h], 0.f)); });
+  __syncthreads();
+  if (threadIdx.x < kMlpTile) {
+    const int m = threadIdx.x;
+    float s = W.theta[kOffB3];
+    for (int h = 0; h < kMlpHid; ++h) s += bf2f(W.w3b[h]) * bf2f(H2[m * kMlpHid + h]);
+    y[m] = s;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void mlp_predict_kernel(MlpView W,
+                                                          const uint16_t* __restrict__ feat,
+                                                          int32_t n, float* __restrict__ out) {
+  __shared__ uint16_t X[kMlpTile * kMlpIn], H1[kMlpTile * kMlpHid], H2[kMlpTile * kMlpHid];
+  __shared__ float y[kMlpTile];
+  const int row0 = blockIdx.x * kMlpTile;
+  tile_forward(W, feat, row0, n, X, H1, H2, y);
+  if (threadIdx.x < kMlpTile && row0 + (int)threadIdx.x < n) out[row0 + threadIdx.x] = y[threadIdx.x];
+}
+
+// Loss 0.5 * mean((y - t)^2).  partial[blockIdx.x][0 .. kMlpParams) receives
+// this tile's gradient, partial[...][kMlpParams] its summed squared error.
+__global__ __launch_bounds__(256) void mlp_grad_kernel(MlpView W,
+                                                       const uint16_t* __restrict__ feat,
+                                                       const float* __restrict__ target,
+                                                       int32_t n, float* __restrict__ partial) {
+  __shared__ uint16_t X[kMlpTile * kMlpIn], H1[kMlpTile * kMlpHid], H2[kMlpTile * kMlpHid];
+  __shared__ uint16_t D2[kMlpTile * kMlpHid], D1[kMlpTile * kMlpHid];
+  __shared__ float y[kMlpTile], dy[kMlpTile], sq[kMlpTile];
+  const int row0 = blockIdx.x * kMlpTile;
+  tile_forward(W, feat, row0, n, X, H1, H2, y);
+  float* P = partial + (size_t)blockIdx.x * kMlpPartialStride;
+  if (threadIdx.x < kMlpTile) {
+    const int m = threadIdx.x;
+    const bool ok = row0 + m < n;
+    const float diff = ok ? (y[m] - target[row0 + m]) : 0.f;
+    dy[m] = diff / (float)n;
+    sq[m] = diff * diff;
+  }
+  __syncthreads();
+  // dH2 = dy * w3 * relu'(H2), rounded to bf16; dw3, db3, db2.
+  for (int e = threadIdx.x; e < kMlpTile * kMlpHid; e += blockDim.x) {
+    const int m = e / kMlpHid, h = e % kMlpHid;
+    const float g = (H2[e] != 0) ? dy[m] * bf2f(W.w3b[h]) : 0.f;
+    D2[e] = f2bf(g);
+  }
+  __syncthreads();
+  if (threadIdx.x < kMlpHid) {
+    const int h = threadIdx.x;
+    float gw3 = 0.f, gb2 = 0.f;
+    for (int m = 0; m < kMlpTile; ++m) {
+      gw3 += dy[m] * bf2f(H2[m * kMlpHid + h]);
+      gb2 += bf2f(D2[m * kMlpHid + h]);
+    }
+    P[kOffW3 + h] = gw3;
+    P[kOffB2 + h] = gb2;
+  }
+  if (threadIdx.x == 0) {
+    float gb3 = 0.f, s = 0.f;
+    for (int m = 0; m < kMlpTile; ++m) { gb3 += dy[m]; s += sq[m]; }
+    P[kOffB3] = gb3;
+    P[kMlpParams] = s;
+  }
+  // dH1 = (dH2 W2) * relu'(H1), rounded to bf16.
+  wg_gemm(D2, kMlpHid, 1, W.w2b, kMlpHid, 1, kMlpTile, kMlpHid, kMlpHid,
+          [&](int m, int h, float acc) {
+            D1[m * kMlpHid + h] = (H1[m * kMlpHid + h] != 0) ? f2bf(acc) : (uint16_t)0;
+          });
+  __syncthreads();
+  // dW2[o][i] = sum_m dH2[m][o] H1[m][i];  dW1[o][i] = sum_m dH1[m][o] X[m][i].
+  wg_gemm(D2, 1, kMlpHid, H1, kMlpHid, 1, kMlpHid, kMlpHid, kMlpTile,
+          [&](int o, int i, float acc) { P[kOffW2 + o * kMlpHid + i] = acc; });
+  wg_gemm(D1, 1, kMlpHid, X, kMlpIn, 1, kMlpHid, kMlpIn, kMlpTile,
+          [&](int o, int i, float acc) { P[kOffW1 + o * kMlpIn + i] = acc; });
+  if (threadIdx.x < kMlpHid) {
+    const int h = threadIdx.x;
+    float gb1 = 0.f;
+    for (int m = 0; m < kMlpTile; ++m) gb1 += bf2f(D1[m * kMlpHid + h]);
+    P[kOffB1 + h] = gb1;
+  }
+}
+
+// grads[p] = sum over tiles in index order (deterministic); slot kMlpParams is
+// the loss 0.5 * sum (y - t)^2 / n.
+__global__ void mlp_reduce_kernel(const float* __restrict__ partial, int32_t tiles, int32_t n,
+                                  float* __restrict__ grads) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > kMlpParams) return;
+  float s = 0.f;
+  for (int t = 0; t < tiles; ++t) s += partial[(size_t)t * kMlpPartialStride + p];
+  grads[p] = (p == kMlpParams) ? 0.5f * s / (float)n : s;
+}
+
+__global__ void mlp_adam_kernel(float* __restrict__ theta, const float* __restrict__ grads,
+                                float* __restrict__ m1, float* __restrict__ m2, float lr,
+                                float beta1, float beta2, float eps, float c1, float c2,
+                                uint16_t* __restrict__ w1b, uint16_t* __restrict__ w2b,
+                                uint16_t* __restrict__ w3b) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kMlpParams) return;
+  const float g = grads[p];
+  const float a = beta1 * m1[p] + (1.f - beta1) * g;
+  const float b = beta2 * m2[p] + (1.f - beta2) * g * g;
+  m1[p] = a;
+  m2[p] = b;
+  const float w = theta[p] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+  theta[p] = w;
+  if (p < kOffB1) w1b[p - kOffW1] = f2bf(w);
+  else if (p >= kOffW2 && p < kOffB2) w2b[p - kOffW2] = f2bf(w);
+  else if (p >= kOffW3 && p < kOffB3) w3b[p - kOffW3] = f2bf(w);
+}
+
+__global__ void mlp_cast_kernel(const float* __restrict__ theta, uint16_t* __restrict__ w1b,
+                                uint16_t* __restrict__ w2b, uint16_t* __restrict__ w3b) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kMlpParams) return;
+  const float w = theta[p];
+  if (p < kOffB1) w1b[p - kOffW1] = f2bf(w);
+  else if (p >= kOffW2 && p < kOffB2) w2b[p - kOffW2] = f2bf(w);
+  else if (p >= kOffW3 && p < kOffB3) w3b[p - kOffW3] = f2bf(w);
+}
+
+// Plain bf16 GEMM through wg_gemm (row-major A[M][K], B[K][N] -> fp32 C[M][N]),
+// one workgroup per 64 x 64 block of C; M, N, K multiples of 16.
+__global__ __launch_bounds__(256) void bf16_gemm_kernel(const uint16_t* __restrict__ A,
+                                                        const uint16_t* __restrict__ B,
+                                                        float* __restrict__ C, int M, int N,
+                                                        int K) {
+  const int bm = blockIdx.y * 64, bn = blockIdx.x * 64;
+  const int mm = min(64, M - bm), nn = min(64, N - bn);
+  wg_gemm(A + (size_t)bm * K, K, 1, B + bn, N, 1, mm, nn, K,
+          [&](int m, int n2, float acc) { C[(size_t)(bm + m) * N + bn + n2] = acc; });
+}
+
+// ------------------------------------------------------------- launchers --
+
+static MlpView view(const MlpBufs& b) { return MlpView{b.theta, b.w1b, b.w2b, b.w3b}; }
+
+hipError_t launch_mlp_features(const uint64_t* d_masks, int32_t n, uint16_t* d_feat,
+                               hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(mlp_features_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_masks, n,
+                     d_feat);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_predict(const MlpBufs& b, const uint16_t* d_feat, int32_t n, float* d_out,
+                              hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(mlp_predict_kernel, dim3((n + kMlpTile - 1) / kMlpTile), dim3(256), 0, s,
+                     view(b), d_feat, n, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_grads(const MlpBufs& b, const uint16_t* d_feat, const float* d_target,
+                            int32_t n, float* d_partial, hipStream_t s) {
+  const int tiles = (n + kMlpTile - 1) / kMlpTile;
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(tiles), dim3(256), 0, s, view(b), d_feat, d_target, n,
+                     d_partial);
+  hipLaunchKernelGGL(mlp_reduce_kernel, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, s,
+                     d_partial, tiles, n, b.grads);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_adam(const MlpBufs& b, float lr, int step, hipStream_t s) {
+  const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+  const float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
+  hipLaunchKernelGGL(mlp_adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, b.theta,
+                     b.grads, b.m1, b.m2, lr, beta1, beta2, eps, c1, c2, b.w1b, b.w2b, b.w3b);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_cast(const MlpBufs& b, hipStream_t s) {
+  hipLaunchKernelGGL(mlp_cast_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, b.theta,
+                     b.w1b, b.w2b, b.w3b);
+  return hipGetLastError();
+}
+
+hipError_t launch_bf16_gemm(const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(bf16_gemm_kernel, dim3((N + 63) / 64, (M + 63) / 64), dim3(256), 0, s, A, B,
+                     C, M, N, K);
+  return hipGetLastError();
+}
+
+}  // namespace gt

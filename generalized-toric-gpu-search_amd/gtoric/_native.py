@@ -41,6 +41,30 @@ _SIGS = {
     "gt_canon_batch": ([vp, vp, i32, i32, vp, vp, vp, vp], ctypes.c_int),
     "gt_canon_masks": ([vp, vp, i32, vp, vp, vp, vp], ctypes.c_int),
     "gt_canon_masks_dev": ([vp, vp, i32, vp, vp, vp, vp, vp], ctypes.c_int),
+    "gt_capture_begin": ([vp], ctypes.c_int),
+    "gt_capture_end": ([vp, P(vp)], ctypes.c_int),
+    "gt_graph_launch": ([vp, vp], ctypes.c_int),
+    "gt_graph_destroy": ([vp], ctypes.c_int),
+    "gt_serve_start": ([i32, i32, i64, vp, vp, ctypes.c_double, ctypes.c_double, P(vp)],
+                       ctypes.c_int),
+    "gt_serve_submit": ([vp, vp, i32, i32, u64, P(i32)], ctypes.c_int),
+    "gt_serve_poll": ([vp, vp, vp, i32, P(i32)], ctypes.c_int),
+    "gt_serve_in_flight": ([vp, P(i64)], ctypes.c_int),
+    "gt_serve_run_stream": ([vp, vp, vp, i32, vp, vp, vp, vp, P(ctypes.c_double),
+                             ctypes.c_double], ctypes.c_int),
+    "gt_serve_stop": ([vp], ctypes.c_int),
+    "gt_mlp_num_params": ([P(i32)], ctypes.c_int),
+    "gt_mlp_create": ([i32, u64, P(vp)], ctypes.c_int),
+    "gt_mlp_destroy": ([vp], ctypes.c_int),
+    "gt_mlp_get_params": ([vp, vp], ctypes.c_int),
+    "gt_mlp_set_params": ([vp, vp], ctypes.c_int),
+    "gt_mlp_features": ([vp, vp, i32, vp], ctypes.c_int),
+    "gt_mlp_grads": ([vp, vp, vp, i32, vp, P(ctypes.c_float)], ctypes.c_int),
+    "gt_mlp_train_step": ([vp, vp, vp, i32, ctypes.c_float, P(ctypes.c_float)], ctypes.c_int),
+    "gt_mlp_train_step_dev": ([vp, vp, vp, i32, ctypes.c_float, vp], ctypes.c_int),
+    "gt_mlp_predict": ([vp, vp, i32, vp], ctypes.c_int),
+    "gt_mlp_predict_dev": ([vp, vp, i32, vp, vp], ctypes.c_int),
+    "gt_bf16_gemm": ([i32, vp, vp, vp, i32, i32, i32], ctypes.c_int),
     "gt_sync": ([vp, vp], ctypes.c_int),
     "gt_prof_enable": ([vp, i32], ctypes.c_int),
     "gt_prof_read": ([vp, ctypes.c_char_p, P(i32), P(ctypes.c_double)], ctypes.c_int),
@@ -185,6 +209,15 @@ class Context:
         check(lib().gt_canon_masks_dev(self.handle, vp(d_masks), int(n), vp(d_lo), vp(d_hi),
                                        vp(d_canon), vp(d_stab), vp(stream) if stream else None))
 
+    # ---- graphs ----------------------------------------------------------
+    def capture_begin(self) -> None:
+        check(lib().gt_capture_begin(self.handle))
+
+    def capture_end(self) -> "Graph":
+        g = ctypes.c_void_p()
+        check(lib().gt_capture_end(self.handle, ctypes.byref(g)))
+        return Graph(self, g)
+
     # ---- sync / profiling ------------------------------------------------
     def sync(self, stream: int | None = None) -> None:
         check(lib().gt_sync(self.handle, vp(stream) if stream else None))
@@ -196,6 +229,28 @@ class Context:
         n, ms = i32(), ctypes.c_double()
         check(lib().gt_prof_read(self.handle, kernel.encode(), ctypes.byref(n), ctypes.byref(ms)))
         return n.value, ms.value
+
+
+class Graph:
+    """Executable HIP graph of *_dev calls captured on a context's stream."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    def launch(self, stream: int | None = None) -> None:
+        check(lib().gt_graph_launch(self.handle, vp(stream) if stream else None))
+
+    def close(self) -> None:
+        if self.handle:
+            lib().gt_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _contexts: dict[int, Context] = {}

@@ -1,0 +1,134 @@
+"""Autotuner for the min-distance launch parameters, per k.
+
+Tunable per k (gt_set_md_params):
+  L           Gray walk bits per lane; one chunk = 64 lanes x 2^L codewords, so
+              L sets the number of waves per code: code_chunks(k, L).
+  check_log2  early-exit check every 2^check_log2 blocks of 16 steps
+              (thresholded searches only).
+
+For each k the tuner times device kernels (HIP events around md_main, read
+through gt_prof_read) on a seeded single-k batch, keeps the fastest setting,
+and writes a JSON table keyed by device architecture.  Results never depend on
+the parameters (tests/test_gpu_distance.py::test_params_do_not_change_results);
+only speed does.
+"""
+
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+
+from . import _native
+
+TUNED_DIR = Path(__file__).resolve().parent / "tuned"
+KU = 4
+LANE_BITS = 6
+
+
+def code_chunks(k: int, L: int) -> int:
+    """Host copy of gt_common.h:code_chunks (chunks per code)."""
+    if k <= 0:
+        return 0
+    jt = min(k - 1, (LANE_BITS + KU - 1) // 3)
+    n = 1
+    for j in range(k - 1, jt, -1):
+        B = 3 * j
+        n += (1 << (B - LANE_BITS - L)) if B >= LANE_BITS + L else 1
+    return n
+
+
+def candidate_L(k: int) -> list[int]:
+    top = max(KU, 3 * (k - 1) - LANE_BITS)
+    return list(range(KU, min(top, 20) + 1))
+
+
+def _batch(k: int, n: int, seed: int) -> list[list[int]]:
+    rng = np.random.default_rng(seed)
+    return [sorted(rng.choice(49, k, replace=False).tolist()) for _ in range(n)]
+
+
+def _time_md(ctx: _native.Context, flat, offs, tmax, repeats: int) -> float:
+    ctx.md_batch(flat, offs, tmax)              # warm
+    ctx.prof_read("md_main")
+    ctx.prof_enable(True)
+    try:
+        for _ in range(repeats):
+            ctx.md_batch(flat, offs, tmax)
+    finally:
+        ctx.prof_enable(False)
+    n, ms = ctx.prof_read("md_main")
+    return ms / max(n, 1)
+
+
+def tune(ctx: _native.Context, ks=range(3, 11), codes_per_k: dict | None = None,
+         repeats: int = 3, verbose: bool = True) -> dict:
+    """Sweep L (exact mode) then check_log2 (thresholded mode) for each k."""
+    from .distance import flatten
+    from . import workloads  # noqa: F401  (same RNG conventions)
+    codes_per_k = codes_per_k or {3: 8192, 4: 8192, 5: 8192, 6: 4096, 7: 4096, 8: 2048,
+                                  9: 256, 10: 32, 11: 4, 12: 1}
+    table = {}
+    for k in ks:
+        n = codes_per_k.get(k, 1)
+        sets = _batch(k, n, 1000 + k)
+        flat, offs = flatten(sets)
+        saved = ctx.get_md_params(k)
+        best = None
+        for L in candidate_L(k):
+            ctx.set_md_params(k, L, saved[1])
+            t = _time_md(ctx, flat, offs, 48, repeats)
+            if verbose:
+                print(f"[tune] k={k} L={L:2d} chunks/code={code_chunks(k, L):6d} md_main {t:.4f} ms")
+            if best is None or t < best[1]:
+                best = (L, t)
+        L = best[0]
+        # Early-exit interval on a thresholded batch: target = the batch's median
+        # exact distance, so about half the codes stop early.
+        ctx.set_md_params(k, L, saved[1])
+        exact = ctx.md_batch(flat, offs, 48)
+        tmax = int(np.median(exact))
+        bestc = None
+        for c in (0, 1, 2, 4, 6):
+            ctx.set_md_params(k, L, c)
+            t = _time_md(ctx, flat, offs, tmax, repeats)
+            if verbose:
+                print(f"[tune] k={k} L={L} check_log2={c} thresholded md_main {t:.4f} ms")
+            if bestc is None or t < bestc[1]:
+                bestc = (c, t)
+        ctx.set_md_params(k, L, bestc[0])
+        table[str(k)] = {"L": L, "check_log2": bestc[0], "exact_ms": round(best[1], 5),
+                         "thr_ms": round(bestc[1], 5), "codes": n}
+    return {"arch": ctx.arch, "cu_count": ctx.cu_count, "params": table}
+
+
+def tuned_path(arch: str) -> Path:
+    return TUNED_DIR / f"{arch.split(':')[0]}.json"
+
+
+def save(result: dict) -> Path:
+    TUNED_DIR.mkdir(exist_ok=True)
+    p = tuned_path(result["arch"])
+    p.write_text(json.dumps(result, indent=1) + "\n")
+    return p
+
+
+def apply(ctx: _native.Context, path: Path | None = None) -> bool:
+    """Load the tuned table for this device (if any) into the context."""
+    p = path or tuned_path(ctx.arch)
+    if not p.exists():
+        return False
+    data = json.loads(p.read_text())
+    for k, v in data.get("params", {}).items():
+        ctx.set_md_params(int(k), int(v["L"]), int(v["check_log2"]))
+    return True
+
+
+if __name__ == "__main__":
+    import sys
+    c = _native.Context(0)
+    ks = [int(x) for x in sys.argv[1:]] or list(range(3, 11))
+    res = tune(c, ks)
+    print(json.dumps(res, indent=1))
+    print("saved", save(res))

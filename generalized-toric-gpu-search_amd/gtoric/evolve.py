@@ -1,0 +1,223 @@
+"""Fixed-k evolutionary search with exact scoring, canonical dedup, hipGraph
+generations and an optional surrogate ranker.
+
+Plays the role of run_evolution (evolve_search.py:322-539) on the exact
+kernel: every distance recorded here is exact (td = 1 evaluation), never a
+sample.  One generation:
+
+1. propose ``n_prop`` children: tournament parent (size 4), swap 1 or 2 points
+   (remove one or two points of S, add as many outside S);
+2. canonical forms of all proposals (GPU, graph replay), drop proposals whose
+   AGL2 class was already evaluated or repeats within the generation;
+3. choose ``pop`` of them: the surrogate's top scores, or (unranked) the first
+   ones in proposal order; pad with fresh random sets when too few survive;
+4. exact minimum distance of the chosen sets (GPU, graph replay);
+5. keep the best ``pop`` of parents and children (distance, then age).
+
+With a surrogate the network is retrained every ``train_every`` generations on
+all (set, exact distance) pairs seen so far; it only orders proposals.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+
+N_POINTS = 49
+BIT = np.left_shift(np.uint64(1), np.arange(N_POINTS, dtype=np.uint64))
+
+
+def masks_to_bits(masks: np.ndarray) -> np.ndarray:
+    return ((masks[:, None] & BIT[None, :]) != 0)
+
+
+def bits_to_masks(bits: np.ndarray) -> np.ndarray:
+    return np.bitwise_or.reduce(np.where(bits, BIT[None, :], np.uint64(0)), axis=1)
+
+
+def random_masks(rng, n: int, k: int) -> np.ndarray:
+    keys = rng.random((n, N_POINTS))
+    sel = np.argsort(keys, axis=1)[:, :k]
+    bits = np.zeros((n, N_POINTS), dtype=bool)
+    np.put_along_axis(bits, sel, True, axis=1)
+    return bits_to_masks(bits)
+
+
+def mutate(rng, parents: np.ndarray) -> np.ndarray:
+    """Swap 1 or 2 points of each parent mask."""
+    bits = masks_to_bits(parents)
+    n = len(parents)
+    swaps = rng.integers(1, 3, size=n)
+    keys = rng.random((n, N_POINTS))
+    out_keys = np.where(bits, keys, 2.0)           # removal candidates: set bits
+    in_keys = np.where(bits, 2.0, keys)            # addition candidates: clear bits
+    ro = np.argsort(out_keys, axis=1)
+    ri = np.argsort(in_keys, axis=1)
+    rows = np.arange(n)
+    bits[rows, ro[:, 0]] = False
+    bits[rows, ri[:, 0]] = True
+    two = swaps == 2
+    bits[rows[two], ro[two, 1]] = False
+    bits[rows[two], ri[two, 1]] = True
+    return bits_to_masks(bits)
+
+
+def masks_to_flat(masks: np.ndarray, k: int) -> np.ndarray:
+    bits = masks_to_bits(masks)
+    return np.nonzero(bits)[1].astype(np.int32).reshape(len(masks), k).ravel()
+
+
+@dataclass
+class EvolveResult:
+    k: int
+    best_d: int
+    best_set: list
+    generations: int
+    evaluated: int
+    wall_s: float
+    eval_s: float
+    history: list = field(default_factory=list)   # (wall_s, best_d) per generation
+
+    @property
+    def codes_per_s(self) -> float:
+        return self.evaluated / max(self.wall_s, 1e-9)
+
+
+class GenerationGraphs:
+    """Device buffers and two captured graphs (canonical forms of n_prop masks,
+    exact distances of pop sets of size k) replayed every generation."""
+
+    def __init__(self, ctx: _native.Context, k: int, pop: int, n_prop: int):
+        import torch
+        self.torch = torch
+        self.ctx, self.k, self.pop, self.n_prop = ctx, k, pop, n_prop
+        dev = torch.device("cuda", ctx.device)
+        self.d_prop = torch.zeros(n_prop, dtype=torch.int64, device=dev)
+        self.d_lo = torch.zeros(n_prop, dtype=torch.int64, device=dev)
+        self.d_hi = torch.zeros(n_prop, dtype=torch.int64, device=dev)
+        self.d_cm = torch.zeros(n_prop, dtype=torch.int64, device=dev)
+        self.d_st = torch.zeros(n_prop, dtype=torch.int32, device=dev)
+        offs = np.arange(pop + 1, dtype=np.int32) * k
+        self.d_idx = torch.zeros(pop * k, dtype=torch.int32, device=dev)
+        self.d_off = torch.from_numpy(offs).to(dev)
+        self.d_out = torch.zeros(pop, dtype=torch.int32, device=dev)
+        # Warm both paths once so every buffer exists before capture.
+        self._canon_call()
+        self._md_call()
+        ctx.sync()
+        ctx.capture_begin()
+        self._canon_call()
+        self.g_canon = ctx.capture_end()
+        ctx.capture_begin()
+        self._md_call()
+        self.g_md = ctx.capture_end()
+
+    def _canon_call(self):
+        self.ctx.canon_masks_dev(self.d_prop.data_ptr(), self.n_prop, self.d_lo.data_ptr(),
+                                 self.d_hi.data_ptr(), self.d_cm.data_ptr(), self.d_st.data_ptr())
+
+    def _md_call(self):
+        self.ctx.md_batch_dev(self.d_idx.data_ptr(), self.pop * self.k, self.d_off.data_ptr(),
+                              self.pop, self.d_out.data_ptr(), tmax=48, thresholded=False)
+
+    def canon(self, masks: np.ndarray) -> np.ndarray:
+        t = self.torch
+        self.d_prop.copy_(t.from_numpy(masks.view(np.int64)))
+        t.cuda.synchronize()
+        self.g_canon.launch()
+        self.ctx.sync()
+        return self.d_cm.cpu().numpy().view(np.uint64)
+
+    def distances(self, masks: np.ndarray) -> np.ndarray:
+        t = self.torch
+        self.d_idx.copy_(t.from_numpy(masks_to_flat(masks, self.k)))
+        t.cuda.synchronize()
+        self.g_md.launch()
+        self.ctx.sync()
+        return 49 - self.d_out.cpu().numpy()
+
+    def close(self):
+        self.g_canon.close()
+        self.g_md.close()
+
+
+def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogate=None,
+        proposal_factor: int = 4, wall_limit_s: float | None = None, train_every: int = 5,
+        train_epochs: int = 3, ctx: _native.Context | None = None, verbose: bool = False
+        ) -> EvolveResult:
+    ctx = ctx or _native.context(0)
+    rng = np.random.default_rng(seed)
+    n_prop = pop * (proposal_factor if surrogate is not None else 1)
+    gg = GenerationGraphs(ctx, k, pop, n_prop)
+    t0 = time.perf_counter()
+    t_eval = 0.0
+    popm = random_masks(rng, pop, k)
+    seen = set(gg.canon(np.concatenate([popm] * (n_prop // pop))[:n_prop]).tolist()[:pop])
+    te = time.perf_counter()
+    popd = gg.distances(popm)
+    t_eval += time.perf_counter() - te
+    age = np.zeros(pop, dtype=np.int64)
+    evaluated = pop
+    if surrogate is not None:
+        surrogate.fit(popm, popd, epochs=train_epochs, batch=1024, seed=0)
+    data_m, data_d = [popm], [popd]
+    history = []
+    gen = 0
+    for gen in range(1, generations + 1):
+        if wall_limit_s is not None and time.perf_counter() - t0 >= wall_limit_s:
+            gen -= 1
+            break
+        # 1. proposals from tournament parents
+        cand = rng.integers(0, pop, size=(n_prop, 4))
+        win = cand[np.arange(n_prop), np.argmax(popd[cand] * 1000 - age[cand], axis=1)]
+        props = mutate(rng, popm[win])
+        # 2. canonical dedup against everything evaluated so far
+        cm = gg.canon(props)
+        keep, fresh = [], set()
+        for i, c in enumerate(cm.tolist()):
+            if c not in seen and c not in fresh:
+                fresh.add(c)
+                keep.append(i)
+        keep = np.array(keep, dtype=np.int64)
+        # 3. choose pop proposals (surrogate order or proposal order)
+        if surrogate is not None and len(keep) > pop:
+            order = surrogate.rank(props[keep])
+            keep = keep[order[:pop]]
+        else:
+            keep = keep[:pop]
+        chosen = props[keep]
+        ccm = cm[keep]
+        if len(chosen) < pop:
+            extra = random_masks(rng, pop - len(chosen), k)
+            chosen = np.concatenate([chosen, extra])
+        seen.update(ccm.tolist())
+        # 4. exact distances
+        te = time.perf_counter()
+        cd = gg.distances(chosen)
+        t_eval += time.perf_counter() - te
+        evaluated += pop
+        data_m.append(chosen)
+        data_d.append(cd)
+        # 5. survivors: best distance, then youngest
+        allm = np.concatenate([popm, chosen])
+        alld = np.concatenate([popd, cd])
+        alla = np.concatenate([age + 1, np.zeros(pop, dtype=np.int64)])
+        order = np.lexsort((alla, -alld))[:pop]
+        popm, popd, age = allm[order], alld[order], alla[order]
+        if surrogate is not None and gen % train_every == 0:
+            surrogate.fit(np.concatenate(data_m), np.concatenate(data_d), epochs=train_epochs,
+                          batch=1024, seed=gen)
+        history.append((time.perf_counter() - t0, int(popd[0])))
+        if verbose:
+            print(f"[evolve k={k}] gen {gen} best d={popd[0]} unique={len(keep)}", flush=True)
+    wall = time.perf_counter() - t0
+    gg.close()
+    best = popm[0]
+    return EvolveResult(k=k, best_d=int(popd[0]),
+                        best_set=[int(p) for p in range(N_POINTS) if (int(best) >> p) & 1],
+                        generations=gen, evaluated=evaluated, wall_s=wall, eval_s=t_eval,
+                        history=history)

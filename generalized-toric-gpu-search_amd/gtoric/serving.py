@@ -1,0 +1,109 @@
+"""Continuously batched min-distance serving on one persistent HIP kernel.
+
+A :class:`Server` owns one persistent kernel (see csrc/serve_kernel.hip) and a
+request ring in pinned host memory.  Requests are admitted as ring slots
+(memory budget) and outstanding projective codewords (work budget) free up;
+the work of all admitted requests is interleaved chunk by chunk on the device.
+
+Each request is (S, tmax): S with k <= 16 exponent points and tmax =
+48 - d_stop, so the search stops as soon as a codeword of weight <= d_stop is
+found.  The result is max_zeros clamped at tmax + 1, the same value
+``DistanceOracle.max_zeros_batch`` returns for that request alone.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+from ._native import check, i32, i64, lib, ptr, vp
+
+MAX_K = 16
+
+
+class Server:
+    def __init__(self, device: int = 0, slots: int = 1024, work_budget: int = 1 << 36,
+                 params: dict | None = None, idle_s: float = 10.0, life_s: float = 600.0):
+        """params: {k: (L, check_log2)}; defaults come from a context's table."""
+        if params is None:
+            ctx = _native.context(device)
+            params = {k: ctx.get_md_params(k) for k in range(MAX_K + 1)}
+        L = np.array([params.get(k, (10, 2))[0] for k in range(MAX_K + 1)], dtype=np.int32)
+        C = np.array([params.get(k, (10, 2))[1] for k in range(MAX_K + 1)], dtype=np.int32)
+        h = ctypes.c_void_p()
+        check(lib().gt_serve_start(int(device), int(slots), int(work_budget), ptr(L), ptr(C),
+                                   float(idle_s), float(life_s), ctypes.byref(h)))
+        self.handle = h
+        self.slots = slots
+
+    def submit(self, S, tmax: int, tag: int) -> bool:
+        s = np.ascontiguousarray(S if len(S) else [0], dtype=np.int32)
+        acc = i32()
+        check(lib().gt_serve_submit(self.handle, ptr(s), len(S), int(tmax),
+                                    ctypes.c_uint64(int(tag)), ctypes.byref(acc)))
+        return bool(acc.value)
+
+    def poll(self, max_n: int = 4096) -> list[tuple[int, int]]:
+        tags = np.zeros(max_n, dtype=np.uint64)
+        res = np.zeros(max_n, dtype=np.int32)
+        n = i32()
+        check(lib().gt_serve_poll(self.handle, ptr(tags), ptr(res), int(max_n), ctypes.byref(n)))
+        return list(zip(tags[:n.value].tolist(), res[:n.value].tolist()))
+
+    def in_flight(self) -> int:
+        n = i64()
+        check(lib().gt_serve_in_flight(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def run_stream(self, sets, tmax, arrival_s, timeout_s: float = 120.0):
+        """Serve a request stream (arrival offsets in seconds); returns
+        (results int32[n], latency seconds float64[n], wall seconds)."""
+        from .distance import flatten
+        flat, offs = flatten(sets)
+        if flat.size == 0:
+            flat = np.zeros(1, dtype=np.int32)
+        n = len(sets)
+        tm = np.ascontiguousarray(np.broadcast_to(np.asarray(tmax, dtype=np.int32), (n,)))
+        arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        res = np.zeros(n, dtype=np.int32)
+        lat = np.zeros(n, dtype=np.float64)
+        wall = ctypes.c_double()
+        check(lib().gt_serve_run_stream(self.handle, ptr(flat), ptr(offs), n, ptr(tm), ptr(arr),
+                                        ptr(res), ptr(lat), ctypes.byref(wall), float(timeout_s)))
+        return res, lat, wall.value
+
+    def close(self) -> None:
+        if self.handle:
+            h, self.handle = self.handle, None
+            check(lib().gt_serve_stop(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plain_stream(ctx: _native.Context, sets, tmax, arrival_s):
+    """The same stream served by one synchronous launch per request, in
+    arrival order (the reference's max_zeros call pattern, kernel.py:241-274)."""
+    import time
+    n = len(sets)
+    tm = np.broadcast_to(np.asarray(tmax, dtype=np.int32), (n,))
+    res = np.zeros(n, dtype=np.int32)
+    lat = np.zeros(n, dtype=np.float64)
+    t0 = time.perf_counter()
+    for i, S in enumerate(sets):
+        while time.perf_counter() - t0 < arrival_s[i]:
+            pass
+        res[i] = ctx.md_single(S, int(tm[i]))
+        lat[i] = time.perf_counter() - t0 - arrival_s[i]
+    return res, lat, time.perf_counter() - t0

@@ -91,6 +91,79 @@ int gt_canon_masks_dev(gt_ctx* ctx, const uint64_t* d_masks, int32_t n, uint64_t
                        uint64_t* d_key_hi, uint64_t* d_canon_mask, int32_t* d_stab,
                        void* stream);
 
+/* Graph capture of *_dev calls on the context stream (pass stream = NULL to
+ * them while capturing).  Buffers must already have their size: run the same
+ * calls once eagerly first.  A graph records raw buffer addresses, so a later
+ * call that grows a context buffer invalidates it (gt_graph_launch then fails).
+ * No memset nodes are recorded: every reset is a kernel.  No reference
+ * counterpart (the reference launches eagerly, kernel_cuda_bp.py:314-322). */
+typedef struct gt_graph gt_graph;
+int gt_capture_begin(gt_ctx* ctx);
+int gt_capture_end(gt_ctx* ctx, gt_graph** out);
+int gt_graph_launch(gt_graph* graph, void* stream);
+int gt_graph_destroy(gt_graph* graph);
+
+/* Continuously batched serving: one persistent kernel on its own stream
+ * (no reference counterpart; it serves the synchronous max_zeros_batch calls
+ * of champion_search_canonical.py:217-234 and evolve_search.py:223-263 as a
+ * stream).  slots (power of two) bounds the requests in flight (memory
+ * budget); work_budget bounds their projective codewords (work budget; a
+ * request larger than the budget runs alone).  L_by_k / check_by_k (17
+ * entries, k = 0..16, may be NULL) are the walk parameters.  The kernel stops
+ * itself after idle_s seconds without requests or life_s seconds in total and
+ * is relaunched by the next submit.  Requests: k <= 16 points and a threshold
+ * tmax = 48 - d_stop; results as for gt_md_batch (k = 0 gives 0). */
+typedef struct gt_server gt_server;
+int gt_serve_start(int32_t device, int32_t slots, int64_t work_budget, const int32_t* L_by_k,
+                   const int32_t* check_by_k, double idle_s, double life_s, gt_server** out);
+/* Non-blocking admission: *accepted = 0 when no slot or budget is free. */
+int gt_serve_submit(gt_server* srv, const int32_t* idx, int32_t k, int32_t tmax, uint64_t tag,
+                    int32_t* accepted);
+/* Completed requests (tag, result), at most max_n per call. */
+int gt_serve_poll(gt_server* srv, uint64_t* tags, int32_t* results, int32_t max_n,
+                  int32_t* n_done);
+int gt_serve_in_flight(gt_server* srv, int64_t* n);
+/* Run a whole arrival stream from the calling thread: request i is admitted
+ * no earlier than arrival_s[i] seconds after the call starts; latency_s[i] is
+ * completion minus scheduled arrival (queueing included). */
+int gt_serve_run_stream(gt_server* srv, const int32_t* idx, const int32_t* offsets, int32_t n,
+                        const int32_t* tmax, const double* arrival_s, int32_t* results,
+                        double* latency_s, double* wall_s, double timeout_s);
+/* Stop the kernel (waits for it to exit) and free the server. */
+int gt_serve_stop(gt_server* srv);
+
+/* Surrogate ranking MLP (no reference counterpart: it ranks the proposals
+ * of evolve_search.py:509-523 before exact evaluation and never reports a
+ * distance).  Input: 49-bit masks; features = 49 mask bits, |S|/16 and the 48
+ * bins of the ordered difference multiset divided by |S|.  128 -> 128 ReLU ->
+ * 128 ReLU -> 1, fp32 master weights with bf16 copies, fp32 accumulation
+ * (bf16 MFMA), loss 0.5 * mean((y - target)^2), Adam (0.9, 0.999, 1e-8).
+ * Parameters are one flat fp32 vector: W1[128][128], b1[128], W2[128][128],
+ * b2[128], w3[128], b3. */
+typedef struct gt_mlp gt_mlp;
+int gt_mlp_num_params(int32_t* n);
+int gt_mlp_create(int32_t device, uint64_t seed, gt_mlp** out);
+int gt_mlp_destroy(gt_mlp* mlp);
+int gt_mlp_get_params(gt_mlp* mlp, float* theta);
+int gt_mlp_set_params(gt_mlp* mlp, const float* theta);
+/* bf16 features, n x 128 (host output). */
+int gt_mlp_features(gt_mlp* mlp, const uint64_t* masks, int32_t n, uint16_t* feat);
+/* Gradient of the loss at the current parameters (no update). */
+int gt_mlp_grads(gt_mlp* mlp, const uint64_t* masks, const float* target, int32_t n,
+                 float* grads, float* loss);
+/* One Adam step on a batch; *loss is the loss before the step. */
+int gt_mlp_train_step(gt_mlp* mlp, const uint64_t* masks, const float* target, int32_t n,
+                      float lr, float* loss);
+int gt_mlp_train_step_dev(gt_mlp* mlp, const uint64_t* d_masks, const float* d_target,
+                          int32_t n, float lr, void* stream);
+int gt_mlp_predict(gt_mlp* mlp, const uint64_t* masks, int32_t n, float* out);
+int gt_mlp_predict_dev(gt_mlp* mlp, const uint64_t* d_masks, int32_t n, float* d_out,
+                       void* stream);
+/* The MLP's bf16 MFMA tile product as a plain GEMM, C = A B (row-major, fp32
+ * out; M, N, K multiples of 16); exposed so its lane layout is testable. */
+int gt_bf16_gemm(int32_t device, const uint16_t* A, const uint16_t* B, float* C, int32_t M,
+                 int32_t N, int32_t K);
+
 /* Wait for `stream` (NULL = context stream) and report device-side input
  * errors (bad index, bad size) raised by *_dev calls since the last sync. */
 int gt_sync(gt_ctx* ctx, void* stream);

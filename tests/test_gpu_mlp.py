@@ -1,0 +1,98 @@
+"""Surrogate MLP: MFMA tile layout, features, gradients vs the CPU reference,
+and what the trained network gains when ranking proposals.
+
+Tolerances: features are bit-exact (same fp32 ops, round-to-nearest-even
+bf16).  Gradients: GPU sums in fp32, the reference in float64 with the same
+bf16 roundings, so a rounding of an activation or error can land on the other
+side of a bf16 boundary; each parameter block must satisfy
+max|g_gpu - g_cpu| <= 3e-2 * max|g_cpu| and cosine(g_gpu, g_cpu) >= 0.999.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import mlp_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16_bits(x):
+    f = np.asarray(x, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def test_mfma_tile_gemm_layout():
+    from gtoric.surrogate import bf16_gemm
+    rng = np.random.default_rng(0)
+    for (M, N, K) in [(16, 16, 16), (64, 96, 48), (128, 64, 128)]:
+        A = rng.standard_normal((M, K)).astype(np.float32)
+        B = rng.standard_normal((K, N)).astype(np.float32)
+        Ab, Bb = _bf16_bits(A), _bf16_bits(B)
+        Af = (Ab.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+        Bf = (Bb.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+        C = bf16_gemm(Ab, Bb)
+        assert np.allclose(C, Af @ Bf, rtol=1e-5, atol=1e-4), (M, N, K)
+
+
+def test_features_bit_exact():
+    from gtoric import workloads as W
+    from gtoric.surrogate import Surrogate
+    s = Surrogate(0, seed=1)
+    masks = W.canon_masks(500, k_lo=1, k_hi=16, seed=61)
+    got = s.features(masks)
+    exp = _bf16_bits(R.features(masks).astype(np.float32))
+    assert np.array_equal(got, exp)
+
+
+def test_gradients_match_cpu_reference():
+    from gtoric import workloads as W
+    from gtoric.surrogate import Surrogate
+    s = Surrogate(0, seed=2)
+    rng = np.random.default_rng(62)
+    masks = W.canon_masks(300, seed=63)           # 300 rows: a partial last tile
+    target = rng.random(300).astype(np.float32)
+    theta = s.params()
+    g_gpu, loss_gpu = s.grads(masks, target)
+    loss_cpu, g_cpu = R.loss_and_grads(theta, masks, target)
+    assert abs(loss_gpu - loss_cpu) <= 1e-3 * max(1.0, abs(loss_cpu))
+    blocks = {"W1": (R.OFF_W1, R.OFF_B1), "b1": (R.OFF_B1, R.OFF_W2), "W2": (R.OFF_W2, R.OFF_B2),
+              "b2": (R.OFF_B2, R.OFF_W3), "w3": (R.OFF_W3, R.OFF_B3), "b3": (R.OFF_B3, R.N_PARAMS)}
+    for name, (a, b) in blocks.items():
+        gg, gc = g_gpu[a:b].astype(np.float64), g_cpu[a:b]
+        scale = np.max(np.abs(gc))
+        assert np.max(np.abs(gg - gc)) <= 3e-2 * scale + 1e-7, name
+        if b - a > 1:
+            cos = gg @ gc / (np.linalg.norm(gg) * np.linalg.norm(gc) + 1e-30)
+            assert cos >= 0.999, (name, cos)
+    y_gpu = s.predict(masks)
+    y_cpu, _ = R.forward(theta, R.features(masks))
+    assert np.allclose(y_gpu, y_cpu, rtol=1e-3, atol=1e-3)
+
+
+def test_trained_network_ranks_better_than_unranked():
+    """Trained on exact distances of k=8 sets, the network's top decile of a
+    held-out pool holds more high-distance codes than a random decile."""
+    from gtoric import _native
+    from gtoric.distance import flatten
+    from gtoric.surrogate import Surrogate
+    ctx = _native.context(0)
+    rng = np.random.default_rng(64)
+    pool = [sorted(rng.choice(49, 8, replace=False).tolist()) for _ in range(12_000)]
+    d = 49 - ctx.md_batch(*flatten(pool), 48)
+    masks = np.array([sum(1 << p for p in S) for S in pool], dtype=np.uint64)
+    tr, te = slice(0, 8000), slice(8000, None)
+    s = Surrogate(0, seed=3)
+    losses = s.fit(masks[tr], d[tr], epochs=25, batch=512, lr=2e-3)
+    assert losses[-1] < losses[0]
+    score = s.predict(masks[te])
+    dt = d[te]
+    top = np.argsort(-score, kind="stable")[: len(dt) // 10]
+    good = dt >= np.quantile(dt, 0.9)
+    frac_ranked = good[top].mean()
+    frac_random = good.mean()
+    rho = np.corrcoef(np.argsort(np.argsort(score)), np.argsort(np.argsort(dt)))[0, 1]
+    assert rho > 0.2, rho
+    assert frac_ranked > 1.5 * frac_random, (frac_ranked, frac_random)

@@ -1,0 +1,100 @@
+"""Persistent serving kernel: each result is the same alone and in the mix.
+
+Requests carry a stop threshold d_stop (tmax = 48 - d_stop); the served value is
+max_zeros clamped at tmax + 1, deterministic whatever else is in flight, so it
+must equal the one-request launch, the batch path and (small k) the oracle.
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from gtoric import _native
+    return _native.context(0)
+
+
+def _requests(n, seed, kmax=10):
+    from gtoric import workloads as W
+    w = {k: v for k, v in W.SERVE_K_WEIGHTS.items() if k <= kmax}
+    reqs = W.serving_requests(n, seed=seed, weights=w)
+    sets = [S for S, _ in reqs]
+    tmax = np.array([48 - d for _, d in reqs], dtype=np.int32)
+    return sets, tmax
+
+
+def test_stream_matches_alone_and_oracle(ctx):
+    from gtoric import workloads as W
+    from gtoric.distance import flatten
+    from gtoric.serving import Server
+    sets, tmax = _requests(600, seed=51, kmax=8)
+    arrivals = W.poisson_arrivals(len(sets), rate_per_s=20_000.0, seed=52)
+    with Server(0, slots=64, work_budget=1 << 24, life_s=60) as srv:
+        res, lat, wall = srv.run_stream(sets, tmax, arrivals, timeout_s=60)
+    alone = np.array([ctx.md_single(S, int(t)) for S, t in zip(sets, tmax)])
+    batch = ctx.md_batch(*flatten(sets), tmax)
+    exp = oracle.max_zeros_batch(sets, tmax)
+    assert res.tolist() == alone.tolist() == batch.tolist() == exp.tolist()
+    assert np.all(lat >= 0) and wall > 0
+
+
+def test_mixed_large_k_and_small_budget(ctx):
+    from gtoric.distance import flatten
+    from gtoric.serving import Server
+    sets, tmax = _requests(400, seed=53, kmax=10)
+    # Every other request exact (d_stop = 0): full enumeration, up to k = 10.
+    tmax[::2] = 48
+    arrivals = np.zeros(len(sets))          # all at once: admission does the pacing
+    with Server(0, slots=16, work_budget=1 << 20, life_s=60) as srv:
+        res, lat, wall = srv.run_stream(sets, tmax, arrivals, timeout_s=120)
+    batch = ctx.md_batch(*flatten(sets), tmax)
+    assert res.tolist() == batch.tolist()
+
+
+def test_submit_poll_and_relaunch(ctx):
+    from gtoric.serving import Server
+    sets, tmax = _requests(50, seed=54, kmax=7)
+    with Server(0, slots=8, work_budget=1 << 30, idle_s=0.2, life_s=60) as srv:
+        got = {}
+        deadline = time.time() + 60
+        for wave in range(2):
+            pending = list(range(len(sets)))
+            while pending or len(got) < len(sets) * (wave + 1):
+                assert time.time() < deadline, "serving did not complete"
+                while pending and srv.submit(sets[pending[0]], int(tmax[pending[0]]),
+                                             pending[0] + wave * 1000):
+                    pending.pop(0)
+                for tag, r in srv.poll():
+                    got[tag] = r
+            time.sleep(0.5)                  # the kernel stops itself when idle
+        assert srv.in_flight() == 0
+    exp = oracle.max_zeros_batch(sets, tmax).tolist()
+    assert [got[i] for i in range(len(sets))] == exp
+    assert [got[1000 + i] for i in range(len(sets))] == exp
+
+
+def test_serve_errors():
+    from gtoric import _native
+    from gtoric.serving import Server
+    with pytest.raises(_native.GtError):
+        Server(0, slots=3)
+    with Server(0, slots=4, life_s=60) as srv:
+        with pytest.raises(_native.GtError):
+            srv.submit(list(range(17)), 40, 0)
+        with pytest.raises(_native.GtError):
+            srv.submit([0, 50], 40, 0)
+        assert srv.submit([], 40, 7)
+        deadline = time.time() + 10
+        out = []
+        while not out and time.time() < deadline:
+            out = srv.poll()
+        assert out == [(7, 0)]
