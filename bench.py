@@ -114,6 +114,87 @@ def bench_canon(ctx, torch, masks, steps: int):
     return dt, nk, kms, cm, st
 
 
+def bench_serving(ctx, n_req: int, seed: int = 4):
+    """Config 4: Poisson stream of thresholded requests, k in 3..10 weighted
+    toward 6..8, d_stop uniform in [1, 50 - k].  Offered rate = 0.7 x the
+    served capacity measured on the same requests (all arrivals at t = 0)."""
+    from gtoric import workloads
+    from gtoric.serving import Server, plain_stream
+    reqs = workloads.serving_requests(n_req, seed=seed)
+    sets = [S for S, _ in reqs]
+    tmax = np.array([48 - d for _, d in reqs], dtype=np.int32)
+    out = {}
+    with Server(0, slots=1024, work_budget=1 << 36, life_s=600) as srv:
+        res0, _, wall0 = srv.run_stream(sets, tmax, np.zeros(n_req), timeout_s=300)
+        cap = n_req / wall0
+        rate = 0.7 * cap
+        arr = workloads.poisson_arrivals(n_req, rate, seed=seed + 1)
+        res, lat, wall = srv.run_stream(sets, tmax, arr, timeout_s=300)
+    pres0, _, pwall0 = plain_stream(ctx, sets, tmax, np.zeros(n_req))
+    pcap = n_req / pwall0
+    # The plain form at its own 0.7 x capacity (at the served rate it saturates).
+    parr = workloads.poisson_arrivals(n_req, 0.7 * pcap, seed=seed + 1)
+    pres, plat, pwall = plain_stream(ctx, sets, tmax, parr)
+    same = bool(np.array_equal(res, res0) and np.array_equal(res, pres) and
+                np.array_equal(res, pres0))
+    out.update({
+        "requests": n_req,
+        "served_capacity_codes_per_s": round(cap, 1),
+        "offered_rate_per_s": round(rate, 1),
+        "codes_per_s": round(n_req / wall, 1),
+        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
+        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
+        "plain_capacity_codes_per_s": round(pcap, 1),
+        "plain_p50_ms_at_0.7x_own_capacity": round(float(np.percentile(plat, 50)) * 1e3, 4),
+        "plain_p99_ms_at_0.7x_own_capacity": round(float(np.percentile(plat, 99)) * 1e3, 4),
+        "served_vs_plain_capacity": round(cap / pcap, 2),
+        "results_identical_served_and_plain": same,
+    })
+    return out
+
+
+def bench_evolve(ks, pop: int, generations: int, seed: int = 1):
+    """Config 5: unranked search for `generations`, then the surrogate-ranked
+    search for the same wall-clock; best exact distance per k."""
+    from gtoric import _native
+    from gtoric.evolve import run
+    from gtoric.surrogate import Surrogate
+    ctx = _native.context(0)
+    rows = {}
+    for k in ks:
+        base = run(k, pop=pop, generations=generations, seed=seed, ctx=ctx)
+        sur = Surrogate(0, seed=seed)
+        ranked = run(k, pop=pop, generations=generations * 10, seed=seed, surrogate=sur, ctx=ctx,
+                     wall_limit_s=base.wall_s)
+        sur.close()
+        rows[str(k)] = {
+            "unranked": {"best_d": base.best_d, "generations": base.generations,
+                         "verified_codes_per_s": round(base.codes_per_s, 1),
+                         "wall_s": round(base.wall_s, 2)},
+            "ranked": {"best_d": ranked.best_d, "generations": ranked.generations,
+                       "verified_codes_per_s": round(ranked.codes_per_s, 1),
+                       "wall_s": round(ranked.wall_s, 2)},
+        }
+        log(f"[bench] evolve k={k}: unranked d={base.best_d} ({base.generations} gens), "
+            f"ranked d={ranked.best_d} ({ranked.generations} gens), {base.wall_s:.1f}s each")
+    return rows
+
+
+def copy_calibration(torch, nbytes: int = 1 << 30, reps: int = 5):
+    """A plain device copy of known size, for scaling FETCH_SIZE/WRITE_SIZE."""
+    dev = torch.device("cuda", 0)
+    a = torch.empty(nbytes // 4, dtype=torch.int32, device=dev).fill_(1)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes,
+            "GB_per_s": round(2 * nbytes / dt / 1e9, 1)}
+
+
 def cpu_baseline(sets, cores: int, budget_s: float):
     """Oracle (C restatement of kernel.py:97-203, one set per thread) on a
     prefix of the workload sized to about budget_s seconds."""
@@ -139,7 +220,14 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--canon-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline")
-    ap.add_argument("--skip", default="", help="comma list of: plain,canon,cpu")
+    ap.add_argument("--skip", default="",
+                    help="comma list of: plain,canon,cpu,serve,evolve,tune")
+    ap.add_argument("--serve-requests", type=int, default=20_000)
+    ap.add_argument("--evolve-k", default="6,7,8,9,10")
+    ap.add_argument("--evolve-pop", type=int, default=2048)
+    ap.add_argument("--evolve-generations", type=int, default=200)
+    ap.add_argument("--mode", default="bench", choices=["bench", "copy"],
+                    help="copy: only the copy-kernel calibration run for --pmc traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -148,10 +236,17 @@ def main() -> None:
     skip = set(filter(None, args.skip.split(",")))
 
     import torch
-    from gtoric import _native, workloads
+    from gtoric import _native, autotune, workloads
 
-    ctx = _native.Context(0)
-    log(f"[bench] device {ctx.arch}: {ctx.cu_count} CUs, {ctx.clock_khz / 1e3:.0f} MHz")
+    if args.mode == "copy":
+        print(json.dumps({"copy_calibration": copy_calibration(torch)}), flush=True)
+        return
+
+    ctx = _native.context(0)
+    log(f"[bench] device {ctx.arch}: {ctx.cu_count} CUs, {ctx.clock_khz / 1e3:.0f} MHz, "
+        f"{ctx.lds_per_block} B LDS per workgroup")
+    tuned = autotune.apply(ctx) if "tune" not in skip else False
+    log(f"[bench] tuned parameters: {'loaded' if tuned else 'defaults'}")
     sets = workloads.mixed_k()
     n = len(sets)
     codewords = sum(proj(len(s)) for s in sets)
@@ -222,6 +317,17 @@ def main() -> None:
             "classes": classes,
         }
         log(f"[bench] canon: {len(masks) * args.canon_steps / cdt:,.0f} forms/s, {classes} classes")
+
+    if "serve" not in skip:
+        result["serving"] = bench_serving(ctx, args.serve_requests)
+        s = result["serving"]
+        log(f"[bench] serving: {s['codes_per_s']:,.0f} codes/s at {s['offered_rate_per_s']:,.0f}/s "
+            f"offered, p50 {s['p50_ms']} ms, p99 {s['p99_ms']} ms; plain capacity "
+            f"{s['plain_capacity_codes_per_s']:,.0f} codes/s")
+
+    if "evolve" not in skip:
+        ks = [int(x) for x in args.evolve_k.split(",") if x]
+        result["evolve"] = bench_evolve(ks, args.evolve_pop, args.evolve_generations)
 
     if "cpu" not in skip:
         cores = min(os.cpu_count() or 1, 16)

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/gtoric.h"
+#include "gt_bfs.h"
 #include "gt_launch.h"
 
 using namespace gt;
@@ -171,6 +172,7 @@ struct gt_ctx {
   DevBuf idx, offs, tmax, rows, cum, best, out, zmax, stopf;  // min distance
   DevBuf sbuf;                                                // single code
   DevBuf masks, order, hist, klo, khi, canon, stab;            // canonical
+  DevBuf children, bflag, bsum, bidx, bcnt, bmasks, brows, boffs;  // BFS level
 
   bool prof = false;
   std::map<std::string, ProfRec> prof_rec;
@@ -179,7 +181,8 @@ struct gt_ctx {
 
   std::vector<DevBuf*> all_bufs() {
     return {&mplanes, &params_buf, &perm, &ctrl, &idx, &offs, &tmax, &rows, &cum, &best,
-            &out, &zmax, &stopf, &sbuf, &masks, &order, &hist, &klo, &khi, &canon, &stab};
+            &out, &zmax, &stopf, &sbuf, &masks, &order, &hist, &klo, &khi, &canon, &stab,
+            &children, &bflag, &bsum, &bidx, &bcnt, &bmasks, &brows, &boffs};
   }
 
   int activate() {
@@ -586,6 +589,181 @@ int gt_graph_destroy(gt_graph* g) {
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   delete g;
+  return 0;
+}
+
+// ------------------------------------------------------- hash set / BFS --
+
+struct gt_hashset {
+  gt_ctx* ctx = nullptr;
+  uint64_t cap = 0;
+  uint64_t* table = nullptr;
+  uint64_t* tags = nullptr;
+  uint32_t* count = nullptr;
+  uint64_t call = 0;
+};
+
+int gt_hashset_create(gt_ctx* c, int32_t log2_cap, gt_hashset** out) {
+  if (!c || !out) return fail("null argument");
+  if (log2_cap < 4 || log2_cap > 30) return fail("log2 capacity must be in 4..30 (got %d)", log2_cap);
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  gt_hashset* h = new gt_hashset();
+  h->ctx = c;
+  h->cap = 1ull << log2_cap;
+  if (hipMalloc(&h->table, h->cap * 8) != hipSuccess || hipMalloc(&h->tags, h->cap * 8) != hipSuccess ||
+      hipMalloc(&h->count, 256) != hipSuccess) {
+    if (h->table) (void)hipFree(h->table);
+    if (h->tags) (void)hipFree(h->tags);
+    delete h;
+    return fail("hipMalloc of a hash set with 2^%d slots failed", log2_cap);
+  }
+  if (launch_hs_clear(h->table, h->tags, h->cap, h->count, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    (void)hipFree(h->table);
+    (void)hipFree(h->tags);
+    (void)hipFree(h->count);
+    delete h;
+    return fail("hash set initialisation failed");
+  }
+  *out = h;
+  return 0;
+}
+
+int gt_hashset_destroy(gt_hashset* h) {
+  if (!h) return 0;
+  {
+    std::lock_guard<std::mutex> lk(h->ctx->mu);
+    (void)h->ctx->activate();
+    (void)hipStreamSynchronize(h->ctx->stream);
+    (void)hipFree(h->table);
+    (void)hipFree(h->tags);
+    (void)hipFree(h->count);
+  }
+  delete h;
+  return 0;
+}
+
+int gt_hashset_clear(gt_hashset* h, void* stream) {
+  if (!h) return fail("null hash set");
+  gt_ctx* c = h->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  GT_HIP(launch_hs_clear(h->table, h->tags, h->cap, h->count, c->pick(stream)));
+  h->call = 0;
+  return 0;
+}
+
+int gt_hashset_count(gt_hashset* h, int64_t* n) {
+  if (!h || !n) return fail("null argument");
+  gt_ctx* c = h->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  uint32_t v = 0;
+  GT_HIP(hipMemcpyAsync(&v, h->count, 4, hipMemcpyDeviceToHost, c->stream));
+  GT_HIP(hipStreamSynchronize(c->stream));
+  *n = v;
+  return 0;
+}
+
+static int dedup_enqueue(gt_hashset* h, const uint64_t* d_keys, int32_t n, uint32_t* d_flag,
+                         hipStream_t s) {
+  if ((uint64_t)n >= (1ull << 32)) return fail("batch too large");
+  ++h->call;
+  GT_HIP(launch_hs_dedup(d_keys, n, h->table, h->tags, h->cap, h->call, h->count, d_flag,
+                         h->ctx->ctrl.as<uint32_t>(), s));
+  return 0;
+}
+
+int gt_dedup_dev(gt_hashset* h, const uint64_t* d_keys, int32_t n, uint32_t* d_flag, void* stream) {
+  if (!h || (!d_keys && n > 0) || (!d_flag && n > 0) || n < 0) return fail("bad argument");
+  if (n == 0) return 0;
+  gt_ctx* c = h->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  return dedup_enqueue(h, d_keys, n, d_flag, c->pick(stream));
+}
+
+int gt_dedup(gt_hashset* h, const uint64_t* keys, int32_t n, uint32_t* flag) {
+  if (!h || !keys || !flag || n < 0) return fail("bad argument");
+  if (n == 0) return 0;
+  gt_ctx* c = h->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  hipStream_t s = c->stream;
+  GT_OK(c->bmasks.ensure((size_t)n * 8));
+  GT_OK(c->bflag.ensure((size_t)n * 4));
+  GT_HIP(hipMemcpyAsync(c->bmasks.p, keys, (size_t)n * 8, hipMemcpyHostToDevice, s));
+  GT_OK(dedup_enqueue(h, c->bmasks.as<uint64_t>(), n, c->bflag.as<uint32_t>(), s));
+  GT_HIP(hipMemcpyAsync(flag, c->bflag.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  GT_OK(c->check_ctrl(s));
+  return 0;
+}
+
+int gt_compact_dev(gt_ctx* c, const uint32_t* d_flag, int32_t n, uint32_t* d_out_idx,
+                   uint32_t* d_count, void* stream) {
+  if (!c || n < 0) return fail("bad argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  GT_OK(c->bsum.ensure((size_t)((n + 1023) / 1024 + 1) * 4));
+  GT_HIP(launch_compact(d_flag, n, c->bsum.as<uint32_t>(), d_out_idx, d_count, c->pick(stream)));
+  return 0;
+}
+
+// One canonical BFS level: children of the n_par parents (canonical masks of
+// size kp), canonical forms, classes not yet in `seen` (first occurrence in
+// input order), their max_zeros at threshold tmax.  Writes at most cap
+// classes; *n_new is the full count (the call fails if it exceeds cap).
+int gt_bfs_level(gt_ctx* c, gt_hashset* seen, const uint64_t* d_par, int32_t n_par, int32_t kp,
+                 int32_t tmax, uint64_t* d_out_masks, int32_t* d_out_mz, int32_t cap,
+                 int32_t* n_new) {
+  if (!c || !seen || !n_new || n_par < 0 || kp < 0 || kp + 1 > kMaxKCanon)
+    return fail("bad argument (parent size must be 0..%d)", kMaxKCanon - 1);
+  *n_new = 0;
+  if (n_par == 0) return 0;
+  if (seen->ctx != c) return fail("hash set belongs to another context");
+  const int64_t nch = (int64_t)n_par * (kNpts - kp);
+  if (nch >= (1ll << 31)) return fail("level too large (%lld children)", (long long)nch);
+  uint32_t nu = 0;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    GT_OK(c->activate());
+    hipStream_t s = c->stream;
+    const size_t n = (size_t)nch;
+    GT_OK(c->children.ensure(n * 8));
+    GT_OK(c->klo.ensure(n * 8));
+    GT_OK(c->khi.ensure(n * 8));
+    GT_OK(c->canon.ensure(n * 8));
+    GT_OK(c->stab.ensure(n * 4));
+    GT_OK(c->bflag.ensure(n * 4));
+    GT_OK(c->bidx.ensure(n * 4));
+    GT_OK(c->bcnt.ensure(64));
+    GT_OK(c->bsum.ensure((n + 1023) / 1024 * 4 + 4));
+    GT_HIP(launch_bfs_expand(d_par, n_par, kp, c->children.as<uint64_t>(), c->ctrl.as<uint32_t>(), s));
+    GT_OK(canon_enqueue(c, c->children.as<uint64_t>(), (int32_t)n, c->klo.as<uint64_t>(),
+                        c->khi.as<uint64_t>(), c->canon.as<uint64_t>(), c->stab.as<int32_t>(), s));
+    GT_OK(dedup_enqueue(seen, c->canon.as<uint64_t>(), (int32_t)n, c->bflag.as<uint32_t>(), s));
+    GT_HIP(launch_compact(c->bflag.as<uint32_t>(), (int32_t)n, c->bsum.as<uint32_t>(),
+                          c->bidx.as<uint32_t>(), c->bcnt.as<uint32_t>(), s));
+    GT_HIP(hipMemcpyAsync(&nu, c->bcnt.p, 4, hipMemcpyDeviceToHost, s));
+    GT_OK(c->check_ctrl(s));
+  }
+  *n_new = (int32_t)nu;
+  if ((int64_t)nu > cap) return fail("%u new classes exceed the output capacity %d", nu, cap);
+  if (nu == 0) return 0;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t s = c->stream;
+  const int k = kp + 1;
+  GT_OK(c->bmasks.ensure((size_t)nu * 8));
+  GT_OK(c->brows.ensure((size_t)nu * k * 4));
+  GT_OK(c->boffs.ensure((size_t)(nu + 1) * 4));
+  GT_HIP(launch_gather_u64(c->canon.as<uint64_t>(), c->bidx.as<uint32_t>(), c->bcnt.as<uint32_t>(),
+                           d_out_masks, (int32_t)nu, s));
+  GT_HIP(launch_masks_to_rows(d_out_masks, (int32_t)nu, k, c->brows.as<int32_t>(),
+                              c->boffs.as<int32_t>(), c->ctrl.as<uint32_t>(), s));
+  GT_OK(c->md_enqueue(c->brows.as<int32_t>(), (int64_t)nu * k, c->boffs.as<int32_t>(), (int32_t)nu,
+                      nullptr, tmax, tmax < 48, 0, 0, d_out_mz, s));
+  GT_OK(c->check_ctrl(s));
   return 0;
 }
 

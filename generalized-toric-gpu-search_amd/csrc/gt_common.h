@@ -90,6 +90,7 @@ GT_HD void mul_x(uint32_t w[6]) {
 enum : uint32_t {
   kErrBadIndex = 1u,
   kErrBadK = 2u,
+  kErrFull = 4u,   // hash set has no free slot
 };
 
 }  // namespace gt

@@ -65,6 +65,14 @@ _SIGS = {
     "gt_mlp_predict": ([vp, vp, i32, vp], ctypes.c_int),
     "gt_mlp_predict_dev": ([vp, vp, i32, vp, vp], ctypes.c_int),
     "gt_bf16_gemm": ([i32, vp, vp, vp, i32, i32, i32], ctypes.c_int),
+    "gt_hashset_create": ([vp, i32, P(vp)], ctypes.c_int),
+    "gt_hashset_destroy": ([vp], ctypes.c_int),
+    "gt_hashset_clear": ([vp, vp], ctypes.c_int),
+    "gt_hashset_count": ([vp, P(i64)], ctypes.c_int),
+    "gt_dedup": ([vp, vp, i32, vp], ctypes.c_int),
+    "gt_dedup_dev": ([vp, vp, i32, vp, vp], ctypes.c_int),
+    "gt_compact_dev": ([vp, vp, i32, vp, vp, vp], ctypes.c_int),
+    "gt_bfs_level": ([vp, vp, vp, i32, i32, i32, vp, vp, i32, P(i32)], ctypes.c_int),
     "gt_sync": ([vp, vp], ctypes.c_int),
     "gt_prof_enable": ([vp, i32], ctypes.c_int),
     "gt_prof_read": ([vp, ctypes.c_char_p, P(i32), P(ctypes.c_double)], ctypes.c_int),

@@ -164,6 +164,31 @@ int gt_mlp_predict_dev(gt_mlp* mlp, const uint64_t* d_masks, int32_t n, float* d
 int gt_bf16_gemm(int32_t device, const uint16_t* A, const uint16_t* B, float* C, int32_t M,
                  int32_t N, int32_t K);
 
+/* Device hash set of nonzero 64-bit keys (canonical masks) and one canonical
+ * BFS level on the device (replaces the host loops of
+ * champion_search_canonical.py:496-506 and :308-345).  gt_dedup marks
+ * flag[i] = 1 when key i is new: not inserted by an earlier call and the
+ * first occurrence within this call; every key is inserted. */
+typedef struct gt_hashset gt_hashset;
+int gt_hashset_create(gt_ctx* ctx, int32_t log2_capacity, gt_hashset** out);
+int gt_hashset_destroy(gt_hashset* set);
+int gt_hashset_clear(gt_hashset* set, void* stream);
+int gt_hashset_count(gt_hashset* set, int64_t* n);
+int gt_dedup(gt_hashset* set, const uint64_t* keys, int32_t n, uint32_t* flag);
+int gt_dedup_dev(gt_hashset* set, const uint64_t* d_keys, int32_t n, uint32_t* d_flag,
+                 void* stream);
+/* Ordered compaction: d_out_idx[0 .. *d_count) = indices i with d_flag[i] != 0. */
+int gt_compact_dev(gt_ctx* ctx, const uint32_t* d_flag, int32_t n, uint32_t* d_out_idx,
+                   uint32_t* d_count, void* stream);
+/* Children of n_par parent masks of size kp (each parent plus one outside
+ * point), their canonical masks, the classes not yet in `seen` (first
+ * occurrence in child order) into d_out_masks, and their max_zeros at
+ * threshold tmax into d_out_mz.  *n_new = number of new classes (error if
+ * above cap).  Synchronous. */
+int gt_bfs_level(gt_ctx* ctx, gt_hashset* seen, const uint64_t* d_parents, int32_t n_par,
+                 int32_t kp, int32_t tmax, uint64_t* d_out_masks, int32_t* d_out_mz,
+                 int32_t cap, int32_t* n_new);
+
 /* Wait for `stream` (NULL = context stream) and report device-side input
  * errors (bad index, bad size) raised by *_dev calls since the last sync. */
 int gt_sync(gt_ctx* ctx, void* stream);

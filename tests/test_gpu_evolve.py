@@ -32,14 +32,3 @@ def test_evolve_with_surrogate_runs_and_is_exact():
     res = run(7, pop=256, generations=6, seed=4, surrogate=s, ctx=ctx, train_every=2)
     d = 49 - ctx.md_batch(*flatten([res.best_set]), 48)[0]
     assert d == res.best_d and res.generations == 6
-
-
-def test_mutation_keeps_size_and_changes_points():
-    from gtoric.evolve import mutate, random_masks
-    rng = np.random.default_rng(0)
-    par = random_masks(rng, 500, 9)
-    ch = mutate(rng, par)
-    pc = np.array([bin(int(m)).count("1") for m in ch])
-    assert np.all(pc == 9)
-    diff = np.array([bin(int(a) ^ int(b)).count("1") for a, b in zip(par, ch)])
-    assert set(diff.tolist()) <= {2, 4} and np.all(diff > 0)
