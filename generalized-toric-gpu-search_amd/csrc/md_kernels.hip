@@ -105,11 +105,17 @@ __global__ __launch_bounds__(256) void md_main_kernel(
     const MdParams* __restrict__ p) {
   const uint32_t lane = lane_id();
   const uint32_t total = cum[n];
-  for (;;) {
-    uint32_t g = 0;
-    if (lane == 0) g = atomicAdd(&ctrl[0], 1u);
-    g = __builtin_amdgcn_readfirstlane(g);
-    if (g >= total) break;
+  // Chunk claims: lane 0 takes the next id and the wave reads it with
+  // readfirstlane.  Each iteration has exactly one lane-0 region — it folds
+  // the finished chunk into its code and claims the next id — and no
+  // `continue`, so the loop stays wave-uniform: when a separate lane-0 claim
+  // at the loop top followed a lane-0 atomic at the loop bottom, the compiler
+  // threaded the two regions together and the other 63 lanes re-read a stale
+  // id with lane 0 inactive, walking chunk 0 forever.
+  uint32_t first = 0;
+  if (lane == 0) first = atomicAdd(&ctrl[0], 1u);
+  uint32_t g = __builtin_amdgcn_readfirstlane(first);
+  while (g < total) {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -121,14 +127,24 @@ __global__ __launch_bounds__(256) void md_main_kernel(
     const uint32_t cmask = (1u << p->check_log2[k]) - 1u;
     const uint32_t* R = rows + (size_t)offsets[code] * 3 * kRowWords;
     int thr = -1;
+    bool skip = false;
     if (THR) {
       thr = 48 - (tmax_arr ? tmax_arr[code] : tmax_s);
-      if ((int)__hip_atomic_load(&best_w[code], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= thr)
-        continue;
+      const uint32_t bw = __builtin_amdgcn_readfirstlane(
+          __hip_atomic_load(&best_w[code], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      skip = (int)bw <= thr;
     }
-    uint32_t b = run_chunk<THR>(ConstRows{R}, k, L, g - cum[code], lane, thr, cmask, &best_w[code]);
-    b = wave_min(b);
-    if (lane == 0) atomicMin(&best_w[code], b);
+    uint32_t b = 64u;
+    if (!skip) {
+      b = run_chunk<THR>(ConstRows{R}, k, L, g - cum[code], lane, thr, cmask, &best_w[code]);
+      b = wave_min(b);
+    }
+    uint32_t next = 0;
+    if (lane == 0) {
+      if (!skip) atomicMin(&best_w[code], b);
+      next = atomicAdd(&ctrl[0], 1u);
+    }
+    g = __builtin_amdgcn_readfirstlane(next);
   }
 }
 

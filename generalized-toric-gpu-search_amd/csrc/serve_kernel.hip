@@ -49,8 +49,8 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, const uint32_t* __restrict
   const uint64_t ord_mask = 4ull * slots - 1ull;
   for (;;) {
     const uint64_t now = wall_clock64();
-    const bool stop = ld_acq_sys(&h->shutdown) != 0u || now - t_idle > d->idle_ticks ||
-                      now - t_start > d->life_ticks;
+    const bool stop = __builtin_amdgcn_readfirstlane(ld_acq_sys(&h->shutdown)) != 0u ||
+                      now - t_idle > d->idle_ticks || now - t_start > d->life_ticks;
     if (stop) {
       if (lane == 0) {
         __hip_atomic_store(&d->shutdown, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -59,12 +59,14 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, const uint32_t* __restrict
       return;
     }
     const uint64_t pub = ld_acq_sys(&h->published);
-    if (pub <= disp) {
+    const uint64_t pubu = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(pub >> 32)) << 32) |
+                          __builtin_amdgcn_readfirstlane((uint32_t)pub);
+    if (pubu <= disp) {
       __builtin_amdgcn_s_sleep(8);
       continue;
     }
     t_idle = now;
-    const uint64_t n_new = min(pub - disp, (uint64_t)kWave);
+    const uint64_t n_new = min(pubu - disp, (uint64_t)kWave);
     const uint64_t r = disp + lane;
     const uint32_t slot = (uint32_t)(r & (slots - 1));
     uint32_t nch = 0;
@@ -121,45 +123,63 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, const uint32_t* __restrict
   }
 }
 
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+         __builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+
+// Claims as in md_main_kernel: one lane-0 region per iteration (it completes
+// the previous chunk and claims the next id), read back with readfirstlane
+// while every lane is active; all other conditions are made wave-uniform.
 __device__ void worker(ServeDev* d, ServeHost* h, uint32_t lane) {
   const uint64_t slots = d->slots;
   const uint64_t ord_mask = 4ull * slots - 1ull;
+  uint64_t first = 0;
+  if (lane == 0) first = atomicAdd((unsigned long long*)&d->next_chunk, 1ull);
+  uint64_t g = uniform64(first);
   for (;;) {
-    uint64_t g = 0;
-    if (lane == 0) g = atomicAdd((unsigned long long*)&d->next_chunk, 1ull);
-    g = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(g >> 32)) << 32) |
-        __builtin_amdgcn_readfirstlane((uint32_t)g);
     // Wait until chunk g is published, or the server stops.
+    bool stop = false;
     for (;;) {
-      const uint64_t av = __hip_atomic_load(&d->avail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      const uint64_t av = uniform64(
+          __hip_atomic_load(&d->avail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
       if (g < av) break;
-      if (__hip_atomic_load(&d->shutdown, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) return;
+      if (__builtin_amdgcn_readfirstlane(
+              __hip_atomic_load(&d->shutdown, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT))) {
+        stop = true;
+        break;
+      }
       __builtin_amdgcn_s_sleep(4);
     }
+    if (stop) return;
     // Request of chunk g: last r in the in-flight window with ord_start <= g.
-    const uint64_t disp = __hip_atomic_load(&d->dispatched, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t disp = uniform64(
+        __hip_atomic_load(&d->dispatched, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
     uint64_t lo = (disp > slots) ? disp - slots : 0, hi = disp - 1;
     while (lo < hi) {
       const uint64_t mid = (lo + hi + 1) >> 1;
-      if (ld_rlx(&d->ord_start[mid & ord_mask]) <= g) lo = mid; else hi = mid - 1;
+      if (uniform64(ld_rlx(&d->ord_start[mid & ord_mask])) <= g) lo = mid; else hi = mid - 1;
     }
     const uint32_t slot = (uint32_t)(lo & (slots - 1));
-    const int k = ld_rlx(&d->s_k[slot]);
-    const int tmax = ld_rlx(&d->s_tmax[slot]);
-    const uint32_t nch = ld_rlx(&d->s_nch[slot]);
-    const uint64_t seq = ld_rlx(&d->s_seq[slot]);
+    const int k = (int)__builtin_amdgcn_readfirstlane((uint32_t)ld_rlx(&d->s_k[slot]));
+    const int tmax = (int)__builtin_amdgcn_readfirstlane((uint32_t)ld_rlx(&d->s_tmax[slot]));
+    const uint32_t nch = __builtin_amdgcn_readfirstlane(ld_rlx(&d->s_nch[slot]));
+    const uint64_t seq = uniform64(ld_rlx(&d->s_seq[slot]));
+    const uint64_t local = g - uniform64(ld_rlx(&d->ord_start[lo & ord_mask]));
     const int thr = 48 - tmax;
-    const uint64_t local = g - ld_rlx(&d->ord_start[lo & ord_mask]);
     uint32_t* best_g = &d->s_best[slot];
-    if ((int)ld_rlx(best_g) > thr) {
+    const bool skip = (int)__builtin_amdgcn_readfirstlane(ld_rlx(best_g)) <= thr;
+    uint32_t b = 64u;
+    if (!skip) {
       const int L = d->L[k];
       const uint32_t cmask = (1u << d->check_log2[k]) - 1u;
       const CoherentRows rows{d->rows + (size_t)slot * kServeRowWords};
-      uint32_t b = run_chunk<true>(rows, k, L, local, lane, thr, cmask, best_g);
+      b = run_chunk<true>(rows, k, L, local, lane, thr, cmask, best_g);
       b = wave_min(b);
-      if (lane == 0) atomicMin(best_g, b);
     }
+    uint64_t next = 0;
     if (lane == 0) {
+      if (!skip) atomicMin(best_g, b);
       __threadfence();
       const uint32_t done = atomicAdd(&d->s_done[slot], 1u) + 1u;
       if (done == nch) {
@@ -169,7 +189,9 @@ __device__ void worker(ServeDev* d, ServeHost* h, uint32_t lane) {
         __hip_atomic_store(&h->results[slot].seq, seq, __ATOMIC_RELEASE,
                            __HIP_MEMORY_SCOPE_SYSTEM);
       }
+      next = atomicAdd((unsigned long long*)&d->next_chunk, 1ull);
     }
+    g = uniform64(next);
   }
 }
 
