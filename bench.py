@@ -39,37 +39,36 @@ def proj(k: int) -> int:
     return (8 ** k - 1) // 7
 
 
-def bench_md(ctx, torch, sets, steps: int, warmup: int, expect=None):
+def bench_md(ctx, sets, steps: int, warmup: int, expect=None):
     from gtoric.distance import flatten
     flat, offs = flatten(sets)
     n = len(sets)
-    dev = torch.device("cuda", 0)
-    d_idx = torch.from_numpy(flat).to(dev)
-    d_off = torch.from_numpy(offs).to(dev)
-    d_out = torch.zeros(n, dtype=torch.int32, device=dev)
-    stream = torch.cuda.current_stream(dev).cuda_stream
+    d_idx = ctx.to_device(flat)
+    d_off = ctx.to_device(offs)
+    d_out = ctx.array(n, np.int32)
 
     def step():
-        ctx.md_batch_dev(d_idx.data_ptr(), len(flat), d_off.data_ptr(), n, d_out.data_ptr(),
-                         tmax=48, thresholded=False, stream=stream)
+        ctx.md_batch_dev(d_idx.ptr, len(flat), d_off.ptr, n, d_out.ptr, tmax=48,
+                         thresholded=False)
 
     for _ in range(warmup):
         step()
-    ctx.sync(stream)
-    got = d_out.cpu().numpy()
+    ctx.sync()
+    got = d_out.download()
     if expect is not None and not np.array_equal(got, expect):
         raise SystemExit(f"parity failure: {int(np.sum(got != expect))} codes differ from the oracle")
     ctx.prof_read("md_main")
     ctx.prof_enable(True)
-    torch.cuda.synchronize()
+    ctx.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
-    torch.cuda.synchronize()
+    ctx.sync()
     dt = time.perf_counter() - t0
     ctx.prof_enable(False)
-    ctx.sync(stream)
     nk, kms = ctx.prof_read("md_main")
+    for a in (d_idx, d_off, d_out):
+        a.close()
     return dt, nk, kms, got
 
 
@@ -82,35 +81,30 @@ def bench_plain(ctx, sets, expect=None):
     return dt
 
 
-def bench_canon(ctx, torch, masks, steps: int):
+def bench_canon(ctx, masks, steps: int):
     n = len(masks)
-    dev = torch.device("cuda", 0)
-    d_m = torch.from_numpy(masks.view(np.int64)).to(dev)
-    d_lo = torch.empty(n, dtype=torch.int64, device=dev)
-    d_hi = torch.empty(n, dtype=torch.int64, device=dev)
-    d_cm = torch.empty(n, dtype=torch.int64, device=dev)
-    d_st = torch.empty(n, dtype=torch.int32, device=dev)
-    stream = torch.cuda.current_stream(dev).cuda_stream
+    d_m = ctx.to_device(masks)
+    d_lo, d_hi, d_cm = (ctx.array(n, np.uint64) for _ in range(3))
+    d_st = ctx.array(n, np.int32)
 
     def step():
-        ctx.canon_masks_dev(d_m.data_ptr(), n, d_lo.data_ptr(), d_hi.data_ptr(), d_cm.data_ptr(),
-                            d_st.data_ptr(), stream=stream)
+        ctx.canon_masks_dev(d_m.ptr, n, d_lo.ptr, d_hi.ptr, d_cm.ptr, d_st.ptr)
 
     step()
-    ctx.sync(stream)
+    ctx.sync()
     ctx.prof_read("canon")
     ctx.prof_enable(True)
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
-    torch.cuda.synchronize()
+    ctx.sync()
     dt = time.perf_counter() - t0
     ctx.prof_enable(False)
-    ctx.sync(stream)
     nk, kms = ctx.prof_read("canon")
-    cm = d_cm.cpu().numpy().view(np.uint64)
-    st = d_st.cpu().numpy()
+    cm = d_cm.download()
+    st = d_st.download()
+    for a in (d_m, d_lo, d_hi, d_cm, d_st):
+        a.close()
     return dt, nk, kms, cm, st
 
 
@@ -180,19 +174,25 @@ def bench_evolve(ks, pop: int, generations: int, seed: int = 1):
     return rows
 
 
-def copy_calibration(torch, nbytes: int = 1 << 30, reps: int = 5):
-    """A plain device copy of known size, for scaling FETCH_SIZE/WRITE_SIZE."""
-    dev = torch.device("cuda", 0)
-    a = torch.empty(nbytes // 4, dtype=torch.int32, device=dev).fill_(1)
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
+def copy_calibration(ctx, nbytes: int = 1 << 30, reps: int = 5):
+    """A plain device copy kernel of known size, for scaling FETCH_SIZE/WRITE_SIZE."""
+    a = ctx.array(nbytes // 4, np.uint32)
+    b = ctx.array(nbytes // 4, np.uint32)
+    a.fill_u32(1)
+    ctx.copy_dev(b.ptr, a.ptr, nbytes)
+    ctx.sync()
+    ctx.prof_read("copy")
+    ctx.prof_enable(True)
     for _ in range(reps):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / reps
-    return {"bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes,
-            "GB_per_s": round(2 * nbytes / dt / 1e9, 1)}
+        ctx.copy_dev(b.ptr, a.ptr, nbytes)
+    ctx.sync()
+    ctx.prof_enable(False)
+    nk, ms = ctx.prof_read("copy")
+    a.close()
+    b.close()
+    return {"kernel": "copy16_kernel", "bytes_read_per_copy": nbytes,
+            "bytes_written_per_copy": nbytes, "launches": nk,
+            "GB_per_s": round(2 * nbytes / (ms / nk * 1e-3) / 1e9, 1)}
 
 
 def cpu_baseline(sets, cores: int, budget_s: float):
@@ -235,14 +235,13 @@ def main() -> None:
         raise SystemExit(2)
     skip = set(filter(None, args.skip.split(",")))
 
-    import torch
     from gtoric import _native, autotune, workloads
 
+    ctx = _native.context(0)
     if args.mode == "copy":
-        print(json.dumps({"copy_calibration": copy_calibration(torch)}), flush=True)
+        print(json.dumps({"copy_calibration": copy_calibration(ctx)}), flush=True)
         return
 
-    ctx = _native.context(0)
     log(f"[bench] device {ctx.arch}: {ctx.cu_count} CUs, {ctx.clock_khz / 1e3:.0f} MHz, "
         f"{ctx.lds_per_block} B LDS per workgroup")
     tuned = autotune.apply(ctx) if "tune" not in skip else False
@@ -253,7 +252,7 @@ def main() -> None:
     gold = ROOT / "tests" / "golden" / "md_golden.npz"
     expect = np.load(gold)["c2_mz"] if gold.exists() else None
 
-    dt, nk, kms, got = bench_md(ctx, torch, sets, args.steps, args.warmup, expect)
+    dt, nk, kms, got = bench_md(ctx, sets, args.steps, args.warmup, expect)
     value = n * args.steps / dt
     ms_step = dt / args.steps * 1e3
     kern_ms = kms / max(nk, 1)
@@ -308,7 +307,7 @@ def main() -> None:
 
     if "canon" not in skip:
         masks = workloads.canon_masks(1_000_000)
-        cdt, cnk, ckms, cm, st = bench_canon(ctx, torch, masks, args.canon_steps)
+        cdt, cnk, ckms, cm, st = bench_canon(ctx, masks, args.canon_steps)
         classes = int(np.unique(cm).size)
         result["canon"] = {
             "value": round(len(masks) * args.canon_steps / cdt, 1), "unit": "canon forms/s",

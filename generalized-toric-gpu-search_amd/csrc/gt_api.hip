@@ -767,6 +767,70 @@ int gt_bfs_level(gt_ctx* c, gt_hashset* seen, const uint64_t* d_par, int32_t n_p
   return 0;
 }
 
+// ------------------------------------------------------ device memory --
+
+int gt_dev_alloc(gt_ctx* c, int64_t bytes, void** out) {
+  if (!c || !out || bytes < 0) return fail("bad argument");
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  GT_HIP(hipMalloc(out, (size_t)std::max<int64_t>(bytes, 256)));
+  return 0;
+}
+
+int gt_dev_free(gt_ctx* c, void* p) {
+  if (!c) return fail("null context");
+  if (!p) return 0;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  GT_HIP(hipStreamSynchronize(c->stream));
+  GT_HIP(hipFree(p));
+  return 0;
+}
+
+int gt_memcpy_h2d(gt_ctx* c, void* d, const void* h, int64_t bytes, void* stream) {
+  if (!c || (bytes > 0 && (!d || !h)) || bytes < 0) return fail("bad argument");
+  if (bytes == 0) return 0;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  hipStream_t s = c->pick(stream);
+  GT_HIP(hipMemcpyAsync(d, h, (size_t)bytes, hipMemcpyHostToDevice, s));
+  GT_HIP(hipStreamSynchronize(s));   // the host buffer may be reused on return
+  return 0;
+}
+
+int gt_memcpy_d2h(gt_ctx* c, void* h, const void* d, int64_t bytes, void* stream) {
+  if (!c || (bytes > 0 && (!d || !h)) || bytes < 0) return fail("bad argument");
+  if (bytes == 0) return 0;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  hipStream_t s = c->pick(stream);
+  GT_HIP(hipMemcpyAsync(h, d, (size_t)bytes, hipMemcpyDeviceToHost, s));
+  GT_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int gt_fill_u32_dev(gt_ctx* c, uint32_t* d, int64_t n, uint32_t value, void* stream) {
+  if (!c || (n > 0 && !d) || n < 0) return fail("bad argument");
+  if (n == 0) return 0;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  GT_HIP(launch_fill_u32(d, n, value, c->pick(stream)));
+  return 0;
+}
+
+int gt_copy_dev(gt_ctx* c, void* dst, const void* src, int64_t bytes, void* stream) {
+  if (!c || bytes < 0 || (bytes % 16) != 0) return fail("copy size must be a multiple of 16");
+  if (bytes == 0) return 0;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  hipEvent_t e0 = nullptr;
+  GT_OK(c->prof_begin("copy", c->pick(stream), &e0));
+  GT_HIP(launch_copy16(dst, src, bytes, c->pick(stream)));
+  GT_OK(c->prof_end("copy", c->pick(stream), e0));
+  return 0;
+}
+
 int gt_sync(gt_ctx* c, void* stream) {
   if (!c) return fail("null context");
   std::lock_guard<std::mutex> lk(c->mu);

@@ -40,6 +40,10 @@ hipError_t launch_canon(const uint64_t* d_masks, int32_t n, const uint8_t* d_per
                         uint32_t* d_order, uint32_t* d_hist, uint64_t* d_key_lo,
                         uint64_t* d_key_hi, uint64_t* d_canon, int32_t* d_stab,
                         uint32_t* d_ctrl, hipStream_t s);
+// ---- memory ---------------------------------------------------------------
+hipError_t launch_fill_u32(uint32_t* d, int64_t n, uint32_t value, hipStream_t s);
+hipError_t launch_copy16(void* dst, const void* src, int64_t bytes, hipStream_t s);
+
 hipError_t launch_sets_to_masks(const int32_t* d_sets, int32_t n, int32_t k, uint64_t* d_masks,
                                 uint32_t* d_ctrl, hipStream_t s);
 int canon_perm_bytes();

@@ -73,6 +73,12 @@ _SIGS = {
     "gt_dedup_dev": ([vp, vp, i32, vp, vp], ctypes.c_int),
     "gt_compact_dev": ([vp, vp, i32, vp, vp, vp], ctypes.c_int),
     "gt_bfs_level": ([vp, vp, vp, i32, i32, i32, vp, vp, i32, P(i32)], ctypes.c_int),
+    "gt_dev_alloc": ([vp, i64, P(vp)], ctypes.c_int),
+    "gt_dev_free": ([vp, vp], ctypes.c_int),
+    "gt_memcpy_h2d": ([vp, vp, vp, i64, vp], ctypes.c_int),
+    "gt_memcpy_d2h": ([vp, vp, vp, i64, vp], ctypes.c_int),
+    "gt_fill_u32_dev": ([vp, vp, i64, ctypes.c_uint32, vp], ctypes.c_int),
+    "gt_copy_dev": ([vp, vp, vp, i64, vp], ctypes.c_int),
     "gt_sync": ([vp, vp], ctypes.c_int),
     "gt_prof_enable": ([vp, i32], ctypes.c_int),
     "gt_prof_read": ([vp, ctypes.c_char_p, P(i32), P(ctypes.c_double)], ctypes.c_int),
@@ -217,6 +223,16 @@ class Context:
         check(lib().gt_canon_masks_dev(self.handle, vp(d_masks), int(n), vp(d_lo), vp(d_hi),
                                        vp(d_canon), vp(d_stab), vp(stream) if stream else None))
 
+    def copy_dev(self, dst: int, src: int, nbytes: int) -> None:
+        """Device-to-device copy kernel (size a multiple of 16 bytes)."""
+        check(lib().gt_copy_dev(self.handle, vp(dst), vp(src), int(nbytes), None))
+
+    def array(self, n: int, dtype) -> "DeviceArray":
+        return DeviceArray(self, n, dtype)
+
+    def to_device(self, a: np.ndarray) -> "DeviceArray":
+        return DeviceArray.from_numpy(self, a)
+
     # ---- graphs ----------------------------------------------------------
     def capture_begin(self) -> None:
         check(lib().gt_capture_begin(self.handle))
@@ -237,6 +253,57 @@ class Context:
         n, ms = i32(), ctypes.c_double()
         check(lib().gt_prof_read(self.handle, kernel.encode(), ctypes.byref(n), ctypes.byref(ms)))
         return n.value, ms.value
+
+
+class DeviceArray:
+    """A device buffer of one context (gt_dev_alloc) with numpy-typed copies."""
+
+    def __init__(self, ctx: "Context", n: int, dtype):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        p = ctypes.c_void_p()
+        check(lib().gt_dev_alloc(ctx.handle, self.n * self.dtype.itemsize, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, ctx: "Context", a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    @property
+    def nbytes(self) -> int:
+        return self.n * self.dtype.itemsize
+
+    def upload(self, a: np.ndarray, offset: int = 0) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if offset + a.size > self.n:
+            raise ValueError("upload exceeds the device buffer")
+        check(lib().gt_memcpy_h2d(self.ctx.handle, vp(self.ptr + offset * self.dtype.itemsize),
+                                  vp(a.ctypes.data), a.nbytes, None))
+
+    def download(self, n: int | None = None) -> np.ndarray:
+        n = self.n if n is None else int(n)
+        out = np.empty(n, dtype=self.dtype)
+        check(lib().gt_memcpy_d2h(self.ctx.handle, vp(out.ctypes.data), vp(self.ptr),
+                                  out.nbytes, None))
+        return out
+
+    def fill_u32(self, value: int) -> None:
+        assert self.nbytes % 4 == 0
+        check(lib().gt_fill_u32_dev(self.ctx.handle, vp(self.ptr), self.nbytes // 4,
+                                    ctypes.c_uint32(value & 0xFFFFFFFF), None))
+
+    def close(self) -> None:
+        if self.ptr:
+            p, self.ptr = self.ptr, None
+            lib().gt_dev_free(self.ctx.handle, vp(p))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Graph:
@@ -265,11 +332,14 @@ _contexts: dict[int, Context] = {}
 
 
 def context(device: int = 0) -> Context:
-    """Shared per-device context (created on first use)."""
+    """Shared per-device context (created on first use, with the tuned
+    per-k walk parameters of this device when a table exists)."""
     with _lock:
         ctx = _contexts.get(device)
     if ctx is None:
         ctx = Context(device)
+        from . import autotune
+        autotune.apply(ctx)
         with _lock:
             _contexts.setdefault(device, ctx)
             ctx = _contexts[device]

@@ -126,9 +126,18 @@ def apply(ctx: _native.Context, path: Path | None = None) -> bool:
 
 
 if __name__ == "__main__":
-    import sys
+    import argparse
+    ap = argparse.ArgumentParser(description="tune per-k walk parameters on device 0")
+    ap.add_argument("ks", nargs="*", type=int, default=list(range(3, 11)))
+    ap.add_argument("--out", type=Path, help="write the table here instead of tuned/<arch>.json")
+    ap.add_argument("--repeats", type=int, default=3)
+    a = ap.parse_args()
     c = _native.Context(0)
-    ks = [int(x) for x in sys.argv[1:]] or list(range(3, 11))
-    res = tune(c, ks)
+    res = tune(c, a.ks, repeats=a.repeats)
     print(json.dumps(res, indent=1))
-    print("saved", save(res))
+    if a.out:
+        a.out.parent.mkdir(parents=True, exist_ok=True)
+        a.out.write_text(json.dumps(res, indent=1) + "\n")
+        print("saved", a.out)
+    else:
+        print("saved", save(res))

@@ -62,23 +62,19 @@ class HashSet:
 
 def bfs_level(ctx: _native.Context, seen: HashSet, parents: np.ndarray, kp: int, tmax: int):
     """One level on the device: returns (new canonical masks uint64, max_zeros int32)."""
-    import torch
-    dev = torch.device("cuda", ctx.device)
     par = np.ascontiguousarray(parents, dtype=np.uint64)
     n_par = len(par)
     if n_par == 0:
         return np.zeros(0, dtype=np.uint64), np.zeros(0, dtype=np.int32)
     cap = n_par * (49 - kp)
-    d_par = torch.from_numpy(par.view(np.int64)).to(dev)
-    d_out = torch.zeros(cap, dtype=torch.int64, device=dev)
-    d_mz = torch.zeros(cap, dtype=torch.int32, device=dev)
-    torch.cuda.synchronize()
+    d_par = ctx.to_device(par)
+    d_out = ctx.array(cap, np.uint64)
+    d_mz = ctx.array(cap, np.int32)
     n_new = i32()
-    check(lib().gt_bfs_level(ctx.handle, seen.handle, vp(d_par.data_ptr()), n_par, int(kp),
-                             int(tmax), vp(d_out.data_ptr()), vp(d_mz.data_ptr()), cap,
-                             ctypes.byref(n_new)))
+    check(lib().gt_bfs_level(ctx.handle, seen.handle, vp(d_par.ptr), n_par, int(kp), int(tmax),
+                             vp(d_out.ptr), vp(d_mz.ptr), cap, ctypes.byref(n_new)))
     n = n_new.value
-    return d_out[:n].cpu().numpy().view(np.uint64), d_mz[:n].cpu().numpy()
+    return d_out.download(n), d_mz.download(n)
 
 
 def mask_indices(mask: int) -> tuple[int, ...]:

@@ -49,6 +49,8 @@ class DistanceOracle:
         if M is not None and not np.array_equal(np.asarray(M), build_eval_matrix()):
             raise ValueError("M must be the 49x49 GF(8) evaluation matrix of precompute.py")
         self._ctx = _native.Context(self.device_id)
+        from . import autotune
+        autotune.apply(self._ctx)
         self.cu_count = self._ctx.cu_count
 
     @property

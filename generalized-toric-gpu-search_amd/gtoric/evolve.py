@@ -92,19 +92,15 @@ class GenerationGraphs:
     exact distances of pop sets of size k) replayed every generation."""
 
     def __init__(self, ctx: _native.Context, k: int, pop: int, n_prop: int):
-        import torch
-        self.torch = torch
         self.ctx, self.k, self.pop, self.n_prop = ctx, k, pop, n_prop
-        dev = torch.device("cuda", ctx.device)
-        self.d_prop = torch.zeros(n_prop, dtype=torch.int64, device=dev)
-        self.d_lo = torch.zeros(n_prop, dtype=torch.int64, device=dev)
-        self.d_hi = torch.zeros(n_prop, dtype=torch.int64, device=dev)
-        self.d_cm = torch.zeros(n_prop, dtype=torch.int64, device=dev)
-        self.d_st = torch.zeros(n_prop, dtype=torch.int32, device=dev)
-        offs = np.arange(pop + 1, dtype=np.int32) * k
-        self.d_idx = torch.zeros(pop * k, dtype=torch.int32, device=dev)
-        self.d_off = torch.from_numpy(offs).to(dev)
-        self.d_out = torch.zeros(pop, dtype=torch.int32, device=dev)
+        self.d_prop = ctx.to_device(random_masks(np.random.default_rng(0), n_prop, k))
+        self.d_lo = ctx.array(n_prop, np.uint64)
+        self.d_hi = ctx.array(n_prop, np.uint64)
+        self.d_cm = ctx.array(n_prop, np.uint64)
+        self.d_st = ctx.array(n_prop, np.int32)
+        self.d_idx = ctx.to_device(np.tile(np.arange(k, dtype=np.int32), pop))
+        self.d_off = ctx.to_device(np.arange(pop + 1, dtype=np.int32) * k)
+        self.d_out = ctx.array(pop, np.int32)
         # Warm both paths once so every buffer exists before capture.
         self._canon_call()
         self._md_call()
@@ -117,32 +113,31 @@ class GenerationGraphs:
         self.g_md = ctx.capture_end()
 
     def _canon_call(self):
-        self.ctx.canon_masks_dev(self.d_prop.data_ptr(), self.n_prop, self.d_lo.data_ptr(),
-                                 self.d_hi.data_ptr(), self.d_cm.data_ptr(), self.d_st.data_ptr())
+        self.ctx.canon_masks_dev(self.d_prop.ptr, self.n_prop, self.d_lo.ptr, self.d_hi.ptr,
+                                 self.d_cm.ptr, self.d_st.ptr)
 
     def _md_call(self):
-        self.ctx.md_batch_dev(self.d_idx.data_ptr(), self.pop * self.k, self.d_off.data_ptr(),
-                              self.pop, self.d_out.data_ptr(), tmax=48, thresholded=False)
+        self.ctx.md_batch_dev(self.d_idx.ptr, self.pop * self.k, self.d_off.ptr, self.pop,
+                              self.d_out.ptr, tmax=48, thresholded=False)
 
     def canon(self, masks: np.ndarray) -> np.ndarray:
-        t = self.torch
-        self.d_prop.copy_(t.from_numpy(masks.view(np.int64)))
-        t.cuda.synchronize()
+        self.d_prop.upload(masks)
         self.g_canon.launch()
         self.ctx.sync()
-        return self.d_cm.cpu().numpy().view(np.uint64)
+        return self.d_cm.download()
 
     def distances(self, masks: np.ndarray) -> np.ndarray:
-        t = self.torch
-        self.d_idx.copy_(t.from_numpy(masks_to_flat(masks, self.k)))
-        t.cuda.synchronize()
+        self.d_idx.upload(masks_to_flat(masks, self.k))
         self.g_md.launch()
         self.ctx.sync()
-        return 49 - self.d_out.cpu().numpy()
+        return 49 - self.d_out.download()
 
     def close(self):
         self.g_canon.close()
         self.g_md.close()
+        for a in (self.d_prop, self.d_lo, self.d_hi, self.d_cm, self.d_st, self.d_idx, self.d_off,
+                  self.d_out):
+            a.close()
 
 
 def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogate=None,

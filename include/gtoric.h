@@ -189,6 +189,18 @@ int gt_bfs_level(gt_ctx* ctx, gt_hashset* seen, const uint64_t* d_parents, int32
                  int32_t kp, int32_t tmax, uint64_t* d_out_masks, int32_t* d_out_mz,
                  int32_t cap, int32_t* n_new);
 
+/* Device memory of the context's device, for *_dev callers that do not bring
+ * their own allocator.  Copies are ordered on `stream` (NULL = context
+ * stream) and return when the copy has finished.  gt_fill_u32_dev and
+ * gt_copy_dev are kernels (graph-safe; no memset nodes); gt_copy_dev needs a
+ * size that is a multiple of 16 bytes. */
+int gt_dev_alloc(gt_ctx* ctx, int64_t bytes, void** out);
+int gt_dev_free(gt_ctx* ctx, void* d);
+int gt_memcpy_h2d(gt_ctx* ctx, void* d, const void* h, int64_t bytes, void* stream);
+int gt_memcpy_d2h(gt_ctx* ctx, void* h, const void* d, int64_t bytes, void* stream);
+int gt_fill_u32_dev(gt_ctx* ctx, uint32_t* d, int64_t n, uint32_t value, void* stream);
+int gt_copy_dev(gt_ctx* ctx, void* dst, const void* src, int64_t bytes, void* stream);
+
 /* Wait for `stream` (NULL = context stream) and report device-side input
  * errors (bad index, bad size) raised by *_dev calls since the last sync. */
 int gt_sync(gt_ctx* ctx, void* stream);
