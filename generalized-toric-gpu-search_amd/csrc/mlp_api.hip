@@ -122,8 +122,8 @@ int gt_mlp_create(int32_t device, uint64_t seed, gt_mlp** out) {
       !alloc((void**)&m->b.w1b, kMlpHid * kMlpIn * 2) ||
       !alloc((void**)&m->b.w2b, kMlpHid * kMlpHid * 2) || !alloc((void**)&m->b.w3b, kMlpHid * 2))
     return bail(mfail("hipMalloc of MLP parameters failed"));
-  // He-uniform initialisation from a seeded generator; biases 0, output bias
-  // at the middle of the target range (distances / 49 are in [0, 1]).
+  // He-uniform initialisation from a seeded generator; all biases 0 (targets
+  // are standardised by the caller).
   std::vector<float> th(kMlpParams, 0.f);
   std::mt19937_64 rng(seed);
   std::uniform_real_distribution<float> u(-1.f, 1.f);
@@ -132,7 +132,6 @@ int gt_mlp_create(int32_t device, uint64_t seed, gt_mlp** out) {
     for (int i = 0; i < kMlpIn; ++i) th[kOffW1 + o * kMlpIn + i] = (i < 98) ? a1 * u(rng) : 0.f;
   for (int p = 0; p < kMlpHid * kMlpHid; ++p) th[kOffW2 + p] = a2 * u(rng);
   for (int p = 0; p < kMlpHid; ++p) th[kOffW3 + p] = a3 * u(rng);
-  th[kOffB3] = 0.5f;
   std::vector<float> zero(kMlpParams, 0.f);
   if (hipMemcpy(m->b.theta, th.data(), kMlpParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(m->b.m1, zero.data(), kMlpParams * 4, hipMemcpyHostToDevice) != hipSuccess ||

@@ -15,7 +15,8 @@ sample.  One generation:
 5. keep the best ``pop`` of parents and children (distance, then age).
 
 With a surrogate the network is retrained every ``train_every`` generations on
-all (set, exact distance) pairs seen so far; it only orders proposals.
+all (canonical form, exact distance) pairs seen so far; it scores canonical
+forms (distance is an AGL2 class invariant) and only orders proposals.
 """
 
 from __future__ import annotations
@@ -151,15 +152,16 @@ def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogat
     t0 = time.perf_counter()
     t_eval = 0.0
     popm = random_masks(rng, pop, k)
-    seen = set(gg.canon(np.concatenate([popm] * (n_prop // pop))[:n_prop]).tolist()[:pop])
+    popc = gg.canon(np.concatenate([popm] * (n_prop // pop))[:n_prop])[:pop]
+    seen = set(popc.tolist())
     te = time.perf_counter()
     popd = gg.distances(popm)
     t_eval += time.perf_counter() - te
     age = np.zeros(pop, dtype=np.int64)
     evaluated = pop
     if surrogate is not None:
-        surrogate.fit(popm, popd, epochs=train_epochs, batch=1024, seed=0)
-    data_m, data_d = [popm], [popd]
+        surrogate.fit(popc, popd, epochs=train_epochs, batch=1024, seed=0)
+    data_m, data_d = [popc], [popd]
     history = []
     gen = 0
     for gen in range(1, generations + 1):
@@ -180,7 +182,7 @@ def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogat
         keep = np.array(keep, dtype=np.int64)
         # 3. choose pop proposals (surrogate order or proposal order)
         if surrogate is not None and len(keep) > pop:
-            order = surrogate.rank(props[keep])
+            order = surrogate.rank(cm[keep])
             keep = keep[order[:pop]]
         else:
             keep = keep[:pop]
@@ -189,13 +191,14 @@ def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogat
         if len(chosen) < pop:
             extra = random_masks(rng, pop - len(chosen), k)
             chosen = np.concatenate([chosen, extra])
+            ccm = np.concatenate([ccm, ctx.canon_masks(extra)[2]])
         seen.update(ccm.tolist())
         # 4. exact distances
         te = time.perf_counter()
         cd = gg.distances(chosen)
         t_eval += time.perf_counter() - te
         evaluated += pop
-        data_m.append(chosen)
+        data_m.append(ccm)
         data_d.append(cd)
         # 5. survivors: best distance, then youngest
         allm = np.concatenate([popm, chosen])

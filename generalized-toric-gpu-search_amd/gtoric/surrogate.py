@@ -1,7 +1,9 @@
 """Surrogate MLP that ranks exponent-set proposals before exact evaluation.
 
 The network (csrc/mlp_kernels.hip) maps a set S to a score trained to predict
-its exact minimum distance / 49.  It only orders proposals: every distance the
+its standardised exact minimum distance.  Callers pass canonical masks (the
+AGL2 canonical image of S): distance is a class invariant, and on raw masks
+the position features do not generalise.  It only orders proposals: every distance the
 search reports comes from the exact kernel.  With no trained network the
 search falls back to unranked proposals (:meth:`Surrogate.rank` of an
 untrained model is never used for that; callers pass ``surrogate=None``).
@@ -24,6 +26,7 @@ class Surrogate:
         check(lib().gt_mlp_create(int(device), ctypes.c_uint64(int(seed)), ctypes.byref(h)))
         self.handle = h
         self.steps = 0
+        self.mean, self.std = 0.0, 1.0
 
     # ---- parameters --------------------------------------------------------
     def params(self) -> np.ndarray:
@@ -63,9 +66,15 @@ class Surrogate:
 
     def fit(self, masks, distances, epochs: int = 20, batch: int = 1024, lr: float = 2e-3,
             seed: int = 0) -> list[float]:
-        """Minibatch Adam on (mask, exact distance) pairs; target = d / 49."""
+        """Minibatch Adam on (mask, exact distance) pairs.  Targets are the
+        distances standardised over the training set (the spread of d among
+        sets of one size is a few units, so raw d / 49 gives gradients too
+        small to move the hidden layers); the score keeps the order of d."""
         masks = np.asarray(masks, dtype=np.uint64)
-        target = np.asarray(distances, dtype=np.float32) / np.float32(49.0)
+        d = np.asarray(distances, dtype=np.float64)
+        self.mean = float(d.mean()) if len(d) else 0.0
+        self.std = float(d.std()) if len(d) and d.std() > 0 else 1.0
+        target = ((d - self.mean) / self.std).astype(np.float32)
         rng = np.random.default_rng(seed)
         losses = []
         for _ in range(epochs):
@@ -83,6 +92,10 @@ class Surrogate:
         if len(m):
             check(lib().gt_mlp_predict(self.handle, ptr(m), len(m), ptr(out)))
         return out
+
+    def predict_distance(self, masks) -> np.ndarray:
+        """Predicted distance in the units of the last fit (for reporting only)."""
+        return self.predict(masks) * self.std + self.mean
 
     def rank(self, masks) -> np.ndarray:
         """Indices of `masks` from most to least promising (stable)."""

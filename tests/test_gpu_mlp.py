@@ -73,8 +73,9 @@ def test_gradients_match_cpu_reference():
 
 
 def test_trained_network_ranks_better_than_unranked():
-    """Trained on exact distances of k=8 sets, the network's top decile of a
-    held-out pool holds more high-distance codes than a random decile."""
+    """Trained on exact distances of k=8 sets (canonical masks as input), the
+    network's top decile of a held-out pool holds more high-distance codes
+    than a random decile."""
     from gtoric import _native
     from gtoric.distance import flatten
     from gtoric.surrogate import Surrogate
@@ -83,6 +84,7 @@ def test_trained_network_ranks_better_than_unranked():
     pool = [sorted(rng.choice(49, 8, replace=False).tolist()) for _ in range(12_000)]
     d = 49 - ctx.md_batch(*flatten(pool), 48)
     masks = np.array([sum(1 << p for p in S) for S in pool], dtype=np.uint64)
+    masks = ctx.canon_masks(masks)[2]
     tr, te = slice(0, 8000), slice(8000, None)
     s = Surrogate(0, seed=3)
     losses = s.fit(masks[tr], d[tr], epochs=25, batch=512, lr=2e-3)
