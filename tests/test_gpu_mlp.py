@@ -74,8 +74,10 @@ def test_gradients_match_cpu_reference():
 
 def test_trained_network_ranks_better_than_unranked():
     """Trained on exact distances of k=8 sets (canonical masks as input), the
-    network's top decile of a held-out pool holds more high-distance codes
-    than a random decile."""
+    network's top decile of a held-out pool holds at most half as many
+    below-median codes as a random decile does.  (Distances of random k=8
+    sets are concentrated: most sit at the top value, so "top decile by
+    distance" is not a usable target; "below the median" is.)"""
     from gtoric import _native
     from gtoric.distance import flatten
     from gtoric.surrogate import Surrogate
@@ -92,9 +94,11 @@ def test_trained_network_ranks_better_than_unranked():
     score = s.predict(masks[te])
     dt = d[te]
     top = np.argsort(-score, kind="stable")[: len(dt) // 10]
-    good = dt >= np.quantile(dt, 0.9)
-    frac_ranked = good[top].mean()
-    frac_random = good.mean()
+    bad = dt < np.median(dt)
+    bad_ranked = bad[top].mean()
+    bad_random = bad.mean()
     rho = np.corrcoef(np.argsort(np.argsort(score)), np.argsort(np.argsort(dt)))[0, 1]
+    print(f"rho={rho:.3f} below-median: ranked decile {bad_ranked:.3f}, random {bad_random:.3f}")
+    assert bad_random > 0.05, bad_random
     assert rho > 0.2, rho
-    assert frac_ranked > 1.5 * frac_random, (frac_ranked, frac_random)
+    assert bad_ranked <= 0.5 * bad_random, (bad_ranked, bad_random)
