@@ -169,7 +169,7 @@ struct gt_ctx {
   bool params_dirty = true;
 
   DevBuf mplanes, params_buf, perm, ctrl;
-  DevBuf idx, offs, tmax, rows, cum, best, out, zmax, stopf;  // min distance
+  DevBuf idx, offs, tmax, rows, plan, best, out, zmax, stopf;  // min distance
   DevBuf sbuf;                                                // single code
   DevBuf masks, order, hist, klo, khi, canon, stab;            // canonical
   DevBuf children, bflag, bsum, bidx, bcnt, bmasks, brows, boffs;  // BFS level
@@ -180,7 +180,7 @@ struct gt_ctx {
   hipStream_t capture_stream = nullptr;
 
   std::vector<DevBuf*> all_bufs() {
-    return {&mplanes, &params_buf, &perm, &ctrl, &idx, &offs, &tmax, &rows, &cum, &best,
+    return {&mplanes, &params_buf, &perm, &ctrl, &idx, &offs, &tmax, &rows, &plan, &best,
             &out, &zmax, &stopf, &sbuf, &masks, &order, &hist, &klo, &khi, &canon, &stab,
             &children, &bflag, &bsum, &bidx, &bcnt, &bmasks, &brows, &boffs};
   }
@@ -250,15 +250,20 @@ struct gt_ctx {
       GT_OK(prof_end("md_sampled", s, e0));
       return 0;
     }
-    GT_OK(cum.ensure((size_t)(n + 1) * 4));
+    // plan buffer: claim counters (kClaimShards x 4 KB), MdPlan, order (n entries).
+    constexpr size_t kClaimBytes = (size_t)kClaimShards * kClaimStride * 4;
+    GT_OK(plan.ensure(kClaimBytes + sizeof(MdPlan) + (size_t)n * 4));
     GT_OK(best.ensure((size_t)n * 4));
-    GT_HIP(launch_md_plan(d_offsets, n, params_buf.as<MdParams>(), cum.as<uint32_t>(),
+    uint32_t* d_claims = plan.as<uint32_t>();
+    MdPlan* d_plan = reinterpret_cast<MdPlan*>(plan.as<char>() + kClaimBytes);
+    uint32_t* d_order = reinterpret_cast<uint32_t*>(plan.as<char>() + kClaimBytes + sizeof(MdPlan));
+    GT_HIP(launch_md_plan(d_offsets, n, params_buf.as<MdParams>(), d_plan, d_order, d_claims,
                           best.as<uint32_t>(), ctrl.as<uint32_t>(), s));
     hipEvent_t e0 = nullptr;
     GT_OK(prof_begin("md_main", s, &e0));
     GT_HIP(launch_md_main(thresholded, md_grid[thresholded ? 1 : 0], rows.as<uint32_t>(),
-                          d_offsets, cum.as<uint32_t>(), n, best.as<uint32_t>(), d_tmax, tmax_s,
-                          ctrl.as<uint32_t>(), params_buf.as<MdParams>(), s));
+                          d_offsets, d_plan, d_order, d_claims, best.as<uint32_t>(), d_tmax,
+                          tmax_s, params_buf.as<MdParams>(), s));
     GT_OK(prof_end("md_main", s, e0));
     GT_HIP(launch_md_finalize(d_offsets, n, best.as<uint32_t>(), d_tmax, tmax_s, d_out, s));
     return 0;

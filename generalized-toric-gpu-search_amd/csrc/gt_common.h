@@ -47,6 +47,26 @@ struct MdParams {
   uint8_t check_log2[kMaxKExact + 1];  // early-exit check every 2^x blocks
 };
 
+// Chunk plan of one batch, largest codes first.  Bucket j (0 <= j < kMaxKExact)
+// holds the codes with k = kMaxKExact - j, in batch order, at positions
+// pb[j] .. of the order array; its codes own chunk ids cb[j] + q * cc[j] ..
+// + cc[j] - 1 for the q-th code.  cb[kMaxKExact] is the total chunk count (0
+// when the plan overflowed 2^31 chunks).  Claiming ids in increasing order
+// hands out the biggest chunks first, so the last chunks of a launch are the
+// small ones of the small codes.
+struct MdPlan {
+  uint32_t cb[kMaxKExact + 1];
+  uint32_t pb[kMaxKExact];
+  uint32_t cc[kMaxKExact];
+};
+
+// Chunk claims are spread over kClaimShards counters, kClaimStride words
+// (4 KB) apart: one device-scope atomic on one address completes about every
+// 11 ns, which small chunks would otherwise queue on.  Shard s hands out the
+// chunk ids s, s + kClaimShards, s + 2 kClaimShards, ... in increasing order.
+constexpr int kClaimShards = 16;
+constexpr int kClaimStride = 1024;
+
 // Static unroll of the inner Gray walk: 2^kU steps per block with rows
 // 0..kU-1 held in scalar registers.
 constexpr int kU = 4;

@@ -17,12 +17,15 @@ constexpr int kCtrlWords = 16;
 
 hipError_t launch_md_rows(const int32_t* d_idx, int64_t n_idx, const uint32_t* d_mplanes,
                           uint32_t* d_rows, uint32_t* d_ctrl, hipStream_t s);
+// d_claims: kClaimShards counters, kClaimStride words apart (reset by the plan).
 hipError_t launch_md_plan(const int32_t* d_offsets, int32_t n, const MdParams* d_params,
-                          uint32_t* d_cum, uint32_t* d_best, uint32_t* d_ctrl, hipStream_t s);
+                          MdPlan* d_plan, uint32_t* d_order, uint32_t* d_claims,
+                          uint32_t* d_best, uint32_t* d_ctrl, hipStream_t s);
 hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
-                          const int32_t* d_offsets, const uint32_t* d_cum, int32_t n,
-                          uint32_t* d_best, const int32_t* d_tmax, int32_t tmax,
-                          uint32_t* d_ctrl, const MdParams* d_params, hipStream_t s);
+                          const int32_t* d_offsets, const MdPlan* d_plan,
+                          const uint32_t* d_order, uint32_t* d_claims, uint32_t* d_best,
+                          const int32_t* d_tmax, int32_t tmax, const MdParams* d_params,
+                          hipStream_t s);
 hipError_t launch_md_finalize(const int32_t* d_offsets, int32_t n, const uint32_t* d_best,
                               const int32_t* d_tmax, int32_t tmax, int32_t* d_out,
                               hipStream_t s);

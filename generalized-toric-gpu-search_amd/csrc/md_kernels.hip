@@ -6,10 +6,11 @@
 // kernel_cuda_bp.py:99-127).  Work decomposition (see gt_common.h):
 //
 //   md_rows     one thread per generator: the three bit rows x^b * g_i.
-//   md_plan     one workgroup: chunk count per code, exclusive scan -> cum[],
-//               reset of the per-code minimum and of the chunk counter.
-//   md_main     persistent: each wave claims a chunk id with one atomic,
-//               finds its code by binary search in cum[] (scalar loads),
+//   md_plan     one workgroup: codes bucketed by k, largest k first (MdPlan +
+//               order[]), reset of the per-code minimum and of the chunk counter.
+//   md_main     persistent: each wave claims a chunk id with one atomic on
+//               one of 16 claim counters, finds its bucket by binary search
+//               over 21 entries and its code by one division (scalar work),
 //               walks 64 lanes x 2^L Gray steps, and folds its wave minimum
 //               into the code's minimum with one atomicMin.
 //   md_finalize minimum weight -> max_zeros, clamped at tmax + 1.
@@ -51,101 +52,210 @@ __global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict_
   }
 }
 
+// Largest-first plan (MdPlan).  One workgroup of 16 waves takes the codes in
+// rounds of 1024; per round every wave counts its codes per bucket with one
+// ballot per bucket.  Pass 1 sizes the buckets, pass 2 writes each code's
+// position: bucket base + codes of that bucket in earlier rounds + in earlier
+// waves of this round + lanes below it.  The order inside a bucket is batch
+// order, so the plan is deterministic.
 __global__ __launch_bounds__(1024) void md_plan_kernel(const int32_t* __restrict__ offsets,
                                                        int32_t n,
                                                        const MdParams* __restrict__ p,
-                                                       uint32_t* __restrict__ cum,
+                                                       MdPlan* __restrict__ plan,
+                                                       uint32_t* __restrict__ order,
+                                                       uint32_t* __restrict__ claims,
                                                        uint32_t* __restrict__ best,
                                                        uint32_t* ctrl) {
-  __shared__ uint64_t s[1024];
-  const int tid = threadIdx.x;
-  const int per = (n + 1023) / 1024;
-  const int b = min(n, tid * per), e = min(n, b + per);
-  uint64_t sum = 0;
+  constexpr int kB = kMaxKExact;
+  constexpr int kWaves = 1024 / kWave;
+  __shared__ uint32_t wc[kWaves][kB];
+  __shared__ uint32_t run[kB];
+  const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+  const uint64_t below = (1ull << lane) - 1ull;
   uint32_t err = 0;
-  for (int i = b; i < e; ++i) {
-    const int k = offsets[i + 1] - offsets[i];
-    if (k < 0 || k > kMaxKExact) {
-      err |= kErrBadK;
-    } else if (k > 0) {
-      sum += code_chunks(k, p->L[k]);
+  if (tid < kClaimShards) claims[tid * kClaimStride] = 0u;
+
+  // Pass 1: lane b of each wave accumulates the size of bucket b.
+  uint32_t mine = 0;
+  for (int base = 0; base < n; base += 1024) {
+    const int i = base + tid;
+    int j = -1;
+    if (i < n) {
+      const int k = offsets[i + 1] - offsets[i];
+      if (k < 0 || k > kMaxKExact) err |= kErrBadK;
+      else if (k > 0) j = kMaxKExact - k;
+      best[i] = 64u;
     }
-    best[i] = 64u;
+    for (int b = 0; b < kB; ++b) {
+      const uint32_t c = (uint32_t)__popcll(__ballot(j == b));
+      if (lane == b) mine += c;
+    }
   }
-  s[tid] = sum;
+  if (lane < kB) wc[wave][lane] = mine;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const uint64_t t = (tid >= off) ? s[tid - off] : 0;
-    __syncthreads();
-    s[tid] += t;
-    __syncthreads();
+  if (tid < kB) {
+    uint32_t t = 0;
+    for (int w = 0; w < kWaves; ++w) t += wc[w][tid];
+    run[tid] = t;
   }
-  const uint64_t total = s[1023];
-  const bool overflow = total >= (1ull << 31);
-  uint64_t run = s[tid] - sum;
-  for (int i = b; i < e; ++i) {
-    cum[i] = overflow ? 0u : (uint32_t)run;
-    const int k = offsets[i + 1] - offsets[i];
-    if (k > 0 && k <= kMaxKExact) run += code_chunks(k, p->L[k]);
-  }
-  if (err) atomicOr(&ctrl[2], err);
-  if (tid == 1023) {
-    cum[n] = overflow ? 0u : (uint32_t)total;
+  __syncthreads();
+  if (tid == 0) {
+    uint64_t chunks = 0;
+    uint32_t pos = 0;
+    for (int b = 0; b < kB; ++b) {
+      const uint32_t t = run[b];
+      const int k = kMaxKExact - b;
+      const uint64_t cc = code_chunks(k, p->L[k]);
+      plan->cb[b] = (uint32_t)chunks;
+      plan->pb[b] = pos;
+      plan->cc[b] = (uint32_t)cc;
+      run[b] = pos;
+      if (t > 0 && cc >= (1ull << 31)) chunks = 1ull << 31;
+      chunks += (uint64_t)t * cc;
+      if (chunks > (1ull << 31)) chunks = 1ull << 31;
+      pos += t;
+    }
+    const bool overflow = chunks >= (1ull << 31);
+    plan->cb[kB] = overflow ? 0u : (uint32_t)chunks;
     ctrl[0] = 0u;
-    ctrl[3] = overflow ? 0xFFFFFFFFu : (uint32_t)total;
+    ctrl[3] = overflow ? 0xFFFFFFFFu : (uint32_t)chunks;
     if (overflow) atomicOr(&ctrl[2], kErrBadK);
   }
+  __syncthreads();
+
+  // Pass 2: positions.
+  for (int base = 0; base < n; base += 1024) {
+    const int i = base + tid;
+    int j = -1;
+    if (i < n) {
+      const int k = offsets[i + 1] - offsets[i];
+      if (k > 0 && k <= kMaxKExact) j = kMaxKExact - k;
+    }
+    uint32_t rank = 0;
+    for (int b = 0; b < kB; ++b) {
+      const uint64_t m = __ballot(j == b);
+      if (j == b) rank = (uint32_t)__popcll(m & below);
+      if (lane == b) wc[wave][b] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (j >= 0) {
+      uint32_t off = run[j] + rank;
+      for (int w = 0; w < wave; ++w) off += wc[w][j];
+      order[off] = (uint32_t)i;
+    }
+    __syncthreads();
+    if (tid < kB) {
+      uint32_t t = 0;
+      for (int w = 0; w < kWaves; ++w) t += wc[w][tid];
+      run[tid] += t;
+    }
+    __syncthreads();
+  }
+  if (err) atomicOr(&ctrl[2], err);
 }
 
+// Claim loop.  A workgroup's home shard is blockIdx.x % kClaimShards
+// (workgroups go round-robin over the XCDs, so a shard's counter is mostly
+// hit from one XCD).  When the home shard runs dry the wave reads all shard
+// counters at once (lane t reads shard t), takes the next shard in ring order
+// that still has ids, and leaves once none has.
+//
+// Every iteration has exactly one lane-0 region, at the bottom: it folds the
+// finished chunk into its code and claims the next id; there is no
+// `continue`, and every value that steers the loop is made wave-uniform with
+// readfirstlane.  (With a separate lane-0 claim at the loop top the compiler
+// threaded the two regions together and the other 63 lanes re-read a stale
+// id with lane 0 inactive, walking chunk 0 forever.)
 template <bool THR>
 __global__ __launch_bounds__(256) void md_main_kernel(
     const uint32_t* __restrict__ rows, const int32_t* __restrict__ offsets,
-    const uint32_t* __restrict__ cum, int32_t n, uint32_t* best_w,
-    const int32_t* __restrict__ tmax_arr, int32_t tmax_s, uint32_t* ctrl,
+    const MdPlan* __restrict__ plan, const uint32_t* __restrict__ order,
+    uint32_t* claims, uint32_t* best_w, const int32_t* __restrict__ tmax_arr, int32_t tmax_s,
     const MdParams* __restrict__ p) {
   const uint32_t lane = lane_id();
-  const uint32_t total = cum[n];
-  // Chunk claims: lane 0 takes the next id and the wave reads it with
-  // readfirstlane.  Each iteration has exactly one lane-0 region — it folds
-  // the finished chunk into its code and claims the next id — and no
-  // `continue`, so the loop stays wave-uniform: when a separate lane-0 claim
-  // at the loop top followed a lane-0 atomic at the loop bottom, the compiler
-  // threaded the two regions together and the other 63 lanes re-read a stale
-  // id with lane 0 inactive, walking chunk 0 forever.
+  const uint32_t total = plan->cb[kMaxKExact];
+  uint32_t shard = blockIdx.x % kClaimShards;
+  uint32_t dry = 0;                       // shards seen without ids (bit mask)
   uint32_t first = 0;
-  if (lane == 0) first = atomicAdd(&ctrl[0], 1u);
-  uint32_t g = __builtin_amdgcn_readfirstlane(first);
-  while (g < total) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (cum[mid] <= g) lo = mid; else hi = mid - 1;
-    }
-    const int code = lo;
-    const int k = offsets[code + 1] - offsets[code];
-    const int L = p->L[k];
-    const uint32_t cmask = (1u << p->check_log2[k]) - 1u;
-    const uint32_t* R = rows + (size_t)offsets[code] * 3 * kRowWords;
-    int thr = -1;
-    bool skip = false;
-    if (THR) {
-      thr = 48 - (tmax_arr ? tmax_arr[code] : tmax_s);
-      const uint32_t bw = __builtin_amdgcn_readfirstlane(
-          __hip_atomic_load(&best_w[code], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      skip = (int)bw <= thr;
-    }
+  if (lane == 0) first = atomicAdd(&claims[shard * kClaimStride], 1u);
+  uint32_t g = __builtin_amdgcn_readfirstlane(first) * kClaimShards + shard;
+  // Exact mode keeps the running minimum of the code it is walking and folds
+  // it (one atomicMin) only when it moves to another code or leaves: a code
+  // with thousands of chunks would otherwise serialise thousands of atomics on
+  // one address.  Thresholded mode folds every chunk at once, because other
+  // waves stop early on the folded value.
+  int pcode = -1;
+  uint32_t pmin = 64u;
+  for (;;) {
+    const bool have = g < total;
+    int code = 0;
+    bool fold = false;
     uint32_t b = 64u;
-    if (!skip) {
-      b = run_chunk<THR>(ConstRows{R}, k, L, g - cum[code], lane, thr, cmask, &best_w[code]);
-      b = wave_min(b);
+    if (have) {
+      // Last bucket whose first chunk id is <= g (empty buckets tie with their
+      // successor, so the last tie is the nonempty one).
+      int lo = 0, hi = kMaxKExact - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (plan->cb[mid] <= g) lo = mid; else hi = mid - 1;
+      }
+      const uint32_t rel = g - plan->cb[lo];
+      const uint32_t cc = plan->cc[lo];
+      const uint32_t q = rel / cc;
+      code = (int)order[plan->pb[lo] + q];
+      const int k = kMaxKExact - lo;
+      const int L = p->L[k];
+      const uint32_t cmask = (1u << p->check_log2[k]) - 1u;
+      const uint32_t* R = rows + (size_t)offsets[code] * 3 * kRowWords;
+      int thr = -1;
+      bool skip = false;
+      if (THR) {
+        thr = 48 - (tmax_arr ? tmax_arr[code] : tmax_s);
+        const uint32_t bw = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(&best_w[code], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        skip = (int)bw <= thr;
+      }
+      if (!skip) {
+        b = run_chunk<THR>(ConstRows{R}, k, L, rel - q * cc, lane, thr, cmask, &best_w[code]);
+        b = wave_min(b);
+        fold = THR;
+      }
+      if (!THR) {
+        if (code != pcode) {
+          fold = pcode >= 0;
+          const int c0 = pcode;
+          const uint32_t m0 = pmin;
+          pcode = code;
+          pmin = b;
+          code = c0;
+          b = m0;
+        } else {
+          pmin = min(pmin, b);
+        }
+      }
+    } else {
+      dry |= 1u << shard;
+      uint32_t c = 0xFFFFFFFFu;
+      if (lane < (uint32_t)kClaimShards)
+        c = __hip_atomic_load(&claims[lane * kClaimStride], __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+      const bool open = lane < (uint32_t)kClaimShards && !((dry >> lane) & 1u) &&
+                        (uint64_t)c * kClaimShards + lane < total;
+      const uint32_t m = (uint32_t)__ballot(open);
+      if (m == 0u) break;
+      // Next open shard after `shard` in ring order.
+      const uint32_t r = ((m >> (shard + 1)) | (m << (kClaimShards - 1 - shard))) &
+                         ((1u << kClaimShards) - 1u);
+      shard = (shard + 1u + (uint32_t)__builtin_ctz(r)) % kClaimShards;
     }
     uint32_t next = 0;
     if (lane == 0) {
-      if (!skip) atomicMin(&best_w[code], b);
-      next = atomicAdd(&ctrl[0], 1u);
+      if (fold) atomicMin(&best_w[code], b);
+      next = atomicAdd(&claims[shard * kClaimStride], 1u);
     }
-    g = __builtin_amdgcn_readfirstlane(next);
+    g = __builtin_amdgcn_readfirstlane(next) * kClaimShards + shard;
   }
+  if (!THR && pcode >= 0 && lane == 0) atomicMin(&best_w[pcode], pmin);
 }
 
 __global__ __launch_bounds__(256) void md_finalize_kernel(const int32_t* __restrict__ offsets,
@@ -301,9 +411,10 @@ hipError_t launch_md_rows(const int32_t* d_idx, int64_t n_idx, const uint32_t* d
 }
 
 hipError_t launch_md_plan(const int32_t* d_offsets, int32_t n, const MdParams* d_params,
-                          uint32_t* d_cum, uint32_t* d_best, uint32_t* d_ctrl, hipStream_t s) {
-  hipLaunchKernelGGL(md_plan_kernel, dim3(1), dim3(1024), 0, s, d_offsets, n, d_params, d_cum,
-                     d_best, d_ctrl);
+                          MdPlan* d_plan, uint32_t* d_order, uint32_t* d_claims,
+                          uint32_t* d_best, uint32_t* d_ctrl, hipStream_t s) {
+  hipLaunchKernelGGL(md_plan_kernel, dim3(1), dim3(1024), 0, s, d_offsets, n, d_params, d_plan,
+                     d_order, d_claims, d_best, d_ctrl);
   return hipGetLastError();
 }
 
@@ -317,15 +428,16 @@ int md_main_blocks_per_cu(bool thresholded) {
 }
 
 hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
-                          const int32_t* d_offsets, const uint32_t* d_cum, int32_t n,
-                          uint32_t* d_best, const int32_t* d_tmax, int32_t tmax,
-                          uint32_t* d_ctrl, const MdParams* d_params, hipStream_t s) {
+                          const int32_t* d_offsets, const MdPlan* d_plan,
+                          const uint32_t* d_order, uint32_t* d_claims, uint32_t* d_best,
+                          const int32_t* d_tmax, int32_t tmax, const MdParams* d_params,
+                          hipStream_t s) {
   if (thresholded)
     hipLaunchKernelGGL(md_main_kernel<true>, dim3(grid), dim3(256), 0, s, d_rows, d_offsets,
-                       d_cum, n, d_best, d_tmax, tmax, d_ctrl, d_params);
+                       d_plan, d_order, d_claims, d_best, d_tmax, tmax, d_params);
   else
     hipLaunchKernelGGL(md_main_kernel<false>, dim3(grid), dim3(256), 0, s, d_rows, d_offsets,
-                       d_cum, n, d_best, d_tmax, tmax, d_ctrl, d_params);
+                       d_plan, d_order, d_claims, d_best, d_tmax, tmax, d_params);
   return hipGetLastError();
 }
 

@@ -89,6 +89,28 @@ def test_random_mixed_vs_oracle(dist):
     assert dist.max_zeros_batch(sets, td) == exp
 
 
+def test_batches_smaller_than_the_claim_shards(dist):
+    """1, 2, 5 and 17 codes: fewer chunks than claim counters, so most shards
+    start dry and waves must find the few open ones (or leave at once)."""
+    rng = np.random.default_rng(103)
+    for n in (1, 2, 5, 17):
+        sets = [sorted(rng.choice(49, int(k), replace=False).tolist())
+                for k in rng.integers(1, 6, size=n)]
+        assert dist.max_zeros_batch(sets, 1) == oracle.max_zeros_batch(sets, 48).tolist(), n
+
+
+def test_one_large_code_among_many_small(dist):
+    """Bucketing by k puts the k=8 code's chunks first and the 2000 tiny codes
+    after them, interleaved over the claim shards; empty sets get no chunks."""
+    rng = np.random.default_rng(104)
+    sets = [sorted(rng.choice(49, int(k), replace=False).tolist())
+            for k in rng.integers(1, 5, size=2000)]
+    big = sorted(rng.choice(49, 8, replace=False).tolist())
+    sets = sets[:700] + [big] + sets[700:] + [[]] * 3
+    exp = oracle.max_zeros_batch(sets, 48).tolist()
+    assert dist.max_zeros_batch(sets, 1) == exp
+
+
 def test_single_launch_matches_batch(dist):
     rng = np.random.default_rng(102)
     for k in range(1, 9):
