@@ -31,6 +31,22 @@ OPS_PER_CODEWORD = 11       # 6 XOR + 2 OR3 + 2 BCNT + 1 MIN per projective code
 LANES_PER_CU_CLK = 64       # 4 SIMD x 16 lanes: one int32 VALU op per lane per clock
 
 
+PROFILE = ROOT / "profiles" / "md_main_mixed4096.md"
+
+
+def profiled_traffic(path: Path = PROFILE):
+    """HBM bytes per md_main launch from the committed rocprofv3 summary (a
+    `traffic_bytes_per_launch: N` line: FETCH_SIZE and WRITE_SIZE of
+    md_main, each scaled by the copy-kernel calibration), or None."""
+    try:
+        for line in path.read_text().splitlines():
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1].strip())
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -289,7 +305,8 @@ def main() -> None:
             "peak": round(peak, 3),
             "unit": "TOP/s",
             "frac": round(achieved / peak, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(),
+            "traffic_source": str(PROFILE.relative_to(ROOT)) if profiled_traffic() else None,
             "ops_per_codeword": OPS_PER_CODEWORD,
             "kernel_ms": round(kern_ms, 4),
             "kernel_launches": nk,

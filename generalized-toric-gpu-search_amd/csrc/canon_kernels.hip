@@ -8,14 +8,15 @@
 // rel(A s_t, A s_i) = A (s_i - s_t) packed 6 bits per position.  For sets of
 // one size the packed order equals the order of the 49-bit image masks
 // (both compare the largest element first), so the kernel minimises masks:
-//   image(A, t) = translate(mask(A S), -(A s_t)).
+//   image(A, t) = translate(mask(A S), -(A s_t)),
+// held in a byte-row layout (below) that makes the translation cheap.
 // Only images that contain point 0 are candidates (the anchored images); the
 // minimum over all 98,784 affine images is a different function.  The number
 // of (A, t) pairs attaining the minimum is |Stab_AGL2(S)|.
 //
 // Layout: one lane per set, A uniform across the wave (2016 iterations), the
 // GL2 permutation table staged in LDS in tiles.  Sets are first bucketed by
-// size so that every wave runs a loop specialised for its k.
+// size, largest first, so that every wave runs a loop specialised for its k.
 #include "gt_launch.h"
 
 namespace gt {
@@ -23,31 +24,59 @@ namespace gt {
 constexpr int kPermStride = 52;        // 49 bytes per matrix, padded to 13 dwords
 constexpr int kPermTile = 504;         // matrices per LDS tile (4 tiles)
 constexpr int kCanonBlock = 256;
-constexpr uint64_t kMask49 = (1ull << 49) - 1ull;
 
 int canon_perm_bytes() { return 2016 * kPermStride; }
 
-// Bits whose position inside their 7-bit row is < 7 - v, plus every bit above
-// 49 (so that v_bfi keeps those bits from the right-shifted operand, which has
-// them clear).
-__device__ __forceinline__ uint64_t keep_low(int v) {
-  const uint64_t rowbits = (0x7Full >> v);
-  uint64_t m = 0;
-#pragma unroll
-  for (int g = 0; g < 7; ++g) m |= rowbits << (7 * g);
-  return m | ~kMask49;
+// Byte-row layout.  Inside the kernel a subset of the 7x7 torus is a 56-bit
+// value with row u in byte u and column v in bit v of that byte (position
+// 8u + v; bit 7 of every byte and byte 7 stay 0).  Integer order on this
+// layout equals integer order on the 49-bit mask (position 7u + v), because
+// both orders compare row 6 first and, inside a row, column 6 first; the
+// permutation table holds image positions in this layout.
+//
+// Translating by (-u, -v) is then a byte rotation of the 7 rows (two
+// v_perm_b32, whose selectors are the row numbers (r + u) mod 7 and 12 for a
+// zero byte) and a rotation of every 7-bit row by v: v_bfi(lo, x >> v,
+// x << (7 - v)) with lo = bits 0..6-v of every row, then an AND that clears
+// bit 7 of every row and byte 7, where the left shift leaves stray bits.
+// One 16-byte entry per anchor point (one ds_read_b128 per image).
+struct Anchor {
+  uint32_t sel_lo, sel_hi;   // v_perm selectors for rows 0..3 and 4..6
+  uint32_t vv;               // v | (7 - v) << 8: the 64-bit shifts read bits 0..5
+  uint32_t lo;               // bits 0..6-v of every byte
+};
+
+constexpr uint32_t kRowBitsLo = 0x7F7F7F7Fu;   // rows 0..3
+constexpr uint32_t kRowBitsHi = 0x007F7F7Fu;   // rows 4..6
+
+__device__ __forceinline__ uint64_t translate(uint64_t M, const Anchor& e) {
+  const uint32_t m0 = (uint32_t)M, m1 = (uint32_t)(M >> 32);
+  const uint64_t X = ((uint64_t)__builtin_amdgcn_perm(m1, m0, e.sel_hi) << 32) |
+                     __builtin_amdgcn_perm(m1, m0, e.sel_lo);
+  const uint64_t a = X >> (e.vv & 63u), b = X << ((e.vv >> 8) & 63u);
+  // v_bitop3 truth table 0xE4: (a & lo) | (b & ~lo).
+  const uint32_t r0 = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, e.lo, 0xE4) &
+                      kRowBitsLo;
+  const uint32_t r1 = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(b >> 32), e.lo,
+                                                  0xE4) & kRowBitsHi;
+  return ((uint64_t)r1 << 32) | r0;
 }
 
-// Translate a 49-bit mask on the 7x7 torus by (-u0, -v0).
-__device__ __forceinline__ uint64_t translate(uint64_t M, uint32_t s7, uint32_t v0, uint64_t K) {
-  const uint64_t R = ((M >> s7) | (M << (49u - s7))) & kMask49;
-  const uint64_t a = R >> v0, b = R << (7u - v0);
-  return (a & K) | (b & ~K);
+__host__ __device__ __forceinline__ Anchor make_anchor(uint32_t u, uint32_t v) {
+  Anchor e{};
+  uint32_t sel[8];
+  for (uint32_t r = 0; r < 7; ++r) sel[r] = (r + u) % 7u;
+  sel[7] = 12u;
+  e.sel_lo = sel[0] | (sel[1] << 8) | (sel[2] << 16) | (sel[3] << 24);
+  e.sel_hi = sel[4] | (sel[5] << 8) | (sel[6] << 16) | (sel[7] << 24);
+  e.vv = v | ((7u - v) << 8);
+  e.lo = (0x7Fu >> v) * 0x01010101u;
+  return e;
 }
 
 template <int K>
 __device__ __forceinline__ void canon_tile(const uint8_t* __restrict__ lperm, int na,
-                                           const uint32_t* pts, const uint64_t* lkeep,
+                                           const uint32_t* pts, const Anchor* lanc,
                                            uint64_t& best, uint32_t& cnt) {
   for (int a = 0; a < na; ++a) {
     const uint8_t* pa = lperm + a * kPermStride;
@@ -60,15 +89,28 @@ __device__ __forceinline__ void canon_tile(const uint8_t* __restrict__ lperm, in
     }
 #pragma unroll
     for (int t = 0; t < K; ++t) {
-      const uint32_t u0 = (q[t] * 37u) >> 8;      // q / 7 for q <= 48
-      const uint32_t v0 = q[t] - 7u * u0;
-      const uint64_t img = translate(MA, 7u * u0, v0, lkeep[v0]);
-      if (img < best) { best = img; cnt = 1u; }
-      else if (img == best) { ++cnt; }
+      const uint64_t img = translate(MA, lanc[q[t]]);
+      // A lane sees a new minimum or a tie in a few dozen of its 2016 k
+      // images, so the update sits behind a wave-uniform test.
+      if (__any(img <= best)) {
+        // Empty asm: keeps this a taken-rarely branch instead of selects that
+        // would run the update on every image.
+        asm volatile("" ::: "memory");
+        const bool lt = img < best;
+        cnt = lt ? 1u : cnt + (uint32_t)(img == best);
+        best = lt ? img : best;
+      }
     }
   }
 }
 
+// Byte-row layout -> 49-bit mask.
+__device__ __forceinline__ uint64_t rows_to_mask(uint64_t x) {
+  uint64_t m = 0;
+#pragma unroll
+  for (int u = 0; u < 7; ++u) m |= ((x >> (8 * u)) & 0x7Full) << (7 * u);
+  return m;
+}
 
 // Counting sort of set indices by |S| (1..16); bad sizes are flagged.
 __global__ void canon_hist_kernel(const uint64_t* __restrict__ masks, int32_t n,
@@ -83,9 +125,11 @@ __global__ void canon_hist_kernel(const uint64_t* __restrict__ masks, int32_t n,
 
 __global__ void canon_scan_kernel(uint32_t* hist) {
   // hist[0..16] counts -> hist[17..33] running offsets (single thread; 17 bins).
+  // Largest sets first: their waves run longest, so starting them first keeps
+  // the end of the launch to short waves.
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     uint32_t run = 0;
-    for (int k = 0; k <= kMaxKCanon; ++k) {
+    for (int k = kMaxKCanon; k >= 0; --k) {
       hist[17 + k] = run;
       // Round each bucket up to whole waves so no wave mixes two sizes.
       run += (hist[k] + kWave - 1) / kWave * kWave;
@@ -116,9 +160,12 @@ __global__ __launch_bounds__(kCanonBlock) void canon_kernel(
     const uint8_t* __restrict__ gperm, uint64_t* __restrict__ key_lo,
     uint64_t* __restrict__ key_hi, uint64_t* __restrict__ canon, int32_t* __restrict__ stab) {
   __shared__ uint32_t lperm_words[kPermTile * kPermStride / 4];
-  __shared__ uint64_t lkeep[8];
+  __shared__ Anchor lanc[56];                // indexed by byte-row position 8u + v
   uint8_t* lperm = reinterpret_cast<uint8_t*>(lperm_words);
-  if (threadIdx.x < 7) lkeep[threadIdx.x] = keep_low(threadIdx.x);
+  if (threadIdx.x < 56) {
+    const uint32_t u = threadIdx.x >> 3, v = threadIdx.x & 7u;
+    lanc[threadIdx.x] = (u < 7 && v < 7) ? make_anchor(u, v) : Anchor{};
+  }
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t idx = (slot < slots) ? order[slot] : 0xFFFFFFFFu;
   const bool active = idx != 0xFFFFFFFFu;
@@ -154,7 +201,7 @@ __global__ __launch_bounds__(kCanonBlock) void canon_kernel(
     if (!active) continue;
     switch (kw) {
 #define GT_CANON_CASE(KK) \
-  case KK: canon_tile<KK>(lperm, kPermTile, pts, lkeep, best, cnt); break;
+  case KK: canon_tile<KK>(lperm, kPermTile, pts, lanc, best, cnt); break;
       GT_CANON_CASE(1) GT_CANON_CASE(2) GT_CANON_CASE(3) GT_CANON_CASE(4)
       GT_CANON_CASE(5) GT_CANON_CASE(6) GT_CANON_CASE(7) GT_CANON_CASE(8)
       GT_CANON_CASE(9) GT_CANON_CASE(10) GT_CANON_CASE(11) GT_CANON_CASE(12)
@@ -164,6 +211,7 @@ __global__ __launch_bounds__(kCanonBlock) void canon_kernel(
     }
   }
   if (!active) return;
+  best = rows_to_mask(best);
   // Sorted positions of the canonical image, packed as kernel_canon.py:84-104:
   // k <= 10 packs positions 0..k-1, k >= 11 skips position 0 (always 0).
   const int pack_from = (k <= 10) ? 0 : 1;

@@ -119,7 +119,8 @@ void build_mplanes(uint32_t out[kNpts * kRowWords]) {
 
 // GL2(F7) as permutations of the 49 points, 52-byte stride, matrices
 // (a, b, c, d) with a*d - b*c != 0 mod 7 in lexicographic order; point
-// (x, y) = divmod(p, 7) maps to ((a x + b y) % 7) * 7 + (c x + d y) % 7.
+// (x, y) = divmod(p, 7) maps to (u, v) = ((a x + b y) % 7, (c x + d y) % 7),
+// stored as the byte-row position 8 u + v that the canonical-form kernel uses.
 std::vector<uint8_t> build_gl2_perm() {
   std::vector<uint8_t> t(2016 * 52, 0);
   int n = 0;
@@ -130,7 +131,7 @@ std::vector<uint8_t> build_gl2_perm() {
           if (((a * d - b * c) % 7 + 7) % 7 == 0) continue;
           for (int p = 0; p < kNpts; ++p) {
             const int x = p / 7, y = p % 7;
-            t[n * 52 + p] = (uint8_t)(((a * x + b * y) % 7) * 7 + (c * x + d * y) % 7);
+            t[n * 52 + p] = (uint8_t)(((a * x + b * y) % 7) * 8 + (c * x + d * y) % 7);
           }
           ++n;
         }
