@@ -17,6 +17,8 @@
 // Layout: one lane per set, A uniform across the wave (2016 iterations), the
 // GL2 permutation table staged in LDS in tiles.  Sets are first bucketed by
 // size, largest first, so that every wave runs a loop specialised for its k.
+#include <algorithm>
+
 #include "gt_launch.h"
 
 namespace gt {
@@ -112,15 +114,29 @@ __device__ __forceinline__ uint64_t rows_to_mask(uint64_t x) {
   return m;
 }
 
-// Counting sort of set indices by |S| (1..16); bad sizes are flagged.
-__global__ void canon_hist_kernel(const uint64_t* __restrict__ masks, int32_t n,
-                                  uint32_t* hist, uint32_t* ctrl) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint64_t m = masks[i];
-  const int k = __popcll(m);
-  if (k < 1 || k > kMaxKCanon || (m >> 49) != 0) { atomicOr(&ctrl[2], kErrBadK); return; }
-  atomicAdd(&hist[k], 1u);
+// Counting sort of set indices by |S| (1..16); bad sizes are flagged.  Each
+// workgroup counts its grid-stride share in LDS and adds one global atomic per
+// size: a global atomic per set would queue a million atomics on 17 addresses
+// (about 3 ms per pass for 10^6 sets).
+constexpr int kSortBlock = 256;
+constexpr int kSortGrid = 512;
+
+__global__ __launch_bounds__(kSortBlock) void canon_hist_kernel(const uint64_t* __restrict__ masks,
+                                                                int32_t n, uint32_t* hist,
+                                                                uint32_t* ctrl) {
+  __shared__ uint32_t lh[kMaxKCanon + 1];
+  if (threadIdx.x <= kMaxKCanon) lh[threadIdx.x] = 0u;
+  __syncthreads();
+  bool bad = false;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint64_t m = masks[i];
+    const int k = __popcll(m);
+    if (k < 1 || k > kMaxKCanon || (m >> 49) != 0) { bad = true; continue; }
+    atomicAdd(&lh[k], 1u);
+  }
+  if (bad) atomicOr(&ctrl[2], kErrBadK);
+  __syncthreads();
+  if (threadIdx.x <= kMaxKCanon && lh[threadIdx.x] != 0u) atomicAdd(&hist[threadIdx.x], lh[threadIdx.x]);
 }
 
 __global__ void canon_scan_kernel(uint32_t* hist) {
@@ -138,15 +154,35 @@ __global__ void canon_scan_kernel(uint32_t* hist) {
   }
 }
 
-__global__ void canon_scatter_kernel(const uint64_t* __restrict__ masks, int32_t n,
-                                     uint32_t* hist, uint32_t* order) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint64_t m = masks[i];
-  const int k = __popcll(m);
-  if (k < 1 || k > kMaxKCanon || (m >> 49) != 0) return;
-  const uint32_t pos = atomicAdd(&hist[17 + k], 1u);
-  order[pos] = (uint32_t)i;
+// Same grid and the same grid-stride share as canon_hist_kernel: each
+// workgroup counts again, reserves one range per size with one global atomic,
+// then places its sets inside its ranges (LDS atomics; the order inside a
+// bucket does not change any result).
+__global__ __launch_bounds__(kSortBlock) void canon_scatter_kernel(
+    const uint64_t* __restrict__ masks, int32_t n, uint32_t* hist, uint32_t* order) {
+  __shared__ uint32_t lh[kMaxKCanon + 1];
+  __shared__ uint32_t base[kMaxKCanon + 1];
+  if (threadIdx.x <= kMaxKCanon) lh[threadIdx.x] = 0u;
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint64_t m = masks[i];
+    const int k = __popcll(m);
+    if (k < 1 || k > kMaxKCanon || (m >> 49) != 0) continue;
+    atomicAdd(&lh[k], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x <= kMaxKCanon) {
+    const uint32_t c = lh[threadIdx.x];
+    base[threadIdx.x] = (c != 0u) ? atomicAdd(&hist[17 + threadIdx.x], c) : 0u;
+    lh[threadIdx.x] = 0u;
+  }
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint64_t m = masks[i];
+    const int k = __popcll(m);
+    if (k < 1 || k > kMaxKCanon || (m >> 49) != 0) continue;
+    order[base[k] + atomicAdd(&lh[k], 1u)] = (uint32_t)i;
+  }
 }
 
 __global__ void canon_fill_kernel(uint32_t* order, uint32_t len, uint32_t* hist) {
@@ -263,10 +299,11 @@ hipError_t launch_canon(const uint64_t* d_masks, int32_t n, const uint8_t* d_per
   const uint32_t slots = (uint32_t)n + (uint32_t)(kMaxKCanon + 1) * kWave;
   hipLaunchKernelGGL(canon_fill_kernel, dim3((slots + 255) / 256), dim3(256), 0, s, d_order,
                      slots, d_hist);
-  hipLaunchKernelGGL(canon_hist_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_masks, n,
+  const int sort_grid = std::min(kSortGrid, (n + kSortBlock - 1) / kSortBlock);
+  hipLaunchKernelGGL(canon_hist_kernel, dim3(sort_grid), dim3(kSortBlock), 0, s, d_masks, n,
                      d_hist, d_ctrl);
   hipLaunchKernelGGL(canon_scan_kernel, dim3(1), dim3(64), 0, s, d_hist);
-  hipLaunchKernelGGL(canon_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_masks, n,
+  hipLaunchKernelGGL(canon_scatter_kernel, dim3(sort_grid), dim3(kSortBlock), 0, s, d_masks, n,
                      d_hist, d_order);
   hipLaunchKernelGGL(canon_kernel, dim3((slots + kCanonBlock - 1) / kCanonBlock),
                      dim3(kCanonBlock), 0, s, d_masks, d_order, slots, d_perm, d_key_lo,
