@@ -134,12 +134,13 @@ def bench_serving(ctx, n_req: int, seed: int = 4):
     sets = [S for S, _ in reqs]
     tmax = np.array([48 - d for _, d in reqs], dtype=np.int32)
     out = {}
-    with Server(0, slots=1024, work_budget=1 << 36, life_s=600) as srv:
+    with Server(0, work_budget=1 << 36, life_s=600) as srv:
         res0, _, wall0 = srv.run_stream(sets, tmax, np.zeros(n_req), timeout_s=300)
         cap = n_req / wall0
         rate = 0.7 * cap
         arr = workloads.poisson_arrivals(n_req, rate, seed=seed + 1)
         res, lat, wall = srv.run_stream(sets, tmax, arr, timeout_s=300)
+        stats = srv.stats()
     pres0, _, pwall0 = plain_stream(ctx, sets, tmax, np.zeros(n_req))
     pcap = n_req / pwall0
     # The plain form at its own 0.7 x capacity (at the served rate it saturates).
@@ -159,6 +160,8 @@ def bench_serving(ctx, n_req: int, seed: int = 4):
         "plain_p99_ms_at_0.7x_own_capacity": round(float(np.percentile(plat, 99)) * 1e3, 4),
         "served_vs_plain_capacity": round(cap / pcap, 2),
         "results_identical_served_and_plain": same,
+        "slots": srv.slots,
+        "kernel_counters_both_runs": {k: round(v, 4) for k, v in stats.items()},
     })
     return out
 

@@ -18,13 +18,15 @@ struct ServeReq {
   uint64_t tag;
 };
 
-// One result (fine-grained host memory, written by the device): seq = request
-// index + 1 once `result` is valid.
+// One result (fine-grained host memory, written by the device with one 64-bit
+// store): bits 0..31 the result, bits 32..63 (request index + 1) mod 2^32, so
+// the host sees a result and the request it belongs to at once.
 struct ServeRes {
-  int32_t result;
-  uint32_t pad;
-  uint64_t seq;
+  uint64_t word;
 };
+__host__ __device__ __forceinline__ uint64_t serve_res_word(uint64_t r, int32_t result) {
+  return ((r + 1ull) << 32) | (uint32_t)result;
+}
 
 // Host-side control block (fine-grained host memory).
 struct ServeHost {
@@ -34,6 +36,24 @@ struct ServeHost {
   uint32_t exited;      // device -> host: the kernel has stopped
   ServeReq* ring;       // `slots` descriptors, request r in ring[r % slots]
   ServeRes* results;    // `slots` results, request r in results[r % slots]
+};
+
+// Counters the kernel accumulates (per wave, added once when the wave exits);
+// times are device wall-clock ticks.
+enum ServeStat : int {
+  kStDispPasses = 0,   // dispatcher passes that took at least one request
+  kStDispRequests,     // requests dispatched
+  kStDispTicks,        // dispatcher time inside those passes
+  kStChunks,           // chunks walked
+  kStSkipped,          // chunks skipped (their request had already stopped)
+  kStWaitTicks,        // worker time waiting for a chunk to be published
+  kStLookupTicks,      // worker time finding the request, reading its state, staging rows
+  kStWalkTicks,        // worker time in the Gray walk
+  kStFoldTicks,        // worker time folding, counting and claiming
+  kStStaged,           // row stagings into LDS
+  kStLifeTicks,        // dispatcher lifetime
+  kStWorkers,          // worker waves
+  kServeStats
 };
 
 // Device-side state (device memory).
@@ -49,13 +69,26 @@ struct ServeDev {
   uint8_t check_log2[kServeMaxK + 1];
   int32_t* s_k;
   int32_t* s_tmax;
-  uint32_t* s_best;
-  uint32_t* s_done;
+  uint64_t* s_state;    // per slot: bits 0..31 minimum weight, 32..63 chunks finished
   uint32_t* s_nch;
   uint64_t* s_seq;
-  uint64_t* ord_start;  // 4 * slots entries, request r at r % (4 * slots)
+  uint64_t* ord_start;  // 4 * slots tagged entries (ord_entry), request r at r % (4 * slots)
   uint32_t* rows;       // slots * kServeRowWords
+  uint64_t stats[kServeStats];
 };
+
+// ord_start entry of request r: its first global chunk id in bits 0..39 and
+// the tag (r mod 2^23) | 2^23 in bits 40..63.  The tag is never 0 (cleared
+// entries never match) and differs between requests that share a ring entry
+// (4 * slots <= 2^22), so a reader that sees an entry of another request
+// (older, or not yet visible to it) can tell.
+constexpr uint64_t kOrdStartBits = 40;
+__host__ __device__ __forceinline__ uint64_t ord_tag(uint64_t r) {
+  return (r & 0x7FFFFFull) | 0x800000ull;
+}
+__host__ __device__ __forceinline__ uint64_t ord_entry(uint64_t r, uint64_t start) {
+  return (ord_tag(r) << kOrdStartBits) | start;
+}
 
 int serve_blocks_per_cu();
 hipError_t launch_serve(int grid, ServeDev* d, ServeHost* h, const uint32_t* mplanes,

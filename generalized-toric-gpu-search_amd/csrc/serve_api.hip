@@ -74,6 +74,8 @@ struct gt_server {
   std::vector<uint8_t> pending;
   bool running = false;
   uint64_t idle_ticks = 0, life_ticks = 0;
+  double tick_s = 1e-8;                     // device wall-clock period
+  uint64_t last_stats[kServeStats] = {};    // counters of the last kernel that stopped
 
   template <class T>
   int dalloc(T** p, size_t n) {
@@ -90,7 +92,7 @@ struct gt_server {
     __atomic_store_n(&h->consumed, 0ull, __ATOMIC_RELEASE);
     __atomic_store_n(&h->shutdown, 0u, __ATOMIC_RELEASE);
     __atomic_store_n(&h->exited, 0u, __ATOMIC_RELEASE);
-    for (uint32_t s = 0; s < slots; ++s) __atomic_store_n(&res[s].seq, 0ull, __ATOMIC_RELEASE);
+    for (uint32_t s = 0; s < slots; ++s) __atomic_store_n(&res[s].word, 0ull, __ATOMIC_RELEASE);
     next = oldest = outstanding = 0;
     std::fill(pending.begin(), pending.end(), 0);
     SV_HIP(hipSetDevice(device));
@@ -105,6 +107,8 @@ struct gt_server {
     SV_HIP(hipSetDevice(device));
     SV_HIP(hipStreamSynchronize(stream));
     running = false;
+    SV_HIP(hipMemcpyAsync(last_stats, d->stats, sizeof(last_stats), hipMemcpyDeviceToHost, stream));
+    SV_HIP(hipStreamSynchronize(stream));
     return 0;
   }
 
@@ -115,9 +119,10 @@ struct gt_server {
     for (uint64_t r = oldest; r < next && n < max_n; ++r) {
       const uint32_t s = (uint32_t)(r & (slots - 1));
       if (!pending[s]) continue;
-      if (__atomic_load_n(&res[s].seq, __ATOMIC_ACQUIRE) != r + 1) continue;
+      const uint64_t w = __atomic_load_n(&res[s].word, __ATOMIC_ACQUIRE);
+      if ((w >> 32) != ((r + 1) & 0xFFFFFFFFull)) continue;
       if (tags) tags[n] = tag[s];
-      if (results) results[n] = res[s].result;
+      if (results) results[n] = (int32_t)(uint32_t)w;
       if (t_done) t_done[n] = t;
       pending[s] = 0;
       outstanding -= work[s];
@@ -209,6 +214,7 @@ int gt_serve_start(int32_t device, int32_t slots, int64_t work_budget, const int
   if (hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess ||
       clk_khz <= 0)
     clk_khz = 100000;
+  sv->tick_s = 1.0 / (clk_khz * 1e3);
   dh.idle_ticks = (uint64_t)(std::max(idle_s, 0.01) * clk_khz * 1e3);
   dh.life_ticks = (uint64_t)(std::max(life_s, 0.01) * clk_khz * 1e3);
   for (int k = 0; k <= kServeMaxK; ++k) {
@@ -219,7 +225,7 @@ int gt_serve_start(int32_t device, int32_t slots, int64_t work_budget, const int
     dh.check_log2[k] = (uint8_t)c;
   }
   if (sv->dalloc(&dh.s_k, slots) || sv->dalloc(&dh.s_tmax, slots) ||
-      sv->dalloc(&dh.s_best, slots) || sv->dalloc(&dh.s_done, slots) ||
+      sv->dalloc(&dh.s_state, slots) ||
       sv->dalloc(&dh.s_nch, slots) || sv->dalloc(&dh.s_seq, slots) ||
       sv->dalloc(&dh.ord_start, (size_t)4 * slots) ||
       sv->dalloc(&dh.rows, (size_t)slots * kServeRowWords) || sv->dalloc(&sv->d, 1) ||
@@ -317,6 +323,20 @@ int gt_serve_run_stream(gt_server* sv, const int32_t* idx, const int32_t* offset
     if (got == 0) std::this_thread::yield();
   }
   if (wall_s) *wall_s = now_s() - t0;
+  return 0;
+}
+
+int gt_serve_stats(gt_server* sv, double* out, int32_t n) {
+  if (!sv || !out) return sfail("null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (sv->next != sv->oldest) return sfail("serve stats: %llu requests still in flight",
+                                           (unsigned long long)(sv->next - sv->oldest));
+  SV_OK(sv->stop());   // the next submit relaunches the kernel
+  for (int i = 0; i < n && i < kServeStats; ++i) {
+    const bool ticks = i == kStDispTicks || i == kStWaitTicks || i == kStLookupTicks ||
+                       i == kStWalkTicks || i == kStFoldTicks || i == kStLifeTicks;
+    out[i] = ticks ? sv->last_stats[i] * sv->tick_s : (double)sv->last_stats[i];
+  }
   return 0;
 }
 

@@ -8,8 +8,8 @@
 // chunk total.  Every other wave is a worker: it claims the next global chunk
 // id with one atomic, waits until that id is published, finds the request by
 // binary search over the in-flight window, walks the chunk (the Gray walk of
-// the batch kernel, rows read through coherent loads), and folds the wave
-// minimum into the slot.  The wave that finishes the last chunk of a request
+// the batch kernel, rows staged in LDS), and folds the wave minimum into the
+// slot.  The wave that finishes the last chunk of a request
 // writes its result and sequence number to host memory.
 //
 // Requests that arrive while others run are admitted as soon as a slot and
@@ -31,10 +31,6 @@
 namespace gt {
 
 template <class T>
-__device__ __forceinline__ T ld_acq_sys(const T* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-template <class T>
 __device__ __forceinline__ T ld_rlx(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -45,26 +41,37 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, const uint32_t* __restrict
   uint64_t t_idle = t_start;
   uint64_t disp = 0;          // requests dispatched so far
   uint64_t total = 0;         // chunks published so far
+  uint64_t passes = 0, busy = 0;
   const uint32_t slots = d->slots;
   const uint64_t ord_mask = 4ull * slots - 1ull;
   for (;;) {
     const uint64_t now = wall_clock64();
-    const bool stop = __builtin_amdgcn_readfirstlane(ld_acq_sys(&h->shutdown)) != 0u ||
+    // Relaxed polls (an acquire per poll would invalidate this XCD's caches
+    // every time); one system-scope acquire fence once new requests are seen.
+    const bool stop = __builtin_amdgcn_readfirstlane(
+                          __hip_atomic_load(&h->shutdown, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_SYSTEM)) != 0u ||
                       now - t_idle > d->idle_ticks || now - t_start > d->life_ticks;
     if (stop) {
       if (lane == 0) {
+        atomicAdd((unsigned long long*)&d->stats[kStDispPasses], passes);
+        atomicAdd((unsigned long long*)&d->stats[kStDispRequests], disp);
+        atomicAdd((unsigned long long*)&d->stats[kStDispTicks], busy);
+        atomicAdd((unsigned long long*)&d->stats[kStLifeTicks], now - t_start);
         __hip_atomic_store(&d->shutdown, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&h->exited, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
       }
       return;
     }
-    const uint64_t pub = ld_acq_sys(&h->published);
+    const uint64_t pub =
+        __hip_atomic_load(&h->published, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     const uint64_t pubu = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(pub >> 32)) << 32) |
                           __builtin_amdgcn_readfirstlane((uint32_t)pub);
     if (pubu <= disp) {
       __builtin_amdgcn_s_sleep(8);
       continue;
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
     t_idle = now;
     const uint64_t n_new = min(pubu - disp, (uint64_t)kWave);
     const uint64_t r = disp + lane;
@@ -72,12 +79,27 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, const uint32_t* __restrict
     uint32_t nch = 0;
     int k = 0;
     if (lane < n_new) {
-      const ServeReq* q = &h->ring[slot];
-      k = q->k;
+      // The descriptor (80 bytes) in five 16-byte loads: one round trip to
+      // host memory.
+      static_assert(sizeof(ServeReq) == 5 * sizeof(uint4), "ServeReq layout");
+      const uint4* qv = reinterpret_cast<const uint4*>(&h->ring[slot]);
+      uint4 v[5];
+#pragma unroll
+      for (int i = 0; i < 5; ++i) v[i] = qv[i];
+      int32_t qi[4 * 5];
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        qi[4 * i] = (int32_t)v[i].x; qi[4 * i + 1] = (int32_t)v[i].y;
+        qi[4 * i + 2] = (int32_t)v[i].z; qi[4 * i + 3] = (int32_t)v[i].w;
+      }
+      k = qi[0];
+      const int32_t q_tmax = qi[1];
       nch = (k >= 1 && k <= kServeMaxK) ? (uint32_t)code_chunks(k, d->L[k]) : 0u;
       uint32_t* rows = d->rows + (size_t)slot * kServeRowWords;
-      for (int i = 0; i < k && i < kServeMaxK; ++i) {
-        int g = q->idx[i];
+#pragma unroll
+      for (int i = 0; i < kServeMaxK; ++i) {
+        if (i >= k) break;
+        int g = qi[2 + i];
         if (g < 0 || g >= kNpts) g = 0;
         uint32_t w[6];
 #pragma unroll
@@ -90,9 +112,8 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, const uint32_t* __restrict
         }
       }
       d->s_k[slot] = (nch != 0u) ? k : 0;
-      d->s_tmax[slot] = q->tmax;
-      d->s_best[slot] = 64u;
-      d->s_done[slot] = 0u;
+      d->s_tmax[slot] = q_tmax;
+      d->s_state[slot] = 64ull;
       d->s_nch[slot] = nch;
       d->s_seq[slot] = r + 1;
     }
@@ -105,21 +126,22 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, const uint32_t* __restrict
     }
     const uint32_t sum = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
     if (lane < n_new) {
-      d->ord_start[r & ord_mask] = total + (incl - nch);
-      if (nch == 0u) {   // k = 0 (result 0) or unsupported k (result -1): done now
-        h->results[slot].result = (k == 0) ? 0 : -1;
-        __hip_atomic_store(&h->results[slot].seq, r + 1, __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
-      }
+      d->ord_start[r & ord_mask] = ord_entry(r, total + (incl - nch));
+      if (nch == 0u)   // k = 0 (result 0) or unsupported k (result -1): done now
+        __hip_atomic_store(&h->results[slot].word, serve_res_word(r, (k == 0) ? 0 : -1),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
-    __threadfence();
     disp += n_new;
     total += sum;
+    // One release per pass: it orders every row, state and ord_start store of
+    // this pass (and the relaxed `dispatched`) before the new chunk total.
     if (lane == 0) {
       __hip_atomic_store(&d->dispatched, disp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&d->avail, total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&h->consumed, disp, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&h->consumed, disp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
+    ++passes;
+    busy += wall_clock64() - now;
   }
 }
 
@@ -128,90 +150,174 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
          __builtin_amdgcn_readfirstlane((uint32_t)v);
 }
 
+// Rows of the request a wave is walking, staged in that wave's LDS area.
+struct LdsRows {
+  const uint32_t* R;
+  __device__ __forceinline__ RowV operator()(uint32_t r) const {
+    const uint32_t* p = R + r * kRowWords;
+    RowV v;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) v.w[i] = p[i];
+    return v;
+  }
+};
+
 // Claims as in md_main_kernel: one lane-0 region per iteration (it completes
 // the previous chunk and claims the next id), read back with readfirstlane
 // while every lane is active; all other conditions are made wave-uniform.
-__device__ void worker(ServeDev* d, ServeHost* h, uint32_t lane) {
+//
+// Memory ordering.  A worker acquires `avail` once it holds a chunk id that
+// its last observed value does not cover; that acquire makes the rows, slot
+// state and ord_start entries of every request published up to that value
+// visible to plain loads.  The request of chunk g is the last r in the
+// in-flight window whose ord_start entry carries r's tag and starts at or
+// before g; entries of requests this wave cannot see yet carry other tags
+// and start after g anyway.  The request's rows are copied into LDS once per
+// visit, and the Gray walk reads them there.
+__device__ void worker(ServeDev* d, ServeHost* h, uint32_t lane, uint32_t* lrows) {
   const uint64_t slots = d->slots;
   const uint64_t ord_mask = 4ull * slots - 1ull;
+  const uint64_t start_mask = (1ull << kOrdStartBits) - 1ull;
+  uint64_t av_known = 0;
+  uint32_t staged_slot = 0xFFFFFFFFu;
+  uint64_t staged_seq = 0;
+  uint64_t st_chunks = 0, st_skipped = 0, st_staged = 0;
+  uint64_t st_wait = 0, st_lookup = 0, st_walk = 0, st_fold = 0;
   uint64_t first = 0;
   if (lane == 0) first = atomicAdd((unsigned long long*)&d->next_chunk, 1ull);
   uint64_t g = uniform64(first);
   for (;;) {
     // Wait until chunk g is published, or the server stops.
+    uint64_t t0 = wall_clock64();
     bool stop = false;
-    for (;;) {
-      const uint64_t av = uniform64(
-          __hip_atomic_load(&d->avail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
-      if (g < av) break;
-      if (__builtin_amdgcn_readfirstlane(
-              __hip_atomic_load(&d->shutdown, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT))) {
-        stop = true;
-        break;
+    if (g >= av_known) {
+      // Spin on relaxed loads; one acquire fence once the chunk is published
+      // (an acquire per poll would invalidate this XCD's caches every time).
+      for (;;) {
+        const uint64_t av = uniform64(ld_rlx(&d->avail));
+        if (g < av) { av_known = av; break; }
+        if (__builtin_amdgcn_readfirstlane(ld_rlx(&d->shutdown))) { stop = true; break; }
+        __builtin_amdgcn_s_sleep(8);
       }
-      __builtin_amdgcn_s_sleep(4);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    if (stop) return;
-    // Request of chunk g: last r in the in-flight window with ord_start <= g.
-    const uint64_t disp = uniform64(
-        __hip_atomic_load(&d->dispatched, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+    uint64_t t1 = wall_clock64();
+    st_wait += t1 - t0;
+    if (stop) break;
+    // Request of chunk g.
+    const uint64_t disp = uniform64(ld_rlx(&d->dispatched));
     uint64_t lo = (disp > slots) ? disp - slots : 0, hi = disp - 1;
     while (lo < hi) {
       const uint64_t mid = (lo + hi + 1) >> 1;
-      if (uniform64(ld_rlx(&d->ord_start[mid & ord_mask])) <= g) lo = mid; else hi = mid - 1;
+      const uint64_t e = uniform64(d->ord_start[mid & ord_mask]);
+      if ((e >> kOrdStartBits) == ord_tag(mid) && (e & start_mask) <= g) lo = mid; else hi = mid - 1;
     }
     const uint32_t slot = (uint32_t)(lo & (slots - 1));
-    const int k = (int)__builtin_amdgcn_readfirstlane((uint32_t)ld_rlx(&d->s_k[slot]));
-    const int tmax = (int)__builtin_amdgcn_readfirstlane((uint32_t)ld_rlx(&d->s_tmax[slot]));
-    const uint32_t nch = __builtin_amdgcn_readfirstlane(ld_rlx(&d->s_nch[slot]));
-    const uint64_t seq = uniform64(ld_rlx(&d->s_seq[slot]));
-    const uint64_t local = g - uniform64(ld_rlx(&d->ord_start[lo & ord_mask]));
+    const int k = (int)__builtin_amdgcn_readfirstlane((uint32_t)d->s_k[slot]);
+    const int tmax = (int)__builtin_amdgcn_readfirstlane((uint32_t)d->s_tmax[slot]);
+    const uint32_t nch = __builtin_amdgcn_readfirstlane(d->s_nch[slot]);
+    const uint64_t seq = uniform64(d->s_seq[slot]);
+    const uint64_t local = g - (uniform64(d->ord_start[lo & ord_mask]) & start_mask);
     const int thr = 48 - tmax;
-    uint32_t* best_g = &d->s_best[slot];
+    uint64_t* state = &d->s_state[slot];
+    // Low half of the slot state: the running minimum weight.
+    const uint32_t* best_g = reinterpret_cast<const uint32_t*>(state);
     const bool skip = (int)__builtin_amdgcn_readfirstlane(ld_rlx(best_g)) <= thr;
     uint32_t b = 64u;
+    uint64_t t2 = t1;
     if (!skip) {
+      if (slot != staged_slot || seq != staged_seq) {
+        const uint32_t* src = d->rows + (size_t)slot * kServeRowWords;
+        const int nw = 3 * k * kRowWords;
+        uint32_t tmp[kServeRowWords / kWave];
+#pragma unroll
+        for (int i = 0; i < kServeRowWords / kWave; ++i) {
+          const int e = (int)lane + i * kWave;
+          tmp[i] = (e < nw) ? src[e] : 0u;
+        }
+#pragma unroll
+        for (int i = 0; i < kServeRowWords / kWave; ++i) lrows[(int)lane + i * kWave] = tmp[i];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        staged_slot = slot;
+        staged_seq = seq;
+        ++st_staged;
+      }
+      t2 = wall_clock64();
       const int L = d->L[k];
       const uint32_t cmask = (1u << d->check_log2[k]) - 1u;
-      const CoherentRows rows{d->rows + (size_t)slot * kServeRowWords};
-      b = run_chunk<true>(rows, k, L, local, lane, thr, cmask, best_g);
+      b = run_chunk<true>(LdsRows{lrows}, k, L, local, lane, thr, cmask, best_g);
       b = wave_min(b);
+      ++st_chunks;
+    } else {
+      t2 = wall_clock64();
+      ++st_skipped;
     }
+    const uint64_t t3 = wall_clock64();
+    st_lookup += t2 - t1;
+    st_walk += t3 - t2;
     uint64_t next = 0;
     if (lane == 0) {
-      if (!skip) atomicMin(best_g, b);
-      __threadfence();
-      const uint32_t done = atomicAdd(&d->s_done[slot], 1u) + 1u;
+      // Fold the minimum, wait until that atomic has completed (s_waitcnt 0:
+      // every counter of this wave at zero; an agent-scope release would add
+      // a write-back of the whole L2), then count the chunk.  Every wave's
+      // minimum has therefore been performed at the device coherence point
+      // before its count is issued, and the wave whose count completes the
+      // request reads the final minimum there.
+      uint32_t* st = reinterpret_cast<uint32_t*>(state);
+      if (!skip) __hip_atomic_fetch_min(&st[0], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_s_waitcnt(0);
+      const uint32_t done =
+          __hip_atomic_fetch_add(&st[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
       if (done == nch) {
-        __threadfence();
-        const int bw = (int)ld_rlx(best_g);
-        h->results[slot].result = (bw <= thr) ? tmax + 1 : 49 - bw;
-        __hip_atomic_store(&h->results[slot].seq, seq, __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
+        const int bw = (int)ld_rlx(&st[0]);
+        __hip_atomic_store(&h->results[slot].word,
+                           serve_res_word(seq - 1, (bw <= thr) ? tmax + 1 : 49 - bw),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       }
       next = atomicAdd((unsigned long long*)&d->next_chunk, 1ull);
     }
     g = uniform64(next);
+    st_fold += wall_clock64() - t3;
+  }
+  if (lane == 0) {
+    atomicAdd((unsigned long long*)&d->stats[kStChunks], st_chunks);
+    atomicAdd((unsigned long long*)&d->stats[kStSkipped], st_skipped);
+    atomicAdd((unsigned long long*)&d->stats[kStWaitTicks], st_wait);
+    atomicAdd((unsigned long long*)&d->stats[kStLookupTicks], st_lookup);
+    atomicAdd((unsigned long long*)&d->stats[kStWalkTicks], st_walk);
+    atomicAdd((unsigned long long*)&d->stats[kStFoldTicks], st_fold);
+    atomicAdd((unsigned long long*)&d->stats[kStStaged], st_staged);
+    atomicAdd((unsigned long long*)&d->stats[kStWorkers], 1ull);
   }
 }
 
 __global__ __launch_bounds__(256) void serve_kernel(ServeDev* d, ServeHost* h,
                                                     const uint32_t* __restrict__ mplanes) {
+  __shared__ uint32_t lrows[256 / kWave][kServeRowWords];
   const uint32_t lane = __lane_id();
   const uint32_t wave = threadIdx.x / kWave;
   if (blockIdx.x == 0 && wave == 0)
     dispatcher(d, h, mplanes, lane);
   else
-    worker(d, h, lane);
+    worker(d, h, lane, lrows[wave]);
 }
 
+// Resets the counters and clears the ord_start ring (a cleared entry carries
+// tag 0, which no request has).
 __global__ void serve_init_kernel(ServeDev* d) {
+  const uint64_t n = 4ull * d->slots;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    d->ord_start[i] = 0ull;
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     d->next_chunk = 0;
     d->avail = 0;
     d->dispatched = 0;
     d->shutdown = 0;
   }
+  if (blockIdx.x == 0 && threadIdx.x < kServeStats) d->stats[threadIdx.x] = 0ull;
 }
 
 int serve_blocks_per_cu() {
@@ -224,7 +330,7 @@ int serve_blocks_per_cu() {
 
 hipError_t launch_serve(int grid, ServeDev* d, ServeHost* h, const uint32_t* mplanes,
                         hipStream_t s) {
-  hipLaunchKernelGGL(serve_init_kernel, dim3(1), dim3(64), 0, s, d);
+  hipLaunchKernelGGL(serve_init_kernel, dim3(64), dim3(256), 0, s, d);
   hipLaunchKernelGGL(serve_kernel, dim3(grid), dim3(256), 0, s, d, h, mplanes);
   return hipGetLastError();
 }

@@ -52,6 +52,7 @@ _SIGS = {
     "gt_serve_in_flight": ([vp, P(i64)], ctypes.c_int),
     "gt_serve_run_stream": ([vp, vp, vp, i32, vp, vp, vp, vp, P(ctypes.c_double),
                              ctypes.c_double], ctypes.c_int),
+    "gt_serve_stats": ([vp, vp, i32], ctypes.c_int),
     "gt_serve_stop": ([vp], ctypes.c_int),
     "gt_mlp_num_params": ([P(i32)], ctypes.c_int),
     "gt_mlp_create": ([i32, u64, P(vp)], ctypes.c_int),

@@ -23,13 +23,25 @@ from ._native import check, i32, i64, lib, ptr, vp
 MAX_K = 16
 
 
+# Serving chunks are 2^SERVE_L_OFFSET times the batch kernel's: a served chunk
+# also pays a request lookup, a row staging and a fold at the device coherence
+# point, so it needs more walk to amortise them.  Measured on the 20,000-request
+# mixed-k serving stream (all arrivals at t = 0, 4096 slots): L + 0 149k, L + 2
+# 374k, L + 3 561k, L + 4 781k requests/s.
+SERVE_L_OFFSET = 4
+
+
 class Server:
-    def __init__(self, device: int = 0, slots: int = 1024, work_budget: int = 1 << 36,
+    def __init__(self, device: int = 0, slots: int = 4096, work_budget: int = 1 << 36,
                  params: dict | None = None, idle_s: float = 10.0, life_s: float = 600.0):
-        """params: {k: (L, check_log2)}; defaults come from a context's table."""
+        """params: {k: (L, check_log2)}; defaults: the context's (tuned) batch
+        table with L raised by SERVE_L_OFFSET."""
         if params is None:
             ctx = _native.context(device)
-            params = {k: ctx.get_md_params(k) for k in range(MAX_K + 1)}
+            params = {}
+            for k in range(MAX_K + 1):
+                L, c = ctx.get_md_params(k)
+                params[k] = (min(20, L + SERVE_L_OFFSET), c)
         L = np.array([params.get(k, (10, 2))[0] for k in range(MAX_K + 1)], dtype=np.int32)
         C = np.array([params.get(k, (10, 2))[1] for k in range(MAX_K + 1)], dtype=np.int32)
         h = ctypes.c_void_p()
@@ -73,6 +85,18 @@ class Server:
         check(lib().gt_serve_run_stream(self.handle, ptr(flat), ptr(offs), n, ptr(tm), ptr(arr),
                                         ptr(res), ptr(lat), ctypes.byref(wall), float(timeout_s)))
         return res, lat, wall.value
+
+    STAT_NAMES = ("dispatcher_passes", "requests_dispatched", "dispatcher_busy_s",
+                  "chunks_walked", "chunks_skipped", "worker_wait_s", "worker_lookup_s",
+                  "worker_walk_s", "worker_fold_s", "row_stagings", "kernel_life_s",
+                  "worker_waves")
+
+    def stats(self) -> dict:
+        """Stop the kernel (nothing may be in flight; the next submit relaunches
+        it) and return its counters; worker times are summed over waves."""
+        out = np.zeros(len(self.STAT_NAMES), dtype=np.float64)
+        check(lib().gt_serve_stats(self.handle, ptr(out), len(out)))
+        return dict(zip(self.STAT_NAMES, out.tolist()))
 
     def close(self) -> None:
         if self.handle:

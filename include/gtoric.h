@@ -129,6 +129,15 @@ int gt_serve_in_flight(gt_server* srv, int64_t* n);
 int gt_serve_run_stream(gt_server* srv, const int32_t* idx, const int32_t* offsets, int32_t n,
                         const int32_t* tmax, const double* arrival_s, int32_t* results,
                         double* latency_s, double* wall_s, double timeout_s);
+/* Stop the kernel (only with nothing in flight; the next submit relaunches
+ * it) and return its counters: out[0] dispatcher passes, [1] requests
+ * dispatched, [2] dispatcher busy seconds, [3] chunks walked, [4] chunks
+ * skipped, [5..8] worker seconds waiting / finding the request and staging
+ * rows / walking / folding and claiming (summed over waves), [9] row
+ * stagings, [10] dispatcher lifetime seconds, [11] worker waves.  Writes
+ * min(n, 12) values.  No reference counterpart (observability). */
+int gt_serve_stats(gt_server* srv, double* out, int32_t n);
+
 /* Stop the kernel (waits for it to exit) and free the server. */
 int gt_serve_stop(gt_server* srv);
 

@@ -47,6 +47,28 @@ def test_stream_matches_alone_and_oracle(ctx):
     assert np.all(lat >= 0) and wall > 0
 
 
+def test_counters_account_for_every_chunk(ctx):
+    """gt_serve_stats: every request dispatched once, and chunks walked plus
+    chunks skipped equal the chunk count of the served parameter table."""
+    from gtoric.autotune import code_chunks
+    from gtoric.serving import SERVE_L_OFFSET, Server
+    sets, tmax = _requests(500, seed=56, kmax=9)
+    with Server(0, slots=256, life_s=60) as srv:
+        res, lat, wall = srv.run_stream(sets, tmax, np.zeros(len(sets)), timeout_s=60)
+        st = srv.stats()
+        L = {k: min(20, ctx.get_md_params(k)[0] + SERVE_L_OFFSET) for k in range(1, 17)}
+    assert st["requests_dispatched"] == len(sets)
+    want = sum(code_chunks(len(S), L[len(S)]) for S in sets if len(S))
+    assert st["chunks_walked"] + st["chunks_skipped"] == want
+    assert st["worker_waves"] > 0 and st["kernel_life_s"] > 0
+    # The server relaunches on the next submit after stats() stopped it.
+    with Server(0, slots=64, life_s=60) as srv2:
+        a, _, _ = srv2.run_stream(sets[:50], tmax[:50], np.zeros(50), timeout_s=60)
+        srv2.stats()
+        b, _, _ = srv2.run_stream(sets[:50], tmax[:50], np.zeros(50), timeout_s=60)
+    assert a.tolist() == b.tolist() == res[:50].tolist()
+
+
 def test_mixed_large_k_and_small_budget(ctx):
     from gtoric.distance import flatten
     from gtoric.serving import Server
