@@ -166,9 +166,17 @@ def bench_serving(ctx, n_req: int, seed: int = 4):
     return out
 
 
+# Number of AGL2(F7) classes of k-subsets of the 7x7 torus (Burnside; the oracle
+# tests pin k = 1..4 and the sum rule).
+BURNSIDE_CLASSES = {1: 1, 2: 1, 3: 3, 4: 8, 5: 32, 6: 179, 7: 954, 8: 4758, 9: 21225,
+                    10: 84050, 11: 296557, 12: 937022, 13: 2663422, 14: 6843998}
+
+
 def bench_evolve(ks, pop: int, generations: int, seed: int = 1):
     """Config 5: unranked search for `generations`, then the surrogate-ranked
-    search for the same wall-clock; best exact distance per k."""
+    search for the same wall-clock; best exact distance per k, and the number
+    of distinct AGL2 classes each run evaluated at the top distance reached
+    (the discovery rate the ranking is for)."""
     from gtoric import _native
     from gtoric.evolve import run
     from gtoric.surrogate import Surrogate
@@ -180,16 +188,35 @@ def bench_evolve(ks, pop: int, generations: int, seed: int = 1):
         ranked = run(k, pop=pop, generations=generations * 10, seed=seed, surrogate=sur, ctx=ctx,
                      wall_limit_s=base.wall_s)
         sur.close()
+        top = max(base.best_d, ranked.best_d)
         rows[str(k)] = {
+            "top_d": top,
+            "agl2_classes_of_size_k": BURNSIDE_CLASSES.get(k),
             "unranked": {"best_d": base.best_d, "generations": base.generations,
+                         "time_to_best_s": round(base.time_to_best_s, 3),
+                         "classes_at_top_d": base.hits(top),
+                         "classes_at_top_d_minus_1": base.hits(top - 1),
+                         "classes_at_top_d_minus_2": base.hits(top - 2),
+                         "mean_d_evaluated": round(float(base.dists.mean()), 3),
+                         "distinct_classes_evaluated": int(np.unique(base.classes).size),
                          "verified_codes_per_s": round(base.codes_per_s, 1),
                          "wall_s": round(base.wall_s, 2)},
             "ranked": {"best_d": ranked.best_d, "generations": ranked.generations,
+                       "time_to_best_s": round(ranked.time_to_best_s, 3),
+                       "classes_at_top_d": ranked.hits(top),
+                       "classes_at_top_d_minus_1": ranked.hits(top - 1),
+                       "classes_at_top_d_minus_2": ranked.hits(top - 2),
+                       "mean_d_evaluated": round(float(ranked.dists.mean()), 3),
+                       "distinct_classes_evaluated": int(np.unique(ranked.classes).size),
                        "verified_codes_per_s": round(ranked.codes_per_s, 1),
                        "wall_s": round(ranked.wall_s, 2)},
         }
-        log(f"[bench] evolve k={k}: unranked d={base.best_d} ({base.generations} gens), "
-            f"ranked d={ranked.best_d} ({ranked.generations} gens), {base.wall_s:.1f}s each")
+        log(f"[bench] evolve k={k} ({base.wall_s:.1f}s each): unranked d={base.best_d} "
+            f"({base.generations} gens; classes at d>={top}/{top - 1}/{top - 2}: "
+            f"{base.hits(top)}/{base.hits(top - 1)}/{base.hits(top - 2)}; mean d "
+            f"{base.dists.mean():.2f}), ranked d={ranked.best_d} ({ranked.generations} gens; "
+            f"{ranked.hits(top)}/{ranked.hits(top - 1)}/{ranked.hits(top - 2)}; mean d "
+            f"{ranked.dists.mean():.2f})")
     return rows
 
 
@@ -245,6 +272,9 @@ def main() -> None:
     ap.add_argument("--evolve-k", default="6,7,8,9,10")
     ap.add_argument("--evolve-pop", type=int, default=2048)
     ap.add_argument("--evolve-generations", type=int, default=200)
+    ap.add_argument("--evolve-hard-k", default="11",
+                    help="k values run for --evolve-hard-generations (class space not exhausted)")
+    ap.add_argument("--evolve-hard-generations", type=int, default=20)
     ap.add_argument("--mode", default="bench", choices=["bench", "copy"],
                     help="copy: only the copy-kernel calibration run for --pmc traffic")
     args = ap.parse_args()
@@ -347,6 +377,10 @@ def main() -> None:
     if "evolve" not in skip:
         ks = [int(x) for x in args.evolve_k.split(",") if x]
         result["evolve"] = bench_evolve(ks, args.evolve_pop, args.evolve_generations)
+        hard = [int(x) for x in args.evolve_hard_k.split(",") if x]
+        if hard:
+            result["evolve_unexhausted"] = bench_evolve(hard, args.evolve_pop,
+                                                        args.evolve_hard_generations)
 
     if "cpu" not in skip:
         cores = min(os.cpu_count() or 1, 16)

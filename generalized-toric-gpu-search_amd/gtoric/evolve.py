@@ -33,11 +33,17 @@ BIT = np.left_shift(np.uint64(1), np.arange(N_POINTS, dtype=np.uint64))
 
 
 def masks_to_bits(masks: np.ndarray) -> np.ndarray:
-    return ((masks[:, None] & BIT[None, :]) != 0)
+    """(n,) uint64 masks -> (n, 49) bool, bit p of mask i at [i, p]."""
+    m = np.ascontiguousarray(masks, dtype=np.uint64)
+    return np.unpackbits(m.view(np.uint8).reshape(-1, 8), axis=1,
+                         bitorder="little")[:, :N_POINTS].astype(bool)
 
 
 def bits_to_masks(bits: np.ndarray) -> np.ndarray:
-    return np.bitwise_or.reduce(np.where(bits, BIT[None, :], np.uint64(0)), axis=1)
+    p = np.packbits(bits, axis=1, bitorder="little")          # (n, 7) bytes
+    out = np.zeros((len(bits), 8), dtype=np.uint8)
+    out[:, :p.shape[1]] = p
+    return out.view(np.uint64).ravel()
 
 
 def random_masks(rng, n: int, k: int) -> np.ndarray:
@@ -49,21 +55,26 @@ def random_masks(rng, n: int, k: int) -> np.ndarray:
 
 
 def mutate(rng, parents: np.ndarray) -> np.ndarray:
-    """Swap 1 or 2 points of each parent mask."""
+    """Swap 1 or 2 points of each parent mask (all of one size k, 2 <= k <=
+    47): remove one or two distinct points of S and add as many distinct
+    points outside S."""
     bits = masks_to_bits(parents)
     n = len(parents)
-    swaps = rng.integers(1, 3, size=n)
-    keys = rng.random((n, N_POINTS))
-    out_keys = np.where(bits, keys, 2.0)           # removal candidates: set bits
-    in_keys = np.where(bits, 2.0, keys)            # addition candidates: clear bits
-    ro = np.argsort(out_keys, axis=1)
-    ri = np.argsort(in_keys, axis=1)
+    if n == 0:
+        return parents.copy()
+    k = int(bits[0].sum())
     rows = np.arange(n)
-    bits[rows, ro[:, 0]] = False
-    bits[rows, ri[:, 0]] = True
-    two = swaps == 2
-    bits[rows[two], ro[two, 1]] = False
-    bits[rows[two], ri[two, 1]] = True
+    inside = np.nonzero(bits)[1].reshape(n, k)                    # points of S, sorted
+    outside = np.nonzero(~bits)[1].reshape(n, N_POINTS - k)       # points not in S
+    a1 = rng.integers(0, k, size=n)
+    b1 = rng.integers(0, N_POINTS - k, size=n)
+    a2 = (a1 + 1 + rng.integers(0, k - 1, size=n)) % k            # another point of S
+    b2 = (b1 + 1 + rng.integers(0, N_POINTS - k - 1, size=n)) % (N_POINTS - k)
+    two = rng.random(n) < 0.5
+    bits[rows, inside[rows, a1]] = False
+    bits[rows, outside[rows, b1]] = True
+    bits[rows[two], inside[rows, a2][two]] = False
+    bits[rows[two], outside[rows, b2][two]] = True
     return bits_to_masks(bits)
 
 
@@ -82,10 +93,25 @@ class EvolveResult:
     wall_s: float
     eval_s: float
     history: list = field(default_factory=list)   # (wall_s, best_d) per generation
+    classes: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint64))  # canonical forms evaluated
+    dists: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))     # their exact distances
+
+    def hits(self, target_d: int) -> int:
+        """Distinct AGL2 classes evaluated with exact distance >= target_d
+        (every evaluated set is a new class: the driver deduplicates)."""
+        return int(np.unique(self.classes[self.dists >= target_d]).size)
 
     @property
     def codes_per_s(self) -> float:
         return self.evaluated / max(self.wall_s, 1e-9)
+
+    @property
+    def time_to_best_s(self) -> float:
+        """Wall-clock time at which the final best distance was first reached."""
+        for t, d in self.history:
+            if d >= self.best_d:
+                return t
+        return 0.0
 
 
 class GenerationGraphs:
@@ -143,8 +169,8 @@ class GenerationGraphs:
 
 def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogate=None,
         proposal_factor: int = 4, wall_limit_s: float | None = None, train_every: int = 5,
-        train_epochs: int = 3, ctx: _native.Context | None = None, verbose: bool = False
-        ) -> EvolveResult:
+        train_epochs: int = 3, train_window: int = 65536, ctx: _native.Context | None = None,
+        verbose: bool = False) -> EvolveResult:
     ctx = ctx or _native.context(0)
     rng = np.random.default_rng(seed)
     n_prop = pop * (proposal_factor if surrogate is not None else 1)
@@ -161,7 +187,9 @@ def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogat
     evaluated = pop
     if surrogate is not None:
         surrogate.fit(popc, popd, epochs=train_epochs, batch=1024, seed=0)
-    data_m, data_d = [popc], [popd]
+    # data_m / data_d collect every evaluated class (both modes): the surrogate
+    # trains on them, and they give the discovery counts of EvolveResult.hits.
+    data_m, data_d = [popc], [popd]         # every evaluated class and its distance
     history = []
     gen = 0
     for gen in range(1, generations + 1):
@@ -174,12 +202,9 @@ def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogat
         props = mutate(rng, popm[win])
         # 2. canonical dedup against everything evaluated so far
         cm = gg.canon(props)
-        keep, fresh = [], set()
-        for i, c in enumerate(cm.tolist()):
-            if c not in seen and c not in fresh:
-                fresh.add(c)
-                keep.append(i)
-        keep = np.array(keep, dtype=np.int64)
+        u, first = np.unique(cm, return_index=True)
+        new = np.fromiter((c not in seen for c in u.tolist()), dtype=bool, count=len(u))
+        keep = np.sort(first[new]).astype(np.int64)       # proposal order
         # 3. choose pop proposals (surrogate order or proposal order)
         if surrogate is not None and len(keep) > pop:
             order = surrogate.rank(cm[keep])
@@ -189,9 +214,13 @@ def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogat
         chosen = props[keep]
         ccm = cm[keep]
         if len(chosen) < pop:
-            extra = random_masks(rng, pop - len(chosen), k)
-            chosen = np.concatenate([chosen, extra])
-            ccm = np.concatenate([ccm, ctx.canon_masks(extra)[2]])
+            # Fresh random sets, canonicalised by the captured graph (a host
+            # call on this context could grow its buffers and invalidate the
+            # graphs).
+            m = pop - len(chosen)
+            extra = random_masks(rng, n_prop, k)
+            chosen = np.concatenate([chosen, extra[:m]])
+            ccm = np.concatenate([ccm, gg.canon(extra)[:m]])
         seen.update(ccm.tolist())
         # 4. exact distances
         te = time.perf_counter()
@@ -207,8 +236,11 @@ def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogat
         order = np.lexsort((alla, -alld))[:pop]
         popm, popd, age = allm[order], alld[order], alla[order]
         if surrogate is not None and gen % train_every == 0:
-            surrogate.fit(np.concatenate(data_m), np.concatenate(data_d), epochs=train_epochs,
-                          batch=1024, seed=gen)
+            # A sliding window of the most recent exact results keeps a
+            # retraining step at a fixed cost as the run grows.
+            wm = np.concatenate(data_m)[-train_window:]
+            wd = np.concatenate(data_d)[-train_window:]
+            surrogate.fit(wm, wd, epochs=train_epochs, batch=1024, seed=gen)
         history.append((time.perf_counter() - t0, int(popd[0])))
         if verbose:
             print(f"[evolve k={k}] gen {gen} best d={popd[0]} unique={len(keep)}", flush=True)
@@ -218,4 +250,5 @@ def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogat
     return EvolveResult(k=k, best_d=int(popd[0]),
                         best_set=[int(p) for p in range(N_POINTS) if (int(best) >> p) & 1],
                         generations=gen, evaluated=evaluated, wall_s=wall, eval_s=t_eval,
-                        history=history)
+                        history=history, classes=np.concatenate(data_m),
+                        dists=np.concatenate(data_d).astype(np.int32))
