@@ -70,10 +70,12 @@ __global__ __launch_bounds__(1024) void md_plan_kernel(const int32_t* __restrict
   constexpr int kWaves = 1024 / kWave;
   __shared__ uint32_t wc[kWaves][kB];
   __shared__ uint32_t run[kB];
+  __shared__ uint64_t ccs[kB];       // chunks per code of bucket b, one thread each
   const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
   const uint64_t below = (1ull << lane) - 1ull;
   uint32_t err = 0;
   if (tid < kClaimShards) claims[tid * kClaimStride] = 0u;
+  if (tid < kB) ccs[tid] = code_chunks(kMaxKExact - tid, p->L[kMaxKExact - tid]);
 
   // Pass 1: lane b of each wave accumulates the size of bucket b.
   uint32_t mine = 0;
@@ -104,8 +106,7 @@ __global__ __launch_bounds__(1024) void md_plan_kernel(const int32_t* __restrict
     uint32_t pos = 0;
     for (int b = 0; b < kB; ++b) {
       const uint32_t t = run[b];
-      const int k = kMaxKExact - b;
-      const uint64_t cc = code_chunks(k, p->L[k]);
+      const uint64_t cc = ccs[b];
       plan->cb[b] = (uint32_t)chunks;
       plan->pb[b] = pos;
       plan->cc[b] = (uint32_t)cc;
