@@ -220,6 +220,28 @@ def bench_evolve(ks, pop: int, generations: int, seed: int = 1):
     return rows
 
 
+def bench_sampled(ctx, ks=(12, 16), n: int = 2048, samples: int = 200_000, seed: int = 5):
+    """Deep-k sampled screening (kernel.py:136-178 stream; evolve_search.py
+    default 200,000 samples per set): n seeded sets per k, no early stop
+    (tmax = 48), samples/s of the md_sampled kernel (HIP events)."""
+    from gtoric.distance import flatten
+    rng = np.random.default_rng(seed)
+    rows = {}
+    for k in ks:
+        sets = [sorted(rng.choice(49, k, replace=False).tolist()) for _ in range(n)]
+        flat, offs = flatten(sets)
+        ctx.md_batch(flat, offs, 48, samples, 1)
+        ctx.prof_read("md_sampled")
+        ctx.prof_enable(True)
+        ctx.md_batch(flat, offs, 48, samples, 1)
+        ctx.prof_enable(False)
+        nk, ms = ctx.prof_read("md_sampled")
+        rows[str(k)] = {"sets": n, "samples_per_set": samples, "kernel_ms": round(ms / max(nk, 1), 3),
+                        "samples_per_s": round(n * samples / (ms / max(nk, 1) * 1e-3), 1)}
+        log(f"[bench] sampled k={k}: {rows[str(k)]['samples_per_s'] / 1e9:.1f} G samples/s")
+    return rows
+
+
 def copy_calibration(ctx, nbytes: int = 1 << 30, reps: int = 5):
     """A plain device copy kernel of known size, for scaling FETCH_SIZE/WRITE_SIZE."""
     a = ctx.array(nbytes // 4, np.uint32)
@@ -267,7 +289,7 @@ def main() -> None:
     ap.add_argument("--canon-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline")
     ap.add_argument("--skip", default="",
-                    help="comma list of: plain,canon,cpu,serve,evolve,tune")
+                    help="comma list of: plain,canon,sampled,cpu,serve,evolve,tune")
     ap.add_argument("--serve-requests", type=int, default=20_000)
     ap.add_argument("--evolve-k", default="6,7,8,9,10")
     ap.add_argument("--evolve-pop", type=int, default=2048)
@@ -366,6 +388,9 @@ def main() -> None:
             "classes": classes,
         }
         log(f"[bench] canon: {len(masks) * args.canon_steps / cdt:,.0f} forms/s, {classes} classes")
+
+    if "sampled" not in skip:
+        result["sampled"] = bench_sampled(ctx)
 
     if "serve" not in skip:
         result["serving"] = bench_serving(ctx, args.serve_requests)
