@@ -28,6 +28,9 @@ for _p in (str(ROOT), str(ROOT / "generalized-toric-gpu-search_amd")):
 
 METRIC = "exact min-distance codes/sec (mixed k=3..10) + p50/p99 latency; canon forms/sec"
 OPS_PER_CODEWORD = 11       # 6 XOR + 2 OR3 + 2 BCNT + 1 MIN per projective codeword
+# What the compiled walk issues: one 16-step block is 167 VALU instructions
+# (96 XOR, 32 OR3, 30 BCNT, 9 MIN3 - one MIN3 folds two minima).
+VALU_PER_CODEWORD = 167 / 16
 LANES_PER_CU_CLK = 64       # 4 SIMD x 16 lanes: one int32 VALU op per lane per clock
 
 
@@ -83,9 +86,26 @@ def bench_md(ctx, sets, steps: int, warmup: int, expect=None):
     dt = time.perf_counter() - t0
     ctx.prof_enable(False)
     nk, kms = ctx.prof_read("md_main")
+    # The same step captured once as a hipGraph and replayed (how the
+    # evolutionary driver runs its generations): four kernels, one launch.
+    ctx.capture_begin()
+    step()
+    graph = ctx.capture_end()
+    for _ in range(warmup):
+        graph.launch()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        graph.launch()
+    ctx.sync()
+    dt_graph = time.perf_counter() - t0
+    got_g = d_out.download()
+    graph.close()
+    if expect is not None and not np.array_equal(got_g, expect):
+        raise SystemExit("parity failure in the graph replay")
     for a in (d_idx, d_off, d_out):
         a.close()
-    return dt, nk, kms, got
+    return dt, nk, kms, got, dt_graph
 
 
 def bench_plain(ctx, sets, expect=None):
@@ -289,7 +309,7 @@ def main() -> None:
     ap.add_argument("--canon-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline")
     ap.add_argument("--skip", default="",
-                    help="comma list of: plain,canon,sampled,cpu,serve,evolve,tune")
+                    help="comma list of: wide,plain,canon,sampled,cpu,serve,evolve,tune")
     ap.add_argument("--serve-requests", type=int, default=20_000)
     ap.add_argument("--evolve-k", default="6,7,8,9,10")
     ap.add_argument("--evolve-pop", type=int, default=2048)
@@ -323,7 +343,7 @@ def main() -> None:
     gold = ROOT / "tests" / "golden" / "md_golden.npz"
     expect = np.load(gold)["c2_mz"] if gold.exists() else None
 
-    dt, nk, kms, got = bench_md(ctx, sets, args.steps, args.warmup, expect)
+    dt, nk, kms, got, dt_graph = bench_md(ctx, sets, args.steps, args.warmup, expect)
     value = n * args.steps / dt
     ms_step = dt / args.steps * 1e3
     kern_ms = kms / max(nk, 1)
@@ -363,10 +383,39 @@ def main() -> None:
             "traffic": profiled_traffic(),
             "traffic_source": str(PROFILE.relative_to(ROOT)) if profiled_traffic() else None,
             "ops_per_codeword": OPS_PER_CODEWORD,
+            "valu_issue_frac": round(VALU_PER_CODEWORD * codewords / (kern_ms * 1e-3) / 1e12 / peak, 4),
             "kernel_ms": round(kern_ms, 4),
             "kernel_launches": nk,
         },
     }
+
+    result["graph_replay"] = {
+        "value": round(n * args.steps / dt_graph, 1), "unit": "codes/s",
+        "ms_per_step": round(dt_graph / args.steps * 1e3, 4),
+        "form": "the same step (rows, plan, walk, finalize) captured once as a hipGraph and "
+                "replayed",
+    }
+    log(f"[bench] graph replay: {n * args.steps / dt_graph:,.0f} codes/s, "
+        f"{dt_graph / args.steps * 1e3:.3f} ms/step")
+
+    if "wide" not in skip:
+        wsets = workloads.mixed_k(4096, 3, 10, seed=12)
+        wcw = sum(proj(len(s)) for s in wsets)
+        wdt, wnk, wkms, _, _ = bench_md(ctx, wsets, 3, 1, None)
+        wk = wkms / max(wnk, 1)
+        result["mixed_k3_10"] = {
+            "value": round(4096 * 3 / wdt, 1), "unit": "codes/s",
+            "workload": "4096 codes, k uniform in 3..10 (default_rng(12)); no oracle fixture at "
+                        "this size (checked by the k <= 10 parity tests instead)",
+            "projective_codewords": wcw, "kernel_ms": round(wk, 3),
+            "achieved_TOPs": round(OPS_PER_CODEWORD * wcw / (wk * 1e-3) / 1e12, 3),
+            "valu_issue_frac": round(VALU_PER_CODEWORD * wcw / (wk * 1e-3) / 1e12 / peak, 4),
+            "note": "achieved counts 11 int32 ops per codeword; the walk issues 10.44 VALU "
+                    "instructions per codeword, so achieved/peak can exceed 1 while the issue "
+                    "fraction (valu_issue_frac) cannot",
+        }
+        log(f"[bench] mixed k=3..10: {4096 * 3 / wdt:,.0f} codes/s, md_main {wk:.2f} ms, "
+            f"{OPS_PER_CODEWORD * wcw / (wk * 1e-3) / 1e12:.2f} TOP/s")
 
     if "plain" not in skip:
         pdt = bench_plain(ctx, sets, expect.tolist() if expect is not None else None)

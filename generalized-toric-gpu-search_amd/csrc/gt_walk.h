@@ -25,22 +25,6 @@ struct ConstRows {
   }
 };
 
-// Rows written by another wave of the same (persistent) kernel: relaxed
-// agent-scope atomic loads, which are vector loads that bypass the
-// non-coherent scalar cache.  Ordering comes from the acquire that published
-// the request.
-struct CoherentRows {
-  const uint32_t* R;
-  __device__ __forceinline__ RowV operator()(uint32_t r) const {
-    const uint32_t* p = R + (size_t)r * kRowWords;
-    RowV v;
-#pragma unroll
-    for (int i = 0; i < 6; ++i)
-      v.w[i] = __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return v;
-  }
-};
-
 __device__ __forceinline__ void xor_row(RowV& v, const RowV& r) {
 #pragma unroll
   for (int i = 0; i < 6; ++i) v.w[i] ^= r.w[i];
