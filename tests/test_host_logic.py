@@ -50,3 +50,48 @@ def test_chunk_plan_covers_projective_space():
                     n += 1
             assert total == (8 ** k - 1) // 7, (k, L)
             assert n == code_chunks(k, L)
+
+
+def test_mask_bit_conversions_round_trip():
+    from gtoric.evolve import bits_to_masks, masks_to_bits, random_masks
+    rng = np.random.default_rng(1)
+    for k in (1, 5, 24, 49):
+        m = random_masks(rng, 300, k)
+        b = masks_to_bits(m)
+        assert b.shape == (300, 49) and np.all(b.sum(1) == k)
+        assert np.array_equal(bits_to_masks(b), m)
+        i = 17
+        assert [p for p in range(49) if b[i, p]] == [p for p in range(49) if (int(m[i]) >> p) & 1]
+
+
+def test_mutation_at_the_size_extremes():
+    from gtoric.evolve import mutate, random_masks
+    rng = np.random.default_rng(2)
+    for k in (2, 47):
+        par = random_masks(rng, 400, k)
+        ch = mutate(rng, par)
+        assert all(bin(int(m)).count("1") == k for m in ch)
+        diff = {bin(int(a) ^ int(b)).count("1") for a, b in zip(par, ch)}
+        assert diff == {2, 4}
+
+
+def test_evolve_result_discovery_counts():
+    from gtoric.evolve import EvolveResult
+    r = EvolveResult(k=6, best_d=36, best_set=[], generations=3, evaluated=6, wall_s=1.0,
+                     eval_s=0.5, history=[(0.1, 34), (0.4, 36), (0.9, 36)],
+                     classes=np.array([5, 6, 7, 8, 9, 10], dtype=np.uint64),
+                     dists=np.array([36, 35, 36, 30, 35, 34], dtype=np.int32))
+    assert r.hits(36) == 2 and r.hits(35) == 4 and r.hits(37) == 0
+    assert r.time_to_best_s == 0.4 and r.codes_per_s == 6.0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    from pathlib import Path
+    root = Path(__file__).resolve().parents[1]
+    p = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 2
+    line = json.loads(p.stdout.strip().splitlines()[-1])
+    assert line["n_gpus"] == 2 and "one GPU" in line["error"]
