@@ -262,6 +262,31 @@ def bench_sampled(ctx, ks=(12, 16), n: int = 2048, samples: int = 200_000, seed:
     return rows
 
 
+def bench_bfs(ctx, max_k: int = 10):
+    """Canonical BFS (champion_search_canonical.py:386-604 with every level on
+    the device): level k-1 classes -> children -> canonical forms -> device
+    hash-set dedup -> exact distances, k = 2..max_k, nothing pruned.  The class
+    count of every level must equal the Burnside count."""
+    from gtoric.bfs import HashSet, bfs_level
+    seen = HashSet(ctx, 24)
+    parents = np.array([1], dtype=np.uint64)          # k = 1: the class of {0}
+    rows = {}
+    for k in range(2, max_k + 1):
+        seen.clear()
+        t0 = time.perf_counter()
+        masks, mz = bfs_level(ctx, seen, parents, k - 1, 48)
+        dt = time.perf_counter() - t0
+        if len(masks) != BURNSIDE_CLASSES[k]:
+            raise SystemExit(f"BFS level {k}: {len(masks)} classes, Burnside says {BURNSIDE_CLASSES[k]}")
+        rows[str(k)] = {"children": int(len(parents) * (49 - (k - 1))), "classes": int(len(masks)),
+                        "seconds": round(dt, 4), "classes_per_s": round(len(masks) / dt, 1),
+                        "best_d": int(49 - mz.min())}
+        log(f"[bench] bfs k={k}: {len(masks):,} classes in {dt:.3f}s, best d {49 - mz.min()}")
+        parents = masks
+    seen.close()
+    return rows
+
+
 def copy_calibration(ctx, nbytes: int = 1 << 30, reps: int = 5):
     """A plain device copy kernel of known size, for scaling FETCH_SIZE/WRITE_SIZE."""
     a = ctx.array(nbytes // 4, np.uint32)
@@ -309,7 +334,7 @@ def main() -> None:
     ap.add_argument("--canon-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline")
     ap.add_argument("--skip", default="",
-                    help="comma list of: wide,plain,canon,sampled,cpu,serve,evolve,tune")
+                    help="comma list of: wide,plain,canon,sampled,bfs,cpu,serve,evolve,tune")
     ap.add_argument("--serve-requests", type=int, default=20_000)
     ap.add_argument("--evolve-k", default="6,7,8,9,10")
     ap.add_argument("--evolve-pop", type=int, default=2048)
@@ -440,6 +465,9 @@ def main() -> None:
 
     if "sampled" not in skip:
         result["sampled"] = bench_sampled(ctx)
+
+    if "bfs" not in skip:
+        result["canonical_bfs"] = bench_bfs(ctx)
 
     if "serve" not in skip:
         result["serving"] = bench_serving(ctx, args.serve_requests)

@@ -259,17 +259,19 @@ __device__ void worker(ServeDev* d, ServeHost* h, uint32_t lane, uint32_t* lrows
     st_walk += t3 - t2;
     uint64_t next = 0;
     if (lane == 0) {
-      // Fold the minimum, wait until that atomic has completed (s_waitcnt 0:
-      // every counter of this wave at zero; an agent-scope release would add
-      // a write-back of the whole L2), then count the chunk.  Every wave's
-      // minimum has therefore been performed at the device coherence point
-      // before its count is issued, and the wave whose count completes the
-      // request reads the final minimum there.
+      // Fold the minimum with a returning atomic, then count the chunk with
+      // an operand computed from the returned value (1 + (prev >> 31) == 1:
+      // weights are at most 64), and an s_waitcnt 0 between them: the count
+      // cannot issue before the minimum has been performed at the device
+      // coherence point (an agent-scope release would instead write back the
+      // whole L2).  The wave whose count completes the request therefore reads
+      // every wave's minimum there.
       uint32_t* st = reinterpret_cast<uint32_t*>(state);
-      if (!skip) __hip_atomic_fetch_min(&st[0], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t prev = 0u;
+      if (!skip) prev = __hip_atomic_fetch_min(&st[0], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __builtin_amdgcn_s_waitcnt(0);
-      const uint32_t done =
-          __hip_atomic_fetch_add(&st[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+      const uint32_t done = __hip_atomic_fetch_add(&st[1], 1u + (prev >> 31), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT) + 1u;
       if (done == nch) {
         const int bw = (int)ld_rlx(&st[0]);
         __hip_atomic_store(&h->results[slot].word,
