@@ -14,9 +14,20 @@
 // minimum over all 98,784 affine images is a different function.  The number
 // of (A, t) pairs attaining the minimum is |Stab_AGL2(S)|.
 //
-// Layout: one lane per set, A uniform across the wave (2016 iterations), the
-// GL2 permutation table staged in LDS in tiles.  Sets are first bucketed by
-// size, largest first, so that every wave runs a loop specialised for its k.
+// Search (one wavefront per set).  Write A = (r1; r2): r1 = (a, b) gives the
+// row u = r1.p of an image point, r2 = (c, d) its column v.  The highest bit of
+// an image is in its top occupied row, and after translating by anchor t the
+// rows are u_i - u_t (mod 7), so the top row is 6 - (the run of empty rows
+// just below row u_t), which depends on r1 and t only.  Phase 1: lane r1
+// (48 nonzero first rows) finds the best top row its r1 allows (6 - the
+// longest cyclic run of empty rows of its row-occupancy set); the wave minimum
+// U* is the top row of the canonical image.  Phase 2: for every first row
+// reaching U* (typically a handful of the 48), lane r2 builds the image mask
+// for A = (r1; r2) (42 valid second rows) and translates it by every anchor
+// whose row starts the longest empty run; those are the only (A, t) pairs
+// whose image can be minimal, so both the minimum and the count of pairs that
+// attain it (|Stab|) are exact.  A wave reduction combines the lanes.  Sets are
+// bucketed by size, largest first, and walked by the waves with a fixed stride.
 #include <algorithm>
 
 #include "gt_launch.h"
@@ -24,8 +35,10 @@
 namespace gt {
 
 constexpr int kPermStride = 52;        // 49 bytes per matrix, padded to 13 dwords
-constexpr int kPermTile = 504;         // matrices per LDS tile (4 tiles)
 constexpr int kCanonBlock = 256;
+constexpr int kCanonMaxBlocks = 2048;  // persistent grid: 8 waves per SIMD on 256 CUs
+constexpr int kRows = 48;              // nonzero vectors (a, b) of F7^2, index 7a + b - 1
+constexpr int kRowStride = 52;
 
 int canon_perm_bytes() { return 2016 * kPermStride; }
 
@@ -76,34 +89,106 @@ __host__ __device__ __forceinline__ Anchor make_anchor(uint32_t u, uint32_t v) {
   return e;
 }
 
+// Tables of one workgroup (4.5 KB of LDS).
+struct CanonTables {
+  uint8_t rowval[kRows * kRowStride];  // rowval[r][p] = (a x + b y) mod 7, r = 7a + b - 1, p = 7x + y
+  uint64_t par[kRows];                 // bit r2 set when vector r2 is parallel to vector r
+  uint8_t gap[128 * 8];                // gap[R][u]: empty rows just below row u (cyclic) in the
+                                       // row set R; gap[R][7]: the longest such run over u in R
+  Anchor anc[56];                      // translation by byte-row position 8u + v
+};
+
+__device__ void build_canon_tables(CanonTables& T) {
+  for (int e = threadIdx.x; e < kRows * kRowStride; e += blockDim.x) {
+    const int r = e / kRowStride, p = e % kRowStride;
+    const int a = (r + 1) / 7, b = (r + 1) % 7;
+    T.rowval[e] = (uint8_t)((p < kNpts) ? (a * (p / 7) + b * (p % 7)) % 7 : 0);
+  }
+  for (int r = threadIdx.x; r < kRows; r += blockDim.x) {
+    const int a = (r + 1) / 7, b = (r + 1) % 7;
+    uint64_t m = 0;
+    for (int r2 = 0; r2 < kRows; ++r2) {
+      const int c = (r2 + 1) / 7, d = (r2 + 1) % 7;
+      if ((a * d - b * c + 49) % 7 == 0) m |= 1ull << r2;
+    }
+    T.par[r] = m;
+  }
+  for (int R = threadIdx.x; R < 128; R += blockDim.x) {
+    int longest = 0;
+    for (int u = 0; u < 7; ++u) {
+      int g = 0;
+      while (g < 6 && !((R >> ((u + 6 - g) % 7)) & 1)) ++g;
+      T.gap[R * 8 + u] = (uint8_t)g;
+      if ((R >> u) & 1) longest = max(longest, g);
+    }
+    T.gap[R * 8 + 7] = (uint8_t)longest;
+  }
+  if (threadIdx.x < 56) {
+    const uint32_t u = threadIdx.x >> 3, v = threadIdx.x & 7u;
+    T.anc[threadIdx.x] = (u < 7 && v < 7) ? make_anchor(u, v) : Anchor{};
+  }
+}
+
+// Minimum byte-row image and the number of (A, t) attaining it, for the set
+// with points p[0..K-1] (wave-uniform); every lane returns the same pair.
 template <int K>
-__device__ __forceinline__ void canon_tile(const uint8_t* __restrict__ lperm, int na,
-                                           const uint32_t* pts, const Anchor* lanc,
-                                           uint64_t& best, uint32_t& cnt) {
-  for (int a = 0; a < na; ++a) {
-    const uint8_t* pa = lperm + a * kPermStride;
-    uint32_t q[K];
-    uint64_t MA = 0;
+__device__ __forceinline__ void canon_set(const CanonTables& T, const uint32_t* p, uint32_t lane,
+                                          uint64_t& best_out, uint32_t& cnt_out) {
+  // Phase 1: lane r1 < 48 finds the best top row its first row allows.
+  uint32_t mytop = 7u;
+  if (lane < (uint32_t)kRows) {
+    uint32_t R = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) R |= 1u << T.rowval[lane * kRowStride + p[i]];
+    mytop = 6u - T.gap[R * 8 + 7];
+  }
+  uint32_t top = mytop;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) top = min(top, (uint32_t)__shfl_xor((int)top, off, kWave));
+  uint64_t cand = __ballot(mytop == top);
+  // Phase 2: lane r2 < 48, for every first row r1 reaching the top row.
+  uint64_t best = ~0ull;
+  uint32_t cnt = 0;
+  while (cand != 0) {
+    const int r1 = __builtin_ctzll(cand);
+    cand &= cand - 1;
+    uint32_t w[K];
+    uint32_t R = 0;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
-      q[i] = pa[pts[i]];
-      MA |= 1ull << q[i];
+      w[i] = T.rowval[r1 * kRowStride + p[i]];
+      R |= 1u << w[i];
     }
+    const uint32_t longest = T.gap[R * 8 + 7];
+    if (lane < (uint32_t)kRows && !((T.par[r1] >> lane) & 1ull)) {
+      uint32_t z[K];
+      uint64_t MA = 0;
 #pragma unroll
-    for (int t = 0; t < K; ++t) {
-      const uint64_t img = translate(MA, lanc[q[t]]);
-      // A lane sees a new minimum or a tie in a few dozen of its 2016 k
-      // images, so the update sits behind a wave-uniform test.
-      if (__any(img <= best)) {
-        // Empty asm: keeps this a taken-rarely branch instead of selects that
-        // would run the update on every image.
-        asm volatile("" ::: "memory");
-        const bool lt = img < best;
-        cnt = lt ? 1u : cnt + (uint32_t)(img == best);
-        best = lt ? img : best;
+      for (int i = 0; i < K; ++i) {
+        z[i] = T.rowval[lane * kRowStride + p[i]];
+        MA |= 1ull << (8u * w[i] + z[i]);
+      }
+#pragma unroll
+      for (int t = 0; t < K; ++t) {
+        if (T.gap[R * 8 + w[t]] == longest) {
+          const uint64_t img = translate(MA, T.anc[8u * w[t] + z[t]]);
+          const bool lt = img < best;
+          cnt = lt ? 1u : cnt + (uint32_t)(img == best);
+          best = lt ? img : best;
+        }
       }
     }
   }
+  // Combine the lanes: minimum, and the counts of the lanes holding it.
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t ob = __shfl_xor(best, off, kWave);
+    const uint32_t oc = (uint32_t)__shfl_xor((int)cnt, off, kWave);
+    cnt = (ob < best) ? oc : cnt + ((ob == best) ? oc : 0u);
+    best = (ob < best) ? ob : best;
+  }
+  best_out = best;
+  cnt_out = cnt;
 }
 
 // Byte-row layout -> 49-bit mask.
@@ -193,78 +278,61 @@ __global__ void canon_fill_kernel(uint32_t* order, uint32_t len, uint32_t* hist)
 
 __global__ __launch_bounds__(kCanonBlock) void canon_kernel(
     const uint64_t* __restrict__ masks, const uint32_t* __restrict__ order, uint32_t slots,
-    const uint8_t* __restrict__ gperm, uint64_t* __restrict__ key_lo,
-    uint64_t* __restrict__ key_hi, uint64_t* __restrict__ canon, int32_t* __restrict__ stab) {
-  __shared__ uint32_t lperm_words[kPermTile * kPermStride / 4];
-  __shared__ Anchor lanc[56];                // indexed by byte-row position 8u + v
-  uint8_t* lperm = reinterpret_cast<uint8_t*>(lperm_words);
-  if (threadIdx.x < 56) {
-    const uint32_t u = threadIdx.x >> 3, v = threadIdx.x & 7u;
-    lanc[threadIdx.x] = (u < 7 && v < 7) ? make_anchor(u, v) : Anchor{};
-  }
-  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t idx = (slot < slots) ? order[slot] : 0xFFFFFFFFu;
-  const bool active = idx != 0xFFFFFFFFu;
-  const uint64_t mask = active ? masks[idx] : 0ull;
-  // Buckets are wave aligned: the first active lane's size is the wave's size.
-  int k = active ? __popcll(mask) : 0;
-  const uint64_t act = __ballot(active);
-  if (act != 0) {
-    const int src = __builtin_ctzll(act);
-    k = __shfl(k, src, kWave);
-  }
-  const int kw = __builtin_amdgcn_readfirstlane(k);
-  uint32_t pts[kMaxKCanon];
-  {
-    uint64_t m = mask;
+    uint64_t* __restrict__ key_lo, uint64_t* __restrict__ key_hi, uint64_t* __restrict__ canon,
+    int32_t* __restrict__ stab) {
+  __shared__ CanonTables T;
+  build_canon_tables(T);
+  __syncthreads();
+  const uint32_t lane = __lane_id();
+  const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t j = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; j < slots;
+       j += nwaves) {
+    const uint32_t idx = __builtin_amdgcn_readfirstlane(order[j]);
+    if (idx == 0xFFFFFFFFu) continue;
+    const uint64_t mask = masks[idx];
+    const int k = __popcll(mask);
+    uint32_t pts[kMaxKCanon];
+    {
+      uint64_t m = mask;
 #pragma unroll
-    for (int i = 0; i < kMaxKCanon; ++i) {
-      const uint32_t p = (uint32_t)__builtin_ctzll(m | (1ull << 63));
-      pts[i] = (p > 48) ? 0u : p;
-      m &= m - 1;
+      for (int i = 0; i < kMaxKCanon; ++i) {
+        const uint32_t q = (uint32_t)__builtin_ctzll(m | (1ull << 63));
+        pts[i] = (q > 48) ? 0u : q;
+        m &= m - 1;
+      }
     }
-  }
-  uint64_t best = ~0ull;
-  uint32_t cnt = 0;
-  // Every thread of the block loads its share of every tile, whatever its
-  // wave's k (or none), before the wave runs the loop specialised for k.
-  for (int t0 = 0; t0 < 2016; t0 += kPermTile) {
-    __syncthreads();
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(gperm + t0 * kPermStride);
-    for (int e = threadIdx.x; e < kPermTile * kPermStride / 4; e += blockDim.x)
-      lperm_words[e] = src[e];
-    __syncthreads();
-    if (!active) continue;
-    switch (kw) {
+    uint64_t best = ~0ull;
+    uint32_t cnt = 0;
+    switch (k) {
 #define GT_CANON_CASE(KK) \
-  case KK: canon_tile<KK>(lperm, kPermTile, pts, lanc, best, cnt); break;
+  case KK: canon_set<KK>(T, pts, lane, best, cnt); break;
       GT_CANON_CASE(1) GT_CANON_CASE(2) GT_CANON_CASE(3) GT_CANON_CASE(4)
       GT_CANON_CASE(5) GT_CANON_CASE(6) GT_CANON_CASE(7) GT_CANON_CASE(8)
       GT_CANON_CASE(9) GT_CANON_CASE(10) GT_CANON_CASE(11) GT_CANON_CASE(12)
       GT_CANON_CASE(13) GT_CANON_CASE(14) GT_CANON_CASE(15) GT_CANON_CASE(16)
 #undef GT_CANON_CASE
-      default: break;
+      default: continue;   // sizes outside 1..16 never reach the order array
     }
+    if (lane != 0) continue;
+    best = rows_to_mask(best);
+    // Sorted positions of the canonical image, packed as kernel_canon.py:84-104:
+    // k <= 10 packs positions 0..k-1, k >= 11 skips position 0 (always 0).
+    const int pack_from = (k <= 10) ? 0 : 1;
+    uint64_t lo = 0, hi = 0, m = best;
+    for (int i = 0; i < k; ++i) {
+      const uint64_t v = (uint64_t)__builtin_ctzll(m);
+      m &= m - 1;
+      if (i < pack_from) continue;
+      const int off = 6 * (i - pack_from);
+      if (off + 6 <= 64) lo |= v << off;
+      else if (off < 64) { lo |= v << off; hi |= v >> (64 - off); }
+      else hi |= v << (off - 64);
+    }
+    key_lo[idx] = lo;
+    key_hi[idx] = hi;
+    canon[idx] = best;
+    stab[idx] = (int32_t)cnt;
   }
-  if (!active) return;
-  best = rows_to_mask(best);
-  // Sorted positions of the canonical image, packed as kernel_canon.py:84-104:
-  // k <= 10 packs positions 0..k-1, k >= 11 skips position 0 (always 0).
-  const int pack_from = (k <= 10) ? 0 : 1;
-  uint64_t lo = 0, hi = 0, m = best;
-  for (int i = 0; i < k; ++i) {
-    const uint64_t v = (uint64_t)__builtin_ctzll(m);
-    m &= m - 1;
-    if (i < pack_from) continue;
-    const int off = 6 * (i - pack_from);
-    if (off + 6 <= 64) lo |= v << off;
-    else if (off < 64) { lo |= v << off; hi |= v >> (64 - off); }
-    else hi |= v << (off - 64);
-  }
-  key_lo[idx] = lo;
-  key_hi[idx] = hi;
-  canon[idx] = best;
-  stab[idx] = (int32_t)cnt;
 }
 
 // (n, k) sorted index rows -> masks; a repeated or out-of-range index is flagged.
@@ -305,9 +373,11 @@ hipError_t launch_canon(const uint64_t* d_masks, int32_t n, const uint8_t* d_per
   hipLaunchKernelGGL(canon_scan_kernel, dim3(1), dim3(64), 0, s, d_hist);
   hipLaunchKernelGGL(canon_scatter_kernel, dim3(sort_grid), dim3(kSortBlock), 0, s, d_masks, n,
                      d_hist, d_order);
-  hipLaunchKernelGGL(canon_kernel, dim3((slots + kCanonBlock - 1) / kCanonBlock),
-                     dim3(kCanonBlock), 0, s, d_masks, d_order, slots, d_perm, d_key_lo,
-                     d_key_hi, d_canon, d_stab);
+  const uint32_t waves = (uint32_t)n;
+  const int grid = (int)std::min<uint32_t>(kCanonMaxBlocks,
+                                           (waves + kCanonBlock / kWave - 1) / (kCanonBlock / kWave));
+  hipLaunchKernelGGL(canon_kernel, dim3(grid), dim3(kCanonBlock), 0, s, d_masks, d_order, slots,
+                     d_key_lo, d_key_hi, d_canon, d_stab);
   return hipGetLastError();
 }
 
