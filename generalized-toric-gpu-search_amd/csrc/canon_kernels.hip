@@ -131,47 +131,52 @@ __device__ void build_canon_tables(CanonTables& T) {
 
 // Minimum byte-row image and the number of (A, t) attaining it, for the set
 // with points p[0..K-1] (wave-uniform); every lane returns the same pair.
+// par_mine = T.par[lane] (the relation is symmetric: bit r1 says whether this
+// lane's vector is parallel to r1).
 template <int K>
 __device__ __forceinline__ void canon_set(const CanonTables& T, const uint32_t* p, uint32_t lane,
-                                          uint64_t& best_out, uint32_t& cnt_out) {
-  // Phase 1: lane r1 < 48 finds the best top row its first row allows.
-  uint32_t mytop = 7u;
-  if (lane < (uint32_t)kRows) {
-    uint32_t R = 0;
+                                          uint64_t par_mine, uint64_t& best_out,
+                                          uint32_t& cnt_out) {
+  // Phase 1, lane r1 < 48: row values w_i = r1.p_i, their row set, the best top
+  // row r1 allows, and the anchors whose row starts the longest empty run.
+  // The same w_i are the column values z_i when the lane stands for r2.
+  const bool row_lane = lane < (uint32_t)kRows;
+  const uint32_t base = (row_lane ? lane : 0u) * kRowStride;
+  uint32_t wl[K];
+  uint32_t R = 0;
 #pragma unroll
-    for (int i = 0; i < K; ++i) R |= 1u << T.rowval[lane * kRowStride + p[i]];
-    mytop = 6u - T.gap[R * 8 + 7];
+  for (int i = 0; i < K; ++i) {
+    wl[i] = T.rowval[base + p[i]];
+    R |= 1u << wl[i];
   }
+  const uint32_t longest = T.gap[R * 8 + 7];
+  uint32_t amask = 0;
+#pragma unroll
+  for (int t = 0; t < K; ++t) amask |= (uint32_t)(T.gap[R * 8 + wl[t]] == longest) << t;
+  const uint32_t mytop = row_lane ? 6u - longest : 7u;
   uint32_t top = mytop;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) top = min(top, (uint32_t)__shfl_xor((int)top, off, kWave));
   uint64_t cand = __ballot(mytop == top);
-  // Phase 2: lane r2 < 48, for every first row r1 reaching the top row.
+  // Phase 2, lane r2 < 48 (not parallel to r1), for every r1 reaching the top
+  // row: image of A = (r1; r2), translated by each candidate anchor.
   uint64_t best = ~0ull;
   uint32_t cnt = 0;
   while (cand != 0) {
     const int r1 = __builtin_ctzll(cand);
     cand &= cand - 1;
+    const uint32_t am = __builtin_amdgcn_readlane(amask, r1);
     uint32_t w[K];
-    uint32_t R = 0;
 #pragma unroll
-    for (int i = 0; i < K; ++i) {
-      w[i] = T.rowval[r1 * kRowStride + p[i]];
-      R |= 1u << w[i];
-    }
-    const uint32_t longest = T.gap[R * 8 + 7];
-    if (lane < (uint32_t)kRows && !((T.par[r1] >> lane) & 1ull)) {
-      uint32_t z[K];
+    for (int i = 0; i < K; ++i) w[i] = __builtin_amdgcn_readlane(wl[i], r1);
+    if (row_lane && !((par_mine >> r1) & 1ull)) {
       uint64_t MA = 0;
 #pragma unroll
-      for (int i = 0; i < K; ++i) {
-        z[i] = T.rowval[lane * kRowStride + p[i]];
-        MA |= 1ull << (8u * w[i] + z[i]);
-      }
+      for (int i = 0; i < K; ++i) MA |= 1ull << (8u * w[i] + wl[i]);
 #pragma unroll
       for (int t = 0; t < K; ++t) {
-        if (T.gap[R * 8 + w[t]] == longest) {
-          const uint64_t img = translate(MA, T.anc[8u * w[t] + z[t]]);
+        if ((am >> t) & 1u) {
+          const uint64_t img = translate(MA, T.anc[8u * w[t] + wl[t]]);
           const bool lt = img < best;
           cnt = lt ? 1u : cnt + (uint32_t)(img == best);
           best = lt ? img : best;
@@ -284,6 +289,7 @@ __global__ __launch_bounds__(kCanonBlock) void canon_kernel(
   build_canon_tables(T);
   __syncthreads();
   const uint32_t lane = __lane_id();
+  const uint64_t par_mine = T.par[lane < (uint32_t)kRows ? lane : 0u];
   const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
   for (uint32_t j = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; j < slots;
        j += nwaves) {
@@ -305,7 +311,7 @@ __global__ __launch_bounds__(kCanonBlock) void canon_kernel(
     uint32_t cnt = 0;
     switch (k) {
 #define GT_CANON_CASE(KK) \
-  case KK: canon_set<KK>(T, pts, lane, best, cnt); break;
+  case KK: canon_set<KK>(T, pts, lane, par_mine, best, cnt); break;
       GT_CANON_CASE(1) GT_CANON_CASE(2) GT_CANON_CASE(3) GT_CANON_CASE(4)
       GT_CANON_CASE(5) GT_CANON_CASE(6) GT_CANON_CASE(7) GT_CANON_CASE(8)
       GT_CANON_CASE(9) GT_CANON_CASE(10) GT_CANON_CASE(11) GT_CANON_CASE(12)
