@@ -34,20 +34,16 @@
 
 namespace gt {
 
-constexpr int kPermStride = 52;        // 49 bytes per matrix, padded to 13 dwords
 constexpr int kCanonBlock = 256;
 constexpr int kCanonMaxBlocks = 2048;  // persistent grid: 8 waves per SIMD on 256 CUs
 constexpr int kRows = 48;              // nonzero vectors (a, b) of F7^2, index 7a + b - 1
 constexpr int kRowStride = 52;
 
-int canon_perm_bytes() { return 2016 * kPermStride; }
-
 // Byte-row layout.  Inside the kernel a subset of the 7x7 torus is a 56-bit
 // value with row u in byte u and column v in bit v of that byte (position
 // 8u + v; bit 7 of every byte and byte 7 stay 0).  Integer order on this
 // layout equals integer order on the 49-bit mask (position 7u + v), because
-// both orders compare row 6 first and, inside a row, column 6 first; the
-// permutation table holds image positions in this layout.
+// both orders compare row 6 first and, inside a row, column 6 first.
 //
 // Translating by (-u, -v) is then a byte rotation of the 7 rows (two
 // v_perm_b32, whose selectors are the row numbers (r + u) mod 7 and 12 for a
@@ -365,7 +361,7 @@ hipError_t launch_sets_to_masks(const int32_t* d_sets, int32_t n, int32_t k, uin
 }
 
 // d_order must hold n + 16 * 64 entries; d_hist 35 entries.
-hipError_t launch_canon(const uint64_t* d_masks, int32_t n, const uint8_t* d_perm,
+hipError_t launch_canon(const uint64_t* d_masks, int32_t n,
                         uint32_t* d_order, uint32_t* d_hist, uint64_t* d_key_lo,
                         uint64_t* d_key_hi, uint64_t* d_canon, int32_t* d_stab,
                         uint32_t* d_ctrl, hipStream_t s) {

@@ -117,27 +117,6 @@ void build_mplanes(uint32_t out[kNpts * kRowWords]) {
   }
 }
 
-// GL2(F7) as permutations of the 49 points, 52-byte stride, matrices
-// (a, b, c, d) with a*d - b*c != 0 mod 7 in lexicographic order; point
-// (x, y) = divmod(p, 7) maps to (u, v) = ((a x + b y) % 7, (c x + d y) % 7),
-// stored as the byte-row position 8 u + v that the canonical-form kernel uses.
-std::vector<uint8_t> build_gl2_perm() {
-  std::vector<uint8_t> t(2016 * 52, 0);
-  int n = 0;
-  for (int a = 0; a < 7; ++a)
-    for (int b = 0; b < 7; ++b)
-      for (int c = 0; c < 7; ++c)
-        for (int d = 0; d < 7; ++d) {
-          if (((a * d - b * c) % 7 + 7) % 7 == 0) continue;
-          for (int p = 0; p < kNpts; ++p) {
-            const int x = p / 7, y = p % 7;
-            t[n * 52 + p] = (uint8_t)(((a * x + b * y) % 7) * 8 + (c * x + d * y) % 7);
-          }
-          ++n;
-        }
-  return t;
-}
-
 // Default Gray walk bits per k: big enough that per-chunk set-up is a few
 // percent of the walk, small enough for a balanced tail.
 int default_L(int k) {
@@ -169,7 +148,7 @@ struct gt_ctx {
   MdParams params{};
   bool params_dirty = true;
 
-  DevBuf mplanes, params_buf, perm, ctrl;
+  DevBuf mplanes, params_buf, ctrl;
   DevBuf idx, offs, tmax, rows, plan, best, out, zmax, stopf;  // min distance
   DevBuf sbuf;                                                // single code
   DevBuf masks, order, hist, klo, khi, canon, stab;            // canonical
@@ -181,7 +160,7 @@ struct gt_ctx {
   hipStream_t capture_stream = nullptr;
 
   std::vector<DevBuf*> all_bufs() {
-    return {&mplanes, &params_buf, &perm, &ctrl, &idx, &offs, &tmax, &rows, &plan, &best,
+    return {&mplanes, &params_buf, &ctrl, &idx, &offs, &tmax, &rows, &plan, &best,
             &out, &zmax, &stopf, &sbuf, &masks, &order, &hist, &klo, &khi, &canon, &stab,
             &children, &bflag, &bsum, &bidx, &bcnt, &bmasks, &brows, &boffs};
   }
@@ -309,13 +288,11 @@ int gt_open(int32_t device, gt_ctx** out) {
   c->md_grid[1] = c->cu_count * md_main_blocks_per_cu(true);
   uint32_t mp[kNpts * kRowWords];
   build_mplanes(mp);
-  std::vector<uint8_t> perm = build_gl2_perm();
   if (c->mplanes.ensure(sizeof(mp)) || c->params_buf.ensure(sizeof(MdParams)) ||
-      c->perm.ensure(perm.size()) || c->ctrl.ensure(kCtrlWords * 4))
+      c->ctrl.ensure(kCtrlWords * 4))
     return bail(-1);
   uint32_t zeros[kCtrlWords] = {0};
   if (hipMemcpy(c->mplanes.p, mp, sizeof(mp), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->perm.p, perm.data(), perm.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->ctrl.p, zeros, sizeof(zeros), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->params_buf.p, &c->params, sizeof(MdParams), hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail("gt_open: upload of constant tables failed"));
@@ -476,7 +453,7 @@ static int canon_enqueue(gt_ctx* c, const uint64_t* d_masks, int32_t n, uint64_t
   GT_OK(c->hist.ensure(64 * 4));
   hipEvent_t e0 = nullptr;
   GT_OK(c->prof_begin("canon", s, &e0));
-  GT_HIP(launch_canon(d_masks, n, c->perm.as<uint8_t>(), c->order.as<uint32_t>(),
+  GT_HIP(launch_canon(d_masks, n, c->order.as<uint32_t>(),
                       c->hist.as<uint32_t>(), d_lo, d_hi, d_canon, d_stab, c->ctrl.as<uint32_t>(), s));
   GT_OK(c->prof_end("canon", s, e0));
   return 0;

@@ -39,7 +39,7 @@ hipError_t launch_md_sampled(const uint32_t* d_rows, const int32_t* d_offsets, i
 int md_main_blocks_per_cu(bool thresholded);
 
 // ---- canonical form ------------------------------------------------------
-hipError_t launch_canon(const uint64_t* d_masks, int32_t n, const uint8_t* d_perm,
+hipError_t launch_canon(const uint64_t* d_masks, int32_t n,
                         uint32_t* d_order, uint32_t* d_hist, uint64_t* d_key_lo,
                         uint64_t* d_key_hi, uint64_t* d_canon, int32_t* d_stab,
                         uint32_t* d_ctrl, hipStream_t s);
@@ -49,6 +49,5 @@ hipError_t launch_copy16(void* dst, const void* src, int64_t bytes, hipStream_t 
 
 hipError_t launch_sets_to_masks(const int32_t* d_sets, int32_t n, int32_t k, uint64_t* d_masks,
                                 uint32_t* d_ctrl, hipStream_t s);
-int canon_perm_bytes();
 
 }  // namespace gt
