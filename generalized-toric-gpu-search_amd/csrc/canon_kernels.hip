@@ -19,14 +19,17 @@
 // an image is in its top occupied row, and after translating by anchor t the
 // rows are u_i - u_t (mod 7), so the top row is 6 - (the run of empty rows
 // just below row u_t), which depends on r1 and t only.  Phase 1: lane r1
-// (48 nonzero first rows) finds the best top row its r1 allows (6 - the
-// longest cyclic run of empty rows of its row-occupancy set); the wave minimum
-// U* is the top row of the canonical image.  Phase 2: for every first row
-// reaching U* (typically a handful of the 48), lane r2 builds the image mask
-// for A = (r1; r2) (42 valid second rows) and translates it by every anchor
-// whose row starts the longest empty run; those are the only (A, t) pairs
-// whose image can be minimal, so both the minimum and the count of pairs that
-// attain it (|Stab|) are exact.  A wave reduction combines the lanes.  Sets are
+// (48 nonzero first rows) reads its row values, their row set R, the longest
+// cyclic run of empty rows below an occupied row of R and the rows starting
+// such a run (two 128-byte tables indexed by R); the first rows with the
+// wave's longest run give the top row of the canonical image, and a ballot
+// collects them (typically a handful of the 48).  Phase 2: for every such r1,
+// lane r2 builds the image for A = (r1; r2) (42 valid second rows) from its
+// own phase-1 values (the columns) and r1's row shifts (read from lane r1 in
+// packed form), and translates it by every anchor whose row starts the longest
+// run; those are the only (A, t) pairs whose image can be minimal, so both the
+// minimum and the count of pairs that attain it (|Stab|) are exact.  Ballots
+// combine the lanes and the key is packed in scalar registers.  Sets are
 // bucketed by size, largest first, and walked by the waves with a fixed stride.
 #include <algorithm>
 
@@ -85,20 +88,33 @@ __host__ __device__ __forceinline__ Anchor make_anchor(uint32_t u, uint32_t v) {
   return e;
 }
 
-// Tables of one workgroup (4.5 KB of LDS).
-struct CanonTables {
+// Tables of one workgroup (4 KB of LDS).  Every table a lane indexes with its
+// own value is read conflict-free: rowval rows are 13 dwords apart (odd), the
+// two R-indexed tables are 32 dwords each, and the 7 anchors one image can
+// use are 7 consecutive 16-byte entries.
+struct alignas(16) CanonTables {
+  Anchor anc[56];                      // translation by byte-row position 8u + v
   uint8_t rowval[kRows * kRowStride];  // rowval[r][p] = (a x + b y) mod 7, r = 7a + b - 1, p = 7x + y
   uint64_t par[kRows];                 // bit r2 set when vector r2 is parallel to vector r
-  uint8_t gap[128 * 8];                // gap[R][u]: empty rows just below row u (cyclic) in the
-                                       // row set R; gap[R][7]: the longest such run over u in R
-  Anchor anc[56];                      // translation by byte-row position 8u + v
+  uint8_t longest[128];                // longest cyclic run of empty rows just below an occupied
+                                       // row of the row set R
+  uint8_t topmask[128];                // occupied rows of R whose run below has that length
 };
+static_assert(sizeof(Anchor) == 16 && offsetof(CanonTables, rowval) % 4 == 0 &&
+                  (kRows * kRowStride) % 4 == 0 && offsetof(CanonTables, longest) % 4 == 0 &&
+                  offsetof(CanonTables, topmask) % 4 == 0,
+              "dword-written tables must be dword aligned");
 
 __device__ void build_canon_tables(CanonTables& T) {
-  for (int e = threadIdx.x; e < kRows * kRowStride; e += blockDim.x) {
-    const int r = e / kRowStride, p = e % kRowStride;
-    const int a = (r + 1) / 7, b = (r + 1) % 7;
-    T.rowval[e] = (uint8_t)((p < kNpts) ? (a * (p / 7) + b * (p % 7)) % 7 : 0);
+  // Whole dwords: byte stores from neighbouring lanes into one dword conflict.
+  for (int q = threadIdx.x; q < kRows * kRowStride / 4; q += blockDim.x) {
+    uint32_t w = 0;
+    for (int c = 0; c < 4; ++c) {
+      const int e = 4 * q + c, r = e / kRowStride, p = e % kRowStride;
+      const int a = (r + 1) / 7, b = (r + 1) % 7;
+      w |= (uint32_t)((p < kNpts) ? (a * (p / 7) + b * (p % 7)) % 7 : 0) << (8 * c);
+    }
+    reinterpret_cast<uint32_t*>(T.rowval)[q] = w;
   }
   for (int r = threadIdx.x; r < kRows; r += blockDim.x) {
     const int a = (r + 1) / 7, b = (r + 1) % 7;
@@ -109,87 +125,30 @@ __device__ void build_canon_tables(CanonTables& T) {
     }
     T.par[r] = m;
   }
-  for (int R = threadIdx.x; R < 128; R += blockDim.x) {
-    int longest = 0;
-    for (int u = 0; u < 7; ++u) {
-      int g = 0;
-      while (g < 6 && !((R >> ((u + 6 - g) % 7)) & 1)) ++g;
-      T.gap[R * 8 + u] = (uint8_t)g;
-      if ((R >> u) & 1) longest = max(longest, g);
+  for (int q = threadIdx.x; q < 32; q += blockDim.x) {
+    uint32_t lw = 0, tw = 0;
+    for (int c = 0; c < 4; ++c) {
+      const int R = 4 * q + c;
+      int gap[7], longest = 0;
+      for (int u = 0; u < 7; ++u) {
+        int g = 0;
+        while (g < 6 && !((R >> ((u + 6 - g) % 7)) & 1)) ++g;
+        gap[u] = g;
+        if ((R >> u) & 1) longest = max(longest, g);
+      }
+      uint32_t top = 0;
+      for (int u = 0; u < 7; ++u)
+        if (((R >> u) & 1) && gap[u] == longest) top |= 1u << u;
+      lw |= (uint32_t)longest << (8 * c);
+      tw |= top << (8 * c);
     }
-    T.gap[R * 8 + 7] = (uint8_t)longest;
+    reinterpret_cast<uint32_t*>(T.longest)[q] = lw;
+    reinterpret_cast<uint32_t*>(T.topmask)[q] = tw;
   }
   if (threadIdx.x < 56) {
     const uint32_t u = threadIdx.x >> 3, v = threadIdx.x & 7u;
     T.anc[threadIdx.x] = (u < 7 && v < 7) ? make_anchor(u, v) : Anchor{};
   }
-}
-
-// Minimum byte-row image and the number of (A, t) attaining it, for the set
-// with points p[0..K-1] (wave-uniform); every lane returns the same pair.
-// par_mine = T.par[lane] (the relation is symmetric: bit r1 says whether this
-// lane's vector is parallel to r1).
-template <int K>
-__device__ __forceinline__ void canon_set(const CanonTables& T, const uint32_t* p, uint32_t lane,
-                                          uint64_t par_mine, uint64_t& best_out,
-                                          uint32_t& cnt_out) {
-  // Phase 1, lane r1 < 48: row values w_i = r1.p_i, their row set, the best top
-  // row r1 allows, and the anchors whose row starts the longest empty run.
-  // The same w_i are the column values z_i when the lane stands for r2.
-  const bool row_lane = lane < (uint32_t)kRows;
-  const uint32_t base = (row_lane ? lane : 0u) * kRowStride;
-  uint32_t wl[K];
-  uint32_t R = 0;
-#pragma unroll
-  for (int i = 0; i < K; ++i) {
-    wl[i] = T.rowval[base + p[i]];
-    R |= 1u << wl[i];
-  }
-  const uint32_t longest = T.gap[R * 8 + 7];
-  uint32_t amask = 0;
-#pragma unroll
-  for (int t = 0; t < K; ++t) amask |= (uint32_t)(T.gap[R * 8 + wl[t]] == longest) << t;
-  const uint32_t mytop = row_lane ? 6u - longest : 7u;
-  uint32_t top = mytop;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) top = min(top, (uint32_t)__shfl_xor((int)top, off, kWave));
-  uint64_t cand = __ballot(mytop == top);
-  // Phase 2, lane r2 < 48 (not parallel to r1), for every r1 reaching the top
-  // row: image of A = (r1; r2), translated by each candidate anchor.
-  uint64_t best = ~0ull;
-  uint32_t cnt = 0;
-  while (cand != 0) {
-    const int r1 = __builtin_ctzll(cand);
-    cand &= cand - 1;
-    const uint32_t am = __builtin_amdgcn_readlane(amask, r1);
-    uint32_t w[K];
-#pragma unroll
-    for (int i = 0; i < K; ++i) w[i] = __builtin_amdgcn_readlane(wl[i], r1);
-    if (row_lane && !((par_mine >> r1) & 1ull)) {
-      uint64_t MA = 0;
-#pragma unroll
-      for (int i = 0; i < K; ++i) MA |= 1ull << (8u * w[i] + wl[i]);
-#pragma unroll
-      for (int t = 0; t < K; ++t) {
-        if ((am >> t) & 1u) {
-          const uint64_t img = translate(MA, T.anc[8u * w[t] + wl[t]]);
-          const bool lt = img < best;
-          cnt = lt ? 1u : cnt + (uint32_t)(img == best);
-          best = lt ? img : best;
-        }
-      }
-    }
-  }
-  // Combine the lanes: minimum, and the counts of the lanes holding it.
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const uint64_t ob = __shfl_xor(best, off, kWave);
-    const uint32_t oc = (uint32_t)__shfl_xor((int)cnt, off, kWave);
-    cnt = (ob < best) ? oc : cnt + ((ob == best) ? oc : 0u);
-    best = (ob < best) ? ob : best;
-  }
-  best_out = best;
-  cnt_out = cnt;
 }
 
 // Byte-row layout -> 49-bit mask.
@@ -198,6 +157,142 @@ __device__ __forceinline__ uint64_t rows_to_mask(uint64_t x) {
 #pragma unroll
   for (int u = 0; u < 7; ++u) m |= ((x >> (8 * u)) & 0x7Full) << (7 * u);
   return m;
+}
+
+// v_lshl_or_b32 with the shifted operand in a scalar register: (s << v) | acc.
+// Written out because the compiler otherwise pairs the shifts into v_or3_b32,
+// one more instruction per image point.
+__device__ __forceinline__ uint32_t lshl_or_s(uint32_t s, uint32_t v, uint32_t acc) {
+  uint32_t d;
+  asm("v_lshl_or_b32 %0, %1, %2, %3" : "=v"(d) : "s"(s), "v"(v), "v"(acc));
+  return d;
+}
+
+struct CanonOut {
+  uint64_t key_lo, key_hi, canon;
+  uint32_t stab;
+};
+
+// Canonical key, 49-bit canonical mask and |Stab| of the set `mask` (|S| = K,
+// wave-uniform); every lane returns the same values.  par_mine = T.par[lane]
+// (the relation is symmetric: bit r1 says whether this lane's vector is
+// parallel to r1).
+template <int K>
+__device__ __forceinline__ CanonOut canon_set(const CanonTables& T, uint64_t mask, uint32_t lane,
+                                              uint64_t par_mine) {
+  constexpr int NP = (K + 4) / 5;   // 6-bit fields, five per dword
+  uint32_t p[K];
+  {
+    uint64_t m = mask;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      p[i] = (uint32_t)__builtin_ctzll(m);
+      m &= ~(1ull << p[i]);   // one scalar bit clear; m &= m - 1 takes four
+    }
+  }
+  // Phase 1, lane r1 < 48: values z_i = r1.p_i (the row values when the lane
+  // stands for r1, the column values when it stands for r2), their row set R,
+  // the shifts 8 z_i packed for the readlanes of phase 2, and the anchors
+  // whose row starts the longest empty run.
+  // Lanes 48..63 stand for no vector; they read rows 32..47, the addresses
+  // lanes 32..47 read in the same half-wave (a broadcast; row 0 would share
+  // a bank with row 32 at another address).  Their values are never used.
+  const bool row_lane = lane < (uint32_t)kRows;
+  const uint32_t base = (row_lane ? lane : lane - 16u) * kRowStride;
+  uint32_t z[K];
+  uint32_t P[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) P[j] = 0;
+  uint32_t R = 0;
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    z[i] = T.rowval[base + p[i]];
+    R |= 1u << z[i];
+    P[i / 5] |= z[i] << (6 * (i % 5) + 3);
+  }
+  const uint32_t lg = T.longest[R];
+  const uint32_t tm = T.topmask[R];
+  uint32_t am = 0;
+#pragma unroll
+  for (int t = 0; t < K; ++t) am |= ((tm >> z[t]) & 1u) << t;
+  // First rows reaching the wave's longest run (the top row of the minimum is
+  // 6 minus that run).  Every row lane qualifies at L = 0, so the loop ends.
+  uint64_t cand;
+  uint32_t L = 6;
+  while ((cand = __ballot(row_lane && lg >= L)) == 0) --L;
+  // Phase 2, lane r2 < 48 (not parallel to r1), for every such r1: image of
+  // A = (r1; r2), translated by each candidate anchor.  Row i of the image
+  // enters as (1 << 8 w_i) << z_i, split in 32-bit halves: the shifted value
+  // is the uniform one, so the half that does not hold row w_i receives 0.
+  uint64_t best = ~0ull;
+  uint32_t cnt = 0;
+  while (cand != 0) {
+    const int r1 = __builtin_ctzll(cand);
+    cand &= ~(1ull << r1);
+    const uint32_t amr = __builtin_amdgcn_readlane(am, r1);
+    uint32_t Pr[NP];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) Pr[j] = __builtin_amdgcn_readlane(P[j], r1);
+    if (row_lane & !((par_mine >> r1) & 1ull)) {   // one exec mask, not two nested
+      uint32_t lo = 0, hi = 0;
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const uint64_t pw = 1ull << ((Pr[i / 5] >> (6 * (i % 5))) & 63u);
+        lo = lshl_or_s((uint32_t)pw, z[i], lo);
+        hi = lshl_or_s((uint32_t)(pw >> 32), z[i], hi);
+      }
+      const uint64_t MA = ((uint64_t)hi << 32) | lo;
+#pragma unroll
+      for (int t = 0; t < K; ++t) {
+        if ((amr >> t) & 1u) {
+          const uint32_t sh = (Pr[t / 5] >> (6 * (t % 5))) & 63u;
+          const uint64_t img = translate(MA, T.anc[sh + z[t]]);
+          const bool lt = img < best, eq = img == best;
+          cnt = (lt ? 0u : cnt) + (uint32_t)(lt | eq);
+          best = lt ? img : best;
+        }
+      }
+    }
+  }
+  // Wave minimum: follow lanes holding a smaller value (each step lowers m,
+  // so at most 64 steps, a few on typical sets), then add the counts of the
+  // lanes that hold it.
+  uint64_t m = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(best >> 32)) << 32) |
+               __builtin_amdgcn_readfirstlane((uint32_t)best);
+  for (;;) {
+    const uint64_t lower = __ballot(best < m);
+    if (lower == 0) break;
+    const int l = __builtin_ctzll(lower);
+    m = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(best >> 32), l) << 32) |
+        __builtin_amdgcn_readlane((uint32_t)best, l);
+  }
+  uint64_t eq = __ballot(best == m);
+  uint32_t total = 0;
+  while (eq != 0) {
+    const int l = __builtin_ctzll(eq);
+    eq &= ~(1ull << l);
+    total += __builtin_amdgcn_readlane(cnt, l);
+  }
+  // Sorted positions of the canonical image, packed as kernel_canon.py:84-104:
+  // K <= 10 packs positions 0..K-1, K >= 11 skips position 0 (always 0).
+  CanonOut o;
+  o.canon = rows_to_mask(m);
+  o.stab = total;
+  constexpr int pack_from = (K <= 10) ? 0 : 1;
+  uint64_t lo = 0, hi = 0, mm = o.canon;
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const uint64_t v = (uint64_t)__builtin_ctzll(mm);
+    mm &= ~(1ull << v);
+    if (i < pack_from) continue;
+    const int off = 6 * (i - pack_from);
+    if (off + 6 <= 64) lo |= v << off;
+    else if (off < 64) { lo |= v << off; hi |= v >> (64 - off); }
+    else hi |= v << (off - 64);
+  }
+  o.key_lo = lo;
+  o.key_hi = hi;
+  return o;
 }
 
 // Counting sort of set indices by |S| (1..16); bad sizes are flagged.  Each
@@ -277,7 +372,7 @@ __global__ void canon_fill_kernel(uint32_t* order, uint32_t len, uint32_t* hist)
   if (i < 35) hist[i] = 0u;
 }
 
-__global__ __launch_bounds__(kCanonBlock) void canon_kernel(
+__global__ __launch_bounds__(kCanonBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void canon_kernel(
     const uint64_t* __restrict__ masks, const uint32_t* __restrict__ order, uint32_t slots,
     uint64_t* __restrict__ key_lo, uint64_t* __restrict__ key_hi, uint64_t* __restrict__ canon,
     int32_t* __restrict__ stab) {
@@ -292,22 +387,10 @@ __global__ __launch_bounds__(kCanonBlock) void canon_kernel(
     const uint32_t idx = __builtin_amdgcn_readfirstlane(order[j]);
     if (idx == 0xFFFFFFFFu) continue;
     const uint64_t mask = masks[idx];
-    const int k = __popcll(mask);
-    uint32_t pts[kMaxKCanon];
-    {
-      uint64_t m = mask;
-#pragma unroll
-      for (int i = 0; i < kMaxKCanon; ++i) {
-        const uint32_t q = (uint32_t)__builtin_ctzll(m | (1ull << 63));
-        pts[i] = (q > 48) ? 0u : q;
-        m &= m - 1;
-      }
-    }
-    uint64_t best = ~0ull;
-    uint32_t cnt = 0;
-    switch (k) {
+    CanonOut o;
+    switch (__popcll(mask)) {
 #define GT_CANON_CASE(KK) \
-  case KK: canon_set<KK>(T, pts, lane, par_mine, best, cnt); break;
+  case KK: o = canon_set<KK>(T, mask, lane, par_mine); break;
       GT_CANON_CASE(1) GT_CANON_CASE(2) GT_CANON_CASE(3) GT_CANON_CASE(4)
       GT_CANON_CASE(5) GT_CANON_CASE(6) GT_CANON_CASE(7) GT_CANON_CASE(8)
       GT_CANON_CASE(9) GT_CANON_CASE(10) GT_CANON_CASE(11) GT_CANON_CASE(12)
@@ -315,25 +398,12 @@ __global__ __launch_bounds__(kCanonBlock) void canon_kernel(
 #undef GT_CANON_CASE
       default: continue;   // sizes outside 1..16 never reach the order array
     }
-    if (lane != 0) continue;
-    best = rows_to_mask(best);
-    // Sorted positions of the canonical image, packed as kernel_canon.py:84-104:
-    // k <= 10 packs positions 0..k-1, k >= 11 skips position 0 (always 0).
-    const int pack_from = (k <= 10) ? 0 : 1;
-    uint64_t lo = 0, hi = 0, m = best;
-    for (int i = 0; i < k; ++i) {
-      const uint64_t v = (uint64_t)__builtin_ctzll(m);
-      m &= m - 1;
-      if (i < pack_from) continue;
-      const int off = 6 * (i - pack_from);
-      if (off + 6 <= 64) lo |= v << off;
-      else if (off < 64) { lo |= v << off; hi |= v >> (64 - off); }
-      else hi |= v << (off - 64);
+    if (lane == 0) {
+      key_lo[idx] = o.key_lo;
+      key_hi[idx] = o.key_hi;
+      canon[idx] = o.canon;
+      stab[idx] = (int32_t)o.stab;
     }
-    key_lo[idx] = lo;
-    key_hi[idx] = hi;
-    canon[idx] = best;
-    stab[idx] = (int32_t)cnt;
   }
 }
 
