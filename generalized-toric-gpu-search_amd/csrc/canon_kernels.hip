@@ -257,21 +257,19 @@ __device__ __forceinline__ CanonOut canon_set(const CanonTables& T, uint64_t mas
   // Wave minimum: follow lanes holding a smaller value (each step lowers m,
   // so at most 64 steps, a few on typical sets), then add the counts of the
   // lanes that hold it.
-  uint64_t m = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(best >> 32)) << 32) |
-               __builtin_amdgcn_readfirstlane((uint32_t)best);
+  uint64_t m = uniform64(best);
   for (;;) {
     const uint64_t lower = __ballot(best < m);
     if (lower == 0) break;
     const int l = __builtin_ctzll(lower);
-    m = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(best >> 32), l) << 32) |
-        __builtin_amdgcn_readlane((uint32_t)best, l);
+    m = readlane64(best, l);
   }
   uint64_t eq = __ballot(best == m);
   uint32_t total = 0;
   while (eq != 0) {
     const int l = __builtin_ctzll(eq);
     eq &= ~(1ull << l);
-    total += __builtin_amdgcn_readlane(cnt, l);
+    total += (uint32_t)__builtin_amdgcn_readlane(cnt, l);
   }
   // Sorted positions of the canonical image, packed as kernel_canon.py:84-104:
   // K <= 10 packs positions 0..K-1, K >= 11 skips position 0 (always 0).
@@ -372,38 +370,57 @@ __global__ void canon_fill_kernel(uint32_t* order, uint32_t len, uint32_t* hist)
   if (i < 35) hist[i] = 0u;
 }
 
+constexpr uint32_t kNoSet = 0xFFFFFFFFu;
+
 __global__ __launch_bounds__(kCanonBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void canon_kernel(
-    const uint64_t* __restrict__ masks, const uint32_t* __restrict__ order, uint32_t slots,
-    uint64_t* __restrict__ key_lo, uint64_t* __restrict__ key_hi, uint64_t* __restrict__ canon,
-    int32_t* __restrict__ stab) {
+    const uint64_t* __restrict__ masks, const uint32_t* __restrict__ order,
+    uint32_t slots, uint64_t* __restrict__ key_lo, uint64_t* __restrict__ key_hi,
+    uint64_t* __restrict__ canon, int32_t* __restrict__ stab) {
   __shared__ CanonTables T;
   build_canon_tables(T);
   __syncthreads();
   const uint32_t lane = __lane_id();
   const uint64_t par_mine = T.par[lane < (uint32_t)kRows ? lane : 0u];
   const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
-  for (uint32_t j = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; j < slots;
-       j += nwaves) {
-    const uint32_t idx = __builtin_amdgcn_readfirstlane(order[j]);
-    if (idx == 0xFFFFFFFFu) continue;
-    const uint64_t mask = masks[idx];
-    CanonOut o;
-    switch (__popcll(mask)) {
+  // Each wave's order entries and masks are fetched ahead: the mask of the
+  // next set and the order entry of the one after it are issued before the
+  // current set is worked on.  The wave index j is not known to be uniform
+  // here, so these are vector loads, counted on vmcnt, and the LDS waits of
+  // the current set do not wait for them, as they would for scalar loads
+  // (scalar loads and LDS share lgkmcnt).  Every address is in bounds: order
+  // positions are checked against slots and a padding entry reads mask 0.
+  uint32_t j = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  uint32_t idx_v = (j < slots) ? order[j] : kNoSet;
+  uint64_t mv = masks[(idx_v == kNoSet) ? 0u : idx_v];
+  uint32_t next_v = (j + nwaves < slots) ? order[j + nwaves] : kNoSet;
+  for (; j < slots; j += nwaves) {
+    const uint64_t mv_next = masks[(next_v == kNoSet) ? 0u : next_v];
+    const uint32_t after_v = (j + 2u * nwaves < slots) ? order[j + 2u * nwaves] : kNoSet;
+    const uint32_t idx = __builtin_amdgcn_readfirstlane(idx_v);
+    if (idx != kNoSet) {
+      const uint64_t mask = uniform64(mv);
+      CanonOut o;
+      bool ok = true;
+      switch (__popcll(mask)) {
 #define GT_CANON_CASE(KK) \
   case KK: o = canon_set<KK>(T, mask, lane, par_mine); break;
-      GT_CANON_CASE(1) GT_CANON_CASE(2) GT_CANON_CASE(3) GT_CANON_CASE(4)
-      GT_CANON_CASE(5) GT_CANON_CASE(6) GT_CANON_CASE(7) GT_CANON_CASE(8)
-      GT_CANON_CASE(9) GT_CANON_CASE(10) GT_CANON_CASE(11) GT_CANON_CASE(12)
-      GT_CANON_CASE(13) GT_CANON_CASE(14) GT_CANON_CASE(15) GT_CANON_CASE(16)
+        GT_CANON_CASE(1) GT_CANON_CASE(2) GT_CANON_CASE(3) GT_CANON_CASE(4)
+        GT_CANON_CASE(5) GT_CANON_CASE(6) GT_CANON_CASE(7) GT_CANON_CASE(8)
+        GT_CANON_CASE(9) GT_CANON_CASE(10) GT_CANON_CASE(11) GT_CANON_CASE(12)
+        GT_CANON_CASE(13) GT_CANON_CASE(14) GT_CANON_CASE(15) GT_CANON_CASE(16)
 #undef GT_CANON_CASE
-      default: continue;   // sizes outside 1..16 never reach the order array
+        default: ok = false; break;   // sizes outside 1..16 never reach the order array
+      }
+      if (ok && lane == 0) {
+        key_lo[idx] = o.key_lo;
+        key_hi[idx] = o.key_hi;
+        canon[idx] = o.canon;
+        stab[idx] = (int32_t)o.stab;
+      }
     }
-    if (lane == 0) {
-      key_lo[idx] = o.key_lo;
-      key_hi[idx] = o.key_hi;
-      canon[idx] = o.canon;
-      stab[idx] = (int32_t)o.stab;
-    }
+    idx_v = next_v;
+    mv = mv_next;
+    next_v = after_v;
   }
 }
 
@@ -448,8 +465,8 @@ hipError_t launch_canon(const uint64_t* d_masks, int32_t n,
   const uint32_t waves = (uint32_t)n;
   const int grid = (int)std::min<uint32_t>(kCanonMaxBlocks,
                                            (waves + kCanonBlock / kWave - 1) / (kCanonBlock / kWave));
-  hipLaunchKernelGGL(canon_kernel, dim3(grid), dim3(kCanonBlock), 0, s, d_masks, d_order, slots,
-                     d_key_lo, d_key_hi, d_canon, d_stab);
+  hipLaunchKernelGGL(canon_kernel, dim3(grid), dim3(kCanonBlock), 0, s, d_masks, d_order,
+                     slots, d_key_lo, d_key_hi, d_canon, d_stab);
   return hipGetLastError();
 }
 

@@ -113,4 +113,20 @@ enum : uint32_t {
   kErrFull = 4u,   // hash set has no free slot
 };
 
+#if defined(__HIPCC__)
+// Wave-uniform copies of 64-bit values.  The lane-read builtins return int:
+// each half is converted to uint32_t before widening, or a low half with
+// bit 31 set would be sign-extended over the high half.
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((uint32_t)v, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+#endif
+
 }  // namespace gt

@@ -65,8 +65,7 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, const uint32_t* __restrict
     }
     const uint64_t pub =
         __hip_atomic_load(&h->published, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    const uint64_t pubu = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(pub >> 32)) << 32) |
-                          __builtin_amdgcn_readfirstlane((uint32_t)pub);
+    const uint64_t pubu = uniform64(pub);
     if (pubu <= disp) {
       __builtin_amdgcn_s_sleep(8);
       continue;
@@ -143,11 +142,6 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, const uint32_t* __restrict
     ++passes;
     busy += wall_clock64() - now;
   }
-}
-
-__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
-  return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
-         __builtin_amdgcn_readfirstlane((uint32_t)v);
 }
 
 // Rows of the request a wave is walking, staged in that wave's LDS area.

@@ -44,6 +44,24 @@ def test_fixture_sets_all_k(canon, cgold):
         assert np.array_equal(st, cgold[f"x{k}_stab"]), k
 
 
+def test_sets_holding_points_31_and_32(canon):
+    """Masks whose bit 31 (low word) and bit 32 (high word) are set, at every
+    size: a 64-bit mask read back from a lane in two 32-bit halves must not
+    sign-extend the low half."""
+    rng = np.random.default_rng(311)
+    for k in range(2, 17):
+        rows = []
+        for _ in range(24):
+            rest = rng.choice([p for p in range(49) if p not in (31, 32)], k - 2, replace=False)
+            rows.append(sorted([31, 32, *map(int, rest)]))
+        sets = np.array(rows, dtype=np.int32)
+        lo, hi, cm, st = canon.compute_full(sets)
+        rlo, rhi, rst = oracle.canon_batch(sets)
+        assert np.array_equal(lo, rlo), k
+        assert np.array_equal(hi, rhi), k
+        assert np.array_equal(st, rst), k
+
+
 def test_compute_object_keys(canon):
     rng = np.random.default_rng(201)
     for k in (5, 11, 13):
