@@ -20,17 +20,24 @@
 // rows are u_i - u_t (mod 7), so the top row is 6 - (the run of empty rows
 // just below row u_t), which depends on r1 and t only.  Phase 1: lane r1
 // (48 nonzero first rows) reads its row values, their row set R, the longest
-// cyclic run of empty rows below an occupied row of R and the rows starting
-// such a run (two 128-byte tables indexed by R); the first rows with the
-// wave's longest run give the top row of the canonical image, and a ballot
-// collects them (typically a handful of the 48).  Phase 2: for every such r1,
-// lane r2 builds the image for A = (r1; r2) (42 valid second rows) from its
-// own phase-1 values (the columns) and r1's row shifts (read from lane r1 in
-// packed form), and translates it by every anchor whose row starts the longest
-// run; those are the only (A, t) pairs whose image can be minimal, so both the
-// minimum and the count of pairs that attain it (|Stab|) are exact.  Ballots
-// combine the lanes and the key is packed in scalar registers.  Sets are
-// bucketed by size, largest first, and walked by the waves with a fixed stride.
+// cyclic run of empty rows below an occupied row of R, and the rows just
+// below such a run (the rows that become the top row); the first rows with
+// the wave's longest run give the top row of the canonical image.  Next in
+// the order is the content of that top row.  For fixed (r1, t) the 42 second
+// rows r2 = alpha r1 + beta r1' (beta != 0, r1' = (-b, a)) place the top-row
+// points q at columns alpha (v - u_t) - beta s_t + beta s_q, s = r1'.p: every
+// affine map x -> a + b x (b != 0) of the points' positions s_q, and nothing
+// else, so the smallest top-row content a pair can reach is a table value of
+// the 7-bit position pattern, the same for every anchor in the row.  Only the
+// first rows and anchor rows reaching the smallest content go on.  Phase 2:
+// for every such r1, lane r2 builds the image for A = (r1; r2) (42 valid
+// second rows) from its own phase-1 values (the columns) and r1's row shifts
+// (read from lane r1 in packed form), and translates it by every remaining
+// anchor; those are the only (A, t) pairs whose image can be minimal, so both
+// the minimum and the count of pairs that attain it (|Stab|) are exact.
+// Ballots combine the lanes and the key is packed in scalar registers.  Sets
+// are bucketed by size, largest first, and walked by the waves with a fixed
+// stride.
 #include <algorithm>
 
 #include "gt_launch.h"
@@ -88,33 +95,36 @@ __host__ __device__ __forceinline__ Anchor make_anchor(uint32_t u, uint32_t v) {
   return e;
 }
 
-// Tables of one workgroup (4 KB of LDS).  Every table a lane indexes with its
-// own value is read conflict-free: rowval rows are 13 dwords apart (odd), the
-// two R-indexed tables are 32 dwords each, and the 7 anchors one image can
+// Tables of one workgroup (4.1 KB of LDS).  Every table a lane indexes with
+// its own value is read conflict-free: rowpos rows are 13 dwords apart (odd),
+// the 128-byte tables are 32 dwords each, and the 7 anchors one image can
 // use are 7 consecutive 16-byte entries.
 struct alignas(16) CanonTables {
   Anchor anc[56];                      // translation by byte-row position 8u + v
-  uint8_t rowval[kRows * kRowStride];  // rowval[r][p] = (a x + b y) mod 7, r = 7a + b - 1, p = 7x + y
+  uint8_t rowpos[kRows * kRowStride];  // 8 w + s for r = 7a + b - 1 and p = 7x + y:
+                                       // w = (a x + b y) mod 7, s = (-b x + a y) mod 7
   uint64_t par[kRows];                 // bit r2 set when vector r2 is parallel to vector r
   uint8_t longest[128];                // longest cyclic run of empty rows just below an occupied
                                        // row of the row set R
-  uint8_t topmask[128];                // occupied rows of R whose run below has that length
+  uint8_t toprows[128];                // occupied rows of R just below such a run
+  uint8_t amin[128];                   // smallest image of a 7-bit position set under x -> a + b x
 };
-static_assert(sizeof(Anchor) == 16 && offsetof(CanonTables, rowval) % 4 == 0 &&
+static_assert(sizeof(Anchor) == 16 && offsetof(CanonTables, rowpos) % 4 == 0 &&
                   (kRows * kRowStride) % 4 == 0 && offsetof(CanonTables, longest) % 4 == 0 &&
-                  offsetof(CanonTables, topmask) % 4 == 0,
+                  offsetof(CanonTables, toprows) % 4 == 0 && offsetof(CanonTables, amin) % 4 == 0,
               "dword-written tables must be dword aligned");
 
 __device__ void build_canon_tables(CanonTables& T) {
   // Whole dwords: byte stores from neighbouring lanes into one dword conflict.
   for (int q = threadIdx.x; q < kRows * kRowStride / 4; q += blockDim.x) {
-    uint32_t w = 0;
+    uint32_t wd = 0;
     for (int c = 0; c < 4; ++c) {
       const int e = 4 * q + c, r = e / kRowStride, p = e % kRowStride;
-      const int a = (r + 1) / 7, b = (r + 1) % 7;
-      w |= (uint32_t)((p < kNpts) ? (a * (p / 7) + b * (p % 7)) % 7 : 0) << (8 * c);
+      const int a = (r + 1) / 7, b = (r + 1) % 7, x = p / 7, y = p % 7;
+      const int w = (a * x + b * y) % 7, sv = ((7 - b) * x + a * y) % 7;
+      wd |= (uint32_t)((p < kNpts) ? 8 * w + sv : 0) << (8 * c);
     }
-    reinterpret_cast<uint32_t*>(T.rowval)[q] = w;
+    reinterpret_cast<uint32_t*>(T.rowpos)[q] = wd;
   }
   for (int r = threadIdx.x; r < kRows; r += blockDim.x) {
     const int a = (r + 1) / 7, b = (r + 1) % 7;
@@ -126,7 +136,7 @@ __device__ void build_canon_tables(CanonTables& T) {
     T.par[r] = m;
   }
   for (int q = threadIdx.x; q < 32; q += blockDim.x) {
-    uint32_t lw = 0, tw = 0;
+    uint32_t lw = 0, tw = 0, aw = 0;
     for (int c = 0; c < 4; ++c) {
       const int R = 4 * q + c;
       int gap[7], longest = 0;
@@ -138,12 +148,25 @@ __device__ void build_canon_tables(CanonTables& T) {
       }
       uint32_t top = 0;
       for (int u = 0; u < 7; ++u)
-        if (((R >> u) & 1) && gap[u] == longest) top |= 1u << u;
+        if (((R >> u) & 1) && gap[u] == longest) top |= 1u << ((u + 6 - longest) % 7);
+      // Smallest image of the position set R under the 42 maps x -> a + b x.
+      uint32_t best = (R == 0) ? 0u : 0x7Fu;
+      for (int bm = 1; bm < 7 && R != 0; ++bm) {
+        uint32_t img = 0;
+        for (int x = 0; x < 7; ++x)
+          if ((R >> x) & 1) img |= 1u << ((bm * x) % 7);
+        for (int am = 0; am < 7; ++am) {
+          const uint32_t rot = ((img << am) | (img >> (7 - am))) & 0x7Fu;
+          best = min(best, rot);
+        }
+      }
       lw |= (uint32_t)longest << (8 * c);
       tw |= top << (8 * c);
+      aw |= best << (8 * c);
     }
     reinterpret_cast<uint32_t*>(T.longest)[q] = lw;
-    reinterpret_cast<uint32_t*>(T.topmask)[q] = tw;
+    reinterpret_cast<uint32_t*>(T.toprows)[q] = tw;
+    reinterpret_cast<uint32_t*>(T.amin)[q] = aw;
   }
   if (threadIdx.x < 56) {
     const uint32_t u = threadIdx.x >> 3, v = threadIdx.x & 7u;
@@ -192,8 +215,8 @@ __device__ __forceinline__ CanonOut canon_set(const CanonTables& T, uint64_t mas
   }
   // Phase 1, lane r1 < 48: values z_i = r1.p_i (the row values when the lane
   // stands for r1, the column values when it stands for r2), their row set R,
-  // the shifts 8 z_i packed for the readlanes of phase 2, and the anchors
-  // whose row starts the longest empty run.
+  // the shifts 8 z_i packed for the readlanes of phase 2, and the lane's image
+  // under (r1; r1') in byte-row layout (row z_i, column s_i = r1'.p_i).
   // Lanes 48..63 stand for no vector; they read rows 32..47, the addresses
   // lanes 32..47 read in the same half-wave (a broadcast; row 0 would share
   // a bank with row 32 at another address).  Their values are never used.
@@ -204,22 +227,50 @@ __device__ __forceinline__ CanonOut canon_set(const CanonTables& T, uint64_t mas
 #pragma unroll
   for (int j = 0; j < NP; ++j) P[j] = 0;
   uint32_t R = 0;
+  uint64_t ML = 0;
 #pragma unroll
   for (int i = 0; i < K; ++i) {
-    z[i] = T.rowval[base + p[i]];
+    const uint32_t y = T.rowpos[base + p[i]];
+    z[i] = y >> 3;
     R |= 1u << z[i];
     P[i / 5] |= z[i] << (6 * (i % 5) + 3);
+    ML |= 1ull << y;
   }
   const uint32_t lg = T.longest[R];
-  const uint32_t tm = T.topmask[R];
-  uint32_t am = 0;
-#pragma unroll
-  for (int t = 0; t < K; ++t) am |= ((tm >> z[t]) & 1u) << t;
-  // First rows reaching the wave's longest run (the top row of the minimum is
-  // 6 minus that run).  Every row lane qualifies at L = 0, so the loop ends.
+  // First rows reaching the wave's longest run.  Every row lane qualifies at
+  // L = 0, so the loop ends.
   uint64_t cand;
   uint32_t L = 6;
   while ((cand = __ballot(row_lane && lg >= L)) == 0) --L;
+  const bool is_cand = (cand >> lane) & 1ull;
+  // Smallest top-row content each of them allows, over its top rows (usually
+  // one), and the top rows that reach it.
+  uint32_t cmin = 0xFFu, vq = 0;
+  uint32_t rem = is_cand ? (uint32_t)T.toprows[R] : 0u;
+  while (__ballot(rem != 0) != 0) {
+    if (rem != 0) {
+      const uint32_t v = __builtin_ctz(rem);
+      rem &= rem - 1;
+      const uint32_t c = T.amin[(uint32_t)(ML >> (8 * v)) & 0x7Fu];
+      vq = (c < cmin) ? (1u << v) : ((c == cmin) ? (vq | (1u << v)) : vq);
+      cmin = min(cmin, c);
+    }
+  }
+  // The wave's smallest content (follow lanes holding a smaller one), then
+  // the first rows reaching it and, per lane, the anchor rows: a top row v
+  // comes from anchors in row v + lg + 1.
+  uint32_t cs = (uint32_t)__builtin_amdgcn_readlane(cmin, __builtin_ctzll(cand));
+  for (;;) {
+    const uint64_t lower = __ballot(is_cand && cmin < cs);
+    if (lower == 0) break;
+    cs = (uint32_t)__builtin_amdgcn_readlane(cmin, __builtin_ctzll(lower));
+  }
+  cand = __ballot(is_cand && cmin == cs);
+  const uint32_t n_rot = (lg + 1u) % 7u;
+  const uint32_t uq = ((vq << n_rot) | (vq >> (7u - n_rot))) & 0x7Fu;
+  uint32_t am = 0;
+#pragma unroll
+  for (int t = 0; t < K; ++t) am |= ((uq >> z[t]) & 1u) << t;
   // Phase 2, lane r2 < 48 (not parallel to r1), for every such r1: image of
   // A = (r1; r2), translated by each candidate anchor.  Row i of the image
   // enters as (1 << 8 w_i) << z_i, split in 32-bit halves: the shifted value
