@@ -358,6 +358,7 @@ int gt_md_batch(gt_ctx* c, const int32_t* idx, const int32_t* offsets, int32_t n
   if (n == 0) return 0;
   if (!offsets || !out) return fail("null offsets/out");
   const int64_t n_idx = offsets[n];
+  if (n_idx > 0 && !idx) return fail("null idx");
   if (offsets[0] != 0) return fail("offsets[0] must be 0");
   bool thresholded = false;
   uint64_t chunks = 0;
@@ -398,6 +399,8 @@ int gt_md_batch_dev(gt_ctx* c, const int32_t* d_idx, int64_t n_idx, const int32_
                     int64_t sample_count, uint64_t sample_seed, int32_t* d_out, void* stream) {
   if (!c) return fail("null context");
   if (n < 0 || n_idx < 0) return fail("negative sizes");
+  if (n > 0 && (!d_offsets || !d_out)) return fail("null device offsets/out");
+  if (n_idx > 0 && !d_idx) return fail("null device idx");
   std::lock_guard<std::mutex> lk(c->mu);
   GT_OK(c->activate());
   return c->md_enqueue(d_idx, n_idx, d_offsets, n, d_tmax, tmax, thresholded != 0, sample_count,
@@ -509,6 +512,8 @@ int gt_canon_masks(gt_ctx* c, const uint64_t* masks, int32_t n, uint64_t* key_lo
 int gt_canon_masks_dev(gt_ctx* c, const uint64_t* d_masks, int32_t n, uint64_t* d_key_lo,
                        uint64_t* d_key_hi, uint64_t* d_canon_mask, int32_t* d_stab, void* stream) {
   if (!c) return fail("null context");
+  if (n < 0) return fail("negative batch size");
+  if (n > 0 && !d_masks) return fail("null device masks");
   if (!d_key_lo || !d_key_hi || !d_canon_mask || !d_stab) return fail("null device output");
   std::lock_guard<std::mutex> lk(c->mu);
   GT_OK(c->activate());
@@ -685,7 +690,7 @@ int gt_dedup(gt_hashset* h, const uint64_t* keys, int32_t n, uint32_t* flag) {
 
 int gt_compact_dev(gt_ctx* c, const uint32_t* d_flag, int32_t n, uint32_t* d_out_idx,
                    uint32_t* d_count, void* stream) {
-  if (!c || n < 0) return fail("bad argument");
+  if (!c || n < 0 || !d_count || (n > 0 && (!d_flag || !d_out_idx))) return fail("bad argument");
   std::lock_guard<std::mutex> lk(c->mu);
   GT_OK(c->activate());
   GT_OK(c->bsum.ensure((size_t)((n + 1023) / 1024 + 1) * 4));
@@ -704,14 +709,17 @@ int gt_bfs_level(gt_ctx* c, gt_hashset* seen, const uint64_t* d_par, int32_t n_p
     return fail("bad argument (parent size must be 0..%d)", kMaxKCanon - 1);
   *n_new = 0;
   if (n_par == 0) return 0;
+  if (!d_par || cap < 0 || (cap > 0 && (!d_out_masks || !d_out_mz))) return fail("bad argument");
   if (seen->ctx != c) return fail("hash set belongs to another context");
   const int64_t nch = (int64_t)n_par * (kNpts - kp);
   if (nch >= (1ll << 31)) return fail("level too large (%lld children)", (long long)nch);
+  // One lock for the whole level: the expand/canonical/dedup scratch read by
+  // the second half must not be reused by another call on this context.
   uint32_t nu = 0;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  hipStream_t s = c->stream;
   {
-    std::lock_guard<std::mutex> lk(c->mu);
-    GT_OK(c->activate());
-    hipStream_t s = c->stream;
     const size_t n = (size_t)nch;
     GT_OK(c->children.ensure(n * 8));
     GT_OK(c->klo.ensure(n * 8));
@@ -734,8 +742,6 @@ int gt_bfs_level(gt_ctx* c, gt_hashset* seen, const uint64_t* d_par, int32_t n_p
   *n_new = (int32_t)nu;
   if ((int64_t)nu > cap) return fail("%u new classes exceed the output capacity %d", nu, cap);
   if (nu == 0) return 0;
-  std::lock_guard<std::mutex> lk(c->mu);
-  hipStream_t s = c->stream;
   const int k = kp + 1;
   GT_OK(c->bmasks.ensure((size_t)nu * 8));
   GT_OK(c->brows.ensure((size_t)nu * k * 4));
