@@ -267,13 +267,13 @@ int gt_serve_start(int32_t device, int32_t slots, int64_t work_budget, const int
 
 int gt_serve_submit(gt_server* sv, const int32_t* idx, int32_t k, int32_t tmax, uint64_t tag,
                     int32_t* accepted) {
-  if (!sv || !accepted) return sfail("null argument");
+  if (!sv || !accepted || (k > 0 && !idx)) return sfail("null argument");
   std::lock_guard<std::mutex> lk(sv->mu);
   return sv->submit(idx, k, tmax, tag, accepted);
 }
 
 int gt_serve_poll(gt_server* sv, uint64_t* tags, int32_t* results, int32_t max_n, int32_t* n_done) {
-  if (!sv || !n_done) return sfail("null argument");
+  if (!sv || !n_done || max_n < 0) return sfail("bad argument");   // null tags/results: discarded
   std::lock_guard<std::mutex> lk(sv->mu);
   *n_done = sv->poll(tags, results, max_n, nullptr);
   return 0;
@@ -292,8 +292,9 @@ int gt_serve_in_flight(gt_server* sv, int64_t* n) {
 int gt_serve_run_stream(gt_server* sv, const int32_t* idx, const int32_t* offsets, int32_t n,
                         const int32_t* tmax, const double* arrival_s, int32_t* results,
                         double* latency_s, double* wall_s, double timeout_s) {
-  if (!sv || !offsets || !tmax || !arrival_s || !results || !latency_s)
+  if (!sv || n < 0 || !offsets || !tmax || !arrival_s || !results || !latency_s)
     return sfail("null argument");
+  if (n > 0 && offsets[n] > offsets[0] && !idx) return sfail("null argument");
   std::lock_guard<std::mutex> lk(sv->mu);
   for (int32_t i = 0; i < n; ++i)
     if (offsets[i + 1] - offsets[i] > kServeMaxK || offsets[i + 1] < offsets[i])
