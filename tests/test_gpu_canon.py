@@ -44,6 +44,16 @@ def test_fixture_sets_all_k(canon, cgold):
         assert np.array_equal(st, cgold[f"x{k}_stab"]), k
 
 
+def test_reference_vectors(canon):
+    """The kernel against keys written by the reference's own CPU path
+    (tests/golden/make_ref_canon.py), k = 1..10."""
+    ref = np.load(GOLD / "canon_ref_vectors.npz")
+    for k in range(1, 11):
+        lo, hi, cm, st = canon.compute_full(ref[f"k{k}_sets"])
+        assert np.array_equal(lo, ref[f"k{k}_keys"]), k
+        assert np.all(hi == 0), k
+
+
 def test_sets_holding_points_31_and_32(canon):
     """Masks whose bit 31 (low word) and bit 32 (high word) are set, at every
     size: a 64-bit mask read back from a lane in two 32-bit halves must not

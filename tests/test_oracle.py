@@ -186,6 +186,18 @@ def test_numpy_and_c_restatements_agree():
         assert np.array_equal(oracle.canonical_forms_batch_np(sets, perms), lo)
 
 
+def test_canonical_keys_equal_reference_vectors():
+    """Keys written by the reference's own canonical_forms_batch
+    (champion_search_canonical.py:115-140) for k = 1..10, through
+    tests/golden/make_ref_canon.py: they pin the packed layout (sorted
+    anchored positions, 6 bits each) independently of the restatement."""
+    ref = np.load(GOLD / "canon_ref_vectors.npz")
+    for k in range(1, 11):
+        lo, hi, _ = oracle.canon_batch(ref[f"k{k}_sets"])
+        assert np.array_equal(lo, ref[f"k{k}_keys"]), k
+        assert np.all(hi == 0), k
+
+
 def test_canonical_equals_direct_affine_definition():
     rng = np.random.default_rng(6)
     for k in (2, 3, 4, 5, 6, 11):
