@@ -217,9 +217,9 @@ struct gt_ctx {
     if (n <= 0) return 0;
     GT_OK(rows.ensure((size_t)std::max<int64_t>(n_idx, 1) * 3 * kRowWords * 4));
     GT_OK(upload_params(s));
-    GT_HIP(launch_md_rows(d_idx, n_idx, mplanes.as<uint32_t>(), rows.as<uint32_t>(),
-                          ctrl.as<uint32_t>(), s));
     if (sample_count > 0) {
+      GT_HIP(launch_md_rows(d_idx, n_idx, mplanes.as<uint32_t>(), rows.as<uint32_t>(),
+                            ctrl.as<uint32_t>(), s));
       GT_OK(zmax.ensure((size_t)n * 4));
       GT_OK(stopf.ensure((size_t)n * 4));
       hipEvent_t e0 = nullptr;
@@ -237,7 +237,8 @@ struct gt_ctx {
     uint32_t* d_claims = plan.as<uint32_t>();
     MdPlan* d_plan = reinterpret_cast<MdPlan*>(plan.as<char>() + kClaimBytes);
     uint32_t* d_order = reinterpret_cast<uint32_t*>(plan.as<char>() + kClaimBytes + sizeof(MdPlan));
-    GT_HIP(launch_md_plan(d_offsets, n, params_buf.as<MdParams>(), d_plan, d_order, d_claims,
+    GT_HIP(launch_md_prep(d_idx, n_idx, mplanes.as<uint32_t>(), rows.as<uint32_t>(), d_offsets, n,
+                          params_buf.as<MdParams>(), d_plan, d_order, d_claims,
                           best.as<uint32_t>(), ctrl.as<uint32_t>(), s));
     hipEvent_t e0 = nullptr;
     GT_OK(prof_begin("md_main", s, &e0));

@@ -18,9 +18,11 @@ constexpr int kCtrlWords = 16;
 hipError_t launch_md_rows(const int32_t* d_idx, int64_t n_idx, const uint32_t* d_mplanes,
                           uint32_t* d_rows, uint32_t* d_ctrl, hipStream_t s);
 // d_claims: kClaimShards counters, kClaimStride words apart (reset by the plan).
-hipError_t launch_md_plan(const int32_t* d_offsets, int32_t n, const MdParams* d_params,
-                          MdPlan* d_plan, uint32_t* d_order, uint32_t* d_claims,
-                          uint32_t* d_best, uint32_t* d_ctrl, hipStream_t s);
+hipError_t launch_md_prep(const int32_t* d_idx, int64_t n_idx, const uint32_t* d_mplanes,
+                          uint32_t* d_rows, const int32_t* d_offsets, int32_t n,
+                          const MdParams* d_params, MdPlan* d_plan, uint32_t* d_order,
+                          uint32_t* d_claims, uint32_t* d_best, uint32_t* d_ctrl,
+                          hipStream_t s);
 hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
                           const int32_t* d_offsets, const MdPlan* d_plan,
                           const uint32_t* d_order, uint32_t* d_claims, uint32_t* d_best,

@@ -5,9 +5,10 @@
 // and eval_min_distance / eval_min_distance_single_bp (kernel.py:63-94,
 // kernel_cuda_bp.py:99-127).  Work decomposition (see gt_common.h):
 //
-//   md_rows     one thread per generator: the three bit rows x^b * g_i.
-//   md_plan     one workgroup: codes bucketed by k, largest k first (MdPlan +
-//               order[]), reset of the per-code minimum and of the chunk counter.
+//   md_prep     workgroup 0: codes bucketed by k, largest k first (MdPlan +
+//               order[]), reset of the per-code minimum and of the chunk
+//               counters; workgroups 1..: one thread per generator, the three
+//               bit rows x^b * g_i (md_rows alone for the sampled path).
 //   md_main     persistent: each wave claims a chunk id with one atomic on
 //               one of 16 claim counters, finds its bucket by binary search
 //               over 21 entries and its code by one division (scalar work),
@@ -27,12 +28,9 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 // --------------------------------------------------------------- kernels --
 
-__global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict__ idx,
-                                                      int64_t n_idx,
-                                                      const uint32_t* __restrict__ mplanes,
-                                                      uint32_t* __restrict__ rows,
-                                                      uint32_t* ctrl) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void md_rows_one(int64_t t, const int32_t* __restrict__ idx,
+                                            int64_t n_idx, const uint32_t* __restrict__ mplanes,
+                                            uint32_t* __restrict__ rows, uint32_t* ctrl) {
   if (t >= n_idx) return;
   int g = idx[t];
   uint32_t w[6] = {0, 0, 0, 0, 0, 0};
@@ -52,22 +50,33 @@ __global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict_
   }
 }
 
+__global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict__ idx,
+                                                      int64_t n_idx,
+                                                      const uint32_t* __restrict__ mplanes,
+                                                      uint32_t* __restrict__ rows,
+                                                      uint32_t* ctrl) {
+  md_rows_one((int64_t)blockIdx.x * blockDim.x + threadIdx.x, idx, n_idx, mplanes, rows, ctrl);
+}
+
 // Largest-first plan (MdPlan).  One workgroup of 16 waves takes the codes in
 // rounds of 1024; per round every wave counts its codes per bucket with one
 // ballot per bucket.  Pass 1 sizes the buckets, pass 2 writes each code's
 // position: bucket base + codes of that bucket in earlier rounds + in earlier
 // waves of this round + lanes below it.  The order inside a bucket is batch
-// order, so the plan is deterministic.
-__global__ __launch_bounds__(1024) void md_plan_kernel(const int32_t* __restrict__ offsets,
-                                                       int32_t n,
-                                                       const MdParams* __restrict__ p,
-                                                       MdPlan* __restrict__ plan,
-                                                       uint32_t* __restrict__ order,
-                                                       uint32_t* __restrict__ claims,
-                                                       uint32_t* __restrict__ best,
-                                                       uint32_t* ctrl) {
+// order, so the plan is deterministic.  The sizes of four rounds are loaded
+// together (one memory latency per 4096 codes, not one per round), and those
+// of the first 4096 codes stay in registers for pass 2.
+constexpr int kPlanThreads = 1024;
+constexpr int kPlanRounds = 4;
+
+__device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets, int32_t n,
+                                             const MdParams* __restrict__ p,
+                                             MdPlan* __restrict__ plan,
+                                             uint32_t* __restrict__ order,
+                                             uint32_t* __restrict__ claims,
+                                             uint32_t* __restrict__ best, uint32_t* ctrl) {
   constexpr int kB = kMaxKExact;
-  constexpr int kWaves = 1024 / kWave;
+  constexpr int kWaves = kPlanThreads / kWave;
   __shared__ uint32_t wc[kWaves][kB];
   __shared__ uint32_t run[kB];
   __shared__ uint64_t ccs[kB];       // chunks per code of bucket b, one thread each
@@ -78,19 +87,30 @@ __global__ __launch_bounds__(1024) void md_plan_kernel(const int32_t* __restrict
   if (tid < kB) ccs[tid] = code_chunks(kMaxKExact - tid, p->L[kMaxKExact - tid]);
 
   // Pass 1: lane b of each wave accumulates the size of bucket b.
+  int jr[kPlanRounds];
   uint32_t mine = 0;
-  for (int base = 0; base < n; base += 1024) {
-    const int i = base + tid;
-    int j = -1;
-    if (i < n) {
-      const int k = offsets[i + 1] - offsets[i];
-      if (k < 0 || k > kMaxKExact) err |= kErrBadK;
-      else if (k > 0) j = kMaxKExact - k;
-      best[i] = 64u;
+  for (int base = 0; base < n; base += kPlanThreads * kPlanRounds) {
+    int kk[kPlanRounds];
+#pragma unroll
+    for (int r = 0; r < kPlanRounds; ++r) {
+      const int i = base + r * kPlanThreads + tid;
+      kk[r] = (i < n) ? offsets[i + 1] - offsets[i] : 0;
     }
-    for (int b = 0; b < kB; ++b) {
-      const uint32_t c = (uint32_t)__popcll(__ballot(j == b));
-      if (lane == b) mine += c;
+#pragma unroll
+    for (int r = 0; r < kPlanRounds; ++r) {
+      const int i = base + r * kPlanThreads + tid;
+      int j = -1;
+      if (i < n) {
+        const int k = kk[r];
+        if (k < 0 || k > kMaxKExact) err |= kErrBadK;
+        else if (k > 0) j = kMaxKExact - k;
+        best[i] = 64u;
+      }
+      if (base == 0) jr[r] = j;
+      for (int b = 0; b < kB; ++b) {
+        const uint32_t c = (uint32_t)__popcll(__ballot(j == b));
+        if (lane == b) mine += c;
+      }
     }
   }
   if (lane < kB) wc[wave][lane] = mine;
@@ -125,34 +145,59 @@ __global__ __launch_bounds__(1024) void md_plan_kernel(const int32_t* __restrict
   __syncthreads();
 
   // Pass 2: positions.
-  for (int base = 0; base < n; base += 1024) {
-    const int i = base + tid;
-    int j = -1;
-    if (i < n) {
-      const int k = offsets[i + 1] - offsets[i];
-      if (k > 0 && k <= kMaxKExact) j = kMaxKExact - k;
+  for (int base = 0; base < n; base += kPlanThreads * kPlanRounds) {
+    int jj[kPlanRounds];
+#pragma unroll
+    for (int r = 0; r < kPlanRounds; ++r) {
+      const int i = base + r * kPlanThreads + tid;
+      if (base == 0) {
+        jj[r] = jr[r];
+      } else {
+        const int k = (i < n) ? offsets[i + 1] - offsets[i] : 0;
+        jj[r] = (i < n && k > 0 && k <= kMaxKExact) ? kMaxKExact - k : -1;
+      }
     }
-    uint32_t rank = 0;
-    for (int b = 0; b < kB; ++b) {
-      const uint64_t m = __ballot(j == b);
-      if (j == b) rank = (uint32_t)__popcll(m & below);
-      if (lane == b) wc[wave][b] = (uint32_t)__popcll(m);
+#pragma unroll
+    for (int r = 0; r < kPlanRounds; ++r) {
+      const int i = base + r * kPlanThreads + tid;
+      const int j = jj[r];
+      uint32_t rank = 0;
+      for (int b = 0; b < kB; ++b) {
+        const uint64_t m = __ballot(j == b);
+        if (j == b) rank = (uint32_t)__popcll(m & below);
+        if (lane == b) wc[wave][b] = (uint32_t)__popcll(m);
+      }
+      __syncthreads();
+      if (j >= 0) {
+        uint32_t off = run[j] + rank;
+        for (int w = 0; w < wave; ++w) off += wc[w][j];
+        order[off] = (uint32_t)i;
+      }
+      __syncthreads();
+      if (tid < kB) {
+        uint32_t t = 0;
+        for (int w = 0; w < kWaves; ++w) t += wc[w][tid];
+        run[tid] += t;
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    if (j >= 0) {
-      uint32_t off = run[j] + rank;
-      for (int w = 0; w < wave; ++w) off += wc[w][j];
-      order[off] = (uint32_t)i;
-    }
-    __syncthreads();
-    if (tid < kB) {
-      uint32_t t = 0;
-      for (int w = 0; w < kWaves; ++w) t += wc[w][tid];
-      run[tid] += t;
-    }
-    __syncthreads();
   }
   if (err) atomicOr(&ctrl[2], err);
+}
+
+// Plan and bit rows in one launch: workgroup 0 builds the plan, workgroups
+// 1.. write the rows (the two are independent).
+__global__ __launch_bounds__(kPlanThreads) void md_prep_kernel(
+    const int32_t* __restrict__ idx, int64_t n_idx, const uint32_t* __restrict__ mplanes,
+    uint32_t* __restrict__ rows, const int32_t* __restrict__ offsets, int32_t n,
+    const MdParams* __restrict__ p, MdPlan* __restrict__ plan, uint32_t* __restrict__ order,
+    uint32_t* __restrict__ claims, uint32_t* __restrict__ best, uint32_t* ctrl) {
+  if (blockIdx.x == 0) {
+    md_plan_body(offsets, n, p, plan, order, claims, best, ctrl);
+    return;
+  }
+  md_rows_one((int64_t)(blockIdx.x - 1) * kPlanThreads + threadIdx.x, idx, n_idx, mplanes, rows,
+              ctrl);
 }
 
 // Claim loop.  A workgroup's home shard is blockIdx.x % kClaimShards
@@ -411,11 +456,15 @@ hipError_t launch_md_rows(const int32_t* d_idx, int64_t n_idx, const uint32_t* d
   return hipGetLastError();
 }
 
-hipError_t launch_md_plan(const int32_t* d_offsets, int32_t n, const MdParams* d_params,
-                          MdPlan* d_plan, uint32_t* d_order, uint32_t* d_claims,
-                          uint32_t* d_best, uint32_t* d_ctrl, hipStream_t s) {
-  hipLaunchKernelGGL(md_plan_kernel, dim3(1), dim3(1024), 0, s, d_offsets, n, d_params, d_plan,
-                     d_order, d_claims, d_best, d_ctrl);
+hipError_t launch_md_prep(const int32_t* d_idx, int64_t n_idx, const uint32_t* d_mplanes,
+                          uint32_t* d_rows, const int32_t* d_offsets, int32_t n,
+                          const MdParams* d_params, MdPlan* d_plan, uint32_t* d_order,
+                          uint32_t* d_claims, uint32_t* d_best, uint32_t* d_ctrl,
+                          hipStream_t s) {
+  const int64_t row_blocks = (std::max<int64_t>(n_idx, 0) + kPlanThreads - 1) / kPlanThreads;
+  hipLaunchKernelGGL(md_prep_kernel, dim3((unsigned)(1 + row_blocks)), dim3(kPlanThreads), 0, s,
+                     d_idx, n_idx, d_mplanes, d_rows, d_offsets, n, d_params, d_plan, d_order,
+                     d_claims, d_best, d_ctrl);
   return hipGetLastError();
 }
 
