@@ -59,15 +59,43 @@ __global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict_
 }
 
 // Largest-first plan (MdPlan).  One workgroup of 16 waves takes the codes in
-// rounds of 1024; per round every wave counts its codes per bucket with one
-// ballot per bucket.  Pass 1 sizes the buckets, pass 2 writes each code's
-// position: bucket base + codes of that bucket in earlier rounds + in earlier
-// waves of this round + lanes below it.  The order inside a bucket is batch
-// order, so the plan is deterministic.  The sizes of four rounds are loaded
-// together (one memory latency per 4096 codes, not one per round), and those
-// of the first 4096 codes stay in registers for pass 2.
+// batches of 4096: their sizes are loaded together (one memory latency per
+// batch) and their bucket ids kept in LDS, and the buckets present anywhere
+// in the batch are collected once, so the per-wave ballot loops visit those
+// buckets only (a mixed batch uses a handful of the 21).  Pass 1 sizes the
+// buckets, pass 2 writes each code's position: bucket base + codes of that
+// bucket in earlier rounds + in earlier waves of this round + lanes below it
+// (mbcnt of the bucket's ballot).  The order inside a bucket is batch order,
+// so the plan is deterministic.
 constexpr int kPlanThreads = 1024;
 constexpr int kPlanRounds = 4;
+constexpr int kPlanBatch = kPlanThreads * kPlanRounds;
+
+__device__ __forceinline__ void md_plan_load_batch(const int32_t* __restrict__ offsets, int32_t n,
+                                                   int base, int8_t* jb, uint32_t* best,
+                                                   bool first_pass, uint32_t& err,
+                                                   uint32_t& present) {
+  const int tid = threadIdx.x;
+  int kk[kPlanRounds];
+#pragma unroll
+  for (int r = 0; r < kPlanRounds; ++r) {
+    const int i = base + r * kPlanThreads + tid;
+    kk[r] = (i < n) ? offsets[i + 1] - offsets[i] : 0;
+  }
+#pragma unroll
+  for (int r = 0; r < kPlanRounds; ++r) {
+    const int i = base + r * kPlanThreads + tid;
+    int j = -1;
+    if (i < n) {
+      const int k = kk[r];
+      if (k < 0 || k > kMaxKExact) err |= kErrBadK;
+      else if (k > 0) j = kMaxKExact - k;
+      if (first_pass) best[i] = 64u;
+    }
+    jb[r * kPlanThreads + tid] = (int8_t)j;
+    if (j >= 0) present |= 1u << j;
+  }
+}
 
 __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets, int32_t n,
                                              const MdParams* __restrict__ p,
@@ -80,38 +108,32 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
   __shared__ uint32_t wc[kWaves][kB];
   __shared__ uint32_t run[kB];
   __shared__ uint64_t ccs[kB];       // chunks per code of bucket b, one thread each
+  __shared__ int8_t jb[kPlanBatch];  // bucket id per code of the batch, -1 for none
+  __shared__ uint32_t present_s;     // buckets holding at least one code
   const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
-  const uint64_t below = (1ull << lane) - 1ull;
   uint32_t err = 0;
   if (tid < kClaimShards) claims[tid * kClaimStride] = 0u;
   if (tid < kB) ccs[tid] = code_chunks(kMaxKExact - tid, p->L[kMaxKExact - tid]);
+  if (tid == 0) present_s = 0u;
+  __syncthreads();
 
   // Pass 1: lane b of each wave accumulates the size of bucket b.
-  int jr[kPlanRounds];
   uint32_t mine = 0;
-  for (int base = 0; base < n; base += kPlanThreads * kPlanRounds) {
-    int kk[kPlanRounds];
-#pragma unroll
+  for (int base = 0; base < n; base += kPlanBatch) {
+    uint32_t pres = 0;
+    md_plan_load_batch(offsets, n, base, jb, best, true, err, pres);
+    if (pres) atomicOr(&present_s, pres);
+    __syncthreads();
+    const uint32_t present = __builtin_amdgcn_readfirstlane(present_s);
     for (int r = 0; r < kPlanRounds; ++r) {
-      const int i = base + r * kPlanThreads + tid;
-      kk[r] = (i < n) ? offsets[i + 1] - offsets[i] : 0;
-    }
-#pragma unroll
-    for (int r = 0; r < kPlanRounds; ++r) {
-      const int i = base + r * kPlanThreads + tid;
-      int j = -1;
-      if (i < n) {
-        const int k = kk[r];
-        if (k < 0 || k > kMaxKExact) err |= kErrBadK;
-        else if (k > 0) j = kMaxKExact - k;
-        best[i] = 64u;
-      }
-      if (base == 0) jr[r] = j;
-      for (int b = 0; b < kB; ++b) {
+      const int j = jb[r * kPlanThreads + tid];
+      for (uint32_t rem = present; rem != 0; rem &= rem - 1) {
+        const int b = __builtin_ctz(rem);
         const uint32_t c = (uint32_t)__popcll(__ballot(j == b));
-        if (lane == b) mine += c;
+        mine += (lane == b) ? c : 0u;
       }
     }
+    __syncthreads();   // jb is rewritten by the next batch
   }
   if (lane < kB) wc[wave][lane] = mine;
   __syncthreads();
@@ -143,30 +165,28 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
     if (overflow) atomicOr(&ctrl[2], kErrBadK);
   }
   __syncthreads();
+  const uint32_t present = __builtin_amdgcn_readfirstlane(present_s);
 
-  // Pass 2: positions.
-  for (int base = 0; base < n; base += kPlanThreads * kPlanRounds) {
-    int jj[kPlanRounds];
-#pragma unroll
-    for (int r = 0; r < kPlanRounds; ++r) {
-      const int i = base + r * kPlanThreads + tid;
-      if (base == 0) {
-        jj[r] = jr[r];
-      } else {
-        const int k = (i < n) ? offsets[i + 1] - offsets[i] : 0;
-        jj[r] = (i < n && k > 0 && k <= kMaxKExact) ? kMaxKExact - k : -1;
-      }
+  // Pass 2: positions.  A batch of at most 4096 codes is still in jb.
+  for (int base = 0; base < n; base += kPlanBatch) {
+    if (n > kPlanBatch) {
+      uint32_t unused = 0;
+      md_plan_load_batch(offsets, n, base, jb, best, false, err, unused);
+      __syncthreads();
     }
-#pragma unroll
     for (int r = 0; r < kPlanRounds; ++r) {
       const int i = base + r * kPlanThreads + tid;
-      const int j = jj[r];
-      uint32_t rank = 0;
-      for (int b = 0; b < kB; ++b) {
+      const int j = jb[r * kPlanThreads + tid];
+      uint32_t rank = 0, cnt = 0;
+      for (uint32_t rem = present; rem != 0; rem &= rem - 1) {
+        const int b = __builtin_ctz(rem);
         const uint64_t m = __ballot(j == b);
-        if (j == b) rank = (uint32_t)__popcll(m & below);
-        if (lane == b) wc[wave][b] = (uint32_t)__popcll(m);
+        const uint32_t below =
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        rank = (j == b) ? below : rank;
+        cnt = (lane == b) ? (uint32_t)__popcll(m) : cnt;
       }
+      if (lane < kB) wc[wave][lane] = cnt;
       __syncthreads();
       if (j >= 0) {
         uint32_t off = run[j] + rank;
@@ -181,6 +201,7 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
       }
       __syncthreads();
     }
+    if (n > kPlanBatch) __syncthreads();   // jb is rewritten by the next batch
   }
   if (err) atomicOr(&ctrl[2], err);
 }
