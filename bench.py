@@ -161,13 +161,21 @@ def bench_serving(ctx, n_req: int, seed: int = 4):
         arr = workloads.poisson_arrivals(n_req, rate, seed=seed + 1)
         res, lat, wall = srv.run_stream(sets, tmax, arr, timeout_s=300)
         stats = srv.stats()
+    # The same stream with large-k chunks capped at 64 x 2^13 codewords: one
+    # wave walks a chunk, so shorter chunks finish a large request sooner.
+    with Server(0, work_budget=1 << 36, life_s=600, max_chunk_log2=13) as srv:
+        rl0, _, wl0 = srv.run_stream(sets, tmax, np.zeros(n_req), timeout_s=300)
+        cap_l = n_req / wl0
+        arr_l = workloads.poisson_arrivals(n_req, 0.7 * cap_l, seed=seed + 1)
+        rl, lat_l, _ = srv.run_stream(sets, tmax, arr_l, timeout_s=300)
     pres0, _, pwall0 = plain_stream(ctx, sets, tmax, np.zeros(n_req))
     pcap = n_req / pwall0
     # The plain form at its own 0.7 x capacity (at the served rate it saturates).
     parr = workloads.poisson_arrivals(n_req, 0.7 * pcap, seed=seed + 1)
     pres, plat, pwall = plain_stream(ctx, sets, tmax, parr)
     same = bool(np.array_equal(res, res0) and np.array_equal(res, pres) and
-                np.array_equal(res, pres0))
+                np.array_equal(res, pres0) and np.array_equal(res, rl0) and
+                np.array_equal(res, rl))
     out.update({
         "requests": n_req,
         "served_capacity_codes_per_s": round(cap, 1),
@@ -181,6 +189,12 @@ def bench_serving(ctx, n_req: int, seed: int = 4):
         "served_vs_plain_capacity": round(cap / pcap, 2),
         "results_identical_served_and_plain": same,
         "slots": srv.slots,
+        "chunk_cap_13": {
+            "served_capacity_codes_per_s": round(cap_l, 1),
+            "offered_rate_per_s": round(0.7 * cap_l, 1),
+            "p50_ms": round(float(np.percentile(lat_l, 50)) * 1e3, 4),
+            "p99_ms": round(float(np.percentile(lat_l, 99)) * 1e3, 4),
+        },
         "kernel_counters_both_runs": {k: round(v, 4) for k, v in stats.items()},
     })
     return out
@@ -475,6 +489,10 @@ def main() -> None:
         log(f"[bench] serving: {s['codes_per_s']:,.0f} codes/s at {s['offered_rate_per_s']:,.0f}/s "
             f"offered, p50 {s['p50_ms']} ms, p99 {s['p99_ms']} ms; plain capacity "
             f"{s['plain_capacity_codes_per_s']:,.0f} codes/s")
+        c = s["chunk_cap_13"]
+        log(f"[bench] serving, large-k chunks capped at 2^13: capacity "
+            f"{c['served_capacity_codes_per_s']:,.0f} codes/s, p50 {c['p50_ms']} ms, "
+            f"p99 {c['p99_ms']} ms at 0.7x")
 
     if "evolve" not in skip:
         ks = [int(x) for x in args.evolve_k.split(",") if x]

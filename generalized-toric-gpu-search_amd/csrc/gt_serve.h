@@ -7,8 +7,8 @@
 
 namespace gt {
 
-constexpr int kServeMaxK = 16;
-constexpr int kServeRowWords = 3 * kServeMaxK * kRowWords;   // bit rows of one slot
+constexpr int kServeMaxK = 16;   // the indices of a request fit one uint4
+constexpr int kServeRowWords = 3 * kServeMaxK * kRowWords;   // bit rows of one request
 
 // One request descriptor (fine-grained host memory, written by the host).
 struct ServeReq {
@@ -73,7 +73,7 @@ struct ServeDev {
   uint32_t* s_nch;
   uint64_t* s_seq;
   uint64_t* ord_start;  // 4 * slots tagged entries (ord_entry), request r at r % (4 * slots)
-  uint32_t* rows;       // slots * kServeRowWords
+  uint4* s_idx;         // per slot: the request's exponent indices, one byte each
   uint64_t stats[kServeStats];
 };
 

@@ -228,7 +228,7 @@ int gt_serve_start(int32_t device, int32_t slots, int64_t work_budget, const int
       sv->dalloc(&dh.s_state, slots) ||
       sv->dalloc(&dh.s_nch, slots) || sv->dalloc(&dh.s_seq, slots) ||
       sv->dalloc(&dh.ord_start, (size_t)4 * slots) ||
-      sv->dalloc(&dh.rows, (size_t)slots * kServeRowWords) || sv->dalloc(&sv->d, 1) ||
+      sv->dalloc(&dh.s_idx, (size_t)slots) || sv->dalloc(&sv->d, 1) ||
       sv->dalloc(&sv->mplanes, (size_t)kNpts * kRowWords))
     return bail(-1);
   // Evaluation-matrix bit planes (same construction as the batch path).

@@ -2,14 +2,18 @@
 //
 // Requests (k <= kServeMaxK exponent points and a threshold tmax) arrive in a
 // ring in fine-grained host memory.  Wave 0 of workgroup 0 is the dispatcher:
-// it polls the ring's publish counter, builds the bit rows of each new request
-// into the request's slot (slot = request index mod slots), appends the
-// request's chunk range to the global chunk sequence and publishes the new
-// chunk total.  Every other wave is a worker: it claims the next global chunk
-// id with one atomic, waits until that id is published, finds the request by
-// binary search over the in-flight window, walks the chunk (the Gray walk of
-// the batch kernel, rows staged in LDS), and folds the wave minimum into the
-// slot.  The wave that finishes the last chunk of a request
+// it polls the ring's publish counter, copies each new request's exponent
+// indices (16 bytes) and state into the request's slot (slot = request index
+// mod slots), appends the request's chunk range to the global chunk sequence
+// and publishes the new chunk total.  Every other wave is a worker: it claims
+// the next global chunk id with one atomic, waits until that id is published,
+// finds the request by binary search over the in-flight window, builds the
+// request's bit rows in its LDS area (lane 3i + b: generator i times x^b, from
+// the evaluation planes), walks the chunk (the Gray walk of the batch kernel)
+// and folds the wave minimum into the slot.  Building the rows in the workers
+// costs them what copying prebuilt rows did and keeps the dispatcher, one wave
+// for the whole device, to a few stores per request.  The wave that finishes
+// the last chunk of a request
 // writes its result and sequence number to host memory.
 //
 // Requests that arrive while others run are admitted as soon as a slot and
@@ -35,8 +39,7 @@ __device__ __forceinline__ T ld_rlx(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ void dispatcher(ServeDev* d, ServeHost* h, const uint32_t* __restrict__ mplanes,
-                           uint32_t lane) {
+__device__ void dispatcher(ServeDev* d, ServeHost* h, uint32_t lane) {
   const uint64_t t_start = wall_clock64();
   uint64_t t_idle = t_start;
   uint64_t disp = 0;          // requests dispatched so far
@@ -94,22 +97,14 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, const uint32_t* __restrict
       k = qi[0];
       const int32_t q_tmax = qi[1];
       nch = (k >= 1 && k <= kServeMaxK) ? (uint32_t)code_chunks(k, d->L[k]) : 0u;
-      uint32_t* rows = d->rows + (size_t)slot * kServeRowWords;
+      uint32_t packed[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
       for (int i = 0; i < kServeMaxK; ++i) {
-        if (i >= k) break;
         int g = qi[2 + i];
-        if (g < 0 || g >= kNpts) g = 0;
-        uint32_t w[6];
-#pragma unroll
-        for (int t = 0; t < 6; ++t) w[t] = mplanes[g * kRowWords + t];
-        for (int b = 0; b < 3; ++b) {
-          uint32_t* o = rows + (3 * i + b) * kRowWords;
-#pragma unroll
-          for (int t = 0; t < 6; ++t) o[t] = w[t];
-          mul_x(w);
-        }
+        if (i >= k || g < 0 || g >= kNpts) g = 0;   // the host checked the indices
+        packed[i >> 2] |= (uint32_t)g << (8 * (i & 3));
       }
+      d->s_idx[slot] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
       d->s_k[slot] = (nch != 0u) ? k : 0;
       d->s_tmax[slot] = q_tmax;
       d->s_state[slot] = 64ull;
@@ -168,7 +163,8 @@ struct LdsRows {
 // before g; entries of requests this wave cannot see yet carry other tags
 // and start after g anyway.  The request's rows are copied into LDS once per
 // visit, and the Gray walk reads them there.
-__device__ void worker(ServeDev* d, ServeHost* h, uint32_t lane, uint32_t* lrows) {
+__device__ void worker(ServeDev* d, ServeHost* h, const uint32_t* __restrict__ mplanes,
+                       uint32_t lane, uint32_t* lrows) {
   const uint64_t slots = d->slots;
   const uint64_t ord_mask = 4ull * slots - 1ull;
   const uint64_t start_mask = (1ull << kOrdStartBits) - 1ull;
@@ -221,16 +217,24 @@ __device__ void worker(ServeDev* d, ServeHost* h, uint32_t lane, uint32_t* lrows
     uint64_t t2 = t1;
     if (!skip) {
       if (slot != staged_slot || seq != staged_seq) {
-        const uint32_t* src = d->rows + (size_t)slot * kServeRowWords;
-        const int nw = 3 * k * kRowWords;
-        uint32_t tmp[kServeRowWords / kWave];
+        // Lane 3i + b < 48 writes row 3i + b: generator i times x^b (zero for
+        // i >= k).
+        const uint4 iv = d->s_idx[slot];
+        const uint32_t i = lane / 3u, bpow = lane - 3u * i;
+        const uint32_t word = (i < 4u) ? iv.x : (i < 8u) ? iv.y : (i < 12u) ? iv.z : iv.w;
+        const uint32_t gidx = (word >> (8u * (i & 3u))) & 0xFFu;
+        uint32_t w[6];
 #pragma unroll
-        for (int i = 0; i < kServeRowWords / kWave; ++i) {
-          const int e = (int)lane + i * kWave;
-          tmp[i] = (e < nw) ? src[e] : 0u;
+        for (int t = 0; t < 6; ++t) w[t] = (i < (uint32_t)k) ? mplanes[gidx * kRowWords + t] : 0u;
+        uint32_t w1[6] = {w[0], w[1], w[2], w[3], w[4], w[5]};
+        mul_x(w1);
+        uint32_t w2[6] = {w1[0], w1[1], w1[2], w1[3], w1[4], w1[5]};
+        mul_x(w2);
+        if (lane < 3u * kServeMaxK) {
+#pragma unroll
+          for (int t = 0; t < 6; ++t)
+            lrows[lane * kRowWords + t] = (bpow == 0u) ? w[t] : (bpow == 1u) ? w1[t] : w2[t];
         }
-#pragma unroll
-        for (int i = 0; i < kServeRowWords / kWave; ++i) lrows[(int)lane + i * kWave] = tmp[i];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -295,9 +299,9 @@ __global__ __launch_bounds__(256) void serve_kernel(ServeDev* d, ServeHost* h,
   const uint32_t lane = __lane_id();
   const uint32_t wave = threadIdx.x / kWave;
   if (blockIdx.x == 0 && wave == 0)
-    dispatcher(d, h, mplanes, lane);
+    dispatcher(d, h, lane);
   else
-    worker(d, h, lane, lrows[wave]);
+    worker(d, h, mplanes, lane, lrows[wave]);
 }
 
 // Resets the counters and clears the ord_start ring (a cleared entry carries

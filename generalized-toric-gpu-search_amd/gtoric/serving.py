@@ -33,15 +33,25 @@ SERVE_L_OFFSET = 4
 
 class Server:
     def __init__(self, device: int = 0, slots: int = 4096, work_budget: int = 1 << 36,
-                 params: dict | None = None, idle_s: float = 10.0, life_s: float = 600.0):
+                 params: dict | None = None, idle_s: float = 10.0, life_s: float = 600.0,
+                 max_chunk_log2: int | None = None):
         """params: {k: (L, check_log2)}; defaults: the context's (tuned) batch
-        table with L raised by SERVE_L_OFFSET."""
+        table with L raised by SERVE_L_OFFSET.
+
+        max_chunk_log2 caps the served L (chunks of 64 x 2^L codewords) at
+        max(batch L, cap): a chunk is walked by one wave, so its length bounds
+        how fast one large request can finish.  Smaller chunks trade capacity
+        for tail latency (on the mixed-k stream, cap 13: p99 about half, capacity
+        about 0.8x)."""
         if params is None:
             ctx = _native.context(device)
             params = {}
             for k in range(MAX_K + 1):
                 L, c = ctx.get_md_params(k)
-                params[k] = (min(20, L + SERVE_L_OFFSET), c)
+                Ls = min(20, L + SERVE_L_OFFSET)
+                if max_chunk_log2 is not None:
+                    Ls = min(Ls, max(L, int(max_chunk_log2)))
+                params[k] = (Ls, c)
         L = np.array([params.get(k, (10, 2))[0] for k in range(MAX_K + 1)], dtype=np.int32)
         C = np.array([params.get(k, (10, 2))[1] for k in range(MAX_K + 1)], dtype=np.int32)
         h = ctypes.c_void_p()
