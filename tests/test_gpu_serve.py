@@ -47,6 +47,26 @@ def test_stream_matches_alone_and_oracle(ctx):
     assert np.all(lat >= 0) and wall > 0
 
 
+def test_chunk_cap_changes_chunks_not_results(ctx):
+    """max_chunk_log2 splits large requests into more, shorter chunks; the
+    results stay those of the batch kernel and the chunk count follows the
+    capped table."""
+    from gtoric.autotune import code_chunks
+    from gtoric.serving import SERVE_L_OFFSET, Server
+    sets, tmax = _requests(400, seed=57, kmax=10)
+    with Server(0, slots=256, life_s=60, max_chunk_log2=9) as srv:
+        res, _, _ = srv.run_stream(sets, tmax, np.zeros(len(sets)), timeout_s=60)
+        st = srv.stats()
+    from gtoric.distance import flatten
+    assert res.tolist() == ctx.md_batch(*flatten(sets), tmax).tolist()
+    L = {}
+    for k in range(1, 17):
+        Lb = ctx.get_md_params(k)[0]
+        L[k] = min(min(20, Lb + SERVE_L_OFFSET), max(Lb, 9))
+    want = sum(code_chunks(len(S), L[len(S)]) for S in sets if len(S))
+    assert st["chunks_walked"] + st["chunks_skipped"] == want
+
+
 def test_counters_account_for_every_chunk(ctx):
     """gt_serve_stats: every request dispatched once, and chunks walked plus
     chunks skipped equal the chunk count of the served parameter table."""
