@@ -194,20 +194,38 @@ __device__ void worker(ServeDev* d, ServeHost* h, const uint32_t* __restrict__ m
     uint64_t t1 = wall_clock64();
     st_wait += t1 - t0;
     if (stop) break;
-    // Request of chunk g.
+    // Request of chunk g: the last r in the window whose entry carries r's tag
+    // and starts at or before g (true for the oldest in-window request and
+    // monotone over the window).  64-ary search: each level probes 64
+    // entries with one load, one per lane, and keeps the last true one; two
+    // dependent loads for a window of 4096 requests instead of twelve.
     const uint64_t disp = uniform64(ld_rlx(&d->dispatched));
     uint64_t lo = (disp > slots) ? disp - slots : 0, hi = disp - 1;
-    while (lo < hi) {
-      const uint64_t mid = (lo + hi + 1) >> 1;
-      const uint64_t e = uniform64(d->ord_start[mid & ord_mask]);
-      if ((e >> kOrdStartBits) == ord_tag(mid) && (e & start_mask) <= g) lo = mid; else hi = mid - 1;
+    uint64_t step = 1;
+    while (step * kWave < hi - lo + 1) step *= kWave;
+    uint64_t e_lo = 0;
+    for (;;) {
+      const uint64_t r = lo + (uint64_t)lane * step;
+      uint64_t e = 0;
+      bool ok = false;
+      if (r <= hi) {
+        e = d->ord_start[r & ord_mask];
+        ok = (e >> kOrdStartBits) == ord_tag(r) && (e & start_mask) <= g;
+      }
+      const uint64_t bal = __ballot(ok);
+      const int last = (bal != 0) ? 63 - __builtin_clzll(bal) : 0;
+      lo += (uint64_t)last * step;
+      e_lo = readlane64(e, last);
+      if (step == 1) break;
+      hi = (lo + step - 1 < hi) ? lo + step - 1 : hi;
+      step /= kWave;
     }
     const uint32_t slot = (uint32_t)(lo & (slots - 1));
     const int k = (int)__builtin_amdgcn_readfirstlane((uint32_t)d->s_k[slot]);
     const int tmax = (int)__builtin_amdgcn_readfirstlane((uint32_t)d->s_tmax[slot]);
     const uint32_t nch = __builtin_amdgcn_readfirstlane(d->s_nch[slot]);
     const uint64_t seq = uniform64(d->s_seq[slot]);
-    const uint64_t local = g - (uniform64(d->ord_start[lo & ord_mask]) & start_mask);
+    const uint64_t local = g - (e_lo & start_mask);
     const int thr = 48 - tmax;
     uint64_t* state = &d->s_state[slot];
     // Low half of the slot state: the running minimum weight.
