@@ -139,6 +139,24 @@ def test_orbit_counts_k3(canon):
     assert sum(98784 // s for s in classes.values()) == comb(49, 3)
 
 
+@pytest.mark.parametrize("k,classes", [(4, 8), (5, 32)])
+def test_every_subset_of_size_4_and_5(canon, k, classes):
+    """All C(49, k) subsets: the number of distinct keys is the Burnside class
+    count, every key's orbit size 98,784/|Stab| agrees within its class, and
+    the orbit sizes sum to C(49, k)."""
+    from itertools import combinations
+    masks = np.fromiter((sum(1 << p for p in c) for c in combinations(range(49), k)),
+                        dtype=np.uint64, count=comb(49, k))
+    lo, hi, cm, st = canon.compute_masks(masks)
+    keys, first, counts = np.unique(lo, return_index=True, return_counts=True)
+    assert len(keys) == classes
+    orbit = 98784 // st[first]
+    assert np.array_equal(orbit, counts)          # each class appears exactly orbit-size times
+    assert int(orbit.sum()) == comb(49, k)
+    for key in keys:                               # |Stab| is a class invariant
+        assert len(np.unique(st[lo == key])) == 1
+
+
 def test_canon_errors(canon):
     with pytest.raises(ValueError):
         canon.compute(np.zeros((2, 17), dtype=np.int32))
