@@ -142,6 +142,25 @@ def test_params_do_not_change_results(dist, md_gold):
             ctx.set_md_params(k, L, c)
 
 
+@pytest.mark.parametrize("k", [3, 4])
+def test_every_subset_of_size_3_and_4(dist, k):
+    """All C(49, k) codes: exact distances equal the oracle's, and are
+    constant on AGL2 classes (the canonical kernel's keys)."""
+    from itertools import combinations
+    from math import comb
+    from gtoric.canonical import CanonicalOracle
+    sets = np.array(list(combinations(range(49), k)), dtype=np.int32)
+    assert len(sets) == comb(49, k)
+    got = np.asarray(dist.max_zeros_batch(sets.tolist(), 1))
+    exp = oracle.max_zeros_batch(sets.tolist(), np.full(len(sets), 48, dtype=np.int32))
+    assert np.array_equal(got, np.asarray(exp))
+    lo, hi, _, _ = CanonicalOracle(0).compute_full(sets)
+    keys, inv = np.unique(lo, return_inverse=True)
+    per_class = np.full(len(keys), -1)
+    per_class[inv] = got
+    assert np.array_equal(per_class[inv], got)     # one distance per class
+
+
 def test_agl2_invariance_on_gpu(dist):
     # d(S) = d(A S + b): a size-independent property at k = 8.
     from gtoric.canonical import precompute_gl2_perms
