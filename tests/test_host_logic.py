@@ -95,3 +95,20 @@ def test_bench_refuses_more_than_one_gpu():
     assert p.returncode == 2
     line = json.loads(p.stdout.strip().splitlines()[-1])
     assert line["n_gpus"] == 2 and "one GPU" in line["error"]
+
+
+def test_flatten_array_and_lists_agree():
+    """The (n, k) array fast path of distance.flatten gives what the row
+    iteration gives, including the empty batch."""
+    from gtoric.distance import flatten
+    rng = np.random.default_rng(9)
+    arr = np.array([sorted(rng.choice(49, 5, replace=False)) for _ in range(300)], dtype=np.int64)
+    f1, o1 = flatten(arr.tolist())
+    f2, o2 = flatten(arr)
+    assert f2.dtype == np.int32 and o2.dtype == np.int32
+    assert np.array_equal(f1, f2) and np.array_equal(o1, o2)
+    f3, o3 = flatten(np.zeros((0, 4), dtype=np.int32))
+    assert f3.size == 0 and o3.tolist() == [0]
+    mixed = [[0, 1, 2], [3, 4], [5, 6, 7, 8]]
+    fm, om = flatten(mixed)
+    assert fm.tolist() == [0, 1, 2, 3, 4, 5, 6, 7, 8] and om.tolist() == [0, 3, 5, 9]
