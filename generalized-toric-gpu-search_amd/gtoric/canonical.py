@@ -98,7 +98,11 @@ class CanonicalOracle:
         if k > MAX_K:
             raise ValueError("GPU canonical kernel supports k <= 16")
         lo, hi, _, _ = self._ctx.canon_sets(sets.astype(np.int32))
-        return lo.astype(object) + hi.astype(object) * (1 << 64)
+        keys = lo.astype(object)
+        nz = np.nonzero(hi)[0]             # k <= 10 keys fit 64 bits: hi is all zero
+        if nz.size:
+            keys[nz] = keys[nz] + hi[nz].astype(object) * (1 << 64)
+        return keys
 
     def compute_full(self, sets: np.ndarray):
         """(n, k) rows -> (key_lo, key_hi, canonical mask, |Stab|) arrays."""
