@@ -13,7 +13,9 @@
 
 #include "../../include/gtoric.h"
 #include "gt_bfs.h"
+#include "gt_evolve.h"
 #include "gt_launch.h"
+#include "gt_mlp.h"
 
 using namespace gt;
 
@@ -754,6 +756,291 @@ int gt_bfs_level(gt_ctx* c, gt_hashset* seen, const uint64_t* d_par, int32_t n_p
   GT_OK(c->md_enqueue(c->brows.as<int32_t>(), (int64_t)nu * k, c->boffs.as<int32_t>(), (int32_t)nu,
                       nullptr, tmax, tmax < 48, 0, 0, d_out_mz, s));
   GT_OK(c->check_ctrl(s));
+  return 0;
+}
+
+// --------------------------------------------------- evolutionary search --
+// Every generation runs on the device: one captured graph per search,
+// replayed once per generation (the population, the set of evaluated classes
+// and the log stay in HBM).
+}  // extern "C"
+
+struct gt_evolve {
+  gt_ctx* ctx = nullptr;
+  gt_mlp* ranker = nullptr;
+  MlpBufs mb{};
+  hipStream_t rstream = nullptr;
+  int k = 0, pop = 0, n_prop = 0, max_gen = 0;
+  uint64_t seed = 0, gcap = 0, scap = 0;
+  int64_t launched = 0;
+  std::vector<void*> allocs;
+  uint64_t *pop_m = nullptr, *props = nullptr, *plo = nullptr, *phi = nullptr, *pcm = nullptr;
+  uint64_t *chosen = nullptr, *clo = nullptr, *chi = nullptr, *ccm = nullptr;
+  uint64_t *gkeys = nullptr, *gtags = nullptr, *skeys = nullptr, *stags = nullptr;
+  uint64_t* log_cls = nullptr;
+  int32_t *pop_d = nullptr, *pop_age = nullptr, *pstab = nullptr, *cstab = nullptr;
+  int32_t *idx = nullptr, *offs = nullptr, *mz = nullptr;
+  int32_t *log_d = nullptr, *log_best = nullptr, *log_new = nullptr;
+  uint32_t *flag = nullptr, *keep = nullptr, *count = nullptr, *bsum = nullptr;
+  uint32_t *gcount = nullptr, *scount = nullptr, *state = nullptr;
+  float* score = nullptr;
+  uint16_t* feat = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t graph_gen = 0;
+
+  template <class T> int alloc(T** p, size_t n) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(n * sizeof(T), 256)) != hipSuccess)
+      return fail("hipMalloc of %zu evolve bytes failed", n * sizeof(T));
+    allocs.push_back(q);
+    *p = reinterpret_cast<T*>(q);
+    return 0;
+  }
+
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+
+  void release() {
+    drop_graph();
+    for (void* p : allocs) (void)hipFree(p);
+    allocs.clear();
+  }
+
+  // One generation, stream-ordered.
+  int enqueue(hipStream_t s) {
+    gt_ctx* c = ctx;
+    uint32_t* ctrl = c->ctrl.as<uint32_t>();
+    GT_HIP(launch_evo_propose(pop_m, pop_d, pop_age, pop, n_prop, seed, state, props, s));
+    GT_OK(canon_enqueue(c, props, n_prop, plo, phi, pcm, pstab, s));
+    GT_HIP(launch_hs_clear(gkeys, gtags, gcap, gcount, s));
+    GT_HIP(launch_hs_insert(pcm, n_prop, gkeys, gtags, gcap, 0, gcount, ctrl, s));
+    GT_HIP(launch_hs_first_new(pcm, n_prop, gkeys, gtags, gcap, skeys, scap, flag, s));
+    GT_HIP(launch_compact(flag, n_prop, bsum, keep, count, s));
+    if (ranker) {
+      GT_HIP(launch_mlp_features(pcm, n_prop, feat, s));
+      GT_HIP(launch_mlp_predict(mb, feat, n_prop, score, s));
+    }
+    GT_HIP(launch_evo_select(props, keep, count, ranker ? score : nullptr, pop, n_prop, k, seed,
+                             state, chosen, s));
+    GT_OK(canon_enqueue(c, chosen, pop, clo, chi, ccm, cstab, s));
+    GT_HIP(launch_hs_insert(ccm, pop, skeys, stags, scap, 0, scount, ctrl, s));
+    GT_HIP(launch_masks_to_rows(chosen, pop, k, idx, offs, ctrl, s));
+    GT_OK(c->md_enqueue(idx, (int64_t)pop * k, offs, pop, nullptr, 48, false, 0, 0, mz, s));
+    GT_HIP(launch_evo_survive(pop_m, pop_d, pop_age, chosen, ccm, mz, count, pop, max_gen, state,
+                              log_cls, log_d, log_best, log_new, s));
+    return 0;
+  }
+
+  // (Re)capture the generation graph on the context stream; needed again
+  // after a context buffer it reads was reallocated.
+  int capture() {
+    gt_ctx* c = ctx;
+    drop_graph();
+    GT_OK(c->upload_params(c->stream));
+    GT_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->bstate.frozen = true;
+    const int r = enqueue(c->stream);
+    c->bstate.frozen = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r != 0) {
+      if (g) (void)hipGraphDestroy(g);
+      return r;
+    }
+    GT_HIP(e);
+    const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      exec = nullptr;
+      return fail("hipGraphInstantiate: %s", hipGetErrorString(ei));
+    }
+    graph = g;
+    graph_gen = c->bstate.gen;
+    return 0;
+  }
+};
+
+extern "C" {
+
+int gt_evolve_create(gt_ctx* c, int32_t k, int32_t pop, int32_t n_prop, uint64_t seed,
+                     int32_t max_generations, int32_t log2_seen, gt_mlp* ranker,
+                     const uint64_t* init_masks, gt_evolve** out) {
+  if (!c || !out || !init_masks) return fail("null argument");
+  *out = nullptr;
+  if (k < 2 || k > kMaxKCanon) return fail("set size must be in 2..%d (got %d)", kMaxKCanon, k);
+  if (pop < 1 || pop > kEvoMaxPop) return fail("population must be in 1..%d", kEvoMaxPop);
+  if (n_prop < pop || n_prop > kEvoMaxProp)
+    return fail("proposal count must be in pop..%d", kEvoMaxProp);
+  if (max_generations < 0 || max_generations >= (1 << 24))
+    return fail("max_generations must be in 0..2^24-1");
+  const int64_t entries = (int64_t)pop * (max_generations + 1);
+  if (log2_seen == 0) {
+    log2_seen = 12;
+    while ((1ll << log2_seen) < 2 * entries && log2_seen < 30) ++log2_seen;
+  }
+  if (log2_seen < 4 || log2_seen > 30 || (1ll << log2_seen) <= entries)
+    return fail("seen-set capacity 2^%d cannot hold %lld classes", log2_seen, (long long)entries);
+  for (int i = 0; i < pop; ++i)
+    if (__builtin_popcountll(init_masks[i]) != k || (init_masks[i] >> kNpts) != 0)
+      return fail("initial set %d is not a %d-subset of the 49 points", i, k);
+  if (ranker && mlp_device(ranker) != c->device)
+    return fail("the ranker lives on another device");
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  gt_evolve* e = new gt_evolve();
+  e->ctx = c;
+  e->ranker = ranker;
+  if (ranker) {
+    e->mb = *mlp_buffers(ranker);
+    e->rstream = mlp_stream(ranker);
+  }
+  e->k = k;
+  e->pop = pop;
+  e->n_prop = n_prop;
+  e->max_gen = max_generations;
+  e->seed = seed;
+  e->scap = 1ull << log2_seen;
+  e->gcap = 64;
+  while (e->gcap < 2ull * (uint64_t)n_prop) e->gcap <<= 1;
+  const size_t P = (size_t)pop, N = (size_t)n_prop, slots = (size_t)max_generations + 1;
+  auto bail = [&](int r) {
+    e->release();
+    delete e;
+    return r;
+  };
+  int r = 0;
+  if ((r = e->alloc(&e->pop_m, P)) || (r = e->alloc(&e->pop_d, P)) || (r = e->alloc(&e->pop_age, P)) ||
+      (r = e->alloc(&e->props, N)) || (r = e->alloc(&e->plo, N)) || (r = e->alloc(&e->phi, N)) ||
+      (r = e->alloc(&e->pcm, N)) || (r = e->alloc(&e->pstab, N)) || (r = e->alloc(&e->flag, N)) ||
+      (r = e->alloc(&e->keep, N)) || (r = e->alloc(&e->count, 64)) ||
+      (r = e->alloc(&e->bsum, (N + 1023) / 1024 + 1)) || (r = e->alloc(&e->chosen, P)) ||
+      (r = e->alloc(&e->clo, P)) || (r = e->alloc(&e->chi, P)) || (r = e->alloc(&e->ccm, P)) ||
+      (r = e->alloc(&e->cstab, P)) || (r = e->alloc(&e->idx, P * k)) ||
+      (r = e->alloc(&e->offs, P + 1)) || (r = e->alloc(&e->mz, P)) ||
+      (r = e->alloc(&e->gkeys, e->gcap)) || (r = e->alloc(&e->gtags, e->gcap)) ||
+      (r = e->alloc(&e->gcount, 64)) || (r = e->alloc(&e->skeys, e->scap)) ||
+      (r = e->alloc(&e->stags, e->scap)) || (r = e->alloc(&e->scount, 64)) ||
+      (r = e->alloc(&e->state, 64)) || (r = e->alloc(&e->log_cls, slots * P)) ||
+      (r = e->alloc(&e->log_d, slots * P)) || (r = e->alloc(&e->log_best, slots)) ||
+      (r = e->alloc(&e->log_new, slots)))
+    return bail(r);
+  if (ranker && ((r = e->alloc(&e->score, N)) || (r = e->alloc(&e->feat, N * kMlpIn))))
+    return bail(r);
+  // Initial population: canonical forms, seen set, exact distances, log slot
+  // 0.  The same calls size the context buffers the captured graph uses.
+  hipStream_t s = c->stream;
+  uint32_t* ctrl = c->ctrl.as<uint32_t>();
+  auto init = [&]() -> int {
+    GT_HIP(hipMemcpyAsync(e->pop_m, init_masks, P * 8, hipMemcpyHostToDevice, s));
+    GT_HIP(hipMemcpyAsync(e->chosen, init_masks, P * 8, hipMemcpyHostToDevice, s));
+    for (size_t o = 0; o < N; o += P)
+      GT_HIP(hipMemcpyAsync(e->props + o, init_masks, std::min(P, N - o) * 8,
+                            hipMemcpyHostToDevice, s));
+    GT_HIP(launch_hs_clear(e->skeys, e->stags, e->scap, e->scount, s));
+    GT_HIP(launch_hs_clear(e->gkeys, e->gtags, e->gcap, e->gcount, s));
+    GT_OK(canon_enqueue(c, e->props, n_prop, e->plo, e->phi, e->pcm, e->pstab, s));
+    GT_OK(canon_enqueue(c, e->chosen, pop, e->clo, e->chi, e->ccm, e->cstab, s));
+    GT_HIP(launch_hs_insert(e->ccm, pop, e->skeys, e->stags, e->scap, 0, e->scount, ctrl, s));
+    GT_HIP(launch_masks_to_rows(e->chosen, pop, k, e->idx, e->offs, ctrl, s));
+    GT_OK(c->md_enqueue(e->idx, (int64_t)pop * k, e->offs, pop, nullptr, 48, false, 0, 0, e->mz, s));
+    GT_HIP(launch_evo_init(e->ccm, e->mz, pop, e->pop_d, e->pop_age, e->state, e->log_cls,
+                           e->log_d, e->log_best, e->log_new, s));
+    GT_OK(c->check_ctrl(s));
+    return e->capture();
+  };
+  if ((r = init()) != 0) return bail(r);
+  *out = e;
+  return 0;
+}
+
+int gt_evolve_run(gt_evolve* e, int32_t generations) {
+  if (!e) return fail("null search");
+  if (generations < 0) return fail("negative generation count");
+  if (e->launched + generations > e->max_gen)
+    return fail("%lld generations would exceed max_generations = %d",
+                (long long)(e->launched + generations), e->max_gen);
+  gt_ctx* c = e->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  if (generations == 0) return 0;
+  if (!e->exec || e->graph_gen != c->bstate.gen) GT_OK(e->capture());
+  GT_OK(c->upload_params(c->stream));
+  if (e->rstream) {
+    // Order the generations after ranker work already queued (training).
+    hipEvent_t ev;
+    GT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    GT_HIP(hipEventRecord(ev, e->rstream));
+    GT_HIP(hipStreamWaitEvent(c->stream, ev, 0));
+    GT_HIP(hipEventDestroy(ev));
+  }
+  for (int32_t i = 0; i < generations; ++i) GT_HIP(hipGraphLaunch(e->exec, c->stream));
+  e->launched += generations;
+  return 0;
+}
+
+int gt_evolve_sync(gt_evolve* e, int32_t* generations_done) {
+  if (!e) return fail("null search");
+  gt_ctx* c = e->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  GT_OK(c->check_ctrl(c->stream));
+  if (generations_done) *generations_done = (int32_t)e->launched;
+  return 0;
+}
+
+int gt_evolve_log(gt_evolve* e, int32_t slot0, int32_t slot1, uint64_t* classes, int32_t* dists,
+                  int32_t* best, int32_t* n_new) {
+  if (!e) return fail("null search");
+  if (slot0 < 0 || slot1 < slot0 || slot1 > e->launched + 1)
+    return fail("log slots [%d, %d) outside [0, %lld]", slot0, slot1, (long long)e->launched + 1);
+  gt_ctx* c = e->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  hipStream_t s = c->stream;
+  const size_t P = (size_t)e->pop, ns = (size_t)(slot1 - slot0);
+  if (ns == 0) return 0;
+  if (classes)
+    GT_HIP(hipMemcpyAsync(classes, e->log_cls + slot0 * P, ns * P * 8, hipMemcpyDeviceToHost, s));
+  if (dists)
+    GT_HIP(hipMemcpyAsync(dists, e->log_d + slot0 * P, ns * P * 4, hipMemcpyDeviceToHost, s));
+  if (best) GT_HIP(hipMemcpyAsync(best, e->log_best + slot0, ns * 4, hipMemcpyDeviceToHost, s));
+  if (n_new) GT_HIP(hipMemcpyAsync(n_new, e->log_new + slot0, ns * 4, hipMemcpyDeviceToHost, s));
+  GT_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int gt_evolve_population(gt_evolve* e, uint64_t* masks, int32_t* dists, int32_t* age,
+                         int64_t* seen_classes) {
+  if (!e) return fail("null search");
+  gt_ctx* c = e->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GT_OK(c->activate());
+  hipStream_t s = c->stream;
+  const size_t P = (size_t)e->pop;
+  uint32_t seen = 0;
+  if (masks) GT_HIP(hipMemcpyAsync(masks, e->pop_m, P * 8, hipMemcpyDeviceToHost, s));
+  if (dists) GT_HIP(hipMemcpyAsync(dists, e->pop_d, P * 4, hipMemcpyDeviceToHost, s));
+  if (age) GT_HIP(hipMemcpyAsync(age, e->pop_age, P * 4, hipMemcpyDeviceToHost, s));
+  GT_HIP(hipMemcpyAsync(&seen, e->scount, 4, hipMemcpyDeviceToHost, s));
+  GT_HIP(hipStreamSynchronize(s));
+  if (seen_classes) *seen_classes = seen;
+  return 0;
+}
+
+int gt_evolve_destroy(gt_evolve* e) {
+  if (!e) return 0;
+  {
+    std::lock_guard<std::mutex> lk(e->ctx->mu);
+    (void)e->ctx->activate();
+    (void)hipStreamSynchronize(e->ctx->stream);
+    e->release();
+  }
+  delete e;
   return 0;
 }
 

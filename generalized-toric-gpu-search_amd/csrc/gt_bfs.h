@@ -14,6 +14,16 @@ hipError_t launch_hs_clear(uint64_t* table, uint64_t* tags, uint64_t cap, uint32
 hipError_t launch_hs_dedup(const uint64_t* d_keys, int32_t n, uint64_t* table, uint64_t* tags,
                            uint64_t cap, uint64_t call, uint32_t* count, uint32_t* d_flag,
                            uint32_t* d_ctrl, hipStream_t s);
+// Insert only (tags as for launch_hs_dedup; call 0 gives tag = index).
+hipError_t launch_hs_insert(const uint64_t* d_keys, int32_t n, uint64_t* table, uint64_t* tags,
+                           uint64_t cap, uint64_t call, uint32_t* count, uint32_t* d_ctrl,
+                           hipStream_t s);
+// d_flag[i] = first occurrence of key i in the batch table `first` (filled by
+// launch_hs_insert with call 0) and absent from `seen` (read only).
+hipError_t launch_hs_first_new(const uint64_t* d_keys, int32_t n, const uint64_t* first,
+                               const uint64_t* first_tags, uint64_t first_cap,
+                               const uint64_t* seen, uint64_t seen_cap, uint32_t* d_flag,
+                               hipStream_t s);
 hipError_t launch_compact(const uint32_t* d_flag, int32_t n, uint32_t* d_bsum, uint32_t* d_out,
                           uint32_t* d_total, hipStream_t s);
 hipError_t launch_gather_u64(const uint64_t* src, const uint32_t* idx, const uint32_t* count,

@@ -39,4 +39,13 @@ hipError_t launch_mlp_cast(const MlpBufs& b, hipStream_t s);
 hipError_t launch_bf16_gemm(const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K,
                             hipStream_t s);
 
+// Accessors for other translation units (the evolutionary search reads the
+// ranker's parameters inside its captured generation).
+}  // namespace gt
+struct gt_mlp;
+namespace gt {
+const MlpBufs* mlp_buffers(gt_mlp* m);
+hipStream_t mlp_stream(gt_mlp* m);
+int mlp_device(const gt_mlp* m);
+
 }  // namespace gt

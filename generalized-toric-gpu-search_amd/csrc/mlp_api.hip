@@ -87,6 +87,12 @@ struct gt_mlp {
   }
 };
 
+namespace gt {
+const MlpBufs* mlp_buffers(gt_mlp* m) { return &m->b; }
+hipStream_t mlp_stream(gt_mlp* m) { return m->stream; }
+int mlp_device(const gt_mlp* m) { return m->device; }
+}  // namespace gt
+
 extern "C" {
 
 int gt_mlp_num_params(int32_t* n) {

@@ -74,6 +74,12 @@ _SIGS = {
     "gt_dedup_dev": ([vp, vp, i32, vp, vp], ctypes.c_int),
     "gt_compact_dev": ([vp, vp, i32, vp, vp, vp], ctypes.c_int),
     "gt_bfs_level": ([vp, vp, vp, i32, i32, i32, vp, vp, i32, P(i32)], ctypes.c_int),
+    "gt_evolve_create": ([vp, i32, i32, i32, u64, i32, i32, vp, vp, P(vp)], ctypes.c_int),
+    "gt_evolve_run": ([vp, i32], ctypes.c_int),
+    "gt_evolve_sync": ([vp, P(i32)], ctypes.c_int),
+    "gt_evolve_log": ([vp, i32, i32, vp, vp, vp, vp], ctypes.c_int),
+    "gt_evolve_population": ([vp, vp, vp, vp, P(i64)], ctypes.c_int),
+    "gt_evolve_destroy": ([vp], ctypes.c_int),
     "gt_dev_alloc": ([vp, i64, P(vp)], ctypes.c_int),
     "gt_dev_free": ([vp, vp], ctypes.c_int),
     "gt_memcpy_h2d": ([vp, vp, vp, i64, vp], ctypes.c_int),

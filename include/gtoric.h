@@ -198,6 +198,45 @@ int gt_bfs_level(gt_ctx* ctx, gt_hashset* seen, const uint64_t* d_parents, int32
                  int32_t kp, int32_t tmax, uint64_t* d_out_masks, int32_t* d_out_mz,
                  int32_t cap, int32_t* n_new);
 
+/* Evolutionary search with every generation on the device (replaces the
+ * generation loop of run_evolution, evolve_search.py:322-539, with exact
+ * scores).  A generation: n_prop children, each a 1- or 2-point swap of a
+ * size-4 tournament winner (highest 1000 d - age); their canonical masks;
+ * the classes not evaluated before (first occurrence in proposal order);
+ * pop of them chosen in proposal order or, with a ranker, by the ranker's
+ * score (ties: proposal order), random k-sets filling a short generation;
+ * exact distances of the chosen; survivors = best pop of parents and
+ * children by distance, then age.  Random draws are a function of (seed,
+ * generation, index), so a search is reproducible.  One captured graph is
+ * replayed per generation; the population, the hash set of evaluated
+ * classes and the per-generation log stay in device memory.
+ *   - 2 <= k <= 16, 1 <= pop <= 4096, pop <= n_prop <= 8192.
+ *   - init_masks: pop host masks of size k (the initial population, slot 0).
+ *   - log2_seen: capacity of the evaluated-class set (0 = chosen from
+ *     pop * (max_generations + 1)).
+ *   - ranker (may be NULL): its parameters are read by every generation;
+ *     update them only between gt_evolve_sync and the next gt_evolve_run.
+ * Destroy a search before closing its context. */
+typedef struct gt_evolve gt_evolve;
+int gt_evolve_create(gt_ctx* ctx, int32_t k, int32_t pop, int32_t n_prop, uint64_t seed,
+                     int32_t max_generations, int32_t log2_seen, gt_mlp* ranker,
+                     const uint64_t* init_masks, gt_evolve** out);
+/* Enqueue `generations` more generations (asynchronous). */
+int gt_evolve_run(gt_evolve* ev, int32_t generations);
+/* Wait for them; device-side errors are reported here. */
+int gt_evolve_sync(gt_evolve* ev, int32_t* generations_done);
+/* Log slots [slot0, slot1): slot 0 is the initial population, slot g the sets
+ * evaluated by generation g (pop canonical masks and exact distances each),
+ * best[] the population's best distance after it, n_new[] its count of new
+ * classes among the proposals.  Any output may be NULL. */
+int gt_evolve_log(gt_evolve* ev, int32_t slot0, int32_t slot1, uint64_t* classes,
+                  int32_t* dists, int32_t* best, int32_t* n_new);
+/* Current population (sorted best first) and the number of classes in the
+ * evaluated set. */
+int gt_evolve_population(gt_evolve* ev, uint64_t* masks, int32_t* dists, int32_t* age,
+                         int64_t* seen_classes);
+int gt_evolve_destroy(gt_evolve* ev);
+
 /* Device memory of the context's device, for *_dev callers that do not bring
  * their own allocator.  Copies are ordered on `stream` (NULL = context
  * stream) and return when the copy has finished.  gt_fill_u32_dev and

@@ -1,4 +1,6 @@
-"""Evolutionary driver: exact scores, canonical dedup, graph replays, ranking."""
+"""Evolutionary search on the device: generation by generation equal to the
+CPU model (tests/evolve_model.py: same draws, oracle canonical forms and
+distances), exact reported distances, ranking by the surrogate."""
 
 from __future__ import annotations
 
@@ -8,18 +10,78 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
+def _compare(ev, model, gens, scorer=None):
+    for g in range(1, gens + 1):
+        ev.run(1)
+        assert ev.sync() == g
+        model.step(scorer)
+        m, d, a, seen = ev.population()
+        assert np.array_equal(m, model.pop_m), g
+        assert np.array_equal(d, model.pop_d), g
+        assert np.array_equal(a, model.pop_age), g
+        assert seen == len(model.seen), g
+        cls, dist, best, new = ev.log(g, g + 1)
+        mcls, mdist, mbest, mnew = model.log[g]
+        assert np.array_equal(cls[0], mcls), g
+        assert np.array_equal(dist[0], mdist), g
+        assert best[0] == mbest and new[0] == mnew, g
+
+
+@pytest.mark.parametrize("k,pop,n_prop,gens", [(6, 64, 64, 5), (4, 16, 16, 4), (8, 32, 96, 3)])
+def test_device_generations_equal_the_model(k, pop, n_prop, gens):
+    """k = 4 has 8 classes, so its generations are filled by random sets."""
+    import evolve_model as M
+    from gtoric import _native
+    from gtoric.evolve import Evolve, random_masks
+    ctx = _native.context(0)
+    init = random_masks(np.random.default_rng(k), pop, k)
+    ev = Evolve(ctx, k, pop, n_prop, 1234 + k, gens, init)
+    try:
+        model = M.Model(k, pop, n_prop, 1234 + k, init)
+        cls, dist, best, new = ev.log(0, 1)
+        assert np.array_equal(cls[0], model.log[0][0])
+        assert np.array_equal(dist[0], model.log[0][1])
+        _compare(ev, model, gens)
+    finally:
+        ev.close()
+
+
+def test_ranked_generations_equal_the_model():
+    """With a ranker the chosen sets are the top scores of the new
+    proposals; the model takes the scores from the same network."""
+    import evolve_model as M
+    from gtoric import _native
+    from gtoric.evolve import Evolve, random_masks
+    from gtoric.surrogate import Surrogate
+    ctx = _native.context(0)
+    sur = Surrogate(0, seed=11)
+    k, pop, n_prop = 7, 48, 192
+    init = random_masks(np.random.default_rng(3), pop, k)
+    ev = Evolve(ctx, k, pop, n_prop, 77, 3, init, ranker=sur)
+    try:
+        model = M.Model(k, pop, n_prop, 77, init)
+        _compare(ev, model, 3, scorer=sur.predict)
+    finally:
+        ev.close()
+        sur.close()
+
+
 def test_evolve_reports_exact_distances():
     from gtoric import _native
     from gtoric.distance import flatten
     from gtoric.evolve import run
     ctx = _native.context(0)
-    res = run(6, pop=256, generations=8, seed=3, ctx=ctx)
+    res = run(6, pop=256, generations=8, seed=3, ctx=ctx, burst=3)
     assert res.generations == 8 and res.evaluated == 256 * 9
     assert len(res.best_set) == 6
     d = 49 - ctx.md_batch(*flatten([res.best_set]), 48)[0]
     assert d == res.best_d
     best = [h[1] for h in res.history]
-    assert best == sorted(best)
+    assert len(best) == 8 and best == sorted(best)
+    sel = np.random.default_rng(0).choice(len(res.classes), 200, replace=False)
+    from gtoric.evolve import mask_points
+    sets = [mask_points(c) for c in res.classes[sel]]
+    assert np.array_equal(49 - ctx.md_batch(*flatten(sets), 48), res.dists[sel])
 
 
 def test_evolve_with_surrogate_runs_and_is_exact():
@@ -32,3 +94,27 @@ def test_evolve_with_surrogate_runs_and_is_exact():
     res = run(7, pop=256, generations=6, seed=4, surrogate=s, ctx=ctx, train_every=2)
     d = 49 - ctx.md_batch(*flatten([res.best_set]), 48)[0]
     assert d == res.best_d and res.generations == 6
+    s.close()
+
+
+def test_evolve_errors():
+    from gtoric import _native
+    from gtoric.evolve import Evolve, random_masks
+    ctx = _native.context(0)
+    init = random_masks(np.random.default_rng(0), 8, 5)
+    with pytest.raises(_native.GtError):
+        Evolve(ctx, 5, 8, 4, 0, 2, init)                 # n_prop < pop
+    with pytest.raises(_native.GtError):
+        Evolve(ctx, 6, 8, 8, 0, 2, init)                 # sets are not of size k
+    with pytest.raises(_native.GtError):
+        Evolve(ctx, 5, 8, 9000, 0, 2, init)              # above 8192 proposals
+    ev = Evolve(ctx, 5, 8, 8, 0, 2, init)
+    try:
+        with pytest.raises(_native.GtError):
+            ev.run(3)                                    # beyond max_generations
+        ev.run(2)
+        assert ev.sync() == 2
+        with pytest.raises(_native.GtError):
+            ev.log(0, 4)
+    finally:
+        ev.close()

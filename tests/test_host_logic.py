@@ -7,17 +7,90 @@ import json
 import numpy as np
 
 
-def test_mutation_keeps_size_and_changes_points():
-    from gtoric.evolve import masks_to_flat, mutate, random_masks
+def test_device_generation_model_mutation():
+    """The CPU model of the device generation: children keep size k, differ
+    from their parent by a 1- or 2-point swap, about half of each."""
+    import evolve_model as M
+    from gtoric.evolve import random_masks
     rng = np.random.default_rng(0)
-    par = random_masks(rng, 500, 9)
-    ch = mutate(rng, par)
-    pc = np.array([bin(int(m)).count("1") for m in ch])
-    assert np.all(pc == 9)
-    diff = np.array([bin(int(a) ^ int(b)).count("1") for a, b in zip(par, ch)])
-    assert set(diff.tolist()) == {2, 4}
-    flat = masks_to_flat(ch[:3], 9).reshape(3, 9)
-    assert all(sum(1 << int(p) for p in row) == int(m) for row, m in zip(flat, ch[:3]))
+    for k in (2, 9, 16, 47):
+        par = random_masks(rng, 500, k)
+        d = np.full(500, 30)
+        age = np.zeros(500, dtype=np.int64)
+        ch = M.propose(7, 3, par, d, age, 2000)
+        assert np.all(np.bitwise_count(ch) == k) and np.all(ch >> np.uint64(49) == 0)
+        # Equal scores: the tournament keeps its first candidate.
+        parent = par[M.below(M.evo_rand(7, 3, np.arange(2000), 0), 500)]
+        out, add = parent & ~ch, ch & ~parent
+        two = (M.evo_rand(7, 3, np.arange(2000), 8) >> np.uint64(63)) != 0
+        assert np.array_equal(np.bitwise_count(out), np.where(two, 2, 1))
+        assert np.array_equal(np.bitwise_count(add), np.where(two, 2, 1))
+        assert 0.4 < two.mean() < 0.6
+
+
+def test_device_generation_model_draws():
+    """evo_rand equals a scalar restatement (wrapping 64-bit arithmetic);
+    below() stays in range; nth_bit finds the t-th set bit; random_set gives
+    uniform k-subsets (marginals within 5 sigma)."""
+    import evolve_model as M
+    mask64 = (1 << 64) - 1
+
+    def scalar(seed, g, i, j):
+        z = seed ^ (g << 40) ^ (i << 8) ^ j
+        z = (z + 0x9E3779B97F4A7C15) & mask64
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask64
+        return z ^ (z >> 31)
+
+    got = M.evo_rand(12345, 77, np.arange(1000), 5)
+    assert [int(x) for x in got] == [scalar(12345, 77, i, 5) for i in range(1000)]
+    r = M.evo_rand(1, 2, np.arange(100000), 3)
+    b = M.below(r, 7)
+    assert b.min() == 0 and b.max() == 6
+    m = np.array([0b1011010, (1 << 48) | (1 << 31) | 1], dtype=np.uint64)
+    assert M.nth_bit(m, np.array([2, 1])).tolist() == [4, 31]
+    for k in (1, 6, 48):
+        sets = M.random_set(9, 4, np.arange(40000), k)
+        assert np.all(np.bitwise_count(sets) == k)
+        freq = np.array([((sets >> np.uint64(p)) & np.uint64(1)).sum() for p in range(49)])
+        p = k / 49
+        sigma = np.sqrt(40000 * p * (1 - p)) if 0 < p < 1 else 1
+        assert np.all(np.abs(freq - 40000 * p) < 5 * sigma + 1e-9), k
+
+
+def test_device_generation_model_step():
+    """Five generations of the model at k = 4 (8 classes: the class space runs
+    out and random sets fill the generation): population sorted by distance
+    then age, each generation's chosen classes new unless they fill."""
+    import evolve_model as M
+    from gtoric.evolve import random_masks
+    rng = np.random.default_rng(4)
+    m = M.Model(4, 16, 32, 99, random_masks(rng, 16, 4))
+    for _ in range(5):
+        seen_before = set(m.seen)
+        m.step()
+        cls, d, best, cnt = m.log[-1]
+        real = min(cnt, 16)
+        assert not (set(cls[:real].tolist()) & seen_before)
+        assert len(set(cls[:real].tolist())) == real
+        assert best == m.pop_d[0]
+        key = list(zip((-m.pop_d).tolist(), m.pop_age.tolist()))
+        assert key == sorted(key)
+        assert len(m.seen) <= 8
+    assert len(m.seen) == 8
+
+
+def test_random_masks_are_uniform_k_subsets():
+    from gtoric.evolve import mask_points, random_masks
+    rng = np.random.default_rng(1)
+    for k in (0, 1, 5, 24, 49):
+        m = random_masks(rng, 30000, k)
+        assert np.all(np.bitwise_count(m) == k)
+        if 0 < k < 49:
+            freq = np.array([((m >> np.uint64(p)) & np.uint64(1)).sum() for p in range(49)])
+            p = k / 49
+            assert np.all(np.abs(freq - 30000 * p) < 5 * np.sqrt(30000 * p * (1 - p))), k
+    assert mask_points(np.uint64(0b1000000000000000000000000000000000000000000000101)) == [0, 2, 48]
 
 
 def test_load_canonical_state(tmp_path):
@@ -52,33 +125,10 @@ def test_chunk_plan_covers_projective_space():
             assert n == code_chunks(k, L)
 
 
-def test_mask_bit_conversions_round_trip():
-    from gtoric.evolve import bits_to_masks, masks_to_bits, random_masks
-    rng = np.random.default_rng(1)
-    for k in (1, 5, 24, 49):
-        m = random_masks(rng, 300, k)
-        b = masks_to_bits(m)
-        assert b.shape == (300, 49) and np.all(b.sum(1) == k)
-        assert np.array_equal(bits_to_masks(b), m)
-        i = 17
-        assert [p for p in range(49) if b[i, p]] == [p for p in range(49) if (int(m[i]) >> p) & 1]
-
-
-def test_mutation_at_the_size_extremes():
-    from gtoric.evolve import mutate, random_masks
-    rng = np.random.default_rng(2)
-    for k in (2, 47):
-        par = random_masks(rng, 400, k)
-        ch = mutate(rng, par)
-        assert all(bin(int(m)).count("1") == k for m in ch)
-        diff = {bin(int(a) ^ int(b)).count("1") for a, b in zip(par, ch)}
-        assert diff == {2, 4}
-
-
 def test_evolve_result_discovery_counts():
     from gtoric.evolve import EvolveResult
     r = EvolveResult(k=6, best_d=36, best_set=[], generations=3, evaluated=6, wall_s=1.0,
-                     eval_s=0.5, history=[(0.1, 34), (0.4, 36), (0.9, 36)],
+                     history=[(0.1, 34), (0.4, 36), (0.9, 36)],
                      classes=np.array([5, 6, 7, 8, 9, 10], dtype=np.uint64),
                      dists=np.array([36, 35, 36, 30, 35, 34], dtype=np.int32))
     assert r.hits(36) == 2 and r.hits(35) == 4 and r.hits(37) == 0
