@@ -1,6 +1,7 @@
 // C ABI of the surrogate MLP (declared in include/gtoric.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -61,7 +62,7 @@ struct gt_mlp {
   std::mutex mu;
   MlpBufs b;
   std::vector<void*> fixed;
-  Buf masks, feat, target, out, partial;
+  Buf masks, feat, target, out, partial, losses;
   int step = 0;
 
   int features(const uint64_t* d_masks, int32_t n, hipStream_t s) {
@@ -108,7 +109,7 @@ int gt_mlp_create(int32_t device, uint64_t seed, gt_mlp** out) {
   m->device = device;
   auto bail = [&](int r) {
     for (void* p : m->fixed) (void)hipFree(p);
-    for (Buf* x : {&m->masks, &m->feat, &m->target, &m->out, &m->partial})
+    for (Buf* x : {&m->masks, &m->feat, &m->target, &m->out, &m->partial, &m->losses})
       if (x->p) (void)hipFree(x->p);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
@@ -153,7 +154,7 @@ int gt_mlp_destroy(gt_mlp* m) {
   (void)hipSetDevice(m->device);
   (void)hipStreamSynchronize(m->stream);
   for (void* p : m->fixed) (void)hipFree(p);
-  for (Buf* x : {&m->masks, &m->feat, &m->target, &m->out, &m->partial})
+  for (Buf* x : {&m->masks, &m->feat, &m->target, &m->out, &m->partial, &m->losses})
     if (x->p) (void)hipFree(x->p);
   (void)hipStreamDestroy(m->stream);
   delete m;
@@ -228,6 +229,29 @@ int gt_mlp_train_step(gt_mlp* m, const uint64_t* masks, const float* target, int
   ML_OK(gt_mlp_train_step_dev(m, m->masks.as<uint64_t>(), m->target.as<float>(), n, lr, nullptr));
   std::lock_guard<std::mutex> lk(m->mu);
   if (loss) ML_HIP(hipMemcpyAsync(loss, m->b.grads + kMlpParams, 4, hipMemcpyDeviceToHost, m->stream));
+  ML_HIP(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+int gt_mlp_train_batches(gt_mlp* m, const uint64_t* masks, const float* target, int32_t n,
+                         int32_t batch, float lr, float* losses) {
+  if (!m || !masks || !target || n <= 0 || batch <= 0) return mfail("bad argument");
+  std::lock_guard<std::mutex> lk(m->mu);
+  ML_HIP(hipSetDevice(m->device));
+  ML_OK(m->upload(masks, target, n));
+  const int nb = (n + batch - 1) / batch;
+  ML_OK(m->losses.ensure((size_t)nb * 4));
+  // Steps are queued back to back; each step's loss is copied aside before
+  // the next step overwrites it.
+  for (int b = 0; b < nb; ++b) {
+    const int r0 = b * batch, rows = std::min(batch, n - r0);
+    ML_OK(m->grads(m->masks.as<uint64_t>() + r0, m->target.as<float>() + r0, rows, m->stream));
+    ML_HIP(launch_mlp_adam(m->b, lr, ++m->step, m->stream));
+    ML_HIP(hipMemcpyAsync(m->losses.as<float>() + b, m->b.grads + kMlpParams, 4,
+                          hipMemcpyDeviceToDevice, m->stream));
+  }
+  if (losses)
+    ML_HIP(hipMemcpyAsync(losses, m->losses.p, (size_t)nb * 4, hipMemcpyDeviceToHost, m->stream));
   ML_HIP(hipStreamSynchronize(m->stream));
   return 0;
 }

@@ -62,6 +62,7 @@ _SIGS = {
     "gt_mlp_features": ([vp, vp, i32, vp], ctypes.c_int),
     "gt_mlp_grads": ([vp, vp, vp, i32, vp, P(ctypes.c_float)], ctypes.c_int),
     "gt_mlp_train_step": ([vp, vp, vp, i32, ctypes.c_float, P(ctypes.c_float)], ctypes.c_int),
+    "gt_mlp_train_batches": ([vp, vp, vp, i32, i32, ctypes.c_float, vp], ctypes.c_int),
     "gt_mlp_train_step_dev": ([vp, vp, vp, i32, ctypes.c_float, vp], ctypes.c_int),
     "gt_mlp_predict": ([vp, vp, i32, vp], ctypes.c_int),
     "gt_mlp_predict_dev": ([vp, vp, i32, vp, vp], ctypes.c_int),

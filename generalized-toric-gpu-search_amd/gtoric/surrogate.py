@@ -79,11 +79,14 @@ class Surrogate:
         losses = []
         for _ in range(epochs):
             order = rng.permutation(len(masks))
-            ep = []
-            for s in range(0, len(order), batch):
-                sel = order[s:s + batch]
-                ep.append(self.train_step(masks[sel], target[sel], lr))
-            losses.append(float(np.mean(ep)))
+            m = np.ascontiguousarray(masks[order])
+            t = np.ascontiguousarray(target[order])
+            ep = np.zeros((len(order) + batch - 1) // batch, dtype=np.float32)
+            if len(order):
+                check(lib().gt_mlp_train_batches(self.handle, ptr(m), ptr(t), len(m), int(batch),
+                                                 float(lr), ptr(ep)))
+                self.steps += len(ep)
+            losses.append(float(np.mean(ep)) if len(ep) else float("nan"))
         return losses
 
     def predict(self, masks) -> np.ndarray:

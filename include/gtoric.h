@@ -163,6 +163,12 @@ int gt_mlp_grads(gt_mlp* mlp, const uint64_t* masks, const float* target, int32_
 /* One Adam step on a batch; *loss is the loss before the step. */
 int gt_mlp_train_step(gt_mlp* mlp, const uint64_t* masks, const float* target, int32_t n,
                       float lr, float* loss);
+/* Adam steps on consecutive batches of `batch` rows (the last may be
+ * shorter), queued back to back with one synchronisation at the end;
+ * losses[b] (may be NULL) is batch b's loss before its step.  Same steps as
+ * gt_mlp_train_step on each batch in turn. */
+int gt_mlp_train_batches(gt_mlp* mlp, const uint64_t* masks, const float* target, int32_t n,
+                         int32_t batch, float lr, float* losses);
 int gt_mlp_train_step_dev(gt_mlp* mlp, const uint64_t* d_masks, const float* d_target,
                           int32_t n, float lr, void* stream);
 int gt_mlp_predict(gt_mlp* mlp, const uint64_t* masks, int32_t n, float* out);

@@ -102,3 +102,26 @@ def test_trained_network_ranks_better_than_unranked():
     assert bad_random > 0.05, bad_random
     assert rho > 0.2, rho
     assert bad_ranked <= 0.5 * bad_random, (bad_ranked, bad_random)
+
+
+def test_batched_training_equals_single_steps():
+    """gt_mlp_train_batches queues the same Adam steps as train_step on each
+    batch in turn: identical parameters and per-batch losses."""
+    import ctypes
+    from gtoric._native import check, lib, ptr
+    from gtoric.surrogate import Surrogate
+    rng = np.random.default_rng(71)
+    masks = np.array([sum(1 << int(p) for p in rng.choice(49, 7, replace=False))
+                      for _ in range(1300)], dtype=np.uint64)
+    target = rng.standard_normal(1300).astype(np.float32)
+    a, b = Surrogate(0, seed=9), Surrogate(0, seed=9)
+    try:
+        la = [a.train_step(masks[s:s + 512], target[s:s + 512], 1e-3) for s in range(0, 1300, 512)]
+        lb = np.zeros(3, dtype=np.float32)
+        check(lib().gt_mlp_train_batches(b.handle, ptr(masks), ptr(target), 1300, 512,
+                                         ctypes.c_float(1e-3), ptr(lb)))
+        assert np.array_equal(np.asarray(la, dtype=np.float32), lb)
+        assert np.array_equal(a.params(), b.params())
+    finally:
+        a.close()
+        b.close()
