@@ -961,11 +961,11 @@ int gt_evolve_create(gt_ctx* c, int32_t k, int32_t pop, int32_t n_prop, uint64_t
 int gt_evolve_run(gt_evolve* e, int32_t generations) {
   if (!e) return fail("null search");
   if (generations < 0) return fail("negative generation count");
+  gt_ctx* c = e->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
   if (e->launched + generations > e->max_gen)
     return fail("%lld generations would exceed max_generations = %d",
                 (long long)(e->launched + generations), e->max_gen);
-  gt_ctx* c = e->ctx;
-  std::lock_guard<std::mutex> lk(c->mu);
   GT_OK(c->activate());
   if (generations == 0) return 0;
   if (!e->exec || e->graph_gen != c->bstate.gen) GT_OK(e->capture());
@@ -996,10 +996,10 @@ int gt_evolve_sync(gt_evolve* e, int32_t* generations_done) {
 int gt_evolve_log(gt_evolve* e, int32_t slot0, int32_t slot1, uint64_t* classes, int32_t* dists,
                   int32_t* best, int32_t* n_new) {
   if (!e) return fail("null search");
-  if (slot0 < 0 || slot1 < slot0 || slot1 > e->launched + 1)
-    return fail("log slots [%d, %d) outside [0, %lld]", slot0, slot1, (long long)e->launched + 1);
   gt_ctx* c = e->ctx;
   std::lock_guard<std::mutex> lk(c->mu);
+  if (slot0 < 0 || slot1 < slot0 || slot1 > e->launched + 1)
+    return fail("log slots [%d, %d) outside [0, %lld]", slot0, slot1, (long long)e->launched + 1);
   GT_OK(c->activate());
   hipStream_t s = c->stream;
   const size_t P = (size_t)e->pop, ns = (size_t)(slot1 - slot0);
