@@ -9,15 +9,6 @@
 
 namespace gt {
 
-__device__ __forceinline__ uint64_t mix64(uint64_t x) {
-  x ^= x >> 33;
-  x *= 0xff51afd7ed558ccdull;
-  x ^= x >> 33;
-  x *= 0xc4ceb9fe1a85ec53ull;
-  x ^= x >> 33;
-  return x;
-}
-
 // Children of parent i are parent | bit p for the points p outside it, in
 // increasing p, at positions i * (49 - kp) + rank(p).
 __global__ void bfs_expand_kernel(const uint64_t* __restrict__ parents, int32_t n, int32_t kp,
@@ -82,40 +73,6 @@ __global__ void hs_flag_kernel(const uint64_t* __restrict__ keys, int32_t n,
       if (cur == key) { f = (tags[h] == ((call << 32) | (uint64_t)i)) ? 1u : 0u; break; }
       if (cur == 0) break;
       h = (h + 1) & mask;
-    }
-  }
-  flag[i] = f;
-}
-
-// flag[i] = 1 when key i is the first occurrence of its key in this batch
-// (the per-batch table `first` holds every key with tag = smallest index,
-// from hs_insert_kernel with call 0) and is absent from the table `seen`;
-// `seen` is only read.
-__global__ void hs_first_new_kernel(const uint64_t* __restrict__ keys, int32_t n,
-                                    const uint64_t* __restrict__ first,
-                                    const uint64_t* __restrict__ first_tags, uint64_t first_mask,
-                                    const uint64_t* __restrict__ seen, uint64_t seen_mask,
-                                    uint32_t* __restrict__ flag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint64_t key = keys[i];
-  uint32_t f = 0;
-  if (key != 0) {
-    uint64_t h = mix64(key) & first_mask;
-    for (uint64_t probe = 0; probe <= first_mask; ++probe) {
-      const uint64_t cur = first[h];
-      if (cur == key) { f = (first_tags[h] == (uint64_t)i) ? 1u : 0u; break; }
-      if (cur == 0) break;
-      h = (h + 1) & first_mask;
-    }
-    if (f) {
-      h = mix64(key) & seen_mask;
-      for (uint64_t probe = 0; probe <= seen_mask; ++probe) {
-        const uint64_t cur = seen[h];
-        if (cur == key) { f = 0; break; }
-        if (cur == 0) break;
-        h = (h + 1) & seen_mask;
-      }
     }
   }
   flag[i] = f;
@@ -246,16 +203,6 @@ hipError_t launch_hs_insert(const uint64_t* d_keys, int32_t n, uint64_t* table, 
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(hs_insert_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_keys, n, table,
                      tags, cap - 1, call, count, d_ctrl);
-  return hipGetLastError();
-}
-
-hipError_t launch_hs_first_new(const uint64_t* d_keys, int32_t n, const uint64_t* first,
-                               const uint64_t* first_tags, uint64_t first_cap,
-                               const uint64_t* seen, uint64_t seen_cap, uint32_t* d_flag,
-                               hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(hs_first_new_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_keys, n,
-                     first, first_tags, first_cap - 1, seen, seen_cap - 1, d_flag);
   return hipGetLastError();
 }
 

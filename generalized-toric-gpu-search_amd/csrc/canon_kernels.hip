@@ -114,7 +114,7 @@ static_assert(sizeof(Anchor) == 16 && offsetof(CanonTables, rowpos) % 4 == 0 &&
                   offsetof(CanonTables, toprows) % 4 == 0 && offsetof(CanonTables, amin) % 4 == 0,
               "dword-written tables must be dword aligned");
 
-__device__ void build_canon_tables(CanonTables& T) {
+__device__ __forceinline__ void build_canon_tables(CanonTables& T) {
   // Whole dwords: byte stores from neighbouring lanes into one dword conflict.
   for (int q = threadIdx.x; q < kRows * kRowStride / 4; q += blockDim.x) {
     uint32_t wd = 0;
@@ -423,6 +423,12 @@ __global__ void canon_fill_kernel(uint32_t* order, uint32_t len, uint32_t* hist)
 
 constexpr uint32_t kNoSet = 0xFFFFFFFFu;
 
+// kSorted: wave j takes set order[j] (sets grouped by size, largest first;
+// kNoSet pads each group to whole waves; canon_hist_kernel flagged bad
+// masks).  Otherwise wave j takes set j; the callers of that form pass masks
+// that are valid by construction or 0 (already flagged), and a mask whose
+// size is outside 1..16 is skipped.
+template <bool kSorted>
 __global__ __launch_bounds__(kCanonBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void canon_kernel(
     const uint64_t* __restrict__ masks, const uint32_t* __restrict__ order,
     uint32_t slots, uint64_t* __restrict__ key_lo, uint64_t* __restrict__ key_hi,
@@ -441,12 +447,14 @@ __global__ __launch_bounds__(kCanonBlock) __attribute__((amdgpu_waves_per_eu(8, 
   // (scalar loads and LDS share lgkmcnt).  Every address is in bounds: order
   // positions are checked against slots and a padding entry reads mask 0.
   uint32_t j = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-  uint32_t idx_v = (j < slots) ? order[j] : kNoSet;
+  uint32_t idx_v = (j < slots) ? (kSorted ? order[j] : j) : kNoSet;
   uint64_t mv = masks[(idx_v == kNoSet) ? 0u : idx_v];
-  uint32_t next_v = (j + nwaves < slots) ? order[j + nwaves] : kNoSet;
+  uint32_t next_v = (j + nwaves < slots) ? (kSorted ? order[j + nwaves] : j + nwaves) : kNoSet;
   for (; j < slots; j += nwaves) {
     const uint64_t mv_next = masks[(next_v == kNoSet) ? 0u : next_v];
-    const uint32_t after_v = (j + 2u * nwaves < slots) ? order[j + 2u * nwaves] : kNoSet;
+    const uint32_t after_v = (j + 2u * nwaves < slots)
+                                 ? (kSorted ? order[j + 2u * nwaves] : j + 2u * nwaves)
+                                 : kNoSet;
     const uint32_t idx = __builtin_amdgcn_readfirstlane(idx_v);
     if (idx != kNoSet) {
       const uint64_t mask = uniform64(mv);
@@ -460,7 +468,7 @@ __global__ __launch_bounds__(kCanonBlock) __attribute__((amdgpu_waves_per_eu(8, 
         GT_CANON_CASE(9) GT_CANON_CASE(10) GT_CANON_CASE(11) GT_CANON_CASE(12)
         GT_CANON_CASE(13) GT_CANON_CASE(14) GT_CANON_CASE(15) GT_CANON_CASE(16)
 #undef GT_CANON_CASE
-        default: ok = false; break;   // sizes outside 1..16 never reach the order array
+        default: ok = false; break;   // sizes outside 1..16
       }
       if (ok && lane == 0) {
         key_lo[idx] = o.key_lo;
@@ -499,11 +507,18 @@ hipError_t launch_sets_to_masks(const int32_t* d_sets, int32_t n, int32_t k, uin
 }
 
 // d_order must hold n + 16 * 64 entries; d_hist 35 entries.
-hipError_t launch_canon(const uint64_t* d_masks, int32_t n,
+hipError_t launch_canon(const uint64_t* d_masks, int32_t n, bool by_size,
                         uint32_t* d_order, uint32_t* d_hist, uint64_t* d_key_lo,
                         uint64_t* d_key_hi, uint64_t* d_canon, int32_t* d_stab,
                         uint32_t* d_ctrl, hipStream_t s) {
   if (n <= 0) return hipSuccess;
+  if (!by_size) {
+    const int grid = (int)std::min<uint32_t>(
+        kCanonMaxBlocks, ((uint32_t)n + kCanonBlock / kWave - 1) / (kCanonBlock / kWave));
+    hipLaunchKernelGGL(canon_kernel<false>, dim3(grid), dim3(kCanonBlock), 0, s, d_masks,
+                       nullptr, (uint32_t)n, d_key_lo, d_key_hi, d_canon, d_stab);
+    return hipGetLastError();
+  }
   const uint32_t slots = (uint32_t)n + (uint32_t)(kMaxKCanon + 1) * kWave;
   hipLaunchKernelGGL(canon_fill_kernel, dim3((slots + 255) / 256), dim3(256), 0, s, d_order,
                      slots, d_hist);
@@ -516,7 +531,7 @@ hipError_t launch_canon(const uint64_t* d_masks, int32_t n,
   const uint32_t waves = (uint32_t)n;
   const int grid = (int)std::min<uint32_t>(kCanonMaxBlocks,
                                            (waves + kCanonBlock / kWave - 1) / (kCanonBlock / kWave));
-  hipLaunchKernelGGL(canon_kernel, dim3(grid), dim3(kCanonBlock), 0, s, d_masks, d_order,
+  hipLaunchKernelGGL(canon_kernel<true>, dim3(grid), dim3(kCanonBlock), 0, s, d_masks, d_order,
                      slots, d_key_lo, d_key_hi, d_canon, d_stab);
   return hipGetLastError();
 }

@@ -7,6 +7,7 @@
 // chosen -> min distance -> survivors.  This file holds the steps that are
 // specific to the search; the others are the canonical, hash-set and
 // min-distance kernels.
+#include "gt_bfs.h"
 #include "gt_evolve.h"
 
 namespace gt {
@@ -125,100 +126,224 @@ __global__ __launch_bounds__(1024) void evo_select_kernel(
     chosen[j] = random_set(seed, g, (uint32_t)(n_prop + j), k);
 }
 
+// First-occurrence dedup of one generation's canonical masks in one
+// workgroup: a table of slot -> smallest index in LDS (linear probing, keys
+// compared through pcm), then "first occurrence and not in the evaluated set"
+// (read-only probe of the device hash set), then ordered compaction.
+__global__ __launch_bounds__(1024) void evo_dedup_kernel(
+    const uint64_t* __restrict__ pcm, int32_t n, int32_t slots,
+    const uint64_t* __restrict__ seen, uint64_t seen_mask, uint32_t* __restrict__ keep,
+    uint32_t* __restrict__ count) {
+  extern __shared__ uint32_t slot[];
+  __shared__ uint32_t wbase[17];
+  constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+  const uint32_t smask = (uint32_t)slots - 1u;
+  for (int h = threadIdx.x; h < slots; h += blockDim.x) slot[h] = kEmpty;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint64_t key = pcm[i];
+    uint32_t h = (uint32_t)mix64(key) & smask;
+    for (int probe = 0; probe < slots; ++probe) {
+      uint32_t cur = slot[h];
+      if (cur == kEmpty) {
+        cur = atomicCAS(&slot[h], kEmpty, (uint32_t)i);
+        if (cur == kEmpty) break;
+      }
+      if (pcm[cur] == key) {
+        atomicMin(&slot[h], (uint32_t)i);
+        break;
+      }
+      h = (h + 1u) & smask;
+    }
+  }
+  __syncthreads();
+  // Thread t owns items [t * per, t * per + per): contiguous, so the block
+  // scan of the per-thread counts gives proposal order.
+  const int per = (n + (int)blockDim.x - 1) / (int)blockDim.x;   // <= 8 (n <= 8192)
+  const int i0 = (int)threadIdx.x * per;
+  uint32_t bits = 0, cnt = 0;
+  for (int r = 0; r < per; ++r) {
+    const int i = i0 + r;
+    if (i >= n) break;
+    const uint64_t key = pcm[i];
+    uint32_t h = (uint32_t)mix64(key) & smask;
+    bool first = false;
+    for (int probe = 0; probe < slots; ++probe) {
+      const uint32_t cur = slot[h];
+      if (cur == kEmpty) break;                  // not reached: every key was inserted
+      if (pcm[cur] == key) {
+        first = (cur == (uint32_t)i);
+        break;
+      }
+      h = (h + 1u) & smask;
+    }
+    if (first) {
+      uint64_t g = mix64(key) & seen_mask;
+      for (uint64_t probe = 0; probe <= seen_mask; ++probe) {
+        const uint64_t cur = seen[g];
+        if (cur == key) {
+          first = false;
+          break;
+        }
+        if (cur == 0) break;
+        g = (g + 1) & seen_mask;
+      }
+    }
+    if (first) {
+      bits |= 1u << r;
+      ++cnt;
+    }
+  }
+  // Exclusive scan of cnt over the workgroup (lanes, then waves).
+  const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+  uint32_t inc = cnt;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t v = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += v;
+  }
+  if (lane == 63) wbase[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    const int nw = (int)(blockDim.x >> 6);
+    for (int w = 0; w < nw; ++w) {
+      const uint32_t v = wbase[w];
+      wbase[w] = run;
+      run += v;
+    }
+    wbase[16] = run;
+    *count = run;
+  }
+  __syncthreads();
+  uint32_t pos = wbase[wave] + inc - cnt;
+  for (int r = 0; r < per; ++r)
+    if ((bits >> r) & 1u) keep[pos++] = (uint32_t)(i0 + r);
+}
+
+// Survivors by counting.  Parents are sorted by (distance desc, age asc,
+// position) -- the initial population is put in that order here too -- and
+// children have age 0, so within one distance the children come first in
+// their own order and then the parents in theirs.  Destinations:
+//   child j, distance d:  P(> d) + C(> d) + (children with d before j)
+//   parent i, distance d: i + C(>= d)
+// with P/C the parent/child counts by distance.  Writing only destinations
+// below pop gives the best pop of both.  n_par = 0 sorts the initial
+// population (the children) and logs it in slot 0.
 __global__ __launch_bounds__(1024) void evo_survive_kernel(
     uint64_t* __restrict__ pop_m, int32_t* __restrict__ pop_d, int32_t* __restrict__ pop_age,
     const uint64_t* __restrict__ chosen, const uint64_t* __restrict__ chosen_cls,
     const int32_t* __restrict__ chosen_mz, const uint32_t* __restrict__ count, int32_t pop,
-    int32_t max_gen, uint32_t* __restrict__ state, uint64_t* __restrict__ log_cls,
+    int32_t n_par, int32_t max_gen, uint32_t* __restrict__ state, uint64_t* __restrict__ log_cls,
     int32_t* __restrict__ log_d, int32_t* __restrict__ log_best, int32_t* __restrict__ log_new) {
-  extern __shared__ uint64_t keys[];
+  constexpr int kD = kNpts + 1;                        // distances 0..49
+  constexpr int kChunks = kEvoMaxPop / kWave;          // 64 children per chunk
+  __shared__ uint16_t chunk_cnt[kChunks * kD];         // then: children of d in earlier chunks
+  __shared__ uint32_t cc[kD], pc[kD], cgt[kD], pgt[kD];
+  __shared__ int best_d;
   const uint32_t g = state[0];
-  const int n = 2 * pop;
-  const int n2 = pow2_at_least(n);
-  // Key = (64 - d, age, position): ascending order is best distance first,
-  // then youngest, then parents before children in their own order.
-  for (int j = threadIdx.x; j < n2; j += blockDim.x) {
-    uint64_t key = ~0ull;
-    if (j < n) {
-      int d, age;
-      if (j < pop) {
-        d = pop_d[j];
-        age = pop_age[j] + 1;
-      } else {
-        d = kNpts - chosen_mz[j - pop];
-        age = 0;
+  const bool init = n_par == 0;
+  const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+  const int nwaves = (int)(blockDim.x >> 6);
+  const int nchunks = (pop + kWave - 1) / kWave;
+  for (int t = threadIdx.x; t < kChunks * kD; t += blockDim.x) chunk_cnt[t] = 0;
+  if (threadIdx.x < kD) pc[threadIdx.x] = 0;
+  __syncthreads();
+  // Children: in-chunk stable rank by one ballot per distinct distance.
+  int cd[4], crank[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int chunk = wave + r * nwaves;
+    const int j = chunk * kWave + lane;
+    const bool valid = chunk < nchunks && j < pop;
+    cd[r] = valid ? min(max(kNpts - chosen_mz[j], 0), kNpts) : -1;
+    crank[r] = 0;
+    if (chunk < nchunks) {
+      uint64_t todo = __ballot(valid);
+      while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const int v = __shfl(cd[r], leader, 64);
+        const uint64_t m = __ballot(cd[r] == v);
+        if (cd[r] == v) crank[r] = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == leader) chunk_cnt[chunk * kD + v] = (uint16_t)__popcll(m);
+        todo &= ~m;
       }
-      key = ((uint64_t)(64 - d) << 56) | ((uint64_t)(uint32_t)age << 24) | (uint64_t)j;
     }
-    keys[j] = key;
   }
-  __syncthreads();
-  bitonic_sort(keys, n2);
-  // Gather every survivor before any write: parents are read from the
-  // arrays being overwritten.  1024 threads x 4 covers kEvoMaxPop.
-  uint64_t sm[4];
-  int sd[4], sa[4];
+  int pd[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int j = threadIdx.x + r * (int)blockDim.x;
-    sm[r] = 0;
-    sd[r] = sa[r] = 0;
-    if (j < pop) {
-      const uint64_t key = keys[j];
-      const int src = (int)(key & 0xFFFFFFu);
-      sm[r] = (src < pop) ? pop_m[src] : chosen[src - pop];
-      sd[r] = 64 - (int)(key >> 56);
-      sa[r] = (int)((key >> 24) & 0xFFFFFFFFu);
+    const int i = (int)threadIdx.x + r * (int)blockDim.x;
+    pd[r] = (i < n_par) ? min(max(pop_d[i], 0), kNpts) : -1;
+    if (pd[r] >= 0) atomicAdd(&pc[pd[r]], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kD) {
+    uint32_t run = 0;
+    for (int c = 0; c < nchunks; ++c) {
+      const uint32_t v = chunk_cnt[c * kD + threadIdx.x];
+      chunk_cnt[c * kD + threadIdx.x] = (uint16_t)run;
+      run += v;
+    }
+    cc[threadIdx.x] = run;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t rc = 0, rp = 0;
+    int top = -1;
+    for (int d = kNpts; d >= 0; --d) {
+      cgt[d] = rc;
+      pgt[d] = rp;
+      rc += cc[d];
+      rp += pc[d];
+      if (top < 0 && cc[d] + pc[d] != 0) top = d;
+    }
+    best_d = top;
+  }
+  __syncthreads();
+  // Read every source before any write: parents move within the arrays
+  // being written.
+  uint64_t vm[8];
+  int vd[8], va[8], vpos[8];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    vpos[r] = -1;
+    if (cd[r] >= 0) {
+      const int chunk = wave + r * nwaves, j = chunk * kWave + lane, d = cd[r];
+      vpos[r] = (int)(pgt[d] + cgt[d] + chunk_cnt[chunk * kD + d]) + crank[r];
+      vm[r] = chosen[j];
+      vd[r] = d;
+      va[r] = 0;
+    }
+    vpos[4 + r] = -1;
+    if (pd[r] >= 0) {
+      const int i = (int)threadIdx.x + r * (int)blockDim.x, d = pd[r];
+      vpos[4 + r] = i + (int)(cgt[d] + cc[d]);
+      vm[4 + r] = pop_m[i];
+      vd[4 + r] = d;
+      va[4 + r] = pop_age[i] + 1;
     }
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int j = threadIdx.x + r * (int)blockDim.x;
-    if (j < pop) {
-      pop_m[j] = sm[r];
-      pop_d[j] = sd[r];
-      pop_age[j] = sa[r];
+  for (int r = 0; r < 8; ++r) {
+    if (vpos[r] >= 0 && vpos[r] < pop) {
+      pop_m[vpos[r]] = vm[r];
+      pop_d[vpos[r]] = vd[r];
+      pop_age[vpos[r]] = va[r];
     }
   }
-  const uint32_t slot = g + 1;
+  const uint32_t slot = init ? 0u : g + 1;
   if (slot <= (uint32_t)max_gen) {
     for (int j = threadIdx.x; j < pop; j += blockDim.x) {
       log_cls[(size_t)slot * pop + j] = chosen_cls[j];
       log_d[(size_t)slot * pop + j] = kNpts - chosen_mz[j];
     }
     if (threadIdx.x == 0) {
-      log_best[slot] = 64 - (int)(keys[0] >> 56);
-      log_new[slot] = (int)*count;
+      log_best[slot] = best_d;
+      log_new[slot] = init ? pop : (int)*count;
     }
   }
   if (threadIdx.x == 0) state[0] = slot;
-}
-
-__global__ __launch_bounds__(1024) void evo_init_kernel(
-    const uint64_t* __restrict__ cls, const int32_t* __restrict__ mz, int32_t pop,
-    int32_t* __restrict__ pop_d, int32_t* __restrict__ pop_age, uint32_t* __restrict__ state,
-    uint64_t* __restrict__ log_cls, int32_t* __restrict__ log_d, int32_t* __restrict__ log_best,
-    int32_t* __restrict__ log_new) {
-  __shared__ int best;
-  if (threadIdx.x == 0) best = 0;
-  __syncthreads();
-  int local = 0;
-  for (int j = threadIdx.x; j < pop; j += blockDim.x) {
-    const int d = kNpts - mz[j];
-    pop_d[j] = d;
-    pop_age[j] = 0;
-    log_cls[j] = cls[j];
-    log_d[j] = d;
-    local = max(local, d);
-  }
-  atomicMax(&best, local);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    log_best[0] = best;
-    log_new[0] = pop;
-    state[0] = 0;
-  }
 }
 
 // ------------------------------------------------------------- launchers --
@@ -244,26 +369,25 @@ hipError_t launch_evo_select(const uint64_t* props, const uint32_t* keep, const 
   return hipGetLastError();
 }
 
-hipError_t launch_evo_survive(uint64_t* pop_m, int32_t* pop_d, int32_t* pop_age,
-                              const uint64_t* chosen, const uint64_t* chosen_cls,
-                              const int32_t* chosen_mz, const uint32_t* count, int32_t pop,
-                              int32_t max_gen, uint32_t* state, uint64_t* log_cls,
-                              int32_t* log_d, int32_t* log_best, int32_t* log_new,
-                              hipStream_t s) {
-  if (pop <= 0 || pop > kEvoMaxPop) return hipErrorInvalidValue;
-  const size_t lds = (size_t)pow2_at_least(2 * pop) * 8;
-  hipLaunchKernelGGL(evo_survive_kernel, dim3(1), dim3(1024), lds, s, pop_m, pop_d, pop_age,
-                     chosen, chosen_cls, chosen_mz, count, pop, max_gen, state, log_cls, log_d,
-                     log_best, log_new);
+hipError_t launch_evo_dedup(const uint64_t* pcm, int32_t n, const uint64_t* seen, uint64_t seen_cap,
+                            uint32_t* keep, uint32_t* count, hipStream_t s) {
+  if (n <= 0 || n > kEvoMaxProp) return hipErrorInvalidValue;
+  const int slots = pow2_at_least(2 * n);
+  hipLaunchKernelGGL(evo_dedup_kernel, dim3(1), dim3(1024), (size_t)slots * 4, s, pcm, n, slots,
+                     seen, seen_cap - 1, keep, count);
   return hipGetLastError();
 }
 
-hipError_t launch_evo_init(const uint64_t* cls, const int32_t* mz, int32_t pop, int32_t* pop_d,
-                           int32_t* pop_age, uint32_t* state, uint64_t* log_cls, int32_t* log_d,
-                           int32_t* log_best, int32_t* log_new, hipStream_t s) {
+hipError_t launch_evo_survive(uint64_t* pop_m, int32_t* pop_d, int32_t* pop_age,
+                              const uint64_t* chosen, const uint64_t* chosen_cls,
+                              const int32_t* chosen_mz, const uint32_t* count, int32_t pop,
+                              bool init, int32_t max_gen, uint32_t* state, uint64_t* log_cls,
+                              int32_t* log_d, int32_t* log_best, int32_t* log_new,
+                              hipStream_t s) {
   if (pop <= 0 || pop > kEvoMaxPop) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(evo_init_kernel, dim3(1), dim3(1024), 0, s, cls, mz, pop, pop_d, pop_age,
-                     state, log_cls, log_d, log_best, log_new);
+  hipLaunchKernelGGL(evo_survive_kernel, dim3(1), dim3(1024), 0, s, pop_m, pop_d, pop_age,
+                     chosen, chosen_cls, chosen_mz, count, pop, init ? 0 : pop, max_gen, state,
+                     log_cls, log_d, log_best, log_new);
   return hipGetLastError();
 }
 

@@ -452,14 +452,21 @@ int gt_md_single(gt_ctx* c, const int32_t* idx, int32_t k, int32_t tmax, int32_t
   return 0;
 }
 
+// one_size: masks of one size built on the device (valid by construction,
+// or 0 after a flagged input error) go to the kernel in input order.  Caller
+// masks of any sizes are grouped by size first, which also validates them.
 static int canon_enqueue(gt_ctx* c, const uint64_t* d_masks, int32_t n, uint64_t* d_lo,
-                         uint64_t* d_hi, uint64_t* d_canon, int32_t* d_stab, hipStream_t s) {
+                         uint64_t* d_hi, uint64_t* d_canon, int32_t* d_stab, hipStream_t s,
+                         bool one_size) {
   if (n <= 0) return 0;
-  GT_OK(c->order.ensure(((size_t)n + (kMaxKCanon + 1) * kWave) * 4));
-  GT_OK(c->hist.ensure(64 * 4));
+  const bool by_size = !one_size;
+  if (by_size) {
+    GT_OK(c->order.ensure(((size_t)n + (kMaxKCanon + 1) * kWave) * 4));
+    GT_OK(c->hist.ensure(64 * 4));
+  }
   hipEvent_t e0 = nullptr;
   GT_OK(c->prof_begin("canon", s, &e0));
-  GT_HIP(launch_canon(d_masks, n, c->order.as<uint32_t>(),
+  GT_HIP(launch_canon(d_masks, n, by_size, c->order.as<uint32_t>(),
                       c->hist.as<uint32_t>(), d_lo, d_hi, d_canon, d_stab, c->ctrl.as<uint32_t>(), s));
   GT_OK(c->prof_end("canon", s, e0));
   return 0;
@@ -484,7 +491,7 @@ static int canon_host(gt_ctx* c, const uint64_t* masks, const int32_t* sets, int
     GT_HIP(hipMemcpyAsync(c->masks.p, masks, (size_t)n * 8, hipMemcpyHostToDevice, s));
   }
   GT_OK(canon_enqueue(c, c->masks.as<uint64_t>(), n, c->klo.as<uint64_t>(), c->khi.as<uint64_t>(),
-                      c->canon.as<uint64_t>(), c->stab.as<int32_t>(), s));
+                      c->canon.as<uint64_t>(), c->stab.as<int32_t>(), s, sets != nullptr));
   if (key_lo) GT_HIP(hipMemcpyAsync(key_lo, c->klo.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
   if (key_hi) GT_HIP(hipMemcpyAsync(key_hi, c->khi.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
   if (canon_mask) GT_HIP(hipMemcpyAsync(canon_mask, c->canon.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
@@ -520,7 +527,8 @@ int gt_canon_masks_dev(gt_ctx* c, const uint64_t* d_masks, int32_t n, uint64_t* 
   if (!d_key_lo || !d_key_hi || !d_canon_mask || !d_stab) return fail("null device output");
   std::lock_guard<std::mutex> lk(c->mu);
   GT_OK(c->activate());
-  return canon_enqueue(c, d_masks, n, d_key_lo, d_key_hi, d_canon_mask, d_stab, c->pick(stream));
+  return canon_enqueue(c, d_masks, n, d_key_lo, d_key_hi, d_canon_mask, d_stab, c->pick(stream),
+                       false);
 }
 
 struct gt_graph {
@@ -735,7 +743,8 @@ int gt_bfs_level(gt_ctx* c, gt_hashset* seen, const uint64_t* d_par, int32_t n_p
     GT_OK(c->bsum.ensure((n + 1023) / 1024 * 4 + 4));
     GT_HIP(launch_bfs_expand(d_par, n_par, kp, c->children.as<uint64_t>(), c->ctrl.as<uint32_t>(), s));
     GT_OK(canon_enqueue(c, c->children.as<uint64_t>(), (int32_t)n, c->klo.as<uint64_t>(),
-                        c->khi.as<uint64_t>(), c->canon.as<uint64_t>(), c->stab.as<int32_t>(), s));
+                        c->khi.as<uint64_t>(), c->canon.as<uint64_t>(), c->stab.as<int32_t>(), s,
+                        true));
     GT_OK(dedup_enqueue(seen, c->canon.as<uint64_t>(), (int32_t)n, c->bflag.as<uint32_t>(), s));
     GT_HIP(launch_compact(c->bflag.as<uint32_t>(), (int32_t)n, c->bsum.as<uint32_t>(),
                           c->bidx.as<uint32_t>(), c->bcnt.as<uint32_t>(), s));
@@ -771,18 +780,17 @@ struct gt_evolve {
   MlpBufs mb{};
   hipStream_t rstream = nullptr;
   int k = 0, pop = 0, n_prop = 0, max_gen = 0;
-  uint64_t seed = 0, gcap = 0, scap = 0;
+  uint64_t seed = 0, scap = 0;
   int64_t launched = 0;
   std::vector<void*> allocs;
   uint64_t *pop_m = nullptr, *props = nullptr, *plo = nullptr, *phi = nullptr, *pcm = nullptr;
   uint64_t *chosen = nullptr, *clo = nullptr, *chi = nullptr, *ccm = nullptr;
-  uint64_t *gkeys = nullptr, *gtags = nullptr, *skeys = nullptr, *stags = nullptr;
+  uint64_t *skeys = nullptr, *stags = nullptr;
   uint64_t* log_cls = nullptr;
   int32_t *pop_d = nullptr, *pop_age = nullptr, *pstab = nullptr, *cstab = nullptr;
   int32_t *idx = nullptr, *offs = nullptr, *mz = nullptr;
   int32_t *log_d = nullptr, *log_best = nullptr, *log_new = nullptr;
-  uint32_t *flag = nullptr, *keep = nullptr, *count = nullptr, *bsum = nullptr;
-  uint32_t *gcount = nullptr, *scount = nullptr, *state = nullptr;
+  uint32_t *keep = nullptr, *count = nullptr, *scount = nullptr, *state = nullptr;
   float* score = nullptr;
   uint16_t* feat = nullptr;
   hipGraph_t graph = nullptr;
@@ -816,23 +824,20 @@ struct gt_evolve {
     gt_ctx* c = ctx;
     uint32_t* ctrl = c->ctrl.as<uint32_t>();
     GT_HIP(launch_evo_propose(pop_m, pop_d, pop_age, pop, n_prop, seed, state, props, s));
-    GT_OK(canon_enqueue(c, props, n_prop, plo, phi, pcm, pstab, s));
-    GT_HIP(launch_hs_clear(gkeys, gtags, gcap, gcount, s));
-    GT_HIP(launch_hs_insert(pcm, n_prop, gkeys, gtags, gcap, 0, gcount, ctrl, s));
-    GT_HIP(launch_hs_first_new(pcm, n_prop, gkeys, gtags, gcap, skeys, scap, flag, s));
-    GT_HIP(launch_compact(flag, n_prop, bsum, keep, count, s));
+    GT_OK(canon_enqueue(c, props, n_prop, plo, phi, pcm, pstab, s, true));
+    GT_HIP(launch_evo_dedup(pcm, n_prop, skeys, scap, keep, count, s));
     if (ranker) {
       GT_HIP(launch_mlp_features(pcm, n_prop, feat, s));
       GT_HIP(launch_mlp_predict(mb, feat, n_prop, score, s));
     }
     GT_HIP(launch_evo_select(props, keep, count, ranker ? score : nullptr, pop, n_prop, k, seed,
                              state, chosen, s));
-    GT_OK(canon_enqueue(c, chosen, pop, clo, chi, ccm, cstab, s));
+    GT_OK(canon_enqueue(c, chosen, pop, clo, chi, ccm, cstab, s, true));
     GT_HIP(launch_hs_insert(ccm, pop, skeys, stags, scap, 0, scount, ctrl, s));
     GT_HIP(launch_masks_to_rows(chosen, pop, k, idx, offs, ctrl, s));
     GT_OK(c->md_enqueue(idx, (int64_t)pop * k, offs, pop, nullptr, 48, false, 0, 0, mz, s));
-    GT_HIP(launch_evo_survive(pop_m, pop_d, pop_age, chosen, ccm, mz, count, pop, max_gen, state,
-                              log_cls, log_d, log_best, log_new, s));
+    GT_HIP(launch_evo_survive(pop_m, pop_d, pop_age, chosen, ccm, mz, count, pop, false, max_gen,
+                              state, log_cls, log_d, log_best, log_new, s));
     return 0;
   }
 
@@ -905,8 +910,6 @@ int gt_evolve_create(gt_ctx* c, int32_t k, int32_t pop, int32_t n_prop, uint64_t
   e->max_gen = max_generations;
   e->seed = seed;
   e->scap = 1ull << log2_seen;
-  e->gcap = 64;
-  while (e->gcap < 2ull * (uint64_t)n_prop) e->gcap <<= 1;
   const size_t P = (size_t)pop, N = (size_t)n_prop, slots = (size_t)max_generations + 1;
   auto bail = [&](int r) {
     e->release();
@@ -916,14 +919,12 @@ int gt_evolve_create(gt_ctx* c, int32_t k, int32_t pop, int32_t n_prop, uint64_t
   int r = 0;
   if ((r = e->alloc(&e->pop_m, P)) || (r = e->alloc(&e->pop_d, P)) || (r = e->alloc(&e->pop_age, P)) ||
       (r = e->alloc(&e->props, N)) || (r = e->alloc(&e->plo, N)) || (r = e->alloc(&e->phi, N)) ||
-      (r = e->alloc(&e->pcm, N)) || (r = e->alloc(&e->pstab, N)) || (r = e->alloc(&e->flag, N)) ||
-      (r = e->alloc(&e->keep, N)) || (r = e->alloc(&e->count, 64)) ||
-      (r = e->alloc(&e->bsum, (N + 1023) / 1024 + 1)) || (r = e->alloc(&e->chosen, P)) ||
+      (r = e->alloc(&e->pcm, N)) || (r = e->alloc(&e->pstab, N)) ||
+      (r = e->alloc(&e->keep, N)) || (r = e->alloc(&e->count, 64)) || (r = e->alloc(&e->chosen, P)) ||
       (r = e->alloc(&e->clo, P)) || (r = e->alloc(&e->chi, P)) || (r = e->alloc(&e->ccm, P)) ||
       (r = e->alloc(&e->cstab, P)) || (r = e->alloc(&e->idx, P * k)) ||
       (r = e->alloc(&e->offs, P + 1)) || (r = e->alloc(&e->mz, P)) ||
-      (r = e->alloc(&e->gkeys, e->gcap)) || (r = e->alloc(&e->gtags, e->gcap)) ||
-      (r = e->alloc(&e->gcount, 64)) || (r = e->alloc(&e->skeys, e->scap)) ||
+      (r = e->alloc(&e->skeys, e->scap)) ||
       (r = e->alloc(&e->stags, e->scap)) || (r = e->alloc(&e->scount, 64)) ||
       (r = e->alloc(&e->state, 64)) || (r = e->alloc(&e->log_cls, slots * P)) ||
       (r = e->alloc(&e->log_d, slots * P)) || (r = e->alloc(&e->log_best, slots)) ||
@@ -942,14 +943,14 @@ int gt_evolve_create(gt_ctx* c, int32_t k, int32_t pop, int32_t n_prop, uint64_t
       GT_HIP(hipMemcpyAsync(e->props + o, init_masks, std::min(P, N - o) * 8,
                             hipMemcpyHostToDevice, s));
     GT_HIP(launch_hs_clear(e->skeys, e->stags, e->scap, e->scount, s));
-    GT_HIP(launch_hs_clear(e->gkeys, e->gtags, e->gcap, e->gcount, s));
-    GT_OK(canon_enqueue(c, e->props, n_prop, e->plo, e->phi, e->pcm, e->pstab, s));
-    GT_OK(canon_enqueue(c, e->chosen, pop, e->clo, e->chi, e->ccm, e->cstab, s));
+    GT_OK(canon_enqueue(c, e->props, n_prop, e->plo, e->phi, e->pcm, e->pstab, s, true));
+    GT_OK(canon_enqueue(c, e->chosen, pop, e->clo, e->chi, e->ccm, e->cstab, s, true));
     GT_HIP(launch_hs_insert(e->ccm, pop, e->skeys, e->stags, e->scap, 0, e->scount, ctrl, s));
     GT_HIP(launch_masks_to_rows(e->chosen, pop, k, e->idx, e->offs, ctrl, s));
     GT_OK(c->md_enqueue(e->idx, (int64_t)pop * k, e->offs, pop, nullptr, 48, false, 0, 0, e->mz, s));
-    GT_HIP(launch_evo_init(e->ccm, e->mz, pop, e->pop_d, e->pop_age, e->state, e->log_cls,
-                           e->log_d, e->log_best, e->log_new, s));
+    GT_HIP(launch_evo_survive(e->pop_m, e->pop_d, e->pop_age, e->chosen, e->ccm, e->mz, e->count,
+                              pop, true, max_generations, e->state, e->log_cls, e->log_d,
+                              e->log_best, e->log_new, s));
     GT_OK(c->check_ctrl(s));
     return e->capture();
   };

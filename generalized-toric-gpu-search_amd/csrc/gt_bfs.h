@@ -7,6 +7,18 @@
 
 namespace gt {
 
+#if defined(__HIPCC__)
+// Slot hash of the open-addressing hash sets (the murmur3 64-bit finaliser).
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+#endif
+
 hipError_t launch_bfs_expand(const uint64_t* d_par, int32_t n, int32_t kp, uint64_t* d_child,
                              uint32_t* d_ctrl, hipStream_t s);
 hipError_t launch_hs_clear(uint64_t* table, uint64_t* tags, uint64_t cap, uint32_t* count,
@@ -18,12 +30,6 @@ hipError_t launch_hs_dedup(const uint64_t* d_keys, int32_t n, uint64_t* table, u
 hipError_t launch_hs_insert(const uint64_t* d_keys, int32_t n, uint64_t* table, uint64_t* tags,
                            uint64_t cap, uint64_t call, uint32_t* count, uint32_t* d_ctrl,
                            hipStream_t s);
-// d_flag[i] = first occurrence of key i in the batch table `first` (filled by
-// launch_hs_insert with call 0) and absent from `seen` (read only).
-hipError_t launch_hs_first_new(const uint64_t* d_keys, int32_t n, const uint64_t* first,
-                               const uint64_t* first_tags, uint64_t first_cap,
-                               const uint64_t* seen, uint64_t seen_cap, uint32_t* d_flag,
-                               hipStream_t s);
 hipError_t launch_compact(const uint32_t* d_flag, int32_t n, uint32_t* d_bsum, uint32_t* d_out,
                           uint32_t* d_total, hipStream_t s);
 hipError_t launch_gather_u64(const uint64_t* src, const uint32_t* idx, const uint32_t* count,

@@ -42,18 +42,21 @@ hipError_t launch_evo_select(const uint64_t* props, const uint32_t* keep, const 
                              const float* score, int32_t pop, int32_t n_prop, int32_t k,
                              uint64_t seed, const uint32_t* state, uint64_t* chosen,
                              hipStream_t s);
+// Proposals that are the first of their class in this generation and not in
+// the evaluated set (device hash set `seen`, read only), in proposal order,
+// into keep[0 .. *count) (one workgroup; LDS table of 2 n slots).
+hipError_t launch_evo_dedup(const uint64_t* pcm, int32_t n, const uint64_t* seen, uint64_t seen_cap,
+                            uint32_t* keep, uint32_t* count, hipStream_t s);
 // Survivors: the best pop of parents (age + 1) and children (age 0) by
 // distance, then age, then position (parents first); logs the children's
-// classes and distances in slot g + 1 and advances state[0].
+// classes and distances in slot g + 1 and advances state[0].  init: no
+// parents; the children (the initial population) are sorted into the
+// population and logged in slot 0.
 hipError_t launch_evo_survive(uint64_t* pop_m, int32_t* pop_d, int32_t* pop_age,
                               const uint64_t* chosen, const uint64_t* chosen_cls,
                               const int32_t* chosen_mz, const uint32_t* count, int32_t pop,
-                              int32_t max_gen, uint32_t* state, uint64_t* log_cls,
+                              bool init, int32_t max_gen, uint32_t* state, uint64_t* log_cls,
                               int32_t* log_d, int32_t* log_best, int32_t* log_new,
                               hipStream_t s);
-// Initial population: d = 49 - mz, age 0, log slot 0.
-hipError_t launch_evo_init(const uint64_t* cls, const int32_t* mz, int32_t pop, int32_t* pop_d,
-                           int32_t* pop_age, uint32_t* state, uint64_t* log_cls, int32_t* log_d,
-                           int32_t* log_best, int32_t* log_new, hipStream_t s);
 
 }  // namespace gt

@@ -41,7 +41,10 @@ hipError_t launch_md_sampled(const uint32_t* d_rows, const int32_t* d_offsets, i
 int md_main_blocks_per_cu(bool thresholded);
 
 // ---- canonical form ------------------------------------------------------
-hipError_t launch_canon(const uint64_t* d_masks, int32_t n,
+// by_size: group the sets by size first and flag bad masks (four small
+// kernels; needed for caller-supplied masks), else one wave per set in input
+// order for masks that are valid by construction (or 0, skipped).
+hipError_t launch_canon(const uint64_t* d_masks, int32_t n, bool by_size,
                         uint32_t* d_order, uint32_t* d_hist, uint64_t* d_key_lo,
                         uint64_t* d_key_hi, uint64_t* d_canon, int32_t* d_stab,
                         uint32_t* d_ctrl, hipStream_t s);

@@ -91,14 +91,17 @@ class Model:
 
     def __init__(self, k: int, pop: int, n_prop: int, seed: int, init_masks: np.ndarray):
         self.k, self.pop, self.n_prop, self.seed = k, pop, n_prop, seed
-        self.pop_m = np.asarray(init_masks, dtype=np.uint64).copy()
-        cls = canon_masks(self.pop_m, k)
+        init = np.asarray(init_masks, dtype=np.uint64)
+        cls = canon_masks(init, k)
         self.seen = set(cls.tolist())
-        mz = self.max_zeros(self.pop_m)
-        self.pop_d = (N - mz).astype(np.int64)
+        mz = self.max_zeros(init)
+        d = (N - mz).astype(np.int64)
+        # The population is kept sorted by (distance desc, age asc, position).
+        order = np.lexsort((np.arange(pop), -d))
+        self.pop_m, self.pop_d = init[order], d[order]
         self.pop_age = np.zeros(pop, dtype=np.int64)
         self.g = 0
-        self.log = [(cls, N - mz, int(self.pop_d.max()), pop)]
+        self.log = [(cls, N - mz, int(d.max()), pop)]
 
     def max_zeros(self, masks: np.ndarray) -> np.ndarray:
         sets = [[p for p in range(N) if (int(m) >> p) & 1] for m in masks]
