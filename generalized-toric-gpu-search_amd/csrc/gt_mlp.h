@@ -34,7 +34,12 @@ hipError_t launch_mlp_predict(const MlpBufs& b, const uint16_t* d_feat, int32_t 
                               hipStream_t s);
 hipError_t launch_mlp_grads(const MlpBufs& b, const uint16_t* d_feat, const float* d_target,
                             int32_t n, float* d_partial, hipStream_t s);
-hipError_t launch_mlp_adam(const MlpBufs& b, float lr, int step, hipStream_t s);
+// Gradient of one batch and the Adam step (lr, step >= 1) in two kernels;
+// the batch loss (before the step) goes to grads[kMlpParams] and, if
+// d_loss != NULL, to *d_loss.
+hipError_t launch_mlp_train_step(const MlpBufs& b, const uint16_t* d_feat, const float* d_target,
+                                 int32_t n, float* d_partial, float lr, int step, float* d_loss,
+                                 hipStream_t s);
 hipError_t launch_mlp_cast(const MlpBufs& b, hipStream_t s);
 hipError_t launch_bf16_gemm(const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K,
                             hipStream_t s);

@@ -77,6 +77,15 @@ struct gt_mlp {
     ML_HIP(launch_mlp_grads(b, feat.as<uint16_t>(), d_target, n, partial.as<float>(), s));
     return 0;
   }
+  int train_step(const uint64_t* d_masks, const float* d_target, int32_t n, float lr,
+                 float* d_loss, hipStream_t s) {
+    ML_OK(features(d_masks, n, s));
+    const size_t tiles = (size_t)(n + kMlpTile - 1) / kMlpTile;
+    ML_OK(partial.ensure(tiles * kMlpPartialStride * 4));
+    ML_HIP(launch_mlp_train_step(b, feat.as<uint16_t>(), d_target, n, partial.as<float>(), lr,
+                                 ++step, d_loss, s));
+    return 0;
+  }
   int upload(const uint64_t* masks_h, const float* target_h, int32_t n) {
     ML_OK(masks.ensure((size_t)n * 8));
     ML_HIP(hipMemcpyAsync(masks.p, masks_h, (size_t)n * 8, hipMemcpyHostToDevice, stream));
@@ -213,8 +222,7 @@ int gt_mlp_train_step_dev(gt_mlp* m, const uint64_t* d_masks, const float* d_tar
   std::lock_guard<std::mutex> lk(m->mu);
   ML_HIP(hipSetDevice(m->device));
   hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : m->stream;
-  ML_OK(m->grads(d_masks, d_target, n, s));
-  ML_HIP(launch_mlp_adam(m->b, lr, ++m->step, s));
+  ML_OK(m->train_step(d_masks, d_target, n, lr, nullptr, s));
   return 0;
 }
 
@@ -241,14 +249,11 @@ int gt_mlp_train_batches(gt_mlp* m, const uint64_t* masks, const float* target, 
   ML_OK(m->upload(masks, target, n));
   const int nb = (n + batch - 1) / batch;
   ML_OK(m->losses.ensure((size_t)nb * 4));
-  // Steps are queued back to back; each step's loss is copied aside before
-  // the next step overwrites it.
+  // Steps are queued back to back; each writes its loss to its own slot.
   for (int b = 0; b < nb; ++b) {
     const int r0 = b * batch, rows = std::min(batch, n - r0);
-    ML_OK(m->grads(m->masks.as<uint64_t>() + r0, m->target.as<float>() + r0, rows, m->stream));
-    ML_HIP(launch_mlp_adam(m->b, lr, ++m->step, m->stream));
-    ML_HIP(hipMemcpyAsync(m->losses.as<float>() + b, m->b.grads + kMlpParams, 4,
-                          hipMemcpyDeviceToDevice, m->stream));
+    ML_OK(m->train_step(m->masks.as<uint64_t>() + r0, m->target.as<float>() + r0, rows, lr,
+                        m->losses.as<float>() + b, m->stream));
   }
   if (losses)
     ML_HIP(hipMemcpyAsync(losses, m->losses.p, (size_t)nb * 4, hipMemcpyDeviceToHost, m->stream));

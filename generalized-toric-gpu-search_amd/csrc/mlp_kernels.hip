@@ -59,32 +59,50 @@ __device__ __forceinline__ void wg_gemm(const uint16_t* A, int sam, int sak, con
 
 // ------------------------------------------------------------- features --
 
-// Thread per set: mask bits, |S| / 16, and for every ordered pair s != t the
-// bin of (s - t) mod 7 (index 1..48) counted and divided by |S|.
+// Features of a set: mask bits, |S| / 16, and for each nonzero torus
+// difference d = (du, dv) (index 1..48) the number of ordered pairs (s, t) of
+// S with s - t = d, divided by |S|.
+//
+// One wave per set.  The pair count for d is popcount(S & (S + d)), S + d
+// being the mask translated by d: the seven 7-bit rows rotate by du rows
+// (a rotation of the 49-bit mask by 7 du) and each row's bits by dv.  Lane l
+// writes features l and 64 + l.
+constexpr uint64_t kF49 = (1ull << kNpts) - 1ull;
+
+GT_HD uint64_t rep7(uint64_t row) {   // the 7-bit pattern `row` in all seven rows
+  uint64_t r = 0;
+  for (int u = 0; u < 7; ++u) r |= row << (7 * u);
+  return r;
+}
+
+__device__ __forceinline__ uint32_t pairs_at(uint64_t m, int d) {
+  const int du = d / 7, dv = d % 7;
+  uint64_t t = du ? (((m << (7 * du)) | (m >> (kNpts - 7 * du))) & kF49) : m;
+  if (dv) {
+    const uint64_t low = rep7((1ull << (7 - dv)) - 1ull);   // columns 0 .. 6 - dv
+    t = ((t & low) << dv) | ((t & ~low & kF49) >> (7 - dv));
+  }
+  return (uint32_t)__popcll(m & t);
+}
+
 __global__ __launch_bounds__(256) void mlp_features_kernel(const uint64_t* __restrict__ masks,
                                                            int32_t n,
                                                            uint16_t* __restrict__ feat) {
-  __shared__ uint8_t cnt[256][kNpts];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint8_t* c = cnt[threadIdx.x];
-  for (int d = 0; d < kNpts; ++d) c[d] = 0;
-  const uint64_t m = masks[i] & ((1ull << 49) - 1ull);
-  int pts[kNpts];
-  int k = 0;
-  for (uint64_t w = m; w; w &= w - 1) pts[k++] = __builtin_ctzll(w);
-  for (int a = 0; a < k; ++a)
-    for (int b = 0; b < k; ++b) {
-      if (a == b) continue;
-      const int du = (pts[a] / 7 - pts[b] / 7 + 7) % 7, dv = (pts[a] % 7 - pts[b] % 7 + 7) % 7;
-      ++c[du * 7 + dv];
-    }
-  uint16_t* x = feat + (size_t)i * kMlpIn;
+  const int lane = (int)(threadIdx.x & 63u);
+  const int set = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (set >= n) return;
+  const uint64_t m = masks[set] & kF49;
+  const int k = __popcll(m);
   const float invk = (k > 0) ? 1.0f / (float)k : 0.0f;
-  for (int p = 0; p < kNpts; ++p) x[p] = f2bf(((m >> p) & 1ull) ? 1.0f : 0.0f);
-  x[kNpts] = f2bf((float)k / 16.0f);
-  for (int d = 1; d < kNpts; ++d) x[kNpts + d] = f2bf((float)c[d] * invk);
-  for (int p = 2 * kNpts; p < kMlpIn; ++p) x[p] = 0;
+  float lo;
+  if (lane < kNpts) lo = ((m >> lane) & 1ull) ? 1.0f : 0.0f;
+  else if (lane == kNpts) lo = (float)k / 16.0f;
+  else lo = (float)pairs_at(m, lane - kNpts) * invk;           // d = 1 .. 14
+  const int d2 = lane + 64 - kNpts;                              // d = 15 .. 78
+  const float hi = (d2 < kNpts) ? (float)pairs_at(m, d2) * invk : 0.0f;
+  uint16_t* x = feat + (size_t)set * kMlpIn;
+  x[lane] = f2bf(lo);
+  x[64 + lane] = f2bf(hi);
 }
 
 // ------------------------------------------------------------ train step --
@@ -205,14 +223,26 @@ __global__ void mlp_reduce_kernel(const float* __restrict__ partial, int32_t til
   grads[p] = (p == kMlpParams) ? 0.5f * s / (float)n : s;
 }
 
-__global__ void mlp_adam_kernel(float* __restrict__ theta, const float* __restrict__ grads,
-                                float* __restrict__ m1, float* __restrict__ m2, float lr,
-                                float beta1, float beta2, float eps, float c1, float c2,
-                                uint16_t* __restrict__ w1b, uint16_t* __restrict__ w2b,
-                                uint16_t* __restrict__ w3b) {
+// One Adam step fused with the tile reduction: grads[p] = sum over tiles in
+// index order (the same sums as mlp_reduce_kernel), then the update of p and
+// its bf16 copy.  Slot kMlpParams takes the loss, also copied to *loss_out.
+__global__ void mlp_reduce_adam_kernel(const float* __restrict__ partial, int32_t tiles, int32_t n,
+                                       float* __restrict__ theta, float* __restrict__ grads,
+                                       float* __restrict__ m1, float* __restrict__ m2, float lr,
+                                       float beta1, float beta2, float eps, float c1, float c2,
+                                       uint16_t* __restrict__ w1b, uint16_t* __restrict__ w2b,
+                                       uint16_t* __restrict__ w3b, float* __restrict__ loss_out) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kMlpParams) return;
-  const float g = grads[p];
+  if (p > kMlpParams) return;
+  float g = 0.f;
+  for (int t = 0; t < tiles; ++t) g += partial[(size_t)t * kMlpPartialStride + p];
+  if (p == kMlpParams) {
+    const float loss = 0.5f * g / (float)n;
+    grads[p] = loss;
+    if (loss_out) *loss_out = loss;
+    return;
+  }
+  grads[p] = g;
   const float a = beta1 * m1[p] + (1.f - beta1) * g;
   const float b = beta2 * m2[p] + (1.f - beta2) * g * g;
   m1[p] = a;
@@ -253,8 +283,7 @@ static MlpView view(const MlpBufs& b) { return MlpView{b.theta, b.w1b, b.w2b, b.
 hipError_t launch_mlp_features(const uint64_t* d_masks, int32_t n, uint16_t* d_feat,
                                hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(mlp_features_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_masks, n,
-                     d_feat);
+  hipLaunchKernelGGL(mlp_features_kernel, dim3((n + 3) / 4), dim3(256), 0, s, d_masks, n, d_feat);
   return hipGetLastError();
 }
 
@@ -276,11 +305,17 @@ hipError_t launch_mlp_grads(const MlpBufs& b, const uint16_t* d_feat, const floa
   return hipGetLastError();
 }
 
-hipError_t launch_mlp_adam(const MlpBufs& b, float lr, int step, hipStream_t s) {
+hipError_t launch_mlp_train_step(const MlpBufs& b, const uint16_t* d_feat, const float* d_target,
+                                 int32_t n, float* d_partial, float lr, int step, float* d_loss,
+                                 hipStream_t s) {
+  const int tiles = (n + kMlpTile - 1) / kMlpTile;
   const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
   const float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
-  hipLaunchKernelGGL(mlp_adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, b.theta,
-                     b.grads, b.m1, b.m2, lr, beta1, beta2, eps, c1, c2, b.w1b, b.w2b, b.w3b);
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(tiles), dim3(256), 0, s, view(b), d_feat, d_target, n,
+                     d_partial);
+  hipLaunchKernelGGL(mlp_reduce_adam_kernel, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, s,
+                     d_partial, tiles, n, b.theta, b.grads, b.m1, b.m2, lr, beta1, beta2, eps, c1,
+                     c2, b.w1b, b.w2b, b.w3b, d_loss);
   return hipGetLastError();
 }
 
