@@ -46,14 +46,16 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t x) {
   return x;
 }
 
-// Two Gray steps: add row a, evaluate, add row b, evaluate.
+// Two Gray steps: add row a, evaluate, add row b, evaluate.  The running
+// minimum absorbs both weights in one v_min3_u32 (associated this way the
+// compiler fuses it; min(best, min(wa, wb)) costs 1.5 instructions per pair).
 #define GT_STEP2(A, B)                              \
   {                                                 \
     xor_row(v, A);                                  \
     const uint32_t wa = weight(v);                  \
     xor_row(v, B);                                  \
     const uint32_t wb = weight(v);                  \
-    best = min(best, min(wa, wb));                  \
+    best = min(min(best, wa), wb);                  \
   }
 
 // One chunk of Gray piece j: lanes own Gray indices (c*64 + lane) * 2^L and
