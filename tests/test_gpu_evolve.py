@@ -120,3 +120,34 @@ def test_evolve_errors():
             ev.log(0, 4)
     finally:
         ev.close()
+
+
+def test_search_recaptures_after_the_context_grows():
+    """Other calls on the search's context between generations reallocate the
+    buffers its graph reads (min-distance rows and plan, canonical sort
+    order); the next run recaptures and the generations still equal the
+    model's."""
+    import evolve_model as M
+    from gtoric import _native
+    from gtoric.distance import flatten
+    from gtoric.evolve import Evolve, random_masks
+    ctx = _native.context(0)
+    k, pop, n_prop = 6, 64, 128
+    init = random_masks(np.random.default_rng(21), pop, k)
+    ev = Evolve(ctx, k, pop, n_prop, 5150, 4, init)
+    try:
+        model = M.Model(k, pop, n_prop, 5150, init)
+        _compare(ev, model, 2)
+        rng = np.random.default_rng(22)
+        big = [sorted(rng.choice(49, 5, replace=False).tolist()) for _ in range(60_000)]
+        ctx.md_batch(*flatten(big), 48)                          # grows rows, plan, best
+        ctx.canon_masks(random_masks(rng, 300_000, 7))           # grows the size-sort order
+        for g in (3, 4):
+            ev.run(1)
+            assert ev.sync() == g
+            model.step()
+            m, d, a, seen = ev.population()
+            assert np.array_equal(m, model.pop_m) and np.array_equal(d, model.pop_d), g
+            assert np.array_equal(a, model.pop_age) and seen == len(model.seen), g
+    finally:
+        ev.close()
