@@ -149,6 +149,8 @@ struct LdsRows {
     for (int i = 0; i < 6; ++i) v.w[i] = p[i];
     return v;
   }
+  // Rows read from LDS are already in VGPRs.
+  __device__ __forceinline__ RowV vgpr(const RowV& v) const { return v; }
 };
 
 // Claims as in md_main_kernel: one lane-0 region per iteration (it completes
