@@ -32,7 +32,11 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint
 // (bf16, LDS or global).  MFMA 16x16x16 lane layout: lane l supplies
 // A(l % 16, 4 * (l / 16) + e) and B(4 * (l / 16) + e, l % 16), e = 0..3, and
 // receives C(4 * (l / 16) + e, l % 16).  epi(m, n, value) consumes each result.
-template <class Epi>
+// kAk / kBk: A / B is contiguous along k (sak / sbk = 1, rows 8-byte
+// aligned), so a lane's four values are one 8-byte load.  kK > 0: K is that
+// compile-time constant and the k loop is unrolled, so the fragment loads of
+// all k-steps are in flight before the MFMAs wait.
+template <bool kAk, bool kBk, int kK, class Epi>
 __device__ __forceinline__ void wg_gemm(const uint16_t* A, int sam, int sak, const uint16_t* B,
                                         int sbk, int sbn, int M, int N, int K, Epi epi) {
   const int nw = blockDim.x / kWave;
@@ -42,15 +46,27 @@ __device__ __forceinline__ void wg_gemm(const uint16_t* A, int sam, int sak, con
   for (int t = wave; t < tiles; t += nw) {
     const int m0 = (t / tn) * 16, n0 = (t % tn) * 16;
     f4v acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    auto step = [&](int k0) {
       s4v a, b;
+      if constexpr (kAk) {
+        a = *reinterpret_cast<const s4v*>(A + (m0 + r) * sam + k0 + 4 * q);
+      } else {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int kk = k0 + 4 * q + e;
-        a[e] = (short)A[(m0 + r) * sam + kk * sak];
-        b[e] = (short)B[kk * sbk + (n0 + r) * sbn];
+        for (int e = 0; e < 4; ++e) a[e] = (short)A[(m0 + r) * sam + (k0 + 4 * q + e) * sak];
+      }
+      if constexpr (kBk) {
+        b = *reinterpret_cast<const s4v*>(B + (n0 + r) * sbn + k0 + 4 * q);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) b[e] = (short)B[(k0 + 4 * q + e) * sbk + (n0 + r) * sbn];
       }
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    };
+    if constexpr (kK > 0) {
+#pragma unroll
+      for (int k0 = 0; k0 < kK; k0 += 16) step(k0);
+    } else {
+      for (int k0 = 0; k0 < K; k0 += 16) step(k0);
     }
 #pragma unroll
     for (int e = 0; e < 4; ++e) epi(m0 + 4 * q + e, n0 + r, acc[e]);
@@ -118,24 +134,35 @@ struct MlpView {
 __device__ __forceinline__ void tile_forward(const MlpView& W, const uint16_t* __restrict__ feat,
                                              int row0, int n, uint16_t* X, uint16_t* H1,
                                              uint16_t* H2, float* y) {
-  for (int e = threadIdx.x; e < kMlpTile * kMlpIn; e += blockDim.x) {
-    const int m = e / kMlpIn, i = e % kMlpIn;
-    X[e] = (row0 + m < n) ? feat[(size_t)(row0 + m) * kMlpIn + i] : (uint16_t)0;
+  // Four features per 8-byte load (rows are 256 bytes, X is 16-byte aligned).
+  for (int e = threadIdx.x; e < kMlpTile * kMlpIn / 4; e += blockDim.x) {
+    const int m = e / (kMlpIn / 4), i4 = e % (kMlpIn / 4);
+    uint2 v = make_uint2(0u, 0u);
+    if (row0 + m < n) v = *reinterpret_cast<const uint2*>(feat + (size_t)(row0 + m) * kMlpIn + 4 * i4);
+    *reinterpret_cast<uint2*>(X + 4 * e) = v;
   }
   __syncthreads();
   const float* b1 = W.theta + kOffB1;
   const float* b2 = W.theta + kOffB2;
-  wg_gemm(X, kMlpIn, 1, W.w1b, 1, kMlpIn, kMlpTile, kMlpHid, kMlpIn,
+  wg_gemm<true, true, kMlpIn>(X, kMlpIn, 1, W.w1b, 1, kMlpIn, kMlpTile, kMlpHid, kMlpIn,
           [&](int m, int h, float acc) { H1[m * kMlpHid + h] = f2bf(fmaxf(acc + b1[h], 0.f)); });
   __syncthreads();
-  wg_gemm(H1, kMlpHid, 1, W.w2b, 1, kMlpHid, kMlpTile, kMlpHid, kMlpHid,
+  wg_gemm<true, true, kMlpHid>(H1, kMlpHid, 1, W.w2b, 1, kMlpHid, kMlpTile, kMlpHid, kMlpHid,
           [&](int m, int h, float acc) { H2[m * kMlpHid + h] = f2bf(fmaxf(acc + b2[h], 0.f)); });
   __syncthreads();
-  if (threadIdx.x < kMlpTile) {
-    const int m = threadIdx.x;
-    float s = W.theta[kOffB3];
-    for (int h = 0; h < kMlpHid; ++h) s += bf2f(W.w3b[h]) * bf2f(H2[m * kMlpHid + h]);
-    y[m] = s;
+  {
+    // y[m] = b3 + sum_h w3[h] H2[m][h]: eight lanes per row, 16 terms each,
+    // then a butterfly over the eight (every lane ends with the same sum).
+    static_assert(kMlpTile * 8 == 256 && kMlpHid == 8 * 16, "256 threads: 32 rows x 8 parts");
+    const int m = (int)threadIdx.x >> 3, part = (int)threadIdx.x & 7;
+    float s = 0.f;
+#pragma unroll
+    for (int h = part * 16; h < part * 16 + 16; ++h)
+      s += bf2f(W.w3b[h]) * bf2f(H2[m * kMlpHid + h]);
+    s += __shfl_xor(s, 1, 8);
+    s += __shfl_xor(s, 2, 8);
+    s += __shfl_xor(s, 4, 8);
+    if (part == 0) y[m] = W.theta[kOffB3] + s;
   }
   __syncthreads();
 }
@@ -143,7 +170,8 @@ __device__ __forceinline__ void tile_forward(const MlpView& W, const uint16_t* _
 __global__ __launch_bounds__(256) void mlp_predict_kernel(MlpView W,
                                                           const uint16_t* __restrict__ feat,
                                                           int32_t n, float* __restrict__ out) {
-  __shared__ uint16_t X[kMlpTile * kMlpIn], H1[kMlpTile * kMlpHid], H2[kMlpTile * kMlpHid];
+  __shared__ __attribute__((aligned(16))) uint16_t X[kMlpTile * kMlpIn], H1[kMlpTile * kMlpHid],
+      H2[kMlpTile * kMlpHid];
   __shared__ float y[kMlpTile];
   const int row0 = blockIdx.x * kMlpTile;
   tile_forward(W, feat, row0, n, X, H1, H2, y);
@@ -156,8 +184,9 @@ __global__ __launch_bounds__(256) void mlp_grad_kernel(MlpView W,
                                                        const uint16_t* __restrict__ feat,
                                                        const float* __restrict__ target,
                                                        int32_t n, float* __restrict__ partial) {
-  __shared__ uint16_t X[kMlpTile * kMlpIn], H1[kMlpTile * kMlpHid], H2[kMlpTile * kMlpHid];
-  __shared__ uint16_t D2[kMlpTile * kMlpHid], D1[kMlpTile * kMlpHid];
+  __shared__ __attribute__((aligned(16))) uint16_t X[kMlpTile * kMlpIn], H1[kMlpTile * kMlpHid],
+      H2[kMlpTile * kMlpHid];
+  __shared__ __attribute__((aligned(16))) uint16_t D2[kMlpTile * kMlpHid], D1[kMlpTile * kMlpHid];
   __shared__ float y[kMlpTile], dy[kMlpTile], sq[kMlpTile];
   const int row0 = blockIdx.x * kMlpTile;
   tile_forward(W, feat, row0, n, X, H1, H2, y);
@@ -194,15 +223,15 @@ __global__ __launch_bounds__(256) void mlp_grad_kernel(MlpView W,
     P[kMlpParams] = s;
   }
   // dH1 = (dH2 W2) * relu'(H1), rounded to bf16.
-  wg_gemm(D2, kMlpHid, 1, W.w2b, kMlpHid, 1, kMlpTile, kMlpHid, kMlpHid,
+  wg_gemm<true, false, kMlpHid>(D2, kMlpHid, 1, W.w2b, kMlpHid, 1, kMlpTile, kMlpHid, kMlpHid,
           [&](int m, int h, float acc) {
             D1[m * kMlpHid + h] = (H1[m * kMlpHid + h] != 0) ? f2bf(acc) : (uint16_t)0;
           });
   __syncthreads();
   // dW2[o][i] = sum_m dH2[m][o] H1[m][i];  dW1[o][i] = sum_m dH1[m][o] X[m][i].
-  wg_gemm(D2, 1, kMlpHid, H1, kMlpHid, 1, kMlpHid, kMlpHid, kMlpTile,
+  wg_gemm<false, false, kMlpTile>(D2, 1, kMlpHid, H1, kMlpHid, 1, kMlpHid, kMlpHid, kMlpTile,
           [&](int o, int i, float acc) { P[kOffW2 + o * kMlpHid + i] = acc; });
-  wg_gemm(D1, 1, kMlpHid, X, kMlpIn, 1, kMlpHid, kMlpIn, kMlpTile,
+  wg_gemm<false, false, kMlpTile>(D1, 1, kMlpHid, X, kMlpIn, 1, kMlpHid, kMlpIn, kMlpTile,
           [&](int o, int i, float acc) { P[kOffW1 + o * kMlpIn + i] = acc; });
   if (threadIdx.x < kMlpHid) {
     const int h = threadIdx.x;
@@ -272,7 +301,7 @@ __global__ __launch_bounds__(256) void bf16_gemm_kernel(const uint16_t* __restri
                                                         int K) {
   const int bm = blockIdx.y * 64, bn = blockIdx.x * 64;
   const int mm = min(64, M - bm), nn = min(64, N - bn);
-  wg_gemm(A + (size_t)bm * K, K, 1, B + bn, N, 1, mm, nn, K,
+  wg_gemm<true, false, 0>(A + (size_t)bm * K, K, 1, B + bn, N, 1, mm, nn, K,
           [&](int m, int n2, float acc) { C[(size_t)(bm + m) * N + bn + n2] = acc; });
 }
 
