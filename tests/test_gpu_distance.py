@@ -45,6 +45,17 @@ def test_kernel_backup_kats(dist):
     assert dist.max_zeros_batch([[0, 1, 7]], 49) == [1]
 
 
+def test_structured_candidates_of_champion_search(dist):
+    """circle_conic_k8 and parabola_conic_k7 (champion_search.py:278-287):
+    d = 24 and 34, as the oracle gives."""
+    circle = [1, 6, 7, 16, 19, 37, 40, 42]
+    parabola = [0, 8, 18, 23, 30, 39, 43]
+    assert dist.max_zeros_batch([circle, parabola], 1) == [25, 15]
+    assert dist.max_zeros(circle, 1) == 25 and dist.max_zeros(parabola, 1) == 15
+    # td = 24 is met (exact max_zeros); td = 26 is not (tmax + 1 = 24).
+    assert dist.max_zeros_batch([circle], 24) == [25] and dist.max_zeros_batch([circle], 26) == [24]
+
+
 def test_config1_single_k5(dist, md_gold):
     sets = _rows(md_gold["c1_flat"], md_gold["c1_offsets"])
     exp = md_gold["c1_mz"].tolist()

@@ -271,3 +271,17 @@ def test_golden_fixtures_consistent_with_oracle():
     c = np.load(GOLD / "canon_golden.npz")
     lo, hi, st = oracle.canon_batch(c["x16_sets"][:10])
     assert np.array_equal(lo, c["x16_lo"][:10]) and np.array_equal(st, c["x16_stab"][:10])
+
+
+def test_structured_candidates_of_champion_search():
+    """The two structured sets champion_search.py:278-287 checks first (idx(a, b)
+    = 7 a + b, champion_search.py:27-28): the conic x^2 + y^2 = 1 point set
+    (circle_conic_k8) and the parabola y = x^2 (parabola_conic_k7).  d = 24 and
+    34; the parabola also by brute force over all 8^7 - 1 polynomials."""
+    idx = lambda a, b: 7 * a + b
+    circle = sorted([idx(0, 1), idx(0, 6), idx(1, 0), idx(2, 2), idx(2, 5), idx(5, 2), idx(5, 5),
+                     idx(6, 0)])
+    parabola = sorted(idx(t, (t * t) % 7) for t in range(7))
+    assert 49 - oracle.max_zeros(circle, 48) == 24
+    assert 49 - oracle.max_zeros(parabola, 48) == 34
+    assert 49 - oracle.max_zeros_brute(parabola) == 34
