@@ -12,7 +12,7 @@ constexpr int kServeRowWords = 3 * kServeMaxK * kRowWords;   // bit rows of one 
 
 // One request descriptor (fine-grained host memory, written by the host).
 struct ServeReq {
-  int32_t k;
+  int32_t k;            // bits 0..7 k, bits 8..15 walk length L for this request (0: default)
   int32_t tmax;
   int32_t idx[kServeMaxK];
   uint64_t tag;
@@ -67,7 +67,7 @@ struct ServeDev {
   uint64_t life_ticks;  // kernel lifetime limit
   uint8_t L[kServeMaxK + 1];
   uint8_t check_log2[kServeMaxK + 1];
-  int32_t* s_k;
+  int32_t* s_k;         // per slot: k | (L << 8), 0 when the request has no chunks
   int32_t* s_tmax;
   uint64_t* s_state;    // per slot: bits 0..31 minimum weight, 32..63 chunks finished
   uint32_t* s_nch;

@@ -70,6 +70,11 @@ struct gt_server {
   uint64_t next = 0;        // next request index
   uint64_t oldest = 0;      // oldest request whose result is not consumed
   uint64_t outstanding = 0; // codewords of published, unconsumed requests
+  // Load-dependent chunk length: a request admitted while fewer than
+  // short_below codewords are outstanding walks chunks of short_L[k] steps.
+  bool short_on = false;
+  uint64_t short_below = 0;
+  uint8_t short_L[kServeMaxK + 1] = {};
   std::vector<uint64_t> tag, work;
   std::vector<uint8_t> pending;
   bool running = false;
@@ -152,7 +157,9 @@ struct gt_server {
     if (!can_admit(w)) return 0;
     const uint32_t s = (uint32_t)(next & (slots - 1));
     ServeReq& q = ring[s];
-    q.k = k;
+    int32_t Lq = 0;
+    if (short_on && k >= 1 && outstanding < short_below && short_L[k] < dh.L[k]) Lq = short_L[k];
+    q.k = k | (Lq << 8);
     q.tmax = tmax;
     for (int i = 0; i < kServeMaxK; ++i) q.idx[i] = (i < k) ? idx[i] : 0;
     q.tag = t;
@@ -270,6 +277,23 @@ int gt_serve_submit(gt_server* sv, const int32_t* idx, int32_t k, int32_t tmax, 
   if (!sv || !accepted || (k > 0 && !idx)) return sfail("null argument");
   std::lock_guard<std::mutex> lk(sv->mu);
   return sv->submit(idx, k, tmax, tag, accepted);
+}
+
+int gt_serve_set_short_chunks(gt_server* sv, const int32_t* L_short_by_k, int64_t below_work) {
+  if (!sv) return sfail("null server");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (!L_short_by_k) {
+    sv->short_on = false;
+    return 0;
+  }
+  if (below_work < 0) return sfail("negative work threshold");
+  for (int k = 0; k <= kServeMaxK; ++k)
+    if (L_short_by_k[k] < kU || L_short_by_k[k] > 30)
+      return sfail("bad short chunk length for k=%d", k);
+  for (int k = 0; k <= kServeMaxK; ++k) sv->short_L[k] = (uint8_t)L_short_by_k[k];
+  sv->short_below = (uint64_t)below_work;
+  sv->short_on = true;
+  return 0;
 }
 
 int gt_serve_poll(gt_server* sv, uint64_t* tags, int32_t* results, int32_t max_n, int32_t* n_done) {

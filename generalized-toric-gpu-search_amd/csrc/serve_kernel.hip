@@ -94,9 +94,14 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, uint32_t lane) {
         qi[4 * i] = (int32_t)v[i].x; qi[4 * i + 1] = (int32_t)v[i].y;
         qi[4 * i + 2] = (int32_t)v[i].z; qi[4 * i + 3] = (int32_t)v[i].w;
       }
-      k = qi[0];
+      // Bits 0..7: k; bits 8..15: the request's walk length L (0: the
+      // server's per-k default).
+      k = qi[0] & 0xFF;
+      const int Lq = (qi[0] >> 8) & 0xFF;
       const int32_t q_tmax = qi[1];
-      nch = (k >= 1 && k <= kServeMaxK) ? (uint32_t)code_chunks(k, d->L[k]) : 0u;
+      int L = 0;
+      if (k >= 1 && k <= kServeMaxK) L = (Lq != 0) ? Lq : (int)d->L[k];
+      nch = (L != 0) ? (uint32_t)code_chunks(k, L) : 0u;
       uint32_t packed[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
       for (int i = 0; i < kServeMaxK; ++i) {
@@ -105,7 +110,7 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, uint32_t lane) {
         packed[i >> 2] |= (uint32_t)g << (8 * (i & 3));
       }
       d->s_idx[slot] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
-      d->s_k[slot] = (nch != 0u) ? k : 0;
+      d->s_k[slot] = (nch != 0u) ? (k | (L << 8)) : 0;
       d->s_tmax[slot] = q_tmax;
       d->s_state[slot] = 64ull;
       d->s_nch[slot] = nch;
@@ -223,7 +228,8 @@ __device__ void worker(ServeDev* d, ServeHost* h, const uint32_t* __restrict__ m
       step /= kWave;
     }
     const uint32_t slot = (uint32_t)(lo & (slots - 1));
-    const int k = (int)__builtin_amdgcn_readfirstlane((uint32_t)d->s_k[slot]);
+    const int kl = (int)__builtin_amdgcn_readfirstlane((uint32_t)d->s_k[slot]);
+    const int k = kl & 0xFF, L = kl >> 8;
     const int tmax = (int)__builtin_amdgcn_readfirstlane((uint32_t)d->s_tmax[slot]);
     const uint32_t nch = __builtin_amdgcn_readfirstlane(d->s_nch[slot]);
     const uint64_t seq = uniform64(d->s_seq[slot]);
@@ -263,7 +269,6 @@ __device__ void worker(ServeDev* d, ServeHost* h, const uint32_t* __restrict__ m
         ++st_staged;
       }
       t2 = wall_clock64();
-      const int L = d->L[k];
       const uint32_t cmask = (1u << d->check_log2[k]) - 1u;
       b = run_chunk<true>(LdsRows{lrows}, k, L, local, lane, thr, cmask, best_g);
       b = wave_min(b);

@@ -48,6 +48,7 @@ _SIGS = {
     "gt_serve_start": ([i32, i32, i64, vp, vp, ctypes.c_double, ctypes.c_double, P(vp)],
                        ctypes.c_int),
     "gt_serve_submit": ([vp, vp, i32, i32, u64, P(i32)], ctypes.c_int),
+    "gt_serve_set_short_chunks": ([vp, vp, i64], ctypes.c_int),
     "gt_serve_poll": ([vp, vp, vp, i32, P(i32)], ctypes.c_int),
     "gt_serve_in_flight": ([vp, P(i64)], ctypes.c_int),
     "gt_serve_run_stream": ([vp, vp, vp, i32, vp, vp, vp, vp, P(ctypes.c_double),

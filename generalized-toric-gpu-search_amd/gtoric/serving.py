@@ -34,7 +34,8 @@ SERVE_L_OFFSET = 4
 class Server:
     def __init__(self, device: int = 0, slots: int = 4096, work_budget: int = 1 << 36,
                  params: dict | None = None, idle_s: float = 10.0, life_s: float = 600.0,
-                 max_chunk_log2: int | None = None):
+                 max_chunk_log2: int | None = None, short_chunk_log2: int | None = None,
+                 short_below: int = 1 << 33):
         """params: {k: (L, check_log2)}; defaults: the context's (tuned) batch
         table with L raised by SERVE_L_OFFSET.
 
@@ -42,9 +43,15 @@ class Server:
         max(batch L, cap): a chunk is walked by one wave, so its length bounds
         how fast one large request can finish.  Smaller chunks trade capacity
         for tail latency (on the mixed-k stream, cap 13: p99 about half, capacity
-        about 0.8x)."""
+        about 0.8x).
+
+        short_chunk_log2 applies that cap only to requests admitted while fewer
+        than short_below projective codewords are outstanding
+        (gt_serve_set_short_chunks): short chunks when the server has spare
+        capacity, the long ones when it is loaded."""
+        ctx = _native.context(device)
+        batch_L = [ctx.get_md_params(k)[0] for k in range(MAX_K + 1)]
         if params is None:
-            ctx = _native.context(device)
             params = {}
             for k in range(MAX_K + 1):
                 L, c = ctx.get_md_params(k)
@@ -59,6 +66,21 @@ class Server:
                                    float(idle_s), float(life_s), ctypes.byref(h)))
         self.handle = h
         self.slots = slots
+        self.L = L
+        if short_chunk_log2 is not None:
+            short = np.array([min(int(L[k]), max(batch_L[k], int(short_chunk_log2)))
+                              for k in range(MAX_K + 1)], dtype=np.int32)
+            self.set_short_chunks(short, short_below)
+
+    def set_short_chunks(self, L_short, below_work: int) -> None:
+        """Walk length per k (17 entries) for requests admitted below
+        below_work outstanding codewords; None turns it off."""
+        if L_short is None:
+            check(lib().gt_serve_set_short_chunks(self.handle, None, 0))
+            return
+        Ls = np.ascontiguousarray(L_short, dtype=np.int32)
+        assert Ls.shape == (MAX_K + 1,)
+        check(lib().gt_serve_set_short_chunks(self.handle, ptr(Ls), int(below_work)))
 
     def submit(self, S, tmax: int, tag: int) -> bool:
         s = np.ascontiguousarray(S if len(S) else [0], dtype=np.int32)

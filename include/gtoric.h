@@ -119,6 +119,13 @@ int gt_serve_start(int32_t device, int32_t slots, int64_t work_budget, const int
 /* Non-blocking admission: *accepted = 0 when no slot or budget is free. */
 int gt_serve_submit(gt_server* srv, const int32_t* idx, int32_t k, int32_t tmax, uint64_t tag,
                     int32_t* accepted);
+/* Load-dependent chunk length (no reference counterpart): a request admitted
+ * while fewer than below_work projective codewords are outstanding walks
+ * chunks of L_short_by_k[k] steps (17 entries, 4..30; used only where
+ * shorter than the start call's L_by_k), so a lightly loaded server answers
+ * large requests sooner and a loaded one keeps its long, cheaper chunks.
+ * Results do not change.  NULL turns it off. */
+int gt_serve_set_short_chunks(gt_server* srv, const int32_t* L_short_by_k, int64_t below_work);
 /* Completed requests (tag, result), at most max_n per call. */
 int gt_serve_poll(gt_server* srv, uint64_t* tags, int32_t* results, int32_t max_n,
                   int32_t* n_done);

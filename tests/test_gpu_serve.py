@@ -140,3 +140,23 @@ def test_serve_errors():
         while not out and time.time() < deadline:
             out = srv.poll()
         assert out == [(7, 0)]
+
+
+def test_short_chunks_change_chunks_not_results(ctx):
+    """gt_serve_set_short_chunks: below a huge work threshold every request
+    walks the short chunks, below 0 none does; results stay the batch's."""
+    from gtoric.autotune import code_chunks
+    from gtoric.distance import flatten
+    from gtoric.serving import Server
+    sets, tmax = _requests(300, seed=58, kmax=10)
+    exp = ctx.md_batch(*flatten(sets), tmax).tolist()
+    for below, short in ((1 << 62, True), (0, False)):
+        with Server(0, slots=256, life_s=60, short_chunk_log2=9, short_below=below) as srv:
+            res, _, _ = srv.run_stream(sets, tmax, np.zeros(len(sets)), timeout_s=60)
+            st = srv.stats()
+            L = srv.L
+        assert res.tolist() == exp
+        Lb = {k: ctx.get_md_params(k)[0] for k in range(17)}
+        Luse = {k: (min(int(L[k]), max(Lb[k], 9)) if short else int(L[k])) for k in range(17)}
+        want = sum(code_chunks(len(S), Luse[len(S)]) for S in sets if len(S))
+        assert st["chunks_walked"] + st["chunks_skipped"] == want, short
