@@ -399,15 +399,19 @@ __global__ __launch_bounds__(kSampledBlock) void md_sampled_kernel(
   int tz = 0;
   uint32_t it = 0;
   const uint64_t stride = (uint64_t)bpc * blockDim.x;
-  for (uint64_t wid = 1ull + (uint64_t)part * blockDim.x + threadIdx.x; wid <= sample_count;
-       wid += stride, ++it) {
+  const uint64_t wid0 = 1ull + (uint64_t)part * blockDim.x + threadIdx.x;
+  // wid * C for the stream's two multipliers, advanced by stride * C per
+  // sample (products modulo 2^64, so the same values as multiplying afresh).
+  uint64_t wc = wid0 * 0xBF58476D1CE4E5B9ull, wf = wid0 * 0xA5A3564E27F8861Full;
+  const uint64_t dwc = stride * 0xBF58476D1CE4E5B9ull, dwf = stride * 0xA5A3564E27F8861Full;
+  for (uint64_t wid = wid0; wid <= sample_count;
+       wid += stride, ++it, wc += dwc, wf += dwf) {
     if ((it & 31u) == 0 &&
         __hip_atomic_load(&stopf[code], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
       break;
     RowV v = {{0, 0, 0, 0, 0, 0}};
     if (use_pid) {
-      const uint64_t x = seed ^ ((uint64_t)code * 0x9E3779B97F4A7C15ull) ^
-                         (wid * 0xBF58476D1CE4E5B9ull);
+      const uint64_t x = seed ^ ((uint64_t)code * 0x9E3779B97F4A7C15ull) ^ wc;
       uint64_t pid = mod_mersenne(splitmix64(x), m) + 1ull;
       for (int i = 0; i < k; ++i, pid >>= 3) {
         const uint32_t* src = bp + ((i << 3) + (int)(pid & 7ull)) * kRowWords;
@@ -415,12 +419,10 @@ __global__ __launch_bounds__(kSampledBlock) void md_sampled_kernel(
         for (int q = 0; q < 6; ++q) v.w[q] ^= src[q];
       }
     } else {
-      const uint64_t xf = seed ^ ((uint64_t)code * 0xD6E8FEB86659FD93ull) ^
-                          (wid * 0xA5A3564E27F8861Full);
+      const uint64_t xf = seed ^ ((uint64_t)code * 0xD6E8FEB86659FD93ull) ^ wf;
       const int forced = (int)(splitmix64(xf) % (uint64_t)k);
       for (int i = 0; i < k; ++i) {
-        const uint64_t x = seed ^ ((uint64_t)code * 0x9E3779B97F4A7C15ull) ^
-                           (wid * 0xBF58476D1CE4E5B9ull) ^
+        const uint64_t x = seed ^ ((uint64_t)code * 0x9E3779B97F4A7C15ull) ^ wc ^
                            ((uint64_t)(i + 1) * 0x94D049BB133111EBull);
         int c = (int)(splitmix64(x) & 7ull);
         if (i == forced && c == 0) c = 1;
