@@ -178,16 +178,62 @@ def test_agl2_invariance_on_gpu(dist):
     perms = precompute_gl2_perms()
     rng = np.random.default_rng(103)
     base = [sorted(rng.choice(49, 8, replace=False).tolist()) for _ in range(64)]
-    imgs = []
-    for S in base:
-        A = perms[rng.integers(0, len(perms))]
-        b = divmod(int(rng.integers(0, 49)), 7)
-        img = []
-        for p in S:
-            u, v = divmod(int(A[p]), 7)
-            img.append(((u + b[0]) % 7) * 7 + (v + b[1]) % 7)
-        imgs.append(sorted(img))
+    imgs = [_agl2_image(S, perms[rng.integers(0, len(perms))],
+                        divmod(int(rng.integers(0, 49)), 7)) for S in base]
     assert dist.max_zeros_batch(base, 1) == dist.max_zeros_batch(imgs, 1)
+
+
+def _agl2_image(S, A, b):
+    img = []
+    for p in S:
+        u, v = divmod(int(A[p]), 7)
+        img.append(((u + b[0]) % 7) * 7 + (v + b[1]) % 7)
+    return sorted(img)
+
+
+def test_large_k_exact_properties(dist):
+    """Exact mode at k = 12 and 13 (1e10 and 8e10 codewords per code), where
+    the oracle cannot finish in seconds, pinned by what the mathematics fixes:
+    * AGL2 invariance: d(S) = d(A S + b);
+    * the code of S is a subcode of the code of S + {p}, so its max_zeros
+      cannot be larger;
+    * Singleton: d <= n - k + 1, so max_zeros >= k - 1 (independent rows);
+    * sampling only visits codewords, so a sampled max_zeros never exceeds
+      the exact one;
+    * a thresholded call returns the exact value when the target is met and
+      min(exact, tmax + 1) when it is not."""
+    from gtoric.canonical import precompute_gl2_perms
+    perms = precompute_gl2_perms()
+    rng = np.random.default_rng(107)
+    base = [sorted(rng.choice(49, 12, replace=False).tolist()) for _ in range(4)]
+    imgs = [_agl2_image(S, perms[rng.integers(0, len(perms))],
+                        divmod(int(rng.integers(0, 49)), 7)) for S in base]
+    ext = [sorted(S + [int(rng.choice(sorted(set(range(49)) - set(S))))]) for S in base]
+    got = dist.max_zeros_batch(base + imgs + ext, 1)
+    mz, mz_img, mz_ext = got[:4], got[4:8], got[8:]
+    assert mz == mz_img
+    for a, b in zip(mz, mz_ext):
+        assert 11 <= a <= b, (a, b)
+    assert all(12 <= b for b in mz_ext)
+    # The k = 13 sets alone (other bucket order, other claim spread) agree.
+    assert dist.max_zeros_batch(ext, 1) == mz_ext
+    samp = dist.max_zeros_batch(base, 1, sample_count=20000, sample_seed=99)
+    assert all(s <= e for s, e in zip(samp, mz))
+    for S, e in zip(base, mz):
+        d = 49 - e
+        assert dist.max_zeros_batch([S], d) == [e]
+        # td = d + 2 is not met: the clamp tmax + 1 = 49 - td + 1 = e - 1.
+        assert dist.max_zeros_batch([S], d + 2) == [e - 1]
+        assert dist.max_zeros(S, 1) == e
+    # Chunk length and check interval change the work split, not the result.
+    ctx = dist.context
+    saved = ctx.get_md_params(12)
+    try:
+        for L, c in ((9, 0), (13, 3), (16, 6)):
+            ctx.set_md_params(12, L, c)
+            assert dist.max_zeros_batch(base, 1) == mz, (L, c)
+    finally:
+        ctx.set_md_params(12, *saved)
 
 
 def test_errors(dist):
