@@ -344,8 +344,11 @@ def cpu_baseline(sets, cores: int, budget_s: float):
 def main() -> None:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # 200 untimed steps (about 0.1 s) first: the device's clocks take tens of
+    # milliseconds to settle after it goes busy, and a few steps would time
+    # part of that ramp.
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--canon-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline")
     ap.add_argument("--skip", default="",

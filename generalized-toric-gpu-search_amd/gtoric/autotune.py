@@ -8,7 +8,9 @@ Tunable per k (gt_set_md_params):
 
 For each k the tuner times device kernels (HIP events around md_main, read
 through gt_prof_read) on a seeded single-k batch, keeps the fastest setting,
-and writes a JSON table keyed by device architecture.  Results never depend on
+and writes a JSON table keyed by device architecture.  refine() then moves L
+by coordinate descent on a mixed batch, where the chunks of all k share the
+waves.  Results never depend on
 the parameters (tests/test_gpu_distance.py::test_params_do_not_change_results);
 only speed does.
 """
@@ -103,6 +105,49 @@ def tune(ctx: _native.Context, ks=range(3, 11), codes_per_k: dict | None = None,
     return {"arch": ctx.arch, "cu_count": ctx.cu_count, "params": table}
 
 
+def refine(ctx: _native.Context, sets, repeats: int = 5, margin: float = 0.005,
+           verbose: bool = True) -> tuple[dict, float]:
+    """Coordinate descent of L on one mixed batch, from the context's current
+    table.  Per-k tuning times one k at a time; in a mix the chunks of every
+    k share the same waves (largest k first), so the best L of a k can move.
+    For each k of the batch, largest share of codewords first, L is stepped
+    up, then down, while md_main on the whole batch gets faster by more than
+    `margin`.  Returns ({k: L} for the moves kept, final md_main ms)."""
+    from .distance import flatten
+    flat, offs = flatten(sets)
+    work: dict[int, int] = {}
+    for s in sets:
+        k = len(s)
+        if k > 0:
+            work[k] = work.get(k, 0) + (8 ** k - 1) // 7
+    # The clocks settle over tens of milliseconds of load: time until two
+    # readings in a row agree within 1 %, so the start is not a cold reading.
+    prev, cur = None, _time_md(ctx, flat, offs, 48, repeats)
+    for _ in range(20):
+        if prev is not None and abs(cur - prev) <= 0.01 * prev:
+            break
+        prev, cur = cur, _time_md(ctx, flat, offs, 48, repeats)
+    if verbose:
+        print(f"[refine] start md_main {cur:.4f} ms")
+    moved = {}
+    for k in sorted(work, key=lambda k: -work[k]):
+        L, c = ctx.get_md_params(k)
+        L0 = L
+        for step in (1, -1):
+            while L + step in candidate_L(k):
+                ctx.set_md_params(k, L + step, c)
+                t = _time_md(ctx, flat, offs, 48, repeats)
+                if verbose:
+                    print(f"[refine] k={k} L={L + step:2d} md_main {t:.4f} ms")
+                if t >= cur * (1.0 - margin):
+                    ctx.set_md_params(k, L, c)
+                    break
+                cur, L = t, L + step
+        if L != L0:
+            moved[k] = L
+    return moved, cur
+
+
 def tuned_path(arch: str) -> Path:
     return TUNED_DIR / f"{arch.split(':')[0]}.json"
 
@@ -131,8 +176,25 @@ if __name__ == "__main__":
     ap.add_argument("ks", nargs="*", type=int, default=list(range(3, 11)))
     ap.add_argument("--out", type=Path, help="write the table here instead of tuned/<arch>.json")
     ap.add_argument("--repeats", type=int, default=3)
+    ap.add_argument("--refine-mix", action="store_true",
+                    help="load the saved table and refine L on the batched workload "
+                         "(4096 codes, k uniform in 3..8) instead of tuning per k")
     a = ap.parse_args()
     c = _native.Context(0)
+    if a.refine_mix:
+        from . import workloads
+        p = a.out or tuned_path(c.arch)
+        res = json.loads(p.read_text())
+        apply(c, p)
+        moved, ms = refine(c, workloads.mixed_k(), repeats=max(a.repeats, 5))
+        for k, L in moved.items():
+            res["params"][str(k)]["L"] = L
+        res["mix_refined"] = {"workload": "4096 codes, k uniform in 3..8 (seed 2)",
+                              "md_main_ms": round(ms, 5),
+                              "moved": {str(k): L for k, L in moved.items()}}
+        p.write_text(json.dumps(res, indent=1) + "\n")
+        print(json.dumps(res["mix_refined"]), "saved", p)
+        raise SystemExit(0)
     res = tune(c, a.ks, repeats=a.repeats)
     print(json.dumps(res, indent=1))
     if a.out:
