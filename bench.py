@@ -429,6 +429,10 @@ def main() -> None:
             "valu_issue_frac": round(VALU_PER_CODEWORD * codewords / (kern_ms * 1e-3) / 1e12 / peak, 4),
             "kernel_ms": round(kern_ms, 4),
             "kernel_launches": nk,
+            "note": "peak is the nominal int32 VALU rate (16 lanes per SIMD per clock); "
+                    "frac can exceed 1 because gfx950 issues the walk's VOP2 XORs with two "
+                    "VGPR operands in fewer than the nominal 4 clocks per wave64 instruction "
+                    "(SQ_INSTS_VALU against GRBM_GUI_ACTIVE in the md_main profile)",
         },
     }
 
