@@ -56,7 +56,15 @@ def log(msg: str) -> None:
 
 
 def proj(k: int) -> int:
-    return (8 ** k - 1) // 7
+    """Projective codewords the exact walk of a k-point code visits: the
+    (k-1)-dimensional subcode vanishing at the torus point (1,1)."""
+    w = 0 if k <= 1 else k - 1
+    return (8 ** w - 1) // 7
+
+
+def ref_polys(k: int) -> int:
+    """Coefficient vectors the reference enumerates per code (kernel.py:128-139)."""
+    return 8 ** k - 1
 
 
 def bench_md(ctx, sets, steps: int, warmup: int, expect=None):
@@ -383,6 +391,7 @@ def main() -> None:
     sets = workloads.mixed_k()
     n = len(sets)
     codewords = sum(proj(len(s)) for s in sets)
+    ref_work = sum(ref_polys(len(s)) for s in sets)
     gold = ROOT / "tests" / "golden" / "md_golden.npz"
     expect = np.load(gold)["c2_mz"] if gold.exists() else None
 
@@ -413,7 +422,8 @@ def main() -> None:
                         "k uniform in 3..8 (numpy default_rng(2)), one device-resident launch "
                         "sequence per step",
             "codes": n,
-            "projective_codewords": codewords,
+            "walked_codewords": codewords,
+            "reference_polys": ref_work,
             "parity": "bit-exact vs oracle fixture" if expect is not None else "unchecked",
         },
         "roofline": {
@@ -512,10 +522,16 @@ def main() -> None:
                                                         args.evolve_hard_generations)
 
     if "cpu" not in skip:
-        cores = min(os.cpu_count() or 1, 16)
+        # Every CPU this process may run on, within the host's thread budget
+        # for this job (OMP_NUM_THREADS, when the environment sets one).
+        aff = len(os.sched_getaffinity(0))
+        omp = os.environ.get("OMP_NUM_THREADS", "")
+        cores = max(1, min(aff, int(omp))) if omp.isdigit() else aff
         m, cdt = cpu_baseline(sets, cores, args.cpu_budget)
         result["cpu_baseline"] = {
             "value": round(m / cdt, 2), "unit": "codes/s", "cores": cores, "kind": "port",
+            "host_cpu_count": os.cpu_count(), "affinity_cpus": aff,
+            "omp_num_threads": omp or None,
             "sample": f"first {m} codes of the same 4096-code workload, oracle C restatement "
                       f"of kernel.py:97-203 (all 8^k-1 polys), one set per thread",
         }

@@ -119,11 +119,13 @@ void build_mplanes(uint32_t out[kNpts * kRowWords]) {
   }
 }
 
-// Default Gray walk bits per k: big enough that per-chunk set-up is a few
-// percent of the walk, small enough for a balanced tail.
+// Default Gray walk bits per k (the walk has dimension k - 1): big enough
+// that per-chunk set-up is a few percent of the walk, small enough for a
+// balanced tail; for k >= 19 large enough that a code has at most about 2^24
+// chunks (the chunk ids of one call must stay below 2^31).
 int default_L(int k) {
-  static const uint8_t tab[kMaxKExact + 1] = {4, 4, 4, 4, 4, 4, 7, 8, 10, 11, 12, 13,
-                                              14, 15, 16, 17, 18, 19, 20, 20, 20, 20};
+  static const uint8_t tab[kMaxKExact + 1] = {4, 4, 4, 4, 4, 4, 4, 7, 8, 10, 11, 12,
+                                              13, 14, 15, 16, 17, 18, 19, 21, 24, 27};
   return tab[k];
 }
 
@@ -373,7 +375,7 @@ int gt_md_batch(gt_ctx* c, const int32_t* idx, const int32_t* offsets, int32_t n
                                    : "exact evaluation supports k <= 21 (set %d has k=%d)", s, k);
     const int t = tmax_per_set ? tmax_per_set[s] : tmax;
     if (t < 48) thresholded = true;
-    if (sample_count <= 0 && k > 0) chunks += code_chunks(k, c->params.L[k]);
+    if (sample_count <= 0 && k > 0) chunks += code_chunks(walk_dim(k), c->params.L[k]);
   }
   if (chunks >= (1ull << 31)) return fail("exact search too large for one call (%llu chunks)", (unsigned long long)chunks);
   for (int64_t i = 0; i < n_idx; ++i)
@@ -417,32 +419,38 @@ int gt_md_single(gt_ctx* c, const int32_t* idx, int32_t k, int32_t tmax, int32_t
   if (k > kMaxKExact) return fail("exact evaluation supports k <= 21 (k=%d)", k);
   for (int i = 0; i < k; ++i)
     if (idx[i] < 0 || idx[i] >= kNpts) return fail("index %d outside 0..48", idx[i]);
+  const int thr = 48 - tmax;
+  const int w = walk_dim(k);
+  if (w == 0) {   // k = 1: a monomial has weight 49
+    *out = (49 <= thr) ? tmax + 1 : 0;
+    return 0;
+  }
   std::lock_guard<std::mutex> lk(c->mu);
   GT_OK(c->activate());
   hipStream_t s = c->stream;
-  // Host-built bit rows + the running minimum in one upload.
+  // Host-built bit rows of the subcode vanishing at (1,1) (rows 3i + b =
+  // x^b * (g_i + g_{k-1}), i < k - 1) + the running minimum in one upload.
   uint32_t mp[kNpts * kRowWords];
   build_mplanes(mp);
-  std::vector<uint32_t> h((size_t)3 * k * kRowWords + kRowWords, 0u);
-  for (int i = 0; i < k; ++i) {
-    uint32_t w[6];
-    for (int q = 0; q < 6; ++q) w[q] = mp[idx[i] * kRowWords + q];
+  std::vector<uint32_t> h((size_t)3 * w * kRowWords + kRowWords, 0u);
+  for (int i = 0; i < w; ++i) {
+    uint32_t v[6];
+    for (int q = 0; q < 6; ++q) v[q] = mp[idx[i] * kRowWords + q] ^ mp[idx[k - 1] * kRowWords + q];
     for (int b = 0; b < 3; ++b) {
-      for (int q = 0; q < 6; ++q) h[(size_t)(3 * i + b) * kRowWords + q] = w[q];
-      mul_x(w);
+      for (int q = 0; q < 6; ++q) h[(size_t)(3 * i + b) * kRowWords + q] = v[q];
+      mul_x(v);
     }
   }
-  const size_t best_off = (size_t)3 * k * kRowWords;
+  const size_t best_off = (size_t)3 * w * kRowWords;
   h[best_off] = 64u;
   // One-code launches want many waves: about 2^11 chunks for the largest piece.
-  const int L = std::max<int>(kU, std::min<int>(c->params.L[k], 3 * (k - 1) - kLaneBits - 11));
-  const int thr = 48 - tmax;
+  const int L = std::max<int>(kU, std::min<int>(c->params.L[k], 3 * (w - 1) - kLaneBits - 11));
   GT_OK(c->sbuf.ensure(h.size() * 4));
   GT_HIP(hipMemcpyAsync(c->sbuf.p, h.data(), h.size() * 4, hipMemcpyHostToDevice, s));
   uint32_t* d_best = c->sbuf.as<uint32_t>() + best_off;
   hipEvent_t e0 = nullptr;
   GT_OK(c->prof_begin("md_single", s, &e0));
-  GT_HIP(launch_md_single(tmax < 48, c->sbuf.as<uint32_t>(), k, L, c->params.check_log2[k], thr,
+  GT_HIP(launch_md_single(tmax < 48, c->sbuf.as<uint32_t>(), w, L, c->params.check_log2[k], thr,
                           d_best, s));
   GT_OK(c->prof_end("md_single", s, e0));
   uint32_t bw = 0;

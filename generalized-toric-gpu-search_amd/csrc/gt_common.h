@@ -19,9 +19,22 @@
 //
 // Projective enumeration.  Scalar multiples have the same zero set, so only
 // codewords whose last nonzero coefficient is 1 are visited: piece j (0 <= j <
-// k) is g_j + span_GF(2){rows 0 .. 3j-1}, 8^j codewords, walked in reflected
-// Gray order so that each step adds one bit row.  In total (8^k - 1) / 7
-// codewords instead of the reference's 8^k - 1 (kernel.py:128-139).
+// w) of a w-dimensional code is h_j + span_GF(2){rows 0 .. 3j-1}, 8^j
+// codewords, walked in reflected Gray order so that each step adds one bit
+// row.  In total (8^w - 1) / 7 codewords.
+//
+// Subcode through the point (1,1).  Every monomial is 1 at torus point 0 =
+// (1,1), so c(1,1) = sum of the coefficients.  The torus translations
+// (x,y) -> (sx, ty) act transitively on the 49 points and map C(S) onto
+// itself (monomial x^a y^b scales by s^a t^b), so a codeword with z > 0 zeros
+// has a translate of the same weight that vanishes at (1,1).  Hence for
+// k >= 2 the maximum number of zeros over C(S) is the maximum over the
+// (k-1)-dimensional subcode of codewords vanishing at (1,1), spanned by
+// h_i = g_i + g_{k-1} (i < k-1); for k = 1 it is 0 (a monomial has no zero
+// on the torus).  A repeated index still gives the zero codeword there
+// (g_a = g_b => h_a + h_b = 0, coefficients summing to 0), as in the
+// reference.  The exact walk therefore visits (8^(k-1) - 1) / 7 codewords
+// per code against the reference's 8^k - 1 (kernel.py:128-139).
 #pragma once
 #include <stdint.h>
 
@@ -41,14 +54,19 @@ constexpr int kMaxKCanon = 16;   // kernel_canon.py:161-162
 constexpr int kWave = 64;
 constexpr int kLaneBits = 6;     // 64 lanes own 6 Gray index bits
 
-// Per-k launch parameters chosen by the autotuner.
+// Dimension of the exact walk of a code with k generators (see above).
+GT_HD int walk_dim(int k) { return (k <= 1) ? 0 : k - 1; }
+
+// Per-k launch parameters chosen by the autotuner, indexed by the code's
+// generator count k (the walk has dimension k - 1).
 struct MdParams {
   uint8_t L[kMaxKExact + 1];      // Gray walk bits per lane for full chunks
   uint8_t check_log2[kMaxKExact + 1];  // early-exit check every 2^x blocks
 };
 
 // Chunk plan of one batch, largest codes first.  Bucket j (0 <= j < kMaxKExact)
-// holds the codes with k = kMaxKExact - j, in batch order, at positions
+// holds the codes whose walk has dimension kMaxKExact - j (k = kMaxKExact -
+// j + 1 generators), in batch order, at positions
 // pb[j] .. of the order array; its codes own chunk ids cb[j] + q * cc[j] ..
 // + cc[j] - 1 for the q-th code.  cb[kMaxKExact] is the total chunk count (0
 // when the plan overflowed 2^31 chunks).  Claiming ids in increasing order
@@ -73,7 +91,8 @@ constexpr int kU = 4;
 // Pieces j with 3j < kLaneBits + kU are enumerated directly by the "tiny" chunk.
 constexpr int kTinyTop = (kLaneBits + kU - 1) / 3;   // largest tiny j
 
-GT_HD int tiny_top(int k) { return (k - 1 < kTinyTop) ? (k - 1) : kTinyTop; }
+// Functions of the walk below take its dimension w (walk_dim(k) for a code).
+GT_HD int tiny_top(int w) { return (w - 1 < kTinyTop) ? (w - 1) : kTinyTop; }
 
 // Number of chunks of Gray piece j with walk length L (L >= kU).
 GT_HD uint64_t piece_chunks(int j, int L) {
@@ -85,17 +104,18 @@ GT_HD int piece_L(int j, int L) {
   return (B >= kLaneBits + L) ? L : (B - kLaneBits);
 }
 
-// Chunks of one code: Gray pieces j = k-1 .. tiny_top(k)+1, then one tiny chunk.
-GT_HD uint64_t code_chunks(int k, int L) {
-  if (k <= 0) return 0;
+// Chunks of a walk of dimension w: Gray pieces j = w-1 .. tiny_top(w)+1, then
+// one tiny chunk.
+GT_HD uint64_t code_chunks(int w, int L) {
+  if (w <= 0) return 0;
   uint64_t n = 1;  // tiny chunk
-  for (int j = k - 1; j > tiny_top(k); --j) n += piece_chunks(j, L);
+  for (int j = w - 1; j > tiny_top(w); --j) n += piece_chunks(j, L);
   return n;
 }
 
-// Projective codewords of a k-dimensional code: (8^k - 1) / 7.
-GT_HD uint64_t proj_codewords(int k) {
-  return (k <= 0) ? 0 : (((1ull << (3 * k)) - 1ull) / 7ull);
+// Projective codewords of a w-dimensional code: (8^w - 1) / 7.
+GT_HD uint64_t proj_codewords(int w) {
+  return (w <= 0) ? 0 : (((1ull << (3 * w)) - 1ull) / 7ull);
 }
 
 // x * v for a bit-plane vector (six words).

@@ -7,16 +7,32 @@
 
 namespace gt {
 
-constexpr int kServeMaxK = 16;   // the indices of a request fit one uint4
-constexpr int kServeRowWords = 3 * kServeMaxK * kRowWords;   // bit rows of one request
+// The exact limit of the batch path (kernel.py:246-247); 21 six-bit indices
+// fit one uint4 in a slot.
+constexpr int kServeMaxK = kMaxKExact;
+// Bit rows of one request's walk: 3 (k - 1) rows of the subcode vanishing at
+// (1,1) (gt_common.h).
+constexpr int kServeRowWords = 3 * (kServeMaxK - 1) * kRowWords;
 
-// One request descriptor (fine-grained host memory, written by the host).
+// One request descriptor (fine-grained host memory, written by the host):
+// 48 bytes, read by the dispatcher with three 16-byte loads.
 struct ServeReq {
   int32_t k;            // bits 0..7 k, bits 8..15 walk length L for this request (0: default)
   int32_t tmax;
-  int32_t idx[kServeMaxK];
   uint64_t tag;
+  uint8_t idx[32];      // exponent indices 0..48, entries k.. unused
 };
+
+// Slot copy of a request's indices: index i in bits 6i .. 6i+5 of 128 bits.
+__host__ __device__ __forceinline__ uint32_t unpack_idx6(const uint4& v, uint32_t i) {
+  const uint64_t lo = ((uint64_t)v.y << 32) | v.x, hi = ((uint64_t)v.w << 32) | v.z;
+  const uint32_t bit = 6u * i;
+  uint64_t x;
+  if (bit + 6u <= 64u) x = lo >> bit;
+  else if (bit >= 64u) x = hi >> (bit - 64u);
+  else x = (lo >> bit) | (hi << (64u - bit));
+  return (uint32_t)(x & 63u);
+}
 
 // One result (fine-grained host memory, written by the device with one 64-bit
 // store): bits 0..31 the result, bits 32..63 (request index + 1) mod 2^32, so
@@ -65,6 +81,7 @@ struct ServeDev {
   uint32_t slots;       // power of two; in-flight requests <= slots
   uint64_t idle_ticks;  // dispatcher idle limit (device wall clock ticks)
   uint64_t life_ticks;  // kernel lifetime limit
+  uint64_t first_req;   // first request index this launch dispatches (set by the init kernel)
   uint8_t L[kServeMaxK + 1];
   uint8_t check_log2[kServeMaxK + 1];
   int32_t* s_k;         // per slot: k | (L << 8), 0 when the request has no chunks
@@ -73,7 +90,7 @@ struct ServeDev {
   uint32_t* s_nch;
   uint64_t* s_seq;
   uint64_t* ord_start;  // 4 * slots tagged entries (ord_entry), request r at r % (4 * slots)
-  uint4* s_idx;         // per slot: the request's exponent indices, one byte each
+  uint4* s_idx;         // per slot: the request's exponent indices, 6 bits each (unpack_idx6)
   uint64_t stats[kServeStats];
 };
 
@@ -91,7 +108,10 @@ __host__ __device__ __forceinline__ uint64_t ord_entry(uint64_t r, uint64_t star
 }
 
 int serve_blocks_per_cu();
+// first_req: the request index the new kernel starts dispatching at (the
+// host keeps its numbering when a kernel that stopped itself left published
+// requests undispatched).
 hipError_t launch_serve(int grid, ServeDev* d, ServeHost* h, const uint32_t* mplanes,
-                        hipStream_t s);
+                        uint64_t first_req, hipStream_t s);
 
 }  // namespace gt

@@ -129,10 +129,11 @@ __device__ __forceinline__ uint32_t tiny_chunk(const Rows& row, int jt, uint32_t
   return best;
 }
 
-// Map a code-local chunk id to (piece j, chunk c); j == tiny_top(k) means tiny.
-__device__ __forceinline__ void locate_chunk(int k, int L, uint64_t local, int& j, uint64_t& c) {
-  const int jt = tiny_top(k);
-  j = k - 1;
+// Map a code-local chunk id of a walk of dimension w to (piece j, chunk c);
+// j == tiny_top(w) means tiny.
+__device__ __forceinline__ void locate_chunk(int w, int L, uint64_t local, int& j, uint64_t& c) {
+  const int jt = tiny_top(w);
+  j = w - 1;
   for (; j > jt; --j) {
     const uint64_t pc = piece_chunks(j, L);
     if (local < pc) break;
@@ -141,17 +142,18 @@ __device__ __forceinline__ void locate_chunk(int k, int L, uint64_t local, int& 
   c = local;
 }
 
+// One chunk of the walk of dimension w over rows 0 .. 3w-1.
 template <bool THR, class Rows>
-__device__ __forceinline__ uint32_t run_chunk(const Rows& row, int k, int L, uint64_t local,
+__device__ __forceinline__ uint32_t run_chunk(const Rows& row, int w, int L, uint64_t local,
                                               uint32_t lane, int thr, uint32_t check_mask,
                                               const uint32_t* best_g) {
   int j;
   uint64_t c;
-  locate_chunk(k, L, local, j, c);
-  if (j > tiny_top(k))
+  locate_chunk(w, L, local, j, c);
+  if (j > tiny_top(w))
     return gray_chunk<THR>(row, (uint32_t)j, (uint32_t)piece_L(j, L), c, lane, thr, check_mask,
                            best_g);
-  return tiny_chunk(row, tiny_top(k), lane);
+  return tiny_chunk(row, tiny_top(w), lane);
 }
 
 }  // namespace gt

@@ -5,10 +5,13 @@
 // and eval_min_distance / eval_min_distance_single_bp (kernel.py:63-94,
 // kernel_cuda_bp.py:99-127).  Work decomposition (see gt_common.h):
 //
-//   md_prep     workgroup 0: codes bucketed by k, largest k first (MdPlan +
-//               order[]), reset of the per-code minimum and of the chunk
-//               counters; workgroups 1..: one thread per generator, the three
-//               bit rows x^b * g_i (md_rows alone for the sampled path).
+//   md_prep     workgroup 0: codes bucketed by walk dimension, largest first
+//               (MdPlan + order[]), reset of the per-code minimum and of the
+//               chunk counters; workgroups 1..: one wave per code, the bit
+//               rows x^b * (g_i + g_{k-1}) of the subcode vanishing at the
+//               torus point (1,1) (gt_common.h), which has the same maximum
+//               number of zeros as C(S) with 1/8 of the codewords.  The
+//               sampled path keeps the full rows x^b * g_i (md_rows).
 //   md_main     persistent: each wave claims a chunk id with one atomic on
 //               one of 16 claim counters, finds its bucket by binary search
 //               over 21 entries and its code by one division (scalar work),
@@ -50,6 +53,44 @@ __device__ __forceinline__ void md_rows_one(int64_t t, const int32_t* __restrict
   }
 }
 
+// Exact-walk rows of one code (one wave): lane 3i + b (i < k - 1) writes
+// x^b * (g_i + g_{k-1}) at row 3i + b of the code.  Lanes i < k check their
+// index.  Codes with k < 2 have no walk; k > 21 is flagged by the plan.
+__device__ __forceinline__ void md_rows_code(int code, int32_t n,
+                                             const int32_t* __restrict__ idx,
+                                             const int32_t* __restrict__ offsets,
+                                             const uint32_t* __restrict__ mplanes,
+                                             uint32_t* __restrict__ rows, uint32_t* ctrl) {
+  if (code >= n) return;
+  const uint32_t lane = lane_id();
+  const int o = offsets[code];
+  const int k = offsets[code + 1] - o;
+  if (k < 1 || k > kMaxKExact) return;
+  const uint32_t i = lane / 3u, bpow = lane - 3u * i;
+  int g = 0;
+  bool bad = false;
+  if (i < (uint32_t)k) {
+    g = idx[o + (int)i];
+    bad = g < 0 || g >= kNpts;
+  }
+  if (__ballot(bad) != 0ull) {
+    if (lane == 0) atomicOr(&ctrl[2], kErrBadIndex);
+    return;
+  }
+  if (k < 2 || i + 1 >= (uint32_t)k) return;
+  const int gl = idx[o + k - 1];
+  const uint4 a0 = *reinterpret_cast<const uint4*>(mplanes + g * kRowWords);
+  const uint2 a1 = *reinterpret_cast<const uint2*>(mplanes + g * kRowWords + 4);
+  const uint4 b0 = *reinterpret_cast<const uint4*>(mplanes + gl * kRowWords);
+  const uint2 b1 = *reinterpret_cast<const uint2*>(mplanes + gl * kRowWords + 4);
+  uint32_t w[6] = {a0.x ^ b0.x, a0.y ^ b0.y, a0.z ^ b0.z, a0.w ^ b0.w, a1.x ^ b1.x, a1.y ^ b1.y};
+  if (bpow >= 1u) mul_x(w);
+  if (bpow >= 2u) mul_x(w);
+  uint4* out = reinterpret_cast<uint4*>(rows + ((size_t)o * 3 + lane) * kRowWords);
+  out[0] = make_uint4(w[0], w[1], w[2], w[3]);
+  out[1] = make_uint4(w[4], w[5], 0u, 0u);
+}
+
 __global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict__ idx,
                                                       int64_t n_idx,
                                                       const uint32_t* __restrict__ mplanes,
@@ -89,7 +130,7 @@ __device__ __forceinline__ void md_plan_load_batch(const int32_t* __restrict__ o
     if (i < n) {
       const int k = kk[r];
       if (k < 0 || k > kMaxKExact) err |= kErrBadK;
-      else if (k > 0) j = kMaxKExact - k;
+      else if (k > 1) j = kMaxKExact - walk_dim(k);
       if (first_pass) best[i] = 64u;
     }
     jb[r * kPlanThreads + tid] = (int8_t)j;
@@ -113,7 +154,10 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
   const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
   uint32_t err = 0;
   if (tid < kClaimShards) claims[tid * kClaimStride] = 0u;
-  if (tid < kB) ccs[tid] = code_chunks(kMaxKExact - tid, p->L[kMaxKExact - tid]);
+  if (tid < kB) {
+    const int w = kMaxKExact - tid;   // walk dimension of bucket tid (k = w + 1)
+    ccs[tid] = code_chunks(w, p->L[min(w + 1, kMaxKExact)]);
+  }
   if (tid == 0) present_s = 0u;
   __syncthreads();
 
@@ -207,7 +251,7 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
 }
 
 // Plan and bit rows in one launch: workgroup 0 builds the plan, workgroups
-// 1.. write the rows (the two are independent).
+// 1.. write the rows, one wave per code (the two are independent).
 __global__ __launch_bounds__(kPlanThreads) void md_prep_kernel(
     const int32_t* __restrict__ idx, int64_t n_idx, const uint32_t* __restrict__ mplanes,
     uint32_t* __restrict__ rows, const int32_t* __restrict__ offsets, int32_t n,
@@ -217,8 +261,9 @@ __global__ __launch_bounds__(kPlanThreads) void md_prep_kernel(
     md_plan_body(offsets, n, p, plan, order, claims, best, ctrl);
     return;
   }
-  md_rows_one((int64_t)(blockIdx.x - 1) * kPlanThreads + threadIdx.x, idx, n_idx, mplanes, rows,
-              ctrl);
+  (void)n_idx;
+  md_rows_code((int)((blockIdx.x - 1) * (kPlanThreads / kWave) + threadIdx.x / kWave), n, idx,
+               offsets, mplanes, rows, ctrl);
 }
 
 // Claim loop.  A workgroup's home shard is blockIdx.x % kClaimShards
@@ -270,9 +315,10 @@ __global__ __launch_bounds__(256) void md_main_kernel(
       const uint32_t cc = plan->cc[lo];
       const uint32_t q = rel / cc;
       code = (int)order[plan->pb[lo] + q];
-      const int k = kMaxKExact - lo;
-      const int L = p->L[k];
-      const uint32_t cmask = (1u << p->check_log2[k]) - 1u;
+      const int w = kMaxKExact - lo;                // walk dimension
+      const int kq = min(w + 1, kMaxKExact);        // generators of the code
+      const int L = p->L[kq];
+      const uint32_t cmask = (1u << p->check_log2[kq]) - 1u;
       const uint32_t* R = rows + (size_t)offsets[code] * 3 * kRowWords;
       int thr = -1;
       bool skip = false;
@@ -283,7 +329,7 @@ __global__ __launch_bounds__(256) void md_main_kernel(
         skip = (int)bw <= thr;
       }
       if (!skip) {
-        b = run_chunk<THR>(ConstRows{R}, k, L, rel - q * cc, lane, thr, cmask, &best_w[code]);
+        b = run_chunk<THR>(ConstRows{R}, w, L, rel - q * cc, lane, thr, cmask, &best_w[code]);
         b = wave_min(b);
         fold = THR;
       }
@@ -336,18 +382,19 @@ __global__ __launch_bounds__(256) void md_finalize_kernel(const int32_t* __restr
   const int k = offsets[i + 1] - offsets[i];
   if (k <= 0 || k > kMaxKExact) { out[i] = 0; return; }
   const int tmax = tmax_arr ? tmax_arr[i] : tmax_s;
-  const int b = (int)best[i];
+  // k = 1: a monomial has weight 49 (no walk was run).
+  const int b = (k == 1) ? 49 : (int)best[i];
   out[i] = (b <= 48 - tmax) ? tmax + 1 : 49 - b;
 }
 
 template <bool THR>
-__global__ __launch_bounds__(64) void md_single_kernel(const uint32_t* __restrict__ R, int k, int L,
+__global__ __launch_bounds__(64) void md_single_kernel(const uint32_t* __restrict__ R, int w, int L,
                                                        uint32_t check_mask, int thr,
                                                        uint32_t* best_out) {
   const uint32_t lane = lane_id();
   if (THR && (int)__hip_atomic_load(best_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= thr)
     return;
-  uint32_t b = run_chunk<THR>(ConstRows{R}, k, L, blockIdx.x, lane, thr, check_mask, best_out);
+  uint32_t b = run_chunk<THR>(ConstRows{R}, w, L, blockIdx.x, lane, thr, check_mask, best_out);
   b = wave_min(b);
   if (lane == 0) atomicMin(best_out, b);
 }
@@ -484,7 +531,8 @@ hipError_t launch_md_prep(const int32_t* d_idx, int64_t n_idx, const uint32_t* d
                           const MdParams* d_params, MdPlan* d_plan, uint32_t* d_order,
                           uint32_t* d_claims, uint32_t* d_best, uint32_t* d_ctrl,
                           hipStream_t s) {
-  const int64_t row_blocks = (std::max<int64_t>(n_idx, 0) + kPlanThreads - 1) / kPlanThreads;
+  constexpr int kCodesPerBlock = kPlanThreads / kWave;   // one wave per code
+  const int64_t row_blocks = ((int64_t)std::max(n, 0) + kCodesPerBlock - 1) / kCodesPerBlock;
   hipLaunchKernelGGL(md_prep_kernel, dim3((unsigned)(1 + row_blocks)), dim3(kPlanThreads), 0, s,
                      d_idx, n_idx, d_mplanes, d_rows, d_offsets, n, d_params, d_plan, d_order,
                      d_claims, d_best, d_ctrl);
@@ -522,17 +570,17 @@ hipError_t launch_md_finalize(const int32_t* d_offsets, int32_t n, const uint32_
   return hipGetLastError();
 }
 
-hipError_t launch_md_single(bool thresholded, const uint32_t* d_rows, int k, int L,
+hipError_t launch_md_single(bool thresholded, const uint32_t* d_rows, int w, int L,
                             int check_log2, int thr, uint32_t* d_best, hipStream_t s) {
-  const uint64_t grid = code_chunks(k, L);
+  const uint64_t grid = code_chunks(w, L);
   if (grid == 0) return hipSuccess;
   if (grid >= (1ull << 31)) return hipErrorInvalidValue;
   const uint32_t cmask = (1u << check_log2) - 1u;
   if (thresholded)
-    hipLaunchKernelGGL(md_single_kernel<true>, dim3((unsigned)grid), dim3(64), 0, s, d_rows, k,
+    hipLaunchKernelGGL(md_single_kernel<true>, dim3((unsigned)grid), dim3(64), 0, s, d_rows, w,
                        L, cmask, thr, d_best);
   else
-    hipLaunchKernelGGL(md_single_kernel<false>, dim3((unsigned)grid), dim3(64), 0, s, d_rows, k,
+    hipLaunchKernelGGL(md_single_kernel<false>, dim3((unsigned)grid), dim3(64), 0, s, d_rows, w,
                        L, cmask, thr, d_best);
   return hipGetLastError();
 }

@@ -45,7 +45,8 @@ double now_s() {
   return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
-uint64_t proj_cw(int k) { return (k <= 0) ? 0 : ((1ull << (3 * k)) - 1ull) / 7ull; }
+// Projective codewords the exact walk of a k-point request visits.
+uint64_t walk_cw(int k) { return proj_codewords(walk_dim(k)); }
 
 }  // namespace
 
@@ -95,14 +96,46 @@ struct gt_server {
     // Fresh numbering for a fresh kernel (only with nothing in flight).
     __atomic_store_n(&h->published, 0ull, __ATOMIC_RELEASE);
     __atomic_store_n(&h->consumed, 0ull, __ATOMIC_RELEASE);
-    __atomic_store_n(&h->shutdown, 0u, __ATOMIC_RELEASE);
-    __atomic_store_n(&h->exited, 0u, __ATOMIC_RELEASE);
     for (uint32_t s = 0; s < slots; ++s) __atomic_store_n(&res[s].word, 0ull, __ATOMIC_RELEASE);
     next = oldest = outstanding = 0;
     std::fill(pending.begin(), pending.end(), 0);
+    return launch_at(0);
+  }
+
+  // A kernel that starts dispatching at request `first` (the numbering, the
+  // ring and `published` are kept).
+  int launch_at(uint64_t first) {
+    __atomic_store_n(&h->shutdown, 0u, __ATOMIC_RELEASE);
+    __atomic_store_n(&h->exited, 0u, __ATOMIC_RELEASE);
     SV_HIP(hipSetDevice(device));
-    SV_HIP(launch_serve(grid, d, h_dev, mplanes, stream));
+    SV_HIP(launch_serve(grid, d, h_dev, mplanes, first, stream));
     running = true;
+    return 0;
+  }
+
+  // Make sure a kernel is running.  A kernel that stopped by itself (idle or
+  // lifetime limit) still finishes every request it dispatched; requests
+  // published after its last dispatcher pass ([consumed, next)) are handed
+  // to a new kernel once the dispatched ones are done, so none is lost.
+  // *live = false: the old kernel's dispatched requests are still finishing.
+  int ensure_live(bool* live) {
+    *live = false;
+    if (running && !__atomic_load_n(&h->exited, __ATOMIC_ACQUIRE)) {
+      *live = true;
+      return 0;
+    }
+    if (running) {
+      const uint64_t consumed = __atomic_load_n(&h->consumed, __ATOMIC_ACQUIRE);
+      if (oldest < consumed) return 0;
+      SV_OK(stop());
+      if (next != oldest) {
+        SV_OK(launch_at(consumed));
+        *live = true;
+        return 0;
+      }
+    }
+    SV_OK(launch());
+    *live = true;
     return 0;
   }
 
@@ -117,8 +150,9 @@ struct gt_server {
     return 0;
   }
 
-  // Collect finished requests in [oldest, next); returns how many were stored.
-  int poll(uint64_t* tags, int32_t* results, int32_t max_n, double* t_done) {
+  // Collect finished requests in [oldest, next) into *n_done entries; hands
+  // undispatched requests of a kernel that stopped by itself to a new one.
+  int poll(uint64_t* tags, int32_t* results, int32_t max_n, double* t_done, int32_t* n_done) {
     int n = 0;
     const double t = t_done ? now_s() : 0.0;
     for (uint64_t r = oldest; r < next && n < max_n; ++r) {
@@ -134,7 +168,12 @@ struct gt_server {
       ++n;
     }
     while (oldest < next && !pending[oldest & (slots - 1)]) ++oldest;
-    return n;
+    *n_done = n;
+    if (running && oldest != next && __atomic_load_n(&h->exited, __ATOMIC_ACQUIRE)) {
+      bool live = false;
+      SV_OK(ensure_live(&live));
+    }
+    return 0;
   }
 
   // Admission: a slot inside the window and room in the work budget.
@@ -148,21 +187,19 @@ struct gt_server {
     if (k < 0 || k > kServeMaxK) return sfail("served requests need 0 <= k <= %d (k=%d)", kServeMaxK, k);
     for (int i = 0; i < k; ++i)
       if (idx[i] < 0 || idx[i] >= kNpts) return sfail("index %d outside 0..48", idx[i]);
-    if (!running || __atomic_load_n(&h->exited, __ATOMIC_ACQUIRE)) {
-      if (next != oldest) return 0;   // wait for in-flight results first
-      if (running) { SV_OK(stop()); }
-      SV_OK(launch());
-    }
-    const uint64_t w = proj_cw(k);
+    bool live = false;
+    SV_OK(ensure_live(&live));
+    if (!live) return 0;   // the stopped kernel's requests are still finishing
+    const uint64_t w = walk_cw(k);
     if (!can_admit(w)) return 0;
     const uint32_t s = (uint32_t)(next & (slots - 1));
     ServeReq& q = ring[s];
     int32_t Lq = 0;
-    if (short_on && k >= 1 && outstanding < short_below && short_L[k] < dh.L[k]) Lq = short_L[k];
+    if (short_on && k >= 2 && outstanding < short_below && short_L[k] < dh.L[k]) Lq = short_L[k];
     q.k = k | (Lq << 8);
     q.tmax = tmax;
-    for (int i = 0; i < kServeMaxK; ++i) q.idx[i] = (i < k) ? idx[i] : 0;
     q.tag = t;
+    for (int i = 0; i < (int)sizeof(q.idx); ++i) q.idx[i] = (uint8_t)((i < k) ? idx[i] : 0);
     tag[s] = t;
     work[s] = w;
     pending[s] = 1;
@@ -299,8 +336,7 @@ int gt_serve_set_short_chunks(gt_server* sv, const int32_t* L_short_by_k, int64_
 int gt_serve_poll(gt_server* sv, uint64_t* tags, int32_t* results, int32_t max_n, int32_t* n_done) {
   if (!sv || !n_done || max_n < 0) return sfail("bad argument");   // null tags/results: discarded
   std::lock_guard<std::mutex> lk(sv->mu);
-  *n_done = sv->poll(tags, results, max_n, nullptr);
-  return 0;
+  return sv->poll(tags, results, max_n, nullptr, n_done);
 }
 
 int gt_serve_in_flight(gt_server* sv, int64_t* n) {
@@ -338,7 +374,8 @@ int gt_serve_run_stream(gt_server* sv, const int32_t* idx, const int32_t* offset
       if (!acc) break;
       ++sent;
     }
-    const int got = sv->poll(tags.data(), rs.data(), 1024, td.data());
+    int32_t got = 0;
+    SV_OK(sv->poll(tags.data(), rs.data(), 1024, td.data(), &got));
     for (int i = 0; i < got; ++i) {
       const uint64_t r = tags[i];
       results[r] = rs[i];

@@ -3,14 +3,16 @@
 // Requests (k <= kServeMaxK exponent points and a threshold tmax) arrive in a
 // ring in fine-grained host memory.  Wave 0 of workgroup 0 is the dispatcher:
 // it polls the ring's publish counter, copies each new request's exponent
-// indices (16 bytes) and state into the request's slot (slot = request index
-// mod slots), appends the request's chunk range to the global chunk sequence
-// and publishes the new chunk total.  Every other wave is a worker: it claims
-// the next global chunk id with one atomic, waits until that id is published,
-// finds the request by binary search over the in-flight window, builds the
-// request's bit rows in its LDS area (lane 3i + b: generator i times x^b, from
-// the evaluation planes), walks the chunk (the Gray walk of the batch kernel)
-// and folds the wave minimum into the slot.  Building the rows in the workers
+// indices (6 bits each, one 16-byte store) and state into the request's slot
+// (slot = request index mod slots), appends the request's chunk range to the
+// global chunk sequence and publishes the new chunk total.  Every other wave
+// is a worker: it claims the next global chunk id with one atomic, waits
+// until that id is published, finds the request by a 64-ary search over the
+// in-flight window, builds the request's bit rows in its LDS area (lane
+// 3i + b: x^b * (g_i + g_{k-1}), the subcode vanishing at the torus point
+// (1,1), which has the same maximum number of zeros; gt_common.h), walks the
+// chunk (the Gray walk of the batch kernel) and folds the wave minimum into
+// the slot.  Building the rows in the workers
 // costs them what copying prebuilt rows did and keeps the dispatcher, one wave
 // for the whole device, to a few stores per request.  The wave that finishes
 // the last chunk of a request
@@ -29,6 +31,11 @@
 // Exit: every wave leaves when the host sets `shutdown`, when the dispatcher
 // has been idle for idle_ticks of the device wall clock, or after life_ticks;
 // workers wait on nothing but the published chunk total and the stop flag.
+// Every chunk published before the stop is still walked (a worker holding a
+// published id walks it before it looks at the flag), so every dispatched
+// request gets its result.  Requests the host published but the dispatcher
+// had not taken when it stopped (`consumed` < `published`) are dispatched by
+// the next launch, which the host starts at request `consumed`.
 #include "gt_serve.h"
 #include "gt_walk.h"
 
@@ -42,7 +49,8 @@ __device__ __forceinline__ T ld_rlx(const T* p) {
 __device__ void dispatcher(ServeDev* d, ServeHost* h, uint32_t lane) {
   const uint64_t t_start = wall_clock64();
   uint64_t t_idle = t_start;
-  uint64_t disp = 0;          // requests dispatched so far
+  uint64_t disp = d->first_req;   // requests dispatched so far (numbered across launches)
+  const uint64_t disp0 = disp;
   uint64_t total = 0;         // chunks published so far
   uint64_t passes = 0, busy = 0;
   const uint32_t slots = d->slots;
@@ -58,7 +66,7 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, uint32_t lane) {
     if (stop) {
       if (lane == 0) {
         atomicAdd((unsigned long long*)&d->stats[kStDispPasses], passes);
-        atomicAdd((unsigned long long*)&d->stats[kStDispRequests], disp);
+        atomicAdd((unsigned long long*)&d->stats[kStDispRequests], disp - disp0);
         atomicAdd((unsigned long long*)&d->stats[kStDispTicks], busy);
         atomicAdd((unsigned long long*)&d->stats[kStLifeTicks], now - t_start);
         __hip_atomic_store(&d->shutdown, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -80,41 +88,47 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, uint32_t lane) {
     const uint32_t slot = (uint32_t)(r & (slots - 1));
     uint32_t nch = 0;
     int k = 0;
+    int32_t imm = 0;   // immediate result of a request without chunks
     if (lane < n_new) {
-      // The descriptor (80 bytes) in five 16-byte loads: one round trip to
+      // The descriptor (48 bytes) in three 16-byte loads: one round trip to
       // host memory.
-      static_assert(sizeof(ServeReq) == 5 * sizeof(uint4), "ServeReq layout");
+      static_assert(sizeof(ServeReq) == 3 * sizeof(uint4), "ServeReq layout");
       const uint4* qv = reinterpret_cast<const uint4*>(&h->ring[slot]);
-      uint4 v[5];
-#pragma unroll
-      for (int i = 0; i < 5; ++i) v[i] = qv[i];
-      int32_t qi[4 * 5];
-#pragma unroll
-      for (int i = 0; i < 5; ++i) {
-        qi[4 * i] = (int32_t)v[i].x; qi[4 * i + 1] = (int32_t)v[i].y;
-        qi[4 * i + 2] = (int32_t)v[i].z; qi[4 * i + 3] = (int32_t)v[i].w;
-      }
+      const uint4 v0 = qv[0], v1 = qv[1], v2 = qv[2];
       // Bits 0..7: k; bits 8..15: the request's walk length L (0: the
       // server's per-k default).
-      k = qi[0] & 0xFF;
-      const int Lq = (qi[0] >> 8) & 0xFF;
-      const int32_t q_tmax = qi[1];
+      k = (int)(v0.x & 0xFFu);
+      const int Lq = (int)((v0.x >> 8) & 0xFFu);
+      const int32_t q_tmax = (int32_t)v0.y;
       int L = 0;
-      if (k >= 1 && k <= kServeMaxK) L = (Lq != 0) ? Lq : (int)d->L[k];
-      nch = (L != 0) ? (uint32_t)code_chunks(k, L) : 0u;
-      uint32_t packed[4] = {0u, 0u, 0u, 0u};
+      if (k >= 2 && k <= kServeMaxK) L = (Lq != 0) ? Lq : (int)d->L[k];
+      nch = (L != 0) ? (uint32_t)code_chunks(walk_dim(k), L) : 0u;
+      const uint32_t ib[8] = {v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, v2.z, v2.w};
+      uint64_t lo = 0, hi = 0;
 #pragma unroll
       for (int i = 0; i < kServeMaxK; ++i) {
-        int g = qi[2 + i];
-        if (i >= k || g < 0 || g >= kNpts) g = 0;   // the host checked the indices
-        packed[i >> 2] |= (uint32_t)g << (8 * (i & 3));
+        uint32_t g = (ib[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+        if (i >= k || g >= (uint32_t)kNpts) g = 0;   // the host checked the indices
+        const int bit = 6 * i;
+        if (bit + 6 <= 64) {
+          lo |= (uint64_t)g << bit;
+        } else if (bit >= 64) {
+          hi |= (uint64_t)g << (bit - 64);
+        } else {
+          lo |= (uint64_t)g << bit;
+          hi |= (uint64_t)g >> (64 - bit);
+        }
       }
-      d->s_idx[slot] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+      d->s_idx[slot] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi,
+                                  (uint32_t)(hi >> 32));
       d->s_k[slot] = (nch != 0u) ? (k | (L << 8)) : 0;
       d->s_tmax[slot] = q_tmax;
       d->s_state[slot] = 64ull;
       d->s_nch[slot] = nch;
       d->s_seq[slot] = r + 1;
+      // k <= 1 needs no walk: k = 0 gives 0, k = 1 (a monomial, weight 49)
+      // gives 0 or, below tmax = -1, the clamp tmax + 1; unsupported k: -1.
+      if (nch == 0u) imm = (k == 0) ? 0 : (k == 1) ? ((49 <= 48 - q_tmax) ? q_tmax + 1 : 0) : -1;
     }
     // Exclusive prefix of chunk counts across the new requests.
     uint32_t incl = nch;
@@ -126,8 +140,8 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, uint32_t lane) {
     const uint32_t sum = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
     if (lane < n_new) {
       d->ord_start[r & ord_mask] = ord_entry(r, total + (incl - nch));
-      if (nch == 0u)   // k = 0 (result 0) or unsupported k (result -1): done now
-        __hip_atomic_store(&h->results[slot].word, serve_res_word(r, (k == 0) ? 0 : -1),
+      if (nch == 0u)   // k <= 1 or unsupported k: done now
+        __hip_atomic_store(&h->results[slot].word, serve_res_word(r, imm),
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     disp += n_new;
@@ -207,7 +221,8 @@ __device__ void worker(ServeDev* d, ServeHost* h, const uint32_t* __restrict__ m
     // entries with one load, one per lane, and keeps the last true one; two
     // dependent loads for a window of 4096 requests instead of twelve.
     const uint64_t disp = uniform64(ld_rlx(&d->dispatched));
-    uint64_t lo = (disp > slots) ? disp - slots : 0, hi = disp - 1;
+    const uint64_t first_req = d->first_req;   // requests before it have no entries here
+    uint64_t lo = (disp > first_req + slots) ? disp - slots : first_req, hi = disp - 1;
     uint64_t step = 1;
     while (step * kWave < hi - lo + 1) step *= kWave;
     uint64_t e_lo = 0;
@@ -243,23 +258,19 @@ __device__ void worker(ServeDev* d, ServeHost* h, const uint32_t* __restrict__ m
     uint64_t t2 = t1;
     if (!skip) {
       if (slot != staged_slot || seq != staged_seq) {
-        // Lane 3i + b < 48 writes row 3i + b: generator i times x^b (zero for
-        // i >= k).
+        // Lane 3i + b < 3(k-1) writes row 3i + b: x^b * (g_i + g_{k-1}).
         const uint4 iv = d->s_idx[slot];
         const uint32_t i = lane / 3u, bpow = lane - 3u * i;
-        const uint32_t word = (i < 4u) ? iv.x : (i < 8u) ? iv.y : (i < 12u) ? iv.z : iv.w;
-        const uint32_t gidx = (word >> (8u * (i & 3u))) & 0xFFu;
-        uint32_t w[6];
+        const uint32_t gi = unpack_idx6(iv, (i < (uint32_t)kServeMaxK) ? i : 0u);
+        const uint32_t gl = unpack_idx6(iv, (uint32_t)(k - 1));
+        if (i + 1u < (uint32_t)k) {
+          uint32_t w[6];
 #pragma unroll
-        for (int t = 0; t < 6; ++t) w[t] = (i < (uint32_t)k) ? mplanes[gidx * kRowWords + t] : 0u;
-        uint32_t w1[6] = {w[0], w[1], w[2], w[3], w[4], w[5]};
-        mul_x(w1);
-        uint32_t w2[6] = {w1[0], w1[1], w1[2], w1[3], w1[4], w1[5]};
-        mul_x(w2);
-        if (lane < 3u * kServeMaxK) {
+          for (int t = 0; t < 6; ++t) w[t] = mplanes[gi * kRowWords + t] ^ mplanes[gl * kRowWords + t];
+          if (bpow >= 1u) mul_x(w);
+          if (bpow >= 2u) mul_x(w);
 #pragma unroll
-          for (int t = 0; t < 6; ++t)
-            lrows[lane * kRowWords + t] = (bpow == 0u) ? w[t] : (bpow == 1u) ? w1[t] : w2[t];
+          for (int t = 0; t < 6; ++t) lrows[lane * kRowWords + t] = w[t];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -270,7 +281,7 @@ __device__ void worker(ServeDev* d, ServeHost* h, const uint32_t* __restrict__ m
       }
       t2 = wall_clock64();
       const uint32_t cmask = (1u << d->check_log2[k]) - 1u;
-      b = run_chunk<true>(LdsRows{lrows}, k, L, local, lane, thr, cmask, best_g);
+      b = run_chunk<true>(LdsRows{lrows}, walk_dim(k), L, local, lane, thr, cmask, best_g);
       b = wave_min(b);
       ++st_chunks;
     } else {
@@ -331,7 +342,7 @@ __global__ __launch_bounds__(256) void serve_kernel(ServeDev* d, ServeHost* h,
 
 // Resets the counters and clears the ord_start ring (a cleared entry carries
 // tag 0, which no request has).
-__global__ void serve_init_kernel(ServeDev* d) {
+__global__ void serve_init_kernel(ServeDev* d, uint64_t first_req) {
   const uint64_t n = 4ull * d->slots;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -339,7 +350,8 @@ __global__ void serve_init_kernel(ServeDev* d) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     d->next_chunk = 0;
     d->avail = 0;
-    d->dispatched = 0;
+    d->dispatched = first_req;
+    d->first_req = first_req;
     d->shutdown = 0;
   }
   if (blockIdx.x == 0 && threadIdx.x < kServeStats) d->stats[threadIdx.x] = 0ull;
@@ -354,8 +366,8 @@ int serve_blocks_per_cu() {
 }
 
 hipError_t launch_serve(int grid, ServeDev* d, ServeHost* h, const uint32_t* mplanes,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(serve_init_kernel, dim3(64), dim3(256), 0, s, d);
+                        uint64_t first_req, hipStream_t s) {
+  hipLaunchKernelGGL(serve_init_kernel, dim3(64), dim3(256), 0, s, d, first_req);
   hipLaunchKernelGGL(serve_kernel, dim3(grid), dim3(256), 0, s, d, h, mplanes);
   return hipGetLastError();
 }

@@ -1,8 +1,9 @@
 """Autotuner for the min-distance launch parameters, per k.
 
-Tunable per k (gt_set_md_params):
+Tunable per k (gt_set_md_params; a code of k points walks the (k-1)-dimensional
+subcode vanishing at the torus point (1,1), csrc/gt_common.h):
   L           Gray walk bits per lane; one chunk = 64 lanes x 2^L codewords, so
-              L sets the number of waves per code: code_chunks(k, L).
+              L sets the number of waves per code: code_chunks(k - 1, L).
   check_log2  early-exit check every 2^check_log2 blocks of 16 steps
               (thresholded searches only).
 
@@ -29,20 +30,31 @@ KU = 4
 LANE_BITS = 6
 
 
-def code_chunks(k: int, L: int) -> int:
-    """Host copy of gt_common.h:code_chunks (chunks per code)."""
-    if k <= 0:
+def walk_dim(k: int) -> int:
+    """Dimension of the exact walk of a k-point code (gt_common.h:walk_dim)."""
+    return 0 if k <= 1 else k - 1
+
+
+def walk_codewords(k: int) -> int:
+    """Projective codewords the exact walk of a k-point code visits."""
+    w = walk_dim(k)
+    return (8 ** w - 1) // 7
+
+
+def code_chunks(w: int, L: int) -> int:
+    """Host copy of gt_common.h:code_chunks (chunks of a walk of dimension w)."""
+    if w <= 0:
         return 0
-    jt = min(k - 1, (LANE_BITS + KU - 1) // 3)
+    jt = min(w - 1, (LANE_BITS + KU - 1) // 3)
     n = 1
-    for j in range(k - 1, jt, -1):
+    for j in range(w - 1, jt, -1):
         B = 3 * j
         n += (1 << (B - LANE_BITS - L)) if B >= LANE_BITS + L else 1
     return n
 
 
 def candidate_L(k: int) -> list[int]:
-    top = max(KU, 3 * (k - 1) - LANE_BITS)
+    top = max(KU, 3 * (walk_dim(k) - 1) - LANE_BITS)
     return list(range(KU, min(top, 20) + 1))
 
 
@@ -82,7 +94,7 @@ def tune(ctx: _native.Context, ks=range(3, 11), codes_per_k: dict | None = None,
             ctx.set_md_params(k, L, saved[1])
             t = _time_md(ctx, flat, offs, 48, repeats)
             if verbose:
-                print(f"[tune] k={k} L={L:2d} chunks/code={code_chunks(k, L):6d} md_main {t:.4f} ms")
+                print(f"[tune] k={k} L={L:2d} chunks/code={code_chunks(walk_dim(k), L):6d} md_main {t:.4f} ms")
             if best is None or t < best[1]:
                 best = (L, t)
         L = best[0]
@@ -119,7 +131,7 @@ def refine(ctx: _native.Context, sets, repeats: int = 5, margin: float = 0.005,
     for s in sets:
         k = len(s)
         if k > 0:
-            work[k] = work.get(k, 0) + (8 ** k - 1) // 7
+            work[k] = work.get(k, 0) + walk_codewords(k)
     # The clocks settle over tens of milliseconds of load: time until two
     # readings in a row agree within 1 %, so the start is not a cold reading.
     prev, cur = None, _time_md(ctx, flat, offs, 48, repeats)

@@ -5,7 +5,7 @@ request ring in pinned host memory.  Requests are admitted as ring slots
 (memory budget) and outstanding projective codewords (work budget) free up;
 the work of all admitted requests is interleaved chunk by chunk on the device.
 
-Each request is (S, tmax): S with k <= 16 exponent points and tmax =
+Each request is (S, tmax): S with k <= 21 exponent points and tmax =
 48 - d_stop, so the search stops as soon as a codeword of weight <= d_stop is
 found.  The result is max_zeros clamped at tmax + 1, the same value
 ``DistanceOracle.max_zeros_batch`` returns for that request alone.
@@ -20,7 +20,7 @@ import numpy as np
 from . import _native
 from ._native import check, i32, i64, lib, ptr, vp
 
-MAX_K = 16
+MAX_K = 21
 
 
 # Serving chunks are 2^SERVE_L_OFFSET times the batch kernel's: a served chunk
@@ -55,7 +55,7 @@ class Server:
             params = {}
             for k in range(MAX_K + 1):
                 L, c = ctx.get_md_params(k)
-                Ls = min(20, L + SERVE_L_OFFSET)
+                Ls = min(30, L + SERVE_L_OFFSET)
                 if max_chunk_log2 is not None:
                     Ls = min(Ls, max(L, int(max_chunk_log2)))
                 params[k] = (Ls, c)
@@ -73,7 +73,7 @@ class Server:
             self.set_short_chunks(short, short_below)
 
     def set_short_chunks(self, L_short, below_work: int) -> None:
-        """Walk length per k (17 entries) for requests admitted below
+        """Walk length per k (22 entries) for requests admitted below
         below_work outstanding codewords; None turns it off."""
         if L_short is None:
             check(lib().gt_serve_set_short_chunks(self.handle, None, 0))

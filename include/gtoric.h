@@ -107,12 +107,14 @@ int gt_graph_destroy(gt_graph* graph);
  * (no reference counterpart; it serves the synchronous max_zeros_batch calls
  * of champion_search_canonical.py:217-234 and evolve_search.py:223-263 as a
  * stream).  slots (power of two) bounds the requests in flight (memory
- * budget); work_budget bounds their projective codewords (work budget; a
- * request larger than the budget runs alone).  L_by_k / check_by_k (17
- * entries, k = 0..16, may be NULL) are the walk parameters.  The kernel stops
- * itself after idle_s seconds without requests or life_s seconds in total and
- * is relaunched by the next submit.  Requests: k <= 16 points and a threshold
- * tmax = 48 - d_stop; results as for gt_md_batch (k = 0 gives 0). */
+ * budget); work_budget bounds the projective codewords their exact walks
+ * visit ((8^(k-1) - 1) / 7 for k points; a request larger than the budget
+ * runs alone).  L_by_k / check_by_k (22 entries, k = 0..21, may be NULL) are
+ * the walk parameters.  The kernel stops itself after idle_s seconds without
+ * requests or life_s seconds in total; requests it had not taken then are
+ * handed to a new launch by the next submit or poll, so none is lost.
+ * Requests: k <= 21 points (the exact limit of kernel.py:246-247) and a
+ * threshold tmax = 48 - d_stop; results as for gt_md_batch (k = 0 gives 0). */
 typedef struct gt_server gt_server;
 int gt_serve_start(int32_t device, int32_t slots, int64_t work_budget, const int32_t* L_by_k,
                    const int32_t* check_by_k, double idle_s, double life_s, gt_server** out);
@@ -121,7 +123,7 @@ int gt_serve_submit(gt_server* srv, const int32_t* idx, int32_t k, int32_t tmax,
                     int32_t* accepted);
 /* Load-dependent chunk length (no reference counterpart): a request admitted
  * while fewer than below_work projective codewords are outstanding walks
- * chunks of L_short_by_k[k] steps (17 entries, 4..30; used only where
+ * chunks of L_short_by_k[k] steps (22 entries, 4..30; used only where
  * shorter than the start call's L_by_k), so a lightly loaded server answers
  * large requests sooner and a loaded one keeps its long, cheaper chunks.
  * Results do not change.  NULL turns it off. */

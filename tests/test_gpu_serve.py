@@ -51,7 +51,7 @@ def test_chunk_cap_changes_chunks_not_results(ctx):
     """max_chunk_log2 splits large requests into more, shorter chunks; the
     results stay those of the batch kernel and the chunk count follows the
     capped table."""
-    from gtoric.autotune import code_chunks
+    from gtoric.autotune import code_chunks, walk_dim
     from gtoric.serving import SERVE_L_OFFSET, Server
     sets, tmax = _requests(400, seed=57, kmax=10)
     with Server(0, slots=256, life_s=60, max_chunk_log2=9) as srv:
@@ -60,25 +60,25 @@ def test_chunk_cap_changes_chunks_not_results(ctx):
     from gtoric.distance import flatten
     assert res.tolist() == ctx.md_batch(*flatten(sets), tmax).tolist()
     L = {}
-    for k in range(1, 17):
+    for k in range(1, 22):
         Lb = ctx.get_md_params(k)[0]
-        L[k] = min(min(20, Lb + SERVE_L_OFFSET), max(Lb, 9))
-    want = sum(code_chunks(len(S), L[len(S)]) for S in sets if len(S))
+        L[k] = min(min(30, Lb + SERVE_L_OFFSET), max(Lb, 9))
+    want = sum(code_chunks(walk_dim(len(S)), L[len(S)]) for S in sets if len(S))
     assert st["chunks_walked"] + st["chunks_skipped"] == want
 
 
 def test_counters_account_for_every_chunk(ctx):
     """gt_serve_stats: every request dispatched once, and chunks walked plus
     chunks skipped equal the chunk count of the served parameter table."""
-    from gtoric.autotune import code_chunks
+    from gtoric.autotune import code_chunks, walk_dim
     from gtoric.serving import SERVE_L_OFFSET, Server
     sets, tmax = _requests(500, seed=56, kmax=9)
     with Server(0, slots=256, life_s=60) as srv:
         res, lat, wall = srv.run_stream(sets, tmax, np.zeros(len(sets)), timeout_s=60)
         st = srv.stats()
-        L = {k: min(20, ctx.get_md_params(k)[0] + SERVE_L_OFFSET) for k in range(1, 17)}
+        L = {k: min(30, ctx.get_md_params(k)[0] + SERVE_L_OFFSET) for k in range(1, 22)}
     assert st["requests_dispatched"] == len(sets)
-    want = sum(code_chunks(len(S), L[len(S)]) for S in sets if len(S))
+    want = sum(code_chunks(walk_dim(len(S)), L[len(S)]) for S in sets if len(S))
     assert st["chunks_walked"] + st["chunks_skipped"] == want
     assert st["worker_waves"] > 0 and st["kernel_life_s"] > 0
     # The server relaunches on the next submit after stats() stopped it.
@@ -131,7 +131,7 @@ def test_serve_errors():
         Server(0, slots=3)
     with Server(0, slots=4, life_s=60) as srv:
         with pytest.raises(_native.GtError):
-            srv.submit(list(range(17)), 40, 0)
+            srv.submit(list(range(22)), 40, 0)
         with pytest.raises(_native.GtError):
             srv.submit([0, 50], 40, 0)
         assert srv.submit([], 40, 7)
@@ -145,7 +145,7 @@ def test_serve_errors():
 def test_short_chunks_change_chunks_not_results(ctx):
     """gt_serve_set_short_chunks: below a huge work threshold every request
     walks the short chunks, below 0 none does; results stay the batch's."""
-    from gtoric.autotune import code_chunks
+    from gtoric.autotune import code_chunks, walk_dim
     from gtoric.distance import flatten
     from gtoric.serving import Server
     sets, tmax = _requests(300, seed=58, kmax=10)
@@ -156,7 +156,43 @@ def test_short_chunks_change_chunks_not_results(ctx):
             st = srv.stats()
             L = srv.L
         assert res.tolist() == exp
-        Lb = {k: ctx.get_md_params(k)[0] for k in range(17)}
-        Luse = {k: (min(int(L[k]), max(Lb[k], 9)) if short else int(L[k])) for k in range(17)}
-        want = sum(code_chunks(len(S), Luse[len(S)]) for S in sets if len(S))
+        Lb = {k: ctx.get_md_params(k)[0] for k in range(22)}
+        Luse = {k: (min(int(L[k]), max(Lb[k], 9)) if short else int(L[k])) for k in range(22)}
+        want = sum(code_chunks(walk_dim(len(S)), Luse[len(S)]) for S in sets if len(S))
         assert st["chunks_walked"] + st["chunks_skipped"] == want, short
+
+
+def test_large_k_requests_match_the_batch_path(ctx):
+    """Requests of 11..21 points (indices packed 6 bits each in the slot, rows
+    of up to 20 generators in LDS): exact answers for k = 11..13 and
+    thresholded ones up to k = 21 equal the batch kernel's."""
+    from gtoric.distance import flatten
+    from gtoric.serving import Server
+    rng = np.random.default_rng(61)
+    sets, tmax = [], []
+    for k in (11, 12, 13, 11, 12, 13):
+        sets.append(sorted(rng.choice(49, k, replace=False).tolist()))
+        tmax.append(48)
+    for k in range(14, 22):
+        for d_stop in (38, 41, 44):
+            sets.append(sorted(rng.choice(49, k, replace=False).tolist()))
+            tmax.append(48 - d_stop)
+    tmax = np.array(tmax, dtype=np.int32)
+    exp = ctx.md_batch(*flatten(sets), tmax).tolist()
+    with Server(0, slots=64, life_s=60) as srv:
+        res, _, _ = srv.run_stream(sets, tmax, np.zeros(len(sets)), timeout_s=90)
+    assert res.tolist() == exp
+
+
+def test_requests_outlive_a_kernel_that_stops_itself(ctx):
+    """A kernel that reaches its lifetime limit leaves requests published
+    after its last dispatcher pass; the next launch takes them, so every
+    result arrives and equals the batch path's."""
+    from gtoric.distance import flatten
+    from gtoric.serving import Server
+    sets, tmax = _requests(3000, seed=62, kmax=8)
+    arrivals = np.linspace(0.0, 1.5, len(sets))        # a steady stream for 1.5 s
+    with Server(0, slots=64, work_budget=1 << 30, idle_s=0.01, life_s=0.02) as srv:
+        res, _, _ = srv.run_stream(sets, tmax, arrivals, timeout_s=60)
+        assert srv.in_flight() == 0
+    assert res.tolist() == ctx.md_batch(*flatten(sets), tmax).tolist()
