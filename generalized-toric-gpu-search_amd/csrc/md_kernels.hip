@@ -12,9 +12,9 @@
 //               torus point (1,1) (gt_common.h), which has the same maximum
 //               number of zeros as C(S) with 1/8 of the codewords.  The
 //               sampled path keeps the full rows x^b * g_i (md_rows).
-//   md_main     persistent: each wave claims a chunk id with one atomic on
-//               one of 16 claim counters, finds its bucket by binary search
-//               over 21 entries and its code by one division (scalar work),
+//   md_main     persistent: each wave takes chunk ids (a static snake deal
+//               in exact mode, claims from 16 counters in thresholded mode),
+//               decodes one against the plan in LDS (bucket, piece, code),
 //               walks 64 lanes x 2^L Gray steps, and folds its wave minimum
 //               into the code's minimum with one atomicMin.
 //   md_finalize minimum weight -> max_zeros, clamped at tmax + 1.
@@ -22,6 +22,8 @@
 // Inner step (per projective codeword, all VALU, rows in SGPRs):
 //   6 v_xor_b32 (row add), 2 v_or3_b32 (plane OR), 2 v_bcnt_u32_b32,
 //   and one v_min3_u32 per two codewords.
+#include <cstddef>
+
 #include "gt_launch.h"
 #include "gt_walk.h"
 
@@ -266,11 +268,137 @@ __global__ __launch_bounds__(kPlanThreads) void md_prep_kernel(
                offsets, mplanes, rows, ctrl);
 }
 
-// Claim loop.  A workgroup's home shard is blockIdx.x % kClaimShards
-// (workgroups go round-robin over the XCDs, so a shard's counter is mostly
-// hit from one XCD).  When the home shard runs dry the wave reads all shard
-// counters at once (lane t reads shard t), takes the next shard in ring order
-// that still has ids, and leaves once none has.
+// Chunk ids.  The plan numbers chunks bucket by bucket, largest walk first;
+// inside a bucket piece-major: every code's chunks of piece w-1, then of
+// piece w-2, ..., then one tiny chunk per code, consecutive ids of a piece
+// going to consecutive codes (code = id mod codes in the bucket).  Chunk
+// sizes therefore fall along the ids, apart from steps at bucket borders.
+//
+// The plan and the per-k parameters are copied into LDS once per workgroup,
+// so decoding an id (a binary search over 21 bucket starts and a walk over
+// the pieces) reads LDS and scalar registers only; the code index and its
+// row offset are the only global loads before the rows.
+struct MdShared {
+  uint32_t plan[sizeof(MdPlan) / 4];
+  uint8_t L[kMaxKExact + 1];
+  uint8_t check_log2[kMaxKExact + 1];
+};
+
+__device__ __forceinline__ void md_load_shared(MdShared& sh, const MdPlan* __restrict__ plan,
+                                               const MdParams* __restrict__ p) {
+  static_assert(sizeof(MdPlan) / 4 <= 256, "one plan word per thread");
+  if (threadIdx.x < sizeof(MdPlan) / 4)
+    sh.plan[threadIdx.x] = reinterpret_cast<const uint32_t*>(plan)[threadIdx.x];
+  if (threadIdx.x < kMaxKExact + 1) {
+    sh.L[threadIdx.x] = p->L[threadIdx.x];
+    sh.check_log2[threadIdx.x] = p->check_log2[threadIdx.x];
+  }
+  __syncthreads();
+}
+
+struct MdChunk {
+  int code;
+  int w;            // walk dimension
+  int L;
+  uint32_t cmask;   // early-exit check mask
+  int j;            // Gray piece, or tiny_top(w) for the tiny chunk
+  uint32_t c;       // chunk index inside the piece
+};
+
+// Wave-uniform decode of chunk id g < total.
+__device__ __forceinline__ MdChunk md_decode(const MdShared& sh, uint32_t g,
+                                             const uint32_t* __restrict__ order) {
+  const uint32_t* cb = sh.plan + offsetof(MdPlan, cb) / 4;
+  const uint32_t* pb = sh.plan + offsetof(MdPlan, pb) / 4;
+  const uint32_t* ccv = sh.plan + offsetof(MdPlan, cc) / 4;
+  // Last bucket whose first chunk id is <= g (empty buckets tie with their
+  // successor, so the last tie is the nonempty one).
+  int lo = 0, hi = kMaxKExact - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (cb[mid] <= g) lo = mid; else hi = mid - 1;
+  }
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  const uint32_t b0 = __builtin_amdgcn_readfirstlane(cb[lo]);
+  const uint32_t cc = __builtin_amdgcn_readfirstlane(ccv[lo]);
+  const uint32_t nb = (__builtin_amdgcn_readfirstlane(cb[lo + 1]) - b0) / cc;   // codes
+  MdChunk r;
+  r.w = kMaxKExact - lo;
+  const int kq = min(r.w + 1, kMaxKExact);
+  r.L = __builtin_amdgcn_readfirstlane(sh.L[kq]);
+  r.cmask = (1u << __builtin_amdgcn_readfirstlane(sh.check_log2[kq])) - 1u;
+  uint32_t rel = g - b0;
+  const int jt = tiny_top(r.w);
+  int j = r.w - 1;
+  for (; j > jt; --j) {
+    const uint32_t span = (uint32_t)piece_chunks(j, r.L) * nb;
+    if (rel < span) break;
+    rel -= span;
+  }
+  r.j = j;
+  r.c = rel / nb;
+  r.code = (int)order[__builtin_amdgcn_readfirstlane(pb[lo]) + (rel - r.c * nb)];
+  return r;
+}
+
+template <bool THR>
+__device__ __forceinline__ uint32_t md_walk(const MdChunk& ck, const uint32_t* R, uint32_t lane,
+                                            int thr, const uint32_t* best_g) {
+  uint32_t b;
+  if (ck.j > tiny_top(ck.w))
+    b = gray_chunk<THR>(ConstRows{R}, (uint32_t)ck.j, (uint32_t)piece_L(ck.j, ck.L), ck.c, lane,
+                        thr, ck.cmask, best_g);
+  else
+    b = tiny_chunk(ConstRows{R}, tiny_top(ck.w), lane);
+  return wave_min(b);
+}
+
+// Exact mode: a static deal, no claim atomics.  W waves; in round r wave i
+// walks chunk r W + i (r even) or r W + W - 1 - i (r odd).  Sizes fall along
+// the ids, so each round hands out chunks of about one size and the snake
+// order evens out what the rounds leave.  (Chunk claims from shared
+// counters cost about 90 ns per claim on one counter, serialised at the
+// device coherence point: 66 us of a 4096-code pass.)  A wave keeps the
+// running minimum of the code it walks and folds it (one atomicMin) only when
+// it moves to another code or leaves.
+__global__ __launch_bounds__(256) void md_main_exact_kernel(
+    const uint32_t* __restrict__ rows, const int32_t* __restrict__ offsets,
+    const MdPlan* __restrict__ plan, const uint32_t* __restrict__ order, uint32_t* best_w,
+    const MdParams* __restrict__ p) {
+  __shared__ MdShared sh;
+  md_load_shared(sh, plan, p);
+  const uint32_t lane = lane_id();
+  const uint32_t total = __builtin_amdgcn_readfirstlane(sh.plan[offsetof(MdPlan, cb) / 4 + kMaxKExact]);
+  const uint32_t W = gridDim.x * (blockDim.x / kWave);
+  const uint32_t wid =
+      __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+  int pcode = -1;
+  uint32_t pmin = 64u;
+  for (uint32_t r = 0;; ++r) {
+    const uint64_t g64 = (uint64_t)r * W + ((r & 1u) ? (W - 1u - wid) : wid);
+    if (g64 >= total) break;
+    const MdChunk ck = md_decode(sh, (uint32_t)g64, order);
+    const uint32_t* R = rows + (size_t)offsets[ck.code] * 3 * kRowWords;
+    const uint32_t b = md_walk<false>(ck, R, lane, -1, nullptr);
+    if (ck.code != pcode) {
+      if (pcode >= 0 && lane == 0) atomicMin(&best_w[pcode], pmin);
+      pcode = ck.code;
+      pmin = b;
+    } else {
+      pmin = min(pmin, b);
+    }
+  }
+  if (pcode >= 0 && lane == 0) atomicMin(&best_w[pcode], pmin);
+}
+
+// Thresholded mode: chunks of codes that already reached their threshold
+// are skipped, so the work per chunk is not known in advance and ids are
+// claimed.  A workgroup's home shard is blockIdx.x % kClaimShards (workgroups
+// go round-robin over the XCDs, so a shard's counter is mostly hit from one
+// XCD).  When the home shard runs dry the wave reads all shard counters at
+// once (lane t reads shard t), takes the next shard in ring order that still
+// has ids, and leaves once none has.  Every chunk is folded at once, because
+// other waves stop early on the folded value.
 //
 // Every iteration has exactly one lane-0 region, at the bottom: it folds the
 // finished chunk into its code and claims the next id; there is no
@@ -278,73 +406,35 @@ __global__ __launch_bounds__(kPlanThreads) void md_prep_kernel(
 // readfirstlane.  (With a separate lane-0 claim at the loop top the compiler
 // threaded the two regions together and the other 63 lanes re-read a stale
 // id with lane 0 inactive, walking chunk 0 forever.)
-template <bool THR>
-__global__ __launch_bounds__(256) void md_main_kernel(
+__global__ __launch_bounds__(256) void md_main_thr_kernel(
     const uint32_t* __restrict__ rows, const int32_t* __restrict__ offsets,
     const MdPlan* __restrict__ plan, const uint32_t* __restrict__ order,
     uint32_t* claims, uint32_t* best_w, const int32_t* __restrict__ tmax_arr, int32_t tmax_s,
     const MdParams* __restrict__ p) {
+  __shared__ MdShared sh;
+  md_load_shared(sh, plan, p);
   const uint32_t lane = lane_id();
-  const uint32_t total = plan->cb[kMaxKExact];
+  const uint32_t total = __builtin_amdgcn_readfirstlane(sh.plan[offsetof(MdPlan, cb) / 4 + kMaxKExact]);
   uint32_t shard = blockIdx.x % kClaimShards;
   uint32_t dry = 0;                       // shards seen without ids (bit mask)
   uint32_t first = 0;
   if (lane == 0) first = atomicAdd(&claims[shard * kClaimStride], 1u);
   uint32_t g = __builtin_amdgcn_readfirstlane(first) * kClaimShards + shard;
-  // Exact mode keeps the running minimum of the code it is walking and folds
-  // it (one atomicMin) only when it moves to another code or leaves: a code
-  // with thousands of chunks would otherwise serialise thousands of atomics on
-  // one address.  Thresholded mode folds every chunk at once, because other
-  // waves stop early on the folded value.
-  int pcode = -1;
-  uint32_t pmin = 64u;
   for (;;) {
     const bool have = g < total;
     int code = 0;
     bool fold = false;
     uint32_t b = 64u;
     if (have) {
-      // Last bucket whose first chunk id is <= g (empty buckets tie with their
-      // successor, so the last tie is the nonempty one).
-      int lo = 0, hi = kMaxKExact - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (plan->cb[mid] <= g) lo = mid; else hi = mid - 1;
-      }
-      const uint32_t rel = g - plan->cb[lo];
-      const uint32_t cc = plan->cc[lo];
-      const uint32_t q = rel / cc;
-      code = (int)order[plan->pb[lo] + q];
-      const int w = kMaxKExact - lo;                // walk dimension
-      const int kq = min(w + 1, kMaxKExact);        // generators of the code
-      const int L = p->L[kq];
-      const uint32_t cmask = (1u << p->check_log2[kq]) - 1u;
+      const MdChunk ck = md_decode(sh, g, order);
+      code = ck.code;
       const uint32_t* R = rows + (size_t)offsets[code] * 3 * kRowWords;
-      int thr = -1;
-      bool skip = false;
-      if (THR) {
-        thr = 48 - (tmax_arr ? tmax_arr[code] : tmax_s);
-        const uint32_t bw = __builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(&best_w[code], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        skip = (int)bw <= thr;
-      }
-      if (!skip) {
-        b = run_chunk<THR>(ConstRows{R}, w, L, rel - q * cc, lane, thr, cmask, &best_w[code]);
-        b = wave_min(b);
-        fold = THR;
-      }
-      if (!THR) {
-        if (code != pcode) {
-          fold = pcode >= 0;
-          const int c0 = pcode;
-          const uint32_t m0 = pmin;
-          pcode = code;
-          pmin = b;
-          code = c0;
-          b = m0;
-        } else {
-          pmin = min(pmin, b);
-        }
+      const int thr = 48 - (tmax_arr ? tmax_arr[code] : tmax_s);
+      const uint32_t bw = __builtin_amdgcn_readfirstlane(
+          __hip_atomic_load(&best_w[code], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if ((int)bw > thr) {
+        b = md_walk<true>(ck, R, lane, thr, &best_w[code]);
+        fold = true;
       }
     } else {
       dry |= 1u << shard;
@@ -368,7 +458,6 @@ __global__ __launch_bounds__(256) void md_main_kernel(
     }
     g = __builtin_amdgcn_readfirstlane(next) * kClaimShards + shard;
   }
-  if (!THR && pcode >= 0 && lane == 0) atomicMin(&best_w[pcode], pmin);
 }
 
 __global__ __launch_bounds__(256) void md_finalize_kernel(const int32_t* __restrict__ offsets,
@@ -542,8 +631,8 @@ hipError_t launch_md_prep(const int32_t* d_idx, int64_t n_idx, const uint32_t* d
 int md_main_blocks_per_cu(bool thresholded) {
   int nb = 0;
   hipError_t e = thresholded
-      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, md_main_kernel<true>, 256, 0)
-      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, md_main_kernel<false>, 256, 0);
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, md_main_thr_kernel, 256, 0)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, md_main_exact_kernel, 256, 0);
   if (e != hipSuccess || nb <= 0) nb = 4;
   return nb;
 }
@@ -554,11 +643,11 @@ hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
                           const int32_t* d_tmax, int32_t tmax, const MdParams* d_params,
                           hipStream_t s) {
   if (thresholded)
-    hipLaunchKernelGGL(md_main_kernel<true>, dim3(grid), dim3(256), 0, s, d_rows, d_offsets,
+    hipLaunchKernelGGL(md_main_thr_kernel, dim3(grid), dim3(256), 0, s, d_rows, d_offsets,
                        d_plan, d_order, d_claims, d_best, d_tmax, tmax, d_params);
   else
-    hipLaunchKernelGGL(md_main_kernel<false>, dim3(grid), dim3(256), 0, s, d_rows, d_offsets,
-                       d_plan, d_order, d_claims, d_best, d_tmax, tmax, d_params);
+    hipLaunchKernelGGL(md_main_exact_kernel, dim3(grid), dim3(256), 0, s, d_rows, d_offsets,
+                       d_plan, d_order, d_best, d_params);
   return hipGetLastError();
 }
 
