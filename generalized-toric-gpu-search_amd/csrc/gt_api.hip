@@ -133,6 +133,12 @@ struct ProfRec {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
 };
 
+// Plan area of one exact pass: claim counters (kClaimShards x 4 KB), then
+// MdPlan (256 bytes), then the order array (4 bytes per code).
+constexpr size_t kMdClaimBytes = (size_t)kClaimShards * kClaimStride * 4;
+constexpr size_t kMdPlanAreaBytes = kMdClaimBytes + 256;
+static_assert(sizeof(MdPlan) <= 256, "MdPlan fits its slot");
+
 }  // namespace
 
 int gt::set_error(const char* msg) {
@@ -234,23 +240,32 @@ struct gt_ctx {
       GT_OK(prof_end("md_sampled", s, e0));
       return 0;
     }
-    // plan buffer: claim counters (kClaimShards x 4 KB), MdPlan, order (n entries).
-    constexpr size_t kClaimBytes = (size_t)kClaimShards * kClaimStride * 4;
-    GT_OK(plan.ensure(kClaimBytes + sizeof(MdPlan) + (size_t)n * 4));
+    GT_OK(plan.ensure(kMdPlanAreaBytes + (size_t)n * 4));
     GT_OK(best.ensure((size_t)n * 4));
-    uint32_t* d_claims = plan.as<uint32_t>();
-    MdPlan* d_plan = reinterpret_cast<MdPlan*>(plan.as<char>() + kClaimBytes);
-    uint32_t* d_order = reinterpret_cast<uint32_t*>(plan.as<char>() + kClaimBytes + sizeof(MdPlan));
-    GT_HIP(launch_md_prep(d_idx, n_idx, mplanes.as<uint32_t>(), rows.as<uint32_t>(), d_offsets, n,
-                          params_buf.as<MdParams>(), d_plan, d_order, d_claims,
-                          best.as<uint32_t>(), ctrl.as<uint32_t>(), s));
+    return md_exact_launch(d_idx, n_idx, d_offsets, n, d_tmax, tmax_s, thresholded, d_out,
+                           rows.as<uint32_t>(), plan.as<uint8_t>(), best.as<uint32_t>(), s, true);
+  }
+
+  // Exact or thresholded pass (prep, walk, finalize) with caller-given
+  // scratch: rows (3 * n_idx rows), the plan area (kMdPlanAreaBytes + 4 n
+  // bytes: claim counters, MdPlan, order) and best (n words).  Every word the
+  // kernels read is written by md_prep first, so no clearing is needed.
+  int md_exact_launch(const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets, int32_t n,
+                      const int32_t* d_tmax, int32_t tmax_s, bool thresholded, int32_t* d_out,
+                      uint32_t* d_rows, uint8_t* d_plan_area, uint32_t* d_best, hipStream_t s,
+                      bool profile) {
+    uint32_t* d_claims = reinterpret_cast<uint32_t*>(d_plan_area);
+    MdPlan* d_plan = reinterpret_cast<MdPlan*>(d_plan_area + kMdClaimBytes);
+    uint32_t* d_order = reinterpret_cast<uint32_t*>(d_plan_area + kMdPlanAreaBytes);
+    GT_HIP(launch_md_prep(d_idx, n_idx, mplanes.as<uint32_t>(), d_rows, d_offsets, n,
+                          params_buf.as<MdParams>(), d_plan, d_order, d_claims, d_best,
+                          ctrl.as<uint32_t>(), s));
     hipEvent_t e0 = nullptr;
-    GT_OK(prof_begin("md_main", s, &e0));
-    GT_HIP(launch_md_main(thresholded, md_grid[thresholded ? 1 : 0], rows.as<uint32_t>(),
-                          d_offsets, d_plan, d_order, d_claims, best.as<uint32_t>(), d_tmax,
-                          tmax_s, params_buf.as<MdParams>(), s));
-    GT_OK(prof_end("md_main", s, e0));
-    GT_HIP(launch_md_finalize(d_offsets, n, best.as<uint32_t>(), d_tmax, tmax_s, d_out, s));
+    if (profile) GT_OK(prof_begin("md_main", s, &e0));
+    GT_HIP(launch_md_main(thresholded, md_grid[thresholded ? 1 : 0], d_rows, d_offsets, d_plan,
+                          d_order, d_claims, d_best, d_tmax, tmax_s, params_buf.as<MdParams>(), s));
+    if (profile) GT_OK(prof_end("md_main", s, e0));
+    GT_HIP(launch_md_finalize(d_offsets, n, d_best, d_tmax, tmax_s, d_out, s));
     return 0;
   }
 };
@@ -341,10 +356,13 @@ int gt_set_md_params(gt_ctx* c, int32_t k, int32_t L, int32_t check_log2) {
   if (L < kU || L > 30) return fail("L=%d outside %d..30", L, kU);
   if (check_log2 < 0 || check_log2 > 16) return fail("check_log2=%d outside 0..16", check_log2);
   std::lock_guard<std::mutex> lk(c->mu);
+  if (c->bstate.frozen) return fail("launch parameters changed during graph capture");
   c->params.L[k] = (uint8_t)L;
   c->params.check_log2[k] = (uint8_t)check_log2;
   c->params_dirty = true;
-  return 0;
+  // Uploaded now (ordered on the context stream), so no later call has to.
+  GT_OK(c->activate());
+  return c->upload_params(c->stream);
 }
 
 int gt_get_md_params(gt_ctx* c, int32_t k, int32_t* L, int32_t* check_log2) {

@@ -57,8 +57,9 @@ __device__ __forceinline__ void md_rows_one(int64_t t, const int32_t* __restrict
 
 // Exact-walk rows of one code (one wave): lane 3i + b (i < k - 1) writes
 // x^b * (g_i + g_{k-1}) at row 3i + b of the code.  Lanes i < k check their
-// index.  Codes with k < 2 have no walk; k > 21 is flagged by the plan.
-__device__ __forceinline__ void md_rows_code(int code, int32_t n,
+// index.  Codes with k < 2 have no walk; k > 21 and offsets outside
+// [0, n_idx] are flagged by the plan (and no row is written for them).
+__device__ __forceinline__ void md_rows_code(int code, int32_t n, int64_t n_idx,
                                              const int32_t* __restrict__ idx,
                                              const int32_t* __restrict__ offsets,
                                              const uint32_t* __restrict__ mplanes,
@@ -67,7 +68,7 @@ __device__ __forceinline__ void md_rows_code(int code, int32_t n,
   const uint32_t lane = lane_id();
   const int o = offsets[code];
   const int k = offsets[code + 1] - o;
-  if (k < 1 || k > kMaxKExact) return;
+  if (k < 1 || k > kMaxKExact || o < 0 || (int64_t)o + k > n_idx) return;
   const uint32_t i = lane / 3u, bpow = lane - 3u * i;
   int g = 0;
   bool bad = false;
@@ -115,15 +116,18 @@ constexpr int kPlanRounds = 4;
 constexpr int kPlanBatch = kPlanThreads * kPlanRounds;
 
 __device__ __forceinline__ void md_plan_load_batch(const int32_t* __restrict__ offsets, int32_t n,
-                                                   int base, int8_t* jb, uint32_t* best,
-                                                   bool first_pass, uint32_t& err,
+                                                   int64_t n_idx, int base, int8_t* jb,
+                                                   uint32_t* best, bool first_pass, uint32_t& err,
                                                    uint32_t& present) {
   const int tid = threadIdx.x;
   int kk[kPlanRounds];
+  bool inside[kPlanRounds];
 #pragma unroll
   for (int r = 0; r < kPlanRounds; ++r) {
     const int i = base + r * kPlanThreads + tid;
-    kk[r] = (i < n) ? offsets[i + 1] - offsets[i] : 0;
+    const int o0 = (i < n) ? offsets[i] : 0, o1 = (i < n) ? offsets[i + 1] : 0;
+    kk[r] = o1 - o0;
+    inside[r] = o0 >= 0 && (int64_t)o1 <= n_idx;   // rows and indices of the code in range
   }
 #pragma unroll
   for (int r = 0; r < kPlanRounds; ++r) {
@@ -131,7 +135,7 @@ __device__ __forceinline__ void md_plan_load_batch(const int32_t* __restrict__ o
     int j = -1;
     if (i < n) {
       const int k = kk[r];
-      if (k < 0 || k > kMaxKExact) err |= kErrBadK;
+      if (k < 0 || k > kMaxKExact || !inside[r]) err |= kErrBadK;
       else if (k > 1) j = kMaxKExact - walk_dim(k);
       if (first_pass) best[i] = 64u;
     }
@@ -141,7 +145,7 @@ __device__ __forceinline__ void md_plan_load_batch(const int32_t* __restrict__ o
 }
 
 __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets, int32_t n,
-                                             const MdParams* __restrict__ p,
+                                             int64_t n_idx, const MdParams* __restrict__ p,
                                              MdPlan* __restrict__ plan,
                                              uint32_t* __restrict__ order,
                                              uint32_t* __restrict__ claims,
@@ -167,7 +171,7 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
   uint32_t mine = 0;
   for (int base = 0; base < n; base += kPlanBatch) {
     uint32_t pres = 0;
-    md_plan_load_batch(offsets, n, base, jb, best, true, err, pres);
+    md_plan_load_batch(offsets, n, n_idx, base, jb, best, true, err, pres);
     if (pres) atomicOr(&present_s, pres);
     __syncthreads();
     const uint32_t present = __builtin_amdgcn_readfirstlane(present_s);
@@ -217,7 +221,7 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
   for (int base = 0; base < n; base += kPlanBatch) {
     if (n > kPlanBatch) {
       uint32_t unused = 0;
-      md_plan_load_batch(offsets, n, base, jb, best, false, err, unused);
+      md_plan_load_batch(offsets, n, n_idx, base, jb, best, false, err, unused);
       __syncthreads();
     }
     for (int r = 0; r < kPlanRounds; ++r) {
@@ -260,12 +264,11 @@ __global__ __launch_bounds__(kPlanThreads) void md_prep_kernel(
     const MdParams* __restrict__ p, MdPlan* __restrict__ plan, uint32_t* __restrict__ order,
     uint32_t* __restrict__ claims, uint32_t* __restrict__ best, uint32_t* ctrl) {
   if (blockIdx.x == 0) {
-    md_plan_body(offsets, n, p, plan, order, claims, best, ctrl);
+    md_plan_body(offsets, n, n_idx, p, plan, order, claims, best, ctrl);
     return;
   }
-  (void)n_idx;
-  md_rows_code((int)((blockIdx.x - 1) * (kPlanThreads / kWave) + threadIdx.x / kWave), n, idx,
-               offsets, mplanes, rows, ctrl);
+  md_rows_code((int)((blockIdx.x - 1) * (kPlanThreads / kWave) + threadIdx.x / kWave), n, n_idx,
+               idx, offsets, mplanes, rows, ctrl);
 }
 
 // Chunk ids.  The plan numbers chunks bucket by bucket, largest walk first;
