@@ -139,7 +139,41 @@ constexpr size_t kMdClaimBytes = (size_t)kClaimShards * kClaimStride * 4;
 constexpr size_t kMdPlanAreaBytes = kMdClaimBytes + 256;
 static_assert(sizeof(MdPlan) <= 256, "MdPlan fits its slot");
 
+// ---- scratch of the allocation-free exact pass (gt_md_batch_dev_noalloc) --
+// One file-scope device array per GPU, held until the process ends: the plan
+// area, the per-code minima and the bit rows of at most kNaMaxCodes codes
+// with kNaMaxIdx indices in total (6.6 MB).  Host code never passes the
+// array's name to a kernel: it uses the address hipGetSymbolAddress returns
+// for the current device (g_scratch_ptr), taken once per device by gt_open.
+constexpr int32_t kNaMaxCodes = 16384;
+constexpr int64_t kNaMaxIdx = 65536;
+constexpr size_t kNaPlanOff = 0;
+constexpr size_t kNaBestOff = kNaPlanOff + kMdPlanAreaBytes + (size_t)kNaMaxCodes * 4;
+constexpr size_t kNaRowsOff = kNaBestOff + (size_t)kNaMaxCodes * 4;
+constexpr size_t kNaBytes = kNaRowsOff + (size_t)kNaMaxIdx * 3 * kRowWords * 4;
+static_assert(kNaBestOff % 256 == 0 && kNaRowsOff % 256 == 0, "aligned scratch regions");
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[kMaxDevices] = {};
+
 }  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kNaBytes];
+
+// Device address of `scratch` on `device` (current), looked up once per device.
+static int scratch_address(int device, uint8_t** out) {
+  if (device < 0 || device >= kMaxDevices) return fail("device %d beyond the scratch table", device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    GT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail("hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[device] = reinterpret_cast<uint8_t*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return 0;
+}
 
 int gt::set_error(const char* msg) {
   g_err = msg;
@@ -155,6 +189,7 @@ struct gt_ctx {
   int lds_per_block = 0;
   std::string name;
   int md_grid[2] = {0, 0};
+  uint8_t* scratch_dev = nullptr;   // this device's `scratch` array
   MdParams params{};
   bool params_dirty = true;
 
@@ -306,6 +341,7 @@ int gt_open(int32_t device, gt_ctx** out) {
   }
   c->md_grid[0] = c->cu_count * md_main_blocks_per_cu(false);
   c->md_grid[1] = c->cu_count * md_main_blocks_per_cu(true);
+  if (scratch_address(device, &c->scratch_dev)) return bail(-1);
   uint32_t mp[kNpts * kRowWords];
   build_mplanes(mp);
   if (c->mplanes.ensure(sizeof(mp)) || c->params_buf.ensure(sizeof(MdParams)) ||
@@ -428,6 +464,33 @@ int gt_md_batch_dev(gt_ctx* c, const int32_t* d_idx, int64_t n_idx, const int32_
   GT_OK(c->activate());
   return c->md_enqueue(d_idx, n_idx, d_offsets, n, d_tmax, tmax, thresholded != 0, sample_count,
                        sample_seed, d_out, c->pick(stream));
+}
+
+int gt_md_batch_dev_noalloc(gt_ctx* c, const int32_t* d_idx, int64_t n_idx,
+                            const int32_t* d_offsets, int32_t n, const int32_t* d_tmax,
+                            int32_t tmax, int32_t thresholded, int32_t* d_out, void* stream) {
+  if (!c) return fail("null context");
+  if (n < 0 || n_idx < 0) return fail("negative sizes");
+  if (n > kNaMaxCodes || n_idx > kNaMaxIdx)
+    return fail("allocation-free call: %d codes / %lld indices exceed the maximum of %d codes / "
+                "%lld indices", n, (long long)n_idx, kNaMaxCodes, (long long)kNaMaxIdx);
+  if (n == 0) return 0;
+  if (!d_offsets || !d_out) return fail("null device offsets/out");
+  if (n_idx > 0 && !d_idx) return fail("null device idx");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (c->params_dirty) return fail("launch parameters not uploaded");
+  GT_OK(c->activate());
+  uint8_t* base = c->scratch_dev;
+  return c->md_exact_launch(d_idx, n_idx, d_offsets, n, d_tmax, tmax, thresholded != 0, d_out,
+                            reinterpret_cast<uint32_t*>(base + kNaRowsOff), base + kNaPlanOff,
+                            reinterpret_cast<uint32_t*>(base + kNaBestOff), c->pick(stream),
+                            false);
+}
+
+int gt_md_batch_dev_noalloc_limits(int32_t* max_codes, int64_t* max_idx) {
+  if (max_codes) *max_codes = kNaMaxCodes;
+  if (max_idx) *max_idx = kNaMaxIdx;
+  return 0;
 }
 
 int gt_md_single(gt_ctx* c, const int32_t* idx, int32_t k, int32_t tmax, int32_t* out) {

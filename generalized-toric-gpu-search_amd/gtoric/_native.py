@@ -37,6 +37,8 @@ _SIGS = {
     "gt_get_md_params": ([vp, i32, P(i32), P(i32)], ctypes.c_int),
     "gt_md_batch": ([vp, vp, vp, i32, vp, i32, i64, u64, vp], ctypes.c_int),
     "gt_md_batch_dev": ([vp, vp, i64, vp, i32, vp, i32, i32, i64, u64, vp, vp], ctypes.c_int),
+    "gt_md_batch_dev_noalloc": ([vp, vp, i64, vp, i32, vp, i32, i32, vp, vp], ctypes.c_int),
+    "gt_md_batch_dev_noalloc_limits": ([P(i32), P(i64)], ctypes.c_int),
     "gt_md_single": ([vp, vp, i32, i32, P(i32)], ctypes.c_int),
     "gt_canon_batch": ([vp, vp, i32, i32, vp, vp, vp, vp], ctypes.c_int),
     "gt_canon_masks": ([vp, vp, i32, vp, vp, vp, vp], ctypes.c_int),
@@ -190,6 +192,22 @@ class Context:
                                     int(bool(thresholded)), int(sample_count),
                                     ctypes.c_uint64(int(sample_seed) & 0xFFFF_FFFF_FFFF_FFFF),
                                     vp(d_out), vp(stream) if stream else None))
+
+    def md_batch_dev_noalloc(self, d_idx: int, n_idx: int, d_offsets: int, n: int, d_out: int,
+                             tmax: int = 48, d_tmax: int | None = None,
+                             thresholded: bool = False, stream: int | None = None):
+        """gt_md_batch_dev_noalloc: the same pass with the library's static
+        per-GPU scratch (no device allocation in the call)."""
+        check(lib().gt_md_batch_dev_noalloc(self.handle, vp(d_idx), int(n_idx), vp(d_offsets),
+                                            int(n), vp(d_tmax) if d_tmax else None, int(tmax),
+                                            int(bool(thresholded)), vp(d_out),
+                                            vp(stream) if stream else None))
+
+    @staticmethod
+    def md_noalloc_limits() -> tuple[int, int]:
+        c, k = i32(), i64()
+        check(lib().gt_md_batch_dev_noalloc_limits(ctypes.byref(c), ctypes.byref(k)))
+        return c.value, k.value
 
     def md_single(self, S, tmax: int) -> int:
         s = np.ascontiguousarray(S, dtype=np.int32)

@@ -70,6 +70,23 @@ int gt_md_batch_dev(gt_ctx* ctx, const int32_t* d_idx, int64_t n_idx,
                     int32_t tmax, int32_t thresholded, int64_t sample_count,
                     uint64_t sample_seed, int32_t* d_out, void* stream);
 
+/* Allocation-free variant of gt_md_batch_dev (exact or thresholded; no
+ * sampling).  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory during the call (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Its scratch (plan,
+ * per-code minima, bit rows) is one file-scope device array per GPU, held
+ * until the process ends; the call itself calls no hipMalloc/hipFree.
+ * Limits: n <= 16384 codes and n_idx <= 65536 indices (larger calls fail
+ * with an error; gt_md_batch_dev_noalloc_limits reports them).  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other contexts, other threads) race and must be ordered by the
+ * caller.  Device-side input errors are reported by gt_sync, as for the
+ * *_dev calls. */
+int gt_md_batch_dev_noalloc(gt_ctx* ctx, const int32_t* d_idx, int64_t n_idx,
+                            const int32_t* d_offsets, int32_t n, const int32_t* d_tmax_per_set,
+                            int32_t tmax, int32_t thresholded, int32_t* d_out, void* stream);
+int gt_md_batch_dev_noalloc_limits(int32_t* max_codes, int64_t* max_idx);
+
 /* One code, one kernel launch (replaces DistanceOracle.max_zeros,
  * kernel.py:241-274, and DistanceOracleCUDABP.max_zeros,
  * kernel_cuda_bp.py:266-288).  k = 0 gives 0; k > 21 is an error. */
