@@ -194,6 +194,13 @@ struct gt_ctx {
   bool params_dirty = true;
 
   DevBuf mplanes, params_buf, ctrl;
+  // Error slot the kernels of the current call write (ctrl[2] of this base):
+  // word 0 for device-pointer calls, asynchronous searches and graphs
+  // (collected by gt_sync / gt_evolve_sync), word kCtrlHostSlot for
+  // host-pointer calls, which check their own slot before they return, so
+  // an uncollected error of an earlier *_dev call is never reported by an
+  // unrelated synchronous call.
+  uint32_t* ctrl_p = nullptr;
   DevBuf idx, offs, tmax, rows, plan, best, out, zmax, stopf;  // min distance
   DevBuf sbuf;                                                // single code
   DevBuf masks, order, hist, klo, khi, canon, stab;            // canonical
@@ -201,6 +208,10 @@ struct gt_ctx {
 
   bool prof = false;
   std::map<std::string, ProfRec> prof_rec;
+  // Stream of the last call that used the context's buffers, and an event
+  // for handing them over to a call on another stream.
+  hipStream_t last_s = nullptr;
+  hipEvent_t handover = nullptr;
   BufState bstate;
   hipStream_t capture_stream = nullptr;
 
@@ -215,6 +226,32 @@ struct gt_ctx {
     return 0;
   }
   hipStream_t pick(void* s) const { return s ? reinterpret_cast<hipStream_t>(s) : stream; }
+
+  // Calls on one context share its buffers (rows, plan, minima, canonical
+  // scratch, ...), so a call on stream s must not start before the previous
+  // call's work on another stream has finished.  A call on a caller's stream
+  // records the handover event there before it returns (the caller may
+  // destroy that stream afterwards); the context stream's work is recorded
+  // when the next call moves to another stream.  The next call on another
+  // stream waits on the event, and so does every call on a caller's stream
+  // (a destroyed stream's handle may come back for a new one).  Nothing is
+  // recorded while calls stay on the context stream, and nothing during
+  // graph capture (capture runs on the context stream and is ordered when it
+  // begins).
+  int order_stream(hipStream_t s) {
+    if (bstate.frozen) return 0;
+    if (last_s && !(last_s == stream && s == stream)) {
+      if (last_s == stream) GT_HIP(hipEventRecord(handover, stream));
+      GT_HIP(hipStreamWaitEvent(s, handover, 0));
+    }
+    last_s = s;
+    return 0;
+  }
+  int release_stream(hipStream_t s) {
+    if (bstate.frozen || s == stream) return 0;
+    GT_HIP(hipEventRecord(handover, s));
+    return 0;
+  }
 
   int upload_params(hipStream_t s) {
     if (!params_dirty) return 0;
@@ -242,12 +279,12 @@ struct gt_ctx {
   }
 
   int check_ctrl(hipStream_t s) {
-    uint32_t h[kCtrlWords];
-    GT_HIP(hipMemcpyAsync(h, ctrl.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    uint32_t h[kCtrlSlotWords];
+    GT_HIP(hipMemcpyAsync(h, ctrl_p, sizeof(h), hipMemcpyDeviceToHost, s));
     GT_HIP(hipStreamSynchronize(s));
     if (h[2] != 0) {
       const uint32_t zero = 0;
-      GT_HIP(hipMemcpyAsync(ctrl.as<uint32_t>() + 2, &zero, 4, hipMemcpyHostToDevice, s));
+      GT_HIP(hipMemcpyAsync(ctrl_p + 2, &zero, 4, hipMemcpyHostToDevice, s));
       GT_HIP(hipStreamSynchronize(s));
       if (h[2] & kErrBadIndex) return fail("input error: index outside 0..48 or repeated in a set");
       return fail("input error: set size outside the supported range, or too much work for one call");
@@ -258,20 +295,21 @@ struct gt_ctx {
   // ---- min distance on device pointers (stream-ordered, no host sync) ----
   int md_enqueue(const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets, int32_t n,
                  const int32_t* d_tmax, int32_t tmax_s, bool thresholded,
-                 int64_t sample_count, uint64_t seed, int32_t* d_out, hipStream_t s) {
+                 int64_t sample_count, uint64_t seed, int32_t* d_out, hipStream_t s,
+                 uint32_t set_period = 0) {
     if (n <= 0) return 0;
     GT_OK(rows.ensure((size_t)std::max<int64_t>(n_idx, 1) * 3 * kRowWords * 4));
     GT_OK(upload_params(s));
     if (sample_count > 0) {
       GT_HIP(launch_md_rows(d_idx, n_idx, mplanes.as<uint32_t>(), rows.as<uint32_t>(),
-                            ctrl.as<uint32_t>(), s));
+                            ctrl_p, s));
       GT_OK(zmax.ensure((size_t)n * 4));
       GT_OK(stopf.ensure((size_t)n * 4));
       hipEvent_t e0 = nullptr;
       GT_OK(prof_begin("md_sampled", s, &e0));
       GT_HIP(launch_md_sampled(rows.as<uint32_t>(), d_offsets, n, d_tmax, tmax_s, sample_count,
-                               seed, zmax.as<int32_t>(), stopf.as<uint32_t>(), d_out,
-                               ctrl.as<uint32_t>(), s));
+                               seed, set_period, zmax.as<int32_t>(), stopf.as<uint32_t>(), d_out,
+                               ctrl_p, s));
       GT_OK(prof_end("md_sampled", s, e0));
       return 0;
     }
@@ -294,7 +332,7 @@ struct gt_ctx {
     uint32_t* d_order = reinterpret_cast<uint32_t*>(d_plan_area + kMdPlanAreaBytes);
     GT_HIP(launch_md_prep(d_idx, n_idx, mplanes.as<uint32_t>(), d_rows, d_offsets, n,
                           params_buf.as<MdParams>(), d_plan, d_order, d_claims, d_best,
-                          ctrl.as<uint32_t>(), s));
+                          ctrl_p, s));
     hipEvent_t e0 = nullptr;
     if (profile) GT_OK(prof_begin("md_main", s, &e0));
     GT_HIP(launch_md_main(thresholded, md_grid[thresholded ? 1 : 0], d_rows, d_offsets, d_plan,
@@ -303,6 +341,23 @@ struct gt_ctx {
     GT_HIP(launch_md_finalize(d_offsets, n, d_best, d_tmax, tmax_s, d_out, s));
     return 0;
   }
+};
+
+// Work of one call on stream s: ordered after the context's previous work
+// (order_stream) and handed over when the call ends (release_stream).
+struct StreamUse {
+  gt_ctx* c;
+  hipStream_t s;
+  int rc;
+  StreamUse(gt_ctx* ctx, hipStream_t st) : c(ctx), s(st) { rc = c->order_stream(s); }
+  ~StreamUse() { (void)c->release_stream(s); }
+};
+
+// Scope of a host-pointer call: its kernels flag errors in the host slot.
+struct HostCall {
+  gt_ctx* c;
+  explicit HostCall(gt_ctx* ctx) : c(ctx) { c->ctrl_p = c->ctrl.as<uint32_t>() + kCtrlHostSlot; }
+  ~HostCall() { c->ctrl_p = c->ctrl.as<uint32_t>(); }
 };
 
 extern "C" {
@@ -335,6 +390,8 @@ int gt_open(int32_t device, gt_ctx** out) {
   c->name = prop.gcnArchName;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail("hipStreamCreate failed"));
+  if (hipEventCreateWithFlags(&c->handover, hipEventDisableTiming) != hipSuccess)
+    return bail(fail("hipEventCreate failed"));
   for (int k = 0; k <= kMaxKExact; ++k) {
     c->params.L[k] = (uint8_t)default_L(k);
     c->params.check_log2[k] = 2;
@@ -353,6 +410,7 @@ int gt_open(int32_t device, gt_ctx** out) {
       hipMemcpy(c->params_buf.p, &c->params, sizeof(MdParams), hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail("gt_open: upload of constant tables failed"));
   c->params_dirty = false;
+  c->ctrl_p = c->ctrl.as<uint32_t>();
   for (DevBuf* b : c->all_bufs()) b->st = &c->bstate;
   *out = c;
   return 0;
@@ -367,6 +425,7 @@ int gt_close(gt_ctx* c) {
     for (auto& kv : c->prof_rec)
       for (auto& pr : kv.second.ev) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
     for (DevBuf* b : c->all_bufs()) b->release();
+    if (c->handover) (void)hipEventDestroy(c->handover);
     (void)hipStreamDestroy(c->stream);
   }
   delete c;
@@ -435,7 +494,9 @@ int gt_md_batch(gt_ctx* c, const int32_t* idx, const int32_t* offsets, int32_t n
   for (int64_t i = 0; i < n_idx; ++i)
     if (idx[i] < 0 || idx[i] >= kNpts) return fail("index %d outside 0..48", idx[i]);
   std::lock_guard<std::mutex> lk(c->mu);
+  HostCall hc(c);
   GT_OK(c->activate());
+  GT_OK(c->order_stream(c->stream));
   hipStream_t s = c->stream;
   GT_OK(c->idx.ensure((size_t)std::max<int64_t>(n_idx, 1) * 4));
   GT_OK(c->offs.ensure((size_t)(n + 1) * 4));
@@ -462,6 +523,8 @@ int gt_md_batch_dev(gt_ctx* c, const int32_t* d_idx, int64_t n_idx, const int32_
   if (n_idx > 0 && !d_idx) return fail("null device idx");
   std::lock_guard<std::mutex> lk(c->mu);
   GT_OK(c->activate());
+  StreamUse su(c, c->pick(stream));
+  GT_OK(su.rc);
   return c->md_enqueue(d_idx, n_idx, d_offsets, n, d_tmax, tmax, thresholded != 0, sample_count,
                        sample_seed, d_out, c->pick(stream));
 }
@@ -480,6 +543,8 @@ int gt_md_batch_dev_noalloc(gt_ctx* c, const int32_t* d_idx, int64_t n_idx,
   std::lock_guard<std::mutex> lk(c->mu);
   if (c->params_dirty) return fail("launch parameters not uploaded");
   GT_OK(c->activate());
+  StreamUse su(c, c->pick(stream));
+  GT_OK(su.rc);
   uint8_t* base = c->scratch_dev;
   return c->md_exact_launch(d_idx, n_idx, d_offsets, n, d_tmax, tmax, thresholded != 0, d_out,
                             reinterpret_cast<uint32_t*>(base + kNaRowsOff), base + kNaPlanOff,
@@ -508,6 +573,7 @@ int gt_md_single(gt_ctx* c, const int32_t* idx, int32_t k, int32_t tmax, int32_t
   }
   std::lock_guard<std::mutex> lk(c->mu);
   GT_OK(c->activate());
+  GT_OK(c->order_stream(c->stream));
   hipStream_t s = c->stream;
   // Host-built bit rows of the subcode vanishing at (1,1) (rows 3i + b =
   // x^b * (g_i + g_{k-1}), i < k - 1) + the running minimum in one upload.
@@ -556,7 +622,7 @@ static int canon_enqueue(gt_ctx* c, const uint64_t* d_masks, int32_t n, uint64_t
   hipEvent_t e0 = nullptr;
   GT_OK(c->prof_begin("canon", s, &e0));
   GT_HIP(launch_canon(d_masks, n, by_size, c->order.as<uint32_t>(),
-                      c->hist.as<uint32_t>(), d_lo, d_hi, d_canon, d_stab, c->ctrl.as<uint32_t>(), s));
+                      c->hist.as<uint32_t>(), d_lo, d_hi, d_canon, d_stab, c->ctrl_p, s));
   GT_OK(c->prof_end("canon", s, e0));
   return 0;
 }
@@ -564,7 +630,9 @@ static int canon_enqueue(gt_ctx* c, const uint64_t* d_masks, int32_t n, uint64_t
 static int canon_host(gt_ctx* c, const uint64_t* masks, const int32_t* sets, int32_t n, int32_t k,
                       uint64_t* key_lo, uint64_t* key_hi, uint64_t* canon_mask, int32_t* stab) {
   std::lock_guard<std::mutex> lk(c->mu);
+  HostCall hc(c);
   GT_OK(c->activate());
+  GT_OK(c->order_stream(c->stream));
   hipStream_t s = c->stream;
   GT_OK(c->masks.ensure((size_t)n * 8));
   GT_OK(c->klo.ensure((size_t)n * 8));
@@ -575,7 +643,7 @@ static int canon_host(gt_ctx* c, const uint64_t* masks, const int32_t* sets, int
     GT_OK(c->idx.ensure((size_t)n * k * 4));
     GT_HIP(hipMemcpyAsync(c->idx.p, sets, (size_t)n * k * 4, hipMemcpyHostToDevice, s));
     GT_HIP(launch_sets_to_masks(c->idx.as<int32_t>(), n, k, c->masks.as<uint64_t>(),
-                                c->ctrl.as<uint32_t>(), s));
+                                c->ctrl_p, s));
   } else {
     GT_HIP(hipMemcpyAsync(c->masks.p, masks, (size_t)n * 8, hipMemcpyHostToDevice, s));
   }
@@ -616,6 +684,8 @@ int gt_canon_masks_dev(gt_ctx* c, const uint64_t* d_masks, int32_t n, uint64_t* 
   if (!d_key_lo || !d_key_hi || !d_canon_mask || !d_stab) return fail("null device output");
   std::lock_guard<std::mutex> lk(c->mu);
   GT_OK(c->activate());
+  StreamUse su(c, c->pick(stream));
+  GT_OK(su.rc);
   return canon_enqueue(c, d_masks, n, d_key_lo, d_key_hi, d_canon_mask, d_stab, c->pick(stream),
                        false);
 }
@@ -632,6 +702,7 @@ int gt_capture_begin(gt_ctx* c) {
   std::lock_guard<std::mutex> lk(c->mu);
   if (c->bstate.frozen) return fail("capture already in progress");
   GT_OK(c->activate());
+  GT_OK(c->order_stream(c->stream));
   GT_OK(c->upload_params(c->stream));
   GT_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   c->bstate.frozen = true;
@@ -668,6 +739,8 @@ int gt_graph_launch(gt_graph* g, void* stream) {
   if (g->gen != c->bstate.gen)
     return fail("context buffers were reallocated after this graph was captured; recapture");
   GT_OK(c->activate());
+  StreamUse su(c, c->pick(stream));
+  GT_OK(su.rc);
   GT_HIP(hipGraphLaunch(g->exec, c->pick(stream)));
   return 0;
 }
@@ -759,7 +832,7 @@ static int dedup_enqueue(gt_hashset* h, const uint64_t* d_keys, int32_t n, uint3
   if ((uint64_t)n >= (1ull << 32)) return fail("batch too large");
   ++h->call;
   GT_HIP(launch_hs_dedup(d_keys, n, h->table, h->tags, h->cap, h->call, h->count, d_flag,
-                         h->ctx->ctrl.as<uint32_t>(), s));
+                         h->ctx->ctrl_p, s));
   return 0;
 }
 
@@ -769,6 +842,8 @@ int gt_dedup_dev(gt_hashset* h, const uint64_t* d_keys, int32_t n, uint32_t* d_f
   gt_ctx* c = h->ctx;
   std::lock_guard<std::mutex> lk(c->mu);
   GT_OK(c->activate());
+  StreamUse su(c, c->pick(stream));
+  GT_OK(su.rc);
   return dedup_enqueue(h, d_keys, n, d_flag, c->pick(stream));
 }
 
@@ -777,7 +852,9 @@ int gt_dedup(gt_hashset* h, const uint64_t* keys, int32_t n, uint32_t* flag) {
   if (n == 0) return 0;
   gt_ctx* c = h->ctx;
   std::lock_guard<std::mutex> lk(c->mu);
+  HostCall hc(c);
   GT_OK(c->activate());
+  GT_OK(c->order_stream(c->stream));
   hipStream_t s = c->stream;
   GT_OK(c->bmasks.ensure((size_t)n * 8));
   GT_OK(c->bflag.ensure((size_t)n * 4));
@@ -793,20 +870,27 @@ int gt_compact_dev(gt_ctx* c, const uint32_t* d_flag, int32_t n, uint32_t* d_out
   if (!c || n < 0 || !d_count || (n > 0 && (!d_flag || !d_out_idx))) return fail("bad argument");
   std::lock_guard<std::mutex> lk(c->mu);
   GT_OK(c->activate());
+  StreamUse su(c, c->pick(stream));
+  GT_OK(su.rc);
   GT_OK(c->bsum.ensure((size_t)((n + 1023) / 1024 + 1) * 4));
   GT_HIP(launch_compact(d_flag, n, c->bsum.as<uint32_t>(), d_out_idx, d_count, c->pick(stream)));
   return 0;
 }
 
-// One canonical BFS level: children of the n_par parents (canonical masks of
-// size kp), canonical forms, classes not yet in `seen` (first occurrence in
-// input order), their max_zeros at threshold tmax.  Writes at most cap
-// classes; *n_new is the full count (the call fails if it exceeds cap).
-int gt_bfs_level(gt_ctx* c, gt_hashset* seen, const uint64_t* d_par, int32_t n_par, int32_t kp,
-                 int32_t tmax, uint64_t* d_out_masks, int32_t* d_out_mz, int32_t cap,
-                 int32_t* n_new) {
-  if (!c || !seen || !n_new || n_par < 0 || kp < 0 || kp + 1 > kMaxKCanon)
-    return fail("bad argument (parent size must be 0..%d)", kMaxKCanon - 1);
+// One BFS level: children of the n_par parents (masks of size kp), their
+// canonical masks (canonical != 0) or the children themselves (raw BFS), the
+// classes not yet in `seen` (first occurrence in child order), and their
+// max_zeros at threshold tmax, exact or, with sample_count > 0, from the
+// sampled stream (seed sample_seed, set ids restarting every set_period
+// classes when it is nonzero).  Writes at most cap classes; *n_new is the
+// full count (the call fails if it exceeds cap).
+int gt_bfs_level_ex(gt_ctx* c, gt_hashset* seen, const uint64_t* d_par, int32_t n_par, int32_t kp,
+                    int32_t tmax, int32_t canonical, int64_t sample_count, uint64_t sample_seed,
+                    int32_t set_period, uint64_t* d_out_masks, int32_t* d_out_mz, int32_t cap,
+                    int32_t* n_new) {
+  const int kmax = canonical ? kMaxKCanon : (sample_count > 0 ? kMaxK : kMaxKExact);
+  if (!c || !seen || !n_new || n_par < 0 || kp < 0 || kp + 1 > kmax || set_period < 0)
+    return fail("bad argument (parent size must be 0..%d for this mode)", kmax - 1);
   *n_new = 0;
   if (n_par == 0) return 0;
   if (!d_par || cap < 0 || (cap > 0 && (!d_out_masks || !d_out_mz))) return fail("bad argument");
@@ -817,24 +901,31 @@ int gt_bfs_level(gt_ctx* c, gt_hashset* seen, const uint64_t* d_par, int32_t n_p
   // the second half must not be reused by another call on this context.
   uint32_t nu = 0;
   std::lock_guard<std::mutex> lk(c->mu);
+  HostCall hc(c);
   GT_OK(c->activate());
+  GT_OK(c->order_stream(c->stream));
   hipStream_t s = c->stream;
+  const uint64_t* keys = nullptr;
   {
     const size_t n = (size_t)nch;
     GT_OK(c->children.ensure(n * 8));
-    GT_OK(c->klo.ensure(n * 8));
-    GT_OK(c->khi.ensure(n * 8));
-    GT_OK(c->canon.ensure(n * 8));
-    GT_OK(c->stab.ensure(n * 4));
     GT_OK(c->bflag.ensure(n * 4));
     GT_OK(c->bidx.ensure(n * 4));
     GT_OK(c->bcnt.ensure(64));
     GT_OK(c->bsum.ensure((n + 1023) / 1024 * 4 + 4));
-    GT_HIP(launch_bfs_expand(d_par, n_par, kp, c->children.as<uint64_t>(), c->ctrl.as<uint32_t>(), s));
-    GT_OK(canon_enqueue(c, c->children.as<uint64_t>(), (int32_t)n, c->klo.as<uint64_t>(),
-                        c->khi.as<uint64_t>(), c->canon.as<uint64_t>(), c->stab.as<int32_t>(), s,
-                        true));
-    GT_OK(dedup_enqueue(seen, c->canon.as<uint64_t>(), (int32_t)n, c->bflag.as<uint32_t>(), s));
+    GT_HIP(launch_bfs_expand(d_par, n_par, kp, c->children.as<uint64_t>(), c->ctrl_p, s));
+    keys = c->children.as<uint64_t>();
+    if (canonical) {
+      GT_OK(c->klo.ensure(n * 8));
+      GT_OK(c->khi.ensure(n * 8));
+      GT_OK(c->canon.ensure(n * 8));
+      GT_OK(c->stab.ensure(n * 4));
+      GT_OK(canon_enqueue(c, c->children.as<uint64_t>(), (int32_t)n, c->klo.as<uint64_t>(),
+                          c->khi.as<uint64_t>(), c->canon.as<uint64_t>(), c->stab.as<int32_t>(), s,
+                          true));
+      keys = c->canon.as<uint64_t>();
+    }
+    GT_OK(dedup_enqueue(seen, keys, (int32_t)n, c->bflag.as<uint32_t>(), s));
     GT_HIP(launch_compact(c->bflag.as<uint32_t>(), (int32_t)n, c->bsum.as<uint32_t>(),
                           c->bidx.as<uint32_t>(), c->bcnt.as<uint32_t>(), s));
     GT_HIP(hipMemcpyAsync(&nu, c->bcnt.p, 4, hipMemcpyDeviceToHost, s));
@@ -847,14 +938,22 @@ int gt_bfs_level(gt_ctx* c, gt_hashset* seen, const uint64_t* d_par, int32_t n_p
   GT_OK(c->bmasks.ensure((size_t)nu * 8));
   GT_OK(c->brows.ensure((size_t)nu * k * 4));
   GT_OK(c->boffs.ensure((size_t)(nu + 1) * 4));
-  GT_HIP(launch_gather_u64(c->canon.as<uint64_t>(), c->bidx.as<uint32_t>(), c->bcnt.as<uint32_t>(),
-                           d_out_masks, (int32_t)nu, s));
+  GT_HIP(launch_gather_u64(keys, c->bidx.as<uint32_t>(), c->bcnt.as<uint32_t>(), d_out_masks,
+                           (int32_t)nu, s));
   GT_HIP(launch_masks_to_rows(d_out_masks, (int32_t)nu, k, c->brows.as<int32_t>(),
-                              c->boffs.as<int32_t>(), c->ctrl.as<uint32_t>(), s));
+                              c->boffs.as<int32_t>(), c->ctrl_p, s));
   GT_OK(c->md_enqueue(c->brows.as<int32_t>(), (int64_t)nu * k, c->boffs.as<int32_t>(), (int32_t)nu,
-                      nullptr, tmax, tmax < 48, 0, 0, d_out_mz, s));
+                      nullptr, tmax, tmax < 48, sample_count > 0 ? sample_count : 0, sample_seed,
+                      d_out_mz, s, (uint32_t)set_period));
   GT_OK(c->check_ctrl(s));
   return 0;
+}
+
+int gt_bfs_level(gt_ctx* c, gt_hashset* seen, const uint64_t* d_par, int32_t n_par, int32_t kp,
+                 int32_t tmax, uint64_t* d_out_masks, int32_t* d_out_mz, int32_t cap,
+                 int32_t* n_new) {
+  return gt_bfs_level_ex(c, seen, d_par, n_par, kp, tmax, 1, 0, 0, 0, d_out_masks, d_out_mz, cap,
+                         n_new);
 }
 
 // --------------------------------------------------- evolutionary search --
@@ -986,6 +1085,7 @@ int gt_evolve_create(gt_ctx* c, int32_t k, int32_t pop, int32_t n_prop, uint64_t
     return fail("the ranker lives on another device");
   std::lock_guard<std::mutex> lk(c->mu);
   GT_OK(c->activate());
+  GT_OK(c->order_stream(c->stream));
   gt_evolve* e = new gt_evolve();
   e->ctx = c;
   e->ranker = ranker;
@@ -1058,6 +1158,7 @@ int gt_evolve_run(gt_evolve* e, int32_t generations) {
                 (long long)(e->launched + generations), e->max_gen);
   GT_OK(c->activate());
   if (generations == 0) return 0;
+  GT_OK(c->order_stream(c->stream));
   if (!e->exec || e->graph_gen != c->bstate.gen) GT_OK(e->capture());
   GT_OK(c->upload_params(c->stream));
   if (e->rstream) {

@@ -11,8 +11,12 @@ namespace gt {
 int set_error(const char* msg);
 
 // ---- min distance -------------------------------------------------------
-// ctrl layout (uint32): [0] chunk counter, [1] unused, [2] error bits,
-// [3] total chunks of the last plan, [4..15] reserved.
+// ctrl layout (uint32): two slots of kCtrlSlotWords, each [0] chunk counter,
+// [1] unused, [2] error bits, [3] total chunks of the last plan; slot 0 at
+// word 0 (device-pointer calls), slot 1 at word kCtrlHostSlot (host-pointer
+// calls).
+constexpr int kCtrlSlotWords = 8;
+constexpr int kCtrlHostSlot = 8;
 constexpr int kCtrlWords = 16;
 
 hipError_t launch_md_rows(const int32_t* d_idx, int64_t n_idx, const uint32_t* d_mplanes,
@@ -34,10 +38,11 @@ hipError_t launch_md_finalize(const int32_t* d_offsets, int32_t n, const uint32_
 // One code, one launch: grid = code_chunks(k, L) single-wave workgroups.
 hipError_t launch_md_single(bool thresholded, const uint32_t* d_rows, int k, int L,
                             int check_log2, int thr, uint32_t* d_best, hipStream_t s);
+// set_period > 0: set ids restart every set_period codes (0: set id = index).
 hipError_t launch_md_sampled(const uint32_t* d_rows, const int32_t* d_offsets, int32_t n,
                              const int32_t* d_tmax, int32_t tmax, int64_t sample_count,
-                             uint64_t sample_seed, int32_t* d_zmax, uint32_t* d_stop,
-                             int32_t* d_out, uint32_t* d_ctrl, hipStream_t s);
+                             uint64_t sample_seed, uint32_t set_period, int32_t* d_zmax,
+                             uint32_t* d_stop, int32_t* d_out, uint32_t* d_ctrl, hipStream_t s);
 int md_main_blocks_per_cu(bool thresholded);
 
 // ---- canonical form ------------------------------------------------------

@@ -511,15 +511,18 @@ __device__ __forceinline__ uint64_t mod_mersenne(uint64_t x, int m) {
 constexpr int kSampledBlock = 256;
 
 // Sampled screening (kernel.py:136-178): work_id = 1..sample_count, set_id =
-// index of the code in this call.  Several workgroups share one code; the
+// index of the code in this call, or that index modulo set_period when it is
+// nonzero (a caller that hands the reference's oracle batches of set_period
+// sets restarts set ids every batch).  Several workgroups share one code; the
 // result min(max_zeros, tmax + 1) does not depend on their schedule.
 __global__ __launch_bounds__(kSampledBlock) void md_sampled_kernel(
     const uint32_t* __restrict__ rows, const int32_t* __restrict__ offsets, int32_t n,
     const int32_t* __restrict__ tmax_arr, int32_t tmax_s, uint64_t sample_count,
-    uint64_t seed, uint32_t bpc, int32_t* zmax, uint32_t* stopf) {
+    uint64_t seed, uint32_t set_period, uint32_t bpc, int32_t* zmax, uint32_t* stopf) {
   __shared__ uint32_t bp[kMaxK * 8 * kRowWords];  // [i][v] -> v * g_i planes
   __shared__ int32_t red[kSampledBlock / kWave];
   const uint32_t code = blockIdx.x / bpc, part = blockIdx.x % bpc;
+  const uint64_t set_id = set_period ? code % set_period : code;
   const int k = offsets[code + 1] - offsets[code];
   if (k <= 0 || k > kMaxK) return;
   const uint32_t* R = rows + (size_t)offsets[code] * 3 * kRowWords;
@@ -550,7 +553,7 @@ __global__ __launch_bounds__(kSampledBlock) void md_sampled_kernel(
       break;
     RowV v = {{0, 0, 0, 0, 0, 0}};
     if (use_pid) {
-      const uint64_t x = seed ^ ((uint64_t)code * 0x9E3779B97F4A7C15ull) ^ wc;
+      const uint64_t x = seed ^ (set_id * 0x9E3779B97F4A7C15ull) ^ wc;
       uint64_t pid = mod_mersenne(splitmix64(x), m) + 1ull;
       for (int i = 0; i < k; ++i, pid >>= 3) {
         const uint32_t* src = bp + ((i << 3) + (int)(pid & 7ull)) * kRowWords;
@@ -558,10 +561,10 @@ __global__ __launch_bounds__(kSampledBlock) void md_sampled_kernel(
         for (int q = 0; q < 6; ++q) v.w[q] ^= src[q];
       }
     } else {
-      const uint64_t xf = seed ^ ((uint64_t)code * 0xD6E8FEB86659FD93ull) ^ wf;
+      const uint64_t xf = seed ^ (set_id * 0xD6E8FEB86659FD93ull) ^ wf;
       const int forced = (int)(splitmix64(xf) % (uint64_t)k);
       for (int i = 0; i < k; ++i) {
-        const uint64_t x = seed ^ ((uint64_t)code * 0x9E3779B97F4A7C15ull) ^ wc ^
+        const uint64_t x = seed ^ (set_id * 0x9E3779B97F4A7C15ull) ^ wc ^
                            ((uint64_t)(i + 1) * 0x94D049BB133111EBull);
         int c = (int)(splitmix64(x) & 7ull);
         if (i == forced && c == 0) c = 1;
@@ -679,8 +682,8 @@ hipError_t launch_md_single(bool thresholded, const uint32_t* d_rows, int w, int
 
 hipError_t launch_md_sampled(const uint32_t* d_rows, const int32_t* d_offsets, int32_t n,
                              const int32_t* d_tmax, int32_t tmax, int64_t sample_count,
-                             uint64_t sample_seed, int32_t* d_zmax, uint32_t* d_stop,
-                             int32_t* d_out, uint32_t* d_ctrl, hipStream_t s) {
+                             uint64_t sample_seed, uint32_t set_period, int32_t* d_zmax,
+                             uint32_t* d_stop, int32_t* d_out, uint32_t* d_ctrl, hipStream_t s) {
   hipLaunchKernelGGL(md_sampled_init_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, d_zmax,
                      d_stop);
   // Workgroups per code: about 16 samples per thread, at most 64 groups.
@@ -690,7 +693,7 @@ hipError_t launch_md_sampled(const uint32_t* d_rows, const int32_t* d_offsets, i
   while (bpc > 1 && (uint64_t)n * bpc > (1ull << 30)) bpc >>= 1;
   hipLaunchKernelGGL(md_sampled_kernel, dim3((unsigned)((uint64_t)n * bpc)), dim3(kSampledBlock),
                      0, s, d_rows, d_offsets, n, d_tmax, tmax, (uint64_t)sample_count,
-                     sample_seed, (uint32_t)bpc, d_zmax, d_stop);
+                     sample_seed, set_period, (uint32_t)bpc, d_zmax, d_stop);
   hipLaunchKernelGGL(md_sampled_finalize_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
                      d_offsets, n, d_zmax, d_tmax, tmax, d_out, d_ctrl);
   return hipGetLastError();

@@ -12,7 +12,13 @@
  *     void*, NULL = the context's stream); they only enqueue work.  Call
  *     gt_sync() to wait and to collect device-side input errors.
  *   - A context owns one HIP stream and grow-only device buffers.  Calls on one
- *     context are serialised by a mutex; separate contexts run concurrently.
+ *     context are serialised by a mutex, and their device work too: a call on
+ *     another stream than the previous call's waits (an event) for the work
+ *     already enqueued there, since both use the context's buffers.  Separate
+ *     contexts run concurrently.
+ *   - Device-side input errors of *_dev calls, graph replays and searches are
+ *     reported by gt_sync (gt_evolve_sync); a host-pointer call reports only
+ *     its own.
  *   - Min-distance results are max_zeros values.  With tmax = 49 - target_distance,
  *     a result r <= tmax is the exact maximum number of zeros of a nonzero
  *     codeword; on early abort r is exactly tmax + 1 (the reference returns a
@@ -229,6 +235,17 @@ int gt_compact_dev(gt_ctx* ctx, const uint32_t* d_flag, int32_t n, uint32_t* d_o
 int gt_bfs_level(gt_ctx* ctx, gt_hashset* seen, const uint64_t* d_parents, int32_t n_par,
                  int32_t kp, int32_t tmax, uint64_t* d_out_masks, int32_t* d_out_mz,
                  int32_t cap, int32_t* n_new);
+/* The same level with the reference's other modes
+ * (champion_search_canonical.py:386-397, 483-512): canonical = 0 is its raw
+ * sorted-set BFS above k_canonical_max (children deduplicated as they are;
+ * kp + 1 <= 21 exact, <= 49 sampled); sample_count > 0 is its sampled
+ * screening (kernel.py:136-178 stream, seed sample_seed; set ids restart
+ * every set_period classes when set_period > 0, as for the reference's
+ * batches of batch_size sets). */
+int gt_bfs_level_ex(gt_ctx* ctx, gt_hashset* seen, const uint64_t* d_parents, int32_t n_par,
+                    int32_t kp, int32_t tmax, int32_t canonical, int64_t sample_count,
+                    uint64_t sample_seed, int32_t set_period, uint64_t* d_out_masks,
+                    int32_t* d_out_mz, int32_t cap, int32_t* n_new);
 
 /* Evolutionary search with every generation on the device (replaces the
  * generation loop of run_evolution, evolve_search.py:322-539, with exact

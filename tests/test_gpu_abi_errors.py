@@ -53,3 +53,62 @@ def test_context_still_works_after_errors(ctx):
     d = DistanceOracle(0)
     assert d.max_zeros_batch([[0, 1, 7], [0, 1, 2, 3, 4]], target_distance=1) == \
         [d.max_zeros([0, 1, 7], 1), d.max_zeros([0, 1, 2, 3, 4], 1)]
+
+
+def _hip():
+    """The HIP runtime libgtoric links (already loaded in this process)."""
+    return ctypes.CDLL("libamdhip64.so.7")
+
+
+def test_dev_calls_on_two_streams_do_not_race(ctx):
+    """Two *_dev calls on different caller streams share the context's rows,
+    plan and minima; the second waits for the first, so both are right."""
+    from gtoric.distance import flatten
+    hip = _hip()
+    streams = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for st in streams:
+        assert hip.hipStreamCreateWithFlags(ctypes.byref(st), 1) == 0
+    rng = np.random.default_rng(81)
+    a = [sorted(rng.choice(49, 8, replace=False).tolist()) for _ in range(1500)]
+    b = [sorted(rng.choice(49, 7, replace=False).tolist()) for _ in range(1500)]
+    exp_a = ctx.md_batch(*flatten(a), 48)      # also sizes the context buffers
+    exp_b = ctx.md_batch(*flatten(b), 48)
+    bufs = []
+    try:
+        outs = []
+        for sets, st in ((a, streams[0]), (b, streams[1]), (a, streams[1]), (b, streams[0])):
+            flat, offs = flatten(sets)
+            di, do, dout = ctx.to_device(flat), ctx.to_device(offs), ctx.array(len(sets), np.int32)
+            bufs += [di, do, dout]
+            outs.append(dout)
+            ctx.md_batch_dev(di.ptr, len(flat), do.ptr, len(sets), dout.ptr, stream=st.value)
+        for st in streams:
+            ctx.sync(st.value)
+        got = [o.download() for o in outs]
+        assert np.array_equal(got[0], exp_a) and np.array_equal(got[2], exp_a)
+        assert np.array_equal(got[1], exp_b) and np.array_equal(got[3], exp_b)
+    finally:
+        for x in bufs:
+            x.close()
+        for st in streams:
+            hip.hipStreamDestroy(st)
+
+
+def test_device_errors_are_reported_by_their_own_call(ctx):
+    """An input error flagged by a *_dev call is reported by gt_sync, not by
+    an unrelated host-pointer call made before the sync."""
+    from gtoric._native import GtError
+    from gtoric.distance import flatten
+    flat = np.array([0, 1, 60, 2, 3, 4], dtype=np.int32)   # index 60 is invalid
+    offs = np.array([0, 3, 6], dtype=np.int32)
+    di, do, dout = ctx.to_device(flat), ctx.to_device(offs), ctx.array(2, np.int32)
+    try:
+        ctx.md_batch_dev(di.ptr, len(flat), do.ptr, 2, dout.ptr)
+        good = ctx.md_batch(*flatten([[0, 1, 7]]), 48)   # its own inputs are valid
+        assert good.tolist() == [7]
+        with pytest.raises(GtError):
+            ctx.sync()
+        ctx.sync()                                        # reported once
+    finally:
+        for x in (di, do, dout):
+            x.close()

@@ -162,3 +162,23 @@ def test_flatten_array_and_lists_agree():
     mixed = [[0, 1, 2], [3, 4], [5, 6, 7, 8]]
     fm, om = flatten(mixed)
     assert fm.tolist() == [0, 1, 2, 3, 4, 5, 6, 7, 8] and om.tolist() == [0, 3, 5, 9]
+
+
+def test_results_file_tail_repair_and_tolerant_loader(tmp_path):
+    """A truncated last line is cut before appending, and the loader skips
+    lines that do not parse instead of stopping there."""
+    import json
+    from gtoric.bfs import _append, _repair_tail, load_canonical_state
+    p = tmp_path / "r.jsonl"
+    p.write_text(json.dumps({"type": "survivor", "k": 2, "packed": 64}) + "\n" +
+                 json.dumps({"type": "level_complete", "k": 2}) + "\n" + '{"type": "surv')
+    _repair_tail(p)
+    _append(p, {"type": "survivor", "k": 3, "packed": 4160})
+    _append(p, {"type": "level_complete", "k": 3})
+    lines = p.read_text().splitlines()
+    assert len(lines) == 4 and all(json.loads(x) for x in lines)
+    assert load_canonical_state(p) == (3, {4160})
+    q = tmp_path / "q.jsonl"
+    q.write_text('{"type": "level_complete", "k": 2}\n{garbage\n'
+                 '{"type": "survivor", "k": 3, "packed": 8256}\n{"type": "level_complete", "k": 3}\n')
+    assert load_canonical_state(q) == (3, {8256})
