@@ -92,6 +92,7 @@ _SIGS = {
     "gt_memcpy_d2h": ([vp, vp, vp, i64, vp], ctypes.c_int),
     "gt_fill_u32_dev": ([vp, vp, i64, ctypes.c_uint32, vp], ctypes.c_int),
     "gt_copy_dev": ([vp, vp, vp, i64, vp], ctypes.c_int),
+    "gt_issue_probe": ([i32, i32, i32, P(ctypes.c_double), P(ctypes.c_double)], ctypes.c_int),
     "gt_sync": ([vp, vp], ctypes.c_int),
     "gt_prof_enable": ([vp, i32], ctypes.c_int),
     "gt_prof_read": ([vp, ctypes.c_char_p, P(i32), P(ctypes.c_double)], ctypes.c_int),
@@ -358,6 +359,18 @@ class Graph:
 
 
 _contexts: dict[int, Context] = {}
+
+
+PROBE_FORMS = ("v_xor_b32 vgpr,vgpr", "v_xor_b32 sgpr,vgpr", "v_or3_b32", "v_bcnt_u32_b32",
+               "v_min3_u32", "v_mov_b32 sgpr", "walk block (16 codewords, 174 instructions)")
+
+
+def issue_probe(device: int, form: int, trips: int) -> tuple[float, float]:
+    """gt_issue_probe: (kernel ms, wave64 instructions issued)."""
+    ms, n = ctypes.c_double(), ctypes.c_double()
+    check(lib().gt_issue_probe(int(device), int(form), int(trips), ctypes.byref(ms),
+                               ctypes.byref(n)))
+    return ms.value, n.value
 
 
 def context(device: int = 0) -> Context:

@@ -298,6 +298,15 @@ int gt_memcpy_d2h(gt_ctx* ctx, void* h, const void* d, int64_t bytes, void* stre
 int gt_fill_u32_dev(gt_ctx* ctx, uint32_t* d, int64_t n, uint32_t value, void* stream);
 int gt_copy_dev(gt_ctx* ctx, void* dst, const void* src, int64_t bytes, void* stream);
 
+/* Issue-rate probe (no reference counterpart; the roofline's attainable
+ * ceiling): times, on `device`, 8 waves per SIMD issuing a straight-line
+ * stream of one instruction form — 0 v_xor_b32 VGPR-VGPR, 1 v_xor_b32 with an
+ * SGPR operand, 2 v_or3_b32, 3 v_bcnt_u32_b32, 4 v_min3_u32, 5 v_mov_b32 from
+ * an SGPR — or 6 the min-distance walk's own 16-codeword block (174 wave64
+ * instructions), `trips` loop trips per wave.  *ms is the kernel time,
+ * *wave_instr the wave64 instructions of the probed form it issued. */
+int gt_issue_probe(int32_t device, int32_t form, int32_t trips, double* ms, double* wave_instr);
+
 /* Wait for `stream` (NULL = context stream) and report device-side input
  * errors (bad index, bad size) raised by *_dev calls since the last sync. */
 int gt_sync(gt_ctx* ctx, void* stream);

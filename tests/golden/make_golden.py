@@ -1,7 +1,8 @@
 """Write the golden fixtures under tests/golden/ from the CPU oracle.
 
-Run from the repository root:  python tests/golden/make_golden.py
-(needs oracle/liboracle.so; about a minute on 8 cores).  Every expected value
+Run from the repository root:  python tests/golden/make_golden.py [kat md wide canon]
+(needs oracle/liboracle.so; about four minutes on 8 cores, most of it the
+k = 9 and 10 codes of the wide fixture).  Every expected value
 comes from the oracle, whose own pins are in tests/test_oracle.py; inputs come
 from gtoric.workloads so tests and the benchmark use the same sets.
 """
@@ -62,6 +63,23 @@ def md_fixture():
     print(f"md fixtures: {time.time() - t0:.1f}s")
 
 
+def wide_fixture(n9: int = 128, n10: int = 32):
+    """The mixed k = 3..10 batch (4096 codes, default_rng(12)): every code
+    with k <= 8 and the first n9 / n10 codes with k = 9 / 10, exact (about
+    three minutes on 8 cores)."""
+    t0 = time.time()
+    sets = W.mixed_k(4096, 3, 10, seed=12)
+    ks = np.array([len(s) for s in sets])
+    pos = np.concatenate([np.nonzero(ks <= 8)[0], np.nonzero(ks == 9)[0][:n9],
+                          np.nonzero(ks == 10)[0][:n10]])
+    pos.sort()
+    mz = oracle.max_zeros_batch([sets[i] for i in pos], 48)
+    flat, offs = oracle.flatten(sets)
+    np.savez_compressed(OUT / "md_wide_golden.npz", flat=flat, offsets=offs,
+                        pos=pos.astype(np.int64), mz=mz)
+    print(f"wide fixture ({len(pos)} codes): {time.time() - t0:.1f}s")
+
+
 def canon_fixture():
     t0 = time.time()
     masks = W.canon_masks(20_000)
@@ -109,6 +127,12 @@ def kat_fixture():
 
 
 if __name__ == "__main__":
-    kat_fixture()
-    md_fixture()
-    canon_fixture()
+    which = set(sys.argv[1:]) or {"kat", "md", "wide", "canon"}
+    if "kat" in which:
+        kat_fixture()
+    if "md" in which:
+        md_fixture()
+    if "wide" in which:
+        wide_fixture()
+    if "canon" in which:
+        canon_fixture()

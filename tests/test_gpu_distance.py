@@ -75,6 +75,25 @@ def test_thresholded_per_set(dist, md_gold):
     assert got == md_gold["thr_mz"].tolist()
 
 
+def test_mixed_k3_10_batch_against_fixture(dist):
+    """The metric's mixed k = 3..10 batch (4096 codes, one launch): every
+    k <= 8 code and the fixture's k = 9 and 10 codes equal the oracle; the
+    whole batch is constant on AGL2 classes where two codes share one."""
+    g = np.load(GOLD / "md_wide_golden.npz")
+    sets = _rows(g["flat"], g["offsets"])
+    got = np.array(dist.max_zeros_batch(sets, 1))
+    assert got[g["pos"]].tolist() == g["mz"].tolist()
+    ks = np.array([len(s) for s in sets])
+    assert np.count_nonzero(ks[g["pos"]] == 9) >= 64 and np.count_nonzero(ks[g["pos"]] == 10) >= 16
+    # Thresholded with tmax = max_zeros (no early exit: the exact value) and
+    # tmax = max_zeros - 1 (early exit: the clamp tmax + 1, the same number).
+    sub = g["pos"][::7]
+    subsets = [sets[i] for i in sub]
+    mz = got[sub]
+    assert dist.max_zeros_batch(subsets, (49 - mz).astype(np.int32)) == mz.tolist()
+    assert dist.max_zeros_batch(subsets, (50 - mz).astype(np.int32)) == mz.tolist()
+
+
 def test_big_k9_k10(dist, md_gold):
     sets = _rows(md_gold["big_flat"], md_gold["big_offsets"])
     assert dist.max_zeros_batch(sets, 1) == md_gold["big_mz"].tolist()
@@ -86,6 +105,41 @@ def test_sampled_stream(dist, md_gold):
     assert got == md_gold["samp_mz"].tolist()
     got = dist.max_zeros_batch(sets, 29, sample_count=5000, sample_seed=987654321)
     assert got == md_gold["samp_thr_mz"].tolist()
+
+
+SET_MUL = 0x9E3779B97F4A7C15
+WORK_MUL = 0xBF58476D1CE4E5B9
+MASK64 = (1 << 64) - 1
+
+
+def test_sampled_stream_seed_shifts_on_device(dist):
+    """The device stream has the structure the oracle's tests pin: set i of a
+    batch = that set alone with seed s ^ (i * SET_MUL) (k <= 21); n samples =
+    the max over single samples with seed s ^ (w * WORK_MUL) ^ WORK_MUL; more
+    samples never lower the value (both paths); and device = oracle."""
+    rng = np.random.default_rng(91)
+    sets = [sorted(rng.choice(49, k, replace=False).tolist())
+            for k in (3, 5, 8, 12, 21, 4, 7, 9, 2, 6, 11, 16, 19, 13)]
+    for seed in (11, 0xFEDCBA9876543210):
+        batch = dist.max_zeros_batch(sets, 1, sample_count=4, sample_seed=seed)
+        alone = [dist.max_zeros_batch([S], 1, sample_count=4,
+                                      sample_seed=(seed ^ (i * SET_MUL)) & MASK64)[0]
+                 for i, S in enumerate(sets)]
+        assert batch == alone
+        assert batch == oracle.max_zeros_batch(sets, 48, sample_count=4, sample_seed=seed).tolist()
+    for S in sets[:6]:
+        whole = dist.max_zeros_batch([S], 1, sample_count=6, sample_seed=23)[0]
+        singles = [dist.max_zeros_batch([S], 1, sample_count=1,
+                                        sample_seed=(23 ^ ((w * WORK_MUL) & MASK64) ^ WORK_MUL)
+                                        & MASK64)[0] for w in range(1, 7)]
+        assert whole == max(singles)
+    deep = sets + [sorted(rng.choice(49, k, replace=False).tolist()) for k in (22, 30, 49)]
+    prev = None
+    for n in (1, 5, 50, 500, 20000):
+        cur = np.array(dist.max_zeros_batch(deep, 1, sample_count=n, sample_seed=29))
+        if prev is not None:
+            assert np.all(cur >= prev)
+        prev = cur
 
 
 def test_random_mixed_vs_oracle(dist):

@@ -160,6 +160,60 @@ def test_sampled_stream_bounds():
     assert oracle.max_zeros_batch([[9]], 48, sample_count=100, sample_seed=1).tolist() == [0]
 
 
+# The stream's set and work multipliers (kernel.py:147-153).  The checks
+# below hold exactly when each enters as seed ^ id * C (XOR of products modulo
+# 2^64), so a wrong constant, a wrong combining operation or a wrong id source
+# fails them; the map from the hash to a coefficient vector (modulo 8^k - 1,
+# plus one, base-8 digits) and the per-coefficient path for k > 21 are not
+# fixed by them.
+SET_MUL = 0x9E3779B97F4A7C15
+WORK_MUL = 0xBF58476D1CE4E5B9
+MASK64 = (1 << 64) - 1
+
+
+def _sample_sets(seed, ks):
+    rng = np.random.default_rng(seed)
+    return [sorted(rng.choice(49, k, replace=False).tolist()) for k in ks]
+
+
+def test_sampled_set_index_is_a_seed_shift():
+    """Set i of a batch with seed s gets the value of that set alone with seed
+    s ^ (i * SET_MUL) (k <= 21: poly-id sampling)."""
+    sets = _sample_sets(41, (3, 5, 8, 12, 21, 4, 7, 9, 2, 6, 11, 16))
+    for seed, tmax in ((7, 48), (0xDEADBEEF12345678, 48), (99, 30)):
+        batch = oracle.max_zeros_batch(sets, tmax, sample_count=4, sample_seed=seed)
+        alone = [int(oracle.max_zeros_batch([S], tmax, sample_count=4,
+                                            sample_seed=(seed ^ (i * SET_MUL)) & MASK64)[0])
+                 for i, S in enumerate(sets)]
+        assert batch.tolist() == alone
+
+
+def test_sampled_work_ids_are_seed_shifts():
+    """n samples of one set = the max over single samples with seed
+    s ^ (w * WORK_MUL) ^ WORK_MUL, w = 1..n (work ids start at 1)."""
+    for S in _sample_sets(42, (3, 6, 10, 17)):
+        for seed in (5, 0x0123456789ABCDEF, 77):
+            n = 6
+            whole = int(oracle.max_zeros_batch([S], 48, sample_count=n, sample_seed=seed)[0])
+            singles = [int(oracle.max_zeros_batch(
+                [S], 48, sample_count=1,
+                sample_seed=(seed ^ ((w * WORK_MUL) & MASK64) ^ WORK_MUL) & MASK64)[0])
+                for w in range(1, n + 1)]
+            assert whole == max(singles)
+
+
+def test_sampled_prefix_monotone():
+    """More samples never lower the value (the first n work ids are a prefix
+    of the first m > n), on both sampling paths (k <= 21 and k > 21)."""
+    sets = _sample_sets(43, (4, 9, 15, 21, 22, 30, 40))
+    prev = None
+    for n in (1, 3, 10, 40, 200, 1000):
+        cur = oracle.max_zeros_batch(sets, 48, sample_count=n, sample_seed=17)
+        if prev is not None:
+            assert np.all(cur >= prev)
+        prev = cur
+
+
 # -------------------------------------------------------------- canonical --
 
 def test_gl2_and_rel_tables():
