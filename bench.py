@@ -321,6 +321,66 @@ def bench_sampled(ctx, ks=(12, 16), n: int = 2048, samples: int = 200_000, seed:
     return rows
 
 
+def bench_deep(ctx, k: int = 12, pop: int = 2048, generations: int = 50,
+               samples: int = 200_000, seed: int = 7) -> dict:
+    """Deep-k work of the reference's drivers at k = 12 (its bundled target
+    d = 28, evolve_search_kaggle.ipynb): (a) one screened canonical BFS level
+    (champion_search_canonical.py:483-512: children of 2048 canonical
+    (k-1)-classes, their canonical forms, dedup, sampled evaluation with set
+    ids restarting every 50,000 classes); (b) the sampled-score GA on the
+    device followed by the exact kernel at the target on its 16 best sets
+    (evolve_search.py:306-319), and their exact distances (td = 1) beside the
+    sampled ones.  200,000 samples per set is evolve_search.py's default."""
+    from gtoric.distance import flatten
+    from gtoric.bfs import HashSet, bfs_level
+    from gtoric.bounds import best_known_d
+    from gtoric.evolve import random_masks, sampled_search
+    td = best_known_d(49, k)
+    rng = np.random.default_rng(seed)
+    parents = np.unique(ctx.canon_masks(random_masks(rng, 2048, k - 1))[2])
+    seen = HashSet(ctx, 22)
+    lvl_seed = 1 + k * 1_000_003
+    bfs_level(ctx, seen, parents[:64], k - 1, 49 - td, True, samples, lvl_seed, 50_000)   # warm
+    seen.clear()
+    t0 = time.perf_counter()
+    masks, mz = bfs_level(ctx, seen, parents, k - 1, 49 - td, True, samples, lvl_seed, 50_000)
+    dt = time.perf_counter() - t0
+    seen.close()
+    d = 49 - mz
+    screened = {"parents": int(len(parents)), "children": int(len(parents)) * (49 - (k - 1)),
+                "classes": int(len(masks)), "samples_per_class": samples, "seconds": round(dt, 4),
+                "classes_per_s": round(len(masks) / dt, 1),
+                "samples_per_s": round(len(masks) * samples / dt, 1),
+                "candidates_at_target": int(np.count_nonzero(d >= td)),
+                "best_sampled_d": int(d.max()), "target_d": td}
+    log(f"[bench] screened k={k} level: {len(masks):,} classes in {dt:.3f}s "
+        f"({len(masks) * samples / dt / 1e9:.1f} G samples/s), "
+        f"{screened['candidates_at_target']} at sampled d>={td}")
+    t0 = time.perf_counter()
+    r = sampled_search(k, target_d=td, pop=pop, generations=generations, seed=seed,
+                       sample_count=samples, complete_top=16, ctx=ctx)
+    res = r["result"]
+    verified = [d for ok, d, _ in r["checks"] if ok]
+    fin = [S for S, _ in r["finalists"]]
+    exact_d = (49 - ctx.md_batch(*flatten(fin), 48)).tolist() if fin else []
+    ga = {"pop": pop, "generations": res.generations, "samples_per_set": samples,
+          "ga_seconds": round(res.wall_s, 3),
+          "generations_per_s": round(res.generations / res.wall_s, 2),
+          "sampled_evaluations_per_s": round(res.evaluated * samples / res.wall_s, 1),
+          "best_sampled_d": res.best_d,
+          "finalists": len(r["finalists"]), "finalists_sampled_d": [d for _, d in r["finalists"]],
+          "verified_at_target": len(verified),
+          "exact_check": [{"verified": ok, "d": d} for ok, d, _ in r["checks"]],
+          "exact_check_seconds": round(r["check_s"], 3), "target_d": td,
+          "finalists_exact_d": exact_d,
+          "note": "sampled distances are upper bounds; 'verified' sets reach the target in the "
+                  "exact kernel (their d is exact), the others are below it"}
+    log(f"[bench] sampled GA k={k}: {res.generations} generations in {res.wall_s:.2f}s, best "
+        f"sampled d={res.best_d}, {len(verified)}/{len(r['finalists'])} finalists verified at "
+        f"d>={td} by the exact kernel in {r['check_s']:.2f}s")
+    return {"screened_level": screened, "sampled_ga": ga}
+
+
 def bench_bfs(ctx, max_k: int = 10):
     """Canonical BFS (champion_search_canonical.py:386-604 with every level on
     the device): level k-1 classes -> children -> canonical forms -> device
@@ -432,7 +492,7 @@ def main() -> None:
     ap.add_argument("--canon-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline")
     ap.add_argument("--skip", default="",
-                    help="comma list of: wide,plain,canon,sampled,bfs,cpu,serve,evolve,tune")
+                    help="comma list of: wide,plain,canon,sampled,bfs,deep,cpu,serve,evolve,tune")
     ap.add_argument("--serve-requests", type=int, default=20_000)
     ap.add_argument("--evolve-k", default="6,7,8,9,10")
     ap.add_argument("--evolve-pop", type=int, default=2048)
@@ -615,6 +675,9 @@ def main() -> None:
 
     if "bfs" not in skip:
         result["canonical_bfs"] = bench_bfs(ctx)
+
+    if "deep" not in skip:
+        result["deep_k12"] = bench_deep(ctx)
 
     if "serve" not in skip:
         result["serving"] = bench_serving(ctx, args.serve_requests)

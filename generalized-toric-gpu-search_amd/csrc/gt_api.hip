@@ -296,7 +296,7 @@ struct gt_ctx {
   int md_enqueue(const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets, int32_t n,
                  const int32_t* d_tmax, int32_t tmax_s, bool thresholded,
                  int64_t sample_count, uint64_t seed, int32_t* d_out, hipStream_t s,
-                 uint32_t set_period = 0) {
+                 uint32_t set_period = 0, const uint32_t* d_gen = nullptr) {
     if (n <= 0) return 0;
     GT_OK(rows.ensure((size_t)std::max<int64_t>(n_idx, 1) * 3 * kRowWords * 4));
     GT_OK(upload_params(s));
@@ -308,8 +308,8 @@ struct gt_ctx {
       hipEvent_t e0 = nullptr;
       GT_OK(prof_begin("md_sampled", s, &e0));
       GT_HIP(launch_md_sampled(rows.as<uint32_t>(), d_offsets, n, d_tmax, tmax_s, sample_count,
-                               seed, set_period, zmax.as<int32_t>(), stopf.as<uint32_t>(), d_out,
-                               ctrl_p, s));
+                               seed, set_period, d_gen, zmax.as<int32_t>(), stopf.as<uint32_t>(),
+                               d_out, ctrl_p, s));
       GT_OK(prof_end("md_sampled", s, e0));
       return 0;
     }
@@ -969,6 +969,8 @@ struct gt_evolve {
   hipStream_t rstream = nullptr;
   int k = 0, pop = 0, n_prop = 0, max_gen = 0;
   uint64_t seed = 0, scap = 0;
+  int64_t sample_count = 0;   // > 0: sampled scores (seed + (g + 1) * 1,000,003 per generation)
+  bool canonical = true;      // false: sets deduplicated as they are (no canonical forms)
   int64_t launched = 0;
   std::vector<void*> allocs;
   uint64_t *pop_m = nullptr, *props = nullptr, *plo = nullptr, *phi = nullptr, *pcm = nullptr;
@@ -1012,19 +1014,22 @@ struct gt_evolve {
     gt_ctx* c = ctx;
     uint32_t* ctrl = c->ctrl.as<uint32_t>();
     GT_HIP(launch_evo_propose(pop_m, pop_d, pop_age, pop, n_prop, seed, state, props, s));
-    GT_OK(canon_enqueue(c, props, n_prop, plo, phi, pcm, pstab, s, true));
-    GT_HIP(launch_evo_dedup(pcm, n_prop, skeys, scap, keep, count, s));
+    const uint64_t* pkey = canonical ? pcm : props;     // class of each proposal
+    if (canonical) GT_OK(canon_enqueue(c, props, n_prop, plo, phi, pcm, pstab, s, true));
+    GT_HIP(launch_evo_dedup(pkey, n_prop, skeys, scap, keep, count, s));
     if (ranker) {
-      GT_HIP(launch_mlp_features(pcm, n_prop, feat, s));
+      GT_HIP(launch_mlp_features(pkey, n_prop, feat, s));
       GT_HIP(launch_mlp_predict(mb, feat, n_prop, score, s));
     }
     GT_HIP(launch_evo_select(props, keep, count, ranker ? score : nullptr, pop, n_prop, k, seed,
                              state, chosen, s));
-    GT_OK(canon_enqueue(c, chosen, pop, clo, chi, ccm, cstab, s, true));
-    GT_HIP(launch_hs_insert(ccm, pop, skeys, stags, scap, 0, scount, ctrl, s));
+    const uint64_t* ckey = canonical ? ccm : chosen;
+    if (canonical) GT_OK(canon_enqueue(c, chosen, pop, clo, chi, ccm, cstab, s, true));
+    GT_HIP(launch_hs_insert(ckey, pop, skeys, stags, scap, 0, scount, ctrl, s));
     GT_HIP(launch_masks_to_rows(chosen, pop, k, idx, offs, ctrl, s));
-    GT_OK(c->md_enqueue(idx, (int64_t)pop * k, offs, pop, nullptr, 48, false, 0, 0, mz, s));
-    GT_HIP(launch_evo_survive(pop_m, pop_d, pop_age, chosen, ccm, mz, count, pop, false, max_gen,
+    GT_OK(c->md_enqueue(idx, (int64_t)pop * k, offs, pop, nullptr, 48, false, sample_count, seed,
+                        mz, s, 0, sample_count > 0 ? state : nullptr));
+    GT_HIP(launch_evo_survive(pop_m, pop_d, pop_age, chosen, ckey, mz, count, pop, false, max_gen,
                               state, log_cls, log_d, log_best, log_new, s));
     return 0;
   }
@@ -1060,12 +1065,18 @@ struct gt_evolve {
 
 extern "C" {
 
-int gt_evolve_create(gt_ctx* c, int32_t k, int32_t pop, int32_t n_prop, uint64_t seed,
-                     int32_t max_generations, int32_t log2_seen, gt_mlp* ranker,
-                     const uint64_t* init_masks, gt_evolve** out) {
+int gt_evolve_create2(gt_ctx* c, int32_t k, int32_t pop, int32_t n_prop, uint64_t seed,
+                      int32_t max_generations, int32_t log2_seen, gt_mlp* ranker,
+                      const uint64_t* init_masks, int64_t sample_count, int32_t canonical,
+                      gt_evolve** out) {
   if (!c || !out || !init_masks) return fail("null argument");
   *out = nullptr;
-  if (k < 2 || k > kMaxKCanon) return fail("set size must be in 2..%d (got %d)", kMaxKCanon, k);
+  if (sample_count < 0) return fail("negative sample count");
+  // Canonical forms reach k = 16, the exact kernel k = 21; a 2-point swap
+  // needs two points inside and two outside (k <= 47).
+  const int kmax = canonical ? kMaxKCanon : (sample_count > 0 ? kNpts - 2 : kMaxKExact);
+  if (k < 2 || k > kmax)
+    return fail("set size must be in 2..%d for this mode (got %d)", kmax, k);
   if (pop < 1 || pop > kEvoMaxPop) return fail("population must be in 1..%d", kEvoMaxPop);
   if (n_prop < pop || n_prop > kEvoMaxProp)
     return fail("proposal count must be in pop..%d", kEvoMaxProp);
@@ -1098,6 +1109,8 @@ int gt_evolve_create(gt_ctx* c, int32_t k, int32_t pop, int32_t n_prop, uint64_t
   e->n_prop = n_prop;
   e->max_gen = max_generations;
   e->seed = seed;
+  e->sample_count = sample_count;
+  e->canonical = canonical != 0;
   e->scap = 1ull << log2_seen;
   const size_t P = (size_t)pop, N = (size_t)n_prop, slots = (size_t)max_generations + 1;
   auto bail = [&](int r) {
@@ -1132,12 +1145,16 @@ int gt_evolve_create(gt_ctx* c, int32_t k, int32_t pop, int32_t n_prop, uint64_t
       GT_HIP(hipMemcpyAsync(e->props + o, init_masks, std::min(P, N - o) * 8,
                             hipMemcpyHostToDevice, s));
     GT_HIP(launch_hs_clear(e->skeys, e->stags, e->scap, e->scount, s));
-    GT_OK(canon_enqueue(c, e->props, n_prop, e->plo, e->phi, e->pcm, e->pstab, s, true));
-    GT_OK(canon_enqueue(c, e->chosen, pop, e->clo, e->chi, e->ccm, e->cstab, s, true));
-    GT_HIP(launch_hs_insert(e->ccm, pop, e->skeys, e->stags, e->scap, 0, e->scount, ctrl, s));
+    const uint64_t* ckey = e->canonical ? e->ccm : e->chosen;
+    if (e->canonical) {
+      GT_OK(canon_enqueue(c, e->props, n_prop, e->plo, e->phi, e->pcm, e->pstab, s, true));
+      GT_OK(canon_enqueue(c, e->chosen, pop, e->clo, e->chi, e->ccm, e->cstab, s, true));
+    }
+    GT_HIP(launch_hs_insert(ckey, pop, e->skeys, e->stags, e->scap, 0, e->scount, ctrl, s));
     GT_HIP(launch_masks_to_rows(e->chosen, pop, k, e->idx, e->offs, ctrl, s));
-    GT_OK(c->md_enqueue(e->idx, (int64_t)pop * k, e->offs, pop, nullptr, 48, false, 0, 0, e->mz, s));
-    GT_HIP(launch_evo_survive(e->pop_m, e->pop_d, e->pop_age, e->chosen, e->ccm, e->mz, e->count,
+    GT_OK(c->md_enqueue(e->idx, (int64_t)pop * k, e->offs, pop, nullptr, 48, false, sample_count,
+                        seed, e->mz, s));
+    GT_HIP(launch_evo_survive(e->pop_m, e->pop_d, e->pop_age, e->chosen, ckey, e->mz, e->count,
                               pop, true, max_generations, e->state, e->log_cls, e->log_d,
                               e->log_best, e->log_new, s));
     GT_OK(c->check_ctrl(s));
@@ -1146,6 +1163,13 @@ int gt_evolve_create(gt_ctx* c, int32_t k, int32_t pop, int32_t n_prop, uint64_t
   if ((r = init()) != 0) return bail(r);
   *out = e;
   return 0;
+}
+
+int gt_evolve_create(gt_ctx* c, int32_t k, int32_t pop, int32_t n_prop, uint64_t seed,
+                     int32_t max_generations, int32_t log2_seen, gt_mlp* ranker,
+                     const uint64_t* init_masks, gt_evolve** out) {
+  return gt_evolve_create2(c, k, pop, n_prop, seed, max_generations, log2_seen, ranker, init_masks,
+                           0, 1, out);
 }
 
 int gt_evolve_run(gt_evolve* e, int32_t generations) {

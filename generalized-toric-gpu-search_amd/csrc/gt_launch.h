@@ -39,10 +39,13 @@ hipError_t launch_md_finalize(const int32_t* d_offsets, int32_t n, const uint32_
 hipError_t launch_md_single(bool thresholded, const uint32_t* d_rows, int k, int L,
                             int check_log2, int thr, uint32_t* d_best, hipStream_t s);
 // set_period > 0: set ids restart every set_period codes (0: set id = index).
+// d_gen (may be null): a search's completed-generation counter; the seed
+// becomes sample_seed + (d_gen[0] + 1) * 1,000,003.
 hipError_t launch_md_sampled(const uint32_t* d_rows, const int32_t* d_offsets, int32_t n,
                              const int32_t* d_tmax, int32_t tmax, int64_t sample_count,
-                             uint64_t sample_seed, uint32_t set_period, int32_t* d_zmax,
-                             uint32_t* d_stop, int32_t* d_out, uint32_t* d_ctrl, hipStream_t s);
+                             uint64_t sample_seed, uint32_t set_period, const uint32_t* d_gen,
+                             int32_t* d_zmax, uint32_t* d_stop, int32_t* d_out, uint32_t* d_ctrl,
+                             hipStream_t s);
 int md_main_blocks_per_cu(bool thresholded);
 
 // ---- canonical form ------------------------------------------------------

@@ -518,11 +518,16 @@ constexpr int kSampledBlock = 256;
 __global__ __launch_bounds__(kSampledBlock) void md_sampled_kernel(
     const uint32_t* __restrict__ rows, const int32_t* __restrict__ offsets, int32_t n,
     const int32_t* __restrict__ tmax_arr, int32_t tmax_s, uint64_t sample_count,
-    uint64_t seed, uint32_t set_period, uint32_t bpc, int32_t* zmax, uint32_t* stopf) {
+    uint64_t seed, uint32_t set_period, const uint32_t* __restrict__ gen, uint32_t bpc,
+    int32_t* zmax, uint32_t* stopf) {
   __shared__ uint32_t bp[kMaxK * 8 * kRowWords];  // [i][v] -> v * g_i planes
   __shared__ int32_t red[kSampledBlock / kWave];
   const uint32_t code = blockIdx.x / bpc, part = blockIdx.x % bpc;
   const uint64_t set_id = set_period ? code % set_period : code;
+  // A search generation g (gen[0] = g completed) draws with seed
+  // sample_seed + (g + 1) * 1,000,003, as evolve_search.py:399-407 does per
+  // generation.
+  if (gen) seed += ((uint64_t)gen[0] + 1ull) * 1000003ull;
   const int k = offsets[code + 1] - offsets[code];
   if (k <= 0 || k > kMaxK) return;
   const uint32_t* R = rows + (size_t)offsets[code] * 3 * kRowWords;
@@ -682,8 +687,9 @@ hipError_t launch_md_single(bool thresholded, const uint32_t* d_rows, int w, int
 
 hipError_t launch_md_sampled(const uint32_t* d_rows, const int32_t* d_offsets, int32_t n,
                              const int32_t* d_tmax, int32_t tmax, int64_t sample_count,
-                             uint64_t sample_seed, uint32_t set_period, int32_t* d_zmax,
-                             uint32_t* d_stop, int32_t* d_out, uint32_t* d_ctrl, hipStream_t s) {
+                             uint64_t sample_seed, uint32_t set_period, const uint32_t* d_gen,
+                             int32_t* d_zmax, uint32_t* d_stop, int32_t* d_out, uint32_t* d_ctrl,
+                             hipStream_t s) {
   hipLaunchKernelGGL(md_sampled_init_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, d_zmax,
                      d_stop);
   // Workgroups per code: about 16 samples per thread, at most 64 groups.
@@ -693,7 +699,7 @@ hipError_t launch_md_sampled(const uint32_t* d_rows, const int32_t* d_offsets, i
   while (bpc > 1 && (uint64_t)n * bpc > (1ull << 30)) bpc >>= 1;
   hipLaunchKernelGGL(md_sampled_kernel, dim3((unsigned)((uint64_t)n * bpc)), dim3(kSampledBlock),
                      0, s, d_rows, d_offsets, n, d_tmax, tmax, (uint64_t)sample_count,
-                     sample_seed, set_period, (uint32_t)bpc, d_zmax, d_stop);
+                     sample_seed, set_period, d_gen, (uint32_t)bpc, d_zmax, d_stop);
   hipLaunchKernelGGL(md_sampled_finalize_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
                      d_offsets, n, d_zmax, d_tmax, tmax, d_out, d_ctrl);
   return hipGetLastError();

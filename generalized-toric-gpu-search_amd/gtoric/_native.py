@@ -81,6 +81,8 @@ _SIGS = {
     "gt_bfs_level_ex": ([vp, vp, vp, i32, i32, i32, i32, i64, u64, i32, vp, vp, i32, P(i32)],
                         ctypes.c_int),
     "gt_evolve_create": ([vp, i32, i32, i32, u64, i32, i32, vp, vp, P(vp)], ctypes.c_int),
+    "gt_evolve_create2": ([vp, i32, i32, i32, u64, i32, i32, vp, vp, i64, i32, P(vp)],
+                          ctypes.c_int),
     "gt_evolve_run": ([vp, i32], ctypes.c_int),
     "gt_evolve_sync": ([vp, P(i32)], ctypes.c_int),
     "gt_evolve_log": ([vp, i32, i32, vp, vp, vp, vp], ctypes.c_int),

@@ -1,8 +1,12 @@
-"""Fixed-k evolutionary search with exact scores, every generation on the device.
+"""Fixed-k evolutionary search, every generation on the device.
 
-Plays the role of run_evolution (evolve_search.py:322-539) on the exact
-kernel: every distance recorded here is exact (td = 1 evaluation), never a
-sample.  One generation is one replay of a captured HIP graph
+Plays the role of run_evolution (evolve_search.py:322-539).  By default on the
+exact kernel: every distance recorded is exact (td = 1 evaluation), never a
+sample.  With ``sample_count > 0`` (the reference's GA, which requires it,
+evolve_search.py:350-351) sets are scored by the sampled stream and the best
+ones are then certified by the exact kernel at the target distance
+(``complete_check``, evolve_search.py:306-319).  One generation is one replay
+of a captured HIP graph
 (csrc/evolve_kernels.hip between the canonical-form, hash-set and
 min-distance kernels; gt_evolve_* in include/gtoric.h):
 
@@ -90,16 +94,18 @@ class Evolve:
     set and per-generation log in device memory."""
 
     def __init__(self, ctx: _native.Context, k: int, pop: int, n_prop: int, seed: int,
-                 max_generations: int, init_masks: np.ndarray, ranker=None, log2_seen: int = 0):
+                 max_generations: int, init_masks: np.ndarray, ranker=None, log2_seen: int = 0,
+                 sample_count: int = 0, canonical: bool = True):
         self.ctx, self.k, self.pop = ctx, int(k), int(pop)
         init = np.ascontiguousarray(init_masks, dtype=np.uint64)
         if init.shape != (self.pop,):
             raise ValueError("init_masks must hold pop masks")
         h = ctypes.c_void_p()
-        check(lib().gt_evolve_create(ctx.handle, self.k, self.pop, int(n_prop),
-                                     ctypes.c_uint64(int(seed)), int(max_generations),
-                                     int(log2_seen), ranker.handle if ranker is not None else None,
-                                     ptr(init), ctypes.byref(h)))
+        check(lib().gt_evolve_create2(ctx.handle, self.k, self.pop, int(n_prop),
+                                      ctypes.c_uint64(int(seed)), int(max_generations),
+                                      int(log2_seen), ranker.handle if ranker is not None else None,
+                                      ptr(init), int(sample_count), int(bool(canonical)),
+                                      ctypes.byref(h)))
         self.handle = h
 
     def run(self, generations: int) -> None:
@@ -145,14 +151,19 @@ class Evolve:
 def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogate=None,
         proposal_factor: int = 4, wall_limit_s: float | None = None, train_every: int = 5,
         train_epochs: int = 3, train_window: int = 65536, ctx: _native.Context | None = None,
-        verbose: bool = False, burst: int = 8) -> EvolveResult:
+        verbose: bool = False, burst: int = 8, sample_count: int = 0,
+        canonical: bool | None = None) -> EvolveResult:
     """Search for `generations` (or until `wall_limit_s`), launching `burst`
-    generations between host checks; returns the log of every evaluation."""
+    generations between host checks; returns the log of every evaluation.
+    sample_count > 0 scores by the sampled stream (distances are then sampled
+    upper bounds); canonical (default: k <= 16) deduplicates by AGL2 class."""
     ctx = ctx or _native.context(0)
     rng = np.random.default_rng(seed)
     n_prop = pop * (proposal_factor if surrogate is not None else 1)
+    if canonical is None:
+        canonical = k <= 16
     ev = Evolve(ctx, k, pop, n_prop, seed, generations, random_masks(rng, pop, k),
-                ranker=surrogate)
+                ranker=surrogate, sample_count=sample_count, canonical=canonical)
     try:
         t0 = time.perf_counter()
         if surrogate is not None:
@@ -189,3 +200,78 @@ def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogat
     return EvolveResult(k=k, best_d=int(popd[0]), best_set=mask_points(popm[0]), generations=done,
                         evaluated=pop * (done + 1), wall_s=wall, history=history,
                         classes=classes.ravel(), dists=dists.ravel().astype(np.int32))
+
+
+def complete_check(ctx: _native.Context, sets, target_d: int) -> list[tuple[bool, int, int]]:
+    """The exact kernel at target_d for each set (complete_check_candidate,
+    evolve_search.py:306-319): (verified, distance, max_zeros).  A verified
+    distance is exact (no early stop happened); a failing one is only
+    49 - (tmax + 1), an upper bound."""
+    from .distance import flatten
+    if len(sets) == 0:
+        return []
+    mz = ctx.md_batch(*flatten([list(S) for S in sets]), 49 - int(target_d))
+    return [(int(z) <= 49 - int(target_d), N_POINTS - int(z), int(z)) for z in mz]
+
+
+def recheck(ctx: _native.Context, candidate, target_d: int, sample_count: int, rounds: int,
+            seed: int) -> list[int]:
+    """Sampled rechecks of one set (recheck_candidate, evolve_search.py:285-303):
+    round r draws sample_count codewords with seed + r * 10,000,019."""
+    from .distance import flatten
+    flat, offs = flatten([list(candidate)])
+    return [N_POINTS - int(ctx.md_batch(flat, offs, 49 - int(target_d), int(sample_count),
+                                        (int(seed) + r * 10_000_019) & 0xFFFF_FFFF_FFFF_FFFF)[0])
+            for r in range(rounds)]
+
+
+def sampled_search(k: int, target_d: int, pop: int = 2048, generations: int = 50, seed: int = 1,
+                   sample_count: int = 20_000, complete_top: int = 16,
+                   results_file=None, ctx: _native.Context | None = None,
+                   verbose: bool = False) -> dict:
+    """The reference's deep-k GA (evolve_search.py:322-539 with
+    complete_recheck): sampled scores on the device, then the exact kernel at
+    target_d on the complete_top best distinct sets of the final population.
+    Writes run_start / candidate (verified, complete_kernel) / run_complete
+    records when results_file is given; returns a summary."""
+    import json
+    from pathlib import Path
+    from .field import bounding_cube_points
+    ctx = ctx or _native.context(0)
+    res = run(k, pop=pop, generations=generations, seed=seed, ctx=ctx, verbose=verbose,
+              sample_count=sample_count)
+    t0 = time.perf_counter()
+    ev_sets, seen = [], set()
+    order = np.argsort(-res.dists, kind="stable")
+    for i in order.tolist():
+        key = int(res.classes[i])
+        if key in seen:
+            continue
+        seen.add(key)
+        ev_sets.append((mask_points(key), int(res.dists[i])))
+        if len(ev_sets) == complete_top:
+            break
+    checks = complete_check(ctx, [S for S, _ in ev_sets], target_d)
+    t_check = time.perf_counter() - t0
+    lattice = bounding_cube_points()
+    recs = [{"type": "run_start", "backend": "hip", "k": k, "target_d": target_d,
+             "generations": res.generations, "population_size": pop,
+             "sample_count": sample_count, "complete_recheck": True, "seed": seed}]
+    for (S, sd), (ok, d, z) in zip(ev_sets, checks):
+        if ok:
+            recs.append({"type": "candidate", "verified": True, "verification": "complete_kernel",
+                         "name": f"evolve_k{k}_d{d}", "indices": S,
+                         "lattice_points": [lattice[i] for i in S], "k": k,
+                         "best_known_d": target_d, "min_distance": d, "max_zeros": z,
+                         "sampled_min_distance": sd, "sampled_max_zeros": N_POINTS - sd,
+                         "sample_count": sample_count, "complete_recheck": True})
+    best = max(ev_sets, key=lambda t: t[1]) if ev_sets else ([], 0)
+    recs.append({"type": "run_complete", "k": k, "target_d": target_d,
+                 "best_sampled_min_distance": best[1], "best_indices": best[0],
+                 "logged_candidates": sum(1 for ok, _, _ in checks if ok)})
+    if results_file is not None:
+        with Path(results_file).open("a") as f:
+            for r in recs:
+                f.write(json.dumps(r) + "\n")
+    return {"result": res, "finalists": ev_sets, "checks": checks, "check_s": t_check,
+            "records": recs}

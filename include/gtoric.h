@@ -270,6 +270,18 @@ typedef struct gt_evolve gt_evolve;
 int gt_evolve_create(gt_ctx* ctx, int32_t k, int32_t pop, int32_t n_prop, uint64_t seed,
                      int32_t max_generations, int32_t log2_seen, gt_mlp* ranker,
                      const uint64_t* init_masks, gt_evolve** out);
+/* The same search with the reference's sampled scores and beyond the
+ * canonical limit (evolve_search.py:266-282, 350-351, where every GA score is
+ * sampled): sample_count > 0 scores each chosen set by the sampled stream of
+ * kernel.py:136-178 (no early stop; seed `seed` for the initial population,
+ * seed + (g + 1) * 1,000,003 for generation g, set id = index among the
+ * chosen), so recorded distances are sampled upper bounds, not certified;
+ * canonical = 0 deduplicates sets as they are instead of by AGL2 class.
+ * Limits: canonical 2 <= k <= 16; otherwise k <= 21 exact, k <= 47 sampled. */
+int gt_evolve_create2(gt_ctx* ctx, int32_t k, int32_t pop, int32_t n_prop, uint64_t seed,
+                      int32_t max_generations, int32_t log2_seen, gt_mlp* ranker,
+                      const uint64_t* init_masks, int64_t sample_count, int32_t canonical,
+                      gt_evolve** out);
 /* Enqueue `generations` more generations (asynchronous). */
 int gt_evolve_run(gt_evolve* ev, int32_t generations);
 /* Wait for them; device-side errors are reported here. */

@@ -3,8 +3,9 @@
 Restates csrc/evolve_kernels.hip and the hash-set steps between its kernels
 with numpy: the same counter-based draws (seed, generation, index, slot), the
 same tournament, point swaps, first-occurrence dedup, choice and survivor
-order.  Canonical forms and exact distances come from the oracle, so a GPU
-search that matches this model generation by generation is checked end to end.
+order.  Canonical forms and exact (or sampled: the oracle's stream with the
+generation's seed) distances come from the oracle, so a GPU search that
+matches this model generation by generation is checked end to end.
 Test infrastructure only.
 """
 
@@ -89,12 +90,14 @@ def score_desc_key(score: np.ndarray) -> np.ndarray:
 class Model:
     """State of one search; step() is one generation."""
 
-    def __init__(self, k: int, pop: int, n_prop: int, seed: int, init_masks: np.ndarray):
+    def __init__(self, k: int, pop: int, n_prop: int, seed: int, init_masks: np.ndarray,
+                 sample_count: int = 0, canonical: bool = True):
         self.k, self.pop, self.n_prop, self.seed = k, pop, n_prop, seed
+        self.sample_count, self.canonical = sample_count, canonical
         init = np.asarray(init_masks, dtype=np.uint64)
-        cls = canon_masks(init, k)
+        cls = self.classes(init)
         self.seen = set(cls.tolist())
-        mz = self.max_zeros(init)
+        mz = self.max_zeros(init, seed)
         d = (N - mz).astype(np.int64)
         # The population is kept sorted by (distance desc, age asc, position).
         order = np.lexsort((np.arange(pop), -d))
@@ -103,14 +106,21 @@ class Model:
         self.g = 0
         self.log = [(cls, N - mz, int(d.max()), pop)]
 
-    def max_zeros(self, masks: np.ndarray) -> np.ndarray:
+    def classes(self, masks: np.ndarray) -> np.ndarray:
+        return canon_masks(masks, self.k) if self.canonical else np.asarray(masks, dtype=np.uint64)
+
+    def max_zeros(self, masks: np.ndarray, sample_seed: int) -> np.ndarray:
+        """Exact, or the sampled stream with the generation's seed (set id =
+        index among the chosen)."""
         sets = [[p for p in range(N) if (int(m) >> p) & 1] for m in masks]
-        return np.asarray(oracle.max_zeros_batch(sets, 48), dtype=np.int64)
+        return np.asarray(oracle.max_zeros_batch(sets, 48, sample_count=self.sample_count,
+                                                 sample_seed=sample_seed & ((1 << 64) - 1)),
+                          dtype=np.int64)
 
     def step(self, scorer=None) -> None:
         k, pop, n_prop, seed, g = self.k, self.pop, self.n_prop, self.seed, self.g
         props = propose(seed, g, self.pop_m, self.pop_d, self.pop_age, n_prop)
-        pcm = canon_masks(props, k)
+        pcm = self.classes(props)
         _, first = np.unique(pcm, return_index=True)
         is_first = np.zeros(n_prop, dtype=bool)
         is_first[first] = True
@@ -126,9 +136,9 @@ class Model:
             sel = keep[:real]
         chosen = np.concatenate([props[sel],
                                  random_set(seed, g, np.arange(n_prop + real, n_prop + pop), k)])
-        ccm = canon_masks(chosen, k)
+        ccm = self.classes(chosen)
         self.seen.update(ccm.tolist())
-        cd = N - self.max_zeros(chosen)
+        cd = N - self.max_zeros(chosen, seed + (g + 1) * 1_000_003)
         all_m = np.concatenate([self.pop_m, chosen])
         all_d = np.concatenate([self.pop_d, cd])
         all_a = np.concatenate([self.pop_age + 1, np.zeros(pop, dtype=np.int64)])

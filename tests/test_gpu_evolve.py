@@ -48,6 +48,52 @@ def test_device_generations_equal_the_model(k, pop, n_prop, gens):
         ev.close()
 
 
+@pytest.mark.parametrize("k,pop,canonical,samples", [(12, 32, True, 400), (17, 24, False, 300),
+                                                     (9, 16, False, 0)])
+def test_sampled_and_raw_generations_equal_the_model(k, pop, canonical, samples):
+    """Sampled scores (the oracle's stream with seed + (g + 1) * 1,000,003 per
+    generation) and raw-set dedup beyond the canonical limit, generation by
+    generation against the model; k = 9 raw is exact."""
+    import evolve_model as M
+    from gtoric import _native
+    from gtoric.evolve import Evolve, random_masks
+    ctx = _native.context(0)
+    init = random_masks(np.random.default_rng(k), pop, k)
+    ev = Evolve(ctx, k, pop, pop, 555 + k, 3, init, sample_count=samples, canonical=canonical)
+    try:
+        model = M.Model(k, pop, pop, 555 + k, init, sample_count=samples, canonical=canonical)
+        cls, dist, _, _ = ev.log(0, 1)
+        assert np.array_equal(cls[0], model.log[0][0])
+        assert np.array_equal(dist[0], model.log[0][1])
+        _compare(ev, model, 3)
+    finally:
+        ev.close()
+
+
+def test_sampled_search_certifies_with_the_exact_kernel(tmp_path):
+    """k = 12 sampled GA, then the exact kernel at the target on the best
+    sets: a verified set's distance equals the exact batch value and is at
+    most its sampled distance; records follow evolve_search.py's format."""
+    import json
+    from gtoric import _native
+    from gtoric.distance import flatten
+    from gtoric.evolve import sampled_search
+    ctx = _native.context(0)
+    out = tmp_path / "ev.jsonl"
+    r = sampled_search(12, target_d=28, pop=128, generations=4, seed=5, sample_count=2000,
+                       complete_top=6, results_file=out, ctx=ctx)
+    sets = [S for S, _ in r["finalists"]]
+    exact = 49 - ctx.md_batch(*flatten(sets), 48)
+    for (S, sd), (ok, d, z), e in zip(r["finalists"], r["checks"], exact.tolist()):
+        assert e <= sd                       # sampling only sees some codewords
+        assert ok == (e >= 28)
+        if ok:
+            assert d == e and z == 49 - e
+    recs = [json.loads(x) for x in out.read_text().splitlines()]
+    assert recs[0]["type"] == "run_start" and recs[-1]["type"] == "run_complete"
+    assert all(x["verified"] is True for x in recs if x.get("type") == "candidate")
+
+
 def test_ranked_generations_equal_the_model():
     """With a ranker the chosen sets are the top scores of the new
     proposals; the model takes the scores from the same network."""
