@@ -251,7 +251,7 @@ BURNSIDE_CLASSES = {1: 1, 2: 1, 3: 3, 4: 8, 5: 32, 6: 179, 7: 954, 8: 4758, 9: 2
                     10: 84050, 11: 296557, 12: 937022, 13: 2663422, 14: 6843998}
 
 
-def bench_evolve(ks, pop: int, generations: int, seed: int = 1):
+def bench_evolve(ks, pop: int, generations: int, seed: int = 1, seeds=None):
     """Config 5: unranked search for `generations`, then the surrogate-ranked
     search for the same wall-clock; best exact distance per k, and the number
     of distinct AGL2 classes each run evaluated at the top distance reached
@@ -261,14 +261,14 @@ def bench_evolve(ks, pop: int, generations: int, seed: int = 1):
     from gtoric.surrogate import Surrogate
     ctx = _native.context(0)
     rows = {}
-    for k in ks:
+    for k, seed in [(k, s) for k in ks for s in (seeds or [seed])]:
         base = run(k, pop=pop, generations=generations, seed=seed, ctx=ctx)
         sur = Surrogate(0, seed=seed)
         ranked = run(k, pop=pop, generations=generations * 10, seed=seed, surrogate=sur, ctx=ctx,
                      wall_limit_s=base.wall_s)
         sur.close()
         top = max(base.best_d, ranked.best_d)
-        rows[str(k)] = {
+        rows[str(k) if not seeds else f"{k}_seed{seed}"] = {
             "top_d": top,
             "agl2_classes_of_size_k": BURNSIDE_CLASSES.get(k),
             "unranked": {"best_d": base.best_d, "generations": base.generations,
@@ -280,6 +280,9 @@ def bench_evolve(ks, pop: int, generations: int, seed: int = 1):
                          "distinct_classes_evaluated": int(np.unique(base.classes).size),
                          "verified_codes_per_s": round(base.codes_per_s, 1),
                          "wall_s": round(base.wall_s, 2)},
+            "ranked_over_unranked_time_per_generation": round(
+                (ranked.wall_s / max(ranked.generations, 1)) / (base.wall_s / max(base.generations, 1)),
+                3),
             "ranked": {"best_d": ranked.best_d, "generations": ranked.generations,
                        "time_to_best_s": round(ranked.time_to_best_s, 3),
                        "classes_at_top_d": ranked.hits(top),
@@ -696,7 +699,8 @@ def main() -> None:
         hard = [int(x) for x in args.evolve_hard_k.split(",") if x]
         if hard:
             result["evolve_unexhausted"] = bench_evolve(hard, args.evolve_pop,
-                                                        args.evolve_hard_generations)
+                                                        args.evolve_hard_generations,
+                                                        seeds=[1, 2, 3])
 
     if "cpu" not in skip:
         # Every CPU this process may run on, within the host's thread budget

@@ -33,25 +33,6 @@ __device__ __forceinline__ uint64_t random_set(uint64_t seed, uint32_t g, uint32
   return m;
 }
 
-// Ascending bitonic sort of n (a power of two) keys in LDS by one workgroup.
-__device__ void bitonic_sort(uint64_t* a, int n) {
-  for (int size = 2; size <= n; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int t = threadIdx.x; t < (n >> 1); t += blockDim.x) {
-        const int lo = 2 * t - (t & (stride - 1));
-        const int hi = lo + stride;
-        const bool up = (lo & size) == 0;
-        const uint64_t x = a[lo], y = a[hi];
-        if ((x > y) == up) {
-          a[lo] = y;
-          a[hi] = x;
-        }
-      }
-      __syncthreads();
-    }
-  }
-}
-
 // Order-reversing map of fp32 scores to uint32: larger score -> smaller key.
 __device__ __forceinline__ uint32_t score_desc_key(float f) {
   const uint32_t u = __float_as_uint(f);
@@ -100,25 +81,101 @@ __global__ __launch_bounds__(256) void evo_propose_kernel(
   props[i] = child;
 }
 
+// Exclusive scan of v over a workgroup of 1024 threads (lanes, then waves);
+// wsum: 17 words of LDS.  Returns the exclusive prefix; *total gets the sum.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
+  const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+  uint32_t inc = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    const int nw = (int)(blockDim.x >> 6);
+    for (int w = 0; w < nw; ++w) {
+      const uint32_t u = wsum[w];
+      wsum[w] = run;
+      run += u;
+    }
+    wsum[16] = run;
+  }
+  __syncthreads();
+  const uint32_t r = wsum[wave] + inc - v;
+  *total = wsum[16];
+  __syncthreads();   // wsum is reused by the next scan
+  return r;
+}
+
+// With scores and more new proposals than pop: the pop smallest keys
+// (descending score, then proposal order) are found by a radix select over
+// the 32-bit score keys (three passes of 11, 11 and 10 bits with an LDS
+// histogram give the pop-th smallest key T and how many keys equal to T to
+// take), and the chosen sets are written in proposal order.
 __global__ __launch_bounds__(1024) void evo_select_kernel(
     const uint64_t* __restrict__ props, const uint32_t* __restrict__ keep,
     const uint32_t* __restrict__ count, const float* __restrict__ score, int32_t pop,
     int32_t n_prop, int32_t k, uint64_t seed, const uint32_t* __restrict__ state,
     uint64_t* __restrict__ chosen) {
-  extern __shared__ uint64_t keys[];
+  extern __shared__ uint32_t key[];        // cnt score keys
+  __shared__ uint32_t hist[2048];
+  __shared__ uint32_t wsum[17];
+  __shared__ uint32_t sel_bin, sel_below;
   const uint32_t g = state[0];
   const int cnt = min((int)*count, n_prop);
   const int real = min(cnt, pop);
   if (score != nullptr && cnt > pop) {
-    // Key = (descending score, proposal order); the pop smallest keys win.
-    const int n2 = pow2_at_least(cnt);
-    for (int j = threadIdx.x; j < n2; j += blockDim.x)
-      keys[j] = (j < cnt) ? (((uint64_t)score_desc_key(score[keep[j]]) << 32) | (uint32_t)j)
-                          : ~0ull;
-    __syncthreads();
-    bitonic_sort(keys, n2);
-    for (int j = threadIdx.x; j < pop; j += blockDim.x)
-      chosen[j] = props[keep[(uint32_t)keys[j]]];
+    for (int j = threadIdx.x; j < cnt; j += blockDim.x) key[j] = score_desc_key(score[keep[j]]);
+    uint32_t prefix = 0, pmask = 0, need = (uint32_t)pop;
+    const int shifts[3] = {21, 10, 0}, widths[3] = {11, 11, 10};
+    for (int pass = 0; pass < 3; ++pass) {
+      const int sh = shifts[pass];
+      const uint32_t bm = (1u << widths[pass]) - 1u;
+      for (int b = threadIdx.x; b < 2048; b += blockDim.x) hist[b] = 0u;
+      __syncthreads();
+      for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
+        const uint32_t x = key[j];
+        if ((x & pmask) == prefix) atomicAdd(&hist[(x >> sh) & bm], 1u);
+      }
+      __syncthreads();
+      const uint32_t h0 = hist[2 * threadIdx.x], h1 = hist[2 * threadIdx.x + 1];
+      uint32_t tot;
+      const uint32_t ex = block_excl_scan(h0 + h1, wsum, &tot);
+      if (ex < need && need <= ex + h0) {
+        sel_bin = 2u * threadIdx.x;
+        sel_below = ex;
+      } else if (ex + h0 < need && need <= ex + h0 + h1) {
+        sel_bin = 2u * threadIdx.x + 1u;
+        sel_below = ex + h0;
+      }
+      __syncthreads();
+      need -= sel_below;
+      prefix |= sel_bin << sh;
+      pmask |= bm << sh;
+      __syncthreads();   // sel_* are rewritten by the next pass
+    }
+    // Keys < prefix are taken, and the first `need` keys equal to it.
+    const int per = (cnt + (int)blockDim.x - 1) / (int)blockDim.x;   // <= 8
+    const int i0 = (int)threadIdx.x * per;
+    uint32_t eq = 0;
+    for (int r = 0; r < per && i0 + r < cnt; ++r) eq += (key[i0 + r] == prefix) ? 1u : 0u;
+    uint32_t tot;
+    uint32_t eq_before = block_excl_scan(eq, wsum, &tot);
+    uint32_t bits = 0, ns = 0;
+    for (int r = 0; r < per && i0 + r < cnt; ++r) {
+      const uint32_t x = key[i0 + r];
+      bool take = x < prefix;
+      if (x == prefix) take = (eq_before++ < need);
+      if (take) {
+        bits |= 1u << r;
+        ++ns;
+      }
+    }
+    uint32_t pos = block_excl_scan(ns, wsum, &tot);
+    for (int r = 0; r < per; ++r)
+      if ((bits >> r) & 1u) chosen[pos++] = props[keep[i0 + r]];
   } else {
     for (int j = threadIdx.x; j < real; j += blockDim.x) chosen[j] = props[keep[j]];
   }
@@ -363,7 +420,7 @@ hipError_t launch_evo_select(const uint64_t* props, const uint32_t* keep, const 
                              hipStream_t s) {
   if (pop <= 0 || pop > kEvoMaxPop || n_prop <= 0 || n_prop > kEvoMaxProp)
     return hipErrorInvalidValue;
-  const size_t lds = score ? (size_t)pow2_at_least(n_prop) * 8 : 0;
+  const size_t lds = score ? (size_t)n_prop * 4 : 0;
   hipLaunchKernelGGL(evo_select_kernel, dim3(1), dim3(1024), lds, s, props, keep, count, score,
                      pop, n_prop, k, seed, state, chosen);
   return hipGetLastError();

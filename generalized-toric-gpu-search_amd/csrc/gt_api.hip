@@ -1198,6 +1198,22 @@ int gt_evolve_run(gt_evolve* e, int32_t generations) {
   return 0;
 }
 
+int gt_evolve_train_ranker(gt_evolve* e, int32_t slot0, int32_t slot1, int32_t batch, float lr,
+                           int32_t epochs) {
+  if (!e) return fail("null search");
+  if (!e->ranker) return fail("the search has no ranker");
+  gt_ctx* c = e->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (slot0 < 0 || slot1 <= slot0 || slot1 > e->launched + 1)
+    return fail("log slots [%d, %d) outside [0, %lld]", slot0, slot1, (long long)e->launched + 1);
+  if (batch < 1 || epochs < 1) return fail("batch and epochs must be positive");
+  GT_OK(c->activate());
+  GT_OK(c->order_stream(c->stream));
+  const size_t P = (size_t)e->pop;
+  return mlp_train_device(e->ranker, e->log_cls + slot0 * P, e->log_d + slot0 * P,
+                          (int32_t)((slot1 - slot0) * P), batch, lr, epochs, c->stream);
+}
+
 int gt_evolve_sync(gt_evolve* e, int32_t* generations_done) {
   if (!e) return fail("null search");
   gt_ctx* c = e->ctx;

@@ -37,7 +37,8 @@ hipError_t launch_evo_propose(const uint64_t* pop_m, const int32_t* pop_d, const
                               uint64_t* props, hipStream_t s);
 // Chosen sets: props[keep[j]] for the first min(*count, pop) new proposals in
 // proposal order or, with scores and more than pop new proposals, the pop
-// best-scored ones (ties: proposal order); random k-sets fill the rest.
+// best-scored ones (ties: proposal order), written in proposal order; random
+// k-sets fill the rest.
 hipError_t launch_evo_select(const uint64_t* props, const uint32_t* keep, const uint32_t* count,
                              const float* score, int32_t pop, int32_t n_prop, int32_t k,
                              uint64_t seed, const uint32_t* state, uint64_t* chosen,

@@ -18,6 +18,10 @@ constexpr int kOffW3 = kOffB2 + kMlpHid;
 constexpr int kOffB3 = kOffW3 + kMlpHid;
 constexpr int kMlpParams = kOffB3 + 1;
 constexpr int kMlpPartialStride = (kMlpParams + 1 + 3) & ~3;   // params + loss slot
+// A training step's gradient comes from at most kMlpMaxParts workgroups, each
+// summing a run of consecutive tiles (so the reduction reads at most this many
+// partials).
+constexpr int kMlpMaxParts = 64;
 
 struct MlpBufs {
   float* theta = nullptr;   // kMlpParams
@@ -32,6 +36,8 @@ struct MlpBufs {
 hipError_t launch_mlp_features(const uint64_t* d_masks, int32_t n, uint16_t* d_feat, hipStream_t s);
 hipError_t launch_mlp_predict(const MlpBufs& b, const uint16_t* d_feat, int32_t n, float* d_out,
                               hipStream_t s);
+// Partials (gradient workgroups) of a step on n rows.
+int mlp_grad_parts(int32_t n);
 hipError_t launch_mlp_grads(const MlpBufs& b, const uint16_t* d_feat, const float* d_target,
                             int32_t n, float* d_partial, hipStream_t s);
 // Gradient of one batch and the Adam step (lr, step >= 1) in two kernels;
@@ -41,6 +47,9 @@ hipError_t launch_mlp_train_step(const MlpBufs& b, const uint16_t* d_feat, const
                                  int32_t n, float* d_partial, float lr, int step, float* d_loss,
                                  hipStream_t s);
 hipError_t launch_mlp_cast(const MlpBufs& b, hipStream_t s);
+// Targets (d - mean) / std of n integer distances (one workgroup).
+hipError_t launch_mlp_standardize(const int32_t* d_dist, int32_t n, float* d_target,
+                                  hipStream_t s);
 hipError_t launch_bf16_gemm(const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K,
                             hipStream_t s);
 
@@ -52,5 +61,10 @@ namespace gt {
 const MlpBufs* mlp_buffers(gt_mlp* m);
 hipStream_t mlp_stream(gt_mlp* m);
 int mlp_device(const gt_mlp* m);
+// Train on n device-resident (mask, distance) pairs, enqueued on stream s
+// with no host synchronisation: features once, standardised targets, then
+// `epochs` passes of Adam steps over consecutive batches in order.
+int mlp_train_device(gt_mlp* m, const uint64_t* d_masks, const int32_t* d_dist, int32_t n,
+                     int32_t batch, float lr, int32_t epochs, hipStream_t s);
 
 }  // namespace gt

@@ -72,18 +72,21 @@ struct gt_mlp {
   }
   int grads(const uint64_t* d_masks, const float* d_target, int32_t n, hipStream_t s) {
     ML_OK(features(d_masks, n, s));
-    const size_t tiles = (size_t)(n + kMlpTile - 1) / kMlpTile;
-    ML_OK(partial.ensure(tiles * kMlpPartialStride * 4));
+    ML_OK(partial.ensure((size_t)mlp_grad_parts(n) * kMlpPartialStride * 4));
     ML_HIP(launch_mlp_grads(b, feat.as<uint16_t>(), d_target, n, partial.as<float>(), s));
     return 0;
   }
   int train_step(const uint64_t* d_masks, const float* d_target, int32_t n, float lr,
                  float* d_loss, hipStream_t s) {
     ML_OK(features(d_masks, n, s));
-    const size_t tiles = (size_t)(n + kMlpTile - 1) / kMlpTile;
-    ML_OK(partial.ensure(tiles * kMlpPartialStride * 4));
-    ML_HIP(launch_mlp_train_step(b, feat.as<uint16_t>(), d_target, n, partial.as<float>(), lr,
-                                 ++step, d_loss, s));
+    return train_step_feat(feat.as<uint16_t>(), d_target, n, lr, d_loss, s);
+  }
+  // One Adam step on rows whose features are already in d_feat.
+  int train_step_feat(const uint16_t* d_feat, const float* d_target, int32_t n, float lr,
+                      float* d_loss, hipStream_t s) {
+    ML_OK(partial.ensure((size_t)mlp_grad_parts(n) * kMlpPartialStride * 4));
+    ML_HIP(launch_mlp_train_step(b, d_feat, d_target, n, partial.as<float>(), lr, ++step, d_loss,
+                                 s));
     return 0;
   }
   int upload(const uint64_t* masks_h, const float* target_h, int32_t n) {
@@ -98,6 +101,23 @@ struct gt_mlp {
 };
 
 namespace gt {
+int mlp_train_device(gt_mlp* m, const uint64_t* d_masks, const int32_t* d_dist, int32_t n,
+                     int32_t batch, float lr, int32_t epochs, hipStream_t s) {
+  if (!m || n <= 0 || batch <= 0 || epochs < 0) return mfail("bad training arguments");
+  std::lock_guard<std::mutex> lk(m->mu);
+  ML_HIP(hipSetDevice(m->device));
+  ML_OK(m->features(d_masks, n, s));
+  ML_OK(m->target.ensure((size_t)n * 4));
+  ML_HIP(launch_mlp_standardize(d_dist, n, m->target.as<float>(), s));
+  for (int e = 0; e < epochs; ++e)
+    for (int r0 = 0; r0 < n; r0 += batch) {
+      const int rows = std::min(batch, n - r0);
+      ML_OK(m->train_step_feat(m->feat.as<uint16_t>() + (size_t)r0 * kMlpIn,
+                               m->target.as<float>() + r0, rows, lr, nullptr, s));
+    }
+  return 0;
+}
+
 const MlpBufs* mlp_buffers(gt_mlp* m) { return &m->b; }
 hipStream_t mlp_stream(gt_mlp* m) { return m->stream; }
 int mlp_device(const gt_mlp* m) { return m->device; }
@@ -249,11 +269,14 @@ int gt_mlp_train_batches(gt_mlp* m, const uint64_t* masks, const float* target, 
   ML_OK(m->upload(masks, target, n));
   const int nb = (n + batch - 1) / batch;
   ML_OK(m->losses.ensure((size_t)nb * 4));
-  // Steps are queued back to back; each writes its loss to its own slot.
+  // Features of all rows in one launch, then the steps back to back; each
+  // writes its loss to its own slot.
+  ML_OK(m->features(m->masks.as<uint64_t>(), n, m->stream));
   for (int b = 0; b < nb; ++b) {
     const int r0 = b * batch, rows = std::min(batch, n - r0);
-    ML_OK(m->train_step(m->masks.as<uint64_t>() + r0, m->target.as<float>() + r0, rows, lr,
-                        m->losses.as<float>() + b, m->stream));
+    ML_OK(m->train_step_feat(m->feat.as<uint16_t>() + (size_t)r0 * kMlpIn,
+                             m->target.as<float>() + r0, rows, lr, m->losses.as<float>() + b,
+                             m->stream));
   }
   if (losses)
     ML_HIP(hipMemcpyAsync(losses, m->losses.p, (size_t)nb * 4, hipMemcpyDeviceToHost, m->stream));

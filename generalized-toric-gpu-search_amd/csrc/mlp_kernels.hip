@@ -7,11 +7,13 @@
 // back-propagated errors are rounded to bf16 where they become MFMA operands;
 // the CPU reference (oracle/mlp_ref.py) applies the same roundings.
 //
-// Training step on n rows: one workgroup per 32-row tile runs the forward
-// pass, the loss derivative and the backward pass with all activations in LDS
-// and writes its gradient contribution to its own row of a partial buffer;
-// a second kernel sums the partials in a fixed order (deterministic), and a
-// third applies Adam and refreshes the bf16 copies.
+// Training step on n rows: each workgroup walks a run of consecutive 32-row
+// tiles, with the forward pass, the loss derivative and the backward pass of
+// one tile at a time in LDS, and accumulates the tiles' weight gradients in
+// registers (the dW1 and dW2 MFMA accumulators of its waves persist across
+// tiles); it writes one gradient partial.  A second kernel sums the partials
+// in a fixed order (deterministic), applies Adam and refreshes the bf16
+// copies.
 #include "gt_mlp.h"
 
 namespace gt {
@@ -130,25 +132,35 @@ struct MlpView {
   const uint16_t* w3b;     // [128] bf16
 };
 
-// Forward pass of one tile into LDS: X (loaded), H1, H2 (bf16), y (fp32).
+// Row stride (bf16 elements) of the 128-wide matrices held in LDS.  Rows 272
+// bytes apart shift by 4 banks, so a 16-lane group reading 16 rows (the MFMA
+// fragment loads, 8 bytes per lane) or 4 rows of 16 columns (the transposed
+// reads of the weight-gradient products) hits distinct banks; with 256-byte
+// rows every such group read one bank.
+constexpr int kLd = 136;
+
+// Forward pass of one tile into LDS: X (loaded), H1, H2 (bf16, row stride
+// kLd), y (fp32).  W.w1b / W.w2b have row stride kWs (128 in global memory,
+// kLd when staged in LDS).
+template <int kWs>
 __device__ __forceinline__ void tile_forward(const MlpView& W, const uint16_t* __restrict__ feat,
                                              int row0, int n, uint16_t* X, uint16_t* H1,
                                              uint16_t* H2, float* y) {
-  // Four features per 8-byte load (rows are 256 bytes, X is 16-byte aligned).
+  // Four features per 8-byte load (global rows are 256 bytes).
   for (int e = threadIdx.x; e < kMlpTile * kMlpIn / 4; e += blockDim.x) {
     const int m = e / (kMlpIn / 4), i4 = e % (kMlpIn / 4);
     uint2 v = make_uint2(0u, 0u);
     if (row0 + m < n) v = *reinterpret_cast<const uint2*>(feat + (size_t)(row0 + m) * kMlpIn + 4 * i4);
-    *reinterpret_cast<uint2*>(X + 4 * e) = v;
+    *reinterpret_cast<uint2*>(X + m * kLd + 4 * i4) = v;
   }
   __syncthreads();
   const float* b1 = W.theta + kOffB1;
   const float* b2 = W.theta + kOffB2;
-  wg_gemm<true, true, kMlpIn>(X, kMlpIn, 1, W.w1b, 1, kMlpIn, kMlpTile, kMlpHid, kMlpIn,
-          [&](int m, int h, float acc) { H1[m * kMlpHid + h] = f2bf(fmaxf(acc + b1[h], 0.f)); });
+  wg_gemm<true, true, kMlpIn>(X, kLd, 1, W.w1b, 1, kWs, kMlpTile, kMlpHid, kMlpIn,
+          [&](int m, int h, float acc) { H1[m * kLd + h] = f2bf(fmaxf(acc + b1[h], 0.f)); });
   __syncthreads();
-  wg_gemm<true, true, kMlpHid>(H1, kMlpHid, 1, W.w2b, 1, kMlpHid, kMlpTile, kMlpHid, kMlpHid,
-          [&](int m, int h, float acc) { H2[m * kMlpHid + h] = f2bf(fmaxf(acc + b2[h], 0.f)); });
+  wg_gemm<true, true, kMlpHid>(H1, kLd, 1, W.w2b, 1, kWs, kMlpTile, kMlpHid, kMlpHid,
+          [&](int m, int h, float acc) { H2[m * kLd + h] = f2bf(fmaxf(acc + b2[h], 0.f)); });
   __syncthreads();
   {
     // y[m] = b3 + sum_h w3[h] H2[m][h]: eight lanes per row, 16 terms each,
@@ -158,7 +170,7 @@ __device__ __forceinline__ void tile_forward(const MlpView& W, const uint16_t* _
     float s = 0.f;
 #pragma unroll
     for (int h = part * 16; h < part * 16 + 16; ++h)
-      s += bf2f(W.w3b[h]) * bf2f(H2[m * kMlpHid + h]);
+      s += bf2f(W.w3b[h]) * bf2f(H2[m * kLd + h]);
     s += __shfl_xor(s, 1, 8);
     s += __shfl_xor(s, 2, 8);
     s += __shfl_xor(s, 4, 8);
@@ -170,89 +182,155 @@ __device__ __forceinline__ void tile_forward(const MlpView& W, const uint16_t* _
 __global__ __launch_bounds__(256) void mlp_predict_kernel(MlpView W,
                                                           const uint16_t* __restrict__ feat,
                                                           int32_t n, float* __restrict__ out) {
-  __shared__ __attribute__((aligned(16))) uint16_t X[kMlpTile * kMlpIn], H1[kMlpTile * kMlpHid],
-      H2[kMlpTile * kMlpHid];
+  __shared__ __attribute__((aligned(16))) uint16_t X[kMlpTile * kLd], H1[kMlpTile * kLd],
+      H2[kMlpTile * kLd];
   __shared__ float y[kMlpTile];
   const int row0 = blockIdx.x * kMlpTile;
-  tile_forward(W, feat, row0, n, X, H1, H2, y);
+  tile_forward<kMlpIn>(W, feat, row0, n, X, H1, H2, y);
   if (threadIdx.x < kMlpTile && row0 + (int)threadIdx.x < n) out[row0 + threadIdx.x] = y[threadIdx.x];
 }
 
-// Loss 0.5 * mean((y - t)^2).  partial[blockIdx.x][0 .. kMlpParams) receives
-// this tile's gradient, partial[...][kMlpParams] its summed squared error.
-__global__ __launch_bounds__(256) void mlp_grad_kernel(MlpView W,
-                                                       const uint16_t* __restrict__ feat,
-                                                       const float* __restrict__ target,
-                                                       int32_t n, float* __restrict__ partial) {
-  __shared__ __attribute__((aligned(16))) uint16_t X[kMlpTile * kMlpIn], H1[kMlpTile * kMlpHid],
-      H2[kMlpTile * kMlpHid];
-  __shared__ __attribute__((aligned(16))) uint16_t D2[kMlpTile * kMlpHid], D1[kMlpTile * kMlpHid];
-  __shared__ float y[kMlpTile], dy[kMlpTile], sq[kMlpTile];
-  const int row0 = blockIdx.x * kMlpTile;
-  tile_forward(W, feat, row0, n, X, H1, H2, y);
-  float* P = partial + (size_t)blockIdx.x * kMlpPartialStride;
-  if (threadIdx.x < kMlpTile) {
-    const int m = threadIdx.x;
-    const bool ok = row0 + m < n;
-    const float diff = ok ? (y[m] - target[row0 + m]) : 0.f;
-    dy[m] = diff / (float)n;
-    sq[m] = diff * diff;
-  }
-  __syncthreads();
-  // dH2 = dy * w3 * relu'(H2), rounded to bf16; dw3, db3, db2.
-  for (int e = threadIdx.x; e < kMlpTile * kMlpHid; e += blockDim.x) {
-    const int m = e / kMlpHid, h = e % kMlpHid;
-    const float g = (H2[e] != 0) ? dy[m] * bf2f(W.w3b[h]) : 0.f;
-    D2[e] = f2bf(g);
-  }
-  __syncthreads();
-  if (threadIdx.x < kMlpHid) {
-    const int h = threadIdx.x;
-    float gw3 = 0.f, gb2 = 0.f;
-    for (int m = 0; m < kMlpTile; ++m) {
-      gw3 += dy[m] * bf2f(H2[m * kMlpHid + h]);
-      gb2 += bf2f(D2[m * kMlpHid + h]);
+// dC[o][i] += sum_m A[m][o] B[m][i] over one tile (m < kMlpTile rows), for
+// the 16 output tiles of this wave (o, i tiles t = wave + 4 j, column-major
+// over 8 x 8 tiles): A and B are row-major bf16 in LDS with row strides sa
+// and sb; acc[j] holds the wave's j-th 16x16 tile (MFMA layout).
+template <int kSa, int kSb>
+__device__ __forceinline__ void acc_tn_gemm(const uint16_t* A, const uint16_t* B, f4v* acc) {
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int r = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int t = wave + 4 * j;
+    const int o0 = (t / 8) * 16, i0 = (t % 8) * 16;
+#pragma unroll
+    for (int k0 = 0; k0 < kMlpTile; k0 += 16) {
+      s4v a, b;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        a[e] = (short)A[(k0 + 4 * q + e) * kSa + o0 + r];
+        b[e] = (short)B[(k0 + 4 * q + e) * kSb + i0 + r];
+      }
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
     }
-    P[kOffW3 + h] = gw3;
-    P[kOffB2 + h] = gb2;
-  }
-  if (threadIdx.x == 0) {
-    float gb3 = 0.f, s = 0.f;
-    for (int m = 0; m < kMlpTile; ++m) { gb3 += dy[m]; s += sq[m]; }
-    P[kOffB3] = gb3;
-    P[kMlpParams] = s;
-  }
-  // dH1 = (dH2 W2) * relu'(H1), rounded to bf16.
-  wg_gemm<true, false, kMlpHid>(D2, kMlpHid, 1, W.w2b, kMlpHid, 1, kMlpTile, kMlpHid, kMlpHid,
-          [&](int m, int h, float acc) {
-            D1[m * kMlpHid + h] = (H1[m * kMlpHid + h] != 0) ? f2bf(acc) : (uint16_t)0;
-          });
-  __syncthreads();
-  // dW2[o][i] = sum_m dH2[m][o] H1[m][i];  dW1[o][i] = sum_m dH1[m][o] X[m][i].
-  wg_gemm<false, false, kMlpTile>(D2, 1, kMlpHid, H1, kMlpHid, 1, kMlpHid, kMlpHid, kMlpTile,
-          [&](int o, int i, float acc) { P[kOffW2 + o * kMlpHid + i] = acc; });
-  wg_gemm<false, false, kMlpTile>(D1, 1, kMlpHid, X, kMlpIn, 1, kMlpHid, kMlpIn, kMlpTile,
-          [&](int o, int i, float acc) { P[kOffW1 + o * kMlpIn + i] = acc; });
-  if (threadIdx.x < kMlpHid) {
-    const int h = threadIdx.x;
-    float gb1 = 0.f;
-    for (int m = 0; m < kMlpTile; ++m) gb1 += bf2f(D1[m * kMlpHid + h]);
-    P[kOffB1 + h] = gb1;
   }
 }
 
-// grads[p] = sum over tiles in index order (deterministic); slot kMlpParams is
-// the loss 0.5 * sum (y - t)^2 / n.
-__global__ void mlp_reduce_kernel(const float* __restrict__ partial, int32_t tiles, int32_t n,
+template <int kCols>
+__device__ __forceinline__ void store_acc(const f4v* acc, float* P) {
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int r = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int t = wave + 4 * j;
+    const int o0 = (t / 8) * 16, i0 = (t % 8) * 16;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) P[(o0 + 4 * q + e) * kCols + i0 + r] = acc[j][e];
+  }
+}
+
+// Loss 0.5 * mean((y - t)^2).  Workgroup b takes tiles b * per .. b * per +
+// per - 1 (in order) and writes their summed gradient to partial[b][0 ..
+// kMlpParams) and their summed squared error to partial[b][kMlpParams].
+__global__ __launch_bounds__(256) void mlp_grad_kernel(MlpView W,
+                                                       const uint16_t* __restrict__ feat,
+                                                       const float* __restrict__ target,
+                                                       int32_t n, int32_t per,
+                                                       float* __restrict__ partial) {
+  static_assert(kMlpIn == 128 && kMlpHid == 128, "8 x 8 output tiles per weight matrix");
+  __shared__ __attribute__((aligned(16))) uint16_t X[kMlpTile * kLd], H1[kMlpTile * kLd],
+      H2[kMlpTile * kLd];
+  __shared__ __attribute__((aligned(16))) uint16_t D2[kMlpTile * kLd], D1[kMlpTile * kLd];
+  __shared__ __attribute__((aligned(16))) uint16_t W1s[kMlpHid * kLd], W2s[kMlpHid * kLd];
+  __shared__ float y[kMlpTile], dy[kMlpTile], sq[kMlpTile];
+  // The bf16 weights stay in LDS for all of this workgroup's tiles: every
+  // MFMA fragment of the forward and backward products is then an LDS read.
+  static_assert(kMlpHid * kMlpIn == kMlpHid * kMlpHid, "W1 and W2 copied together");
+  for (int e = threadIdx.x; e < kMlpHid * kMlpIn / 8; e += blockDim.x) {
+    const int row = e / (kMlpIn / 8), c8 = e % (kMlpIn / 8);
+    *reinterpret_cast<uint4*>(W1s + row * kLd + 8 * c8) = reinterpret_cast<const uint4*>(W.w1b)[e];
+    *reinterpret_cast<uint4*>(W2s + row * kLd + 8 * c8) = reinterpret_cast<const uint4*>(W.w2b)[e];
+  }
+  // (tile_forward's first barrier orders these stores before the products.)
+  MlpView WL = W;
+  WL.w1b = W1s;
+  WL.w2b = W2s;
+  f4v g1[16], g2[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    g1[j] = f4v{0.f, 0.f, 0.f, 0.f};
+    g2[j] = f4v{0.f, 0.f, 0.f, 0.f};
+  }
+  // Threads h < 128 keep the column sums of unit h; thread 0 the scalars.
+  float gw3 = 0.f, gb2 = 0.f, gb1 = 0.f, gb3 = 0.f, sse = 0.f;
+  const int h = threadIdx.x;
+  const int tiles = (n + kMlpTile - 1) / kMlpTile;
+  const int t_end = min(tiles, (int)(blockIdx.x + 1) * per);
+  for (int tile = blockIdx.x * per; tile < t_end; ++tile) {
+    const int row0 = tile * kMlpTile;
+    tile_forward<kLd>(WL, feat, row0, n, X, H1, H2, y);
+    if (threadIdx.x < kMlpTile) {
+      const int m = threadIdx.x;
+      const bool ok = row0 + m < n;
+      const float diff = ok ? (y[m] - target[row0 + m]) : 0.f;
+      dy[m] = diff / (float)n;
+      sq[m] = diff * diff;
+    }
+    __syncthreads();
+    // dH2 = dy * w3 * relu'(H2), rounded to bf16.
+    for (int e = threadIdx.x; e < kMlpTile * kMlpHid; e += blockDim.x) {
+      const int m = e / kMlpHid, hh = e % kMlpHid;
+      const float g = (H2[m * kLd + hh] != 0) ? dy[m] * bf2f(W.w3b[hh]) : 0.f;
+      D2[m * kLd + hh] = f2bf(g);
+    }
+    __syncthreads();
+    if (h < kMlpHid) {
+      for (int m = 0; m < kMlpTile; ++m) {
+        gw3 += dy[m] * bf2f(H2[m * kLd + h]);
+        gb2 += bf2f(D2[m * kLd + h]);
+      }
+    }
+    if (threadIdx.x == 0) {
+      for (int m = 0; m < kMlpTile; ++m) { gb3 += dy[m]; sse += sq[m]; }
+    }
+    // dH1 = (dH2 W2) * relu'(H1), rounded to bf16.
+    wg_gemm<true, false, kMlpHid>(D2, kLd, 1, W2s, kLd, 1, kMlpTile, kMlpHid, kMlpHid,
+            [&](int m, int hh, float acc) {
+              D1[m * kLd + hh] = (H1[m * kLd + hh] != 0) ? f2bf(acc) : (uint16_t)0;
+            });
+    __syncthreads();
+    // dW2[o][i] += sum_m dH2[m][o] H1[m][i];  dW1[o][i] += sum_m dH1[m][o] X[m][i].
+    acc_tn_gemm<kLd, kLd>(D2, H1, g2);
+    acc_tn_gemm<kLd, kLd>(D1, X, g1);
+    if (h < kMlpHid)
+      for (int m = 0; m < kMlpTile; ++m) gb1 += bf2f(D1[m * kLd + h]);
+    __syncthreads();   // the next tile overwrites the LDS operands
+  }
+  float* P = partial + (size_t)blockIdx.x * kMlpPartialStride;
+  store_acc<kMlpHid>(g2, P + kOffW2);
+  store_acc<kMlpIn>(g1, P + kOffW1);
+  if (h < kMlpHid) {
+    P[kOffW3 + h] = gw3;
+    P[kOffB2 + h] = gb2;
+    P[kOffB1 + h] = gb1;
+  }
+  if (threadIdx.x == 0) {
+    P[kOffB3] = gb3;
+    P[kMlpParams] = sse;
+  }
+}
+
+// grads[p] = sum over the partials in index order (deterministic); slot
+// kMlpParams is the loss 0.5 * sum (y - t)^2 / n.
+__global__ void mlp_reduce_kernel(const float* __restrict__ partial, int32_t parts, int32_t n,
                                   float* __restrict__ grads) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p > kMlpParams) return;
   float s = 0.f;
-  for (int t = 0; t < tiles; ++t) s += partial[(size_t)t * kMlpPartialStride + p];
+  for (int t = 0; t < parts; ++t) s += partial[(size_t)t * kMlpPartialStride + p];
   grads[p] = (p == kMlpParams) ? 0.5f * s / (float)n : s;
 }
 
-// One Adam step fused with the tile reduction: grads[p] = sum over tiles in
+// One Adam step fused with the reduction: grads[p] = sum over the partials in
 // index order (the same sums as mlp_reduce_kernel), then the update of p and
 // its bf16 copy.  Slot kMlpParams takes the loss, also copied to *loss_out.
 __global__ void mlp_reduce_adam_kernel(const float* __restrict__ partial, int32_t tiles, int32_t n,
@@ -281,6 +359,33 @@ __global__ void mlp_reduce_adam_kernel(const float* __restrict__ partial, int32_
   if (p < kOffB1) w1b[p - kOffW1] = f2bf(w);
   else if (p >= kOffW2 && p < kOffB2) w2b[p - kOffW2] = f2bf(w);
   else if (p >= kOffW3 && p < kOffB3) w3b[p - kOffW3] = f2bf(w);
+}
+
+// t[i] = (d[i] - mean) / std over the n targets (std 1 when all are equal),
+// one workgroup; sums in double, in a fixed order (deterministic).
+__global__ __launch_bounds__(1024) void mlp_standardize_kernel(const int32_t* __restrict__ d,
+                                                               int32_t n, float* __restrict__ t) {
+  __shared__ double s1[1024], s2[1024];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double x = (double)d[i];
+    a += x;
+    b += x * x;
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  const double mean = s1[0] / (double)n;
+  const double var = s2[0] / (double)n - mean * mean;
+  const double sd = (var > 1e-12) ? sqrt(var) : 1.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) t[i] = (float)(((double)d[i] - mean) / sd);
 }
 
 __global__ void mlp_cast_kernel(const float* __restrict__ theta, uint16_t* __restrict__ w1b,
@@ -324,13 +429,20 @@ hipError_t launch_mlp_predict(const MlpBufs& b, const uint16_t* d_feat, int32_t 
   return hipGetLastError();
 }
 
+int mlp_grad_parts(int32_t n) {
+  const int tiles = (n + kMlpTile - 1) / kMlpTile;
+  const int per = (tiles + kMlpMaxParts - 1) / kMlpMaxParts;
+  return (tiles + per - 1) / per;
+}
+
 hipError_t launch_mlp_grads(const MlpBufs& b, const uint16_t* d_feat, const float* d_target,
                             int32_t n, float* d_partial, hipStream_t s) {
   const int tiles = (n + kMlpTile - 1) / kMlpTile;
-  hipLaunchKernelGGL(mlp_grad_kernel, dim3(tiles), dim3(256), 0, s, view(b), d_feat, d_target, n,
-                     d_partial);
+  const int per = (tiles + kMlpMaxParts - 1) / kMlpMaxParts, parts = mlp_grad_parts(n);
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(parts), dim3(256), 0, s, view(b), d_feat, d_target, n,
+                     per, d_partial);
   hipLaunchKernelGGL(mlp_reduce_kernel, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, s,
-                     d_partial, tiles, n, b.grads);
+                     d_partial, parts, n, b.grads);
   return hipGetLastError();
 }
 
@@ -338,13 +450,21 @@ hipError_t launch_mlp_train_step(const MlpBufs& b, const uint16_t* d_feat, const
                                  int32_t n, float* d_partial, float lr, int step, float* d_loss,
                                  hipStream_t s) {
   const int tiles = (n + kMlpTile - 1) / kMlpTile;
+  const int per = (tiles + kMlpMaxParts - 1) / kMlpMaxParts, parts = mlp_grad_parts(n);
   const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
   const float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
-  hipLaunchKernelGGL(mlp_grad_kernel, dim3(tiles), dim3(256), 0, s, view(b), d_feat, d_target, n,
-                     d_partial);
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(parts), dim3(256), 0, s, view(b), d_feat, d_target, n,
+                     per, d_partial);
   hipLaunchKernelGGL(mlp_reduce_adam_kernel, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, s,
-                     d_partial, tiles, n, b.theta, b.grads, b.m1, b.m2, lr, beta1, beta2, eps, c1,
+                     d_partial, parts, n, b.theta, b.grads, b.m1, b.m2, lr, beta1, beta2, eps, c1,
                      c2, b.w1b, b.w2b, b.w3b, d_loss);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_standardize(const int32_t* d_dist, int32_t n, float* d_target,
+                                  hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(mlp_standardize_kernel, dim3(1), dim3(1024), 0, s, d_dist, n, d_target);
   return hipGetLastError();
 }
 

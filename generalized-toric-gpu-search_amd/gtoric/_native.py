@@ -84,6 +84,7 @@ _SIGS = {
     "gt_evolve_create2": ([vp, i32, i32, i32, u64, i32, i32, vp, vp, i64, i32, P(vp)],
                           ctypes.c_int),
     "gt_evolve_run": ([vp, i32], ctypes.c_int),
+    "gt_evolve_train_ranker": ([vp, i32, i32, i32, ctypes.c_float, i32], ctypes.c_int),
     "gt_evolve_sync": ([vp, P(i32)], ctypes.c_int),
     "gt_evolve_log": ([vp, i32, i32, vp, vp, vp, vp], ctypes.c_int),
     "gt_evolve_population": ([vp, vp, vp, vp, P(i64)], ctypes.c_int),

@@ -111,6 +111,14 @@ class Evolve:
     def run(self, generations: int) -> None:
         check(lib().gt_evolve_run(self.handle, int(generations)))
 
+    def train_ranker(self, slot0: int, slot1: int, batch: int = 1024, lr: float = 2e-3,
+                     epochs: int = 1) -> None:
+        """Train the ranker on log slots [slot0, slot1) on the device
+        (gt_evolve_train_ranker; asynchronous, ordered before the next
+        generation)."""
+        check(lib().gt_evolve_train_ranker(self.handle, int(slot0), int(slot1), int(batch),
+                                           ctypes.c_float(lr), int(epochs)))
+
     def sync(self) -> int:
         n = ctypes.c_int32()
         check(lib().gt_evolve_sync(self.handle, ctypes.byref(n)))
@@ -150,13 +158,21 @@ class Evolve:
 
 def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogate=None,
         proposal_factor: int = 4, wall_limit_s: float | None = None, train_every: int = 5,
-        train_epochs: int = 3, train_window: int = 65536, ctx: _native.Context | None = None,
-        verbose: bool = False, burst: int = 8, sample_count: int = 0,
-        canonical: bool | None = None) -> EvolveResult:
+        train_epochs: int = 1, init_epochs: int = 8, train_window: int | None = None,
+        train_batch: int = 4096,
+        ctx: _native.Context | None = None, verbose: bool = False, burst: int = 8,
+        sample_count: int = 0, canonical: bool | None = None) -> EvolveResult:
     """Search for `generations` (or until `wall_limit_s`), launching `burst`
     generations between host checks; returns the log of every evaluation.
     sample_count > 0 scores by the sampled stream (distances are then sampled
-    upper bounds); canonical (default: k <= 16) deduplicates by AGL2 class."""
+    upper bounds); canonical (default: k <= 16) deduplicates by AGL2 class.
+
+    With a surrogate the network is trained on the device from the search's
+    own log: `init_epochs` passes over the initial population, then every
+    `train_every` generations (while new classes keep appearing)
+    `train_epochs` passes over the last `train_window` generations (default:
+    the ones since the last training) in batches of `train_batch` rows,
+    queued between generations with no host copy."""
     ctx = ctx or _native.context(0)
     rng = np.random.default_rng(seed)
     n_prop = pop * (proposal_factor if surrogate is not None else 1)
@@ -166,11 +182,12 @@ def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogat
                 ranker=surrogate, sample_count=sample_count, canonical=canonical)
     try:
         t0 = time.perf_counter()
+        window = train_every if train_window is None else int(train_window)
         if surrogate is not None:
-            c0, d0, _, _ = ev.log(0, 1)
-            surrogate.fit(c0.ravel(), d0.ravel(), epochs=train_epochs, batch=1024, seed=0)
+            ev.train_ranker(0, 1, batch=train_batch, epochs=init_epochs)
         history = []
         done = 0
+        fresh = 0          # new classes proposed since the last training
         while done < generations:
             if wall_limit_s is not None and time.perf_counter() - t0 >= wall_limit_s:
                 break
@@ -182,16 +199,19 @@ def run(k: int, pop: int = 2048, generations: int = 200, seed: int = 1, surrogat
             now = time.perf_counter() - t0
             _, _, best, new = ev.log(done - step + 1, done + 1, sets=False)
             history.extend((now, int(b)) for b in best)
+            fresh += int(new.sum())
             if verbose:
                 print(f"[evolve k={k}] gen {done} best d={best[-1]} new classes={new[-1]}",
                       flush=True)
-            if surrogate is not None and done % train_every == 0 and done < generations:
-                # A sliding window of the most recent exact results keeps a
-                # retraining step at a fixed cost as the run grows.
-                s0 = max(0, done + 1 - (train_window + pop - 1) // pop)
-                wc, wd, _, _ = ev.log(s0, done + 1)
-                surrogate.fit(wc.ravel()[-train_window:], wd.ravel()[-train_window:],
-                              epochs=train_epochs, batch=1024, seed=done)
+            if (surrogate is not None and done % train_every == 0 and done < generations
+                    and fresh > 0):
+                # The most recent generations' results, at a fixed cost per
+                # training however long the run grows; skipped once the
+                # search stops finding new classes (a class space it has
+                # exhausted gives the network nothing new to learn).
+                ev.train_ranker(max(0, done + 1 - window), done + 1, batch=train_batch,
+                                epochs=train_epochs)
+                fresh = 0
         wall = time.perf_counter() - t0
         classes, dists, _, _ = ev.log(0, done + 1)
         popm, popd, _, _ = ev.population()

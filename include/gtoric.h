@@ -284,6 +284,14 @@ int gt_evolve_create2(gt_ctx* ctx, int32_t k, int32_t pop, int32_t n_prop, uint6
                       gt_evolve** out);
 /* Enqueue `generations` more generations (asynchronous). */
 int gt_evolve_run(gt_evolve* ev, int32_t generations);
+/* Train the search's ranker on its own log slots [slot0, slot1) (classes and
+ * exact distances, already in device memory): features, targets
+ * standardised over the slots, `epochs` passes of Adam steps over batches of
+ * `batch` rows in log order.  Enqueued on the context stream between
+ * generations, with no host copy or synchronisation; the next generation
+ * ranks with the updated network.  No reference counterpart. */
+int gt_evolve_train_ranker(gt_evolve* ev, int32_t slot0, int32_t slot1, int32_t batch, float lr,
+                           int32_t epochs);
 /* Wait for them; device-side errors are reported here. */
 int gt_evolve_sync(gt_evolve* ev, int32_t* generations_done);
 /* Log slots [slot0, slot1): slot 0 is the initial population, slot g the sets

@@ -131,7 +131,7 @@ class Model:
         if scorer is not None and cnt > pop:
             score = scorer(pcm)
             key = (score_desc_key(score[keep]) << U64(32)) | np.arange(cnt, dtype=np.uint64)
-            sel = keep[np.argsort(key, kind="stable")[:pop]]
+            sel = keep[np.sort(np.argsort(key, kind="stable")[:pop])]   # in proposal order
         else:
             sel = keep[:real]
         chosen = np.concatenate([props[sel],
