@@ -3,9 +3,10 @@
 
 Headline (value): exact min-distance codes/s on the mixed-k batch — 4096
 seeded codes, k uniform in 3..8, n = 49 — one step = one device-resident
-gt_md_batch_dev pass over all 4096 codes (rows, plan, persistent walk,
-finalize).  Also reported in the same JSON line: the plain one-launch-per-code
-form on the same codes, AGL2(F7) canonical forms/s on 10^6 mixed-size masks,
+gt_md_batch_dev pass over all 4096 codes (rows and plan, then the persistent
+walk, which writes each code's result as its last chunk finishes).  Also
+reported in the same JSON line: the plain one-launch-per-code form on the
+same codes, AGL2(F7) canonical forms/s on 10^6 mixed-size masks,
 the INT32 VALU roofline of the walk kernel (HIP events around every launch), and
 the CPU oracle timed on the host cores.
 """
@@ -132,7 +133,7 @@ def bench_md(ctx, sets, steps: int, warmup: int, expect=None):
     ctx.prof_enable(False)
     nk, kms = ctx.prof_read("md_main")
     # The same step captured once as a hipGraph and replayed (how the
-    # evolutionary driver runs its generations): four kernels, one launch.
+    # evolutionary driver runs its generations): two kernels, one launch.
     ctx.capture_begin()
     step()
     graph = ctx.capture_end()
@@ -572,7 +573,7 @@ def main() -> None:
         },
         "roofline": {
             "bound": "valu_int32",
-            "kernel": "md_main_kernel",
+            "kernel": "md_main_exact_kernel",
             "achieved": round(achieved, 3),
             "peak": round(peak, 3),
             "unit": "TOP/s",
@@ -602,7 +603,7 @@ def main() -> None:
     result["graph_replay"] = {
         "value": round(n * args.steps / dt_graph, 1), "unit": "codes/s",
         "ms_per_step": round(dt_graph / args.steps * 1e3, 4),
-        "form": "the same step (rows, plan, walk, finalize) captured once as a hipGraph and "
+        "form": "the same step (rows and plan, walk) captured once as a hipGraph and "
                 "replayed",
     }
     log(f"[bench] graph replay: {n * args.steps / dt_graph:,.0f} codes/s, "

@@ -134,22 +134,23 @@ struct ProfRec {
 };
 
 // Plan area of one exact pass: claim counters (kClaimShards x 4 KB), then
-// MdPlan (256 bytes), then the order array (4 bytes per code).
+// MdPlan (512 bytes), then the order and row-offset arrays (4 bytes per code
+// each).
 constexpr size_t kMdClaimBytes = (size_t)kClaimShards * kClaimStride * 4;
-constexpr size_t kMdPlanAreaBytes = kMdClaimBytes + 256;
-static_assert(sizeof(MdPlan) <= 256, "MdPlan fits its slot");
+constexpr size_t kMdPlanAreaBytes = kMdClaimBytes + 512;
+static_assert(sizeof(MdPlan) <= 512, "MdPlan fits its slot");
 
 // ---- scratch of the allocation-free exact pass (gt_md_batch_dev_noalloc) --
 // One file-scope device array per GPU, held until the process ends: the plan
-// area, the per-code minima and the bit rows of at most kNaMaxCodes codes
+// area, the per-code state words and the bit rows of at most kNaMaxCodes codes
 // with kNaMaxIdx indices in total (6.6 MB).  Host code never passes the
 // array's name to a kernel: it uses the address hipGetSymbolAddress returns
 // for the current device (g_scratch_ptr), taken once per device by gt_open.
 constexpr int32_t kNaMaxCodes = 16384;
 constexpr int64_t kNaMaxIdx = 65536;
 constexpr size_t kNaPlanOff = 0;
-constexpr size_t kNaBestOff = kNaPlanOff + kMdPlanAreaBytes + (size_t)kNaMaxCodes * 4;
-constexpr size_t kNaRowsOff = kNaBestOff + (size_t)kNaMaxCodes * 4;
+constexpr size_t kNaBestOff = kNaPlanOff + kMdPlanAreaBytes + (size_t)kNaMaxCodes * 8;
+constexpr size_t kNaRowsOff = kNaBestOff + (size_t)kNaMaxCodes * 8;
 constexpr size_t kNaBytes = kNaRowsOff + (size_t)kNaMaxIdx * 3 * kRowWords * 4;
 static_assert(kNaBestOff % 256 == 0 && kNaRowsOff % 256 == 0, "aligned scratch regions");
 
@@ -313,32 +314,34 @@ struct gt_ctx {
       GT_OK(prof_end("md_sampled", s, e0));
       return 0;
     }
-    GT_OK(plan.ensure(kMdPlanAreaBytes + (size_t)n * 4));
-    GT_OK(best.ensure((size_t)n * 4));
+    GT_OK(plan.ensure(kMdPlanAreaBytes + (size_t)n * 8));
+    GT_OK(best.ensure((size_t)n * 8));
     return md_exact_launch(d_idx, n_idx, d_offsets, n, d_tmax, tmax_s, thresholded, d_out,
-                           rows.as<uint32_t>(), plan.as<uint8_t>(), best.as<uint32_t>(), s, true);
+                           rows.as<uint32_t>(), plan.as<uint8_t>(), best.as<uint64_t>(), s, true);
   }
 
-  // Exact or thresholded pass (prep, walk, finalize) with caller-given
-  // scratch: rows (3 * n_idx rows), the plan area (kMdPlanAreaBytes + 4 n
-  // bytes: claim counters, MdPlan, order) and best (n words).  Every word the
-  // kernels read is written by md_prep first, so no clearing is needed.
+  // Exact or thresholded pass (prep, walk) with caller-given scratch: rows
+  // (3 * n_idx rows), the plan area (kMdPlanAreaBytes + 8 n bytes: claim
+  // counters, MdPlan, order, row offsets) and the per-code state (n 64-bit
+  // words).  Every word the kernels read is written by md_prep first, so no
+  // clearing is needed.
   int md_exact_launch(const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets, int32_t n,
                       const int32_t* d_tmax, int32_t tmax_s, bool thresholded, int32_t* d_out,
-                      uint32_t* d_rows, uint8_t* d_plan_area, uint32_t* d_best, hipStream_t s,
+                      uint32_t* d_rows, uint8_t* d_plan_area, uint64_t* d_state, hipStream_t s,
                       bool profile) {
     uint32_t* d_claims = reinterpret_cast<uint32_t*>(d_plan_area);
     MdPlan* d_plan = reinterpret_cast<MdPlan*>(d_plan_area + kMdClaimBytes);
     uint32_t* d_order = reinterpret_cast<uint32_t*>(d_plan_area + kMdPlanAreaBytes);
+    uint32_t* d_roff = d_order + n;
     GT_HIP(launch_md_prep(d_idx, n_idx, mplanes.as<uint32_t>(), d_rows, d_offsets, n,
-                          params_buf.as<MdParams>(), d_plan, d_order, d_claims, d_best,
-                          ctrl_p, s));
+                          params_buf.as<MdParams>(), d_plan, d_order, d_roff, d_claims, d_state,
+                          d_out, d_tmax, tmax_s, ctrl_p, s));
     hipEvent_t e0 = nullptr;
     if (profile) GT_OK(prof_begin("md_main", s, &e0));
-    GT_HIP(launch_md_main(thresholded, md_grid[thresholded ? 1 : 0], d_rows, d_offsets, d_plan,
-                          d_order, d_claims, d_best, d_tmax, tmax_s, params_buf.as<MdParams>(), s));
+    GT_HIP(launch_md_main(thresholded, md_grid[thresholded ? 1 : 0], d_rows, d_plan, d_order,
+                          d_roff, d_claims, d_state, d_tmax, tmax_s, d_out,
+                          params_buf.as<MdParams>(), s));
     if (profile) GT_OK(prof_end("md_main", s, e0));
-    GT_HIP(launch_md_finalize(d_offsets, n, d_best, d_tmax, tmax_s, d_out, s));
     return 0;
   }
 };
@@ -548,7 +551,7 @@ int gt_md_batch_dev_noalloc(gt_ctx* c, const int32_t* d_idx, int64_t n_idx,
   uint8_t* base = c->scratch_dev;
   return c->md_exact_launch(d_idx, n_idx, d_offsets, n, d_tmax, tmax, thresholded != 0, d_out,
                             reinterpret_cast<uint32_t*>(base + kNaRowsOff), base + kNaPlanOff,
-                            reinterpret_cast<uint32_t*>(base + kNaBestOff), c->pick(stream),
+                            reinterpret_cast<uint64_t*>(base + kNaBestOff), c->pick(stream),
                             false);
 }
 

@@ -66,15 +66,18 @@ struct MdParams {
 
 // Chunk plan of one batch, largest codes first.  Bucket j (0 <= j < kMaxKExact)
 // holds the codes whose walk has dimension kMaxKExact - j (k = kMaxKExact -
-// j + 1 generators), in batch order, at positions
-// pb[j] .. of the order array; its codes own chunk ids cb[j] + q * cc[j] ..
-// + cc[j] - 1 for the q-th code.  cb[kMaxKExact] is the total chunk count (0
-// when the plan overflowed 2^31 chunks).  Claiming ids in increasing order
-// hands out the biggest chunks first, so the last chunks of a launch are the
-// small ones of the small codes.
+// j + 1 generators), in batch order, at positions pb[j] .. pb[j+1] - 1 of the
+// order array (pb[kMaxKExact] = codes in the plan).  For the claimed
+// (thresholded) walk its codes own chunk ids cb[j] + q * cc[j] .. + cc[j] - 1
+// for the q-th code; cb[kMaxKExact] is the total chunk count (0 when the plan
+// overflowed 2^31 chunks).  Claiming ids in increasing order hands out the
+// biggest chunks first, so the last chunks of a launch are the small ones of
+// the small codes.  For the dealt (exact) walk the q-th code owns work units
+// ub[j] + q * code_units(w) .. (ub[kMaxKExact] = total units).
 struct MdPlan {
+  uint64_t ub[kMaxKExact + 1];
   uint32_t cb[kMaxKExact + 1];
-  uint32_t pb[kMaxKExact];
+  uint32_t pb[kMaxKExact + 1];
   uint32_t cc[kMaxKExact];
 };
 
@@ -111,6 +114,30 @@ GT_HD uint64_t code_chunks(int w, int L) {
   uint64_t n = 1;  // tiny chunk
   for (int j = w - 1; j > tiny_top(w); --j) n += piece_chunks(j, L);
   return n;
+}
+
+// Work units of the dealt walk.  A block is 16 Gray steps of all 64 lanes
+// (1024 codewords, 174 VALU instructions).  Piece j > kTinyTop is walked
+// lane-major: lane l owns the Gray indices l * 2^(3j-6) + [0, 2^(3j-6)), and
+// the piece is 2^(3j-10) blocks, one unit each; a wave may walk any run of
+// them.  The tiny part (pieces 0 .. tiny_top(w): 585 codewords for
+// w > kTinyTop, about 300 VALU instructions in one wave) counts as
+// kTinyUnits units, which also prices the start of a code (two dependent
+// loads before its first row) and its fold: measured on a 4096-code mix,
+// weights 6 to 8 balance best, 2 leaves the waves that hold many k = 6 or 7
+// codes the last to finish.  A code's units are its pieces from the largest
+// down, then its tiny part.  Codes of walk dimension <= kSmallW have no piece
+// above kTinyTop; they are dealt as whole codes instead (md_main_exact).
+constexpr int kBlockBits = kLaneBits + kU;
+constexpr int kSmallW = kTinyTop + 1;
+constexpr uint32_t kTinyUnits = 6;
+GT_HD uint64_t piece_blocks(int j) { return 1ull << (3 * j - kBlockBits); }
+// kTinyUnits + sum of 2^(3j-10) over j = kSmallW .. w-1, i.e.
+// (8^w - 8^kSmallW) / (7 * 2^10).
+GT_HD uint64_t code_units(int w) {
+  if (w <= 0) return 0;
+  if (w <= kSmallW) return kTinyUnits;
+  return kTinyUnits + (((1ull << (3 * w)) - (1ull << (3 * kSmallW))) >> kBlockBits) / 7ull;
 }
 
 // Projective codewords of a w-dimensional code: (8^w - 1) / 7.

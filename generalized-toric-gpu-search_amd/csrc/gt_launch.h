@@ -22,19 +22,19 @@ constexpr int kCtrlWords = 16;
 hipError_t launch_md_rows(const int32_t* d_idx, int64_t n_idx, const uint32_t* d_mplanes,
                           uint32_t* d_rows, uint32_t* d_ctrl, hipStream_t s);
 // d_claims: kClaimShards counters, kClaimStride words apart (reset by the plan).
+// d_order, d_roff: n entries each (code index and first row per plan position).
+// d_state: one word per code (minimum weight, chunks finished), initialised
+// by md_prep, which also writes the results of codes without a walk (k <= 1);
+// md_main writes every other code's result when its last chunk is folded.
 hipError_t launch_md_prep(const int32_t* d_idx, int64_t n_idx, const uint32_t* d_mplanes,
                           uint32_t* d_rows, const int32_t* d_offsets, int32_t n,
                           const MdParams* d_params, MdPlan* d_plan, uint32_t* d_order,
-                          uint32_t* d_claims, uint32_t* d_best, uint32_t* d_ctrl,
-                          hipStream_t s);
+                          uint32_t* d_roff, uint32_t* d_claims, uint64_t* d_state, int32_t* d_out,
+                          const int32_t* d_tmax, int32_t tmax, uint32_t* d_ctrl, hipStream_t s);
 hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
-                          const int32_t* d_offsets, const MdPlan* d_plan,
-                          const uint32_t* d_order, uint32_t* d_claims, uint32_t* d_best,
-                          const int32_t* d_tmax, int32_t tmax, const MdParams* d_params,
-                          hipStream_t s);
-hipError_t launch_md_finalize(const int32_t* d_offsets, int32_t n, const uint32_t* d_best,
-                              const int32_t* d_tmax, int32_t tmax, int32_t* d_out,
-                              hipStream_t s);
+                          const MdPlan* d_plan, const uint32_t* d_order, const uint32_t* d_roff,
+                          uint32_t* d_claims, uint64_t* d_state, const int32_t* d_tmax,
+                          int32_t tmax, int32_t* d_out, const MdParams* d_params, hipStream_t s);
 // One code, one launch: grid = code_chunks(k, L) single-wave workgroups.
 hipError_t launch_md_single(bool thresholded, const uint32_t* d_rows, int k, int L,
                             int check_log2, int thr, uint32_t* d_best, hipStream_t s);

@@ -33,6 +33,15 @@ struct ConstRows {
     for (int i = 0; i < 6; ++i) asm volatile("v_mov_b32 %0, %1" : "=v"(v.w[i]) : "s"(s.w[i]));
     return v;
   }
+  // The same copy, ordered after `after` is computed (an operand the copy
+  // does not read), so it is not hoisted above the work before it.
+  __device__ __forceinline__ RowV vgpr_after(const RowV& s, uint32_t after) const {
+    RowV v;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+      asm volatile("v_mov_b32 %0, %1" : "=v"(v.w[i]) : "s"(s.w[i]), "v"(after));
+    return v;
+  }
 };
 
 __device__ __forceinline__ void xor_row(RowV& v, const RowV& r) {
@@ -95,7 +104,9 @@ __device__ __forceinline__ uint32_t gray_chunk(const Rows& row, uint32_t j, uint
   uint32_t best = weight(v);
   const uint32_t nb = 1u << (L - kU);
   for (uint32_t m = 0; m < nb; ++m) {
-    const RowV d = row.vgpr(row(kU + __builtin_ctz(m + 1)));
+    // The block's last row is loaded at its top and copied to VGPRs only
+    // before its last step, so the scalar load's latency is covered.
+    const RowV dl = row(kU + __builtin_ctz(m + 1));
     GT_STEP2(s0, s1)
     GT_STEP2(s0, s2)
     GT_STEP2(s0, s1)
@@ -103,6 +114,7 @@ __device__ __forceinline__ uint32_t gray_chunk(const Rows& row, uint32_t j, uint
     GT_STEP2(s0, s1)
     GT_STEP2(s0, s2)
     GT_STEP2(s0, s1)
+    const RowV d = row.vgpr_after(dl, best);
     GT_STEP2(s0, d)
     if (THR && (m & check_mask) == check_mask) {
       if (__any((int)best <= thr)) break;
@@ -113,21 +125,79 @@ __device__ __forceinline__ uint32_t gray_chunk(const Rows& row, uint32_t j, uint
 }
 static_assert(kU == 4, "gray_chunk unrolls 16 steps");
 
-// Pieces 0..jt enumerated directly: codeword index t in [0, (8^(jt+1)-1)/7).
+// Blocks [m0, m1) of Gray piece j (j > kTinyTop), walked lane-major: lane l
+// owns the Gray indices x = l * 2^P + s, s in [0, 2^P), P = 3j - 6, and walks
+// s = 16 m0 .. 16 m1 (one past the run; a nonzero codeword of the piece, so
+// the minimum is unchanged).  gray(x) = gray(l) << P ^ (l & 1) << (P - 1) ^
+// gray(s), so the start is h_j, the rows of gray(16 m0) (wave-uniform), and
+// the lane's rows P - 1 .. P + 5; step s -> s + 1 adds row ctz(s + 1).
 template <class Rows>
-__device__ __forceinline__ uint32_t tiny_chunk(const Rows& row, int jt, uint32_t lane) {
-  uint32_t best = 64;
-  const uint32_t T = (uint32_t)proj_codewords(jt + 1);
-  for (uint32_t t = lane; t < T; t += kWave) {
-    uint32_t j = 0;
-    while ((int)j < jt && t >= (uint32_t)proj_codewords(j + 1)) ++j;
-    const uint32_t a = t - (uint32_t)proj_codewords(j);
-    RowV v = row(3 * j);
-    for (int r = 0; r < 3 * jt; ++r) xor_row_masked(v, row(r), 0u - ((a >> r) & 1u));
-    best = min(best, weight(v));
+__device__ __forceinline__ uint32_t gray_blocks(const Rows& row, uint32_t j, uint64_t m0,
+                                                uint64_t m1, uint32_t lane) {
+  const uint32_t P = 3 * j - kLaneBits;
+  RowV v = row(3 * j);
+  uint64_t gs = (m0 << kU) ^ (m0 << (kU - 1));
+  for (uint32_t r = kU - 1; gs >> r != 0; ++r)
+    if ((gs >> r) & 1) xor_row(v, row(r));
+  const uint32_t lp = lane ^ (lane >> 1);
+#pragma unroll
+  for (int t = 0; t < kLaneBits; ++t) xor_row_masked(v, row(P + t), 0u - ((lp >> t) & 1u));
+  xor_row_masked(v, row(P - 1), 0u - (lane & 1u));
+
+  const RowV s0 = row.vgpr(row(0)), s1 = row.vgpr(row(1)), s2 = row.vgpr(row(2)),
+             s3 = row.vgpr(row(3));
+  uint32_t best = weight(v);
+  // m0 < m1; the loop tests m != m1, which stays a scalar compare for 64-bit m.
+  for (uint64_t m = m0; m != m1; ++m) {
+    const RowV dl = row(kU + (uint32_t)__builtin_ctzll(m + 1));
+    GT_STEP2(s0, s1)
+    GT_STEP2(s0, s2)
+    GT_STEP2(s0, s1)
+    GT_STEP2(s0, s3)
+    GT_STEP2(s0, s1)
+    GT_STEP2(s0, s2)
+    GT_STEP2(s0, s1)
+    const RowV d = row.vgpr_after(dl, best);
+    GT_STEP2(s0, d)
   }
   return best;
 }
+
+// Pieces 0..jt (jt <= kTinyTop = 3): pieces j <= 2 one codeword per lane
+// (lane a < 8^j: h_j plus rows 0 .. 3j-1 by the bits of a); piece 3, 512
+// codewords, lane-major as in gray_blocks with P = 3: lane l starts at
+// gray(8 l) and takes the 7 Gray steps over rows 0..2.
+template <class Rows>
+__device__ __forceinline__ uint32_t tiny_chunk(const Rows& row, int jt, uint32_t lane) {
+  uint32_t best = 64;
+#pragma unroll
+  for (int j = 0; j <= 2; ++j) {
+    if (j > jt) break;
+    RowV v = row(3 * j);
+#pragma unroll
+    for (int r = 0; r < 3 * j; ++r) xor_row_masked(v, row(r), 0u - ((lane >> r) & 1u));
+    const uint32_t wt = weight(v);
+    if (lane < (1u << (3 * j))) best = min(best, wt);
+  }
+  if (jt >= 3) {
+    RowV v = row(9);
+    const uint32_t lp = lane ^ (lane >> 1);
+#pragma unroll
+    for (int t = 0; t < kLaneBits; ++t) xor_row_masked(v, row(3 + t), 0u - ((lp >> t) & 1u));
+    xor_row_masked(v, row(2), 0u - (lane & 1u));
+    const RowV r0 = row(0), r1 = row(1), r2 = row(2);
+    best = min(best, weight(v));
+    xor_row(v, r0); best = min(best, weight(v));
+    xor_row(v, r1); best = min(best, weight(v));
+    xor_row(v, r0); best = min(best, weight(v));
+    xor_row(v, r2); best = min(best, weight(v));
+    xor_row(v, r0); best = min(best, weight(v));
+    xor_row(v, r1); best = min(best, weight(v));
+    xor_row(v, r0); best = min(best, weight(v));
+  }
+  return best;
+}
+static_assert(kTinyTop == 3, "tiny_chunk enumerates pieces 0..3");
 
 // Map a code-local chunk id of a walk of dimension w to (piece j, chunk c);
 // j == tiny_top(w) means tiny.

@@ -12,12 +12,19 @@
 //               torus point (1,1) (gt_common.h), which has the same maximum
 //               number of zeros as C(S) with 1/8 of the codewords.  The
 //               sampled path keeps the full rows x^b * g_i (md_rows).
-//   md_main     persistent: each wave takes chunk ids (a static snake deal
-//               in exact mode, claims from 16 counters in thresholded mode),
-//               decodes one against the plan in LDS (bucket, piece, code),
-//               walks 64 lanes x 2^L Gray steps, and folds its wave minimum
-//               into the code's minimum with one atomicMin.
-//   md_finalize minimum weight -> max_zeros, clamped at tmax + 1.
+//   md_main     persistent.  Exact mode: two static deals, one share of each
+//               per wave: the codes of walk dimension <= 4 (no Gray piece),
+//               whole, in equal counts; the other codes' work units (16-step
+//               Gray blocks of their pieces and their tiny parts, in plan
+//               order) in equal runs.  A code inside one run is written by
+//               its wave, a code split over several runs is folded into its
+//               state word by each and written by the last.
+//               Thresholded mode: chunk ids claimed from 16 counters,
+//               decoded against the plan in LDS (bucket, piece, code),
+//               64 lanes x 2^L Gray steps each, every chunk folded at once
+//               (other waves stop on the folded value).  Results are
+//               max_zeros clamped at tmax + 1; codes without a walk
+//               (k <= 1) get theirs from md_prep.
 //
 // Inner step (per projective codeword, all VALU, rows in SGPRs):
 //   6 v_xor_b32 (row add), 2 v_or3_b32 (plane OR), 2 v_bcnt_u32_b32,
@@ -102,23 +109,42 @@ __global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict_
   md_rows_one((int64_t)blockIdx.x * blockDim.x + threadIdx.x, idx, n_idx, mplanes, rows, ctrl);
 }
 
-// Largest-first plan (MdPlan).  One workgroup of 16 waves takes the codes in
-// batches of 4096: their sizes are loaded together (one memory latency per
-// batch) and their bucket ids kept in LDS, and the buckets present anywhere
-// in the batch are collected once, so the per-wave ballot loops visit those
-// buckets only (a mixed batch uses a handful of the 21).  Pass 1 sizes the
-// buckets, pass 2 writes each code's position: bucket base + codes of that
-// bucket in earlier rounds + in earlier waves of this round + lanes below it
-// (mbcnt of the bucket's ballot).  The order inside a bucket is batch order,
-// so the plan is deterministic.
+// Largest-first plan (MdPlan, order[] = code index and roff[] = first row
+// per position).  One workgroup of 16 waves takes the codes in batches of
+// 4096, four per thread; their sizes are loaded together (one memory latency
+// per batch) and their bucket ids kept in registers.  Pass 1 counts the
+// buckets: per wave, one ballot per round and bucket present in the wave,
+// the counts gathered in lane b for bucket b, and one LDS atomic
+// instruction (lane b on bucket b).  One wave turns the 21 counts into bucket
+// bases, chunk and unit offsets (lane b sums the entries below b from LDS).
+// Pass 2 places the codes: one LDS atomic instruction per wave reserves its
+// positions in every bucket's cursor, and mbcnt of each round's ballot gives
+// a lane its place in them.  The order inside a bucket therefore follows the waves' atomics, not
+// the batch order; no kernel depends on it, each code's result being its own.
+// (A plan that kept the batch order with per-round prefix sums over the
+// waves took 14.7 us for 4096 codes, against 1.7 us for the rows.)
 constexpr int kPlanThreads = 1024;
 constexpr int kPlanRounds = 4;
 constexpr int kPlanBatch = kPlanThreads * kPlanRounds;
 
+// Result of a code whose minimum weight is b (max_zeros clamped at tmax + 1).
+__device__ __forceinline__ int32_t md_result(int b, int tmax) {
+  return (b <= 48 - tmax) ? tmax + 1 : 49 - b;
+}
+
+// This thread's codes base + r * kPlanThreads + tid of one batch: bucket
+// (-1: none) and first index.
+struct PlanCodes {
+  int j[kPlanRounds];
+  int32_t ofs[kPlanRounds];
+};
+
 __device__ __forceinline__ void md_plan_load_batch(const int32_t* __restrict__ offsets, int32_t n,
-                                                   int64_t n_idx, int base, int8_t* jb,
-                                                   uint32_t* best, bool first_pass, uint32_t& err,
-                                                   uint32_t& present) {
+                                                   int64_t n_idx, int base, uint64_t* state,
+                                                   int32_t* out, const int32_t* tmax_arr,
+                                                   int32_t tmax_s, bool first_pass,
+                                                   uint32_t& err, uint32_t& present,
+                                                   PlanCodes& pc) {
   const int tid = threadIdx.x;
   int kk[kPlanRounds];
   bool inside[kPlanRounds];
@@ -126,6 +152,7 @@ __device__ __forceinline__ void md_plan_load_batch(const int32_t* __restrict__ o
   for (int r = 0; r < kPlanRounds; ++r) {
     const int i = base + r * kPlanThreads + tid;
     const int o0 = (i < n) ? offsets[i] : 0, o1 = (i < n) ? offsets[i + 1] : 0;
+    pc.ofs[r] = o0;
     kk[r] = o1 - o0;
     inside[r] = o0 >= 0 && (int64_t)o1 <= n_idx;   // rows and indices of the code in range
   }
@@ -135,123 +162,158 @@ __device__ __forceinline__ void md_plan_load_batch(const int32_t* __restrict__ o
     int j = -1;
     if (i < n) {
       const int k = kk[r];
-      if (k < 0 || k > kMaxKExact || !inside[r]) err |= kErrBadK;
+      const bool bad = k < 0 || k > kMaxKExact || !inside[r];
+      if (bad) err |= kErrBadK;
       else if (k > 1) j = kMaxKExact - walk_dim(k);
-      if (first_pass) best[i] = 64u;
+      if (first_pass) {
+        state[i] = 64ull;   // minimum weight 64, no chunk finished
+        // No walk: k = 0 (and bad input) gives 0, k = 1 a monomial (weight 49).
+        if (j < 0) out[i] = (!bad && k == 1) ? md_result(49, tmax_arr ? tmax_arr[i] : tmax_s) : 0;
+      }
     }
-    jb[r * kPlanThreads + tid] = (int8_t)j;
+    pc.j[r] = j;
     if (j >= 0) present |= 1u << j;
   }
+  // Buckets present in the wave.
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) present |= (uint32_t)__shfl_xor((int)present, off, kWave);
+  present = __builtin_amdgcn_readfirstlane(present);
+}
+
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets, int32_t n,
                                              int64_t n_idx, const MdParams* __restrict__ p,
                                              MdPlan* __restrict__ plan,
                                              uint32_t* __restrict__ order,
+                                             uint32_t* __restrict__ roff,
                                              uint32_t* __restrict__ claims,
-                                             uint32_t* __restrict__ best, uint32_t* ctrl) {
+                                             uint64_t* __restrict__ state, int32_t* out,
+                                             const int32_t* tmax_arr, int32_t tmax_s,
+                                             uint32_t* ctrl) {
   constexpr int kB = kMaxKExact;
-  constexpr int kWaves = kPlanThreads / kWave;
-  __shared__ uint32_t wc[kWaves][kB];
-  __shared__ uint32_t run[kB];
-  __shared__ uint64_t ccs[kB];       // chunks per code of bucket b, one thread each
-  __shared__ int8_t jb[kPlanBatch];  // bucket id per code of the batch, -1 for none
-  __shared__ uint32_t present_s;     // buckets holding at least one code
-  const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+  __shared__ uint32_t cnt[kB];   // pass 1: codes per bucket; pass 2: next free position
+  __shared__ uint64_t sxc[kB], sxu[kB];
+  const int tid = threadIdx.x, wave = tid / kWave;
+  const uint32_t lane = (uint32_t)(tid % kWave);
   uint32_t err = 0;
+  // Wave 0's lane b reads bucket b's walk length now, with the sizes.
+  const int Lb = (tid < kB) ? (int)p->L[min(kMaxKExact - tid + 1, kMaxKExact)] : 0;
   if (tid < kClaimShards) claims[tid * kClaimStride] = 0u;
-  if (tid < kB) {
-    const int w = kMaxKExact - tid;   // walk dimension of bucket tid (k = w + 1)
-    ccs[tid] = code_chunks(w, p->L[min(w + 1, kMaxKExact)]);
-  }
-  if (tid == 0) present_s = 0u;
+  if (tid < kB) cnt[tid] = 0u;
+  PlanCodes pc;
+  uint32_t present = 0;
+  md_plan_load_batch(offsets, n, n_idx, 0, state, out, tmax_arr, tmax_s, true, err, present, pc);
   __syncthreads();
 
-  // Pass 1: lane b of each wave accumulates the size of bucket b.
-  uint32_t mine = 0;
-  for (int base = 0; base < n; base += kPlanBatch) {
-    uint32_t pres = 0;
-    md_plan_load_batch(offsets, n, n_idx, base, jb, best, true, err, pres);
-    if (pres) atomicOr(&present_s, pres);
-    __syncthreads();
-    const uint32_t present = __builtin_amdgcn_readfirstlane(present_s);
-    for (int r = 0; r < kPlanRounds; ++r) {
-      const int j = jb[r * kPlanThreads + tid];
-      for (uint32_t rem = present; rem != 0; rem &= rem - 1) {
-        const int b = __builtin_ctz(rem);
-        const uint32_t c = (uint32_t)__popcll(__ballot(j == b));
-        mine += (lane == b) ? c : 0u;
+  // Pass 1: bucket sizes.
+  for (int base = 0;;) {
+    uint32_t mine = 0;   // lane b: this wave's codes in bucket b
+    for (uint32_t rem = present; rem != 0; rem &= rem - 1) {
+      const int b = __builtin_ctz(rem);
+      uint32_t c = 0;
+#pragma unroll
+      for (int r = 0; r < kPlanRounds; ++r) c += (uint32_t)__popcll(__ballot(pc.j[r] == b));
+      mine = (lane == (uint32_t)b) ? c : mine;
+    }
+    if (mine != 0u) atomicAdd(&cnt[lane], mine);
+    base += kPlanBatch;
+    if (base >= n) break;
+    present = 0;
+    md_plan_load_batch(offsets, n, n_idx, base, state, out, tmax_arr, tmax_s, true, err, present,
+                       pc);
+  }
+  __syncthreads();
+
+  // Bucket bases, chunk and unit offsets (wave 0, lane b = bucket b).
+  if (wave == 0) {
+    const int b = (int)lane;
+    const int w = kMaxKExact - b;   // walk dimension of bucket b (k = w + 1)
+    const uint32_t t = (b < kB) ? cnt[b] : 0u;
+    const uint64_t cc = (b < kB) ? code_chunks(w, Lb) : 0ull;
+    const uint64_t cu = (b < kB) ? code_units(w) : 0ull;
+    // Chunks of the bucket, clamped at 2^31 (a plan of 2^31 chunks or more
+    // is refused), so the sum of 21 cannot wrap.
+    const uint64_t xc = (t == 0) ? 0ull : (cc >= (1ull << 31)) ? (1ull << 31)
+                                                               : min((uint64_t)t * cc, 1ull << 31);
+    // Units: exact whenever the chunk total is below 2^31 (at most 2^31
+    // chunks of at most 2^23 blocks each).
+    const uint64_t xu = (uint64_t)t * cu;
+    if (b < kB) {
+      sxc[b] = xc;
+      sxu[b] = xu;
+    }
+    // Exclusive sums: the wave's own LDS stores are visible to its later
+    // loads; the loads below b are independent of each other.
+    uint64_t ec = 0, eu = 0;
+    uint32_t ep = 0;
+    for (int q = 0; q < kB; ++q) {
+      if (q < b) {
+        ec += sxc[q];
+        eu += sxu[q];
+        ep += cnt[q];
       }
     }
-    __syncthreads();   // jb is rewritten by the next batch
-  }
-  if (lane < kB) wc[wave][lane] = mine;
-  __syncthreads();
-  if (tid < kB) {
-    uint32_t t = 0;
-    for (int w = 0; w < kWaves; ++w) t += wc[w][tid];
-    run[tid] = t;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    uint64_t chunks = 0;
-    uint32_t pos = 0;
-    for (int b = 0; b < kB; ++b) {
-      const uint32_t t = run[b];
-      const uint64_t cc = ccs[b];
-      plan->cb[b] = (uint32_t)chunks;
-      plan->pb[b] = pos;
+    if (b < kB) {
+      plan->ub[b] = eu;
+      plan->cb[b] = (uint32_t)min(ec, 1ull << 31);
+      plan->pb[b] = ep;
       plan->cc[b] = (uint32_t)cc;
-      run[b] = pos;
-      if (t > 0 && cc >= (1ull << 31)) chunks = 1ull << 31;
-      chunks += (uint64_t)t * cc;
-      if (chunks > (1ull << 31)) chunks = 1ull << 31;
-      pos += t;
+      cnt[b] = ep;
     }
-    const bool overflow = chunks >= (1ull << 31);
-    plan->cb[kB] = overflow ? 0u : (uint32_t)chunks;
-    ctrl[0] = 0u;
-    ctrl[3] = overflow ? 0xFFFFFFFFu : (uint32_t)chunks;
-    if (overflow) atomicOr(&ctrl[2], kErrBadK);
+    if (b == kB - 1) {
+      const uint64_t chunks = ec + xc;
+      const bool overflow = chunks >= (1ull << 31);
+      plan->cb[kB] = overflow ? 0u : (uint32_t)chunks;
+      plan->ub[kB] = overflow ? 0ull : eu + xu;
+      plan->pb[kB] = ep + t;
+      ctrl[0] = 0u;
+      ctrl[3] = overflow ? 0xFFFFFFFFu : (uint32_t)chunks;
+      if (overflow) atomicOr(&ctrl[2], kErrBadK);
+    }
   }
   __syncthreads();
-  const uint32_t present = __builtin_amdgcn_readfirstlane(present_s);
 
-  // Pass 2: positions.  A batch of at most 4096 codes is still in jb.
-  for (int base = 0; base < n; base += kPlanBatch) {
-    if (n > kPlanBatch) {
+  // Pass 2: positions.  The last batch (the only one when n <= 4096) is
+  // still in registers; earlier ones are loaded again.
+  for (int base = (n > kPlanBatch) ? 0 : -1;;) {
+    if (base >= 0) {
+      if (base >= n) break;
+      present = 0;
       uint32_t unused = 0;
-      md_plan_load_batch(offsets, n, n_idx, base, jb, best, false, err, unused);
-      __syncthreads();
+      md_plan_load_batch(offsets, n, n_idx, base, state, out, tmax_arr, tmax_s, false, unused,
+                         present, pc);
     }
-    for (int r = 0; r < kPlanRounds; ++r) {
-      const int i = base + r * kPlanThreads + tid;
-      const int j = jb[r * kPlanThreads + tid];
-      uint32_t rank = 0, cnt = 0;
-      for (uint32_t rem = present; rem != 0; rem &= rem - 1) {
-        const int b = __builtin_ctz(rem);
-        const uint64_t m = __ballot(j == b);
-        const uint32_t below =
-            __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        rank = (j == b) ? below : rank;
-        cnt = (lane == b) ? (uint32_t)__popcll(m) : cnt;
-      }
-      if (lane < kB) wc[wave][lane] = cnt;
-      __syncthreads();
-      if (j >= 0) {
-        uint32_t off = run[j] + rank;
-        for (int w = 0; w < wave; ++w) off += wc[w][j];
-        order[off] = (uint32_t)i;
-      }
-      __syncthreads();
-      if (tid < kB) {
-        uint32_t t = 0;
-        for (int w = 0; w < kWaves; ++w) t += wc[w][tid];
-        run[tid] += t;
-      }
-      __syncthreads();
+    const int b0 = (base >= 0) ? base : 0;
+    // Lane b reserves this wave's positions in bucket b.
+    uint32_t mine = 0;
+    for (uint32_t rem = present; rem != 0; rem &= rem - 1) {
+      const int b = __builtin_ctz(rem);
+      uint32_t c = 0;
+#pragma unroll
+      for (int r = 0; r < kPlanRounds; ++r) c += (uint32_t)__popcll(__ballot(pc.j[r] == b));
+      mine = (lane == (uint32_t)b) ? c : mine;
     }
-    if (n > kPlanBatch) __syncthreads();   // jb is rewritten by the next batch
+    uint32_t first = 0;
+    if (mine != 0u) first = atomicAdd(&cnt[lane], mine);
+    for (uint32_t rem = present; rem != 0; rem &= rem - 1) {
+      const int b = __builtin_ctz(rem);
+      uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)first, b);
+#pragma unroll
+      for (int r = 0; r < kPlanRounds; ++r) {
+        const uint64_t m = __ballot(pc.j[r] == b);
+        if (pc.j[r] == b) {
+          const uint32_t pos = at + mbcnt64(m);
+          order[pos] = (uint32_t)(b0 + r * kPlanThreads + tid);
+          roff[pos] = (uint32_t)pc.ofs[r];
+        }
+        at += (uint32_t)__popcll(m);
+      }
+    }
+    if (base < 0) break;
+    base += kPlanBatch;
   }
   if (err) atomicOr(&ctrl[2], err);
 }
@@ -262,9 +324,12 @@ __global__ __launch_bounds__(kPlanThreads) void md_prep_kernel(
     const int32_t* __restrict__ idx, int64_t n_idx, const uint32_t* __restrict__ mplanes,
     uint32_t* __restrict__ rows, const int32_t* __restrict__ offsets, int32_t n,
     const MdParams* __restrict__ p, MdPlan* __restrict__ plan, uint32_t* __restrict__ order,
-    uint32_t* __restrict__ claims, uint32_t* __restrict__ best, uint32_t* ctrl) {
+    uint32_t* __restrict__ roff, uint32_t* __restrict__ claims, uint64_t* __restrict__ state,
+    int32_t* __restrict__ out, const int32_t* __restrict__ tmax_arr, int32_t tmax_s,
+    uint32_t* ctrl) {
   if (blockIdx.x == 0) {
-    md_plan_body(offsets, n, n_idx, p, plan, order, claims, best, ctrl);
+    md_plan_body(offsets, n, n_idx, p, plan, order, roff, claims, state, out, tmax_arr, tmax_s,
+                 ctrl);
     return;
   }
   md_rows_code((int)((blockIdx.x - 1) * (kPlanThreads / kWave) + threadIdx.x / kWave), n, n_idx,
@@ -280,9 +345,10 @@ __global__ __launch_bounds__(kPlanThreads) void md_prep_kernel(
 // The plan and the per-k parameters are copied into LDS once per workgroup,
 // so decoding an id (a binary search over 21 bucket starts and a walk over
 // the pieces) reads LDS and scalar registers only; the code index and its
-// row offset are the only global loads before the rows.
+// row offset (two independent loads) are the only global loads before the
+// rows.
 struct MdShared {
-  uint32_t plan[sizeof(MdPlan) / 4];
+  alignas(8) uint32_t plan[sizeof(MdPlan) / 4];
   uint8_t L[kMaxKExact + 1];
   uint8_t check_log2[kMaxKExact + 1];
 };
@@ -306,11 +372,14 @@ struct MdChunk {
   uint32_t cmask;   // early-exit check mask
   int j;            // Gray piece, or tiny_top(w) for the tiny chunk
   uint32_t c;       // chunk index inside the piece
+  uint32_t nch;     // chunks of the code
+  uint32_t roff;    // first row of the code (3 rows per index)
 };
 
 // Wave-uniform decode of chunk id g < total.
 __device__ __forceinline__ MdChunk md_decode(const MdShared& sh, uint32_t g,
-                                             const uint32_t* __restrict__ order) {
+                                             const uint32_t* __restrict__ order,
+                                             const uint32_t* __restrict__ roff) {
   const uint32_t* cb = sh.plan + offsetof(MdPlan, cb) / 4;
   const uint32_t* pb = sh.plan + offsetof(MdPlan, pb) / 4;
   const uint32_t* ccv = sh.plan + offsetof(MdPlan, cc) / 4;
@@ -326,6 +395,7 @@ __device__ __forceinline__ MdChunk md_decode(const MdShared& sh, uint32_t g,
   const uint32_t cc = __builtin_amdgcn_readfirstlane(ccv[lo]);
   const uint32_t nb = (__builtin_amdgcn_readfirstlane(cb[lo + 1]) - b0) / cc;   // codes
   MdChunk r;
+  r.nch = cc;
   r.w = kMaxKExact - lo;
   const int kq = min(r.w + 1, kMaxKExact);
   r.L = __builtin_amdgcn_readfirstlane(sh.L[kq]);
@@ -340,7 +410,9 @@ __device__ __forceinline__ MdChunk md_decode(const MdShared& sh, uint32_t g,
   }
   r.j = j;
   r.c = rel / nb;
-  r.code = (int)order[__builtin_amdgcn_readfirstlane(pb[lo]) + (rel - r.c * nb)];
+  const uint32_t pos = __builtin_amdgcn_readfirstlane(pb[lo]) + (rel - r.c * nb);
+  r.code = (int)order[pos];
+  r.roff = roff[pos];
   return r;
 }
 
@@ -356,42 +428,158 @@ __device__ __forceinline__ uint32_t md_walk(const MdChunk& ck, const uint32_t* R
   return wave_min(b);
 }
 
-// Exact mode: a static deal, no claim atomics.  W waves; in round r wave i
-// walks chunk r W + i (r even) or r W + W - 1 - i (r odd).  Sizes fall along
-// the ids, so each round hands out chunks of about one size and the snake
-// order evens out what the rounds leave.  (Chunk claims from shared
-// counters cost about 90 ns per claim on one counter, serialised at the
-// device coherence point: 66 us of a 4096-code pass.)  A wave keeps the
-// running minimum of the code it walks and folds it (one atomicMin) only when
-// it moves to another code or leaves.
+// Fold a wave's minimum weight b over n finished chunks of a code into its
+// state word (bits 0..31 the minimum, 32..63 the chunks finished), from lane
+// 0.  The minimum is a returning atomic and the count's operand depends on
+// its result, with an s_waitcnt 0 between them, so the count cannot be
+// performed before the minimum; both halves of one word share a coherence
+// point.  The wave whose count completes the code (all its nch chunks) reads
+// the final minimum there and writes the code's result.
+__device__ __forceinline__ void md_fold(uint64_t* state, uint32_t b, uint32_t n, uint32_t nch,
+                                        int tmax, int32_t* out_code) {
+  uint32_t* st = reinterpret_cast<uint32_t*>(state);
+  const uint32_t prev = __hip_atomic_fetch_min(&st[0], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __builtin_amdgcn_s_waitcnt(0);
+  const uint32_t done =
+      __hip_atomic_fetch_add(&st[1], n + (prev >> 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + n;
+  if (done == nch) {
+    const int bw = (int)__hip_atomic_load(&st[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *out_code = md_result(bw, tmax);
+  }
+}
+
+// Unsigned division, 32-bit when both operands fit (wave-uniform operands).
+__device__ __forceinline__ uint64_t udiv64(uint64_t a, uint64_t b) {
+  if (((a | b) >> 32) == 0) return (uint32_t)a / (uint32_t)b;
+  return a / b;
+}
+
+// The static deal of U units over W waves: wave i takes units
+// [start(i), start(i + 1)), q = U / W units each and one more for the first
+// r = U % W waves.
+struct UnitDeal {
+  uint64_t q, r;
+  __device__ __forceinline__ uint64_t start(uint64_t i) const { return i * q + min(i, r); }
+  __device__ __forceinline__ uint64_t wave_of(uint64_t u) const {
+    const uint64_t t = r * (q + 1);
+    return (u < t) ? udiv64(u, q + 1) : r + udiv64(u - t, q);
+  }
+};
+
+// Per-lane minimum weight over units [o0, o1) of a code of walk dimension w
+// (its pieces from the largest down, then the tiny part, which is walked by
+// the run that holds its first unit).
+template <class Rows>
+__device__ __forceinline__ uint32_t walk_units(const Rows& row, int w, uint64_t o0, uint64_t o1,
+                                               uint32_t lane) {
+  uint32_t best = 64u;
+  uint64_t ps = 0;
+  for (int j = w - 1; j > kTinyTop; --j) {
+    const uint64_t nbk = piece_blocks(j);
+    const uint64_t s = max(o0, ps), e = min(o1, ps + nbk);
+    if (s < e) best = min(best, gray_blocks(row, (uint32_t)j, s - ps, e - ps, lane));
+    ps += nbk;
+  }
+  if (o0 <= ps && ps < o1) best = min(best, tiny_chunk(row, tiny_top(w), lane));
+  return best;
+}
+
+// Exact mode: static deals, no claim atomics.  Wave i of W takes
+//  * the i-th of W equal shares (to one code) of the small codes, walk
+//    dimension <= kSmallW, which are the plan's last buckets: each is its
+//    tiny part alone, a few hundred instructions behind a chain of dependent
+//    loads, so they are spread over all waves rather than lined up in a few;
+//  * the i-th of W equal runs (to one unit) of the other codes' work units,
+//    numbered bucket by bucket, largest walk first, each code's units
+//    consecutive.
+// Every wave, and so every SIMD whatever waves land on it, gets the same
+// work to within a block, a small code and the set-up of a few runs.  A code
+// inside the wave's share or run is finished by it: one plain store of its
+// result.  A code split over runs is folded (md_fold) by each wave whose run
+// meets it, counted against the number of such waves.  (Chunk claims from
+// shared counters cost about 90 ns each, serialised at the device coherence
+// point: 66 us of a 4096-code pass; a snake deal of whole chunks left SIMDs
+// uneven by up to a 65536-codeword chunk.)
 __global__ __launch_bounds__(256) void md_main_exact_kernel(
-    const uint32_t* __restrict__ rows, const int32_t* __restrict__ offsets,
-    const MdPlan* __restrict__ plan, const uint32_t* __restrict__ order, uint32_t* best_w,
+    const uint32_t* __restrict__ rows, const MdPlan* __restrict__ plan,
+    const uint32_t* __restrict__ order, const uint32_t* __restrict__ roff, uint64_t* state,
+    const int32_t* __restrict__ tmax_arr, int32_t tmax_s, int32_t* __restrict__ out,
     const MdParams* __restrict__ p) {
+  constexpr int kSmallB = kMaxKExact - kSmallW;   // first bucket of small codes
   __shared__ MdShared sh;
   md_load_shared(sh, plan, p);
   const uint32_t lane = lane_id();
-  const uint32_t total = __builtin_amdgcn_readfirstlane(sh.plan[offsetof(MdPlan, cb) / 4 + kMaxKExact]);
-  const uint32_t W = gridDim.x * (blockDim.x / kWave);
-  const uint32_t wid =
+  const uint64_t* ub = reinterpret_cast<const uint64_t*>(sh.plan + offsetof(MdPlan, ub) / 4);
+  const uint32_t* pb = sh.plan + offsetof(MdPlan, pb) / 4;
+  auto pbu = [&](int b) { return (uint32_t)__builtin_amdgcn_readfirstlane(pb[b]); };
+  const uint64_t W = (uint64_t)gridDim.x * (blockDim.x / kWave);
+  const uint64_t wid =
       __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
-  int pcode = -1;
-  uint32_t pmin = 64u;
-  for (uint32_t r = 0;; ++r) {
-    const uint64_t g64 = (uint64_t)r * W + ((r & 1u) ? (W - 1u - wid) : wid);
-    if (g64 >= total) break;
-    const MdChunk ck = md_decode(sh, (uint32_t)g64, order);
-    const uint32_t* R = rows + (size_t)offsets[ck.code] * 3 * kRowWords;
-    const uint32_t b = md_walk<false>(ck, R, lane, -1, nullptr);
-    if (ck.code != pcode) {
-      if (pcode >= 0 && lane == 0) atomicMin(&best_w[pcode], pmin);
-      pcode = ck.code;
-      pmin = b;
-    } else {
-      pmin = min(pmin, b);
+
+  // Small codes, whole.
+  {
+    const uint32_t P0 = pbu(kSmallB);
+    const uint64_t T = pbu(kMaxKExact) - P0;
+    const uint64_t q = udiv64(T, W);
+    const UnitDeal d{q, T - q * W};
+    const uint32_t pe = P0 + (uint32_t)d.start(wid + 1);
+    int lo = kSmallB;
+    for (uint32_t pos = P0 + (uint32_t)d.start(wid); pos < pe; ++pos) {
+      while (pos >= pbu(lo + 1)) ++lo;
+      const int code = (int)order[pos];
+      const uint32_t* R = rows + (size_t)roff[pos] * 3 * kRowWords;
+      const uint32_t b = wave_min(tiny_chunk(ConstRows{R}, tiny_top(kMaxKExact - lo), lane));
+      if (lane == 0) out[code] = md_result((int)b, tmax_arr ? tmax_arr[code] : tmax_s);
     }
   }
-  if (pcode >= 0 && lane == 0) atomicMin(&best_w[pcode], pmin);
+
+  // Units of the other codes.
+  const uint64_t U = uniform64(ub[kSmallB]);
+  const uint64_t dq = udiv64(U, W);
+  const UnitDeal deal{dq, U - dq * W};
+  const uint64_t u0 = deal.start(wid), u1 = deal.start(wid + 1);
+  if (u0 >= u1) return;
+  // Last bucket whose first unit is <= u0 (empty buckets tie with their
+  // successor, so the last tie is the nonempty one).
+  int lo = 0, hi = kSmallB - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ub[mid] <= u0) lo = mid; else hi = mid - 1;
+  }
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  uint64_t cu = code_units(kMaxKExact - lo);
+  const uint64_t rel = u0 - uniform64(ub[lo]);
+  uint32_t qc = (uint32_t)udiv64(rel, cu);
+  uint64_t off = rel - (uint64_t)qc * cu;
+  uint64_t u = u0;
+  for (;;) {
+    const int w = kMaxKExact - lo;
+    const uint32_t pos = pbu(lo) + qc;
+    const int code = (int)order[pos];
+    const uint32_t* R = rows + (size_t)roff[pos] * 3 * kRowWords;
+    const uint64_t a = u - off;           // first unit of the code
+    const uint64_t e = min(a + cu, u1);   // end of this wave's part of it
+    const uint32_t b = wave_min(walk_units(ConstRows{R}, w, off, e - a, lane));
+    const int tmax = tmax_arr ? tmax_arr[code] : tmax_s;
+    if (a >= u0 && a + cu <= u1) {
+      if (lane == 0) out[code] = md_result((int)b, tmax);
+    } else {
+      const uint64_t first = (a < u0) ? deal.wave_of(a) : wid;
+      const uint64_t last = (a + cu > u1) ? deal.wave_of(a + cu - 1) : wid;
+      if (lane == 0) md_fold(&state[code], b, 1u, (uint32_t)(last - first + 1), tmax, &out[code]);
+    }
+    u = e;
+    if (u >= u1) break;
+    off = 0;
+    if (++qc == pbu(lo + 1) - pbu(lo)) {
+      // Next nonempty bucket (one exists: units remain below U).
+      do {
+        ++lo;
+      } while (pbu(lo + 1) == pbu(lo));
+      qc = 0;
+      cu = code_units(kMaxKExact - lo);
+    }
+  }
 }
 
 // Thresholded mode: chunks of codes that already reached their threshold
@@ -410,10 +598,9 @@ __global__ __launch_bounds__(256) void md_main_exact_kernel(
 // threaded the two regions together and the other 63 lanes re-read a stale
 // id with lane 0 inactive, walking chunk 0 forever.)
 __global__ __launch_bounds__(256) void md_main_thr_kernel(
-    const uint32_t* __restrict__ rows, const int32_t* __restrict__ offsets,
-    const MdPlan* __restrict__ plan, const uint32_t* __restrict__ order,
-    uint32_t* claims, uint32_t* best_w, const int32_t* __restrict__ tmax_arr, int32_t tmax_s,
-    const MdParams* __restrict__ p) {
+    const uint32_t* __restrict__ rows, const MdPlan* __restrict__ plan,
+    const uint32_t* __restrict__ order, const uint32_t* __restrict__ roff, uint32_t* claims, uint64_t* state, const int32_t* __restrict__ tmax_arr, int32_t tmax_s,
+    int32_t* __restrict__ out, const MdParams* __restrict__ p) {
   __shared__ MdShared sh;
   md_load_shared(sh, plan, p);
   const uint32_t lane = lane_id();
@@ -425,20 +612,21 @@ __global__ __launch_bounds__(256) void md_main_thr_kernel(
   uint32_t g = __builtin_amdgcn_readfirstlane(first) * kClaimShards + shard;
   for (;;) {
     const bool have = g < total;
-    int code = 0;
+    int code = 0, tmax = 0;
     bool fold = false;
-    uint32_t b = 64u;
+    uint32_t b = 64u, nch = 0;
     if (have) {
-      const MdChunk ck = md_decode(sh, g, order);
+      const MdChunk ck = md_decode(sh, g, order, roff);
       code = ck.code;
-      const uint32_t* R = rows + (size_t)offsets[code] * 3 * kRowWords;
-      const int thr = 48 - (tmax_arr ? tmax_arr[code] : tmax_s);
+      nch = ck.nch;
+      fold = true;   // walked or skipped, the chunk is counted
+      const uint32_t* R = rows + (size_t)ck.roff * 3 * kRowWords;
+      tmax = tmax_arr ? tmax_arr[code] : tmax_s;
+      const int thr = 48 - tmax;
+      const uint32_t* best_g = reinterpret_cast<const uint32_t*>(&state[code]);
       const uint32_t bw = __builtin_amdgcn_readfirstlane(
-          __hip_atomic_load(&best_w[code], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      if ((int)bw > thr) {
-        b = md_walk<true>(ck, R, lane, thr, &best_w[code]);
-        fold = true;
-      }
+          __hip_atomic_load(best_g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if ((int)bw > thr) b = md_walk<true>(ck, R, lane, thr, best_g);
     } else {
       dry |= 1u << shard;
       uint32_t c = 0xFFFFFFFFu;
@@ -456,27 +644,11 @@ __global__ __launch_bounds__(256) void md_main_thr_kernel(
     }
     uint32_t next = 0;
     if (lane == 0) {
-      if (fold) atomicMin(&best_w[code], b);
+      if (fold) md_fold(&state[code], b, 1u, nch, tmax, &out[code]);
       next = atomicAdd(&claims[shard * kClaimStride], 1u);
     }
     g = __builtin_amdgcn_readfirstlane(next) * kClaimShards + shard;
   }
-}
-
-__global__ __launch_bounds__(256) void md_finalize_kernel(const int32_t* __restrict__ offsets,
-                                                          int32_t n,
-                                                          const uint32_t* __restrict__ best,
-                                                          const int32_t* __restrict__ tmax_arr,
-                                                          int32_t tmax_s,
-                                                          int32_t* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int k = offsets[i + 1] - offsets[i];
-  if (k <= 0 || k > kMaxKExact) { out[i] = 0; return; }
-  const int tmax = tmax_arr ? tmax_arr[i] : tmax_s;
-  // k = 1: a monomial has weight 49 (no walk was run).
-  const int b = (k == 1) ? 49 : (int)best[i];
-  out[i] = (b <= 48 - tmax) ? tmax + 1 : 49 - b;
 }
 
 template <bool THR>
@@ -629,13 +801,13 @@ hipError_t launch_md_rows(const int32_t* d_idx, int64_t n_idx, const uint32_t* d
 hipError_t launch_md_prep(const int32_t* d_idx, int64_t n_idx, const uint32_t* d_mplanes,
                           uint32_t* d_rows, const int32_t* d_offsets, int32_t n,
                           const MdParams* d_params, MdPlan* d_plan, uint32_t* d_order,
-                          uint32_t* d_claims, uint32_t* d_best, uint32_t* d_ctrl,
-                          hipStream_t s) {
+                          uint32_t* d_roff, uint32_t* d_claims, uint64_t* d_state, int32_t* d_out,
+                          const int32_t* d_tmax, int32_t tmax, uint32_t* d_ctrl, hipStream_t s) {
   constexpr int kCodesPerBlock = kPlanThreads / kWave;   // one wave per code
   const int64_t row_blocks = ((int64_t)std::max(n, 0) + kCodesPerBlock - 1) / kCodesPerBlock;
   hipLaunchKernelGGL(md_prep_kernel, dim3((unsigned)(1 + row_blocks)), dim3(kPlanThreads), 0, s,
                      d_idx, n_idx, d_mplanes, d_rows, d_offsets, n, d_params, d_plan, d_order,
-                     d_claims, d_best, d_ctrl);
+                     d_roff, d_claims, d_state, d_out, d_tmax, tmax, d_ctrl);
   return hipGetLastError();
 }
 
@@ -649,24 +821,15 @@ int md_main_blocks_per_cu(bool thresholded) {
 }
 
 hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
-                          const int32_t* d_offsets, const MdPlan* d_plan,
-                          const uint32_t* d_order, uint32_t* d_claims, uint32_t* d_best,
-                          const int32_t* d_tmax, int32_t tmax, const MdParams* d_params,
-                          hipStream_t s) {
+                          const MdPlan* d_plan, const uint32_t* d_order, const uint32_t* d_roff,
+                          uint32_t* d_claims, uint64_t* d_state, const int32_t* d_tmax,
+                          int32_t tmax, int32_t* d_out, const MdParams* d_params, hipStream_t s) {
   if (thresholded)
-    hipLaunchKernelGGL(md_main_thr_kernel, dim3(grid), dim3(256), 0, s, d_rows, d_offsets,
-                       d_plan, d_order, d_claims, d_best, d_tmax, tmax, d_params);
+    hipLaunchKernelGGL(md_main_thr_kernel, dim3(grid), dim3(256), 0, s, d_rows, d_plan, d_order,
+                       d_roff, d_claims, d_state, d_tmax, tmax, d_out, d_params);
   else
-    hipLaunchKernelGGL(md_main_exact_kernel, dim3(grid), dim3(256), 0, s, d_rows, d_offsets,
-                       d_plan, d_order, d_best, d_params);
-  return hipGetLastError();
-}
-
-hipError_t launch_md_finalize(const int32_t* d_offsets, int32_t n, const uint32_t* d_best,
-                              const int32_t* d_tmax, int32_t tmax, int32_t* d_out,
-                              hipStream_t s) {
-  hipLaunchKernelGGL(md_finalize_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_offsets, n,
-                     d_best, d_tmax, tmax, d_out);
+    hipLaunchKernelGGL(md_main_exact_kernel, dim3(grid), dim3(256), 0, s, d_rows, d_plan,
+                       d_order, d_roff, d_state, d_tmax, tmax, d_out, d_params);
   return hipGetLastError();
 }
 

@@ -170,9 +170,10 @@ struct LdsRows {
   }
   // Rows read from LDS are already in VGPRs.
   __device__ __forceinline__ RowV vgpr(const RowV& v) const { return v; }
+  __device__ __forceinline__ RowV vgpr_after(const RowV& v, uint32_t) const { return v; }
 };
 
-// Claims as in md_main_kernel: one lane-0 region per iteration (it completes
+// Claims as in md_main_thr_kernel: one lane-0 region per iteration (it completes
 // the previous chunk and claims the next id), read back with readfirstlane
 // while every lane is active; all other conditions are made wave-uniform.
 //
