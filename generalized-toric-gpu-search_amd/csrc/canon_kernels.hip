@@ -35,9 +35,9 @@
 // (read from lane r1 in packed form), and translates it by every remaining
 // anchor; those are the only (A, t) pairs whose image can be minimal, so both
 // the minimum and the count of pairs that attain it (|Stab|) are exact.
-// Ballots combine the lanes and the key is packed in scalar registers.  Sets
-// are bucketed by size, largest first, and walked by the waves with a fixed
-// stride.
+// Ballots combine the lanes and the key is packed in scalar registers.  The
+// waves take the sets in input order, dealt so that each XCD works on
+// contiguous runs (canon_kernel).
 #include <algorithm>
 
 #include "gt_launch.h"
@@ -344,122 +344,45 @@ __device__ __forceinline__ CanonOut canon_set(const CanonTables& T, uint64_t mas
   return o;
 }
 
-// Counting sort of set indices by |S| (1..16); bad sizes are flagged.  Each
-// workgroup counts its grid-stride share in LDS and adds one global atomic per
-// size: a global atomic per set would queue a million atomics on 17 addresses
-// (about 3 ms per pass for 10^6 sets).
-constexpr int kSortBlock = 256;
-constexpr int kSortGrid = 512;
+constexpr int kXcds = 8;
 
-__global__ __launch_bounds__(kSortBlock) void canon_hist_kernel(const uint64_t* __restrict__ masks,
-                                                                int32_t n, uint32_t* hist,
-                                                                uint32_t* ctrl) {
-  __shared__ uint32_t lh[kMaxKCanon + 1];
-  if (threadIdx.x <= kMaxKCanon) lh[threadIdx.x] = 0u;
-  __syncthreads();
-  bool bad = false;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint64_t m = masks[i];
-    const int k = __popcll(m);
-    if (k < 1 || k > kMaxKCanon || (m >> 49) != 0) { bad = true; continue; }
-    atomicAdd(&lh[k], 1u);
-  }
-  if (bad) atomicOr(&ctrl[2], kErrBadK);
-  __syncthreads();
-  if (threadIdx.x <= kMaxKCanon && lh[threadIdx.x] != 0u) atomicAdd(&hist[threadIdx.x], lh[threadIdx.x]);
-}
-
-__global__ void canon_scan_kernel(uint32_t* hist) {
-  // hist[0..16] counts -> hist[17..33] running offsets (single thread; 17 bins).
-  // Largest sets first: their waves run longest, so starting them first keeps
-  // the end of the launch to short waves.
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    uint32_t run = 0;
-    for (int k = kMaxKCanon; k >= 0; --k) {
-      hist[17 + k] = run;
-      // Round each bucket up to whole waves so no wave mixes two sizes.
-      run += (hist[k] + kWave - 1) / kWave * kWave;
-    }
-    hist[34] = run;
-  }
-}
-
-// Same grid and the same grid-stride share as canon_hist_kernel: each
-// workgroup counts again, reserves one range per size with one global atomic,
-// then places its sets inside its ranges (LDS atomics; the order inside a
-// bucket does not change any result).
-__global__ __launch_bounds__(kSortBlock) void canon_scatter_kernel(
-    const uint64_t* __restrict__ masks, int32_t n, uint32_t* hist, uint32_t* order) {
-  __shared__ uint32_t lh[kMaxKCanon + 1];
-  __shared__ uint32_t base[kMaxKCanon + 1];
-  if (threadIdx.x <= kMaxKCanon) lh[threadIdx.x] = 0u;
-  __syncthreads();
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint64_t m = masks[i];
-    const int k = __popcll(m);
-    if (k < 1 || k > kMaxKCanon || (m >> 49) != 0) continue;
-    atomicAdd(&lh[k], 1u);
-  }
-  __syncthreads();
-  if (threadIdx.x <= kMaxKCanon) {
-    const uint32_t c = lh[threadIdx.x];
-    base[threadIdx.x] = (c != 0u) ? atomicAdd(&hist[17 + threadIdx.x], c) : 0u;
-    lh[threadIdx.x] = 0u;
-  }
-  __syncthreads();
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint64_t m = masks[i];
-    const int k = __popcll(m);
-    if (k < 1 || k > kMaxKCanon || (m >> 49) != 0) continue;
-    order[base[k] + atomicAdd(&lh[k], 1u)] = (uint32_t)i;
-  }
-}
-
-__global__ void canon_fill_kernel(uint32_t* order, uint32_t len, uint32_t* hist) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < len) order[i] = 0xFFFFFFFFu;
-  if (i < 35) hist[i] = 0u;
-}
-
-constexpr uint32_t kNoSet = 0xFFFFFFFFu;
-
-// kSorted: wave j takes set order[j] (sets grouped by size, largest first;
-// kNoSet pads each group to whole waves; canon_hist_kernel flagged bad
-// masks).  Otherwise wave j takes set j; the callers of that form pass masks
-// that are valid by construction or 0 (already flagged), and a mask whose
-// size is outside 1..16 is skipped.
-template <bool kSorted>
+// One wavefront per set, in input order.  Workgroups are dealt round-robin
+// over the XCDs (workgroup b on XCD b % 8), so a pass over nwaves sets gives
+// XCD x one contiguous run of them: its workgroups' waves, in order.  Its
+// masks are then read, and its four outputs written, in whole cache lines of
+// its own L2; with sets dealt wave by wave, neighbouring sets sat on other
+// XCDs and every line was written back in pieces from several L2s (six times
+// the algorithmic write traffic).  Each wave fetches the mask of its next set
+// before working on the current one (a vector load, counted on vmcnt, which
+// the LDS waits of the current set do not wait for).  A mask outside 1..16
+// points, or with bits above 48, is flagged and its outputs are not written.
 __global__ __launch_bounds__(kCanonBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void canon_kernel(
-    const uint64_t* __restrict__ masks, const uint32_t* __restrict__ order,
-    uint32_t slots, uint64_t* __restrict__ key_lo, uint64_t* __restrict__ key_hi,
-    uint64_t* __restrict__ canon, int32_t* __restrict__ stab) {
+    const uint64_t* __restrict__ masks, uint32_t n, uint64_t* __restrict__ key_lo,
+    uint64_t* __restrict__ key_hi, uint64_t* __restrict__ canon, int32_t* __restrict__ stab,
+    uint32_t* ctrl) {
   __shared__ CanonTables T;
   build_canon_tables(T);
   __syncthreads();
   const uint32_t lane = __lane_id();
   const uint64_t par_mine = T.par[lane < (uint32_t)kRows ? lane : 0u];
-  const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
-  // Each wave's order entries and masks are fetched ahead: the mask of the
-  // next set and the order entry of the one after it are issued before the
-  // current set is worked on.  The wave index j is not known to be uniform
-  // here, so these are vector loads, counted on vmcnt, and the LDS waits of
-  // the current set do not wait for them, as they would for scalar loads
-  // (scalar loads and LDS share lgkmcnt).  Every address is in bounds: order
-  // positions are checked against slots and a padding entry reads mask 0.
-  uint32_t j = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-  uint32_t idx_v = (j < slots) ? (kSorted ? order[j] : j) : kNoSet;
-  uint64_t mv = masks[(idx_v == kNoSet) ? 0u : idx_v];
-  uint32_t next_v = (j + nwaves < slots) ? (kSorted ? order[j + nwaves] : j + nwaves) : kNoSet;
-  for (; j < slots; j += nwaves) {
-    const uint64_t mv_next = masks[(next_v == kNoSet) ? 0u : next_v];
-    const uint32_t after_v = (j + 2u * nwaves < slots)
-                                 ? (kSorted ? order[j + 2u * nwaves] : j + 2u * nwaves)
-                                 : kNoSet;
-    const uint32_t idx = __builtin_amdgcn_readfirstlane(idx_v);
-    if (idx != kNoSet) {
-      const uint64_t mask = uniform64(mv);
-      CanonOut o;
-      bool ok = true;
+  constexpr uint32_t kWavesPerBlock = kCanonBlock / kWave;
+  const uint32_t G = gridDim.x, nwaves = G * kWavesPerBlock;
+  // This wave's place in a pass: the waves of XCDs below its own, then its
+  // workgroup's rank on its XCD.
+  const uint32_t b = blockIdx.x, x = b % kXcds;
+  uint32_t slot = 0;
+  for (uint32_t y = 0; y < x; ++y) slot += (G - y + kXcds - 1) / kXcds;
+  slot = (slot + b / kXcds) * kWavesPerBlock + threadIdx.x / kWave;
+  uint32_t j = slot;
+  uint64_t mv = (j < n) ? masks[j] : 0ull;
+  bool bad = false;
+  for (; j < n; j += nwaves) {
+    const uint32_t jn = j + nwaves;
+    const uint64_t mv_next = (jn < n) ? masks[jn] : 0ull;
+    const uint64_t mask = uniform64(mv);
+    CanonOut o;
+    bool ok = (mask >> kNpts) == 0;
+    if (ok) {
       switch (__popcll(mask)) {
 #define GT_CANON_CASE(KK) \
   case KK: o = canon_set<KK>(T, mask, lane, par_mine); break;
@@ -470,17 +393,17 @@ __global__ __launch_bounds__(kCanonBlock) __attribute__((amdgpu_waves_per_eu(8, 
 #undef GT_CANON_CASE
         default: ok = false; break;   // sizes outside 1..16
       }
-      if (ok && lane == 0) {
-        key_lo[idx] = o.key_lo;
-        key_hi[idx] = o.key_hi;
-        canon[idx] = o.canon;
-        stab[idx] = (int32_t)o.stab;
-      }
     }
-    idx_v = next_v;
+    bad |= !ok;
+    if (ok && lane == 0) {
+      key_lo[j] = o.key_lo;
+      key_hi[j] = o.key_hi;
+      canon[j] = o.canon;
+      stab[j] = (int32_t)o.stab;
+    }
     mv = mv_next;
-    next_v = after_v;
   }
+  if (bad && lane == 0) atomicOr(&ctrl[2], kErrBadK);
 }
 
 // (n, k) sorted index rows -> masks; a repeated or out-of-range index is flagged.
@@ -506,33 +429,15 @@ hipError_t launch_sets_to_masks(const int32_t* d_sets, int32_t n, int32_t k, uin
   return hipGetLastError();
 }
 
-// d_order must hold n + 16 * 64 entries; d_hist 35 entries.
-hipError_t launch_canon(const uint64_t* d_masks, int32_t n, bool by_size,
-                        uint32_t* d_order, uint32_t* d_hist, uint64_t* d_key_lo,
-                        uint64_t* d_key_hi, uint64_t* d_canon, int32_t* d_stab,
-                        uint32_t* d_ctrl, hipStream_t s) {
+hipError_t launch_canon(const uint64_t* d_masks, int32_t n, uint64_t* d_key_lo,
+                        uint64_t* d_key_hi, uint64_t* d_canon, int32_t* d_stab, uint32_t* d_ctrl,
+                        hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  if (!by_size) {
-    const int grid = (int)std::min<uint32_t>(
-        kCanonMaxBlocks, ((uint32_t)n + kCanonBlock / kWave - 1) / (kCanonBlock / kWave));
-    hipLaunchKernelGGL(canon_kernel<false>, dim3(grid), dim3(kCanonBlock), 0, s, d_masks,
-                       nullptr, (uint32_t)n, d_key_lo, d_key_hi, d_canon, d_stab);
-    return hipGetLastError();
-  }
-  const uint32_t slots = (uint32_t)n + (uint32_t)(kMaxKCanon + 1) * kWave;
-  hipLaunchKernelGGL(canon_fill_kernel, dim3((slots + 255) / 256), dim3(256), 0, s, d_order,
-                     slots, d_hist);
-  const int sort_grid = std::min(kSortGrid, (n + kSortBlock - 1) / kSortBlock);
-  hipLaunchKernelGGL(canon_hist_kernel, dim3(sort_grid), dim3(kSortBlock), 0, s, d_masks, n,
-                     d_hist, d_ctrl);
-  hipLaunchKernelGGL(canon_scan_kernel, dim3(1), dim3(64), 0, s, d_hist);
-  hipLaunchKernelGGL(canon_scatter_kernel, dim3(sort_grid), dim3(kSortBlock), 0, s, d_masks, n,
-                     d_hist, d_order);
-  const uint32_t waves = (uint32_t)n;
+  constexpr uint32_t kWavesPerBlock = kCanonBlock / kWave;
   const int grid = (int)std::min<uint32_t>(kCanonMaxBlocks,
-                                           (waves + kCanonBlock / kWave - 1) / (kCanonBlock / kWave));
-  hipLaunchKernelGGL(canon_kernel<true>, dim3(grid), dim3(kCanonBlock), 0, s, d_masks, d_order,
-                     slots, d_key_lo, d_key_hi, d_canon, d_stab);
+                                           ((uint32_t)n + kWavesPerBlock - 1) / kWavesPerBlock);
+  hipLaunchKernelGGL(canon_kernel, dim3(grid), dim3(kCanonBlock), 0, s, d_masks, (uint32_t)n,
+                     d_key_lo, d_key_hi, d_canon, d_stab, d_ctrl);
   return hipGetLastError();
 }
 

@@ -204,7 +204,7 @@ struct gt_ctx {
   uint32_t* ctrl_p = nullptr;
   DevBuf idx, offs, tmax, rows, plan, best, out, zmax, stopf;  // min distance
   DevBuf sbuf;                                                // single code
-  DevBuf masks, order, hist, klo, khi, canon, stab;            // canonical
+  DevBuf masks, klo, khi, canon, stab;                        // canonical
   DevBuf children, bflag, bsum, bidx, bcnt, bmasks, brows, boffs;  // BFS level
 
   bool prof = false;
@@ -218,7 +218,7 @@ struct gt_ctx {
 
   std::vector<DevBuf*> all_bufs() {
     return {&mplanes, &params_buf, &ctrl, &idx, &offs, &tmax, &rows, &plan, &best,
-            &out, &zmax, &stopf, &sbuf, &masks, &order, &hist, &klo, &khi, &canon, &stab,
+            &out, &zmax, &stopf, &sbuf, &masks, &klo, &khi, &canon, &stab,
             &children, &bflag, &bsum, &bidx, &bcnt, &bmasks, &brows, &boffs};
   }
 
@@ -610,22 +610,14 @@ int gt_md_single(gt_ctx* c, const int32_t* idx, int32_t k, int32_t tmax, int32_t
   return 0;
 }
 
-// one_size: masks of one size built on the device (valid by construction,
-// or 0 after a flagged input error) go to the kernel in input order.  Caller
-// masks of any sizes are grouped by size first, which also validates them.
+// Canonical forms of n device masks in input order (the kernel validates
+// them: a bad mask is flagged and its outputs left unwritten).
 static int canon_enqueue(gt_ctx* c, const uint64_t* d_masks, int32_t n, uint64_t* d_lo,
-                         uint64_t* d_hi, uint64_t* d_canon, int32_t* d_stab, hipStream_t s,
-                         bool one_size) {
+                         uint64_t* d_hi, uint64_t* d_canon, int32_t* d_stab, hipStream_t s) {
   if (n <= 0) return 0;
-  const bool by_size = !one_size;
-  if (by_size) {
-    GT_OK(c->order.ensure(((size_t)n + (kMaxKCanon + 1) * kWave) * 4));
-    GT_OK(c->hist.ensure(64 * 4));
-  }
   hipEvent_t e0 = nullptr;
   GT_OK(c->prof_begin("canon", s, &e0));
-  GT_HIP(launch_canon(d_masks, n, by_size, c->order.as<uint32_t>(),
-                      c->hist.as<uint32_t>(), d_lo, d_hi, d_canon, d_stab, c->ctrl_p, s));
+  GT_HIP(launch_canon(d_masks, n, d_lo, d_hi, d_canon, d_stab, c->ctrl_p, s));
   GT_OK(c->prof_end("canon", s, e0));
   return 0;
 }
@@ -651,7 +643,7 @@ static int canon_host(gt_ctx* c, const uint64_t* masks, const int32_t* sets, int
     GT_HIP(hipMemcpyAsync(c->masks.p, masks, (size_t)n * 8, hipMemcpyHostToDevice, s));
   }
   GT_OK(canon_enqueue(c, c->masks.as<uint64_t>(), n, c->klo.as<uint64_t>(), c->khi.as<uint64_t>(),
-                      c->canon.as<uint64_t>(), c->stab.as<int32_t>(), s, sets != nullptr));
+                      c->canon.as<uint64_t>(), c->stab.as<int32_t>(), s));
   if (key_lo) GT_HIP(hipMemcpyAsync(key_lo, c->klo.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
   if (key_hi) GT_HIP(hipMemcpyAsync(key_hi, c->khi.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
   if (canon_mask) GT_HIP(hipMemcpyAsync(canon_mask, c->canon.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
@@ -689,8 +681,7 @@ int gt_canon_masks_dev(gt_ctx* c, const uint64_t* d_masks, int32_t n, uint64_t* 
   GT_OK(c->activate());
   StreamUse su(c, c->pick(stream));
   GT_OK(su.rc);
-  return canon_enqueue(c, d_masks, n, d_key_lo, d_key_hi, d_canon_mask, d_stab, c->pick(stream),
-                       false);
+  return canon_enqueue(c, d_masks, n, d_key_lo, d_key_hi, d_canon_mask, d_stab, c->pick(stream));
 }
 
 struct gt_graph {
@@ -924,8 +915,7 @@ int gt_bfs_level_ex(gt_ctx* c, gt_hashset* seen, const uint64_t* d_par, int32_t 
       GT_OK(c->canon.ensure(n * 8));
       GT_OK(c->stab.ensure(n * 4));
       GT_OK(canon_enqueue(c, c->children.as<uint64_t>(), (int32_t)n, c->klo.as<uint64_t>(),
-                          c->khi.as<uint64_t>(), c->canon.as<uint64_t>(), c->stab.as<int32_t>(), s,
-                          true));
+                          c->khi.as<uint64_t>(), c->canon.as<uint64_t>(), c->stab.as<int32_t>(), s));
       keys = c->canon.as<uint64_t>();
     }
     GT_OK(dedup_enqueue(seen, keys, (int32_t)n, c->bflag.as<uint32_t>(), s));
@@ -1018,7 +1008,7 @@ struct gt_evolve {
     uint32_t* ctrl = c->ctrl.as<uint32_t>();
     GT_HIP(launch_evo_propose(pop_m, pop_d, pop_age, pop, n_prop, seed, state, props, s));
     const uint64_t* pkey = canonical ? pcm : props;     // class of each proposal
-    if (canonical) GT_OK(canon_enqueue(c, props, n_prop, plo, phi, pcm, pstab, s, true));
+    if (canonical) GT_OK(canon_enqueue(c, props, n_prop, plo, phi, pcm, pstab, s));
     GT_HIP(launch_evo_dedup(pkey, n_prop, skeys, scap, keep, count, s));
     if (ranker) {
       GT_HIP(launch_mlp_features(pkey, n_prop, feat, s));
@@ -1027,7 +1017,7 @@ struct gt_evolve {
     GT_HIP(launch_evo_select(props, keep, count, ranker ? score : nullptr, pop, n_prop, k, seed,
                              state, chosen, s));
     const uint64_t* ckey = canonical ? ccm : chosen;
-    if (canonical) GT_OK(canon_enqueue(c, chosen, pop, clo, chi, ccm, cstab, s, true));
+    if (canonical) GT_OK(canon_enqueue(c, chosen, pop, clo, chi, ccm, cstab, s));
     GT_HIP(launch_hs_insert(ckey, pop, skeys, stags, scap, 0, scount, ctrl, s));
     GT_HIP(launch_masks_to_rows(chosen, pop, k, idx, offs, ctrl, s));
     GT_OK(c->md_enqueue(idx, (int64_t)pop * k, offs, pop, nullptr, 48, false, sample_count, seed,
@@ -1150,8 +1140,8 @@ int gt_evolve_create2(gt_ctx* c, int32_t k, int32_t pop, int32_t n_prop, uint64_
     GT_HIP(launch_hs_clear(e->skeys, e->stags, e->scap, e->scount, s));
     const uint64_t* ckey = e->canonical ? e->ccm : e->chosen;
     if (e->canonical) {
-      GT_OK(canon_enqueue(c, e->props, n_prop, e->plo, e->phi, e->pcm, e->pstab, s, true));
-      GT_OK(canon_enqueue(c, e->chosen, pop, e->clo, e->chi, e->ccm, e->cstab, s, true));
+      GT_OK(canon_enqueue(c, e->props, n_prop, e->plo, e->phi, e->pcm, e->pstab, s));
+      GT_OK(canon_enqueue(c, e->chosen, pop, e->clo, e->chi, e->ccm, e->cstab, s));
     }
     GT_HIP(launch_hs_insert(ckey, pop, e->skeys, e->stags, e->scap, 0, e->scount, ctrl, s));
     GT_HIP(launch_masks_to_rows(e->chosen, pop, k, e->idx, e->offs, ctrl, s));

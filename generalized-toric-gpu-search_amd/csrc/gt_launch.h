@@ -49,13 +49,11 @@ hipError_t launch_md_sampled(const uint32_t* d_rows, const int32_t* d_offsets, i
 int md_main_blocks_per_cu(bool thresholded);
 
 // ---- canonical form ------------------------------------------------------
-// by_size: group the sets by size first and flag bad masks (four small
-// kernels; needed for caller-supplied masks), else one wave per set in input
-// order for masks that are valid by construction (or 0, skipped).
-hipError_t launch_canon(const uint64_t* d_masks, int32_t n, bool by_size,
-                        uint32_t* d_order, uint32_t* d_hist, uint64_t* d_key_lo,
-                        uint64_t* d_key_hi, uint64_t* d_canon, int32_t* d_stab,
-                        uint32_t* d_ctrl, hipStream_t s);
+// One wave per set in input order; a mask with bits above 48 or a size
+// outside 1..16 is flagged (kErrBadK) and its outputs are left unwritten.
+hipError_t launch_canon(const uint64_t* d_masks, int32_t n, uint64_t* d_key_lo,
+                        uint64_t* d_key_hi, uint64_t* d_canon, int32_t* d_stab, uint32_t* d_ctrl,
+                        hipStream_t s);
 // ---- memory ---------------------------------------------------------------
 hipError_t launch_fill_u32(uint32_t* d, int64_t n, uint32_t value, hipStream_t s);
 hipError_t launch_copy16(void* dst, const void* src, int64_t bytes, hipStream_t s);
