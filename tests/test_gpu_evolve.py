@@ -27,11 +27,14 @@ def _compare(ev, model, gens, scorer=None):
         assert best[0] == mbest and new[0] == mnew, g
 
 
-@pytest.mark.parametrize("k,pop,n_prop,gens", [(6, 64, 64, 5), (4, 16, 16, 4), (8, 32, 96, 3), (5, 4096, 8192, 2), (2, 8, 16, 3)])
+@pytest.mark.parametrize("k,pop,n_prop,gens", [(6, 64, 64, 5), (4, 16, 16, 4), (8, 32, 96, 3), (5, 4096, 8192, 2), (2, 8, 16, 3),
+                                               (9, 16, 32, 2)])
 def test_device_generations_equal_the_model(k, pop, n_prop, gens):
     """k = 2, 4 and 5 have 1, 8 and 32 classes, so their generations are
     filled by random sets; (5, 4096, 8192) is the largest population and
-    proposal count; k = 2 is the smallest set a 2-point swap needs."""
+    proposal count; k = 2 is the smallest set a 2-point swap needs; k = 9 is
+    the config-5 search at a population the oracle's exact distances (8^9
+    coefficient vectors a set) finish in seconds."""
     import evolve_model as M
     from gtoric import _native
     from gtoric.evolve import Evolve, random_masks
