@@ -1010,9 +1010,9 @@ struct gt_evolve {
     const uint64_t* pkey = canonical ? pcm : props;     // class of each proposal
     if (canonical) GT_OK(canon_enqueue(c, props, n_prop, plo, phi, pcm, pstab, s));
     GT_HIP(launch_evo_dedup(pkey, n_prop, skeys, scap, keep, count, s));
-    if (ranker) {
-      GT_HIP(launch_mlp_features(pkey, n_prop, feat, s));
-      GT_HIP(launch_mlp_predict(mb, feat, n_prop, score, s));
+    if (ranker) {   // scores are read only when more than pop new proposals remain
+      GT_HIP(launch_mlp_features(pkey, n_prop, feat, s, count, (uint32_t)pop));
+      GT_HIP(launch_mlp_predict(mb, feat, n_prop, score, s, count, (uint32_t)pop));
     }
     GT_HIP(launch_evo_select(props, keep, count, ranker ? score : nullptr, pop, n_prop, k, seed,
                              state, chosen, s));

@@ -33,9 +33,14 @@ struct MlpBufs {
   uint16_t* w3b = nullptr;
 };
 
-hipError_t launch_mlp_features(const uint64_t* d_masks, int32_t n, uint16_t* d_feat, hipStream_t s);
+// With d_gate, a launch does nothing while *d_gate <= gate_min (read on the
+// device: the evolutionary search scores its proposals only when more new
+// ones than the population survive deduplication).
+hipError_t launch_mlp_features(const uint64_t* d_masks, int32_t n, uint16_t* d_feat, hipStream_t s,
+                               const uint32_t* d_gate = nullptr, uint32_t gate_min = 0);
 hipError_t launch_mlp_predict(const MlpBufs& b, const uint16_t* d_feat, int32_t n, float* d_out,
-                              hipStream_t s);
+                              hipStream_t s, const uint32_t* d_gate = nullptr,
+                              uint32_t gate_min = 0);
 // Partials (gradient workgroups) of a step on n rows.
 int mlp_grad_parts(int32_t n);
 hipError_t launch_mlp_grads(const MlpBufs& b, const uint16_t* d_feat, const float* d_target,

@@ -105,10 +105,12 @@ __device__ __forceinline__ uint32_t pairs_at(uint64_t m, int d) {
 
 __global__ __launch_bounds__(256) void mlp_features_kernel(const uint64_t* __restrict__ masks,
                                                            int32_t n,
-                                                           uint16_t* __restrict__ feat) {
+                                                           uint16_t* __restrict__ feat,
+                                                           const uint32_t* __restrict__ gate,
+                                                           uint32_t gate_min) {
   const int lane = (int)(threadIdx.x & 63u);
   const int set = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
-  if (set >= n) return;
+  if (set >= n || (gate != nullptr && *gate <= gate_min)) return;
   const uint64_t m = masks[set] & kF49;
   const int k = __popcll(m);
   const float invk = (k > 0) ? 1.0f / (float)k : 0.0f;
@@ -181,10 +183,13 @@ __device__ __forceinline__ void tile_forward(const MlpView& W, const uint16_t* _
 
 __global__ __launch_bounds__(256) void mlp_predict_kernel(MlpView W,
                                                           const uint16_t* __restrict__ feat,
-                                                          int32_t n, float* __restrict__ out) {
+                                                          int32_t n, float* __restrict__ out,
+                                                          const uint32_t* __restrict__ gate,
+                                                          uint32_t gate_min) {
   __shared__ __attribute__((aligned(16))) uint16_t X[kMlpTile * kLd], H1[kMlpTile * kLd],
       H2[kMlpTile * kLd];
   __shared__ float y[kMlpTile];
+  if (gate != nullptr && *gate <= gate_min) return;   // uniform over the workgroup
   const int row0 = blockIdx.x * kMlpTile;
   tile_forward<kMlpIn>(W, feat, row0, n, X, H1, H2, y);
   if (threadIdx.x < kMlpTile && row0 + (int)threadIdx.x < n) out[row0 + threadIdx.x] = y[threadIdx.x];
@@ -415,17 +420,18 @@ __global__ __launch_bounds__(256) void bf16_gemm_kernel(const uint16_t* __restri
 static MlpView view(const MlpBufs& b) { return MlpView{b.theta, b.w1b, b.w2b, b.w3b}; }
 
 hipError_t launch_mlp_features(const uint64_t* d_masks, int32_t n, uint16_t* d_feat,
-                               hipStream_t s) {
+                               hipStream_t s, const uint32_t* d_gate, uint32_t gate_min) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(mlp_features_kernel, dim3((n + 3) / 4), dim3(256), 0, s, d_masks, n, d_feat);
+  hipLaunchKernelGGL(mlp_features_kernel, dim3((n + 3) / 4), dim3(256), 0, s, d_masks, n, d_feat,
+                     d_gate, gate_min);
   return hipGetLastError();
 }
 
 hipError_t launch_mlp_predict(const MlpBufs& b, const uint16_t* d_feat, int32_t n, float* d_out,
-                              hipStream_t s) {
+                              hipStream_t s, const uint32_t* d_gate, uint32_t gate_min) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(mlp_predict_kernel, dim3((n + kMlpTile - 1) / kMlpTile), dim3(256), 0, s,
-                     view(b), d_feat, n, d_out);
+                     view(b), d_feat, n, d_out, d_gate, gate_min);
   return hipGetLastError();
 }
 
