@@ -206,6 +206,11 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
   PlanCodes pc;
   uint32_t present = 0;
   md_plan_load_batch(offsets, n, n_idx, 0, state, out, tmax_arr, tmax_s, true, err, present, pc);
+  // Wave 0's lane b: chunks and work units per code of bucket b (no counts
+  // needed yet), before the barrier rather than on the scan's path.
+  const int wb = kMaxKExact - tid;   // walk dimension of bucket tid (k = wb + 1)
+  const uint64_t cc = (tid < kB) ? code_chunks(wb, Lb) : 0ull;
+  const uint64_t cu = (tid < kB) ? code_units(wb) : 0ull;
   __syncthreads();
 
   // Pass 1: bucket sizes.
@@ -230,10 +235,7 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
   // Bucket bases, chunk and unit offsets (wave 0, lane b = bucket b).
   if (wave == 0) {
     const int b = (int)lane;
-    const int w = kMaxKExact - b;   // walk dimension of bucket b (k = w + 1)
     const uint32_t t = (b < kB) ? cnt[b] : 0u;
-    const uint64_t cc = (b < kB) ? code_chunks(w, Lb) : 0ull;
-    const uint64_t cu = (b < kB) ? code_units(w) : 0ull;
     // Chunks of the bucket, clamped at 2^31 (a plan of 2^31 chunks or more
     // is refused), so the sum of 21 cannot wrap.
     const uint64_t xc = (t == 0) ? 0ull : (cc >= (1ull << 31)) ? (1ull << 31)
@@ -249,12 +251,12 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
     // loads; the loads below b are independent of each other.
     uint64_t ec = 0, eu = 0;
     uint32_t ep = 0;
-    for (int q = 0; q < kB; ++q) {
-      if (q < b) {
-        ec += sxc[q];
-        eu += sxu[q];
-        ep += cnt[q];
-      }
+#pragma unroll
+    for (int q = 0; q < kB; ++q) {   // branch-free: all 63 loads issue back to back
+      const uint64_t sel = 0ull - (uint64_t)(q < b);
+      ec += sxc[q] & sel;
+      eu += sxu[q] & sel;
+      ep += cnt[q] & (uint32_t)sel;
     }
     if (b < kB) {
       plan->ub[b] = eu;

@@ -18,9 +18,15 @@ max_zeros_batch        kernel.py:97-203, 276-323                   KATs "kernel 
 max_zeros_brute        "kernel backup.py":203-220                  KATs; independent of the bit-plane code
 bf_max_zeros_py        "kernel backup.py":203-220 (pure Python)    agrees with max_zeros_brute on small k
 splitmix64             kernel.py:10-15                             published SplitMix64 output for seed 1234567
-sampled stream         kernel.py:136-178                           PARITY UNPINNED beyond splitmix64 and the
-                                                                   bound sampled <= exact (no runnable
-                                                                   reference or vector exists for the stream)
+sampled stream         kernel.py:136-178                           pinned: splitmix64 (above); the set-index
+                                                                   seed shift s ^ i*0x9E3779B97F4A7C15; the
+                                                                   work-id shift s ^ w*0xBF58476D1CE4E5B9 ^
+                                                                   that constant, w = 1..n; prefix
+                                                                   monotonicity; sampled <= exact.
+                                                                   PARITY UNPINNED: the hash -> coefficient
+                                                                   map (mod 8^k - 1, + 1, base-8 digits) and
+                                                                   the k > 21 per-coefficient path (no
+                                                                   runnable reference or stored vector)
 gl2_perms              champion_search_canonical.py:61-74          |GL2(F7)| = 2016, permutations, det != 0
 rel_table              champion_search_canonical.py:99-108         translation identity
 canon_batch            champion_search_canonical.py:115-140,       numpy restatement (k <= 10); direct
