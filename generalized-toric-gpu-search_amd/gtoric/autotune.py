@@ -2,18 +2,20 @@
 
 Tunable per k (gt_set_md_params; a code of k points walks the (k-1)-dimensional
 subcode vanishing at the torus point (1,1), csrc/gt_common.h):
-  L           Gray walk bits per lane; one chunk = 64 lanes x 2^L codewords, so
-              L sets the number of waves per code: code_chunks(k - 1, L).
-  check_log2  early-exit check every 2^check_log2 blocks of 16 steps
-              (thresholded searches only).
+  L           Gray walk bits per lane of the claimed (thresholded) walk and of
+              the serving kernel; one chunk = 64 lanes x 2^L codewords, so L
+              sets the number of chunks per code: code_chunks(k - 1, L).
+  check_log2  early-exit check every 2^check_log2 blocks of 16 steps.
+The exact walk takes neither: it deals equal runs of 16-step blocks.
 
 For each k the tuner times device kernels (HIP events around md_main, read
-through gt_prof_read) on a seeded single-k batch, keeps the fastest setting,
-and writes a JSON table keyed by device architecture.  refine() then moves L
-by coordinate descent on a mixed batch, where the chunks of all k share the
-waves.  Results never depend on
-the parameters (tests/test_gpu_distance.py::test_params_do_not_change_results);
-only speed does.
+through gt_prof_read) on a seeded single-k batch thresholded at its median
+distance (about half the codes stop early), keeps the fastest setting, and
+writes a JSON table keyed by device architecture.  refine() then moves L by
+coordinate descent on a mixed batch, where the chunks of all k share the
+waves.  Results never depend on the parameters
+(tests/test_gpu_distance.py::test_params_do_not_change_results); only speed
+does.
 """
 
 from __future__ import annotations
@@ -76,9 +78,15 @@ def _time_md(ctx: _native.Context, flat, offs, tmax, repeats: int) -> float:
     return ms / max(n, 1)
 
 
+def _median_tmax(ctx: _native.Context, flat, offs) -> int:
+    """Threshold at the batch's median exact max_zeros: about half the codes
+    stop early, as in a search level."""
+    return int(np.median(ctx.md_batch(flat, offs, 48)))
+
+
 def tune(ctx: _native.Context, ks=range(3, 11), codes_per_k: dict | None = None,
          repeats: int = 3, verbose: bool = True) -> dict:
-    """Sweep L (exact mode) then check_log2 (thresholded mode) for each k."""
+    """Sweep L, then check_log2, of the thresholded walk for each k."""
     from .distance import flatten
     from . import workloads  # noqa: F401  (same RNG conventions)
     codes_per_k = codes_per_k or {3: 8192, 4: 8192, 5: 8192, 6: 4096, 7: 4096, 8: 2048,
@@ -89,20 +97,16 @@ def tune(ctx: _native.Context, ks=range(3, 11), codes_per_k: dict | None = None,
         sets = _batch(k, n, 1000 + k)
         flat, offs = flatten(sets)
         saved = ctx.get_md_params(k)
+        tmax = _median_tmax(ctx, flat, offs)
         best = None
         for L in candidate_L(k):
             ctx.set_md_params(k, L, saved[1])
-            t = _time_md(ctx, flat, offs, 48, repeats)
+            t = _time_md(ctx, flat, offs, tmax, repeats)
             if verbose:
                 print(f"[tune] k={k} L={L:2d} chunks/code={code_chunks(walk_dim(k), L):6d} md_main {t:.4f} ms")
             if best is None or t < best[1]:
                 best = (L, t)
         L = best[0]
-        # Early-exit interval on a thresholded batch: target = the batch's median
-        # exact distance, so about half the codes stop early.
-        ctx.set_md_params(k, L, saved[1])
-        exact = ctx.md_batch(flat, offs, 48)
-        tmax = int(np.median(exact))
         bestc = None
         for c in (0, 1, 2, 4, 6):
             ctx.set_md_params(k, L, c)
@@ -112,21 +116,23 @@ def tune(ctx: _native.Context, ks=range(3, 11), codes_per_k: dict | None = None,
             if bestc is None or t < bestc[1]:
                 bestc = (c, t)
         ctx.set_md_params(k, L, bestc[0])
-        table[str(k)] = {"L": L, "check_log2": bestc[0], "exact_ms": round(best[1], 5),
-                         "thr_ms": round(bestc[1], 5), "codes": n}
+        table[str(k)] = {"L": L, "check_log2": bestc[0], "thr_ms": round(bestc[1], 5),
+                         "tmax": tmax, "codes": n}
     return {"arch": ctx.arch, "cu_count": ctx.cu_count, "params": table}
 
 
 def refine(ctx: _native.Context, sets, repeats: int = 5, margin: float = 0.005,
            verbose: bool = True) -> tuple[dict, float]:
-    """Coordinate descent of L on one mixed batch, from the context's current
-    table.  Per-k tuning times one k at a time; in a mix the chunks of every
-    k share the same waves (largest k first), so the best L of a k can move.
-    For each k of the batch, largest share of codewords first, L is stepped
-    up, then down, while md_main on the whole batch gets faster by more than
-    `margin`.  Returns ({k: L} for the moves kept, final md_main ms)."""
+    """Coordinate descent of L on one mixed batch thresholded at its median
+    distance, from the context's current table.  Per-k tuning times one k at
+    a time; in a mix the chunks of every k share the same waves (largest k
+    first), so the best L of a k can move.  For each k of the batch, largest
+    share of codewords first, L is stepped up, then down, while md_main on the
+    whole batch gets faster by more than `margin`.  Returns ({k: L} for the
+    moves kept, final md_main ms)."""
     from .distance import flatten
     flat, offs = flatten(sets)
+    tmax = _median_tmax(ctx, flat, offs)
     work: dict[int, int] = {}
     for s in sets:
         k = len(s)
@@ -134,11 +140,11 @@ def refine(ctx: _native.Context, sets, repeats: int = 5, margin: float = 0.005,
             work[k] = work.get(k, 0) + walk_codewords(k)
     # The clocks settle over tens of milliseconds of load: time until two
     # readings in a row agree within 1 %, so the start is not a cold reading.
-    prev, cur = None, _time_md(ctx, flat, offs, 48, repeats)
+    prev, cur = None, _time_md(ctx, flat, offs, tmax, repeats)
     for _ in range(20):
         if prev is not None and abs(cur - prev) <= 0.01 * prev:
             break
-        prev, cur = cur, _time_md(ctx, flat, offs, 48, repeats)
+        prev, cur = cur, _time_md(ctx, flat, offs, tmax, repeats)
     if verbose:
         print(f"[refine] start md_main {cur:.4f} ms")
     moved = {}
@@ -148,7 +154,7 @@ def refine(ctx: _native.Context, sets, repeats: int = 5, margin: float = 0.005,
         for step in (1, -1):
             while L + step in candidate_L(k):
                 ctx.set_md_params(k, L + step, c)
-                t = _time_md(ctx, flat, offs, 48, repeats)
+                t = _time_md(ctx, flat, offs, tmax, repeats)
                 if verbose:
                     print(f"[refine] k={k} L={L + step:2d} md_main {t:.4f} ms")
                 if t >= cur * (1.0 - margin):
@@ -190,7 +196,8 @@ if __name__ == "__main__":
     ap.add_argument("--repeats", type=int, default=3)
     ap.add_argument("--refine-mix", action="store_true",
                     help="load the saved table and refine L on the batched workload "
-                         "(4096 codes, k uniform in 3..8) instead of tuning per k")
+                         "(4096 codes, k uniform in 3..8, thresholded at its median) "
+                         "instead of tuning per k")
     a = ap.parse_args()
     c = _native.Context(0)
     if a.refine_mix:
@@ -201,7 +208,8 @@ if __name__ == "__main__":
         moved, ms = refine(c, workloads.mixed_k(), repeats=max(a.repeats, 5))
         for k, L in moved.items():
             res["params"][str(k)]["L"] = L
-        res["mix_refined"] = {"workload": "4096 codes, k uniform in 3..8 (seed 2)",
+        res["mix_refined"] = {"workload": "4096 codes, k uniform in 3..8 (seed 2), thresholded "
+                                          "at the median distance",
                               "md_main_ms": round(ms, 5),
                               "moved": {str(k): L for k, L in moved.items()}}
         p.write_text(json.dumps(res, indent=1) + "\n")

@@ -23,11 +23,16 @@ from ._native import check, i32, i64, lib, ptr, vp
 MAX_K = 21
 
 
-# Serving chunks are 2^SERVE_L_OFFSET times the batch kernel's: a served chunk
-# also pays a request lookup, a row staging and a fold at the device coherence
-# point, so it needs more walk to amortise them.  Measured on the 20,000-request
-# mixed-k serving stream (all arrivals at t = 0, 4096 slots): L + 0 149k, L + 2
-# 374k, L + 3 561k, L + 4 781k requests/s.
+# Served walk length per k (chunks of 64 x 2^L codewords).  A served chunk also
+# pays a request lookup, a row staging and a fold at the device coherence point,
+# so it needs more walk than a batch chunk to amortise them: on the
+# 20,000-request mixed-k serving stream (all arrivals at t = 0, 4096 slots),
+# batch L + 0 gave 149k, + 2 374k, + 3 561k and + 4 781k requests/s.  These are
+# the + 4 lengths of the batch table they were measured with; the batch table
+# is tuned for the thresholded batch walk and may move on its own (k = 10 at
+# L = 16 served cost a third of the capacity and doubled p99).  Other k: the
+# context's batch L + SERVE_L_OFFSET.
+SERVE_L = {3: 8, 4: 8, 5: 8, 6: 8, 7: 13, 8: 14, 9: 14, 10: 14}
 SERVE_L_OFFSET = 4
 
 
@@ -36,8 +41,9 @@ class Server:
                  params: dict | None = None, idle_s: float = 10.0, life_s: float = 600.0,
                  max_chunk_log2: int | None = None, short_chunk_log2: int | None = None,
                  short_below: int = 1 << 33):
-        """params: {k: (L, check_log2)}; defaults: the context's (tuned) batch
-        table with L raised by SERVE_L_OFFSET.
+        """params: {k: (L, check_log2)}; defaults: SERVE_L (other k: the
+        context's batch L raised by SERVE_L_OFFSET) with the context's
+        check_log2.
 
         max_chunk_log2 caps the served L (chunks of 64 x 2^L codewords) at
         max(batch L, cap): a chunk is walked by one wave, so its length bounds
@@ -55,7 +61,7 @@ class Server:
             params = {}
             for k in range(MAX_K + 1):
                 L, c = ctx.get_md_params(k)
-                Ls = min(30, L + SERVE_L_OFFSET)
+                Ls = SERVE_L.get(k, min(30, L + SERVE_L_OFFSET))
                 if max_chunk_log2 is not None:
                     Ls = min(Ls, max(L, int(max_chunk_log2)))
                 params[k] = (Ls, c)

@@ -52,7 +52,7 @@ def test_chunk_cap_changes_chunks_not_results(ctx):
     results stay those of the batch kernel and the chunk count follows the
     capped table."""
     from gtoric.autotune import code_chunks, walk_dim
-    from gtoric.serving import SERVE_L_OFFSET, Server
+    from gtoric.serving import SERVE_L, SERVE_L_OFFSET, Server
     sets, tmax = _requests(400, seed=57, kmax=10)
     with Server(0, slots=256, life_s=60, max_chunk_log2=9) as srv:
         res, _, _ = srv.run_stream(sets, tmax, np.zeros(len(sets)), timeout_s=60)
@@ -62,7 +62,7 @@ def test_chunk_cap_changes_chunks_not_results(ctx):
     L = {}
     for k in range(1, 22):
         Lb = ctx.get_md_params(k)[0]
-        L[k] = min(min(30, Lb + SERVE_L_OFFSET), max(Lb, 9))
+        L[k] = min(SERVE_L.get(k, min(30, Lb + SERVE_L_OFFSET)), max(Lb, 9))
     want = sum(code_chunks(walk_dim(len(S)), L[len(S)]) for S in sets if len(S))
     assert st["chunks_walked"] + st["chunks_skipped"] == want
 
@@ -71,12 +71,13 @@ def test_counters_account_for_every_chunk(ctx):
     """gt_serve_stats: every request dispatched once, and chunks walked plus
     chunks skipped equal the chunk count of the served parameter table."""
     from gtoric.autotune import code_chunks, walk_dim
-    from gtoric.serving import SERVE_L_OFFSET, Server
+    from gtoric.serving import SERVE_L, SERVE_L_OFFSET, Server
     sets, tmax = _requests(500, seed=56, kmax=9)
     with Server(0, slots=256, life_s=60) as srv:
         res, lat, wall = srv.run_stream(sets, tmax, np.zeros(len(sets)), timeout_s=60)
         st = srv.stats()
-        L = {k: min(30, ctx.get_md_params(k)[0] + SERVE_L_OFFSET) for k in range(1, 22)}
+        L = {k: SERVE_L.get(k, min(30, ctx.get_md_params(k)[0] + SERVE_L_OFFSET))
+             for k in range(1, 22)}
     assert st["requests_dispatched"] == len(sets)
     want = sum(code_chunks(walk_dim(len(S)), L[len(S)]) for S in sets if len(S))
     assert st["chunks_walked"] + st["chunks_skipped"] == want
