@@ -258,8 +258,12 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
       eu += sxu[q] & sel;
       ep += cnt[q] & (uint32_t)sel;
     }
+    // A plan of 2^31 chunks or more is refused; its units are zeroed so the
+    // exact walk does nothing either (the call reports the error).
+    const uint64_t all_chunks = readlane64(ec + xc, kB - 1);
+    const bool refused = all_chunks >= (1ull << 31);
     if (b < kB) {
-      plan->ub[b] = eu;
+      plan->ub[b] = refused ? 0ull : eu;
       plan->cb[b] = (uint32_t)min(ec, 1ull << 31);
       plan->pb[b] = ep;
       plan->cc[b] = (uint32_t)cc;

@@ -112,3 +112,30 @@ def test_device_errors_are_reported_by_their_own_call(ctx):
     finally:
         for x in (di, do, dout):
             x.close()
+
+
+def test_device_plan_too_large_is_refused_without_walking(ctx):
+    """k = 21 codes whose claimed chunks total 2^31 or more: the device plan
+    refuses the call (an error from gt_sync) and neither walk runs (the exact
+    walk's work units are zeroed with the refused plan; walking them would
+    take years).  The count is checked here first, from the context's own
+    chunk length, so the call below is one the plan must refuse."""
+    from gtoric._native import GtError
+    from gtoric.autotune import code_chunks
+    from gtoric.distance import flatten
+    L = ctx.get_md_params(21)[0]
+    per_code = code_chunks(20, L)
+    n = (1 << 31) // per_code + 1
+    assert n * per_code >= (1 << 31) and n <= 4096, (L, per_code, n)
+    rng = np.random.default_rng(7)
+    sets = [sorted(rng.choice(49, 21, replace=False).tolist()) for _ in range(n)]
+    flat, offs = flatten(sets)
+    di, do, dout = ctx.to_device(flat), ctx.to_device(offs), ctx.array(n, np.int32)
+    try:
+        for tmax, thr in ((48, False), (30, True)):
+            ctx.md_batch_dev(di.ptr, len(flat), do.ptr, n, dout.ptr, tmax=tmax, thresholded=thr)
+            with pytest.raises(GtError):
+                ctx.sync()
+    finally:
+        for x in (di, do, dout):
+            x.close()
