@@ -119,8 +119,9 @@ __global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict_
 // bases, chunk and unit offsets (lane b sums the entries below b from LDS).
 // Pass 2 places the codes: one LDS atomic instruction per wave reserves its
 // positions in every bucket's cursor, and mbcnt of each round's ballot gives
-// a lane its place in them.  The order inside a bucket therefore follows the waves' atomics, not
-// the batch order; no kernel depends on it, each code's result being its own.
+// a lane its place in them.  The order inside a bucket therefore follows the
+// waves' atomics, not the batch order; no kernel depends on it, each code's
+// result being its own.
 // (A plan that kept the batch order with per-round prefix sums over the
 // waves took 14.7 us for 4096 codes, against 1.7 us for the rows.)
 constexpr int kPlanThreads = 1024;
@@ -519,8 +520,17 @@ __global__ __launch_bounds__(256) void md_main_exact_kernel(
   const uint32_t* pb = sh.plan + offsetof(MdPlan, pb) / 4;
   auto pbu = [&](int b) { return (uint32_t)__builtin_amdgcn_readfirstlane(pb[b]); };
   const uint64_t W = (uint64_t)gridDim.x * (blockDim.x / kWave);
-  const uint64_t wid =
-      __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+  // Wave index in the deal, XCD-major: workgroups go round-robin over the 8
+  // XCDs (workgroup b on XCD b % 8), so numbering the waves of XCD 0 first,
+  // then XCD 1, ... gives each XCD one contiguous range of runs, and a code's
+  // runs (consecutive) mostly sit on one XCD, whose L2 then fetches its rows
+  // once.
+  constexpr uint32_t kXcds = 8;
+  const uint32_t G = gridDim.x, bx = blockIdx.x, x = bx % kXcds;
+  uint32_t before = 0;
+  for (uint32_t y = 0; y < x; ++y) before += (G - y + kXcds - 1) / kXcds;
+  const uint64_t wid = __builtin_amdgcn_readfirstlane(
+      (before + bx / kXcds) * (blockDim.x / kWave) + threadIdx.x / kWave);
 
   // Small codes, whole.
   {
