@@ -197,10 +197,13 @@ struct CanonOut {
 };
 
 // Canonical key, 49-bit canonical mask and |Stab| of the set `mask` (|S| = K,
-// wave-uniform); every lane returns the same values.  par_mine = T.par[lane]
+// wave-uniform); every lane returns the same values.  Without kPack only
+// |Stab| and the canonical image in byte-row layout (in `canon`) are
+// returned: the conversion and the key packing are scalar work that
+// canon_keys_kernel then does per lane.  par_mine = T.par[lane]
 // (the relation is symmetric: bit r1 says whether this lane's vector is
 // parallel to r1).
-template <int K>
+template <int K, bool kPack>
 __device__ __forceinline__ CanonOut canon_set(const CanonTables& T, uint64_t mask, uint32_t lane,
                                               uint64_t par_mine) {
   constexpr int NP = (K + 4) / 5;   // 6-bit fields, five per dword
@@ -325,8 +328,13 @@ __device__ __forceinline__ CanonOut canon_set(const CanonTables& T, uint64_t mas
   // Sorted positions of the canonical image, packed as kernel_canon.py:84-104:
   // K <= 10 packs positions 0..K-1, K >= 11 skips position 0 (always 0).
   CanonOut o;
-  o.canon = rows_to_mask(m);
   o.stab = total;
+  if (!kPack) {
+    o.canon = m;
+    o.key_lo = o.key_hi = 0;
+    return o;
+  }
+  o.canon = rows_to_mask(m);
   constexpr int pack_from = (K <= 10) ? 0 : 1;
   uint64_t lo = 0, hi = 0, mm = o.canon;
 #pragma unroll
@@ -345,6 +353,7 @@ __device__ __forceinline__ CanonOut canon_set(const CanonTables& T, uint64_t mas
 }
 
 constexpr int kXcds = 8;
+constexpr int32_t kCanonSplitMin = 1 << 16;
 
 // One wavefront per set, in input order.  Workgroups are dealt round-robin
 // over the XCDs (workgroup b on XCD b % 8), so a pass over nwaves sets gives
@@ -356,6 +365,9 @@ constexpr int kXcds = 8;
 // before working on the current one (a vector load, counted on vmcnt, which
 // the LDS waits of the current set do not wait for).  A mask outside 1..16
 // points, or with bits above 48, is flagged and its outputs are not written.
+// Without kPack the kernel writes |Stab| and the byte-row canonical image
+// (into `canon`), and canon_keys_kernel finishes the outputs.
+template <bool kPack>
 __global__ __launch_bounds__(kCanonBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void canon_kernel(
     const uint64_t* __restrict__ masks, uint32_t n, uint64_t* __restrict__ key_lo,
     uint64_t* __restrict__ key_hi, uint64_t* __restrict__ canon, int32_t* __restrict__ stab,
@@ -385,7 +397,7 @@ __global__ __launch_bounds__(kCanonBlock) __attribute__((amdgpu_waves_per_eu(8, 
     if (ok) {
       switch (__popcll(mask)) {
 #define GT_CANON_CASE(KK) \
-  case KK: o = canon_set<KK>(T, mask, lane, par_mine); break;
+  case KK: o = canon_set<KK, kPack>(T, mask, lane, par_mine); break;
         GT_CANON_CASE(1) GT_CANON_CASE(2) GT_CANON_CASE(3) GT_CANON_CASE(4)
         GT_CANON_CASE(5) GT_CANON_CASE(6) GT_CANON_CASE(7) GT_CANON_CASE(8)
         GT_CANON_CASE(9) GT_CANON_CASE(10) GT_CANON_CASE(11) GT_CANON_CASE(12)
@@ -396,8 +408,10 @@ __global__ __launch_bounds__(kCanonBlock) __attribute__((amdgpu_waves_per_eu(8, 
     }
     bad |= !ok;
     if (ok && lane == 0) {
-      key_lo[j] = o.key_lo;
-      key_hi[j] = o.key_hi;
+      if (kPack) {
+        key_lo[j] = o.key_lo;
+        key_hi[j] = o.key_hi;
+      }
       canon[j] = o.canon;
       stab[j] = (int32_t)o.stab;
     }
@@ -429,6 +443,33 @@ hipError_t launch_sets_to_masks(const int32_t* d_sets, int32_t n, int32_t k, uin
   return hipGetLastError();
 }
 
+// Byte-row canonical images -> 49-bit masks and packed keys, one lane per
+// set (kernel_canon.py:84-104: sorted positions 6 bits each, position 0
+// skipped for |S| >= 11).  A set the canonical kernel flagged left its entry
+// unwritten; whatever it holds, at most 16 positions are packed.
+__global__ __launch_bounds__(256) void canon_keys_kernel(uint32_t n, uint64_t* __restrict__ canon,
+                                                         uint64_t* __restrict__ key_lo,
+                                                         uint64_t* __restrict__ key_hi) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t mask = rows_to_mask(canon[i]);
+  const int k = __popcll(mask);
+  int skip = (k >= 11) ? 1 : 0;
+  uint64_t lo = 0, hi = 0, mm = mask;
+  int off = 0;
+  for (int t = 0; t < kMaxKCanon && mm != 0; ++t) {
+    const uint64_t v = (uint64_t)__builtin_ctzll(mm);
+    mm &= mm - 1;
+    if (skip) { skip = 0; continue; }
+    if (off < 64) lo |= v << off;
+    if (off + 6 > 64) hi |= (off < 64) ? (v >> (64 - off)) : (v << (off - 64));
+    off += 6;
+  }
+  canon[i] = mask;
+  key_lo[i] = lo;
+  key_hi[i] = hi;
+}
+
 hipError_t launch_canon(const uint64_t* d_masks, int32_t n, uint64_t* d_key_lo,
                         uint64_t* d_key_hi, uint64_t* d_canon, int32_t* d_stab, uint32_t* d_ctrl,
                         hipStream_t s) {
@@ -436,8 +477,20 @@ hipError_t launch_canon(const uint64_t* d_masks, int32_t n, uint64_t* d_key_lo,
   constexpr uint32_t kWavesPerBlock = kCanonBlock / kWave;
   const int grid = (int)std::min<uint32_t>(kCanonMaxBlocks,
                                            ((uint32_t)n + kWavesPerBlock - 1) / kWavesPerBlock);
-  hipLaunchKernelGGL(canon_kernel, dim3(grid), dim3(kCanonBlock), 0, s, d_masks, (uint32_t)n,
-                     d_key_lo, d_key_hi, d_canon, d_stab, d_ctrl);
+  // Large batches leave the key packing to a second, per-lane pass: about a
+  // quarter of the wavefront's scalar instructions per set, and the kernel is
+  // bound by issue (6.5 % of its time on 10^6 masks).  Small batches (the
+  // search's generations) keep one launch.
+  const bool split = n >= kCanonSplitMin;
+  if (split)
+    hipLaunchKernelGGL(canon_kernel<false>, dim3(grid), dim3(kCanonBlock), 0, s, d_masks,
+                       (uint32_t)n, d_key_lo, d_key_hi, d_canon, d_stab, d_ctrl);
+  else
+    hipLaunchKernelGGL(canon_kernel<true>, dim3(grid), dim3(kCanonBlock), 0, s, d_masks,
+                       (uint32_t)n, d_key_lo, d_key_hi, d_canon, d_stab, d_ctrl);
+  if (split)
+    hipLaunchKernelGGL(canon_keys_kernel, dim3(((uint32_t)n + 255) / 256), dim3(256), 0, s,
+                       (uint32_t)n, d_canon, d_key_lo, d_key_hi);
   return hipGetLastError();
 }
 

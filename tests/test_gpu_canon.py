@@ -95,6 +95,28 @@ def test_mixed_masks_match_uniform(canon):
         assert np.array_equal(st[sel], ost)
 
 
+def test_large_batches_equal_small_ones(canon):
+    """From 65,536 sets on, the kernel leaves the key packing to a per-lane
+    pass (canon_keys_kernel); the same masks in batches below that size take
+    the one-kernel path.  Every output must agree, over all sizes 1..16 and
+    the oracle on a sample."""
+    from gtoric import workloads as W
+    from gtoric.canonical import masks_to_sets
+    masks = np.concatenate([W.canon_masks(100_000, k_lo=1, k_hi=16, seed=37),
+                            W.canon_masks(1_000, k_lo=11, k_hi=16, seed=38)])
+    ctx = canon.context
+    big = ctx.canon_masks(masks)
+    parts = [ctx.canon_masks(masks[i:i + 50_000]) for i in range(0, len(masks), 50_000)]
+    for a, b in zip(big, (np.concatenate([p[j] for p in parts]) for j in range(4))):
+        assert np.array_equal(a, b)
+    ks = np.array([bin(int(m)).count("1") for m in masks])
+    for k in (1, 10, 11, 16):
+        sel = np.nonzero(ks == k)[0][:20]
+        olo, ohi, ost = oracle.canon_batch(masks_to_sets(masks[sel], k))
+        assert np.array_equal(big[0][sel], olo) and np.array_equal(big[1][sel], ohi), k
+        assert np.array_equal(big[3][sel], ost), k
+
+
 def test_canonical_properties_at_scale(canon):
     # Size-independent: canonical masks contain point 0, have the input size,
     # |Stab| divides 98,784, canon(canon) = canon, canon(g S) = canon(S).
