@@ -6,8 +6,9 @@
 // kernel_cuda_bp.py:99-127).  Work decomposition (see gt_common.h):
 //
 //   md_prep     workgroup 0: codes bucketed by walk dimension, largest first
-//               (MdPlan + order[]), reset of the per-code minimum and of the
-//               chunk counters; workgroups 1..: one wave per code, the bit
+//               (MdPlan, order[], roff[]), the per-code state words and
+//               claim counters reset, the results of codes without a walk
+//               (k <= 1) written; workgroups 1..: one wave per code, the bit
 //               rows x^b * (g_i + g_{k-1}) of the subcode vanishing at the
 //               torus point (1,1) (gt_common.h), which has the same maximum
 //               number of zeros as C(S) with 1/8 of the codewords.  The
@@ -26,9 +27,10 @@
 //               max_zeros clamped at tmax + 1; codes without a walk
 //               (k <= 1) get theirs from md_prep.
 //
-// Inner step (per projective codeword, all VALU, rows in SGPRs):
-//   6 v_xor_b32 (row add), 2 v_or3_b32 (plane OR), 2 v_bcnt_u32_b32,
-//   and one v_min3_u32 per two codewords.
+// Inner step (per projective codeword, all VALU; the block's rows are
+// wave-uniform, loaded with scalar loads and copied to VGPRs):
+//   6 v_xor_b32 (row add, VOP2 with two VGPR operands), 2 v_or3_b32 (plane
+//   OR), 2 v_bcnt_u32_b32, and one v_min3_u32 per two codewords.
 #include <cstddef>
 
 #include "gt_launch.h"
