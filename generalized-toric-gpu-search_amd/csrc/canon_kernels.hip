@@ -364,7 +364,8 @@ constexpr int32_t kCanonSplitMin = 1 << 16;
 // the algorithmic write traffic).  Each wave fetches the mask of its next set
 // before working on the current one (a vector load, counted on vmcnt, which
 // the LDS waits of the current set do not wait for).  A mask outside 1..16
-// points, or with bits above 48, is flagged and its outputs are not written.
+// points, or with bits above 48, is flagged and its outputs are not written
+// here (canon_keys_kernel may still overwrite them: they are unspecified).
 // Without kPack the kernel writes |Stab| and the byte-row canonical image
 // (into `canon`), and canon_keys_kernel finishes the outputs.
 template <bool kPack>

@@ -611,7 +611,7 @@ int gt_md_single(gt_ctx* c, const int32_t* idx, int32_t k, int32_t tmax, int32_t
 }
 
 // Canonical forms of n device masks in input order (the kernel validates
-// them: a bad mask is flagged and its outputs left unwritten).
+// them: a bad mask is flagged and its outputs are unspecified).
 static int canon_enqueue(gt_ctx* c, const uint64_t* d_masks, int32_t n, uint64_t* d_lo,
                          uint64_t* d_hi, uint64_t* d_canon, int32_t* d_stab, hipStream_t s) {
   if (n <= 0) return 0;

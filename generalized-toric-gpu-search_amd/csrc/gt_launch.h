@@ -50,7 +50,7 @@ int md_main_blocks_per_cu(bool thresholded);
 
 // ---- canonical form ------------------------------------------------------
 // One wave per set in input order; a mask with bits above 48 or a size
-// outside 1..16 is flagged (kErrBadK) and its outputs are left unwritten.
+// outside 1..16 is flagged (kErrBadK) and its outputs are unspecified.
 hipError_t launch_canon(const uint64_t* d_masks, int32_t n, uint64_t* d_key_lo,
                         uint64_t* d_key_hi, uint64_t* d_canon, int32_t* d_stab, uint32_t* d_ctrl,
                         hipStream_t s);
