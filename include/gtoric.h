@@ -53,8 +53,11 @@ int gt_close(gt_ctx* ctx);
 int gt_device_info(gt_ctx* ctx, int32_t* cu_count, int32_t* clock_khz,
                    int32_t* lds_per_block, char* name, int32_t name_len);
 
-/* Per-k launch parameters (autotuner): Gray walk bits per lane L (4..30) and
- * the early-exit check interval 2^check_log2 blocks of 16 steps. */
+/* Per-k launch parameters (autotuner) of the thresholded batch walk and the
+ * one-code launch: Gray walk bits per lane L (4..30; chunks of 64 x 2^L
+ * codewords) and the early-exit check interval 2^check_log2 blocks of 16
+ * steps.  The exact batch walk deals equal runs of blocks and reads neither;
+ * results never depend on them. */
 int gt_set_md_params(gt_ctx* ctx, int32_t k, int32_t L, int32_t check_log2);
 int gt_get_md_params(gt_ctx* ctx, int32_t k, int32_t* L, int32_t* check_log2);
 
