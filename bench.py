@@ -649,7 +649,7 @@ def main() -> None:
         pdt = bench_plain(ctx, sets, expect.tolist() if expect is not None else None)
         result["plain_per_code"] = {
             "value": round(n / pdt, 1), "unit": "codes/s",
-            "form": "gt_md_single: one upload + one kernel launch + one download per code",
+            "form": "gt_md_single: per code one kernel launch (rows in its arguments) and a poll of the pinned word its last wave writes",
             "batched_speedup": round(value / (n / pdt), 2),
         }
         log(f"[bench] plain per-code: {n / pdt:,.0f} codes/s")

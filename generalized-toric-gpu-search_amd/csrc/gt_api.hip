@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -203,7 +204,12 @@ struct gt_ctx {
   // unrelated synchronous call.
   uint32_t* ctrl_p = nullptr;
   DevBuf idx, offs, tmax, rows, plan, best, out, zmax, stopf;  // min distance
-  DevBuf sbuf;                                                // single code
+  // One-code call: its device state {minimum, waves done} ({64, 0} between
+  // launches), the pinned host word the kernel writes (tag, result) to, and
+  // the last tag.
+  DevBuf sbuf;
+  uint64_t* spin = nullptr;
+  uint32_t single_tag = 0;
   DevBuf masks, klo, khi, canon, stab;                        // canonical
   DevBuf children, bflag, bsum, bidx, bcnt, bmasks, brows, boffs;  // BFS level
 
@@ -428,6 +434,7 @@ int gt_close(gt_ctx* c) {
     for (auto& kv : c->prof_rec)
       for (auto& pr : kv.second.ev) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
     for (DevBuf* b : c->all_bufs()) b->release();
+    if (c->spin) (void)hipHostFree(c->spin);
     if (c->handover) (void)hipEventDestroy(c->handover);
     (void)hipStreamDestroy(c->stream);
   }
@@ -579,34 +586,50 @@ int gt_md_single(gt_ctx* c, const int32_t* idx, int32_t k, int32_t tmax, int32_t
   GT_OK(c->order_stream(c->stream));
   hipStream_t s = c->stream;
   // Host-built bit rows of the subcode vanishing at (1,1) (rows 3i + b =
-  // x^b * (g_i + g_{k-1}), i < k - 1) + the running minimum in one upload.
+  // x^b * (g_i + g_{k-1}), i < k - 1), passed in the kernel arguments.
   uint32_t mp[kNpts * kRowWords];
   build_mplanes(mp);
-  std::vector<uint32_t> h((size_t)3 * w * kRowWords + kRowWords, 0u);
+  SingleRows rows{};
   for (int i = 0; i < w; ++i) {
     uint32_t v[6];
     for (int q = 0; q < 6; ++q) v[q] = mp[idx[i] * kRowWords + q] ^ mp[idx[k - 1] * kRowWords + q];
     for (int b = 0; b < 3; ++b) {
-      for (int q = 0; q < 6; ++q) h[(size_t)(3 * i + b) * kRowWords + q] = v[q];
+      for (int q = 0; q < 6; ++q) rows.r[(3 * i + b) * kRowWords + q] = v[q];
       mul_x(v);
     }
   }
-  const size_t best_off = (size_t)3 * w * kRowWords;
-  h[best_off] = 64u;
+  if (!c->spin) {
+    GT_HIP(hipHostMalloc((void**)&c->spin, 256, hipHostMallocCoherent | hipHostMallocMapped));
+    __atomic_store_n(c->spin, 0ull, __ATOMIC_RELEASE);   // no tag yet (tags start at 1)
+    GT_OK(c->sbuf.ensure(8));
+    const uint32_t init[2] = {64u, 0u};
+    GT_HIP(hipMemcpyAsync(c->sbuf.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    GT_HIP(hipStreamSynchronize(s));
+  }
   // One-code launches want many waves: about 2^11 chunks for the largest piece.
   const int L = std::max<int>(kU, std::min<int>(c->params.L[k], 3 * (w - 1) - kLaneBits - 11));
-  GT_OK(c->sbuf.ensure(h.size() * 4));
-  GT_HIP(hipMemcpyAsync(c->sbuf.p, h.data(), h.size() * 4, hipMemcpyHostToDevice, s));
-  uint32_t* d_best = c->sbuf.as<uint32_t>() + best_off;
   hipEvent_t e0 = nullptr;
   GT_OK(c->prof_begin("md_single", s, &e0));
-  GT_HIP(launch_md_single(tmax < 48, c->sbuf.as<uint32_t>(), w, L, c->params.check_log2[k], thr,
-                          d_best, s));
+  const uint32_t tag = ++c->single_tag;
+  GT_HIP(launch_md_single(tmax < 48, rows, w, L, c->params.check_log2[k], thr, tmax, tag,
+                          c->sbuf.as<uint32_t>(), c->spin, s));
   GT_OK(c->prof_end("md_single", s, e0));
-  uint32_t bw = 0;
-  GT_HIP(hipMemcpyAsync(&bw, d_best, 4, hipMemcpyDeviceToHost, s));
-  GT_HIP(hipStreamSynchronize(s));
-  *out = ((int)bw <= thr) ? tmax + 1 : 49 - (int)bw;
+  // A short launch is awaited by polling the pinned word (a stream
+  // synchronisation costs more than most one-code launches run); after
+  // 200 us the call blocks on the stream instead.
+  uint64_t word = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    word = __atomic_load_n(c->spin, __ATOMIC_ACQUIRE);
+    if ((uint32_t)(word >> 32) == tag) break;
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(200)) {
+      GT_HIP(hipStreamSynchronize(s));
+      word = __atomic_load_n(c->spin, __ATOMIC_ACQUIRE);
+      if ((uint32_t)(word >> 32) != tag) return fail("one-code launch left no result");
+      break;
+    }
+  }
+  *out = (int32_t)(uint32_t)word;
   return 0;
 }
 

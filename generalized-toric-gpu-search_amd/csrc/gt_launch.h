@@ -35,9 +35,17 @@ hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
                           const MdPlan* d_plan, const uint32_t* d_order, const uint32_t* d_roff,
                           uint32_t* d_claims, uint64_t* d_state, const int32_t* d_tmax,
                           int32_t tmax, int32_t* d_out, const MdParams* d_params, hipStream_t s);
-// One code, one launch: grid = code_chunks(k, L) single-wave workgroups.
-hipError_t launch_md_single(bool thresholded, const uint32_t* d_rows, int k, int L,
-                            int check_log2, int thr, uint32_t* d_best, hipStream_t s);
+// One code, one launch: grid = code_chunks(w, L) single-wave workgroups, the
+// 3w rows (w <= 20) in the kernel arguments.  d_state: {64, 0} before the
+// first launch; each launch leaves it so.  h_result: a host-visible word
+// (pinned memory) that receives tag << 32 | min(max_zeros, tmax + 1) (the
+// low half as int32) when the launch's last wave finishes.
+struct alignas(16) SingleRows {
+  uint32_t r[3 * (kMaxKExact - 1) * kRowWords];
+};
+hipError_t launch_md_single(bool thresholded, const SingleRows& rows, int w, int L,
+                            int check_log2, int thr, int tmax, uint32_t tag, uint32_t* d_state,
+                            uint64_t* h_result, hipStream_t s);
 // set_period > 0: set ids restart every set_period codes (0: set id = index).
 // d_gen (may be null): a search's completed-generation counter; the seed
 // becomes sample_seed + (d_gen[0] + 1) * 1,000,003.

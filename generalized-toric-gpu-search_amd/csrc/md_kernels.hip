@@ -669,16 +669,41 @@ __global__ __launch_bounds__(256) void md_main_thr_kernel(
   }
 }
 
+// One code, one launch (kernel.py:241-274).  The rows travel in the kernel
+// arguments (no upload); each wave folds its minimum into state[0] and counts
+// itself in state[1] (the minimum first, as in md_fold), and the wave that
+// completes the launch writes (tag << 32 | result) to the caller's
+// host-visible word in one 64-bit store and resets the state for the next
+// launch, so a call is one launch and a poll of that word.  With THR, waves
+// that start after the threshold is reached only count themselves.
 template <bool THR>
-__global__ __launch_bounds__(64) void md_single_kernel(const uint32_t* __restrict__ R, int w, int L,
-                                                       uint32_t check_mask, int thr,
-                                                       uint32_t* best_out) {
+__global__ __launch_bounds__(64) void md_single_kernel(const SingleRows rows, int w, int L,
+                                                       uint32_t check_mask, int thr, int tmax,
+                                                       uint32_t tag, uint32_t* state,
+                                                       uint64_t* result) {
   const uint32_t lane = lane_id();
-  if (THR && (int)__hip_atomic_load(best_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= thr)
-    return;
-  uint32_t b = run_chunk<THR>(ConstRows{R}, w, L, blockIdx.x, lane, thr, check_mask, best_out);
-  b = wave_min(b);
-  if (lane == 0) atomicMin(best_out, b);
+  bool walk = true;
+  if (THR)
+    walk = (int)__builtin_amdgcn_readfirstlane(
+               __hip_atomic_load(&state[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) > thr;
+  uint32_t b = 64u;
+  if (walk)
+    b = wave_min(run_chunk<THR>(ConstRows{rows.r}, w, L, blockIdx.x, lane, thr, check_mask,
+                                &state[0]));
+  if (lane == 0) {
+    const uint32_t prev =
+        __hip_atomic_fetch_min(&state[0], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);
+    const uint32_t done = __hip_atomic_fetch_add(&state[1], 1u + (prev >> 31), __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT) + 1u;
+    if (done == gridDim.x) {
+      const int bw = (int)__hip_atomic_load(&state[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint64_t word = ((uint64_t)tag << 32) | (uint32_t)md_result(bw, tmax);
+      __hip_atomic_store(result, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&state[0], 64u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&state[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
 }
 
 // ------------------------------------------------------------- sampled ----
@@ -851,18 +876,19 @@ hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
   return hipGetLastError();
 }
 
-hipError_t launch_md_single(bool thresholded, const uint32_t* d_rows, int w, int L,
-                            int check_log2, int thr, uint32_t* d_best, hipStream_t s) {
+hipError_t launch_md_single(bool thresholded, const SingleRows& rows, int w, int L,
+                            int check_log2, int thr, int tmax, uint32_t tag, uint32_t* d_state,
+                            uint64_t* h_result, hipStream_t s) {
   const uint64_t grid = code_chunks(w, L);
-  if (grid == 0) return hipSuccess;
+  if (grid == 0 || w > kMaxKExact - 1) return hipErrorInvalidValue;
   if (grid >= (1ull << 31)) return hipErrorInvalidValue;
   const uint32_t cmask = (1u << check_log2) - 1u;
   if (thresholded)
-    hipLaunchKernelGGL(md_single_kernel<true>, dim3((unsigned)grid), dim3(64), 0, s, d_rows, w,
-                       L, cmask, thr, d_best);
+    hipLaunchKernelGGL(md_single_kernel<true>, dim3((unsigned)grid), dim3(64), 0, s, rows, w, L,
+                       cmask, thr, tmax, tag, d_state, h_result);
   else
-    hipLaunchKernelGGL(md_single_kernel<false>, dim3((unsigned)grid), dim3(64), 0, s, d_rows, w,
-                       L, cmask, thr, d_best);
+    hipLaunchKernelGGL(md_single_kernel<false>, dim3((unsigned)grid), dim3(64), 0, s, rows, w, L,
+                       cmask, thr, tmax, tag, d_state, h_result);
   return hipGetLastError();
 }
 
