@@ -4,11 +4,13 @@
 Headline (value): exact min-distance codes/s on the mixed-k batch — 4096
 seeded codes, k uniform in 3..8, n = 49 — one step = one device-resident
 gt_md_batch_dev pass over all 4096 codes (rows and plan, then the persistent
-walk, which writes each code's result as its last chunk finishes).  Also
-reported in the same JSON line: the plain one-launch-per-code form on the
-same codes, AGL2(F7) canonical forms/s on 10^6 mixed-size masks,
-the INT32 VALU roofline of the walk kernel (HIP events around every launch), and
-the CPU oracle timed on the host cores.
+walk, which writes each code's result as its last chunk finishes), the steps
+issued round-robin on three contexts (streams) so that consecutive steps
+overlap on the device.  Also reported in the same JSON line: the same steps on
+one stream, the plain one-launch-per-code form on the same codes, AGL2(F7)
+canonical forms/s on 10^6 mixed-size masks, the INT32 VALU roofline of the
+walk kernel (timed alone, on one stream, by HIP events that every launch's own
+dispatch records), and the CPU oracle timed on the host cores.
 """
 
 from __future__ import annotations
@@ -104,31 +106,47 @@ def ref_polys(k: int) -> int:
     return 8 ** k - 1
 
 
-def bench_md(ctx, sets, steps: int, warmup: int, expect=None, before_warmup=None):
-    """before_warmup (the attainable-rate probe: about 40 ms of full-device
-    VALU work) runs right before the warm-up steps, once the inputs are on
-    the device: the device leaves its idle clocks over the first milliseconds
-    of work, and without it a short run (--steps 20 --warmup 5) times md_main
+def bench_md(ctxs, sets, steps: int, warmup: int, expect=None, before_warmup=None):
+    """Time gt_md_batch_dev steps over one batch, inputs resident on the device.
+
+    * one stream: `steps` steps on ctxs[0], each md_main launch timed by HIP
+      events recorded by its own dispatch (the kernel alone on the device:
+      the steps of one stream never overlap) - the roofline's kernel time;
+    * pipelined (when len(ctxs) > 1): the same steps issued round-robin on the
+      contexts, each with its own stream, scratch and output, so that one
+      step's rows and plan and the start of its walk overlap the end of the
+      previous step's walk;
+    * graph replay: the step captured once as a hipGraph on ctxs[0].
+
+    before_warmup (the attainable-rate probe: about 40 ms of full-device VALU
+    work) runs right before the warm-up steps, once the inputs are on the
+    device: the device leaves its idle clocks over the first milliseconds of
+    work, and without it a short run (--steps 20 --warmup 5) times md_main
     about 9 % slower than the 200 + 200 default."""
     from gtoric.distance import flatten
+    ctx = ctxs[0]
     flat, offs = flatten(sets)
     n = len(sets)
     d_idx = ctx.to_device(flat)
     d_off = ctx.to_device(offs)
-    d_out = ctx.array(n, np.int32)
+    outs = [c.array(n, np.int32) for c in ctxs]
     if before_warmup is not None:
         before_warmup()
 
-    def step():
-        ctx.md_batch_dev(d_idx.ptr, len(flat), d_off.ptr, n, d_out.ptr, tmax=48,
-                         thresholded=False)
+    def step(i=0):
+        ctxs[i].md_batch_dev(d_idx.ptr, len(flat), d_off.ptr, n, outs[i].ptr, tmax=48,
+                             thresholded=False)
+
+    def check(got, what):
+        if expect is not None and not np.array_equal(got, expect):
+            raise SystemExit(f"parity failure ({what}): {int(np.sum(got != expect))} codes "
+                             "differ from the oracle")
 
     for _ in range(warmup):
         step()
     ctx.sync()
-    got = d_out.download()
-    if expect is not None and not np.array_equal(got, expect):
-        raise SystemExit(f"parity failure: {int(np.sum(got != expect))} codes differ from the oracle")
+    got = outs[0].download()
+    check(got, "one stream, warm-up")
     ctx.prof_read("md_main")
     ctx.prof_enable(True)
     ctx.sync()
@@ -139,6 +157,24 @@ def bench_md(ctx, sets, steps: int, warmup: int, expect=None, before_warmup=None
     dt = time.perf_counter() - t0
     ctx.prof_enable(False)
     nk, kms = ctx.prof_read("md_main")
+    check(outs[0].download(), "one stream")
+
+    dt_pipe = None
+    if len(ctxs) > 1:
+        m = len(ctxs)
+        for i in range(warmup):
+            step(i % m)
+        for c in ctxs:
+            c.sync()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            step(i % m)
+        for c in ctxs:
+            c.sync()
+        dt_pipe = time.perf_counter() - t0
+        for i in range(m):
+            check(outs[i].download(), f"pipelined, context {i}")
+
     # The same step captured once as a hipGraph and replayed (how the
     # evolutionary driver runs its generations): two kernels, one launch.
     ctx.capture_begin()
@@ -152,13 +188,12 @@ def bench_md(ctx, sets, steps: int, warmup: int, expect=None, before_warmup=None
         graph.launch()
     ctx.sync()
     dt_graph = time.perf_counter() - t0
-    got_g = d_out.download()
+    got_g = outs[0].download()
     graph.close()
-    if expect is not None and not np.array_equal(got_g, expect):
-        raise SystemExit("parity failure in the graph replay")
-    for a in (d_idx, d_off, d_out):
+    check(got_g, "graph replay")
+    for a in (d_idx, d_off, *outs):
         a.close()
-    return dt, nk, kms, got, dt_graph
+    return dt, nk, kms, got, dt_graph, dt_pipe
 
 
 def bench_plain(ctx, sets, expect=None):
@@ -500,6 +535,10 @@ def main() -> None:
     # part of that ramp.
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=200)
+    # Contexts (streams) the headline steps are issued on, round-robin: three
+    # measured best on MI355X (one step's rows, plan and walk start overlap
+    # the end of the previous walk); 1 times the one-stream form only.
+    ap.add_argument("--streams", type=int, default=3)
     ap.add_argument("--canon-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline")
     ap.add_argument("--skip", default="",
@@ -546,16 +585,27 @@ def main() -> None:
     expect = np.load(gold)["c2_mz"] if gold.exists() else None
 
     ceil = {}
-    dt, nk, kms, got, dt_graph = bench_md(ctx, sets, args.steps, args.warmup, expect,
-                                          before_warmup=lambda: ceil.update(walk_ceiling()))
-    value = n * args.steps / dt
-    ms_step = dt / args.steps * 1e3
+    ctxs = [ctx]
+    for _ in range(max(args.streams, 1) - 1):
+        c = _native.Context(0)
+        if "tune" not in skip:
+            autotune.apply(c)
+        ctxs.append(c)
+    dt, nk, kms, got, dt_graph, dt_pipe = bench_md(
+        ctxs, sets, args.steps, args.warmup, expect,
+        before_warmup=lambda: ceil.update(walk_ceiling()))
+    value_one = n * args.steps / dt
+    dt_value = dt_pipe if dt_pipe is not None else dt
+    value = n * args.steps / dt_value
+    ms_step = dt_value / args.steps * 1e3
     kern_ms = kms / max(nk, 1)
     nominal = ctx.cu_count * LANES_PER_CU_CLK * ctx.clock_khz * 1e3 / 1e12   # T int32 op/s
     peak = ceil["T_int32_ops_per_s"]
     achieved = OPS_PER_CODEWORD * codewords / (kern_ms * 1e-3) / 1e12
-    log(f"[bench] batched: {value:,.0f} codes/s, {ms_step:.3f} ms/step, md_main {kern_ms:.3f} ms, "
-        f"{achieved:.2f} TOP/s of {peak:.2f} attainable ({nominal:.2f} nominal)")
+    log(f"[bench] batched, one stream: {value_one:,.0f} codes/s, {dt / args.steps * 1e3:.3f} "
+        f"ms/step, md_main {kern_ms:.3f} ms, {achieved:.2f} TOP/s of {peak:.2f} attainable "
+        f"({nominal:.2f} nominal)")
+    log(f"[bench] batched, {len(ctxs)} streams: {value:,.0f} codes/s, {ms_step:.3f} ms/step")
 
     result = {
         "metric": METRIC,
@@ -572,9 +622,11 @@ def main() -> None:
         "data": "synthetic",
         "config": {
             "workload": "batched exact min distance, 4096 GF(8) toric codes n=49, "
-                        "k uniform in 3..8 (numpy default_rng(2)), one device-resident launch "
-                        "sequence per step",
+                        "k uniform in 3..8 (numpy default_rng(2)), one device-resident "
+                        "gt_md_batch_dev pass (rows and plan, walk) per step, the steps issued "
+                        "round-robin on `streams` contexts",
             "codes": n,
+            "streams": len(ctxs),
             "walked_codewords": codewords,
             "reference_polys": ref_work,
             "parity": "bit-exact vs oracle fixture" if expect is not None else "unchecked",
@@ -600,6 +652,13 @@ def main() -> None:
                                              / nominal, 4),
             "kernel_ms": round(kern_ms, 4),
             "kernel_launches": nk,
+            "timed_in": "the one-stream region (md_main alone on the device; HIP events "
+                        "recorded by each launch's own dispatch)",
+            "step_achieved": round(OPS_PER_CODEWORD * codewords / (ms_step * 1e-3) / 1e12, 3),
+            "step_frac": round(OPS_PER_CODEWORD * codewords / (ms_step * 1e-3) / 1e12 / peak, 4),
+            "step_note": "step_achieved: the walk's int32 ops per step over the whole step time "
+                         "of the value's region (rows, plan and walk of the steps overlapped over "
+                         "the streams), against the same attainable peak",
             "note": "peak_nominal is the nominal int32 VALU rate (16 lanes per SIMD per "
                     "clock, 4 clocks per wave64 instruction); gfx950 issues the walk's VOP2 "
                     "XORs with two VGPR operands faster than that, so the attainable rate of "
@@ -608,6 +667,12 @@ def main() -> None:
         },
     }
 
+    result["one_stream"] = {
+        "value": round(value_one, 1), "unit": "codes/s",
+        "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "form": "the same steps on one context and stream (each step's kernels after the "
+                "previous step's), md_main bracketed by HIP events",
+    }
     result["graph_replay"] = {
         "value": round(n * args.steps / dt_graph, 1), "unit": "codes/s",
         "ms_per_step": round(dt_graph / args.steps * 1e3, 4),
@@ -620,7 +685,7 @@ def main() -> None:
     if "wide" not in skip:
         wsets = workloads.mixed_k(4096, 3, 10, seed=12)
         wcw = sum(proj(len(s)) for s in wsets)
-        wdt, wnk, wkms, wgot, _ = bench_md(ctx, wsets, 3, 1, None)
+        wdt, wnk, wkms, wgot, _, _ = bench_md([ctx], wsets, 3, 1, None)
         wk = wkms / max(wnk, 1)
         wgold = ROOT / "tests" / "golden" / "md_wide_golden.npz"
         wparity = "unchecked"

@@ -342,12 +342,15 @@ struct gt_ctx {
     GT_HIP(launch_md_prep(d_idx, n_idx, mplanes.as<uint32_t>(), d_rows, d_offsets, n,
                           params_buf.as<MdParams>(), d_plan, d_order, d_roff, d_claims, d_state,
                           d_out, d_tmax, tmax_s, ctrl_p, s));
-    hipEvent_t e0 = nullptr;
-    if (profile) GT_OK(prof_begin("md_main", s, &e0));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (profile && prof && !bstate.frozen) {
+      GT_HIP(hipEventCreate(&e0));
+      GT_HIP(hipEventCreate(&e1));
+    }
     GT_HIP(launch_md_main(thresholded, md_grid[thresholded ? 1 : 0], d_rows, d_plan, d_order,
                           d_roff, d_claims, d_state, d_tmax, tmax_s, d_out,
-                          params_buf.as<MdParams>(), s));
-    if (profile) GT_OK(prof_end("md_main", s, e0));
+                          params_buf.as<MdParams>(), s, e0, e1));
+    if (e0) prof_rec["md_main"].ev.emplace_back(e0, e1);
     return 0;
   }
 };

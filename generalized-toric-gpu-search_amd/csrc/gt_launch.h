@@ -34,7 +34,8 @@ hipError_t launch_md_prep(const int32_t* d_idx, int64_t n_idx, const uint32_t* d
 hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
                           const MdPlan* d_plan, const uint32_t* d_order, const uint32_t* d_roff,
                           uint32_t* d_claims, uint64_t* d_state, const int32_t* d_tmax,
-                          int32_t tmax, int32_t* d_out, const MdParams* d_params, hipStream_t s);
+                          int32_t tmax, int32_t* d_out, const MdParams* d_params, hipStream_t s,
+                          hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 // One code, one launch: grid = code_chunks(w, L) single-wave workgroups, the
 // 3w rows (w <= 20) in the kernel arguments.  d_state: {64, 0} before the
 // first launch; each launch leaves it so.  h_result: a host-visible word

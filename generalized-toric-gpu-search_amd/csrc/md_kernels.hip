@@ -33,6 +33,8 @@
 //   OR), 2 v_bcnt_u32_b32, and one v_min3_u32 per two codewords.
 #include <cstddef>
 
+#include <hip/hip_ext.h>
+
 #include "gt_launch.h"
 #include "gt_walk.h"
 
@@ -112,79 +114,80 @@ __global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict_
 }
 
 // Largest-first plan (MdPlan, order[] = code index and roff[] = first row
-// per position).  One workgroup of 16 waves takes the codes in batches of
-// 4096, four per thread; their sizes are loaded together (one memory latency
-// per batch) and their bucket ids kept in registers.  Pass 1 counts the
-// buckets: per wave, one ballot per round and bucket present in the wave,
-// the counts gathered in lane b for bucket b, and one LDS atomic
-// instruction (lane b on bucket b).  One wave turns the 21 counts into bucket
-// bases, chunk and unit offsets (lane b sums the entries below b from LDS).
-// Pass 2 places the codes: one LDS atomic instruction per wave reserves its
-// positions in every bucket's cursor, and mbcnt of each round's ballot gives
-// a lane its place in them.  The order inside a bucket therefore follows the
-// waves' atomics, not the batch order; no kernel depends on it, each code's
-// result being its own.
-// (A plan that kept the batch order with per-round prefix sums over the
-// waves took 14.7 us for 4096 codes, against 1.7 us for the rows.)
-constexpr int kPlanThreads = 1024;
-constexpr int kPlanRounds = 4;
+// per position).  One workgroup of 256 threads takes the codes in batches of
+// 4096, 16 consecutive codes per thread: the 17 offsets they span are loaded
+// together (one memory latency per batch) and kept in registers.  Every
+// thread counts its codes per bucket in its own LDS column (hist[b][thread]:
+// the threads of a wave hit consecutive words, so the adds never conflict).
+// Scan threads (bucket, segment of 22 columns) sum their segments; one wave
+// turns the bucket totals into bucket bases, chunk and unit offsets; the scan
+// threads then rewrite every column entry as that thread's first position in
+// the bucket; pass 2 takes each code's position with a returning add on its
+// own column entry.  Positions inside a bucket therefore follow the thread,
+// then the batch, then the code: for one batch, the batch order.
+// Workgroups of 256 threads and at most 72 VGPRs: with md_main's five
+// 88-VGPR waves on every SIMD, a wave of each still fits, so the prep of a
+// batch issued on another stream runs beside a walk instead of after it.
+// (Per-wave ballots over the buckets present cost 16 rounds x buckets per
+// pass at this width: 10 us more per 4096 codes.)
+constexpr int kPlanThreads = 256;
+constexpr int kPlanRounds = 16;
 constexpr int kPlanBatch = kPlanThreads * kPlanRounds;
+constexpr int kPlanSegs = 12;                                             // segments per bucket
+constexpr int kPlanSegLen = (kPlanThreads + kPlanSegs - 1) / kPlanSegs;   // columns per segment
+static_assert(kMaxKExact * kPlanSegs <= kPlanThreads, "one scan thread per bucket segment");
 
 // Result of a code whose minimum weight is b (max_zeros clamped at tmax + 1).
 __device__ __forceinline__ int32_t md_result(int b, int tmax) {
   return (b <= 48 - tmax) ? tmax + 1 : 49 - b;
 }
 
-// This thread's codes base + r * kPlanThreads + tid of one batch: bucket
-// (-1: none) and first index.
+// This thread's codes first + r, r < kPlanRounds, of one batch: the offsets
+// o[r] (first index of code first + r; o[kPlanRounds] ends the last) and bit
+// r set when code r has a walk.  Its bucket kMaxKExact - walk_dim(k) =
+// kMaxKExact + 1 - k, k = o[r + 1] - o[r] >= 2, is recomputed where needed,
+// which keeps the batch in 18 VGPRs.
 struct PlanCodes {
-  int j[kPlanRounds];
-  int32_t ofs[kPlanRounds];
+  int32_t o[kPlanRounds + 1];
+  uint32_t walk;
+  __device__ __forceinline__ bool has(int r) const { return (walk >> r) & 1u; }
+  __device__ __forceinline__ int bucket(int r) const { return kMaxKExact + 1 - (o[r + 1] - o[r]); }
 };
 
 __device__ __forceinline__ void md_plan_load_batch(const int32_t* __restrict__ offsets, int32_t n,
                                                    int64_t n_idx, int base, uint64_t* state,
                                                    int32_t* out, const int32_t* tmax_arr,
                                                    int32_t tmax_s, bool first_pass,
-                                                   uint32_t& err, uint32_t& present,
-                                                   PlanCodes& pc) {
-  const int tid = threadIdx.x;
-  int kk[kPlanRounds];
-  bool inside[kPlanRounds];
+                                                   uint32_t& err, PlanCodes& pc) {
+  const int first = base + (int)threadIdx.x * kPlanRounds;
+  // One base address and immediate offsets (no per-round 64-bit addresses).
+  const int32_t* po = offsets + first;
+  uint64_t* ps = state + first;
+  if (first + kPlanRounds <= n) {
 #pragma unroll
-  for (int r = 0; r < kPlanRounds; ++r) {
-    const int i = base + r * kPlanThreads + tid;
-    const int o0 = (i < n) ? offsets[i] : 0, o1 = (i < n) ? offsets[i + 1] : 0;
-    pc.ofs[r] = o0;
-    kk[r] = o1 - o0;
-    inside[r] = o0 >= 0 && (int64_t)o1 <= n_idx;   // rows and indices of the code in range
+    for (int r = 0; r <= kPlanRounds; ++r) pc.o[r] = po[r];
+  } else {
+#pragma unroll
+    for (int r = 0; r <= kPlanRounds; ++r) pc.o[r] = (first + r <= n) ? po[r] : 0;
   }
+  pc.walk = 0u;
 #pragma unroll
   for (int r = 0; r < kPlanRounds; ++r) {
-    const int i = base + r * kPlanThreads + tid;
-    int j = -1;
+    const int i = first + r;
     if (i < n) {
-      const int k = kk[r];
-      const bool bad = k < 0 || k > kMaxKExact || !inside[r];
+      const int k = pc.o[r + 1] - pc.o[r];
+      // Rows and indices of the code in range.
+      const bool bad = k < 0 || k > kMaxKExact || pc.o[r] < 0 || (int64_t)pc.o[r + 1] > n_idx;
       if (bad) err |= kErrBadK;
-      else if (k > 1) j = kMaxKExact - walk_dim(k);
+      else if (k > 1) pc.walk |= 1u << r;
       if (first_pass) {
-        state[i] = 64ull;   // minimum weight 64, no chunk finished
+        ps[r] = 64ull;   // minimum weight 64, no chunk finished
         // No walk: k = 0 (and bad input) gives 0, k = 1 a monomial (weight 49).
-        if (j < 0) out[i] = (!bad && k == 1) ? md_result(49, tmax_arr ? tmax_arr[i] : tmax_s) : 0;
+        if (bad || k <= 1)
+          out[i] = (!bad && k == 1) ? md_result(49, tmax_arr ? tmax_arr[i] : tmax_s) : 0;
       }
     }
-    pc.j[r] = j;
-    if (j >= 0) present |= 1u << j;
   }
-  // Buckets present in the wave.
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) present |= (uint32_t)__shfl_xor((int)present, off, kWave);
-  present = __builtin_amdgcn_readfirstlane(present);
-}
-
-__device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
-  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets, int32_t n,
@@ -197,48 +200,56 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
                                              const int32_t* tmax_arr, int32_t tmax_s,
                                              uint32_t* ctrl) {
   constexpr int kB = kMaxKExact;
-  __shared__ uint32_t cnt[kB];   // pass 1: codes per bucket; pass 2: next free position
+  __shared__ uint32_t cnt[kB];   // codes per bucket, then bucket bases
   __shared__ uint64_t sxc[kB], sxu[kB];
+  // Per bucket and thread: its codes, then its next position.
+  __shared__ uint32_t hist[kB][kPlanThreads];
+  __shared__ uint32_t segsum[kB][kPlanSegs];
   const int tid = threadIdx.x, wave = tid / kWave;
   const uint32_t lane = (uint32_t)(tid % kWave);
   uint32_t err = 0;
   // Wave 0's lane b reads bucket b's walk length now, with the sizes.
   const int Lb = (tid < kB) ? (int)p->L[min(kMaxKExact - tid + 1, kMaxKExact)] : 0;
   if (tid < kClaimShards) claims[tid * kClaimStride] = 0u;
-  if (tid < kB) cnt[tid] = 0u;
-  PlanCodes pc;
-  uint32_t present = 0;
-  md_plan_load_batch(offsets, n, n_idx, 0, state, out, tmax_arr, tmax_s, true, err, present, pc);
-  // Wave 0's lane b: chunks and work units per code of bucket b (no counts
-  // needed yet), before the barrier rather than on the scan's path.
-  const int wb = kMaxKExact - tid;   // walk dimension of bucket tid (k = wb + 1)
-  const uint64_t cc = (tid < kB) ? code_chunks(wb, Lb) : 0ull;
-  const uint64_t cu = (tid < kB) ? code_units(wb) : 0ull;
-  __syncthreads();
-
-  // Pass 1: bucket sizes.
-  for (int base = 0;;) {
-    uint32_t mine = 0;   // lane b: this wave's codes in bucket b
-    for (uint32_t rem = present; rem != 0; rem &= rem - 1) {
-      const int b = __builtin_ctz(rem);
-      uint32_t c = 0;
 #pragma unroll
-      for (int r = 0; r < kPlanRounds; ++r) c += (uint32_t)__popcll(__ballot(pc.j[r] == b));
-      mine = (lane == (uint32_t)b) ? c : mine;
-    }
-    if (mine != 0u) atomicAdd(&cnt[lane], mine);
+  for (int b = 0; b < kB; ++b) hist[b][tid] = 0u;   // own column: no barrier before own adds
+  PlanCodes pc;
+  md_plan_load_batch(offsets, n, n_idx, 0, state, out, tmax_arr, tmax_s, true, err, pc);
+
+  // Pass 1: this thread's codes per bucket (non-returning LDS adds).
+  for (int base = 0;;) {
+#pragma unroll
+    for (int r = 0; r < kPlanRounds; ++r)
+      if (pc.has(r)) atomicAdd(&hist[pc.bucket(r)][tid], 1u);
     base += kPlanBatch;
     if (base >= n) break;
-    present = 0;
-    md_plan_load_batch(offsets, n, n_idx, base, state, out, tmax_arr, tmax_s, true, err, present,
-                       pc);
+    md_plan_load_batch(offsets, n, n_idx, base, state, out, tmax_arr, tmax_s, true, err, pc);
+  }
+  __syncthreads();
+
+  // Segment sums: scan thread (sb, sg) takes columns [c0, c0 + kPlanSegLen).
+  const int sb = tid / kPlanSegs, sg = tid % kPlanSegs, c0 = sg * kPlanSegLen;
+  if (sb < kB) {
+    uint32_t v = 0;
+#pragma unroll
+    for (int c = 0; c < kPlanSegLen; ++c)
+      if (c0 + c < kPlanThreads) v += hist[sb][c0 + c];
+    segsum[sb][sg] = v;
   }
   __syncthreads();
 
   // Bucket bases, chunk and unit offsets (wave 0, lane b = bucket b).
   if (wave == 0) {
     const int b = (int)lane;
-    const uint32_t t = (b < kB) ? cnt[b] : 0u;
+    // Chunks and work units per code of bucket b (walk dimension kB - b).
+    const uint64_t cc = (b < kB) ? code_chunks(kB - b, Lb) : 0ull;
+    const uint64_t cu = (b < kB) ? code_units(kB - b) : 0ull;
+    uint32_t t = 0;
+    if (b < kB) {
+#pragma unroll
+      for (int g = 0; g < kPlanSegs; ++g) t += segsum[b][g];
+      cnt[b] = t;
+    }
     // Chunks of the bucket, clamped at 2^31 (a plan of 2^31 chunks or more
     // is refused), so the sum of 21 cannot wrap.
     const uint64_t xc = (t == 0) ? 0ull : (cc >= (1ull << 31)) ? (1ull << 31)
@@ -270,8 +281,10 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
       plan->cb[b] = (uint32_t)min(ec, 1ull << 31);
       plan->pb[b] = ep;
       plan->cc[b] = (uint32_t)cc;
-      cnt[b] = ep;
     }
+    // Every lane has read cnt[] above (the loads are ordered before these
+    // stores within the wave).
+    if (b < kB) cnt[b] = ep;
     if (b == kB - 1) {
       const uint64_t chunks = ec + xc;
       const bool overflow = chunks >= (1ull << 31);
@@ -285,40 +298,44 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
   }
   __syncthreads();
 
-  // Pass 2: positions.  The last batch (the only one when n <= 4096) is
-  // still in registers; earlier ones are loaded again.
+  // Each column entry becomes that thread's first position in the bucket.
+  if (sb < kB) {
+    uint32_t at = cnt[sb];
+#pragma unroll
+    for (int g = 0; g < kPlanSegs; ++g) at += (g < sg) ? segsum[sb][g] : 0u;
+    // Two halves of loads in flight (registers: the batch stays live).
+    constexpr int kHalf = (kPlanSegLen + 1) / 2;
+#pragma unroll
+    for (int h = 0; h < kPlanSegLen; h += kHalf) {
+      uint32_t v[kHalf];
+#pragma unroll
+      for (int c = 0; c < kHalf; ++c)
+        v[c] = (h + c < kPlanSegLen && c0 + h + c < kPlanThreads) ? hist[sb][c0 + h + c] : 0u;
+#pragma unroll
+      for (int c = 0; c < kHalf; ++c) {
+        if (h + c < kPlanSegLen && c0 + h + c < kPlanThreads) hist[sb][c0 + h + c] = at;
+        at += v[c];
+      }
+    }
+  }
+  __syncthreads();
+
+  // Pass 2: positions (returning adds on the own column).  The last batch
+  // (the only one when n <= 4096) is still in registers; earlier ones are
+  // loaded again.
   for (int base = (n > kPlanBatch) ? 0 : -1;;) {
     if (base >= 0) {
       if (base >= n) break;
-      present = 0;
       uint32_t unused = 0;
-      md_plan_load_batch(offsets, n, n_idx, base, state, out, tmax_arr, tmax_s, false, unused,
-                         present, pc);
+      md_plan_load_batch(offsets, n, n_idx, base, state, out, tmax_arr, tmax_s, false, unused, pc);
     }
-    const int b0 = (base >= 0) ? base : 0;
-    // Lane b reserves this wave's positions in bucket b.
-    uint32_t mine = 0;
-    for (uint32_t rem = present; rem != 0; rem &= rem - 1) {
-      const int b = __builtin_ctz(rem);
-      uint32_t c = 0;
+    const int first = ((base >= 0) ? base : 0) + tid * kPlanRounds;
 #pragma unroll
-      for (int r = 0; r < kPlanRounds; ++r) c += (uint32_t)__popcll(__ballot(pc.j[r] == b));
-      mine = (lane == (uint32_t)b) ? c : mine;
-    }
-    uint32_t first = 0;
-    if (mine != 0u) first = atomicAdd(&cnt[lane], mine);
-    for (uint32_t rem = present; rem != 0; rem &= rem - 1) {
-      const int b = __builtin_ctz(rem);
-      uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)first, b);
-#pragma unroll
-      for (int r = 0; r < kPlanRounds; ++r) {
-        const uint64_t m = __ballot(pc.j[r] == b);
-        if (pc.j[r] == b) {
-          const uint32_t pos = at + mbcnt64(m);
-          order[pos] = (uint32_t)(b0 + r * kPlanThreads + tid);
-          roff[pos] = (uint32_t)pc.ofs[r];
-        }
-        at += (uint32_t)__popcll(m);
+    for (int r = 0; r < kPlanRounds; ++r) {
+      if (pc.has(r)) {
+        const uint32_t pos = atomicAdd(&hist[pc.bucket(r)][tid], 1u);
+        order[pos] = (uint32_t)(first + r);
+        roff[pos] = (uint32_t)pc.o[r];
       }
     }
     if (base < 0) break;
@@ -329,7 +346,7 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
 
 // Plan and bit rows in one launch: workgroup 0 builds the plan, workgroups
 // 1.. write the rows, one wave per code (the two are independent).
-__global__ __launch_bounds__(kPlanThreads) void md_prep_kernel(
+__global__ __launch_bounds__(kPlanThreads, 7) void md_prep_kernel(
     const int32_t* __restrict__ idx, int64_t n_idx, const uint32_t* __restrict__ mplanes,
     uint32_t* __restrict__ rows, const int32_t* __restrict__ offsets, int32_t n,
     const MdParams* __restrict__ p, MdPlan* __restrict__ plan, uint32_t* __restrict__ order,
@@ -866,13 +883,17 @@ int md_main_blocks_per_cu(bool thresholded) {
 hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
                           const MdPlan* d_plan, const uint32_t* d_order, const uint32_t* d_roff,
                           uint32_t* d_claims, uint64_t* d_state, const int32_t* d_tmax,
-                          int32_t tmax, int32_t* d_out, const MdParams* d_params, hipStream_t s) {
+                          int32_t tmax, int32_t* d_out, const MdParams* d_params, hipStream_t s,
+                          hipEvent_t ev_start, hipEvent_t ev_stop) {
+  // With events, the launch records them from its own dispatch (start and
+  // end of the kernel), without marker packets of their own on the stream.
   if (thresholded)
-    hipLaunchKernelGGL(md_main_thr_kernel, dim3(grid), dim3(256), 0, s, d_rows, d_plan, d_order,
-                       d_roff, d_claims, d_state, d_tmax, tmax, d_out, d_params);
+    hipExtLaunchKernelGGL(md_main_thr_kernel, dim3(grid), dim3(256), 0, s, ev_start, ev_stop, 0,
+                          d_rows, d_plan, d_order, d_roff, d_claims, d_state, d_tmax, tmax, d_out,
+                          d_params);
   else
-    hipLaunchKernelGGL(md_main_exact_kernel, dim3(grid), dim3(256), 0, s, d_rows, d_plan,
-                       d_order, d_roff, d_state, d_tmax, tmax, d_out, d_params);
+    hipExtLaunchKernelGGL(md_main_exact_kernel, dim3(grid), dim3(256), 0, s, ev_start, ev_stop, 0,
+                          d_rows, d_plan, d_order, d_roff, d_state, d_tmax, tmax, d_out, d_params);
   return hipGetLastError();
 }
 
