@@ -404,10 +404,7 @@ int gt_open(int32_t device, gt_ctx** out) {
     return bail(fail("hipStreamCreate failed"));
   if (hipEventCreateWithFlags(&c->handover, hipEventDisableTiming) != hipSuccess)
     return bail(fail("hipEventCreate failed"));
-  for (int k = 0; k <= kMaxKExact; ++k) {
-    c->params.L[k] = (uint8_t)default_L(k);
-    c->params.check_log2[k] = 2;
-  }
+  for (int k = 0; k <= kMaxKExact; ++k) md_params_set(c->params, k, default_L(k), 2);
   c->md_grid[0] = c->cu_count * md_main_blocks_per_cu(false);
   c->md_grid[1] = c->cu_count * md_main_blocks_per_cu(true);
   if (scratch_address(device, &c->scratch_dev)) return bail(-1);
@@ -465,8 +462,7 @@ int gt_set_md_params(gt_ctx* c, int32_t k, int32_t L, int32_t check_log2) {
   if (check_log2 < 0 || check_log2 > 16) return fail("check_log2=%d outside 0..16", check_log2);
   std::lock_guard<std::mutex> lk(c->mu);
   if (c->bstate.frozen) return fail("launch parameters changed during graph capture");
-  c->params.L[k] = (uint8_t)L;
-  c->params.check_log2[k] = (uint8_t)check_log2;
+  md_params_set(c->params, k, L, check_log2);
   c->params_dirty = true;
   // Uploaded now (ordered on the context stream), so no later call has to.
   GT_OK(c->activate());

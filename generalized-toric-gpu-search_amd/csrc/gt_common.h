@@ -62,6 +62,9 @@ GT_HD int walk_dim(int k) { return (k <= 1) ? 0 : k - 1; }
 struct MdParams {
   uint8_t L[kMaxKExact + 1];      // Gray walk bits per lane for full chunks
   uint8_t check_log2[kMaxKExact + 1];  // early-exit check every 2^x blocks
+  // Chunks of a code of k generators at L[k] (code_chunks), saturated at
+  // 2^31; kept with L by md_params_set.
+  uint32_t chunks[kMaxKExact + 1];
 };
 
 // Chunk plan of one batch, largest codes first.  Bucket j (0 <= j < kMaxKExact)
@@ -116,6 +119,14 @@ GT_HD uint64_t code_chunks(int w, int L) {
   return n;
 }
 
+// L and check interval of k, with the chunk count the plan reads.
+inline void md_params_set(MdParams& p, int k, int L, int check_log2) {
+  p.L[k] = (uint8_t)L;
+  p.check_log2[k] = (uint8_t)check_log2;
+  const uint64_t c = code_chunks(walk_dim(k), L);
+  p.chunks[k] = (c >= (1ull << 31)) ? (1u << 31) : (uint32_t)c;
+}
+
 // Work units of the dealt walk.  A block is 16 Gray steps of all 64 lanes
 // (1024 codewords, 174 VALU instructions).  Piece j > kTinyTop is walked
 // lane-major: lane l owns the Gray indices l * 2^(3j-6) + [0, 2^(3j-6)), and
@@ -132,12 +143,16 @@ constexpr int kBlockBits = kLaneBits + kU;
 constexpr int kSmallW = kTinyTop + 1;
 constexpr uint32_t kTinyUnits = 6;
 GT_HD uint64_t piece_blocks(int j) { return 1ull << (3 * j - kBlockBits); }
-// kTinyUnits + sum of 2^(3j-10) over j = kSmallW .. w-1, i.e.
-// (8^w - 8^kSmallW) / (7 * 2^10).
+// kTinyUnits + sum of 2^(3j-10) over j = kSmallW .. w-1 = kTinyUnits +
+// 2^(3 kSmallW - 10) * (8^m - 1) / 7, m = w - kSmallW, where (8^m - 1) / 7 is
+// the base-8 number of m ones: bits 0, 3, ..., 3(m-1), a mask (no division).
+constexpr uint64_t kOctalOnes = 0x9249249249249249ull;   // bits 0, 3, 6, ..., 63
+static_assert(3 * kSmallW >= kBlockBits, "pieces above the tiny part hold whole blocks");
 GT_HD uint64_t code_units(int w) {
   if (w <= 0) return 0;
   if (w <= kSmallW) return kTinyUnits;
-  return kTinyUnits + (((1ull << (3 * w)) - (1ull << (3 * kSmallW))) >> kBlockBits) / 7ull;
+  return kTinyUnits + ((kOctalOnes & ((1ull << (3 * (w - kSmallW))) - 1ull))
+                       << (3 * kSmallW - kBlockBits));
 }
 
 // Projective codewords of a w-dimensional code: (8^w - 1) / 7.

@@ -6,9 +6,9 @@
 // kernel_cuda_bp.py:99-127).  Work decomposition (see gt_common.h):
 //
 //   md_prep     workgroup 0: codes bucketed by walk dimension, largest first
-//               (MdPlan, order[], roff[]), the per-code state words and
-//               claim counters reset, the results of codes without a walk
-//               (k <= 1) written; workgroups 1..: one wave per code, the bit
+//               (MdPlan, order[], roff[]), claim counters reset;
+//               workgroups 1..: one wave per code, its state word reset, the
+//               result of a code without a walk (k <= 1) written, and the bit
 //               rows x^b * (g_i + g_{k-1}) of the subcode vanishing at the
 //               torus point (1,1) (gt_common.h), which has the same maximum
 //               number of zeros as C(S) with 1/8 of the codewords.  The
@@ -70,16 +70,32 @@ __device__ __forceinline__ void md_rows_one(int64_t t, const int32_t* __restrict
 // x^b * (g_i + g_{k-1}) at row 3i + b of the code.  Lanes i < k check their
 // index.  Codes with k < 2 have no walk; k > 21 and offsets outside
 // [0, n_idx] are flagged by the plan (and no row is written for them).
+// Result of a code whose minimum weight is b (max_zeros clamped at tmax + 1).
+__device__ __forceinline__ int32_t md_result(int b, int tmax) {
+  return (b <= 48 - tmax) ? tmax + 1 : 49 - b;
+}
+
+// The wave of a code also resets its state word and writes the result of a
+// code without a walk.
 __device__ __forceinline__ void md_rows_code(int code, int32_t n, int64_t n_idx,
                                              const int32_t* __restrict__ idx,
                                              const int32_t* __restrict__ offsets,
                                              const uint32_t* __restrict__ mplanes,
-                                             uint32_t* __restrict__ rows, uint32_t* ctrl) {
+                                             uint32_t* __restrict__ rows, uint64_t* state,
+                                             int32_t* out, const int32_t* tmax_arr,
+                                             int32_t tmax_s, uint32_t* ctrl) {
   if (code >= n) return;
   const uint32_t lane = lane_id();
   const int o = offsets[code];
   const int k = offsets[code + 1] - o;
-  if (k < 1 || k > kMaxKExact || o < 0 || (int64_t)o + k > n_idx) return;
+  const bool bad_k = k < 0 || k > kMaxKExact || o < 0 || (int64_t)o + k > n_idx;
+  if (lane == 0) {
+    state[code] = 64ull;   // minimum weight 64, no chunk finished
+    // No walk: k = 0 (and bad input) gives 0, k = 1 a monomial (weight 49).
+    if (bad_k || k <= 1)
+      out[code] = (!bad_k && k == 1) ? md_result(49, tmax_arr ? tmax_arr[code] : tmax_s) : 0;
+  }
+  if (k < 1 || bad_k) return;
   const uint32_t i = lane / 3u, bpow = lane - 3u * i;
   int g = 0;
   bool bad = false;
@@ -100,9 +116,9 @@ __device__ __forceinline__ void md_rows_code(int code, int32_t n, int64_t n_idx,
   uint32_t w[6] = {a0.x ^ b0.x, a0.y ^ b0.y, a0.z ^ b0.z, a0.w ^ b0.w, a1.x ^ b1.x, a1.y ^ b1.y};
   if (bpow >= 1u) mul_x(w);
   if (bpow >= 2u) mul_x(w);
-  uint4* out = reinterpret_cast<uint4*>(rows + ((size_t)o * 3 + lane) * kRowWords);
-  out[0] = make_uint4(w[0], w[1], w[2], w[3]);
-  out[1] = make_uint4(w[4], w[5], 0u, 0u);
+  uint4* row = reinterpret_cast<uint4*>(rows + ((size_t)o * 3 + lane) * kRowWords);
+  row[0] = make_uint4(w[0], w[1], w[2], w[3]);
+  row[1] = make_uint4(w[4], w[5], 0u, 0u);
 }
 
 __global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict__ idx,
@@ -115,19 +131,28 @@ __global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict_
 
 // Largest-first plan (MdPlan, order[] = code index and roff[] = first row
 // per position).  One workgroup of 256 threads takes the codes in batches of
-// 4096, 16 consecutive codes per thread: the 17 offsets they span are loaded
-// together (one memory latency per batch) and kept in registers.  Every
-// thread counts its codes per bucket in its own LDS column (hist[b][thread]:
-// the threads of a wave hit consecutive words, so the adds never conflict).
-// Scan threads (bucket, segment of 22 columns) sum their segments; one wave
-// turns the bucket totals into bucket bases, chunk and unit offsets; the scan
-// threads then rewrite every column entry as that thread's first position in
-// the bucket; pass 2 takes each code's position with a returning add on its
-// own column entry.  Positions inside a bucket therefore follow the thread,
-// then the batch, then the code: for one batch, the batch order.
+// 4096, thread t the codes base + 256 r + t, r < 16, so every global access
+// of a round is one contiguous 1 KB run (a single CU issues all of them: 64
+// separate lines per instruction cost 4 us of a 10 us plan).  The offsets of
+// a batch are loaded together (one memory latency) and kept in registers
+// with the sizes packed five bits each.  Every thread counts its codes per
+// bucket in its own LDS column (hist[b][thread]: the threads of a wave hit
+// consecutive words, so the adds never conflict).  Scan threads (bucket,
+// segment of 22 columns) sum their segments; one wave turns the bucket
+// totals into bucket bases, chunk and unit offsets; the scan threads then
+// rewrite every column entry as that thread's first position in the bucket;
+// pass 2 takes each code's position with a returning add on its own column
+// entry.  For a call of at most 4096 codes the code indices and first rows
+// are staged in LDS by position (the rows in the column words, free by then)
+// and copied out in contiguous runs; larger calls store them where they
+// belong.
 // Workgroups of 256 threads and at most 72 VGPRs: with md_main's five
 // 88-VGPR waves on every SIMD, a wave of each still fits, so the prep of a
 // batch issued on another stream runs beside a walk instead of after it.
+// The column counts and the staging area are dynamic LDS (kPlanDynLds bytes
+// per workgroup, given at launch): the compiler sizes registers for the
+// occupancy that static LDS allows, and 29 KB of static LDS would let it
+// spend more than 72.
 // (Per-wave ballots over the buckets present cost 16 rounds x buckets per
 // pass at this width: 10 us more per 4096 codes.)
 constexpr int kPlanThreads = 256;
@@ -135,57 +160,58 @@ constexpr int kPlanRounds = 16;
 constexpr int kPlanBatch = kPlanThreads * kPlanRounds;
 constexpr int kPlanSegs = 12;                                             // segments per bucket
 constexpr int kPlanSegLen = (kPlanThreads + kPlanSegs - 1) / kPlanSegs;   // columns per segment
+constexpr int kPlanPack = 6;                                              // sizes per word
 static_assert(kMaxKExact * kPlanSegs <= kPlanThreads, "one scan thread per bucket segment");
+static_assert(kMaxKExact < 32, "a size fits five bits");
+constexpr size_t kPlanHistWords = (size_t)kMaxKExact * kPlanThreads;
+constexpr size_t kPlanDynLds = kPlanHistWords * 4 + ((size_t)kPlanBatch + 2) * 2;   // + spare slot
+static_assert(kPlanBatch <= 65536, "staged code indices fit 16 bits");
+static_assert(kPlanBatch + 1 <= kPlanHistWords, "staged first rows fit the column words");
 
-// Result of a code whose minimum weight is b (max_zeros clamped at tmax + 1).
-__device__ __forceinline__ int32_t md_result(int b, int tmax) {
-  return (b <= 48 - tmax) ? tmax + 1 : 49 - b;
-}
-
-// This thread's codes first + r, r < kPlanRounds, of one batch: the offsets
-// o[r] (first index of code first + r; o[kPlanRounds] ends the last) and bit
-// r set when code r has a walk.  Its bucket kMaxKExact - walk_dim(k) =
-// kMaxKExact + 1 - k, k = o[r + 1] - o[r] >= 2, is recomputed where needed,
-// which keeps the batch in 18 VGPRs.
+// This thread's codes base + r * kPlanThreads + tid of one batch: first
+// index o[r] and size k(r), 0 when the code has no walk (its bucket is
+// otherwise kMaxKExact - walk_dim(k) = kMaxKExact + 1 - k, k >= 2).
 struct PlanCodes {
-  int32_t o[kPlanRounds + 1];
-  uint32_t walk;
-  __device__ __forceinline__ bool has(int r) const { return (walk >> r) & 1u; }
-  __device__ __forceinline__ int bucket(int r) const { return kMaxKExact + 1 - (o[r + 1] - o[r]); }
+  int32_t o[kPlanRounds];
+  uint32_t kp[(kPlanRounds + kPlanPack - 1) / kPlanPack];
+  __device__ __forceinline__ uint32_t k(int r) const {
+    return (kp[r / kPlanPack] >> (5 * (r % kPlanPack))) & 31u;
+  }
+  __device__ __forceinline__ int bucket(int r) const { return kMaxKExact + 1 - (int)k(r); }
+  // The bucket, or 0 (never a code's: k <= kMaxKExact) for no walk.
+  __device__ __forceinline__ int slot(int r) const { return k(r) != 0u ? bucket(r) : 0; }
 };
 
 __device__ __forceinline__ void md_plan_load_batch(const int32_t* __restrict__ offsets, int32_t n,
-                                                   int64_t n_idx, int base, uint64_t* state,
-                                                   int32_t* out, const int32_t* tmax_arr,
-                                                   int32_t tmax_s, bool first_pass,
-                                                   uint32_t& err, PlanCodes& pc) {
-  const int first = base + (int)threadIdx.x * kPlanRounds;
-  // One base address and immediate offsets (no per-round 64-bit addresses).
-  const int32_t* po = offsets + first;
-  uint64_t* ps = state + first;
-  if (first + kPlanRounds <= n) {
-#pragma unroll
-    for (int r = 0; r <= kPlanRounds; ++r) pc.o[r] = po[r];
-  } else {
-#pragma unroll
-    for (int r = 0; r <= kPlanRounds; ++r) pc.o[r] = (first + r <= n) ? po[r] : 0;
-  }
-  pc.walk = 0u;
+                                                   int64_t n_idx, int base, uint32_t& err,
+                                                   PlanCodes& pc) {
+  const int tid = (int)threadIdx.x;
+  const uint32_t lane = (uint32_t)tid % kWave;
+  // offsets[i] for every code i of the thread (and for i = n: the end of
+  // code n - 1); a code's end is the next lane's value, and for lane 63 the
+  // value lane r of the wave loads as `nx` in round r.
+  const int wnext = base + (tid - (int)lane) + kWave;   // lane 63's next code in round 0
+  const int inx = wnext + (int)lane * kPlanThreads;
+  const int32_t nx = (lane < (uint32_t)kPlanRounds && inx <= n) ? offsets[inx] : 0;
 #pragma unroll
   for (int r = 0; r < kPlanRounds; ++r) {
-    const int i = first + r;
+    const int i = base + r * kPlanThreads + tid;
+    pc.o[r] = (i <= n) ? offsets[i] : 0;
+  }
+#pragma unroll
+  for (int q = 0; q < (kPlanRounds + kPlanPack - 1) / kPlanPack; ++q) pc.kp[q] = 0u;
+#pragma unroll
+  for (int r = 0; r < kPlanRounds; ++r) {
+    const int i = base + r * kPlanThreads + tid;
+    const int32_t down = __shfl_down(pc.o[r], 1, kWave);
+    const int32_t last = __builtin_amdgcn_readlane(nx, r);
+    const int32_t o1 = (lane == kWave - 1) ? last : down;
     if (i < n) {
-      const int k = pc.o[r + 1] - pc.o[r];
+      const int k = o1 - pc.o[r];
       // Rows and indices of the code in range.
-      const bool bad = k < 0 || k > kMaxKExact || pc.o[r] < 0 || (int64_t)pc.o[r + 1] > n_idx;
+      const bool bad = k < 0 || k > kMaxKExact || pc.o[r] < 0 || (int64_t)o1 > n_idx;
       if (bad) err |= kErrBadK;
-      else if (k > 1) pc.walk |= 1u << r;
-      if (first_pass) {
-        ps[r] = 64ull;   // minimum weight 64, no chunk finished
-        // No walk: k = 0 (and bad input) gives 0, k = 1 a monomial (weight 49).
-        if (bad || k <= 1)
-          out[i] = (!bad && k == 1) ? md_result(49, tmax_arr ? tmax_arr[i] : tmax_s) : 0;
-      }
+      else if (k > 1) pc.kp[r / kPlanPack] |= (uint32_t)k << (5 * (r % kPlanPack));
     }
   }
 }
@@ -195,35 +221,34 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
                                              MdPlan* __restrict__ plan,
                                              uint32_t* __restrict__ order,
                                              uint32_t* __restrict__ roff,
-                                             uint32_t* __restrict__ claims,
-                                             uint64_t* __restrict__ state, int32_t* out,
-                                             const int32_t* tmax_arr, int32_t tmax_s,
-                                             uint32_t* ctrl) {
+                                             uint32_t* __restrict__ claims, uint32_t* ctrl) {
   constexpr int kB = kMaxKExact;
-  __shared__ uint32_t cnt[kB];   // codes per bucket, then bucket bases
+  __shared__ uint32_t cnt[kB + 1];   // codes per bucket, then bucket bases; [kB]: positions
   __shared__ uint64_t sxc[kB], sxu[kB];
-  // Per bucket and thread: its codes, then its next position.
-  __shared__ uint32_t hist[kB][kPlanThreads];
   __shared__ uint32_t segsum[kB][kPlanSegs];
+  extern __shared__ uint32_t plan_lds[];
+  // Per bucket and thread: its codes, then its next position.
+  uint32_t(*hist)[kPlanThreads] = reinterpret_cast<uint32_t(*)[kPlanThreads]>(plan_lds);
+  // Code index by position (calls of at most 4096 codes).
+  uint16_t* stage = reinterpret_cast<uint16_t*>(plan_lds + kPlanHistWords);
   const int tid = threadIdx.x, wave = tid / kWave;
   const uint32_t lane = (uint32_t)(tid % kWave);
   uint32_t err = 0;
-  // Wave 0's lane b reads bucket b's walk length now, with the sizes.
-  const int Lb = (tid < kB) ? (int)p->L[min(kMaxKExact - tid + 1, kMaxKExact)] : 0;
+  // Wave 0's lane b reads the chunks per code of bucket b (k = kB + 1 - b;
+  // bucket 0, always empty, takes k = kB's) now, with the sizes.
+  const uint32_t ccb = (tid < kB) ? p->chunks[min(kB + 1 - tid, kB)] : 0u;
   if (tid < kClaimShards) claims[tid * kClaimStride] = 0u;
 #pragma unroll
   for (int b = 0; b < kB; ++b) hist[b][tid] = 0u;   // own column: no barrier before own adds
   PlanCodes pc;
-  md_plan_load_batch(offsets, n, n_idx, 0, state, out, tmax_arr, tmax_s, true, err, pc);
 
-  // Pass 1: this thread's codes per bucket (non-returning LDS adds).
-  for (int base = 0;;) {
+  // Pass 1: this thread's codes per bucket (non-returning LDS adds).  The
+  // last batch (the only one when n <= 4096) stays in registers.
+  for (int base = 0; base < n; base += kPlanBatch) {
+    md_plan_load_batch(offsets, n, n_idx, base, err, pc);
 #pragma unroll
-    for (int r = 0; r < kPlanRounds; ++r)
-      if (pc.has(r)) atomicAdd(&hist[pc.bucket(r)][tid], 1u);
-    base += kPlanBatch;
-    if (base >= n) break;
-    md_plan_load_batch(offsets, n, n_idx, base, state, out, tmax_arr, tmax_s, true, err, pc);
+    for (int r = 0; r < kPlanRounds; ++r)   // no walk: adds 0 to bucket 0 (never a code's)
+      atomicAdd(&hist[pc.slot(r)][tid], pc.k(r) != 0u ? 1u : 0u);
   }
   __syncthreads();
 
@@ -241,8 +266,9 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
   // Bucket bases, chunk and unit offsets (wave 0, lane b = bucket b).
   if (wave == 0) {
     const int b = (int)lane;
-    // Chunks and work units per code of bucket b (walk dimension kB - b).
-    const uint64_t cc = (b < kB) ? code_chunks(kB - b, Lb) : 0ull;
+    // Chunks (saturated at 2^31) and work units per code of bucket b (walk
+    // dimension kB - b).
+    const uint64_t cc = ccb;
     const uint64_t cu = (b < kB) ? code_units(kB - b) : 0ull;
     uint32_t t = 0;
     if (b < kB) {
@@ -291,6 +317,7 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
       plan->cb[kB] = overflow ? 0u : (uint32_t)chunks;
       plan->ub[kB] = overflow ? 0ull : eu + xu;
       plan->pb[kB] = ep + t;
+      cnt[kB] = ep + t;
       ctrl[0] = 0u;
       ctrl[3] = overflow ? 0xFFFFFFFFu : (uint32_t)chunks;
       if (overflow) atomicOr(&ctrl[2], kErrBadK);
@@ -320,26 +347,62 @@ __device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets
   }
   __syncthreads();
 
-  // Pass 2: positions (returning adds on the own column).  The last batch
-  // (the only one when n <= 4096) is still in registers; earlier ones are
-  // loaded again.
-  for (int base = (n > kPlanBatch) ? 0 : -1;;) {
+  // Pass 2: positions (returning adds on the own column).  One batch: still
+  // in registers; more: each loaded again.
+  const bool staged = n <= kPlanBatch;
+  for (int base = staged ? -1 : 0;;) {
     if (base >= 0) {
       if (base >= n) break;
       uint32_t unused = 0;
-      md_plan_load_batch(offsets, n, n_idx, base, state, out, tmax_arr, tmax_s, false, unused, pc);
+      md_plan_load_batch(offsets, n, n_idx, base, unused, pc);
     }
-    const int first = ((base >= 0) ? base : 0) + tid * kPlanRounds;
+    const int first = ((base >= 0) ? base : 0) + tid;
+    if (staged) {
+      // Branch-free: every returning add issues before the first result is
+      // used; a code without a walk writes the spare slot kPlanBatch.
+      uint32_t pos[kPlanRounds];
 #pragma unroll
-    for (int r = 0; r < kPlanRounds; ++r) {
-      if (pc.has(r)) {
-        const uint32_t pos = atomicAdd(&hist[pc.bucket(r)][tid], 1u);
-        order[pos] = (uint32_t)(first + r);
-        roff[pos] = (uint32_t)pc.o[r];
+      for (int r = 0; r < kPlanRounds; ++r)
+        pos[r] = atomicAdd(&hist[pc.slot(r)][tid], pc.k(r) != 0u ? 1u : 0u);
+      uint32_t pp[kPlanRounds / 2];   // positions (<= 4096) two to a word across the barrier
+#pragma unroll
+      for (int r = 0; r < kPlanRounds; ++r) {
+        pos[r] = pc.k(r) != 0u ? pos[r] : (uint32_t)kPlanBatch;
+        stage[pos[r]] = (uint16_t)(first + r * kPlanThreads);
+        if (r % 2 == 0) pp[r / 2] = pos[r];
+        else pp[r / 2] |= pos[r] << 16;
+      }
+      // The columns are done with: their words take each code's first row
+      // by position.
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < kPlanRounds; ++r)
+        plan_lds[(pp[r / 2] >> (16 * (r % 2))) & 0xFFFFu] = (uint32_t)pc.o[r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < kPlanRounds; ++r) {
+        if (pc.k(r) != 0u) {
+          const uint32_t pos = atomicAdd(&hist[pc.bucket(r)][tid], 1u);
+          order[pos] = (uint32_t)(first + r * kPlanThreads);
+          roff[pos] = (uint32_t)pc.o[r];
+        }
       }
     }
     if (base < 0) break;
     base += kPlanBatch;
+  }
+  if (staged) {
+    __syncthreads();
+    // Contiguous runs out of LDS.
+    const uint32_t npos = cnt[kB];
+#pragma unroll
+    for (int r = 0; r < kPlanRounds; ++r) {
+      const uint32_t q = (uint32_t)(r * kPlanThreads + tid);
+      if (q < npos) {
+        order[q] = stage[q];
+        roff[q] = plan_lds[q];
+      }
+    }
   }
   if (err) atomicOr(&ctrl[2], err);
 }
@@ -354,12 +417,11 @@ __global__ __launch_bounds__(kPlanThreads, 7) void md_prep_kernel(
     int32_t* __restrict__ out, const int32_t* __restrict__ tmax_arr, int32_t tmax_s,
     uint32_t* ctrl) {
   if (blockIdx.x == 0) {
-    md_plan_body(offsets, n, n_idx, p, plan, order, roff, claims, state, out, tmax_arr, tmax_s,
-                 ctrl);
+    md_plan_body(offsets, n, n_idx, p, plan, order, roff, claims, ctrl);
     return;
   }
   md_rows_code((int)((blockIdx.x - 1) * (kPlanThreads / kWave) + threadIdx.x / kWave), n, n_idx,
-               idx, offsets, mplanes, rows, ctrl);
+               idx, offsets, mplanes, rows, state, out, tmax_arr, tmax_s, ctrl);
 }
 
 // Chunk ids.  The plan numbers chunks bucket by bucket, largest walk first;
@@ -865,7 +927,8 @@ hipError_t launch_md_prep(const int32_t* d_idx, int64_t n_idx, const uint32_t* d
                           const int32_t* d_tmax, int32_t tmax, uint32_t* d_ctrl, hipStream_t s) {
   constexpr int kCodesPerBlock = kPlanThreads / kWave;   // one wave per code
   const int64_t row_blocks = ((int64_t)std::max(n, 0) + kCodesPerBlock - 1) / kCodesPerBlock;
-  hipLaunchKernelGGL(md_prep_kernel, dim3((unsigned)(1 + row_blocks)), dim3(kPlanThreads), 0, s,
+  hipLaunchKernelGGL(md_prep_kernel, dim3((unsigned)(1 + row_blocks)), dim3(kPlanThreads),
+                     (unsigned)kPlanDynLds, s,
                      d_idx, n_idx, d_mplanes, d_rows, d_offsets, n, d_params, d_plan, d_order,
                      d_roff, d_claims, d_state, d_out, d_tmax, tmax, d_ctrl);
   return hipGetLastError();
