@@ -90,6 +90,26 @@ def walk_ceiling(device: int = 0) -> dict:
             "T_int32_ops_per_s": OPS_PER_CODEWORD * cw_per_s / 1e12}
 
 
+_HIP = None
+
+
+def device_sync() -> None:
+    """hipDeviceSynchronize of the HIP runtime libgtoric.so runs on (the
+    instance already loaded in this process): waits for every stream of the
+    device, as torch.cuda.synchronize() would.  The timed regions end on it;
+    each context's gt_sync (which also reads back its error word, about 12 us)
+    runs after the clock stops."""
+    global _HIP
+    if _HIP is None:
+        import ctypes
+        from gtoric import _native
+        _native.lib()
+        _HIP = ctypes.CDLL("libamdhip64.so.7")
+    rc = _HIP.hipDeviceSynchronize()
+    if rc != 0:
+        raise SystemExit(f"hipDeviceSynchronize failed: {rc}")
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -153,8 +173,9 @@ def bench_md(ctxs, sets, steps: int, warmup: int, expect=None, before_warmup=Non
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
-    ctx.sync()
+    device_sync()
     dt = time.perf_counter() - t0
+    ctx.sync()
     ctx.prof_enable(False)
     nk, kms = ctx.prof_read("md_main")
     check(outs[0].download(), "one stream")
@@ -169,9 +190,10 @@ def bench_md(ctxs, sets, steps: int, warmup: int, expect=None, before_warmup=Non
         t0 = time.perf_counter()
         for i in range(steps):
             step(i % m)
+        device_sync()
+        dt_pipe = time.perf_counter() - t0
         for c in ctxs:
             c.sync()
-        dt_pipe = time.perf_counter() - t0
         for i in range(m):
             check(outs[i].download(), f"pipelined, context {i}")
 
@@ -186,8 +208,9 @@ def bench_md(ctxs, sets, steps: int, warmup: int, expect=None, before_warmup=Non
     t0 = time.perf_counter()
     for _ in range(steps):
         graph.launch()
-    ctx.sync()
+    device_sync()
     dt_graph = time.perf_counter() - t0
+    ctx.sync()
     got_g = outs[0].download()
     graph.close()
     check(got_g, "graph replay")

@@ -209,6 +209,7 @@ struct gt_ctx {
   // the last tag.
   DevBuf sbuf;
   uint64_t* spin = nullptr;
+  uint32_t* ctrl_host = nullptr;   // pinned: the control slot read back by check_ctrl
   uint32_t single_tag = 0;
   DevBuf masks, klo, khi, canon, stab;                        // canonical
   DevBuf children, bflag, bsum, bidx, bcnt, bmasks, brows, boffs;  // BFS level
@@ -285,9 +286,11 @@ struct gt_ctx {
     return 0;
   }
 
+  // The control slot is read back into pinned memory (a pageable
+  // destination costs a staged copy, several times longer).
   int check_ctrl(hipStream_t s) {
-    uint32_t h[kCtrlSlotWords];
-    GT_HIP(hipMemcpyAsync(h, ctrl_p, sizeof(h), hipMemcpyDeviceToHost, s));
+    uint32_t* h = ctrl_host;
+    GT_HIP(hipMemcpyAsync(h, ctrl_p, kCtrlSlotWords * 4, hipMemcpyDeviceToHost, s));
     GT_HIP(hipStreamSynchronize(s));
     if (h[2] != 0) {
       const uint32_t zero = 0;
@@ -392,7 +395,11 @@ int gt_open(int32_t device, gt_ctx** out) {
   if (device < 0 || device >= ndev) return fail("gt_open: device %d not present (%d devices)", device, ndev);
   gt_ctx* c = new gt_ctx();
   c->device = device;
-  auto bail = [&](int r) { delete c; return r; };
+  auto bail = [&](int r) {
+    if (c->ctrl_host) (void)hipHostFree(c->ctrl_host);
+    delete c;
+    return r;
+  };
   if (c->activate()) return bail(-1);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return bail(fail("hipGetDeviceProperties failed"));
@@ -413,6 +420,8 @@ int gt_open(int32_t device, gt_ctx** out) {
   if (c->mplanes.ensure(sizeof(mp)) || c->params_buf.ensure(sizeof(MdParams)) ||
       c->ctrl.ensure(kCtrlWords * 4))
     return bail(-1);
+  if (hipHostMalloc((void**)&c->ctrl_host, kCtrlSlotWords * 4, hipHostMallocDefault) != hipSuccess)
+    return bail(fail("gt_open: hipHostMalloc of the control readback failed"));
   uint32_t zeros[kCtrlWords] = {0};
   if (hipMemcpy(c->mplanes.p, mp, sizeof(mp), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->ctrl.p, zeros, sizeof(zeros), hipMemcpyHostToDevice) != hipSuccess ||
@@ -435,6 +444,7 @@ int gt_close(gt_ctx* c) {
       for (auto& pr : kv.second.ev) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
     for (DevBuf* b : c->all_bufs()) b->release();
     if (c->spin) (void)hipHostFree(c->spin);
+    if (c->ctrl_host) (void)hipHostFree(c->ctrl_host);
     if (c->handover) (void)hipEventDestroy(c->handover);
     (void)hipStreamDestroy(c->stream);
   }
