@@ -115,10 +115,35 @@ def log(msg: str) -> None:
 
 
 def proj(k: int) -> int:
-    """Projective codewords the exact walk of a k-point code visits: the
-    (k-1)-dimensional subcode vanishing at the torus point (1,1)."""
+    """Projective codewords of the (k-1)-dimensional subcode vanishing at the
+    torus point (1,1) (the level-1 walk of a k-point code)."""
     w = 0 if k <= 1 else k - 1
     return (8 ** w - 1) // 7
+
+
+def level_walk(k: int, m: int) -> int:
+    """Projective codewords of a level-m pass over a k-point code: 3 subcodes
+    for m = 3, 4, one otherwise, of dimension k - m (csrc/gt_common.h)."""
+    return (3 if m in (3, 4) else 1) * ((8 ** (k - m) - 1) // 7)
+
+
+def walked(k: int, z: int, levels: dict, fused_k: int = 8) -> int:
+    """Codewords the exact search walks for a k-point code of exact maximum z:
+    the first pass at its level; when z is below that level's certificate
+    (7m - 6), the one-wave search (k <= fused_k) goes on at the highest lower
+    level that max(z_m, k - 1) certifies - counted here from k - 1, which can
+    only overstate the work - and the planned walk at level 1."""
+    if k <= 1:
+        return 0
+    m = 1 if k <= 2 else levels[k]
+    n = level_walk(k, m)
+    if m > 1 and z < 7 * m - 6:
+        if k <= fused_k:
+            m2 = max([q for q in range(1, m) if 7 * q - 6 <= k - 1] or [1])
+            n += level_walk(k, m2)
+        else:
+            n += proj(k)
+    return n
 
 
 def ref_polys(k: int) -> int:
@@ -126,7 +151,8 @@ def ref_polys(k: int) -> int:
     return 8 ** k - 1
 
 
-def bench_md(ctxs, sets, steps: int, warmup: int, expect=None, before_warmup=None):
+def bench_md(ctxs, sets, steps: int, warmup: int, expect=None, before_warmup=None,
+             kmax=None, kernel="md_main"):
     """Time gt_md_batch_dev steps over one batch, inputs resident on the device.
 
     * one stream: `steps` steps on ctxs[0], each md_main launch timed by HIP
@@ -155,7 +181,7 @@ def bench_md(ctxs, sets, steps: int, warmup: int, expect=None, before_warmup=Non
 
     def step(i=0):
         ctxs[i].md_batch_dev(d_idx.ptr, len(flat), d_off.ptr, n, outs[i].ptr, tmax=48,
-                             thresholded=False)
+                             thresholded=False, kmax=kmax)
 
     def check(got, what):
         if expect is not None and not np.array_equal(got, expect):
@@ -167,7 +193,9 @@ def bench_md(ctxs, sets, steps: int, warmup: int, expect=None, before_warmup=Non
     ctx.sync()
     got = outs[0].download()
     check(got, "one stream, warm-up")
-    ctx.prof_read("md_main")
+    kernels = (kernel,) if isinstance(kernel, str) else tuple(kernel)
+    for name in kernels:
+        ctx.prof_read(name)
     ctx.prof_enable(True)
     ctx.sync()
     t0 = time.perf_counter()
@@ -177,7 +205,8 @@ def bench_md(ctxs, sets, steps: int, warmup: int, expect=None, before_warmup=Non
     dt = time.perf_counter() - t0
     ctx.sync()
     ctx.prof_enable(False)
-    nk, kms = ctx.prof_read("md_main")
+    reads = [ctx.prof_read(name) for name in kernels]
+    nk, kms = reads[0][0], sum(r[1] for r in reads)   # launches of the first, time of all
     check(outs[0].download(), "one stream")
 
     dt_pipe = None
@@ -602,7 +631,7 @@ def main() -> None:
     log(f"[bench] tuned parameters: {'loaded' if tuned else 'defaults'}")
     sets = workloads.mixed_k()
     n = len(sets)
-    codewords = sum(proj(len(s)) for s in sets)
+    levels = {k: ctx.get_md_level(k) for k in range(2, 22)}
     ref_work = sum(ref_polys(len(s)) for s in sets)
     gold = ROOT / "tests" / "golden" / "md_golden.npz"
     expect = np.load(gold)["c2_mz"] if gold.exists() else None
@@ -616,7 +645,12 @@ def main() -> None:
         ctxs.append(c)
     dt, nk, kms, got, dt_graph, dt_pipe = bench_md(
         ctxs, sets, args.steps, args.warmup, expect,
-        before_warmup=lambda: ceil.update(walk_ceiling()))
+        before_warmup=lambda: ceil.update(walk_ceiling()), kmax=max(len(s) for s in sets),
+        kernel="md_fused")
+    # Codewords the search walked: from each code's exact result (the run's
+    # own, bit-exact against the fixture).
+    codewords = sum(walked(len(s), int(z), levels) for s, z in zip(sets, got))
+    level1_codewords = sum(proj(len(s)) for s in sets)
     value_one = n * args.steps / dt
     dt_value = dt_pipe if dt_pipe is not None else dt
     value = n * args.steps / dt_value
@@ -626,8 +660,8 @@ def main() -> None:
     peak = ceil["T_int32_ops_per_s"]
     achieved = OPS_PER_CODEWORD * codewords / (kern_ms * 1e-3) / 1e12
     log(f"[bench] batched, one stream: {value_one:,.0f} codes/s, {dt / args.steps * 1e3:.3f} "
-        f"ms/step, md_main {kern_ms:.3f} ms, {achieved:.2f} TOP/s of {peak:.2f} attainable "
-        f"({nominal:.2f} nominal)")
+        f"ms/step, md_fused {kern_ms:.3f} ms, {codewords:,} codewords, {achieved:.2f} TOP/s of "
+        f"{peak:.2f} attainable ({nominal:.2f} nominal)")
     log(f"[bench] batched, {len(ctxs)} streams: {value:,.0f} codes/s, {ms_step:.3f} ms/step")
 
     result = {
@@ -646,17 +680,19 @@ def main() -> None:
         "config": {
             "workload": "batched exact min distance, 4096 GF(8) toric codes n=49, "
                         "k uniform in 3..8 (numpy default_rng(2)), one device-resident "
-                        "gt_md_batch_dev pass (rows and plan, walk) per step, the steps issued "
-                        "round-robin on `streams` contexts",
+                        "gt_md_batch_dev2 call (kmax 8: one launch, a wave per code) per "
+                        "step, the steps issued round-robin on `streams` contexts",
             "codes": n,
             "streams": len(ctxs),
+            "levels": {str(k): levels[k] for k in range(3, 9)},
             "walked_codewords": codewords,
+            "level1_codewords": level1_codewords,
             "reference_polys": ref_work,
             "parity": "bit-exact vs oracle fixture" if expect is not None else "unchecked",
         },
         "roofline": {
             "bound": "valu_int32",
-            "kernel": "md_main_exact_kernel",
+            "kernel": "md_fused_kernel",
             "achieved": round(achieved, 3),
             "peak": round(peak, 3),
             "unit": "TOP/s",
@@ -675,7 +711,7 @@ def main() -> None:
                                              / nominal, 4),
             "kernel_ms": round(kern_ms, 4),
             "kernel_launches": nk,
-            "timed_in": "the one-stream region (md_main alone on the device; HIP events "
+            "timed_in": "the one-stream region (md_fused alone on the device; HIP events "
                         "recorded by each launch's own dispatch)",
             "step_achieved": round(OPS_PER_CODEWORD * codewords / (ms_step * 1e-3) / 1e12, 3),
             "step_frac": round(OPS_PER_CODEWORD * codewords / (ms_step * 1e-3) / 1e12 / peak, 4),
@@ -693,22 +729,22 @@ def main() -> None:
     result["one_stream"] = {
         "value": round(value_one, 1), "unit": "codes/s",
         "ms_per_step": round(dt / args.steps * 1e3, 4),
-        "form": "the same steps on one context and stream (each step's kernels after the "
-                "previous step's), md_main bracketed by HIP events",
+        "form": "the same steps on one context and stream (each step's kernel after the "
+                "previous step's), md_fused bracketed by HIP events",
     }
     result["graph_replay"] = {
         "value": round(n * args.steps / dt_graph, 1), "unit": "codes/s",
         "ms_per_step": round(dt_graph / args.steps * 1e3, 4),
-        "form": "the same step (rows and plan, walk) captured once as a hipGraph and "
-                "replayed",
+        "form": "the same step captured once as a hipGraph and replayed",
     }
     log(f"[bench] graph replay: {n * args.steps / dt_graph:,.0f} codes/s, "
         f"{dt_graph / args.steps * 1e3:.3f} ms/step")
 
     if "wide" not in skip:
         wsets = workloads.mixed_k(4096, 3, 10, seed=12)
-        wcw = sum(proj(len(s)) for s in wsets)
-        wdt, wnk, wkms, wgot, _, _ = bench_md([ctx], wsets, 3, 1, None)
+        wdt, wnk, wkms, wgot, _, _ = bench_md([ctx], wsets, 3, 1, None, kmax=10,
+                                              kernel=("md_fused", "md_main", "md_main2"))
+        wcw = sum(walked(len(s), int(z), levels) for s, z in zip(wsets, wgot))
         wk = wkms / max(wnk, 1)
         wgold = ROOT / "tests" / "golden" / "md_wide_golden.npz"
         wparity = "unchecked"
@@ -724,13 +760,16 @@ def main() -> None:
             "value": round(4096 * 3 / wdt, 1), "unit": "codes/s",
             "workload": "4096 codes, k uniform in 3..10 (default_rng(12))",
             "parity": wparity,
-            "walked_codewords": wcw, "kernel_ms": round(wk, 3),
+            "walked_codewords": wcw, "level1_codewords": sum(proj(len(s)) for s in wsets),
+            "kernel_ms": round(wk, 3),
+            "kernels": "md_fused (k <= 8, one wave per code) + md_main (k = 9, 10, first "
+                       "pass) + md_main (second pass), device time per call",
             "achieved_TOPs": round(OPS_PER_CODEWORD * wcw / (wk * 1e-3) / 1e12, 3),
             "frac_of_attainable": round(OPS_PER_CODEWORD * wcw / (wk * 1e-3) / 1e12 / peak, 4),
             "note": "achieved counts 11 int32 ops per codeword; the attainable peak is the "
                     "walk block's measured issue rate (roofline.peak)",
         }
-        log(f"[bench] mixed k=3..10: {4096 * 3 / wdt:,.0f} codes/s, md_main {wk:.2f} ms, "
+        log(f"[bench] mixed k=3..10: {4096 * 3 / wdt:,.0f} codes/s, search kernels {wk:.3f} ms, "
             f"{OPS_PER_CODEWORD * wcw / (wk * 1e-3) / 1e12:.2f} TOP/s")
 
     if "plain" not in skip:

@@ -124,6 +124,14 @@ void build_mplanes(uint32_t out[kNpts * kRowWords]) {
 // that per-chunk set-up is a few percent of the walk, small enough for a
 // balanced tail; for k >= 19 large enough that a code has at most about 2^24
 // chunks (the chunk ids of one call must stay below 2^31).
+// Default first-pass level per k: level m walks level_nsub(m) subcodes of
+// dimension k - m and certifies a maximum of at least 7m - 6 zeros.  On the
+// 4096-code mixed batch (k = 3..8) these levels certify every code: k = 4..6
+// have at least 9 zeros (level 2 needs 8), k = 7, 8 at least 15 (level 3);
+// k = 3 codes often have 7 (level 1).  For k >= 9 level 3 leaves to the
+// second pass only codes of distance 35 or more.
+int default_level(int k) { return (k <= 3) ? 1 : (k <= 6) ? 2 : 3; }
+
 int default_L(int k) {
   static const uint8_t tab[kMaxKExact + 1] = {4, 4, 4, 4, 4, 4, 4, 7, 8, 10, 11, 12,
                                               13, 14, 15, 16, 17, 18, 19, 21, 24, 27};
@@ -134,25 +142,32 @@ struct ProfRec {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
 };
 
-// Plan area of one exact pass: claim counters (kClaimShards x 4 KB), then
-// MdPlan (512 bytes), then the order and row-offset arrays (4 bytes per code
-// each).
+// Plan area of one exact search: claim counters (kClaimShards x 4 KB), then
+// a 512-byte slot holding MdPlan and, in its last word, the second-pass list
+// count, then five arrays of 4 bytes per code: order, row offsets, aux
+// words, the second-pass list and the list of codes md_fused leaves.  The
+// count of that last list lives apart (it must read 0 before a call's first
+// kernel, and the plan area is reallocated as batches grow).
 constexpr size_t kMdClaimBytes = (size_t)kClaimShards * kClaimStride * 4;
 constexpr size_t kMdPlanAreaBytes = kMdClaimBytes + 512;
-static_assert(sizeof(MdPlan) <= 512, "MdPlan fits its slot");
+constexpr size_t kMdFbCountOff = kMdPlanAreaBytes - 4;
+constexpr size_t kMdPlanBytesPerCode = 20;
+static_assert(sizeof(MdPlan) <= 508, "MdPlan and the list count fit the slot");
 
 // ---- scratch of the allocation-free exact pass (gt_md_batch_dev_noalloc) --
 // One file-scope device array per GPU, held until the process ends: the plan
-// area, the per-code state words and the bit rows of at most kNaMaxCodes codes
-// with kNaMaxIdx indices in total (6.6 MB).  Host code never passes the
+// area, the per-code state words, the bit rows of at most kNaMaxCodes codes
+// with kNaMaxIdx indices in total, and the list count (19.4 MB).  Host code never passes the
 // array's name to a kernel: it uses the address hipGetSymbolAddress returns
 // for the current device (g_scratch_ptr), taken once per device by gt_open.
 constexpr int32_t kNaMaxCodes = 16384;
 constexpr int64_t kNaMaxIdx = 65536;
 constexpr size_t kNaPlanOff = 0;
-constexpr size_t kNaBestOff = kNaPlanOff + kMdPlanAreaBytes + (size_t)kNaMaxCodes * 8;
+constexpr size_t kNaBestOff =
+    kNaPlanOff + kMdPlanAreaBytes + (size_t)kNaMaxCodes * kMdPlanBytesPerCode;
 constexpr size_t kNaRowsOff = kNaBestOff + (size_t)kNaMaxCodes * 8;
-constexpr size_t kNaBytes = kNaRowsOff + (size_t)kNaMaxIdx * 3 * kRowWords * 4;
+constexpr size_t kNaCountOff = kNaRowsOff + (size_t)kNaMaxIdx * kRowsPerIdx * kRowWords * 4;
+constexpr size_t kNaBytes = kNaCountOff + 256;
 static_assert(kNaBestOff % 256 == 0 && kNaRowsOff % 256 == 0, "aligned scratch regions");
 
 constexpr int kMaxDevices = 64;
@@ -193,6 +208,7 @@ struct gt_ctx {
   int md_grid[2] = {0, 0};
   uint8_t* scratch_dev = nullptr;   // this device's `scratch` array
   MdParams params{};
+  int fused_k = kFusedMaxK;         // codes of k <= fused_k: one wave per code
   bool params_dirty = true;
 
   DevBuf mplanes, params_buf, ctrl;
@@ -204,6 +220,7 @@ struct gt_ctx {
   // unrelated synchronous call.
   uint32_t* ctrl_p = nullptr;
   DevBuf idx, offs, tmax, rows, plan, best, out, zmax, stopf;  // min distance
+  DevBuf mdcnt;   // count of the list md_fused leaves (zeroed at open)
   // One-code call: its device state {minimum, waves done} ({64, 0} between
   // launches), the pinned host word the kernel writes (tag, result) to, and
   // the last tag.
@@ -224,7 +241,7 @@ struct gt_ctx {
   hipStream_t capture_stream = nullptr;
 
   std::vector<DevBuf*> all_bufs() {
-    return {&mplanes, &params_buf, &ctrl, &idx, &offs, &tmax, &rows, &plan, &best,
+    return {&mplanes, &params_buf, &ctrl, &idx, &offs, &tmax, &rows, &plan, &best, &mdcnt,
             &out, &zmax, &stopf, &sbuf, &masks, &klo, &khi, &canon, &stab,
             &children, &bflag, &bsum, &bidx, &bcnt, &bmasks, &brows, &boffs};
   }
@@ -303,14 +320,17 @@ struct gt_ctx {
   }
 
   // ---- min distance on device pointers (stream-ordered, no host sync) ----
+  // kmax: a bound on the sets' sizes known to the caller (larger sets are
+  // input errors); the list search is enqueued only when kmax > fused_k.
   int md_enqueue(const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets, int32_t n,
                  const int32_t* d_tmax, int32_t tmax_s, bool thresholded,
                  int64_t sample_count, uint64_t seed, int32_t* d_out, hipStream_t s,
-                 uint32_t set_period = 0, const uint32_t* d_gen = nullptr) {
+                 uint32_t set_period = 0, const uint32_t* d_gen = nullptr,
+                 int kmax = kMaxKExact) {
     if (n <= 0) return 0;
-    GT_OK(rows.ensure((size_t)std::max<int64_t>(n_idx, 1) * 3 * kRowWords * 4));
     GT_OK(upload_params(s));
     if (sample_count > 0) {
+      GT_OK(rows.ensure((size_t)std::max<int64_t>(n_idx, 1) * 3 * kRowWords * 4));
       GT_HIP(launch_md_rows(d_idx, n_idx, mplanes.as<uint32_t>(), rows.as<uint32_t>(),
                             ctrl_p, s));
       GT_OK(zmax.ensure((size_t)n * 4));
@@ -323,37 +343,60 @@ struct gt_ctx {
       GT_OK(prof_end("md_sampled", s, e0));
       return 0;
     }
-    GT_OK(plan.ensure(kMdPlanAreaBytes + (size_t)n * 8));
+    GT_OK(rows.ensure((size_t)std::max<int64_t>(n_idx, 1) * kRowsPerIdx * kRowWords * 4));
+    GT_OK(plan.ensure(kMdPlanAreaBytes + (size_t)n * kMdPlanBytesPerCode));
     GT_OK(best.ensure((size_t)n * 8));
     return md_exact_launch(d_idx, n_idx, d_offsets, n, d_tmax, tmax_s, thresholded, d_out,
-                           rows.as<uint32_t>(), plan.as<uint8_t>(), best.as<uint64_t>(), s, true);
+                           rows.as<uint32_t>(), plan.as<uint8_t>(), best.as<uint64_t>(),
+                           mdcnt.as<uint32_t>(), s, true, kmax);
   }
 
-  // Exact or thresholded pass (prep, walk) with caller-given scratch: rows
-  // (3 * n_idx rows), the plan area (kMdPlanAreaBytes + 8 n bytes: claim
-  // counters, MdPlan, order, row offsets) and the per-code state (n 64-bit
-  // words).  Every word the kernels read is written by md_prep first, so no
+  // Exact or thresholded search with caller-given scratch: rows
+  // (kRowsPerIdx * n_idx rows), the plan area (kMdPlanAreaBytes + 20 n bytes:
+  // claim counters, MdPlan and the second-pass count, order, row offsets,
+  // aux words, the two lists), the per-code state (n 64-bit words) and the
+  // list count (one word, 0 between calls).  md_fused searches the codes of
+  // k <= fused_k and lists the others; when kmax allows larger codes the
+  // list search follows (md_prepl and md_main at the codes' levels, then at
+  // level 1 for the codes the first pass could not certify).  Every word a
+  // kernel reads is written before by an earlier kernel of the call, so no
   // clearing is needed.
   int md_exact_launch(const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets, int32_t n,
                       const int32_t* d_tmax, int32_t tmax_s, bool thresholded, int32_t* d_out,
-                      uint32_t* d_rows, uint8_t* d_plan_area, uint64_t* d_state, hipStream_t s,
-                      bool profile) {
+                      uint32_t* d_rows, uint8_t* d_plan_area, uint64_t* d_state,
+                      uint32_t* d_big_count, hipStream_t s, bool profile, int kmax) {
     uint32_t* d_claims = reinterpret_cast<uint32_t*>(d_plan_area);
     MdPlan* d_plan = reinterpret_cast<MdPlan*>(d_plan_area + kMdClaimBytes);
     uint32_t* d_order = reinterpret_cast<uint32_t*>(d_plan_area + kMdPlanAreaBytes);
     uint32_t* d_roff = d_order + n;
-    GT_HIP(launch_md_prep(d_idx, n_idx, mplanes.as<uint32_t>(), d_rows, d_offsets, n,
-                          params_buf.as<MdParams>(), d_plan, d_order, d_roff, d_claims, d_state,
-                          d_out, d_tmax, tmax_s, ctrl_p, s));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (profile && prof && !bstate.frozen) {
-      GT_HIP(hipEventCreate(&e0));
-      GT_HIP(hipEventCreate(&e1));
+    const MdAux ax{d_roff + n, reinterpret_cast<uint32_t*>(d_plan_area + kMdFbCountOff),
+                   d_roff + 2 * (size_t)n, d_big_count, d_roff + 3 * (size_t)n};
+    const MdParams* dp = params_buf.as<MdParams>();
+    const int grid = md_grid[thresholded ? 1 : 0];
+    const bool timed = profile && prof && !bstate.frozen;
+    hipEvent_t ev[6] = {};
+    if (timed)
+      for (hipEvent_t& e : ev) GT_HIP(hipEventCreate(&e));
+    GT_HIP(launch_md_fused(d_idx, n_idx, d_offsets, n, mplanes.as<uint32_t>(), dp, d_tmax, tmax_s,
+                           d_out, ax, fused_k, kmax, ctrl_p, s, ev[0], ev[1]));
+    if (timed) prof_rec["md_fused"].ev.emplace_back(ev[0], ev[1]);
+    if (kmax <= fused_k) {
+      if (timed)
+        for (int i = 2; i < 6; ++i) GT_HIP(hipEventDestroy(ev[i]));
+      return 0;
     }
-    GT_HIP(launch_md_main(thresholded, md_grid[thresholded ? 1 : 0], d_rows, d_plan, d_order,
-                          d_roff, d_claims, d_state, d_tmax, tmax_s, d_out,
-                          params_buf.as<MdParams>(), s, e0, e1));
-    if (e0) prof_rec["md_main"].ev.emplace_back(e0, e1);
+    GT_HIP(launch_md_prepl(false, d_idx, n_idx, mplanes.as<uint32_t>(), d_rows, d_offsets, n, dp,
+                           d_plan, d_order, d_roff, d_claims, d_state, d_out, d_tmax, tmax_s, ax,
+                           ctrl_p, s));
+    GT_HIP(launch_md_main(thresholded, grid, d_rows, d_plan, d_order, d_roff, d_claims, d_state,
+                          d_tmax, tmax_s, d_out, dp, ax, false, s, ev[2], ev[3]));
+    if (timed) prof_rec["md_main"].ev.emplace_back(ev[2], ev[3]);
+    GT_HIP(launch_md_prepl(true, d_idx, n_idx, mplanes.as<uint32_t>(), d_rows, d_offsets, n, dp,
+                           d_plan, d_order, d_roff, d_claims, d_state, d_out, d_tmax, tmax_s, ax,
+                           ctrl_p, s));
+    GT_HIP(launch_md_main(thresholded, grid, d_rows, d_plan, d_order, d_roff, d_claims, d_state,
+                          d_tmax, tmax_s, d_out, dp, ax, true, s, ev[4], ev[5]));
+    if (timed) prof_rec["md_main2"].ev.emplace_back(ev[4], ev[5]);
     return 0;
   }
 };
@@ -411,6 +454,7 @@ int gt_open(int32_t device, gt_ctx** out) {
     return bail(fail("hipStreamCreate failed"));
   if (hipEventCreateWithFlags(&c->handover, hipEventDisableTiming) != hipSuccess)
     return bail(fail("hipEventCreate failed"));
+  for (int k = 0; k <= kMaxKExact; ++k) c->params.level[k] = (uint8_t)default_level(k);
   for (int k = 0; k <= kMaxKExact; ++k) md_params_set(c->params, k, default_L(k), 2);
   c->md_grid[0] = c->cu_count * md_main_blocks_per_cu(false);
   c->md_grid[1] = c->cu_count * md_main_blocks_per_cu(true);
@@ -418,13 +462,14 @@ int gt_open(int32_t device, gt_ctx** out) {
   uint32_t mp[kNpts * kRowWords];
   build_mplanes(mp);
   if (c->mplanes.ensure(sizeof(mp)) || c->params_buf.ensure(sizeof(MdParams)) ||
-      c->ctrl.ensure(kCtrlWords * 4))
+      c->ctrl.ensure(kCtrlWords * 4) || c->mdcnt.ensure(16))
     return bail(-1);
   if (hipHostMalloc((void**)&c->ctrl_host, kCtrlSlotWords * 4, hipHostMallocDefault) != hipSuccess)
     return bail(fail("gt_open: hipHostMalloc of the control readback failed"));
   uint32_t zeros[kCtrlWords] = {0};
   if (hipMemcpy(c->mplanes.p, mp, sizeof(mp), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->ctrl.p, zeros, sizeof(zeros), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->mdcnt.p, zeros, 16, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->params_buf.p, &c->params, sizeof(MdParams), hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail("gt_open: upload of constant tables failed"));
   c->params_dirty = false;
@@ -487,6 +532,48 @@ int gt_get_md_params(gt_ctx* c, int32_t k, int32_t* L, int32_t* check_log2) {
   return 0;
 }
 
+int gt_set_md_level(gt_ctx* c, int32_t k, int32_t level) {
+  if (!c) return fail("null context");
+  if (k < 3 || k > kMaxKExact) return fail("k=%d outside 3..%d (k <= 2 walks at level 1)", k, kMaxKExact);
+  if (level < 1 || level > std::min(k - 1, kMaxLevel))
+    return fail("level=%d outside 1..%d for k=%d", level, std::min(k - 1, kMaxLevel), k);
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (c->bstate.frozen) return fail("launch parameters changed during graph capture");
+  MdParams p = c->params;
+  md_level_set(p, k, level);
+  for (int q = 3; q < kMaxKExact; ++q)
+    if (p.level[q + 1] > p.level[q] + 1)
+      return fail("level %d at k=%d would follow level %d at k=%d: the first-pass walk dimension "
+                  "k - level must not decrease with k", (int)p.level[q + 1], q + 1,
+                  (int)p.level[q], q);
+  c->params = p;
+  c->params_dirty = true;
+  GT_OK(c->activate());
+  return c->upload_params(c->stream);
+}
+
+int gt_set_md_fused_max_k(gt_ctx* c, int32_t k) {
+  if (!c) return fail("null context");
+  if (k < 1 || k > kFusedMaxK) return fail("fused bound %d outside 1..%d", k, kFusedMaxK);
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (c->bstate.frozen) return fail("launch parameters changed during graph capture");
+  c->fused_k = k;
+  return 0;
+}
+
+int gt_get_md_fused_max_k(gt_ctx* c, int32_t* k) {
+  if (!c) return fail("null context");
+  if (k) *k = c->fused_k;
+  return 0;
+}
+
+int gt_get_md_level(gt_ctx* c, int32_t k, int32_t* level) {
+  if (!c) return fail("null context");
+  if (k < 0 || k > kMaxKExact) return fail("k=%d outside 0..%d", k, kMaxKExact);
+  if (level) *level = code_level(c->params.level, k, false);
+  return 0;
+}
+
 int gt_md_batch(gt_ctx* c, const int32_t* idx, const int32_t* offsets, int32_t n,
                 const int32_t* tmax_per_set, int32_t tmax, int64_t sample_count,
                 uint64_t sample_seed, int32_t* out) {
@@ -499,9 +586,11 @@ int gt_md_batch(gt_ctx* c, const int32_t* idx, const int32_t* offsets, int32_t n
   if (offsets[0] != 0) return fail("offsets[0] must be 0");
   bool thresholded = false;
   uint64_t chunks = 0;
+  int kmax = 0;
   for (int32_t s = 0; s < n; ++s) {
     const int k = offsets[s + 1] - offsets[s];
     if (k < 0) return fail("offsets must be non-decreasing");
+    kmax = std::max(kmax, k);
     if (sample_count > 0 ? k > kMaxK : k > kMaxKExact)
       return fail(sample_count > 0 ? "sampled evaluation supports k <= 49 (set %d has k=%d)"
                                    : "exact evaluation supports k <= 21 (set %d has k=%d)", s, k);
@@ -527,25 +616,35 @@ int gt_md_batch(gt_ctx* c, const int32_t* idx, const int32_t* offsets, int32_t n
     GT_HIP(hipMemcpyAsync(c->tmax.p, tmax_per_set, (size_t)n * 4, hipMemcpyHostToDevice, s));
   GT_OK(c->md_enqueue(c->idx.as<int32_t>(), n_idx, c->offs.as<int32_t>(), n,
                       tmax_per_set ? c->tmax.as<int32_t>() : nullptr, tmax, thresholded,
-                      sample_count, sample_seed, c->out.as<int32_t>(), s));
+                      sample_count, sample_seed, c->out.as<int32_t>(), s, 0, nullptr, kmax));
   GT_HIP(hipMemcpyAsync(out, c->out.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
   GT_OK(c->check_ctrl(s));
   return 0;
 }
 
-int gt_md_batch_dev(gt_ctx* c, const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets,
-                    int32_t n, const int32_t* d_tmax, int32_t tmax, int32_t thresholded,
-                    int64_t sample_count, uint64_t sample_seed, int32_t* d_out, void* stream) {
+int gt_md_batch_dev2(gt_ctx* c, const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets,
+                     int32_t n, const int32_t* d_tmax, int32_t tmax, int32_t thresholded,
+                     int32_t kmax, int64_t sample_count, uint64_t sample_seed, int32_t* d_out,
+                     void* stream) {
   if (!c) return fail("null context");
   if (n < 0 || n_idx < 0) return fail("negative sizes");
   if (n > 0 && (!d_offsets || !d_out)) return fail("null device offsets/out");
   if (n_idx > 0 && !d_idx) return fail("null device idx");
+  if (kmax < 0 || kmax > kMaxKExact) return fail("kmax=%d outside 0..%d", kmax, kMaxKExact);
   std::lock_guard<std::mutex> lk(c->mu);
   GT_OK(c->activate());
   StreamUse su(c, c->pick(stream));
   GT_OK(su.rc);
   return c->md_enqueue(d_idx, n_idx, d_offsets, n, d_tmax, tmax, thresholded != 0, sample_count,
-                       sample_seed, d_out, c->pick(stream));
+                       sample_seed, d_out, c->pick(stream), 0, nullptr,
+                       sample_count > 0 ? kMaxKExact : kmax);
+}
+
+int gt_md_batch_dev(gt_ctx* c, const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets,
+                    int32_t n, const int32_t* d_tmax, int32_t tmax, int32_t thresholded,
+                    int64_t sample_count, uint64_t sample_seed, int32_t* d_out, void* stream) {
+  return gt_md_batch_dev2(c, d_idx, n_idx, d_offsets, n, d_tmax, tmax, thresholded, kMaxKExact,
+                          sample_count, sample_seed, d_out, stream);
 }
 
 int gt_md_batch_dev_noalloc(gt_ctx* c, const int32_t* d_idx, int64_t n_idx,
@@ -567,8 +666,9 @@ int gt_md_batch_dev_noalloc(gt_ctx* c, const int32_t* d_idx, int64_t n_idx,
   uint8_t* base = c->scratch_dev;
   return c->md_exact_launch(d_idx, n_idx, d_offsets, n, d_tmax, tmax, thresholded != 0, d_out,
                             reinterpret_cast<uint32_t*>(base + kNaRowsOff), base + kNaPlanOff,
-                            reinterpret_cast<uint64_t*>(base + kNaBestOff), c->pick(stream),
-                            false);
+                            reinterpret_cast<uint64_t*>(base + kNaBestOff),
+                            reinterpret_cast<uint32_t*>(base + kNaCountOff), c->pick(stream),
+                            false, kMaxKExact);
 }
 
 int gt_md_batch_dev_noalloc_limits(int32_t* max_codes, int64_t* max_idx) {
@@ -969,7 +1069,7 @@ int gt_bfs_level_ex(gt_ctx* c, gt_hashset* seen, const uint64_t* d_par, int32_t 
                               c->boffs.as<int32_t>(), c->ctrl_p, s));
   GT_OK(c->md_enqueue(c->brows.as<int32_t>(), (int64_t)nu * k, c->boffs.as<int32_t>(), (int32_t)nu,
                       nullptr, tmax, tmax < 48, sample_count > 0 ? sample_count : 0, sample_seed,
-                      d_out_mz, s, (uint32_t)set_period));
+                      d_out_mz, s, (uint32_t)set_period, nullptr, k));
   GT_OK(c->check_ctrl(s));
   return 0;
 }
@@ -1053,7 +1153,7 @@ struct gt_evolve {
     GT_HIP(launch_hs_insert(ckey, pop, skeys, stags, scap, 0, scount, ctrl, s));
     GT_HIP(launch_masks_to_rows(chosen, pop, k, idx, offs, ctrl, s));
     GT_OK(c->md_enqueue(idx, (int64_t)pop * k, offs, pop, nullptr, 48, false, sample_count, seed,
-                        mz, s, 0, sample_count > 0 ? state : nullptr));
+                        mz, s, 0, sample_count > 0 ? state : nullptr, k));
     GT_HIP(launch_evo_survive(pop_m, pop_d, pop_age, chosen, ckey, mz, count, pop, false, max_gen,
                               state, log_cls, log_d, log_best, log_new, s));
     return 0;
@@ -1178,7 +1278,7 @@ int gt_evolve_create2(gt_ctx* c, int32_t k, int32_t pop, int32_t n_prop, uint64_
     GT_HIP(launch_hs_insert(ckey, pop, e->skeys, e->stags, e->scap, 0, e->scount, ctrl, s));
     GT_HIP(launch_masks_to_rows(e->chosen, pop, k, e->idx, e->offs, ctrl, s));
     GT_OK(c->md_enqueue(e->idx, (int64_t)pop * k, e->offs, pop, nullptr, 48, false, sample_count,
-                        seed, e->mz, s));
+                        seed, e->mz, s, 0, nullptr, k));
     GT_HIP(launch_evo_survive(e->pop_m, e->pop_d, e->pop_age, e->chosen, ckey, e->mz, e->count,
                               pop, true, max_generations, e->state, e->log_cls, e->log_d,
                               e->log_best, e->log_new, s));

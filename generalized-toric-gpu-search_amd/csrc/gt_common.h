@@ -35,6 +35,31 @@
 // (g_a = g_b => h_a + h_b = 0, coefficients summing to 0), as in the
 // reference.  The exact walk therefore visits (8^(k-1) - 1) / 7 codewords
 // per code against the reference's 8^k - 1 (kernel.py:128-139).
+//
+// Anchored subcodes (level m).  Write a torus point as (alpha^u, alpha^v),
+// (u, v) in (Z/7)^2, alpha = x.  Besides the translations (u, v) -> (u, v) +
+// t, the Frobenius map c -> c^2 applied to the coefficients preserves weight:
+// f_{c^2}(P^2) = f_c(P)^2, so the codeword of the squared coefficients
+// vanishes on the image of f_c's zero set under (u, v) -> 2 (u, v).  The
+// group G of the maps (u, v) -> 2^i (u, v) + t (order 147) therefore maps
+// zero sets of codewords of C(S) to zero sets of codewords of the same
+// weight.  Fix a direction d (one of the 8 lines through 0 of (Z/7)^2); its
+// 7 cosets partition the 49 points.  A codeword with z >= 7 (m - 1) + 1
+// zeros has m zeros in one coset, {P + r d : r in A}, |A| = m; a map of G
+// sends them to {r d : r in R}, where R is the representative of A's orbit
+// under the maps r -> 2^i r + t of Z/7 (level_rep).  So when the maximum
+// number of zeros z* is at least level_cert(m) = 7 (m - 1) + 1, it equals
+// the maximum over the subcodes C_R = {c : c(r d) = 0 for r in R}, R over
+// the level's representatives (1 for m = 1, 2, 5, 6; 3 for m = 3, 4), and
+// every codeword walked is a codeword of C(S), so the maximum z_m found is
+// at most z* in any case.  Hence z_m >= level_cert(m) proves z_m = z*
+// (otherwise z* < level_cert(m), and z_m <= z*); a code below that is
+// walked again at level 1.  C_R has dimension k - m when the m conditions
+// are independent; the rows are built from a Gauss-Jordan kernel basis in
+// the first direction where all representatives give rank m (codes with
+// none also go to level 1).  Level 1 is the subcode through (1,1) above
+// (certified whenever z* >= 1, i.e. always).  A level-3 code walks 3 (8^(k-3)
+// - 1) / 7 codewords, 21x fewer than level 1.
 #pragma once
 #include <stdint.h>
 
@@ -62,9 +87,15 @@ GT_HD int walk_dim(int k) { return (k <= 1) ? 0 : k - 1; }
 struct MdParams {
   uint8_t L[kMaxKExact + 1];      // Gray walk bits per lane for full chunks
   uint8_t check_log2[kMaxKExact + 1];  // early-exit check every 2^x blocks
-  // Chunks of a code of k generators at L[k] (code_chunks), saturated at
-  // 2^31; kept with L by md_params_set.
+  uint8_t level[kMaxKExact + 1];  // anchor level m of the first pass
+  // Largest k whose first-pass subcodes have no piece above the tiny part
+  // (walk dimension k - level[k] <= kSmallW): the small codes of that pass.
+  uint8_t small_k;
+  // Chunks of a code of k generators at L[k] in the first pass (level[k])
+  // and at level 1 (the second pass), saturated at 2^31; kept with L and
+  // level by md_params_set / md_level_set.
   uint32_t chunks[kMaxKExact + 1];
+  uint32_t chunks1[kMaxKExact + 1];
 };
 
 // Chunk plan of one batch, largest codes first.  Bucket j (0 <= j < kMaxKExact)
@@ -76,7 +107,9 @@ struct MdParams {
 // overflowed 2^31 chunks).  Claiming ids in increasing order hands out the
 // biggest chunks first, so the last chunks of a launch are the small ones of
 // the small codes.  For the dealt (exact) walk the q-th code owns work units
-// ub[j] + q * code_units(w) .. (ub[kMaxKExact] = total units).
+// ub[j] + q * anchor_units(k, m) .. (ub[kMaxKExact] = total units).  A code
+// of level m has level_nsub(m) subcodes of walk dimension k - m; its chunks
+// (and units) cover them one after the other.
 struct MdPlan {
   uint64_t ub[kMaxKExact + 1];
   uint32_t cb[kMaxKExact + 1];
@@ -119,12 +152,53 @@ GT_HD uint64_t code_chunks(int w, int L) {
   return n;
 }
 
-// L and check interval of k, with the chunk count the plan reads.
+// ---- anchored subcodes (see the top of this file) ------------------------
+constexpr int kMaxLevel = 6;   // (a level-7 reduction spills registers; it would need z >= 43)
+// Representatives of the m-subsets of Z/7 under r -> 2^i r + t, as bit
+// masks over r (bit r set <=> r in R).
+constexpr GT_HD int level_nsub(int m) { return (m == 3 || m == 4) ? 3 : 1; }
+constexpr GT_HD uint32_t level_rep(int m, int s) {
+  if (m == 3) return s == 0 ? 0x07u : s == 1 ? 0x0Bu : 0x0Du;   // {0,1,2} {0,1,3} {0,2,3}
+  if (m == 4) return s == 0 ? 0x0Fu : s == 1 ? 0x17u : 0x1Du;   // {0,1,2,3} {0,1,2,4} {0,2,3,4}
+  return (1u << m) - 1u;                                          // {0, .., m-1}
+}
+// Zeros that certify a level-m maximum (7 (m - 1) + 1).
+GT_HD int level_cert(int m) { return 7 * m - 6; }
+// Level of a code of k generators in a pass (pass 2 and codes with k <= 2:
+// level 1; k above the exact range, which no plan holds: 1).
+GT_HD int code_level(const uint8_t* level, int k, bool level1) {
+  return (level1 || k <= 2 || k > kMaxKExact) ? 1 : (int)level[k];
+}
+// Walk dimension of one subcode of a code of k generators at level m.
+GT_HD int anchor_dim(int k, int m) { return (k <= 1) ? 0 : k - m; }
+// Row slots per index: a code's subcodes take nsub * 3 (k - m) <= 9k rows.
+constexpr int kRowsPerIdx = 9;
+// Codes of at most kFusedMaxK generators are searched one wave per code
+// (rows in LDS, every level in turn); larger ones by the plan-and-deal walk.
+constexpr int kFusedMaxK = 8;
+
+// L and check interval of k, with the chunk counts the plan reads.
+inline uint32_t sat31(uint64_t c) { return (c >= (1ull << 31)) ? (1u << 31) : (uint32_t)c; }
+inline void md_params_refresh(MdParams& p, int k) {
+  const int m = code_level(p.level, k, false);
+  p.chunks[k] = sat31((uint64_t)level_nsub(m) * code_chunks(anchor_dim(k, m), p.L[k]));
+  p.chunks1[k] = sat31(code_chunks(walk_dim(k), p.L[k]));
+  int sk = 1;
+  for (int q = 2; q <= kMaxKExact; ++q)
+    if (anchor_dim(q, code_level(p.level, q, false)) <= kTinyTop + 1) sk = q;
+  p.small_k = (uint8_t)sk;
+}
 inline void md_params_set(MdParams& p, int k, int L, int check_log2) {
   p.L[k] = (uint8_t)L;
   p.check_log2[k] = (uint8_t)check_log2;
-  const uint64_t c = code_chunks(walk_dim(k), L);
-  p.chunks[k] = (c >= (1ull << 31)) ? (1u << 31) : (uint32_t)c;
+  md_params_refresh(p, k);
+}
+// Levels must keep the first-pass walk dimension k - level[k] non-decreasing
+// in k (the small codes are then the plan's last buckets): the caller checks
+// level[k + 1] <= level[k] + 1 and 1 <= level[k] <= min(k - 1, kMaxLevel).
+inline void md_level_set(MdParams& p, int k, int m) {
+  p.level[k] = (uint8_t)m;
+  md_params_refresh(p, k);
 }
 
 // Work units of the dealt walk.  A block is 16 Gray steps of all 64 lanes
@@ -153,6 +227,12 @@ GT_HD uint64_t code_units(int w) {
   if (w <= kSmallW) return kTinyUnits;
   return kTinyUnits + ((kOctalOnes & ((1ull << (3 * (w - kSmallW))) - 1ull))
                        << (3 * kSmallW - kBlockBits));
+}
+
+// Work units of a code of k generators at level m: its subcodes one after
+// the other, each code_units(k - m).
+GT_HD uint64_t anchor_units(int k, int m) {
+  return (k <= 1) ? 0 : (uint64_t)level_nsub(m) * code_units(anchor_dim(k, m));
 }
 
 // Projective codewords of a w-dimensional code: (8^w - 1) / 7.

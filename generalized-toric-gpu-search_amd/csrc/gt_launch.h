@@ -21,21 +21,46 @@ constexpr int kCtrlWords = 16;
 
 hipError_t launch_md_rows(const int32_t* d_idx, int64_t n_idx, const uint32_t* d_mplanes,
                           uint32_t* d_rows, uint32_t* d_ctrl, hipStream_t s);
+// Per-call words of the exact search.  big_count / big_list: the codes
+// md_fused leaves to the list search (k > kfused), reset by the first-pass
+// md_main; aux[code] = 1 when a list code's first pass found no direction of
+// full rank; fb_count / fb_list: the codes the first pass hands to the
+// second (level 1), reset by the first-pass md_prepl.
+struct MdAux {
+  uint32_t* aux;
+  uint32_t* fb_count;
+  uint32_t* fb_list;
+  uint32_t* big_count;
+  uint32_t* big_list;
+};
+// Exact (or thresholded) search of one batch: md_fused (one wave per code of
+// k <= kfused, rows in LDS, results written; larger codes listed), then,
+// unless the caller knows that no code has more than kfused generators, the
+// list search: md_prepl and md_main at the codes' levels, md_prepl and
+// md_main at level 1 for the codes the first could not certify.
+// d_rows: kRowsPerIdx row slots per index.
 // d_claims: kClaimShards counters, kClaimStride words apart (reset by the plan).
-// d_order, d_roff: n entries each (code index and first row per plan position).
+// d_order, d_roff: n entries each (code index and first index per plan position).
 // d_state: one word per code (minimum weight, chunks finished), initialised
-// by md_prep, which also writes the results of codes without a walk (k <= 1);
-// md_main writes every other code's result when its last chunk is folded.
-hipError_t launch_md_prep(const int32_t* d_idx, int64_t n_idx, const uint32_t* d_mplanes,
-                          uint32_t* d_rows, const int32_t* d_offsets, int32_t n,
-                          const MdParams* d_params, MdPlan* d_plan, uint32_t* d_order,
-                          uint32_t* d_roff, uint32_t* d_claims, uint64_t* d_state, int32_t* d_out,
-                          const int32_t* d_tmax, int32_t tmax, uint32_t* d_ctrl, hipStream_t s);
+// by md_prepl for its listed codes; md_main writes a listed code's result
+// when its last chunk is folded, or lists it for the second pass.
+hipError_t launch_md_fused(const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets,
+                           int32_t n, const uint32_t* d_mplanes, const MdParams* d_params,
+                           const int32_t* d_tmax, int32_t tmax, int32_t* d_out, const MdAux& ax,
+                           int kfused, int kmax, uint32_t* d_ctrl, hipStream_t s,
+                           hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
+hipError_t launch_md_prepl(bool level1, const int32_t* d_idx, int64_t n_idx,
+                           const uint32_t* d_mplanes, uint32_t* d_rows, const int32_t* d_offsets,
+                           int32_t n, const MdParams* d_params, MdPlan* d_plan, uint32_t* d_order,
+                           uint32_t* d_roff, uint32_t* d_claims, uint64_t* d_state,
+                           int32_t* d_out, const int32_t* d_tmax, int32_t tmax, const MdAux& ax,
+                           uint32_t* d_ctrl, hipStream_t s);
 hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
                           const MdPlan* d_plan, const uint32_t* d_order, const uint32_t* d_roff,
                           uint32_t* d_claims, uint64_t* d_state, const int32_t* d_tmax,
-                          int32_t tmax, int32_t* d_out, const MdParams* d_params, hipStream_t s,
-                          hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
+                          int32_t tmax, int32_t* d_out, const MdParams* d_params, const MdAux& ax,
+                          bool level1, hipStream_t s, hipEvent_t ev_start = nullptr,
+                          hipEvent_t ev_stop = nullptr);
 // One code, one launch: grid = code_chunks(w, L) single-wave workgroups, the
 // 3w rows (w <= 20) in the kernel arguments.  d_state: {64, 0} before the
 // first launch; each launch leaves it so.  h_result: a host-visible word

@@ -44,6 +44,21 @@ struct ConstRows {
   }
 };
 
+// Rows in LDS (a wave's own area, written before the walk): plain loads,
+// which land in VGPRs already.
+struct LdsRows {
+  const uint32_t* R;
+  __device__ __forceinline__ RowV operator()(uint32_t r) const {
+    const uint32_t* p = R + r * kRowWords;
+    RowV v;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) v.w[i] = p[i];
+    return v;
+  }
+  __device__ __forceinline__ RowV vgpr(const RowV& v) const { return v; }
+  __device__ __forceinline__ RowV vgpr_after(const RowV& v, uint32_t) const { return v; }
+};
+
 __device__ __forceinline__ void xor_row(RowV& v, const RowV& r) {
 #pragma unroll
   for (int i = 0; i < 6; ++i) v.w[i] ^= r.w[i];

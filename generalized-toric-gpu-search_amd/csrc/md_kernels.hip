@@ -3,30 +3,36 @@
 //
 // Replaces eval_min_distance_batch_bp (kernel.py:97-203, kernel_cuda_bp.py:146-238)
 // and eval_min_distance / eval_min_distance_single_bp (kernel.py:63-94,
-// kernel_cuda_bp.py:99-127).  Work decomposition (see gt_common.h):
+// kernel_cuda_bp.py:99-127).  Work decomposition (see gt_common.h for the
+// anchored subcodes, levels and their certificates):
 //
-//   md_prep     workgroup 0: codes bucketed by walk dimension, largest first
-//               (MdPlan, order[], roff[]), claim counters reset;
-//               workgroups 1..: one wave per code, its state word reset, the
-//               result of a code without a walk (k <= 1) written, and the bit
-//               rows x^b * (g_i + g_{k-1}) of the subcode vanishing at the
-//               torus point (1,1) (gt_common.h), which has the same maximum
-//               number of zeros as C(S) with 1/8 of the codewords.  The
-//               sampled path keeps the full rows x^b * g_i (md_rows).
+//   md_fused    one wave per code of k <= 8 generators: the rows of the
+//               code's first-pass subcodes built in LDS (a Gauss-Jordan
+//               kernel basis per representative point set), walked by the
+//               wave, certified, and if not certified the next lower level
+//               in the same wave; the result written.  Larger codes are
+//               listed for:
+//   md_prepl    workgroup 0: the listed codes bucketed by size (MdPlan,
+//               order[], roff[]), claim counters reset; workgroups 1..: a
+//               wave per listed code in turn, its state word reset and the
+//               bit rows of its subcodes written to global memory.
 //   md_main     persistent.  Exact mode: two static deals, one share of each
-//               per wave: the codes of walk dimension <= 4 (no Gray piece),
-//               whole, in equal counts; the other codes' work units (16-step
-//               Gray blocks of their pieces and their tiny parts, in plan
-//               order) in equal runs.  A code inside one run is written by
-//               its wave, a code split over several runs is folded into its
-//               state word by each and written by the last.
-//               Thresholded mode: chunk ids claimed from 16 counters,
-//               decoded against the plan in LDS (bucket, piece, code),
-//               64 lanes x 2^L Gray steps each, every chunk folded at once
-//               (other waves stop on the folded value).  Results are
-//               max_zeros clamped at tmax + 1; codes without a walk
-//               (k <= 1) get theirs from md_prep.
-//
+//               per wave: the codes whose subcodes have no Gray piece above
+//               the tiny part, whole, in equal counts; the other codes' work
+//               units (16-step Gray blocks of their subcodes' pieces and
+//               their tiny parts, in plan order) in equal runs.  A code
+//               inside one run is finished by its wave, a code split over
+//               several runs is folded into its state word by each and
+//               finished by the last.  Thresholded mode: chunk ids claimed
+//               from 16 counters, decoded against the plan in LDS (bucket,
+//               piece, subcode, code), 64 lanes x 2^L Gray steps each, every
+//               chunk folded at once (other waves stop on the folded value).
+//               Finishing a code writes max_zeros clamped at tmax + 1, or,
+//               below the level's certificate, lists the code for
+//   md_prepl,   the same two launches at level 1 over that list (the second
+//   md_main     pass), which always finish.
+//   The sampled path keeps the full rows x^b * g_i of C(S) (md_rows).
+
 // Inner step (per projective codeword, all VALU; the block's rows are
 // wave-uniform, loaded with scalar loads and copied to VGPRs):
 //   6 v_xor_b32 (row add, VOP2 with two VGPR operands), 2 v_or3_b32 (plane
@@ -66,24 +72,146 @@ __device__ __forceinline__ void md_rows_one(int64_t t, const int32_t* __restrict
   }
 }
 
-// Exact-walk rows of one code (one wave): lane 3i + b (i < k - 1) writes
-// x^b * (g_i + g_{k-1}) at row 3i + b of the code.  Lanes i < k check their
-// index.  Codes with k < 2 have no walk; k > 21 and offsets outside
-// [0, n_idx] are flagged by the plan (and no row is written for them).
 // Result of a code whose minimum weight is b (max_zeros clamped at tmax + 1).
 __device__ __forceinline__ int32_t md_result(int b, int tmax) {
   return (b <= 48 - tmax) ? tmax + 1 : 49 - b;
 }
 
-// The wave of a code also resets its state word and writes the result of a
-// code without a walk.
-__device__ __forceinline__ void md_rows_code(int code, int32_t n, int64_t n_idx,
-                                             const int32_t* __restrict__ idx,
-                                             const int32_t* __restrict__ offsets,
-                                             const uint32_t* __restrict__ mplanes,
-                                             uint32_t* __restrict__ rows, uint64_t* state,
-                                             int32_t* out, const int32_t* tmax_arr,
-                                             int32_t tmax_s, uint32_t* ctrl) {
+// ---- kernel bases of the anchored subcodes (gt_common.h) -------------------
+// GF(8) elements are 3-bit polynomials; alpha^e (e = 0..6, alpha = x) and
+// inverses packed three bits per entry.
+constexpr uint32_t kF8Exp = 1u | 2u << 3 | 4u << 6 | 3u << 9 | 6u << 12 | 7u << 15 | 5u << 18;
+constexpr uint32_t kF8Inv = 0u | 1u << 3 | 5u << 6 | 6u << 9 | 7u << 12 | 2u << 15 | 3u << 18 |
+                            4u << 21;
+
+// Product in GF(8) = GF(2)[x]/(x^3 + x + 1) of 3-bit polynomials.
+__device__ __forceinline__ uint32_t f8_mul(uint32_t a, uint32_t b) {
+  uint32_t p = ((b & 1u) ? a : 0u) ^ ((b & 2u) ? a << 1 : 0u) ^ ((b & 4u) ? a << 2 : 0u);
+  if (p & 16u) p ^= 0x16u;   // x^4 = x^2 + x
+  if (p & 8u) p ^= 0x0Bu;    // x^3 = x + 1
+  return p;
+}
+
+// Line directions through 0 of (Z/7)^2: (1,0), (0,1), (1,1), ..., (1,6).
+__device__ __forceinline__ int dir_proj(int d, int a, int b) {
+  return (d == 0) ? a : (d == 1) ? b : (a + (d - 1) * b) % 7;
+}
+
+// Bit rows of one code at level M (one wave; lane i < k holds index g of its
+// exponent point (a, b)): subcode s (s < level_nsub(M)) of the
+// representative R_s takes rows s * 3w + 3t + b = x^b * h_t (w = k - M,
+// t < w), where h_t = g_f + sum_r E[r][f] g_{piv[r]} for the t-th free
+// column f of the reduced matrix E (a kernel basis of the evaluations at
+// R_s d), in the first direction d where every R_s has rank M.  Returns
+// false when no direction has.
+//
+// Gauss-Jordan reduction with one column per lane: monomial i at the point
+// r d is alpha^(r * pr_i), pr_i = (a_i, b_i) . d mod 7, and lane i keeps the
+// column's entries for r = 0..6 three bits each (row r at bits 3r).  For
+// each row r of R in turn the pivot is the first column (ballot) with a
+// nonzero entry that is not a pivot yet; the row is scaled to 1 there and
+// eliminated from the other rows of R with the pivot column's entries
+// (read from the pivot lane), all lanes updating their own column.  (M is a
+// template argument so that the row loops unroll with fixed trip counts.)
+template <int M>
+__device__ __forceinline__ bool md_rows_level(int g, int k, uint32_t lane,
+                                              const uint32_t* __restrict__ mplanes,
+                                              uint32_t* __restrict__ base) {
+  constexpr int nsub = level_nsub(M);
+  const int w = k - M;
+  const bool act = lane < (uint32_t)k;
+  const int a = g / 7, b = g - 7 * (g / 7);
+#pragma unroll 1
+  for (int d = 0; d < 8; ++d) {
+    const int pr = dir_proj(d, a, b);
+    uint32_t col0 = 0u;
+#pragma unroll
+    for (int r = 0; r < 7; ++r) col0 |= ((kF8Exp >> (3 * ((r * pr) % 7))) & 7u) << (3 * r);
+    bool ok = true;
+#pragma unroll 1
+    for (int s = 0; s < nsub; ++s) {
+      const uint32_t R = level_rep(M, s);
+      uint32_t col = col0, taken = 0u;
+      int piv[7];
+#pragma unroll
+      for (int r = 0; r < 7; ++r) {
+        piv[r] = 0;
+        if (!ok || !((R >> r) & 1u)) continue;
+        const uint32_t e = (col >> (3 * r)) & 7u;
+        const uint64_t nz = __ballot(act && e != 0u && !((taken >> (lane & 31u)) & 1u));
+        if (nz == 0ull) {
+          ok = false;
+          continue;
+        }
+        const int c = __builtin_ctzll(nz);
+        taken |= 1u << c;
+        piv[r] = c;
+        const uint32_t P = (uint32_t)__builtin_amdgcn_readlane((int)col, c);
+        const uint32_t er = f8_mul(e, (kF8Inv >> (3 * ((P >> (3 * r)) & 7u))) & 7u);
+        col = (col & ~(7u << (3 * r))) | (er << (3 * r));
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+          if (q == r || !((R >> q) & 1u)) continue;
+          col ^= f8_mul((P >> (3 * q)) & 7u, er) << (3 * q);
+        }
+      }
+      if (!ok) break;
+      // h = g_f + sum_r E[r][f] g_{piv[r]} on the lanes f of free columns.
+      uint32_t h[6];
+      {
+        const uint4 r0 = *reinterpret_cast<const uint4*>(mplanes + g * kRowWords);
+        const uint2 r1 = *reinterpret_cast<const uint2*>(mplanes + g * kRowWords + 4);
+        h[0] = r0.x; h[1] = r0.y; h[2] = r0.z; h[3] = r0.w; h[4] = r1.x; h[5] = r1.y;
+      }
+#pragma unroll
+      for (int r = 0; r < 7; ++r) {
+        if (!((R >> r) & 1u)) continue;
+        const int gp = __builtin_amdgcn_readlane(g, piv[r]);
+        uint32_t v[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) v[q] = mplanes[gp * kRowWords + q];
+        const uint32_t e = (col >> (3 * r)) & 7u;
+#pragma unroll
+        for (int bp = 0; bp < 3; ++bp) {
+          const uint32_t msk = 0u - ((e >> bp) & 1u);
+#pragma unroll
+          for (int q = 0; q < 6; ++q) h[q] ^= v[q] & msk;
+          mul_x(v);
+        }
+      }
+      if (act && !((taken >> lane) & 1u)) {
+        const int t = __builtin_popcount(~taken & ((1u << lane) - 1u));
+        uint32_t* out_rows = base + (size_t)(s * 3 * w + 3 * t) * kRowWords;
+#pragma unroll
+        for (int bp = 0; bp < 3; ++bp) {
+          uint4* rr = reinterpret_cast<uint4*>(out_rows + bp * kRowWords);
+          rr[0] = make_uint4(h[0], h[1], h[2], h[3]);
+          rr[1] = make_uint4(h[4], h[5], 0u, 0u);
+          mul_x(h);
+        }
+      }
+    }
+    if (ok) return true;
+  }
+  return false;
+}
+
+// Rows of one code for a pass at level m = code_level(k) (md_rows_level),
+// starting at slot kRowsPerIdx * offsets[code].  The wave also resets the
+// code's state word, writes the result of a code without a walk (k <= 1),
+// and sets aux[code] = 1 when no direction has full rank (the code then goes
+// to the second pass; its rows are not used for a result).  Codes with
+// k > 21 or offsets outside [0, n_idx] are flagged by the plan (no row is
+// written for them).
+__device__ __forceinline__ void md_rows_anchor(int code, int32_t n, int64_t n_idx,
+                                               const int32_t* __restrict__ idx,
+                                               const int32_t* __restrict__ offsets,
+                                               const uint32_t* __restrict__ mplanes,
+                                               const uint8_t* __restrict__ level, bool level1,
+                                               uint32_t* __restrict__ rows, uint64_t* state,
+                                               uint32_t* aux, int32_t* out,
+                                               const int32_t* tmax_arr, int32_t tmax_s,
+                                               uint32_t* ctrl) {
   if (code >= n) return;
   const uint32_t lane = lane_id();
   const int o = offsets[code];
@@ -91,34 +219,35 @@ __device__ __forceinline__ void md_rows_code(int code, int32_t n, int64_t n_idx,
   const bool bad_k = k < 0 || k > kMaxKExact || o < 0 || (int64_t)o + k > n_idx;
   if (lane == 0) {
     state[code] = 64ull;   // minimum weight 64, no chunk finished
+    aux[code] = 0u;
     // No walk: k = 0 (and bad input) gives 0, k = 1 a monomial (weight 49).
     if (bad_k || k <= 1)
       out[code] = (!bad_k && k == 1) ? md_result(49, tmax_arr ? tmax_arr[code] : tmax_s) : 0;
   }
   if (k < 1 || bad_k) return;
-  const uint32_t i = lane / 3u, bpow = lane - 3u * i;
   int g = 0;
   bool bad = false;
-  if (i < (uint32_t)k) {
-    g = idx[o + (int)i];
+  if (lane < (uint32_t)k) {
+    g = idx[o + (int)lane];
     bad = g < 0 || g >= kNpts;
   }
   if (__ballot(bad) != 0ull) {
     if (lane == 0) atomicOr(&ctrl[2], kErrBadIndex);
     return;
   }
-  if (k < 2 || i + 1 >= (uint32_t)k) return;
-  const int gl = idx[o + k - 1];
-  const uint4 a0 = *reinterpret_cast<const uint4*>(mplanes + g * kRowWords);
-  const uint2 a1 = *reinterpret_cast<const uint2*>(mplanes + g * kRowWords + 4);
-  const uint4 b0 = *reinterpret_cast<const uint4*>(mplanes + gl * kRowWords);
-  const uint2 b1 = *reinterpret_cast<const uint2*>(mplanes + gl * kRowWords + 4);
-  uint32_t w[6] = {a0.x ^ b0.x, a0.y ^ b0.y, a0.z ^ b0.z, a0.w ^ b0.w, a1.x ^ b1.x, a1.y ^ b1.y};
-  if (bpow >= 1u) mul_x(w);
-  if (bpow >= 2u) mul_x(w);
-  uint4* row = reinterpret_cast<uint4*>(rows + ((size_t)o * 3 + lane) * kRowWords);
-  row[0] = make_uint4(w[0], w[1], w[2], w[3]);
-  row[1] = make_uint4(w[4], w[5], 0u, 0u);
+  if (k < 2) return;
+  uint32_t* base = rows + (size_t)o * kRowsPerIdx * kRowWords;
+  const int m = __builtin_amdgcn_readfirstlane(code_level(level, k, level1));
+  bool ok;
+  switch (m) {
+    case 1: ok = md_rows_level<1>(g, k, lane, mplanes, base); break;
+    case 2: ok = md_rows_level<2>(g, k, lane, mplanes, base); break;
+    case 3: ok = md_rows_level<3>(g, k, lane, mplanes, base); break;
+    case 4: ok = md_rows_level<4>(g, k, lane, mplanes, base); break;
+    case 5: ok = md_rows_level<5>(g, k, lane, mplanes, base); break;
+    default: ok = md_rows_level<6>(g, k, lane, mplanes, base); break;   // kMaxLevel
+  }
+  if (!ok && lane == 0) aux[code] = 1u;
 }
 
 __global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict__ idx,
@@ -129,299 +258,221 @@ __global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict_
   md_rows_one((int64_t)blockIdx.x * blockDim.x + threadIdx.x, idx, n_idx, mplanes, rows, ctrl);
 }
 
-// Largest-first plan (MdPlan, order[] = code index and roff[] = first row
-// per position).  One workgroup of 256 threads takes the codes in batches of
-// 4096, thread t the codes base + 256 r + t, r < 16, so every global access
-// of a round is one contiguous 1 KB run (a single CU issues all of them: 64
-// separate lines per instruction cost 4 us of a 10 us plan).  The offsets of
-// a batch are loaded together (one memory latency) and kept in registers
-// with the sizes packed five bits each.  Every thread counts its codes per
-// bucket in its own LDS column (hist[b][thread]: the threads of a wave hit
-// consecutive words, so the adds never conflict).  Scan threads (bucket,
-// segment of 22 columns) sum their segments; one wave turns the bucket
-// totals into bucket bases, chunk and unit offsets; the scan threads then
-// rewrite every column entry as that thread's first position in the bucket;
-// pass 2 takes each code's position with a returning add on its own column
-// entry.  For a call of at most 4096 codes the code indices and first rows
-// are staged in LDS by position (the rows in the column words, free by then)
-// and copied out in contiguous runs; larger calls store them where they
-// belong.
-// Workgroups of 256 threads and at most 72 VGPRs: with md_main's five
-// 88-VGPR waves on every SIMD, a wave of each still fits, so the prep of a
-// batch issued on another stream runs beside a walk instead of after it.
-// The column counts and the staging area are dynamic LDS (kPlanDynLds bytes
-// per workgroup, given at launch): the compiler sizes registers for the
-// occupancy that static LDS allows, and 29 KB of static LDS would let it
-// spend more than 72.
-// (Per-wave ballots over the buckets present cost 16 rounds x buckets per
-// pass at this width: 10 us more per 4096 codes.)
-constexpr int kPlanThreads = 256;
-constexpr int kPlanRounds = 16;
-constexpr int kPlanBatch = kPlanThreads * kPlanRounds;
-constexpr int kPlanSegs = 12;                                             // segments per bucket
-constexpr int kPlanSegLen = (kPlanThreads + kPlanSegs - 1) / kPlanSegs;   // columns per segment
-constexpr int kPlanPack = 6;                                              // sizes per word
-static_assert(kMaxKExact * kPlanSegs <= kPlanThreads, "one scan thread per bucket segment");
-static_assert(kMaxKExact < 32, "a size fits five bits");
-constexpr size_t kPlanHistWords = (size_t)kMaxKExact * kPlanThreads;
-constexpr size_t kPlanDynLds = kPlanHistWords * 4 + ((size_t)kPlanBatch + 2) * 2;   // + spare slot
-static_assert(kPlanBatch <= 65536, "staged code indices fit 16 bits");
-static_assert(kPlanBatch + 1 <= kPlanHistWords, "staged first rows fit the column words");
+// ---- one wave per code (k <= kFusedMaxK) -----------------------------------
+// A code of at most 8 generators walks at most 45 rows (level 3: 3 subcodes
+// of 3 x 5 rows) and at most 3 (8^5 - 1) / 7 codewords at its default level,
+// so one wave builds its rows in LDS and walks them itself: no plan, no
+// cross-wave fold, one launch.  A first-level maximum below the level's
+// certificate (or a level without a direction of full rank) is followed, in
+// the same wave, by the highest lower level whose certificate the bound
+// max(z, k - 1) already meets (level 1 at worst: 21 rows, 299,593
+// codewords for k = 8).  Codes with more generators are appended to the
+// list of the plan-and-deal search (md_prepl, md_main).
+constexpr int kFusedRows = 45;   // 3 subcodes x 3 (kFusedMaxK - 3) rows
+static_assert(kFusedRows == 3 * 3 * (kFusedMaxK - 3), "the level-3 rows of the largest fused code");
+constexpr int kFusedWaves = 4;   // waves (codes) per workgroup
 
-// This thread's codes base + r * kPlanThreads + tid of one batch: first
-// index o[r] and size k(r), 0 when the code has no walk (its bucket is
-// otherwise kMaxKExact - walk_dim(k) = kMaxKExact + 1 - k, k >= 2).
-struct PlanCodes {
-  int32_t o[kPlanRounds];
-  uint32_t kp[(kPlanRounds + kPlanPack - 1) / kPlanPack];
-  __device__ __forceinline__ uint32_t k(int r) const {
-    return (kp[r / kPlanPack] >> (5 * (r % kPlanPack))) & 31u;
-  }
-  __device__ __forceinline__ int bucket(int r) const { return kMaxKExact + 1 - (int)k(r); }
-  // The bucket, or 0 (never a code's: k <= kMaxKExact) for no walk.
-  __device__ __forceinline__ int slot(int r) const { return k(r) != 0u ? bucket(r) : 0; }
-};
-
-__device__ __forceinline__ void md_plan_load_batch(const int32_t* __restrict__ offsets, int32_t n,
-                                                   int64_t n_idx, int base, uint32_t& err,
-                                                   PlanCodes& pc) {
-  const int tid = (int)threadIdx.x;
-  const uint32_t lane = (uint32_t)tid % kWave;
-  // offsets[i] for every code i of the thread (and for i = n: the end of
-  // code n - 1); a code's end is the next lane's value, and for lane 63 the
-  // value lane r of the wave loads as `nx` in round r.
-  const int wnext = base + (tid - (int)lane) + kWave;   // lane 63's next code in round 0
-  const int inx = wnext + (int)lane * kPlanThreads;
-  const int32_t nx = (lane < (uint32_t)kPlanRounds && inx <= n) ? offsets[inx] : 0;
-#pragma unroll
-  for (int r = 0; r < kPlanRounds; ++r) {
-    const int i = base + r * kPlanThreads + tid;
-    pc.o[r] = (i <= n) ? offsets[i] : 0;
-  }
-#pragma unroll
-  for (int q = 0; q < (kPlanRounds + kPlanPack - 1) / kPlanPack; ++q) pc.kp[q] = 0u;
-#pragma unroll
-  for (int r = 0; r < kPlanRounds; ++r) {
-    const int i = base + r * kPlanThreads + tid;
-    const int32_t down = __shfl_down(pc.o[r], 1, kWave);
-    const int32_t last = __builtin_amdgcn_readlane(nx, r);
-    const int32_t o1 = (lane == kWave - 1) ? last : down;
-    if (i < n) {
-      const int k = o1 - pc.o[r];
-      // Rows and indices of the code in range.
-      const bool bad = k < 0 || k > kMaxKExact || pc.o[r] < 0 || (int64_t)o1 > n_idx;
-      if (bad) err |= kErrBadK;
-      else if (k > 1) pc.kp[r / kPlanPack] |= (uint32_t)k << (5 * (r % kPlanPack));
-    }
+// Rows of level m into `base` (md_rows_level); false: no full-rank direction.
+__device__ __forceinline__ bool md_rows_switch(int m, int g, int k, uint32_t lane,
+                                               const uint32_t* __restrict__ mplanes,
+                                               uint32_t* __restrict__ base) {
+  switch (m) {
+    case 1: return md_rows_level<1>(g, k, lane, mplanes, base);
+    case 2: return md_rows_level<2>(g, k, lane, mplanes, base);
+    case 3: return md_rows_level<3>(g, k, lane, mplanes, base);
+    case 4: return md_rows_level<4>(g, k, lane, mplanes, base);
+    case 5: return md_rows_level<5>(g, k, lane, mplanes, base);
+    default: return md_rows_level<6>(g, k, lane, mplanes, base);   // kMaxLevel
   }
 }
 
-__device__ __forceinline__ void md_plan_body(const int32_t* __restrict__ offsets, int32_t n,
-                                             int64_t n_idx, const MdParams* __restrict__ p,
+// Per-lane minimum weight over a whole w-dimensional walk (rows in LDS).
+__device__ __forceinline__ uint32_t walk_all(const LdsRows& row, int w, uint32_t lane) {
+  uint32_t best = 64u;
+  for (int j = w - 1; j > kTinyTop; --j)
+    best = min(best, gray_blocks(row, (uint32_t)j, 0ull, piece_blocks(j), lane));
+  return min(best, tiny_chunk(row, tiny_top(w), lane));
+}
+
+// LDS written by lanes of this wave and read by all of them next (and the
+// other way round before the area is rewritten): wait for the wave's LDS
+// operations and keep the compiler from moving loads or stores across.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Codes with kfused < k <= kmax go to the list; k > kmax is an input error
+// (a caller's bound that skips the list search).
+__global__ __launch_bounds__(kFusedWaves * kWave) void md_fused_kernel(
+    const int32_t* __restrict__ idx, int64_t n_idx, const int32_t* __restrict__ offsets,
+    int32_t n, const uint32_t* __restrict__ mplanes, const MdParams* __restrict__ p,
+    const int32_t* __restrict__ tmax_arr, int32_t tmax_s, int32_t* __restrict__ out,
+    uint32_t* big_count, uint32_t* big_list, int32_t kfused, int32_t kmax, uint32_t* ctrl) {
+  __shared__ __align__(16) uint32_t lrows[kFusedWaves][kFusedRows * kRowWords];
+  const int wv = (int)(threadIdx.x / kWave);
+  const int code = (int)blockIdx.x * kFusedWaves + wv;
+  if (code >= n) return;
+  const uint32_t lane = lane_id();
+  const int o = offsets[code];
+  const int k = offsets[code + 1] - o;
+  const int tmax = tmax_arr ? tmax_arr[code] : tmax_s;
+  const bool bad_k = k < 0 || k > kMaxKExact || k > kmax || o < 0 || (int64_t)o + k > n_idx;
+  if (bad_k || k <= 1) {
+    if (lane == 0) {
+      if (bad_k) atomicOr(&ctrl[2], kErrBadK);
+      // No walk: k = 0 (and bad input) gives 0, k = 1 a monomial (weight 49).
+      out[code] = (!bad_k && k == 1) ? md_result(49, tmax) : 0;
+    }
+    return;
+  }
+  if (k > kfused) {
+    if (lane == 0) {
+      const uint32_t i = atomicAdd(big_count, 1u);
+      if (i < (uint32_t)n) big_list[i] = (uint32_t)code;
+      else atomicOr(&ctrl[2], kErrBadK);   // (a count left by an interrupted call)
+    }
+    return;
+  }
+  int g = 0;
+  bool bad = false;
+  if (lane < (uint32_t)k) {
+    g = idx[o + (int)lane];
+    bad = g < 0 || g >= kNpts;
+  }
+  if (__ballot(bad) != 0ull) {
+    if (lane == 0) {
+      atomicOr(&ctrl[2], kErrBadIndex);
+      out[code] = 0;
+    }
+    return;
+  }
+  uint32_t* base = lrows[wv];
+  const int thr = 48 - tmax;
+  int m = __builtin_amdgcn_readfirstlane(code_level(p->level, k, false));
+  for (;;) {
+    wave_lds_sync();
+    const bool ok = md_rows_switch(m, g, k, lane, mplanes, base);
+    wave_lds_sync();
+    uint32_t best = 64u;
+    if (ok) {
+      const int w = k - m;
+      for (int s = 0; s < level_nsub(m) && (int)best > thr; ++s)
+        best = wave_min(min(best, walk_all(LdsRows{base + (size_t)s * 3 * w * kRowWords}, w,
+                                           lane)));
+    }
+    const int z = kNpts - (int)best;
+    if (m == 1 || (ok && (z > tmax || z >= level_cert(m)))) {
+      if (lane == 0) out[code] = md_result((int)best, tmax);
+      return;
+    }
+    // Every lower level m' with 7m' - 6 <= max(z, k - 1) is certified.
+    const int lb = max(ok ? z : 0, k - 1);
+    int m2 = 1;
+    for (int q = m - 1; q >= 2; --q)
+      if (level_cert(q) <= lb) { m2 = q; break; }
+    m = m2;
+  }
+}
+
+// ---- plan and rows of a listed set of codes ----------------------------------
+// The list search: the first pass over the codes md_fused left (k > kfused,
+// at their levels), then the second pass (level 1) over the codes the first
+// pass could not certify.  Workgroup 0 plans the listed codes (positions by
+// bucket from LDS counters; the order inside a bucket does not matter); the
+// other workgroups build their rows and reset their state words, a wave per
+// listed code in turn.  With an empty list the plan is empty and the walk
+// that follows returns at once.
+constexpr int kPlanThreads = 256;
+constexpr int kPrepRowBlocks = 64;
+
+__device__ __forceinline__ void md_list_plan(const int32_t* __restrict__ offsets, int32_t n,
+                                             const MdParams* __restrict__ p,
                                              MdPlan* __restrict__ plan,
                                              uint32_t* __restrict__ order,
                                              uint32_t* __restrict__ roff,
-                                             uint32_t* __restrict__ claims, uint32_t* ctrl) {
+                                             uint32_t* __restrict__ claims, uint32_t nl,
+                                             const uint32_t* __restrict__ list, bool level1,
+                                             uint32_t* fb_count, uint32_t* ctrl) {
   constexpr int kB = kMaxKExact;
-  __shared__ uint32_t cnt[kB + 1];   // codes per bucket, then bucket bases; [kB]: positions
-  __shared__ uint64_t sxc[kB], sxu[kB];
-  __shared__ uint32_t segsum[kB][kPlanSegs];
-  extern __shared__ uint32_t plan_lds[];
-  // Per bucket and thread: its codes, then its next position.
-  uint32_t(*hist)[kPlanThreads] = reinterpret_cast<uint32_t(*)[kPlanThreads]>(plan_lds);
-  // Code index by position (calls of at most 4096 codes).
-  uint16_t* stage = reinterpret_cast<uint16_t*>(plan_lds + kPlanHistWords);
-  const int tid = threadIdx.x, wave = tid / kWave;
-  const uint32_t lane = (uint32_t)(tid % kWave);
-  uint32_t err = 0;
-  // Wave 0's lane b reads the chunks per code of bucket b (k = kB + 1 - b;
-  // bucket 0, always empty, takes k = kB's) now, with the sizes.
-  const uint32_t ccb = (tid < kB) ? p->chunks[min(kB + 1 - tid, kB)] : 0u;
+  __shared__ uint32_t cnt[kB + 1];
+  const int tid = threadIdx.x;
+  if (tid <= kB) cnt[tid] = 0u;
   if (tid < kClaimShards) claims[tid * kClaimStride] = 0u;
-#pragma unroll
-  for (int b = 0; b < kB; ++b) hist[b][tid] = 0u;   // own column: no barrier before own adds
-  PlanCodes pc;
-
-  // Pass 1: this thread's codes per bucket (non-returning LDS adds).  The
-  // last batch (the only one when n <= 4096) stays in registers.
-  for (int base = 0; base < n; base += kPlanBatch) {
-    md_plan_load_batch(offsets, n, n_idx, base, err, pc);
-#pragma unroll
-    for (int r = 0; r < kPlanRounds; ++r)   // no walk: adds 0 to bucket 0 (never a code's)
-      atomicAdd(&hist[pc.slot(r)][tid], pc.k(r) != 0u ? 1u : 0u);
+  if (tid == 0 && !level1) *fb_count = 0u;   // the second-pass list of this call
+  __syncthreads();
+  for (uint32_t i = tid; i < nl; i += blockDim.x) {
+    const int code = (int)list[i];
+    if (code < n) atomicAdd(&cnt[kB + 1 - (offsets[code + 1] - offsets[code])], 1u);
   }
   __syncthreads();
-
-  // Segment sums: scan thread (sb, sg) takes columns [c0, c0 + kPlanSegLen).
-  const int sb = tid / kPlanSegs, sg = tid % kPlanSegs, c0 = sg * kPlanSegLen;
-  if (sb < kB) {
-    uint32_t v = 0;
-#pragma unroll
-    for (int c = 0; c < kPlanSegLen; ++c)
-      if (c0 + c < kPlanThreads) v += hist[sb][c0 + c];
-    segsum[sb][sg] = v;
-  }
-  __syncthreads();
-
-  // Bucket bases, chunk and unit offsets (wave 0, lane b = bucket b).
-  if (wave == 0) {
-    const int b = (int)lane;
-    // Chunks (saturated at 2^31) and work units per code of bucket b (walk
-    // dimension kB - b).
-    const uint64_t cc = ccb;
-    const uint64_t cu = (b < kB) ? code_units(kB - b) : 0ull;
-    uint32_t t = 0;
-    if (b < kB) {
-#pragma unroll
-      for (int g = 0; g < kPlanSegs; ++g) t += segsum[b][g];
-      cnt[b] = t;
-    }
-    // Chunks of the bucket, clamped at 2^31 (a plan of 2^31 chunks or more
-    // is refused), so the sum of 21 cannot wrap.
-    const uint64_t xc = (t == 0) ? 0ull : (cc >= (1ull << 31)) ? (1ull << 31)
-                                                               : min((uint64_t)t * cc, 1ull << 31);
-    // Units: exact whenever the chunk total is below 2^31 (at most 2^31
-    // chunks of at most 2^23 blocks each).
-    const uint64_t xu = (uint64_t)t * cu;
-    if (b < kB) {
-      sxc[b] = xc;
-      sxu[b] = xu;
-    }
-    // Exclusive sums: the wave's own LDS stores are visible to its later
-    // loads; the loads below b are independent of each other.
-    uint64_t ec = 0, eu = 0;
+  if (tid == 0) {
+    // Chunks of the pass (claimed walk) and at level 1 (the call is refused
+    // when those reach 2^31, the limit the host-pointer call checks, so
+    // neither pass can overflow).
+    uint64_t eu = 0, ec = 0, e1 = 0;
     uint32_t ep = 0;
-#pragma unroll
-    for (int q = 0; q < kB; ++q) {   // branch-free: all 63 loads issue back to back
-      const uint64_t sel = 0ull - (uint64_t)(q < b);
-      ec += sxc[q] & sel;
-      eu += sxu[q] & sel;
-      ep += cnt[q] & (uint32_t)sel;
-    }
-    // A plan of 2^31 chunks or more is refused; its units are zeroed so the
-    // exact walk does nothing either (the call reports the error).
-    const uint64_t all_chunks = readlane64(ec + xc, kB - 1);
-    const bool refused = all_chunks >= (1ull << 31);
-    if (b < kB) {
-      plan->ub[b] = refused ? 0ull : eu;
+    for (int b = 0; b < kB; ++b) {
+      const int k = min(kB + 1 - b, kB);
+      const uint32_t t = cnt[b];
+      const uint64_t cc = level1 ? p->chunks1[k] : p->chunks[k];
+      plan->ub[b] = eu;
       plan->cb[b] = (uint32_t)min(ec, 1ull << 31);
       plan->pb[b] = ep;
       plan->cc[b] = (uint32_t)cc;
+      cnt[b] = ep;
+      eu += (uint64_t)t * anchor_units(k, code_level(p->level, k, level1));
+      ec = min(ec + (uint64_t)t * cc, 1ull << 31);
+      e1 = min(e1 + (uint64_t)t * p->chunks1[k], 1ull << 31);
+      ep += t;
     }
-    // Every lane has read cnt[] above (the loads are ordered before these
-    // stores within the wave).
-    if (b < kB) cnt[b] = ep;
-    if (b == kB - 1) {
-      const uint64_t chunks = ec + xc;
-      const bool overflow = chunks >= (1ull << 31);
-      plan->cb[kB] = overflow ? 0u : (uint32_t)chunks;
-      plan->ub[kB] = overflow ? 0ull : eu + xu;
-      plan->pb[kB] = ep + t;
-      cnt[kB] = ep + t;
-      ctrl[0] = 0u;
-      ctrl[3] = overflow ? 0xFFFFFFFFu : (uint32_t)chunks;
-      if (overflow) atomicOr(&ctrl[2], kErrBadK);
+    const bool refused = ec >= (1ull << 31) || e1 >= (1ull << 31);
+    if (refused) {   // no units, no chunks (the call reports the error)
+      for (int b = 0; b < kB; ++b) plan->ub[b] = 0ull;
+      atomicOr(&ctrl[2], kErrBadK);
     }
+    plan->ub[kB] = refused ? 0ull : eu;
+    plan->cb[kB] = refused ? 0u : (uint32_t)ec;
+    plan->pb[kB] = ep;
   }
   __syncthreads();
-
-  // Each column entry becomes that thread's first position in the bucket.
-  if (sb < kB) {
-    uint32_t at = cnt[sb];
-#pragma unroll
-    for (int g = 0; g < kPlanSegs; ++g) at += (g < sg) ? segsum[sb][g] : 0u;
-    // Two halves of loads in flight (registers: the batch stays live).
-    constexpr int kHalf = (kPlanSegLen + 1) / 2;
-#pragma unroll
-    for (int h = 0; h < kPlanSegLen; h += kHalf) {
-      uint32_t v[kHalf];
-#pragma unroll
-      for (int c = 0; c < kHalf; ++c)
-        v[c] = (h + c < kPlanSegLen && c0 + h + c < kPlanThreads) ? hist[sb][c0 + h + c] : 0u;
-#pragma unroll
-      for (int c = 0; c < kHalf; ++c) {
-        if (h + c < kPlanSegLen && c0 + h + c < kPlanThreads) hist[sb][c0 + h + c] = at;
-        at += v[c];
-      }
-    }
+  for (uint32_t i = tid; i < nl; i += blockDim.x) {
+    const int code = (int)list[i];
+    if (code >= n) continue;
+    const int o = offsets[code];
+    const uint32_t q = atomicAdd(&cnt[kB + 1 - (offsets[code + 1] - o)], 1u);
+    order[q] = (uint32_t)code;
+    roff[q] = (uint32_t)o;
   }
-  __syncthreads();
-
-  // Pass 2: positions (returning adds on the own column).  One batch: still
-  // in registers; more: each loaded again.
-  const bool staged = n <= kPlanBatch;
-  for (int base = staged ? -1 : 0;;) {
-    if (base >= 0) {
-      if (base >= n) break;
-      uint32_t unused = 0;
-      md_plan_load_batch(offsets, n, n_idx, base, unused, pc);
-    }
-    const int first = ((base >= 0) ? base : 0) + tid;
-    if (staged) {
-      // Branch-free: every returning add issues before the first result is
-      // used; a code without a walk writes the spare slot kPlanBatch.
-      uint32_t pos[kPlanRounds];
-#pragma unroll
-      for (int r = 0; r < kPlanRounds; ++r)
-        pos[r] = atomicAdd(&hist[pc.slot(r)][tid], pc.k(r) != 0u ? 1u : 0u);
-      uint32_t pp[kPlanRounds / 2];   // positions (<= 4096) two to a word across the barrier
-#pragma unroll
-      for (int r = 0; r < kPlanRounds; ++r) {
-        pos[r] = pc.k(r) != 0u ? pos[r] : (uint32_t)kPlanBatch;
-        stage[pos[r]] = (uint16_t)(first + r * kPlanThreads);
-        if (r % 2 == 0) pp[r / 2] = pos[r];
-        else pp[r / 2] |= pos[r] << 16;
-      }
-      // The columns are done with: their words take each code's first row
-      // by position.
-      __syncthreads();
-#pragma unroll
-      for (int r = 0; r < kPlanRounds; ++r)
-        plan_lds[(pp[r / 2] >> (16 * (r % 2))) & 0xFFFFu] = (uint32_t)pc.o[r];
-    } else {
-#pragma unroll
-      for (int r = 0; r < kPlanRounds; ++r) {
-        if (pc.k(r) != 0u) {
-          const uint32_t pos = atomicAdd(&hist[pc.bucket(r)][tid], 1u);
-          order[pos] = (uint32_t)(first + r * kPlanThreads);
-          roff[pos] = (uint32_t)pc.o[r];
-        }
-      }
-    }
-    if (base < 0) break;
-    base += kPlanBatch;
-  }
-  if (staged) {
-    __syncthreads();
-    // Contiguous runs out of LDS.
-    const uint32_t npos = cnt[kB];
-#pragma unroll
-    for (int r = 0; r < kPlanRounds; ++r) {
-      const uint32_t q = (uint32_t)(r * kPlanThreads + tid);
-      if (q < npos) {
-        order[q] = stage[q];
-        roff[q] = plan_lds[q];
-      }
-    }
-  }
-  if (err) atomicOr(&ctrl[2], err);
 }
 
-// Plan and bit rows in one launch: workgroup 0 builds the plan, workgroups
-// 1.. write the rows, one wave per code (the two are independent).
-__global__ __launch_bounds__(kPlanThreads, 7) void md_prep_kernel(
+// list_count: entries of `list` (at most n are read; the count may be left
+// over from an interrupted call, whose entries, codes of this batch or
+// beyond it, are then searched again or skipped, never mis-attributed).
+__global__ __launch_bounds__(kPlanThreads) void md_prepl_kernel(
     const int32_t* __restrict__ idx, int64_t n_idx, const uint32_t* __restrict__ mplanes,
     uint32_t* __restrict__ rows, const int32_t* __restrict__ offsets, int32_t n,
     const MdParams* __restrict__ p, MdPlan* __restrict__ plan, uint32_t* __restrict__ order,
     uint32_t* __restrict__ roff, uint32_t* __restrict__ claims, uint64_t* __restrict__ state,
-    int32_t* __restrict__ out, const int32_t* __restrict__ tmax_arr, int32_t tmax_s,
-    uint32_t* ctrl) {
+    int32_t* __restrict__ out, const int32_t* __restrict__ tmax_arr, int32_t tmax_s, MdAux ax,
+    int32_t level1, uint32_t* ctrl) {
+  const uint32_t* cnt = level1 ? ax.fb_count : ax.big_count;
+  const uint32_t* list = level1 ? ax.fb_list : ax.big_list;
+  uint32_t nl = __builtin_amdgcn_readfirstlane(*cnt);
+  nl = min(nl, (uint32_t)max(n, 0));
   if (blockIdx.x == 0) {
-    md_plan_body(offsets, n, n_idx, p, plan, order, roff, claims, ctrl);
+    md_list_plan(offsets, n, p, plan, order, roff, claims, nl, list, level1 != 0, ax.fb_count,
+                 ctrl);
     return;
   }
-  md_rows_code((int)((blockIdx.x - 1) * (kPlanThreads / kWave) + threadIdx.x / kWave), n, n_idx,
-               idx, offsets, mplanes, rows, state, out, tmax_arr, tmax_s, ctrl);
+  constexpr uint32_t kWaves = kPrepRowBlocks * (kPlanThreads / kWave);
+  for (uint32_t i = (blockIdx.x - 1) * (kPlanThreads / kWave) + threadIdx.x / kWave; i < nl;
+       i += kWaves) {
+    const uint32_t code = __builtin_amdgcn_readfirstlane(list[i]);
+    if (code < (uint32_t)n)
+      md_rows_anchor((int)code, n, n_idx, idx, offsets, mplanes, p->level, level1 != 0, rows,
+                     state, ax.aux, out, tmax_arr, tmax_s, ctrl);
+  }
 }
 
 // Chunk ids.  The plan numbers chunks bucket by bucket, largest walk first;
@@ -439,6 +490,8 @@ struct MdShared {
   alignas(8) uint32_t plan[sizeof(MdPlan) / 4];
   uint8_t L[kMaxKExact + 1];
   uint8_t check_log2[kMaxKExact + 1];
+  uint8_t level[kMaxKExact + 1];
+  uint8_t small_k;
 };
 
 __device__ __forceinline__ void md_load_shared(MdShared& sh, const MdPlan* __restrict__ plan,
@@ -449,25 +502,30 @@ __device__ __forceinline__ void md_load_shared(MdShared& sh, const MdPlan* __res
   if (threadIdx.x < kMaxKExact + 1) {
     sh.L[threadIdx.x] = p->L[threadIdx.x];
     sh.check_log2[threadIdx.x] = p->check_log2[threadIdx.x];
+    sh.level[threadIdx.x] = p->level[threadIdx.x];
   }
+  if (threadIdx.x == 0) sh.small_k = p->small_k;
   __syncthreads();
 }
 
 struct MdChunk {
   int code;
-  int w;            // walk dimension
+  int m;            // level of the pass
+  int w;            // walk dimension of a subcode
   int L;
   uint32_t cmask;   // early-exit check mask
   int j;            // Gray piece, or tiny_top(w) for the tiny chunk
   uint32_t c;       // chunk index inside the piece
+  uint32_t s;       // subcode
   uint32_t nch;     // chunks of the code
-  uint32_t roff;    // first row of the code (3 rows per index)
+  uint32_t roff;    // first index of the code (kRowsPerIdx row slots per index)
 };
 
-// Wave-uniform decode of chunk id g < total.
+// Wave-uniform decode of chunk id g < total.  Inside a piece, ids run over
+// (chunk, subcode) pairs, then codes: id = ((c * nsub + s) * codes + q).
 __device__ __forceinline__ MdChunk md_decode(const MdShared& sh, uint32_t g,
                                              const uint32_t* __restrict__ order,
-                                             const uint32_t* __restrict__ roff) {
+                                             const uint32_t* __restrict__ roff, bool level1) {
   const uint32_t* cb = sh.plan + offsetof(MdPlan, cb) / 4;
   const uint32_t* pb = sh.plan + offsetof(MdPlan, pb) / 4;
   const uint32_t* ccv = sh.plan + offsetof(MdPlan, cc) / 4;
@@ -484,21 +542,25 @@ __device__ __forceinline__ MdChunk md_decode(const MdShared& sh, uint32_t g,
   const uint32_t nb = (__builtin_amdgcn_readfirstlane(cb[lo + 1]) - b0) / cc;   // codes
   MdChunk r;
   r.nch = cc;
-  r.w = kMaxKExact - lo;
-  const int kq = min(r.w + 1, kMaxKExact);
-  r.L = __builtin_amdgcn_readfirstlane(sh.L[kq]);
-  r.cmask = (1u << __builtin_amdgcn_readfirstlane(sh.check_log2[kq])) - 1u;
+  const int k = kMaxKExact + 1 - lo;
+  r.m = __builtin_amdgcn_readfirstlane(code_level(sh.level, k, level1));
+  const uint32_t nsub = (uint32_t)level_nsub(r.m);
+  r.w = k - r.m;
+  r.L = __builtin_amdgcn_readfirstlane(sh.L[k]);
+  r.cmask = (1u << __builtin_amdgcn_readfirstlane(sh.check_log2[k])) - 1u;
   uint32_t rel = g - b0;
   const int jt = tiny_top(r.w);
   int j = r.w - 1;
   for (; j > jt; --j) {
-    const uint32_t span = (uint32_t)piece_chunks(j, r.L) * nb;
+    const uint32_t span = (uint32_t)piece_chunks(j, r.L) * nsub * nb;
     if (rel < span) break;
     rel -= span;
   }
   r.j = j;
-  r.c = rel / nb;
-  const uint32_t pos = __builtin_amdgcn_readfirstlane(pb[lo]) + (rel - r.c * nb);
+  const uint32_t qi = rel / nb;
+  r.s = qi % nsub;
+  r.c = qi / nsub;
+  const uint32_t pos = __builtin_amdgcn_readfirstlane(pb[lo]) + (rel - qi * nb);
   r.code = (int)order[pos];
   r.roff = roff[pos];
   return r;
@@ -522,17 +584,33 @@ __device__ __forceinline__ uint32_t md_walk(const MdChunk& ck, const uint32_t* R
 // its result, with an s_waitcnt 0 between them, so the count cannot be
 // performed before the minimum; both halves of one word share a coherence
 // point.  The wave whose count completes the code (all its nch chunks) reads
-// the final minimum there and writes the code's result.
+// the final minimum there and finishes the code (md_finish).
+// A pass at level m > 1 certifies its maximum z only when z >= level_cert(m)
+// (or z > tmax: the threshold is exceeded by a real codeword) and the code
+// had a direction of full rank; any other code is appended to the
+// second-pass list instead of written (gt_common.h).  Lane 0 only.
+__device__ __forceinline__ void md_finish(int code, uint32_t bw, int m, int tmax,
+                                          const MdAux& ax, int32_t* out) {
+  const int z = kNpts - (int)bw;
+  if (m > 1 && (ax.aux[code] != 0u || (z <= tmax && z < level_cert(m)))) {
+    const uint32_t i = atomicAdd(ax.fb_count, 1u);
+    ax.fb_list[i] = (uint32_t)code;
+  } else {
+    out[code] = md_result((int)bw, tmax);
+  }
+}
+
 __device__ __forceinline__ void md_fold(uint64_t* state, uint32_t b, uint32_t n, uint32_t nch,
-                                        int tmax, int32_t* out_code) {
+                                        int code, int m, int tmax, const MdAux& ax,
+                                        int32_t* out) {
   uint32_t* st = reinterpret_cast<uint32_t*>(state);
   const uint32_t prev = __hip_atomic_fetch_min(&st[0], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __builtin_amdgcn_s_waitcnt(0);
   const uint32_t done =
       __hip_atomic_fetch_add(&st[1], n + (prev >> 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + n;
   if (done == nch) {
-    const int bw = (int)__hip_atomic_load(&st[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *out_code = md_result(bw, tmax);
+    const uint32_t bw = __hip_atomic_load(&st[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    md_finish(code, bw, m, tmax, ax, out);
   }
 }
 
@@ -572,6 +650,32 @@ __device__ __forceinline__ uint32_t walk_units(const Rows& row, int w, uint64_t 
   return best;
 }
 
+// The same over units [o0, o1) of a code's nsub subcodes (each code_units(w)
+// units, rows 3w apart from R): one loop over the (subcode, piece) pairs in
+// unit order, subcode by subcode, each subcode's pieces from the largest
+// down and then its tiny part.
+__device__ __forceinline__ uint32_t walk_code_units(const uint32_t* R, int w, int nsub,
+                                                    uint64_t o0, uint64_t o1, uint32_t lane) {
+  const int np = max(w - 1 - kTinyTop, 0);   // pieces above the tiny part
+  uint32_t best = 64u;
+  uint64_t ps = 0;
+  for (int t = 0; t < nsub * (np + 1) && ps < o1; ++t) {
+    const int s = t / (np + 1), q = t - s * (np + 1);
+    const ConstRows row{R + (size_t)s * 3 * w * kRowWords};
+    if (q < np) {
+      const int j = w - 1 - q;
+      const uint64_t nbk = piece_blocks(j);
+      const uint64_t lo = max(o0, ps), hi = min(o1, ps + nbk);
+      if (lo < hi) best = min(best, gray_blocks(row, (uint32_t)j, lo - ps, hi - ps, lane));
+      ps += nbk;
+    } else {
+      if (o0 <= ps) best = min(best, tiny_chunk(row, tiny_top(w), lane));
+      ps += kTinyUnits;
+    }
+  }
+  return best;
+}
+
 // Exact mode: static deals, no claim atomics.  Wave i of W takes
 //  * the i-th of W equal shares (to one code) of the small codes, walk
 //    dimension <= kSmallW, which are the plan's last buckets: each is its
@@ -592,10 +696,18 @@ __global__ __launch_bounds__(256) void md_main_exact_kernel(
     const uint32_t* __restrict__ rows, const MdPlan* __restrict__ plan,
     const uint32_t* __restrict__ order, const uint32_t* __restrict__ roff, uint64_t* state,
     const int32_t* __restrict__ tmax_arr, int32_t tmax_s, int32_t* __restrict__ out,
-    const MdParams* __restrict__ p) {
-  constexpr int kSmallB = kMaxKExact - kSmallW;   // first bucket of small codes
+    const MdParams* __restrict__ p, MdAux ax, int32_t level1) {
   __shared__ MdShared sh;
+  // The first-pass list was read by md_prepl: empty for the next call.
+  if (!level1 && blockIdx.x == 0 && threadIdx.x == 0) *ax.big_count = 0u;
   md_load_shared(sh, plan, p);
+  // First bucket of small codes (k <= small_k: subcodes without a piece
+  // above the tiny part).
+  const int kSmallB = kMaxKExact + 1 -
+                      (level1 ? kSmallW + 1 : (int)__builtin_amdgcn_readfirstlane(sh.small_k));
+  auto lvl = [&](int k) {
+    return (int)__builtin_amdgcn_readfirstlane(code_level(sh.level, k, level1 != 0));
+  };
   const uint32_t lane = lane_id();
   const uint64_t* ub = reinterpret_cast<const uint64_t*>(sh.plan + offsetof(MdPlan, ub) / 4);
   const uint32_t* pb = sh.plan + offsetof(MdPlan, pb) / 4;
@@ -623,10 +735,14 @@ __global__ __launch_bounds__(256) void md_main_exact_kernel(
     int lo = kSmallB;
     for (uint32_t pos = P0 + (uint32_t)d.start(wid); pos < pe; ++pos) {
       while (pos >= pbu(lo + 1)) ++lo;
+      const int k = kMaxKExact + 1 - lo, m = lvl(k), w = k - m;
       const int code = (int)order[pos];
-      const uint32_t* R = rows + (size_t)roff[pos] * 3 * kRowWords;
-      const uint32_t b = wave_min(tiny_chunk(ConstRows{R}, tiny_top(kMaxKExact - lo), lane));
-      if (lane == 0) out[code] = md_result((int)b, tmax_arr ? tmax_arr[code] : tmax_s);
+      const uint32_t* R = rows + (size_t)roff[pos] * kRowsPerIdx * kRowWords;
+      uint32_t b = 64u;
+      for (int s = 0; s < level_nsub(m); ++s)
+        b = min(b, tiny_chunk(ConstRows{R + (size_t)s * 3 * w * kRowWords}, tiny_top(w), lane));
+      b = wave_min(b);
+      if (lane == 0) md_finish(code, b, m, tmax_arr ? tmax_arr[code] : tmax_s, ax, out);
     }
   }
 
@@ -644,26 +760,28 @@ __global__ __launch_bounds__(256) void md_main_exact_kernel(
     if (ub[mid] <= u0) lo = mid; else hi = mid - 1;
   }
   lo = __builtin_amdgcn_readfirstlane(lo);
-  uint64_t cu = code_units(kMaxKExact - lo);
+  int m = lvl(kMaxKExact + 1 - lo);
+  uint64_t cu = anchor_units(kMaxKExact + 1 - lo, m);
   const uint64_t rel = u0 - uniform64(ub[lo]);
   uint32_t qc = (uint32_t)udiv64(rel, cu);
   uint64_t off = rel - (uint64_t)qc * cu;
   uint64_t u = u0;
   for (;;) {
-    const int w = kMaxKExact - lo;
+    const int w = kMaxKExact + 1 - lo - m;
     const uint32_t pos = pbu(lo) + qc;
     const int code = (int)order[pos];
-    const uint32_t* R = rows + (size_t)roff[pos] * 3 * kRowWords;
+    const uint32_t* R = rows + (size_t)roff[pos] * kRowsPerIdx * kRowWords;
     const uint64_t a = u - off;           // first unit of the code
     const uint64_t e = min(a + cu, u1);   // end of this wave's part of it
-    const uint32_t b = wave_min(walk_units(ConstRows{R}, w, off, e - a, lane));
+    const uint32_t b = wave_min(walk_code_units(R, w, level_nsub(m), off, e - a, lane));
     const int tmax = tmax_arr ? tmax_arr[code] : tmax_s;
     if (a >= u0 && a + cu <= u1) {
-      if (lane == 0) out[code] = md_result((int)b, tmax);
+      if (lane == 0) md_finish(code, b, m, tmax, ax, out);
     } else {
       const uint64_t first = (a < u0) ? deal.wave_of(a) : wid;
       const uint64_t last = (a + cu > u1) ? deal.wave_of(a + cu - 1) : wid;
-      if (lane == 0) md_fold(&state[code], b, 1u, (uint32_t)(last - first + 1), tmax, &out[code]);
+      if (lane == 0)
+        md_fold(&state[code], b, 1u, (uint32_t)(last - first + 1), code, m, tmax, ax, out);
     }
     u = e;
     if (u >= u1) break;
@@ -674,7 +792,8 @@ __global__ __launch_bounds__(256) void md_main_exact_kernel(
         ++lo;
       } while (pbu(lo + 1) == pbu(lo));
       qc = 0;
-      cu = code_units(kMaxKExact - lo);
+      m = lvl(kMaxKExact + 1 - lo);
+      cu = anchor_units(kMaxKExact + 1 - lo, m);
     }
   }
 }
@@ -696,9 +815,11 @@ __global__ __launch_bounds__(256) void md_main_exact_kernel(
 // id with lane 0 inactive, walking chunk 0 forever.)
 __global__ __launch_bounds__(256) void md_main_thr_kernel(
     const uint32_t* __restrict__ rows, const MdPlan* __restrict__ plan,
-    const uint32_t* __restrict__ order, const uint32_t* __restrict__ roff, uint32_t* claims, uint64_t* state, const int32_t* __restrict__ tmax_arr, int32_t tmax_s,
-    int32_t* __restrict__ out, const MdParams* __restrict__ p) {
+    const uint32_t* __restrict__ order, const uint32_t* __restrict__ roff, uint32_t* claims,
+    uint64_t* state, const int32_t* __restrict__ tmax_arr, int32_t tmax_s,
+    int32_t* __restrict__ out, const MdParams* __restrict__ p, MdAux ax, int32_t level1) {
   __shared__ MdShared sh;
+  if (!level1 && blockIdx.x == 0 && threadIdx.x == 0) *ax.big_count = 0u;
   md_load_shared(sh, plan, p);
   const uint32_t lane = lane_id();
   const uint32_t total = __builtin_amdgcn_readfirstlane(sh.plan[offsetof(MdPlan, cb) / 4 + kMaxKExact]);
@@ -709,15 +830,17 @@ __global__ __launch_bounds__(256) void md_main_thr_kernel(
   uint32_t g = __builtin_amdgcn_readfirstlane(first) * kClaimShards + shard;
   for (;;) {
     const bool have = g < total;
-    int code = 0, tmax = 0;
+    int code = 0, tmax = 0, m = 1;
     bool fold = false;
     uint32_t b = 64u, nch = 0;
     if (have) {
-      const MdChunk ck = md_decode(sh, g, order, roff);
+      const MdChunk ck = md_decode(sh, g, order, roff, level1 != 0);
       code = ck.code;
       nch = ck.nch;
+      m = ck.m;
       fold = true;   // walked or skipped, the chunk is counted
-      const uint32_t* R = rows + (size_t)ck.roff * 3 * kRowWords;
+      const uint32_t* R =
+          rows + ((size_t)ck.roff * kRowsPerIdx + (size_t)ck.s * 3 * ck.w) * kRowWords;
       tmax = tmax_arr ? tmax_arr[code] : tmax_s;
       const int thr = 48 - tmax;
       const uint32_t* best_g = reinterpret_cast<const uint32_t*>(&state[code]);
@@ -741,7 +864,7 @@ __global__ __launch_bounds__(256) void md_main_thr_kernel(
     }
     uint32_t next = 0;
     if (lane == 0) {
-      if (fold) md_fold(&state[code], b, 1u, nch, tmax, &out[code]);
+      if (fold) md_fold(&state[code], b, 1u, nch, code, m, tmax, ax, out);
       next = atomicAdd(&claims[shard * kClaimStride], 1u);
     }
     g = __builtin_amdgcn_readfirstlane(next) * kClaimShards + shard;
@@ -920,17 +1043,28 @@ hipError_t launch_md_rows(const int32_t* d_idx, int64_t n_idx, const uint32_t* d
   return hipGetLastError();
 }
 
-hipError_t launch_md_prep(const int32_t* d_idx, int64_t n_idx, const uint32_t* d_mplanes,
-                          uint32_t* d_rows, const int32_t* d_offsets, int32_t n,
-                          const MdParams* d_params, MdPlan* d_plan, uint32_t* d_order,
-                          uint32_t* d_roff, uint32_t* d_claims, uint64_t* d_state, int32_t* d_out,
-                          const int32_t* d_tmax, int32_t tmax, uint32_t* d_ctrl, hipStream_t s) {
-  constexpr int kCodesPerBlock = kPlanThreads / kWave;   // one wave per code
-  const int64_t row_blocks = ((int64_t)std::max(n, 0) + kCodesPerBlock - 1) / kCodesPerBlock;
-  hipLaunchKernelGGL(md_prep_kernel, dim3((unsigned)(1 + row_blocks)), dim3(kPlanThreads),
-                     (unsigned)kPlanDynLds, s,
-                     d_idx, n_idx, d_mplanes, d_rows, d_offsets, n, d_params, d_plan, d_order,
-                     d_roff, d_claims, d_state, d_out, d_tmax, tmax, d_ctrl);
+hipError_t launch_md_fused(const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets,
+                           int32_t n, const uint32_t* d_mplanes, const MdParams* d_params,
+                           const int32_t* d_tmax, int32_t tmax, int32_t* d_out, const MdAux& ax,
+                           int kfused, int kmax, uint32_t* d_ctrl, hipStream_t s,
+                           hipEvent_t ev_start, hipEvent_t ev_stop) {
+  if (n <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)(((int64_t)n + kFusedWaves - 1) / kFusedWaves);
+  hipExtLaunchKernelGGL(md_fused_kernel, dim3(blocks), dim3(kFusedWaves * kWave), 0, s, ev_start,
+                        ev_stop, 0, d_idx, n_idx, d_offsets, n, d_mplanes, d_params, d_tmax, tmax,
+                        d_out, ax.big_count, ax.big_list, kfused, kmax, d_ctrl);
+  return hipGetLastError();
+}
+
+hipError_t launch_md_prepl(bool level1, const int32_t* d_idx, int64_t n_idx,
+                           const uint32_t* d_mplanes, uint32_t* d_rows, const int32_t* d_offsets,
+                           int32_t n, const MdParams* d_params, MdPlan* d_plan, uint32_t* d_order,
+                           uint32_t* d_roff, uint32_t* d_claims, uint64_t* d_state,
+                           int32_t* d_out, const int32_t* d_tmax, int32_t tmax, const MdAux& ax,
+                           uint32_t* d_ctrl, hipStream_t s) {
+  hipLaunchKernelGGL(md_prepl_kernel, dim3(1 + kPrepRowBlocks), dim3(kPlanThreads), 0, s, d_idx,
+                     n_idx, d_mplanes, d_rows, d_offsets, n, d_params, d_plan, d_order, d_roff,
+                     d_claims, d_state, d_out, d_tmax, tmax, ax, level1 ? 1 : 0, d_ctrl);
   return hipGetLastError();
 }
 
@@ -946,17 +1080,19 @@ int md_main_blocks_per_cu(bool thresholded) {
 hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
                           const MdPlan* d_plan, const uint32_t* d_order, const uint32_t* d_roff,
                           uint32_t* d_claims, uint64_t* d_state, const int32_t* d_tmax,
-                          int32_t tmax, int32_t* d_out, const MdParams* d_params, hipStream_t s,
-                          hipEvent_t ev_start, hipEvent_t ev_stop) {
+                          int32_t tmax, int32_t* d_out, const MdParams* d_params, const MdAux& ax,
+                          bool level1, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
   // With events, the launch records them from its own dispatch (start and
   // end of the kernel), without marker packets of their own on the stream.
+  const int32_t l1 = level1 ? 1 : 0;
   if (thresholded)
     hipExtLaunchKernelGGL(md_main_thr_kernel, dim3(grid), dim3(256), 0, s, ev_start, ev_stop, 0,
                           d_rows, d_plan, d_order, d_roff, d_claims, d_state, d_tmax, tmax, d_out,
-                          d_params);
+                          d_params, ax, l1);
   else
     hipExtLaunchKernelGGL(md_main_exact_kernel, dim3(grid), dim3(256), 0, s, ev_start, ev_stop, 0,
-                          d_rows, d_plan, d_order, d_roff, d_state, d_tmax, tmax, d_out, d_params);
+                          d_rows, d_plan, d_order, d_roff, d_state, d_tmax, tmax, d_out, d_params,
+                          ax, l1);
   return hipGetLastError();
 }
 

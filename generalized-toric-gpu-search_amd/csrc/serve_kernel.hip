@@ -159,20 +159,6 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, uint32_t lane) {
 }
 
 // Rows of the request a wave is walking, staged in that wave's LDS area.
-struct LdsRows {
-  const uint32_t* R;
-  __device__ __forceinline__ RowV operator()(uint32_t r) const {
-    const uint32_t* p = R + r * kRowWords;
-    RowV v;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) v.w[i] = p[i];
-    return v;
-  }
-  // Rows read from LDS are already in VGPRs.
-  __device__ __forceinline__ RowV vgpr(const RowV& v) const { return v; }
-  __device__ __forceinline__ RowV vgpr_after(const RowV& v, uint32_t) const { return v; }
-};
-
 // Claims as in md_main_thr_kernel: one lane-0 region per iteration (it completes
 // the previous chunk and claims the next id), read back with readfirstlane
 // while every lane is active; all other conditions are made wave-uniform.

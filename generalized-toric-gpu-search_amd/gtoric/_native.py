@@ -35,8 +35,14 @@ _SIGS = {
     "gt_device_info": ([vp, P(i32), P(i32), P(i32), ctypes.c_char_p, i32], ctypes.c_int),
     "gt_set_md_params": ([vp, i32, i32, i32], ctypes.c_int),
     "gt_get_md_params": ([vp, i32, P(i32), P(i32)], ctypes.c_int),
+    "gt_set_md_level": ([vp, i32, i32], ctypes.c_int),
+    "gt_get_md_level": ([vp, i32, P(i32)], ctypes.c_int),
     "gt_md_batch": ([vp, vp, vp, i32, vp, i32, i64, u64, vp], ctypes.c_int),
     "gt_md_batch_dev": ([vp, vp, i64, vp, i32, vp, i32, i32, i64, u64, vp, vp], ctypes.c_int),
+    "gt_md_batch_dev2": ([vp, vp, i64, vp, i32, vp, i32, i32, i32, i64, u64, vp, vp],
+                         ctypes.c_int),
+    "gt_set_md_fused_max_k": ([vp, i32], ctypes.c_int),
+    "gt_get_md_fused_max_k": ([vp, P(i32)], ctypes.c_int),
     "gt_md_batch_dev_noalloc": ([vp, vp, i64, vp, i32, vp, i32, i32, vp, vp], ctypes.c_int),
     "gt_md_batch_dev_noalloc_limits": ([P(i32), P(i64)], ctypes.c_int),
     "gt_md_single": ([vp, vp, i32, i32, P(i32)], ctypes.c_int),
@@ -164,6 +170,14 @@ class Context:
     def set_md_params(self, k: int, L: int, check_log2: int) -> None:
         check(lib().gt_set_md_params(self.handle, int(k), int(L), int(check_log2)))
 
+    def set_md_level(self, k: int, level: int) -> None:
+        check(lib().gt_set_md_level(self.handle, int(k), int(level)))
+
+    def get_md_level(self, k: int) -> int:
+        m = ctypes.c_int32()
+        check(lib().gt_get_md_level(self.handle, int(k), ctypes.byref(m)))
+        return m.value
+
     def get_md_params(self, k: int) -> tuple[int, int]:
         L, c = i32(), i32()
         check(lib().gt_get_md_params(self.handle, int(k), ctypes.byref(L), ctypes.byref(c)))
@@ -192,12 +206,29 @@ class Context:
 
     def md_batch_dev(self, d_idx: int, n_idx: int, d_offsets: int, n: int, d_out: int,
                      tmax: int = 48, d_tmax: int | None = None, thresholded: bool = False,
-                     sample_count: int = 0, sample_seed: int = 0, stream: int | None = None):
-        check(lib().gt_md_batch_dev(self.handle, vp(d_idx), int(n_idx), vp(d_offsets), int(n),
-                                    vp(d_tmax) if d_tmax else None, int(tmax),
-                                    int(bool(thresholded)), int(sample_count),
-                                    ctypes.c_uint64(int(sample_seed) & 0xFFFF_FFFF_FFFF_FFFF),
-                                    vp(d_out), vp(stream) if stream else None))
+                     sample_count: int = 0, sample_seed: int = 0, stream: int | None = None,
+                     kmax: int | None = None):
+        """gt_md_batch_dev, or gt_md_batch_dev2 when the caller bounds the set
+        sizes by kmax (a larger set is then an input error)."""
+        seed = ctypes.c_uint64(int(sample_seed) & 0xFFFF_FFFF_FFFF_FFFF)
+        st = vp(stream) if stream else None
+        dt = vp(d_tmax) if d_tmax else None
+        if kmax is None:
+            check(lib().gt_md_batch_dev(self.handle, vp(d_idx), int(n_idx), vp(d_offsets), int(n),
+                                        dt, int(tmax), int(bool(thresholded)), int(sample_count),
+                                        seed, vp(d_out), st))
+        else:
+            check(lib().gt_md_batch_dev2(self.handle, vp(d_idx), int(n_idx), vp(d_offsets),
+                                         int(n), dt, int(tmax), int(bool(thresholded)),
+                                         int(kmax), int(sample_count), seed, vp(d_out), st))
+
+    def set_md_fused_max_k(self, k: int) -> None:
+        check(lib().gt_set_md_fused_max_k(self.handle, int(k)))
+
+    def get_md_fused_max_k(self) -> int:
+        k = ctypes.c_int32()
+        check(lib().gt_get_md_fused_max_k(self.handle, ctypes.byref(k)))
+        return k.value
 
     def md_batch_dev_noalloc(self, d_idx: int, n_idx: int, d_offsets: int, n: int, d_out: int,
                              tmax: int = 48, d_tmax: int | None = None,

@@ -1,21 +1,25 @@
 """Autotuner for the min-distance launch parameters, per k.
 
-Tunable per k (gt_set_md_params; a code of k points walks the (k-1)-dimensional
-subcode vanishing at the torus point (1,1), csrc/gt_common.h):
-  L           Gray walk bits per lane of the claimed (thresholded) walk and of
-              the serving kernel; one chunk = 64 lanes x 2^L codewords, so L
-              sets the number of chunks per code: code_chunks(k - 1, L).
+Tunable per k (csrc/gt_common.h):
+  level       first-pass level (gt_set_md_level): a level-m pass walks
+              level_nsub(m) subcodes of dimension k - m and certifies a maximum
+              of at least 7m - 6 zeros; lower results are searched again at a
+              lower level.  Higher levels walk less but certify less.
+  L           Gray walk bits per lane of the claimed (thresholded) planned walk
+              (codes above the one-wave bound), the one-code launch and the
+              serving kernel; one chunk = 64 lanes x 2^L codewords.
   check_log2  early-exit check every 2^check_log2 blocks of 16 steps.
-The exact walk takes neither: it deals equal runs of 16-step blocks.
+Codes of at most 8 points are searched one wave per code and read neither L
+nor check_log2; the exact planned walk deals equal runs of 16-step blocks.
 
-For each k the tuner times device kernels (HIP events around md_main, read
-through gt_prof_read) on a seeded single-k batch thresholded at its median
-distance (about half the codes stop early), keeps the fastest setting, and
-writes a JSON table keyed by device architecture.  refine() then moves L by
-coordinate descent on a mixed batch, where the chunks of all k share the
-waves.  Results never depend on the parameters
-(tests/test_gpu_distance.py::test_params_do_not_change_results); only speed
-does.
+The tuner times the whole exact search on device (HIP events around the
+one-wave kernel and both planned passes, read through gt_prof_read) on seeded
+single-k batches: tune_levels() exactly, tune() thresholded at the batch's
+median distance (about half the codes stop early).  It keeps the fastest
+setting and writes a JSON table keyed by device architecture.  refine() then
+moves L by coordinate descent on a mixed batch.  Results never depend on the
+parameters (tests/test_gpu_distance.py::test_params_do_not_change_results,
+::test_levels_do_not_change_results); only speed does.
 """
 
 from __future__ import annotations
@@ -65,17 +69,22 @@ def _batch(k: int, n: int, seed: int) -> list[list[int]]:
     return [sorted(rng.choice(49, k, replace=False).tolist()) for _ in range(n)]
 
 
+SEARCH_KERNELS = ("md_fused", "md_main", "md_main2")
+
+
 def _time_md(ctx: _native.Context, flat, offs, tmax, repeats: int) -> float:
+    """Device time of one exact (or thresholded) search: the one-wave kernel
+    plus both planned passes, per call."""
     ctx.md_batch(flat, offs, tmax)              # warm
-    ctx.prof_read("md_main")
+    for name in SEARCH_KERNELS:
+        ctx.prof_read(name)
     ctx.prof_enable(True)
     try:
         for _ in range(repeats):
             ctx.md_batch(flat, offs, tmax)
     finally:
         ctx.prof_enable(False)
-    n, ms = ctx.prof_read("md_main")
-    return ms / max(n, 1)
+    return sum(ctx.prof_read(name)[1] for name in SEARCH_KERNELS) / repeats
 
 
 def _median_tmax(ctx: _native.Context, flat, offs) -> int:
@@ -119,6 +128,47 @@ def tune(ctx: _native.Context, ks=range(3, 11), codes_per_k: dict | None = None,
         table[str(k)] = {"L": L, "check_log2": bestc[0], "thr_ms": round(bestc[1], 5),
                          "tmax": tmax, "codes": n}
     return {"arch": ctx.arch, "cu_count": ctx.cu_count, "params": table}
+
+
+def level_range(ctx: _native.Context, k: int) -> list[int]:
+    """Levels k may take next to its neighbours' levels: k - level must not
+    decrease with k, i.e. level[k] <= level[k-1] + 1 and level[k] >=
+    level[k+1] - 1, and 1 <= level[k] <= min(k - 1, 6)."""
+    lo = ctx.get_md_level(k + 1) - 1 if k < 21 else 1
+    hi = ctx.get_md_level(k - 1) + 1 if k > 3 else 1
+    return [m for m in range(max(lo, 1), min(hi, k - 1, 6) + 1)]
+
+
+def tune_levels(ctx: _native.Context, ks=range(4, 11), codes_per_k: dict | None = None,
+                repeats: int = 3, verbose: bool = True) -> dict:
+    """Time the exact search of a seeded single-k batch at every level k may
+    take (increasing k, each next to the levels already chosen) and keep the
+    fastest."""
+    from .distance import flatten
+    codes_per_k = codes_per_k or {4: 8192, 5: 8192, 6: 4096, 7: 4096, 8: 2048, 9: 1024,
+                                  10: 256, 11: 32, 12: 4}
+    out = {}
+    for k in ks:
+        sets = _batch(k, codes_per_k.get(k, 1), 2000 + k)
+        flat, offs = flatten(sets)
+        exp = None
+        best = None
+        for m in level_range(ctx, k):
+            ctx.set_md_level(k, m)
+            t = _time_md(ctx, flat, offs, 48, repeats)
+            got = ctx.md_batch(flat, offs, 48)
+            if exp is None:
+                exp = got
+            elif not np.array_equal(got, exp):
+                raise RuntimeError(f"k={k}: level {m} changed results")
+            if verbose:
+                print(f"[tune] k={k} level={m} exact search {t:.4f} ms")
+            if best is None or t < best[1]:
+                best = (m, t)
+        ctx.set_md_level(k, best[0])
+        out[str(k)] = {"level": best[0], "exact_ms": round(best[1], 5),
+                       "codes": len(sets)}
+    return out
 
 
 def refine(ctx: _native.Context, sets, repeats: int = 5, margin: float = 0.005,
@@ -185,6 +235,13 @@ def apply(ctx: _native.Context, path: Path | None = None) -> bool:
     data = json.loads(p.read_text())
     for k, v in data.get("params", {}).items():
         ctx.set_md_params(int(k), int(v["L"]), int(v["check_log2"]))
+    levels = {int(k): int(v["level"]) for k, v in data.get("levels", {}).items()}
+    for k in sorted(levels):                  # raised in increasing k,
+        if levels[k] > ctx.get_md_level(k):
+            ctx.set_md_level(k, levels[k])
+    for k in sorted(levels, reverse=True):    # lowered in decreasing k
+        if levels[k] < ctx.get_md_level(k):
+            ctx.set_md_level(k, levels[k])
     return True
 
 
@@ -194,12 +251,25 @@ if __name__ == "__main__":
     ap.add_argument("ks", nargs="*", type=int, default=list(range(3, 11)))
     ap.add_argument("--out", type=Path, help="write the table here instead of tuned/<arch>.json")
     ap.add_argument("--repeats", type=int, default=3)
+    ap.add_argument("--levels", action="store_true",
+                    help="load the saved table, tune the first-pass level of each k in ks "
+                         "(exact search of a single-k batch) and save it with the table")
     ap.add_argument("--refine-mix", action="store_true",
                     help="load the saved table and refine L on the batched workload "
                          "(4096 codes, k uniform in 3..8, thresholded at its median) "
                          "instead of tuning per k")
     a = ap.parse_args()
     c = _native.Context(0)
+    if a.levels:
+        p = a.out or tuned_path(c.arch)
+        res = json.loads(p.read_text()) if p.exists() else {"arch": c.arch, "cu_count": c.cu_count,
+                                                             "params": {}}
+        apply(c, p)
+        res["levels"] = tune_levels(c, [k for k in a.ks if k >= 4], repeats=a.repeats)
+        p.parent.mkdir(parents=True, exist_ok=True)
+        p.write_text(json.dumps(res, indent=1) + "\n")
+        print(json.dumps(res["levels"]), "saved", p)
+        raise SystemExit(0)
     if a.refine_mix:
         from . import workloads
         p = a.out or tuned_path(c.arch)

@@ -61,6 +61,19 @@ int gt_device_info(gt_ctx* ctx, int32_t* cu_count, int32_t* clock_khz,
 int gt_set_md_params(gt_ctx* ctx, int32_t k, int32_t L, int32_t check_log2);
 int gt_get_md_params(gt_ctx* ctx, int32_t k, int32_t* L, int32_t* check_log2);
 
+/* First-pass level of the exact and thresholded batch search for codes of k
+ * generators (3 <= k <= 21; k <= 2 walks at level 1); part of the per-k
+ * parameters above, with the same effect on results (none).  A level-m pass
+ * walks the subcodes that vanish at m points of a line through the torus
+ * point (1,1) (1 subcode for m = 1, 2, 5, 6, 3 for m = 3, 4, of dimension
+ * k - m) and certifies a maximum of at least 7m - 6 zeros; codes below that
+ * are searched again at level 1 (the subcode through (1,1), which the
+ * reference's enumeration of all 8^k - 1 coefficient vectors, kernel.py:
+ * 128-139, amounts to).  1 <= level <= min(k - 1, 6), and k - level must not
+ * decrease with k.  Defaults: 1 for k <= 3, 2 for k = 4..6, 3 above. */
+int gt_set_md_level(gt_ctx* ctx, int32_t k, int32_t level);
+int gt_get_md_level(gt_ctx* ctx, int32_t k, int32_t* level);
+
 /* Batched minimum distance, mixed k (replaces DistanceOracle.max_zeros_batch,
  * kernel.py:276-323, and DistanceOracleCUDABP.max_zeros_batch,
  * kernel_cuda_bp.py:290-323; kernel eval_min_distance_batch_bp,
@@ -78,6 +91,24 @@ int gt_md_batch_dev(gt_ctx* ctx, const int32_t* d_idx, int64_t n_idx,
                     const int32_t* d_offsets, int32_t n, const int32_t* d_tmax_per_set,
                     int32_t tmax, int32_t thresholded, int64_t sample_count,
                     uint64_t sample_seed, int32_t* d_out, void* stream);
+
+/* gt_md_batch_dev for a batch whose sets have at most kmax points (a larger
+ * set is an input error: result 0, reported by gt_sync).  Codes of at most
+ * 8 points are searched one wavefront per code in one launch; with kmax <= 8
+ * (and the one-wave bound at its default, gt_set_md_fused_max_k) that
+ * launch is the whole call.  gt_md_batch_dev is this call with kmax = 21;
+ * host-pointer calls take kmax from their offsets.  Replaces the same
+ * reference interface as gt_md_batch (kernel.py:276-323). */
+int gt_md_batch_dev2(gt_ctx* ctx, const int32_t* d_idx, int64_t n_idx,
+                     const int32_t* d_offsets, int32_t n, const int32_t* d_tmax_per_set,
+                     int32_t tmax, int32_t thresholded, int32_t kmax, int64_t sample_count,
+                     uint64_t sample_seed, int32_t* d_out, void* stream);
+
+/* Largest set size searched one wavefront per code (1..8, default 8); larger
+ * codes go through the planned walk over all CUs.  Results never depend on
+ * it (1 sends every code with a walk through the planned walk). */
+int gt_set_md_fused_max_k(gt_ctx* ctx, int32_t k);
+int gt_get_md_fused_max_k(gt_ctx* ctx, int32_t* k);
 
 /* Allocation-free variant of gt_md_batch_dev (exact or thresholded; no
  * sampling).  No counterpart in the reference: it is for callers that cannot

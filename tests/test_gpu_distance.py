@@ -207,6 +207,70 @@ def test_params_do_not_change_results(dist, md_gold):
             ctx.set_md_params(k, L, c)
 
 
+def _force_levels(ctx, table):
+    """Set first-pass levels (raised in increasing k, then lowered in
+    decreasing k, so every step keeps k - level non-decreasing); returns the
+    previous table."""
+    saved = {k: ctx.get_md_level(k) for k in range(3, 22)}
+    for k in sorted(table):
+        if table[k] > saved[k]:
+            ctx.set_md_level(k, table[k])
+    for k in sorted(table, reverse=True):
+        if table[k] < saved[k]:
+            ctx.set_md_level(k, table[k])
+    return saved
+
+
+def _restore_levels(ctx, saved):
+    _force_levels(ctx, saved)
+
+
+def test_levels_do_not_change_results(dist, md_gold):
+    """First-pass levels and the one-wave bound change the work, never the
+    result.  With the highest level per k (k - 1 up to 6) most codes fall
+    below their level's certificate, or have no direction of full rank, and
+    are searched again at a lower level (in the same wave) or at level 1
+    (the planned walk's second pass): every exact and thresholded result
+    still equals the oracle fixture (config 2, and the per-set thresholded
+    batch), with codes of k <= 8 searched one wave per code and with every
+    code through the planned walk."""
+    ctx = dist.context
+    assert [ctx.get_md_level(k) for k in (2, 3, 4, 6, 7, 8, 12, 21)] == [1, 1, 2, 2, 3, 3, 3, 3]
+    sets = _rows(md_gold["c2_flat"], md_gold["c2_offsets"])
+    exp = md_gold["c2_mz"].tolist()
+    tsets = _rows(md_gold["thr_flat"], md_gold["thr_offsets"])
+    assert ctx.get_md_fused_max_k() == 8
+    for fused in (8, 1):            # one wave per code (k <= 8), or the planned walk for all
+        ctx.set_md_fused_max_k(fused)
+        try:
+            for table in ({},
+                          {k: min(k - 1, 6) for k in range(3, 9)},
+                          {3: 2, 4: 3, 5: 4, 6: 4, 7: 4, 8: 4, 9: 4},
+                          {k: 1 for k in range(3, 22)}):
+                saved = _force_levels(ctx, table)
+                try:
+                    assert dist.max_zeros_batch(sets, 1) == exp, (fused, table)
+                    got = dist.max_zeros_batch(tsets, md_gold["thr_td"])
+                    assert got == md_gold["thr_mz"].tolist(), (fused, table)
+                finally:
+                    _restore_levels(ctx, saved)
+        finally:
+            ctx.set_md_fused_max_k(8)
+    assert dist.max_zeros_batch(sets, 1) == exp
+
+
+def test_level_settings_are_checked(dist):
+    from gtoric._native import GtError
+    ctx = dist.context
+    with pytest.raises(GtError):
+        ctx.set_md_level(5, 5)      # above k - 1
+    with pytest.raises(GtError):
+        ctx.set_md_level(9, 7)      # above the largest level
+    with pytest.raises(GtError):
+        ctx.set_md_level(9, 5)      # k - level would drop from 8 - 3 to 9 - 5
+    assert ctx.get_md_level(9) == 3
+
+
 @pytest.mark.parametrize("k", [3, 4])
 def test_every_subset_of_size_3_and_4(dist, k):
     """All C(49, k) codes: exact distances equal the oracle's, and are
