@@ -120,18 +120,18 @@ void build_mplanes(uint32_t out[kNpts * kRowWords]) {
   }
 }
 
-// Default Gray walk bits per k (the walk has dimension k - 1): big enough
-// that per-chunk set-up is a few percent of the walk, small enough for a
-// balanced tail; for k >= 19 large enough that a code has at most about 2^24
-// chunks (the chunk ids of one call must stay below 2^31).
 // Default first-pass level per k: level m walks level_nsub(m) subcodes of
 // dimension k - m and certifies a maximum of at least 7m - 6 zeros.  On the
 // 4096-code mixed batch (k = 3..8) these levels certify every code: k = 4..6
 // have at least 9 zeros (level 2 needs 8), k = 7, 8 at least 15 (level 3);
 // k = 3 codes often have 7 (level 1).  For k >= 9 level 3 leaves to the
-// second pass only codes of distance 35 or more.
+// second pass (level 2) only codes of distance 35 or more.
 int default_level(int k) { return (k <= 3) ? 1 : (k <= 6) ? 2 : 3; }
 
+// Default Gray walk bits per k (a level-1 walk has dimension k - 1): big
+// enough that per-chunk set-up is a few percent of the walk, small enough
+// for a balanced tail; for k >= 19 large enough that a code has at most
+// about 2^24 chunks (the chunk ids of one call must stay below 2^31).
 int default_L(int k) {
   static const uint8_t tab[kMaxKExact + 1] = {4, 4, 4, 4, 4, 4, 4, 7, 8, 10, 11, 12,
                                               13, 14, 15, 16, 17, 18, 19, 21, 24, 27};
@@ -358,7 +358,7 @@ struct gt_ctx {
   // list count (one word, 0 between calls).  md_fused searches the codes of
   // k <= fused_k and lists the others; when kmax allows larger codes the
   // list search follows (md_prepl and md_main at the codes' levels, then at
-  // level 1 for the codes the first pass could not certify).  Every word a
+  // second_level(k) for the codes the first pass could not certify).  Every word a
   // kernel reads is written before by an earlier kernel of the call, so no
   // clearing is needed.
   int md_exact_launch(const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets, int32_t n,

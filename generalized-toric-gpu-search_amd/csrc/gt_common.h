@@ -54,12 +54,17 @@
 // every codeword walked is a codeword of C(S), so the maximum z_m found is
 // at most z* in any case.  Hence z_m >= level_cert(m) proves z_m = z*
 // (otherwise z* < level_cert(m), and z_m <= z*); a code below that is
-// walked again at level 1.  C_R has dimension k - m when the m conditions
-// are independent; the rows are built from a Gauss-Jordan kernel basis in
-// the first direction where all representatives give rank m (codes with
-// none also go to level 1).  Level 1 is the subcode through (1,1) above
-// (certified whenever z* >= 1, i.e. always).  A level-3 code walks 3 (8^(k-3)
-// - 1) / 7 codewords, 21x fewer than level 1.
+// walked again at a level whose certificate a known bound meets.  Any k - 1
+// points are the zeros of some nonzero codeword (k coefficients, k - 1
+// conditions), so z* >= k - 1: level 2 is certified for every code of k >= 9
+// and level 1 (the subcode through (1,1) above, certified whenever z* >= 1)
+// for all.  C_R has dimension k - m when the m conditions are independent;
+// the rows are a basis from Gaussian elimination on the codewords in the
+// first direction where all representatives give m independent conditions
+// (a code with none is also searched again).  At level 2 that fails only
+// when every index of the set is the same point, and such a set spans the
+// zero codeword (49 zeros).  A level-3 code walks 3 (8^(k-3) - 1) / 7
+// codewords, 21x fewer than level 1.
 #pragma once
 #include <stdint.h>
 
@@ -91,10 +96,12 @@ struct MdParams {
   // Largest k whose first-pass subcodes have no piece above the tiny part
   // (walk dimension k - level[k] <= kSmallW): the small codes of that pass.
   uint8_t small_k;
-  // Chunks of a code of k generators at L[k] in the first pass (level[k])
-  // and at level 1 (the second pass), saturated at 2^31; kept with L and
+  // Chunks of a code of k generators at L[k] in the first pass (level[k]),
+  // in the second pass (second_level(k)) and at level 1 (the limit a call is
+  // refused at: 2^31 chunks in total), saturated at 2^31; kept with L and
   // level by md_params_set / md_level_set.
   uint32_t chunks[kMaxKExact + 1];
+  uint32_t chunks2[kMaxKExact + 1];
   uint32_t chunks1[kMaxKExact + 1];
 };
 
@@ -154,20 +161,29 @@ GT_HD uint64_t code_chunks(int w, int L) {
 
 // ---- anchored subcodes (see the top of this file) ------------------------
 constexpr int kMaxLevel = 6;   // (a level-7 reduction spills registers; it would need z >= 43)
-// Representatives of the m-subsets of Z/7 under r -> 2^i r + t, as bit
-// masks over r (bit r set <=> r in R).
+// Representatives of the m-subsets of Z/7 under r -> 2^i r + t, one per
+// orbit, chosen to share the prefix {0, .., m-2}: representative s is that
+// prefix and the point level_last(m, s).  The orbits have 7 (m = 1, 6), 21
+// (m = 2, 5) and 21 + 7 + 7 (m = 3, 4: {0,1,2}, {0,1,3}, {0,1,5} and
+// {0,1,2,3}, {0,1,2,4}, {0,1,2,5}) members.
 constexpr GT_HD int level_nsub(int m) { return (m == 3 || m == 4) ? 3 : 1; }
+constexpr GT_HD int level_last(int m, int s) {
+  return (m == 3) ? (s == 0 ? 2 : s == 1 ? 3 : 5) : (m == 4) ? (s == 0 ? 3 : s == 1 ? 4 : 5) : m - 1;
+}
+// The representative as a bit mask over r (bit r set <=> r in R).
 constexpr GT_HD uint32_t level_rep(int m, int s) {
-  if (m == 3) return s == 0 ? 0x07u : s == 1 ? 0x0Bu : 0x0Du;   // {0,1,2} {0,1,3} {0,2,3}
-  if (m == 4) return s == 0 ? 0x0Fu : s == 1 ? 0x17u : 0x1Du;   // {0,1,2,3} {0,1,2,4} {0,2,3,4}
-  return (1u << m) - 1u;                                          // {0, .., m-1}
+  return ((1u << (m - 1)) - 1u) | (1u << level_last(m, s));
 }
 // Zeros that certify a level-m maximum (7 (m - 1) + 1).
 GT_HD int level_cert(int m) { return 7 * m - 6; }
-// Level of a code of k generators in a pass (pass 2 and codes with k <= 2:
-// level 1; k above the exact range, which no plan holds: 1).
-GT_HD int code_level(const uint8_t* level, int k, bool level1) {
-  return (level1 || k <= 2 || k > kMaxKExact) ? 1 : (int)level[k];
+// Level of the second pass over a code of k generators: 2 when k - 1 meets
+// its certificate (k >= 9), else 1.
+GT_HD int second_level(int k) { return (k >= 9) ? 2 : 1; }
+// Level of a code of k generators in the first or the second pass (codes
+// with k <= 2: level 1; k above the exact range, which no plan holds: 1).
+GT_HD int code_level(const uint8_t* level, int k, bool second) {
+  if (k <= 2 || k > kMaxKExact) return 1;
+  return second ? second_level(k) : (int)level[k];
 }
 // Walk dimension of one subcode of a code of k generators at level m.
 GT_HD int anchor_dim(int k, int m) { return (k <= 1) ? 0 : k - m; }
@@ -182,6 +198,8 @@ inline uint32_t sat31(uint64_t c) { return (c >= (1ull << 31)) ? (1u << 31) : (u
 inline void md_params_refresh(MdParams& p, int k) {
   const int m = code_level(p.level, k, false);
   p.chunks[k] = sat31((uint64_t)level_nsub(m) * code_chunks(anchor_dim(k, m), p.L[k]));
+  const int m2 = code_level(p.level, k, true);
+  p.chunks2[k] = sat31((uint64_t)level_nsub(m2) * code_chunks(anchor_dim(k, m2), p.L[k]));
   p.chunks1[k] = sat31(code_chunks(walk_dim(k), p.L[k]));
   int sk = 1;
   for (int q = 2; q <= kMaxKExact; ++q)

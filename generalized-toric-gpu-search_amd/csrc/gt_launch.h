@@ -25,7 +25,7 @@ hipError_t launch_md_rows(const int32_t* d_idx, int64_t n_idx, const uint32_t* d
 // md_fused leaves to the list search (k > kfused), reset by the first-pass
 // md_main; aux[code] = 1 when a list code's first pass found no direction of
 // full rank; fb_count / fb_list: the codes the first pass hands to the
-// second (level 1), reset by the first-pass md_prepl.
+// second pass, reset by the first-pass md_prepl.
 struct MdAux {
   uint32_t* aux;
   uint32_t* fb_count;
@@ -37,7 +37,7 @@ struct MdAux {
 // k <= kfused, rows in LDS, results written; larger codes listed), then,
 // unless the caller knows that no code has more than kfused generators, the
 // list search: md_prepl and md_main at the codes' levels, md_prepl and
-// md_main at level 1 for the codes the first could not certify.
+// md_main at second_level(k) for the codes the first could not certify.
 // d_rows: kRowsPerIdx row slots per index.
 // d_claims: kClaimShards counters, kClaimStride words apart (reset by the plan).
 // d_order, d_roff: n entries each (code index and first index per plan position).
@@ -49,7 +49,7 @@ hipError_t launch_md_fused(const int32_t* d_idx, int64_t n_idx, const int32_t* d
                            const int32_t* d_tmax, int32_t tmax, int32_t* d_out, const MdAux& ax,
                            int kfused, int kmax, uint32_t* d_ctrl, hipStream_t s,
                            hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
-hipError_t launch_md_prepl(bool level1, const int32_t* d_idx, int64_t n_idx,
+hipError_t launch_md_prepl(bool second, const int32_t* d_idx, int64_t n_idx,
                            const uint32_t* d_mplanes, uint32_t* d_rows, const int32_t* d_offsets,
                            int32_t n, const MdParams* d_params, MdPlan* d_plan, uint32_t* d_order,
                            uint32_t* d_roff, uint32_t* d_claims, uint64_t* d_state,
@@ -59,7 +59,7 @@ hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
                           const MdPlan* d_plan, const uint32_t* d_order, const uint32_t* d_roff,
                           uint32_t* d_claims, uint64_t* d_state, const int32_t* d_tmax,
                           int32_t tmax, int32_t* d_out, const MdParams* d_params, const MdAux& ax,
-                          bool level1, hipStream_t s, hipEvent_t ev_start = nullptr,
+                          bool second, hipStream_t s, hipEvent_t ev_start = nullptr,
                           hipEvent_t ev_stop = nullptr);
 // One code, one launch: grid = code_chunks(w, L) single-wave workgroups, the
 // 3w rows (w <= 20) in the kernel arguments.  d_state: {64, 0} before the

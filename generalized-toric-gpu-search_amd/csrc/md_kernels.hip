@@ -29,8 +29,9 @@
 //               chunk folded at once (other waves stop on the folded value).
 //               Finishing a code writes max_zeros clamped at tmax + 1, or,
 //               below the level's certificate, lists the code for
-//   md_prepl,   the same two launches at level 1 over that list (the second
-//   md_main     pass), which always finish.
+//   md_prepl,   the same two launches over that list (the second pass), at
+//   md_main     the level the bound z* >= k - 1 certifies (2 for k >= 9,
+//               else 1), which always finish.
 //   The sampled path keeps the full rows x^b * g_i of C(S) (md_rows).
 
 // Inner step (per projective codeword, all VALU; the block's rows are
@@ -92,102 +93,108 @@ __device__ __forceinline__ uint32_t f8_mul(uint32_t a, uint32_t b) {
   return p;
 }
 
-// Line directions through 0 of (Z/7)^2: (1,0), (0,1), (1,1), ..., (1,6).
-__device__ __forceinline__ int dir_proj(int d, int a, int b) {
-  return (d == 0) ? a : (d == 1) ? b : (a + (d - 1) * b) % 7;
+// Evaluation-matrix column (point index) of the exponent vector r d on the
+// line of direction d: (1,0), (0,1), (1,1), ..., (1,6) for d = 0..7.  Point
+// j is (x, y) = (j / 7 + 1, j % 7 + 1) as GF(8) elements, (alpha^u, alpha^v)
+// = (kF8Exp[u], kF8Exp[v]).
+__device__ __forceinline__ int line_point(int d, int r) {
+  const int u = (d == 1) ? 0 : r;
+  const int v = (d == 0) ? 0 : (d == 1) ? r : (r * (d - 1)) % 7;
+  const int x = (int)((kF8Exp >> (3 * u)) & 7u), y = (int)((kF8Exp >> (3 * v)) & 7u);
+  return (x - 1) * 7 + (y - 1);
+}
+
+// Value of the codeword h (bit planes) at point p (wave-uniform).
+__device__ __forceinline__ uint32_t cw_at(const uint32_t h[6], int p) {
+  const bool hi = p >= 32;
+  const int bit = p & 31;
+  return (((hi ? h[1] : h[0]) >> bit) & 1u) | ((((hi ? h[3] : h[2]) >> bit) & 1u) << 1) |
+         ((((hi ? h[5] : h[4]) >> bit) & 1u) << 2);
+}
+
+// One elimination step of the wave's basis (lane t holds h_t, act: still in
+// the basis): the first active lane whose codeword is nonzero at point p
+// becomes the pivot and leaves the basis; every other active lane subtracts
+// the multiple of the pivot codeword that zeroes its own value at p.
+// Returns false when no active codeword is nonzero at p (the condition is
+// dependent on the earlier ones: not of full rank).
+__device__ __forceinline__ bool cw_eliminate(uint32_t h[6], bool& act, int p, uint32_t lane) {
+  const uint32_t e = cw_at(h, p);
+  const uint64_t cand = __ballot(act && e != 0u);
+  if (cand == 0ull) return false;
+  const int t0 = __builtin_ctzll(cand);
+  const uint32_t e0 = (uint32_t)__builtin_amdgcn_readlane((int)e, t0);
+  const uint32_t c = f8_mul(e, (kF8Inv >> (3 * e0)) & 7u);   // h_t(p) / h_t0(p)
+  const bool upd = act && lane != (uint32_t)t0;
+  uint32_t H[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) H[q] = (uint32_t)__builtin_amdgcn_readlane((int)h[q], t0);
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    const uint32_t msk = upd ? 0u - ((c >> b) & 1u) : 0u;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) h[q] ^= H[q] & msk;
+    mul_x(H);
+  }
+  act = upd;
+  return true;
 }
 
 // Bit rows of one code at level M (one wave; lane i < k holds index g of its
-// exponent point (a, b)): subcode s (s < level_nsub(M)) of the
-// representative R_s takes rows s * 3w + 3t + b = x^b * h_t (w = k - M,
-// t < w), where h_t = g_f + sum_r E[r][f] g_{piv[r]} for the t-th free
-// column f of the reduced matrix E (a kernel basis of the evaluations at
-// R_s d), in the first direction d where every R_s has rank M.  Returns
-// false when no direction has.
+// exponent point): subcode s (s < level_nsub(M)) of the representative R_s
+// takes rows s * 3w + 3t + b = x^b * h_t (w = k - M, t < w), where h_0 ..
+// h_{w-1} span the codewords of C(S) that vanish at the points r d, r in
+// R_s, in the first direction d where every R_s gives M independent
+// conditions.  Returns false when no direction does.
 //
-// Gauss-Jordan reduction with one column per lane: monomial i at the point
-// r d is alpha^(r * pr_i), pr_i = (a_i, b_i) . d mod 7, and lane i keeps the
-// column's entries for r = 0..6 three bits each (row r at bits 3r).  For
-// each row r of R in turn the pivot is the first column (ballot) with a
-// nonzero entry that is not a pivot yet; the row is scaled to 1 there and
-// eliminated from the other rows of R with the pivot column's entries
-// (read from the pivot lane), all lanes updating their own column.  (M is a
-// template argument so that the row loops unroll with fixed trip counts.)
+// The basis comes from Gaussian elimination on the codewords themselves:
+// lane i starts with g_i and each point of the representative removes one
+// pivot lane (cw_eliminate); the prefix {0, .., M-2} shared by the
+// representatives is eliminated once per direction, then each
+// representative's last point on a copy.  (M is a template argument so the
+// prefix loop unrolls.)
 template <int M>
 __device__ __forceinline__ bool md_rows_level(int g, int k, uint32_t lane,
                                               const uint32_t* __restrict__ mplanes,
                                               uint32_t* __restrict__ base) {
   constexpr int nsub = level_nsub(M);
   const int w = k - M;
-  const bool act = lane < (uint32_t)k;
-  const int a = g / 7, b = g - 7 * (g / 7);
+  uint32_t own[6];
+  {
+    const uint4 r0 = *reinterpret_cast<const uint4*>(mplanes + g * kRowWords);
+    const uint2 r1 = *reinterpret_cast<const uint2*>(mplanes + g * kRowWords + 4);
+    own[0] = r0.x; own[1] = r0.y; own[2] = r0.z; own[3] = r0.w; own[4] = r1.x; own[5] = r1.y;
+  }
 #pragma unroll 1
   for (int d = 0; d < 8; ++d) {
-    const int pr = dir_proj(d, a, b);
-    uint32_t col0 = 0u;
+    uint32_t h[6];
 #pragma unroll
-    for (int r = 0; r < 7; ++r) col0 |= ((kF8Exp >> (3 * ((r * pr) % 7))) & 7u) << (3 * r);
+    for (int q = 0; q < 6; ++q) h[q] = own[q];
+    bool act = lane < (uint32_t)k;
     bool ok = true;
+#pragma unroll
+    for (int r = 0; r < M - 1; ++r) ok = ok && cw_eliminate(h, act, line_point(d, r), lane);
+    if (!ok) continue;
 #pragma unroll 1
     for (int s = 0; s < nsub; ++s) {
-      const uint32_t R = level_rep(M, s);
-      uint32_t col = col0, taken = 0u;
-      int piv[7];
+      uint32_t hs[6];
 #pragma unroll
-      for (int r = 0; r < 7; ++r) {
-        piv[r] = 0;
-        if (!ok || !((R >> r) & 1u)) continue;
-        const uint32_t e = (col >> (3 * r)) & 7u;
-        const uint64_t nz = __ballot(act && e != 0u && !((taken >> (lane & 31u)) & 1u));
-        if (nz == 0ull) {
-          ok = false;
-          continue;
-        }
-        const int c = __builtin_ctzll(nz);
-        taken |= 1u << c;
-        piv[r] = c;
-        const uint32_t P = (uint32_t)__builtin_amdgcn_readlane((int)col, c);
-        const uint32_t er = f8_mul(e, (kF8Inv >> (3 * ((P >> (3 * r)) & 7u))) & 7u);
-        col = (col & ~(7u << (3 * r))) | (er << (3 * r));
-#pragma unroll
-        for (int q = 0; q < 7; ++q) {
-          if (q == r || !((R >> q) & 1u)) continue;
-          col ^= f8_mul((P >> (3 * q)) & 7u, er) << (3 * q);
-        }
+      for (int q = 0; q < 6; ++q) hs[q] = h[q];
+      bool as = act;
+      if (!cw_eliminate(hs, as, line_point(d, level_last(M, s)), lane)) {
+        ok = false;
+        break;
       }
-      if (!ok) break;
-      // h = g_f + sum_r E[r][f] g_{piv[r]} on the lanes f of free columns.
-      uint32_t h[6];
-      {
-        const uint4 r0 = *reinterpret_cast<const uint4*>(mplanes + g * kRowWords);
-        const uint2 r1 = *reinterpret_cast<const uint2*>(mplanes + g * kRowWords + 4);
-        h[0] = r0.x; h[1] = r0.y; h[2] = r0.z; h[3] = r0.w; h[4] = r1.x; h[5] = r1.y;
-      }
-#pragma unroll
-      for (int r = 0; r < 7; ++r) {
-        if (!((R >> r) & 1u)) continue;
-        const int gp = __builtin_amdgcn_readlane(g, piv[r]);
-        uint32_t v[6];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) v[q] = mplanes[gp * kRowWords + q];
-        const uint32_t e = (col >> (3 * r)) & 7u;
-#pragma unroll
-        for (int bp = 0; bp < 3; ++bp) {
-          const uint32_t msk = 0u - ((e >> bp) & 1u);
-#pragma unroll
-          for (int q = 0; q < 6; ++q) h[q] ^= v[q] & msk;
-          mul_x(v);
-        }
-      }
-      if (act && !((taken >> lane) & 1u)) {
-        const int t = __builtin_popcount(~taken & ((1u << lane) - 1u));
+      const uint64_t live = __ballot(as);
+      if (as) {
+        const int t = __builtin_popcountll(live & ((1ull << lane) - 1ull));
         uint32_t* out_rows = base + (size_t)(s * 3 * w + 3 * t) * kRowWords;
 #pragma unroll
         for (int bp = 0; bp < 3; ++bp) {
           uint4* rr = reinterpret_cast<uint4*>(out_rows + bp * kRowWords);
-          rr[0] = make_uint4(h[0], h[1], h[2], h[3]);
-          rr[1] = make_uint4(h[4], h[5], 0u, 0u);
-          mul_x(h);
+          rr[0] = make_uint4(hs[0], hs[1], hs[2], hs[3]);
+          rr[1] = make_uint4(hs[4], hs[5], 0u, 0u);
+          mul_x(hs);
         }
       }
     }
@@ -207,7 +214,7 @@ __device__ __forceinline__ void md_rows_anchor(int code, int32_t n, int64_t n_id
                                                const int32_t* __restrict__ idx,
                                                const int32_t* __restrict__ offsets,
                                                const uint32_t* __restrict__ mplanes,
-                                               const uint8_t* __restrict__ level, bool level1,
+                                               const uint8_t* __restrict__ level, bool second,
                                                uint32_t* __restrict__ rows, uint64_t* state,
                                                uint32_t* aux, int32_t* out,
                                                const int32_t* tmax_arr, int32_t tmax_s,
@@ -237,7 +244,7 @@ __device__ __forceinline__ void md_rows_anchor(int code, int32_t n, int64_t n_id
   }
   if (k < 2) return;
   uint32_t* base = rows + (size_t)o * kRowsPerIdx * kRowWords;
-  const int m = __builtin_amdgcn_readfirstlane(code_level(level, k, level1));
+  const int m = __builtin_amdgcn_readfirstlane(code_level(level, k, second));
   bool ok;
   switch (m) {
     case 1: ok = md_rows_level<1>(g, k, lane, mplanes, base); break;
@@ -247,7 +254,21 @@ __device__ __forceinline__ void md_rows_anchor(int code, int32_t n, int64_t n_id
     case 5: ok = md_rows_level<5>(g, k, lane, mplanes, base); break;
     default: ok = md_rows_level<6>(g, k, lane, mplanes, base); break;   // kMaxLevel
   }
-  if (!ok && lane == 0) aux[code] = 1u;
+  if (!ok) {
+    if (second) {
+      // Second pass, level 2 without a full-rank direction: every index is
+      // the same point, the set spans the zero codeword, and zero rows give
+      // its 49 zeros.
+      const int w = k - m;
+      for (int r = (int)lane; r < level_nsub(m) * 3 * w; r += kWave) {
+        uint4* rr = reinterpret_cast<uint4*>(base + (size_t)r * kRowWords);
+        rr[0] = make_uint4(0u, 0u, 0u, 0u);
+        rr[1] = make_uint4(0u, 0u, 0u, 0u);
+      }
+    } else if (lane == 0) {
+      aux[code] = 1u;
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict__ idx,
@@ -265,12 +286,15 @@ __global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict_
 // cross-wave fold, one launch.  A first-level maximum below the level's
 // certificate (or a level without a direction of full rank) is followed, in
 // the same wave, by the highest lower level whose certificate the bound
-// max(z, k - 1) already meets (level 1 at worst: 21 rows, 299,593
-// codewords for k = 8).  Codes with more generators are appended to the
-// list of the plan-and-deal search (md_prepl, md_main).
+// max(z, k - 1) already meets (level 1 at worst: 21 rows, 299,593 codewords
+// for k = 8).  Codes with more generators are appended to the list of the
+// planned search (md_prepl, md_main).  (Dealing the walks of a workgroup's
+// four codes over its four waves measured slower: 26.9 against 23.6 us on
+// the 4096-code mixed batch, the dealing's scalar work outweighing the
+// balance it bought.)
 constexpr int kFusedRows = 45;   // 3 subcodes x 3 (kFusedMaxK - 3) rows
 static_assert(kFusedRows == 3 * 3 * (kFusedMaxK - 3), "the level-3 rows of the largest fused code");
-constexpr int kFusedWaves = 4;   // waves (codes) per workgroup
+constexpr int kFusedWaves = 4;   // codes (and waves) per workgroup
 
 // Rows of level m into `base` (md_rows_level); false: no full-rank direction.
 __device__ __forceinline__ bool md_rows_switch(int m, int g, int k, uint32_t lane,
@@ -377,8 +401,8 @@ __global__ __launch_bounds__(kFusedWaves * kWave) void md_fused_kernel(
 
 // ---- plan and rows of a listed set of codes ----------------------------------
 // The list search: the first pass over the codes md_fused left (k > kfused,
-// at their levels), then the second pass (level 1) over the codes the first
-// pass could not certify.  Workgroup 0 plans the listed codes (positions by
+// at their levels), then the second pass (second_level(k)) over the codes
+// the first pass could not certify.  Workgroup 0 plans the listed codes (positions by
 // bucket from LDS counters; the order inside a bucket does not matter); the
 // other workgroups build their rows and reset their state words, a wave per
 // listed code in turn.  With an empty list the plan is empty and the walk
@@ -392,14 +416,14 @@ __device__ __forceinline__ void md_list_plan(const int32_t* __restrict__ offsets
                                              uint32_t* __restrict__ order,
                                              uint32_t* __restrict__ roff,
                                              uint32_t* __restrict__ claims, uint32_t nl,
-                                             const uint32_t* __restrict__ list, bool level1,
+                                             const uint32_t* __restrict__ list, bool second,
                                              uint32_t* fb_count, uint32_t* ctrl) {
   constexpr int kB = kMaxKExact;
   __shared__ uint32_t cnt[kB + 1];
   const int tid = threadIdx.x;
   if (tid <= kB) cnt[tid] = 0u;
   if (tid < kClaimShards) claims[tid * kClaimStride] = 0u;
-  if (tid == 0 && !level1) *fb_count = 0u;   // the second-pass list of this call
+  if (tid == 0 && !second) *fb_count = 0u;   // the second-pass list of this call
   __syncthreads();
   for (uint32_t i = tid; i < nl; i += blockDim.x) {
     const int code = (int)list[i];
@@ -415,13 +439,13 @@ __device__ __forceinline__ void md_list_plan(const int32_t* __restrict__ offsets
     for (int b = 0; b < kB; ++b) {
       const int k = min(kB + 1 - b, kB);
       const uint32_t t = cnt[b];
-      const uint64_t cc = level1 ? p->chunks1[k] : p->chunks[k];
+      const uint64_t cc = second ? p->chunks2[k] : p->chunks[k];
       plan->ub[b] = eu;
       plan->cb[b] = (uint32_t)min(ec, 1ull << 31);
       plan->pb[b] = ep;
       plan->cc[b] = (uint32_t)cc;
       cnt[b] = ep;
-      eu += (uint64_t)t * anchor_units(k, code_level(p->level, k, level1));
+      eu += (uint64_t)t * anchor_units(k, code_level(p->level, k, second));
       ec = min(ec + (uint64_t)t * cc, 1ull << 31);
       e1 = min(e1 + (uint64_t)t * p->chunks1[k], 1ull << 31);
       ep += t;
@@ -455,13 +479,13 @@ __global__ __launch_bounds__(kPlanThreads) void md_prepl_kernel(
     const MdParams* __restrict__ p, MdPlan* __restrict__ plan, uint32_t* __restrict__ order,
     uint32_t* __restrict__ roff, uint32_t* __restrict__ claims, uint64_t* __restrict__ state,
     int32_t* __restrict__ out, const int32_t* __restrict__ tmax_arr, int32_t tmax_s, MdAux ax,
-    int32_t level1, uint32_t* ctrl) {
-  const uint32_t* cnt = level1 ? ax.fb_count : ax.big_count;
-  const uint32_t* list = level1 ? ax.fb_list : ax.big_list;
+    int32_t second, uint32_t* ctrl) {
+  const uint32_t* cnt = second ? ax.fb_count : ax.big_count;
+  const uint32_t* list = second ? ax.fb_list : ax.big_list;
   uint32_t nl = __builtin_amdgcn_readfirstlane(*cnt);
   nl = min(nl, (uint32_t)max(n, 0));
   if (blockIdx.x == 0) {
-    md_list_plan(offsets, n, p, plan, order, roff, claims, nl, list, level1 != 0, ax.fb_count,
+    md_list_plan(offsets, n, p, plan, order, roff, claims, nl, list, second != 0, ax.fb_count,
                  ctrl);
     return;
   }
@@ -470,7 +494,7 @@ __global__ __launch_bounds__(kPlanThreads) void md_prepl_kernel(
        i += kWaves) {
     const uint32_t code = __builtin_amdgcn_readfirstlane(list[i]);
     if (code < (uint32_t)n)
-      md_rows_anchor((int)code, n, n_idx, idx, offsets, mplanes, p->level, level1 != 0, rows,
+      md_rows_anchor((int)code, n, n_idx, idx, offsets, mplanes, p->level, second != 0, rows,
                      state, ax.aux, out, tmax_arr, tmax_s, ctrl);
   }
 }
@@ -525,7 +549,7 @@ struct MdChunk {
 // (chunk, subcode) pairs, then codes: id = ((c * nsub + s) * codes + q).
 __device__ __forceinline__ MdChunk md_decode(const MdShared& sh, uint32_t g,
                                              const uint32_t* __restrict__ order,
-                                             const uint32_t* __restrict__ roff, bool level1) {
+                                             const uint32_t* __restrict__ roff, bool second) {
   const uint32_t* cb = sh.plan + offsetof(MdPlan, cb) / 4;
   const uint32_t* pb = sh.plan + offsetof(MdPlan, pb) / 4;
   const uint32_t* ccv = sh.plan + offsetof(MdPlan, cc) / 4;
@@ -543,7 +567,7 @@ __device__ __forceinline__ MdChunk md_decode(const MdShared& sh, uint32_t g,
   MdChunk r;
   r.nch = cc;
   const int k = kMaxKExact + 1 - lo;
-  r.m = __builtin_amdgcn_readfirstlane(code_level(sh.level, k, level1));
+  r.m = __builtin_amdgcn_readfirstlane(code_level(sh.level, k, second));
   const uint32_t nsub = (uint32_t)level_nsub(r.m);
   r.w = k - r.m;
   r.L = __builtin_amdgcn_readfirstlane(sh.L[k]);
@@ -590,9 +614,10 @@ __device__ __forceinline__ uint32_t md_walk(const MdChunk& ck, const uint32_t* R
 // had a direction of full rank; any other code is appended to the
 // second-pass list instead of written (gt_common.h).  Lane 0 only.
 __device__ __forceinline__ void md_finish(int code, uint32_t bw, int m, int tmax,
-                                          const MdAux& ax, int32_t* out) {
+                                          const MdAux& ax, int32_t* out, bool second) {
   const int z = kNpts - (int)bw;
-  if (m > 1 && (ax.aux[code] != 0u || (z <= tmax && z < level_cert(m)))) {
+  // The second pass's level is certified by the bound z* >= k - 1.
+  if (!second && m > 1 && (ax.aux[code] != 0u || (z <= tmax && z < level_cert(m)))) {
     const uint32_t i = atomicAdd(ax.fb_count, 1u);
     ax.fb_list[i] = (uint32_t)code;
   } else {
@@ -602,7 +627,7 @@ __device__ __forceinline__ void md_finish(int code, uint32_t bw, int m, int tmax
 
 __device__ __forceinline__ void md_fold(uint64_t* state, uint32_t b, uint32_t n, uint32_t nch,
                                         int code, int m, int tmax, const MdAux& ax,
-                                        int32_t* out) {
+                                        int32_t* out, bool second) {
   uint32_t* st = reinterpret_cast<uint32_t*>(state);
   const uint32_t prev = __hip_atomic_fetch_min(&st[0], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __builtin_amdgcn_s_waitcnt(0);
@@ -610,7 +635,7 @@ __device__ __forceinline__ void md_fold(uint64_t* state, uint32_t b, uint32_t n,
       __hip_atomic_fetch_add(&st[1], n + (prev >> 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + n;
   if (done == nch) {
     const uint32_t bw = __hip_atomic_load(&st[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    md_finish(code, bw, m, tmax, ax, out);
+    md_finish(code, bw, m, tmax, ax, out, second);
   }
 }
 
@@ -696,17 +721,17 @@ __global__ __launch_bounds__(256) void md_main_exact_kernel(
     const uint32_t* __restrict__ rows, const MdPlan* __restrict__ plan,
     const uint32_t* __restrict__ order, const uint32_t* __restrict__ roff, uint64_t* state,
     const int32_t* __restrict__ tmax_arr, int32_t tmax_s, int32_t* __restrict__ out,
-    const MdParams* __restrict__ p, MdAux ax, int32_t level1) {
+    const MdParams* __restrict__ p, MdAux ax, int32_t second) {
   __shared__ MdShared sh;
   // The first-pass list was read by md_prepl: empty for the next call.
-  if (!level1 && blockIdx.x == 0 && threadIdx.x == 0) *ax.big_count = 0u;
+  if (!second && blockIdx.x == 0 && threadIdx.x == 0) *ax.big_count = 0u;
   md_load_shared(sh, plan, p);
   // First bucket of small codes (k <= small_k: subcodes without a piece
-  // above the tiny part).
+  // above the tiny part; in the second pass k <= 5, at level 1).
   const int kSmallB = kMaxKExact + 1 -
-                      (level1 ? kSmallW + 1 : (int)__builtin_amdgcn_readfirstlane(sh.small_k));
+                      (second ? kSmallW + 1 : (int)__builtin_amdgcn_readfirstlane(sh.small_k));
   auto lvl = [&](int k) {
-    return (int)__builtin_amdgcn_readfirstlane(code_level(sh.level, k, level1 != 0));
+    return (int)__builtin_amdgcn_readfirstlane(code_level(sh.level, k, second != 0));
   };
   const uint32_t lane = lane_id();
   const uint64_t* ub = reinterpret_cast<const uint64_t*>(sh.plan + offsetof(MdPlan, ub) / 4);
@@ -742,7 +767,8 @@ __global__ __launch_bounds__(256) void md_main_exact_kernel(
       for (int s = 0; s < level_nsub(m); ++s)
         b = min(b, tiny_chunk(ConstRows{R + (size_t)s * 3 * w * kRowWords}, tiny_top(w), lane));
       b = wave_min(b);
-      if (lane == 0) md_finish(code, b, m, tmax_arr ? tmax_arr[code] : tmax_s, ax, out);
+      if (lane == 0)
+        md_finish(code, b, m, tmax_arr ? tmax_arr[code] : tmax_s, ax, out, second != 0);
     }
   }
 
@@ -776,12 +802,13 @@ __global__ __launch_bounds__(256) void md_main_exact_kernel(
     const uint32_t b = wave_min(walk_code_units(R, w, level_nsub(m), off, e - a, lane));
     const int tmax = tmax_arr ? tmax_arr[code] : tmax_s;
     if (a >= u0 && a + cu <= u1) {
-      if (lane == 0) md_finish(code, b, m, tmax, ax, out);
+      if (lane == 0) md_finish(code, b, m, tmax, ax, out, second != 0);
     } else {
       const uint64_t first = (a < u0) ? deal.wave_of(a) : wid;
       const uint64_t last = (a + cu > u1) ? deal.wave_of(a + cu - 1) : wid;
       if (lane == 0)
-        md_fold(&state[code], b, 1u, (uint32_t)(last - first + 1), code, m, tmax, ax, out);
+        md_fold(&state[code], b, 1u, (uint32_t)(last - first + 1), code, m, tmax, ax, out,
+                second != 0);
     }
     u = e;
     if (u >= u1) break;
@@ -817,9 +844,9 @@ __global__ __launch_bounds__(256) void md_main_thr_kernel(
     const uint32_t* __restrict__ rows, const MdPlan* __restrict__ plan,
     const uint32_t* __restrict__ order, const uint32_t* __restrict__ roff, uint32_t* claims,
     uint64_t* state, const int32_t* __restrict__ tmax_arr, int32_t tmax_s,
-    int32_t* __restrict__ out, const MdParams* __restrict__ p, MdAux ax, int32_t level1) {
+    int32_t* __restrict__ out, const MdParams* __restrict__ p, MdAux ax, int32_t second) {
   __shared__ MdShared sh;
-  if (!level1 && blockIdx.x == 0 && threadIdx.x == 0) *ax.big_count = 0u;
+  if (!second && blockIdx.x == 0 && threadIdx.x == 0) *ax.big_count = 0u;
   md_load_shared(sh, plan, p);
   const uint32_t lane = lane_id();
   const uint32_t total = __builtin_amdgcn_readfirstlane(sh.plan[offsetof(MdPlan, cb) / 4 + kMaxKExact]);
@@ -834,7 +861,7 @@ __global__ __launch_bounds__(256) void md_main_thr_kernel(
     bool fold = false;
     uint32_t b = 64u, nch = 0;
     if (have) {
-      const MdChunk ck = md_decode(sh, g, order, roff, level1 != 0);
+      const MdChunk ck = md_decode(sh, g, order, roff, second != 0);
       code = ck.code;
       nch = ck.nch;
       m = ck.m;
@@ -864,7 +891,7 @@ __global__ __launch_bounds__(256) void md_main_thr_kernel(
     }
     uint32_t next = 0;
     if (lane == 0) {
-      if (fold) md_fold(&state[code], b, 1u, nch, code, m, tmax, ax, out);
+      if (fold) md_fold(&state[code], b, 1u, nch, code, m, tmax, ax, out, second != 0);
       next = atomicAdd(&claims[shard * kClaimStride], 1u);
     }
     g = __builtin_amdgcn_readfirstlane(next) * kClaimShards + shard;
@@ -1056,7 +1083,7 @@ hipError_t launch_md_fused(const int32_t* d_idx, int64_t n_idx, const int32_t* d
   return hipGetLastError();
 }
 
-hipError_t launch_md_prepl(bool level1, const int32_t* d_idx, int64_t n_idx,
+hipError_t launch_md_prepl(bool second, const int32_t* d_idx, int64_t n_idx,
                            const uint32_t* d_mplanes, uint32_t* d_rows, const int32_t* d_offsets,
                            int32_t n, const MdParams* d_params, MdPlan* d_plan, uint32_t* d_order,
                            uint32_t* d_roff, uint32_t* d_claims, uint64_t* d_state,
@@ -1064,7 +1091,7 @@ hipError_t launch_md_prepl(bool level1, const int32_t* d_idx, int64_t n_idx,
                            uint32_t* d_ctrl, hipStream_t s) {
   hipLaunchKernelGGL(md_prepl_kernel, dim3(1 + kPrepRowBlocks), dim3(kPlanThreads), 0, s, d_idx,
                      n_idx, d_mplanes, d_rows, d_offsets, n, d_params, d_plan, d_order, d_roff,
-                     d_claims, d_state, d_out, d_tmax, tmax, ax, level1 ? 1 : 0, d_ctrl);
+                     d_claims, d_state, d_out, d_tmax, tmax, ax, second ? 1 : 0, d_ctrl);
   return hipGetLastError();
 }
 
@@ -1081,10 +1108,10 @@ hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
                           const MdPlan* d_plan, const uint32_t* d_order, const uint32_t* d_roff,
                           uint32_t* d_claims, uint64_t* d_state, const int32_t* d_tmax,
                           int32_t tmax, int32_t* d_out, const MdParams* d_params, const MdAux& ax,
-                          bool level1, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
+                          bool second, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
   // With events, the launch records them from its own dispatch (start and
   // end of the kernel), without marker packets of their own on the stream.
-  const int32_t l1 = level1 ? 1 : 0;
+  const int32_t l1 = second ? 1 : 0;
   if (thresholded)
     hipExtLaunchKernelGGL(md_main_thr_kernel, dim3(grid), dim3(256), 0, s, ev_start, ev_stop, 0,
                           d_rows, d_plan, d_order, d_roff, d_claims, d_state, d_tmax, tmax, d_out,

@@ -94,6 +94,56 @@ def test_mixed_k3_10_batch_against_fixture(dist):
     assert dist.max_zeros_batch(subsets, (50 - mz).astype(np.int32)) == mz.tolist()
 
 
+def test_second_pass_of_planned_codes(dist):
+    """Codes above the one-wave bound (k = 9, 10) at forced high first-pass
+    levels (4 and 5: certificates 22 and 29 zeros) mostly fall below their
+    certificate and are searched again by the planned walk's second pass, at
+    level 2 (certified by max_zeros >= k - 1 >= 8): the fixture's k = 9, 10
+    codes still equal the oracle, exact and thresholded.  Sets whose indices
+    are all one point have no direction of full rank at any level; they span
+    the zero codeword (49 zeros), as the reference's enumeration finds."""
+    ctx = dist.context
+    g = np.load(GOLD / "md_wide_golden.npz")
+    sets = _rows(g["flat"], g["offsets"])
+    big = [int(i) for i in g["pos"] if len(sets[int(i)]) >= 9]
+    bsets = [sets[i] for i in big]
+    exp = [int(z) for i, z in zip(g["pos"], g["mz"]) if len(sets[int(i)]) >= 9]
+    same = [[5] * 9, [3] * 10, [48] * 12]
+    saved = _force_levels(ctx, {9: 4, 10: 5})
+    try:
+        assert dist.max_zeros_batch(bsets, 1) == exp
+        assert dist.max_zeros_batch(same, 1) == [49, 49, 49]
+        mz = np.array(exp)
+        assert dist.max_zeros_batch(bsets, (49 - mz).astype(np.int32)) == exp   # exact
+        assert dist.max_zeros_batch(bsets, (50 - mz).astype(np.int32)) == exp   # clamp tmax + 1
+    finally:
+        _restore_levels(ctx, saved)
+    assert dist.max_zeros_batch(same, 1) == [49, 49, 49]
+
+
+def test_dev2_kmax_bound(dist):
+    """gt_md_batch_dev2 with kmax: one launch when kmax <= 8; a set above
+    kmax is an input error (result 0) reported by gt_sync."""
+    from gtoric._native import GtError
+    from gtoric.distance import flatten
+    ctx = dist.context
+    sets = [[0, 1, 7], [2, 9, 17, 30, 44, 47], [0, 1, 2, 3, 4, 5, 6, 7, 8]]
+    exp = oracle.max_zeros_batch(sets, 48).tolist()
+    flat, offs = flatten(sets)
+    di, do, dout = ctx.to_device(flat), ctx.to_device(offs), ctx.array(3, np.int32)
+    try:
+        ctx.md_batch_dev(di.ptr, len(flat), do.ptr, 3, dout.ptr, kmax=9)
+        ctx.sync()
+        assert dout.download().tolist() == exp
+        ctx.md_batch_dev(di.ptr, len(flat), do.ptr, 3, dout.ptr, kmax=8)
+        with pytest.raises(GtError):
+            ctx.sync()
+        assert dout.download().tolist()[:2] == exp[:2] and dout.download()[2] == 0
+    finally:
+        for x in (di, do, dout):
+            x.close()
+
+
 def test_big_k9_k10(dist, md_gold):
     sets = _rows(md_gold["big_flat"], md_gold["big_offsets"])
     assert dist.max_zeros_batch(sets, 1) == md_gold["big_mz"].tolist()
