@@ -120,14 +120,6 @@ void build_mplanes(uint32_t out[kNpts * kRowWords]) {
   }
 }
 
-// Default first-pass level per k: level m walks level_nsub(m) subcodes of
-// dimension k - m and certifies a maximum of at least 7m - 6 zeros.  On the
-// 4096-code mixed batch (k = 3..8) these levels certify every code: k = 4..6
-// have at least 9 zeros (level 2 needs 8), k = 7, 8 at least 15 (level 3);
-// k = 3 codes often have 7 (level 1).  For k >= 9 level 3 leaves to the
-// second pass (level 2) only codes of distance 35 or more.
-int default_level(int k) { return (k <= 3) ? 1 : (k <= 6) ? 2 : 3; }
-
 // Default Gray walk bits per k (a level-1 walk has dimension k - 1): big
 // enough that per-chunk set-up is a few percent of the walk, small enough
 // for a balanced tail; for k >= 19 large enough that a code has at most

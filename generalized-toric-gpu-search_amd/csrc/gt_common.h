@@ -193,6 +193,14 @@ constexpr int kRowsPerIdx = 9;
 // (rows in LDS, every level in turn); larger ones by the plan-and-deal walk.
 constexpr int kFusedMaxK = 8;
 
+// Default first-pass level per k: level m walks level_nsub(m) subcodes of
+// dimension k - m and certifies a maximum of at least 7m - 6 zeros.  On the
+// 4096-code mixed batch (k = 3..8) these levels certify every code: k = 4..6
+// have at least 9 zeros (level 2 needs 8), k = 7, 8 at least 15 (level 3);
+// k = 3 codes often have 7 (level 1).  For k >= 9 level 3 leaves to the
+// second pass (level 2) only codes of distance 35 or more.
+GT_HD int default_level(int k) { return (k <= 3) ? 1 : (k <= 6) ? 2 : 3; }
+
 // L and check interval of k, with the chunk counts the plan reads.
 inline uint32_t sat31(uint64_t c) { return (c >= (1ull << 31)) ? (1u << 31) : (uint32_t)c; }
 inline void md_params_refresh(MdParams& p, int k) {

@@ -10,9 +10,11 @@ namespace gt {
 // The exact limit of the batch path (kernel.py:246-247); 21 six-bit indices
 // fit one uint4 in a slot.
 constexpr int kServeMaxK = kMaxKExact;
-// Bit rows of one request's walk: 3 (k - 1) rows of the subcode vanishing at
-// (1,1) (gt_common.h).
-constexpr int kServeRowWords = 3 * (kServeMaxK - 1) * kRowWords;
+// Bit rows of one request's walk in a worker's LDS: a request of k <= 8
+// points is searched whole by one wave (at most kFusedRows = 45 rows); a
+// larger one is walked chunk by chunk at level 2, 3 (k - 2) rows.
+constexpr int kServeRowWords = 3 * (kServeMaxK - 2) * kRowWords;
+static_assert(3 * (kServeMaxK - 2) >= 45, "the rows of a whole small request fit");
 
 // One request descriptor (fine-grained host memory, written by the host):
 // 48 bytes, read by the dispatcher with three 16-byte loads.
@@ -84,6 +86,7 @@ struct ServeDev {
   uint64_t first_req;   // first request index this launch dispatches (set by the init kernel)
   uint8_t L[kServeMaxK + 1];
   uint8_t check_log2[kServeMaxK + 1];
+  uint8_t level[kServeMaxK + 1];   // first-pass level of a request searched whole
   int32_t* s_k;         // per slot: k | (L << 8), 0 when the request has no chunks
   int32_t* s_tmax;
   uint64_t* s_state;    // per slot: bits 0..31 minimum weight, 32..63 chunks finished

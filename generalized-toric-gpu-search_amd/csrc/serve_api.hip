@@ -45,8 +45,17 @@ double now_s() {
   return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
-// Projective codewords the exact walk of a k-point request visits.
-uint64_t walk_cw(int k) { return proj_codewords(walk_dim(k)); }
+// Projective codewords the search of a k-point request walks: its first
+// level's subcodes (k <= 8, searched whole by one wave; more only when a
+// lower level follows), or the level-2 subcode (larger k).
+uint64_t walk_cw(int k) {
+  if (k <= 1) return 0;
+  if (k <= kFusedMaxK) {
+    const int m = (k <= 2) ? 1 : default_level(k);
+    return (uint64_t)level_nsub(m) * proj_codewords(k - m);
+  }
+  return proj_codewords(k - second_level(k));
+}
 
 }  // namespace
 
@@ -267,6 +276,7 @@ int gt_serve_start(int32_t device, int32_t slots, int64_t work_budget, const int
     if (L < kU || L > 30 || c < 0 || c > 16) return bail(sfail("bad serving parameters for k=%d", k));
     dh.L[k] = (uint8_t)L;
     dh.check_log2[k] = (uint8_t)c;
+    dh.level[k] = (uint8_t)default_level(k);
   }
   if (sv->dalloc(&dh.s_k, slots) || sv->dalloc(&dh.s_tmax, slots) ||
       sv->dalloc(&dh.s_state, slots) ||

@@ -8,14 +8,16 @@
 // global chunk sequence and publishes the new chunk total.  Every other wave
 // is a worker: it claims the next global chunk id with one atomic, waits
 // until that id is published, finds the request by a 64-ary search over the
-// in-flight window, builds the request's bit rows in its LDS area (lane
-// 3i + b: x^b * (g_i + g_{k-1}), the subcode vanishing at the torus point
-// (1,1), which has the same maximum number of zeros; gt_common.h), walks the
-// chunk (the Gray walk of the batch kernel) and folds the wave minimum into
-// the slot.  Building the rows in the workers
-// costs them what copying prebuilt rows did and keeps the dispatcher, one wave
-// for the whole device, to a few stores per request.  The wave that finishes
-// the last chunk of a request
+// in-flight window and folds the wave minimum of its chunk into the slot.
+// A request of k <= 8 points is one chunk: the wave runs the request's
+// whole search (the anchored subcodes of its first level, rows in the
+// wave's LDS, lower levels when the first does not certify: fused_search,
+// gt_anchor.h).  A larger request is chunked at level 2 (certified for
+// k >= 9 by max_zeros >= k - 1; gt_common.h): the wave builds the rows of
+// the subcode vanishing at two points of a line in its LDS area and walks
+// the chunk (the Gray walk of the batch kernel).  Building the rows in the
+// workers keeps the dispatcher, one wave for the whole device, to a few
+// stores per request.  The wave that finishes the last chunk of a request
 // writes its result and sequence number to host memory.
 //
 // Requests that arrive while others run are admitted as soon as a slot and
@@ -36,8 +38,8 @@
 // request gets its result.  Requests the host published but the dispatcher
 // had not taken when it stopped (`consumed` < `published`) are dispatched by
 // the next launch, which the host starts at request `consumed`.
+#include "gt_anchor.h"
 #include "gt_serve.h"
-#include "gt_walk.h"
 
 namespace gt {
 
@@ -100,9 +102,13 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, uint32_t lane) {
       k = (int)(v0.x & 0xFFu);
       const int Lq = (int)((v0.x >> 8) & 0xFFu);
       const int32_t q_tmax = (int32_t)v0.y;
+      // k <= 8: one chunk, the whole search in one wave; larger k: the
+      // chunks of the level-2 walk.
       int L = 0;
-      if (k >= 2 && k <= kServeMaxK) L = (Lq != 0) ? Lq : (int)d->L[k];
-      nch = (L != 0) ? (uint32_t)code_chunks(walk_dim(k), L) : 0u;
+      if (k > kFusedMaxK && k <= kServeMaxK) L = (Lq != 0) ? Lq : (int)d->L[k];
+      nch = (k < 2 || k > kServeMaxK) ? 0u
+            : (k <= kFusedMaxK)       ? 1u
+                                      : (uint32_t)code_chunks(k - second_level(k), L);
       const uint32_t ib[8] = {v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, v2.z, v2.w};
       uint64_t lo = 0, hi = 0;
 #pragma unroll
@@ -244,32 +250,36 @@ __device__ void worker(ServeDev* d, ServeHost* h, const uint32_t* __restrict__ m
     uint32_t b = 64u;
     uint64_t t2 = t1;
     if (!skip) {
-      if (slot != staged_slot || seq != staged_seq) {
-        // Lane 3i + b < 3(k-1) writes row 3i + b: x^b * (g_i + g_{k-1}).
-        const uint4 iv = d->s_idx[slot];
-        const uint32_t i = lane / 3u, bpow = lane - 3u * i;
-        const uint32_t gi = unpack_idx6(iv, (i < (uint32_t)kServeMaxK) ? i : 0u);
-        const uint32_t gl = unpack_idx6(iv, (uint32_t)(k - 1));
-        if (i + 1u < (uint32_t)k) {
-          uint32_t w[6];
-#pragma unroll
-          for (int t = 0; t < 6; ++t) w[t] = mplanes[gi * kRowWords + t] ^ mplanes[gl * kRowWords + t];
-          if (bpow >= 1u) mul_x(w);
-          if (bpow >= 2u) mul_x(w);
-#pragma unroll
-          for (int t = 0; t < 6; ++t) lrows[lane * kRowWords + t] = w[t];
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        staged_slot = slot;
-        staged_seq = seq;
+      // Lane i < k: the request's point i.
+      const uint4 iv = d->s_idx[slot];
+      const int gi = (int)unpack_idx6(iv, (lane < (uint32_t)k) ? lane : 0u);
+      if (k <= kFusedMaxK) {
+        // The whole search of a small request (rows built in the LDS area).
+        t2 = wall_clock64();
+        b = fused_search(gi, k, tmax, lane, mplanes, d->level, lrows);
+        staged_slot = 0xFFFFFFFFu;
         ++st_staged;
+      } else {
+        if (slot != staged_slot || seq != staged_seq) {
+          // Level-2 rows; without a direction of full rank every index is
+          // the same point, the request spans the zero codeword, and zero
+          // rows give its 49 zeros.
+          wave_lds_sync();
+          if (!md_rows_level<2>(gi, k, lane, mplanes, lrows))
+            for (int r = (int)lane; r < 3 * (k - 2); r += kWave)
+#pragma unroll
+              for (int t = 0; t < 6; ++t) lrows[r * kRowWords + t] = 0u;
+          wave_lds_sync();
+          staged_slot = slot;
+          staged_seq = seq;
+          ++st_staged;
+        }
+        t2 = wall_clock64();
+        const uint32_t cmask = (1u << d->check_log2[k]) - 1u;
+        b = run_chunk<true>(LdsRows{lrows}, k - second_level(k), L, local, lane, thr, cmask,
+                            best_g);
+        b = wave_min(b);
       }
-      t2 = wall_clock64();
-      const uint32_t cmask = (1u << d->check_log2[k]) - 1u;
-      b = run_chunk<true>(LdsRows{lrows}, walk_dim(k), L, local, lane, thr, cmask, best_g);
-      b = wave_min(b);
       ++st_chunks;
     } else {
       t2 = wall_clock64();

@@ -21,6 +21,19 @@ from . import _native
 from ._native import check, i32, i64, lib, ptr, vp
 
 MAX_K = 21
+FUSED_MAX_K = 8      # requests of at most 8 points: one chunk, searched whole by one wave
+
+
+def served_chunks(k: int, L: int) -> int:
+    """Chunks the server walks for a k-point request at walk length L: none
+    for k <= 1, one for k <= 8 (the whole search in one wave), else the
+    chunks of the level-2 walk of dimension k - 2 (csrc/serve_kernel.hip)."""
+    from .autotune import code_chunks
+    if k <= 1:
+        return 0
+    if k <= FUSED_MAX_K:
+        return 1
+    return code_chunks(k - 2, L)
 
 
 # Served walk length per k (chunks of 64 x 2^L codewords).  A served chunk also

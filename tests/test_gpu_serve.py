@@ -51,8 +51,7 @@ def test_chunk_cap_changes_chunks_not_results(ctx):
     """max_chunk_log2 splits large requests into more, shorter chunks; the
     results stay those of the batch kernel and the chunk count follows the
     capped table."""
-    from gtoric.autotune import code_chunks, walk_dim
-    from gtoric.serving import SERVE_L, SERVE_L_OFFSET, Server
+    from gtoric.serving import SERVE_L, SERVE_L_OFFSET, Server, served_chunks
     sets, tmax = _requests(400, seed=57, kmax=10)
     with Server(0, slots=256, life_s=60, max_chunk_log2=9) as srv:
         res, _, _ = srv.run_stream(sets, tmax, np.zeros(len(sets)), timeout_s=60)
@@ -63,15 +62,14 @@ def test_chunk_cap_changes_chunks_not_results(ctx):
     for k in range(1, 22):
         Lb = ctx.get_md_params(k)[0]
         L[k] = min(SERVE_L.get(k, min(30, Lb + SERVE_L_OFFSET)), max(Lb, 9))
-    want = sum(code_chunks(walk_dim(len(S)), L[len(S)]) for S in sets if len(S))
+    want = sum(served_chunks(len(S), L[len(S)]) for S in sets)
     assert st["chunks_walked"] + st["chunks_skipped"] == want
 
 
 def test_counters_account_for_every_chunk(ctx):
     """gt_serve_stats: every request dispatched once, and chunks walked plus
     chunks skipped equal the chunk count of the served parameter table."""
-    from gtoric.autotune import code_chunks, walk_dim
-    from gtoric.serving import SERVE_L, SERVE_L_OFFSET, Server
+    from gtoric.serving import SERVE_L, SERVE_L_OFFSET, Server, served_chunks
     sets, tmax = _requests(500, seed=56, kmax=9)
     with Server(0, slots=256, life_s=60) as srv:
         res, lat, wall = srv.run_stream(sets, tmax, np.zeros(len(sets)), timeout_s=60)
@@ -79,7 +77,7 @@ def test_counters_account_for_every_chunk(ctx):
         L = {k: SERVE_L.get(k, min(30, ctx.get_md_params(k)[0] + SERVE_L_OFFSET))
              for k in range(1, 22)}
     assert st["requests_dispatched"] == len(sets)
-    want = sum(code_chunks(walk_dim(len(S)), L[len(S)]) for S in sets if len(S))
+    want = sum(served_chunks(len(S), L[len(S)]) for S in sets)
     assert st["chunks_walked"] + st["chunks_skipped"] == want
     assert st["worker_waves"] > 0 and st["kernel_life_s"] > 0
     # The server relaunches on the next submit after stats() stopped it.
@@ -146,9 +144,8 @@ def test_serve_errors():
 def test_short_chunks_change_chunks_not_results(ctx):
     """gt_serve_set_short_chunks: below a huge work threshold every request
     walks the short chunks, below 0 none does; results stay the batch's."""
-    from gtoric.autotune import code_chunks, walk_dim
     from gtoric.distance import flatten
-    from gtoric.serving import Server
+    from gtoric.serving import Server, served_chunks
     sets, tmax = _requests(300, seed=58, kmax=10)
     exp = ctx.md_batch(*flatten(sets), tmax).tolist()
     for below, short in ((1 << 62, True), (0, False)):
@@ -159,7 +156,7 @@ def test_short_chunks_change_chunks_not_results(ctx):
         assert res.tolist() == exp
         Lb = {k: ctx.get_md_params(k)[0] for k in range(22)}
         Luse = {k: (min(int(L[k]), max(Lb[k], 9)) if short else int(L[k])) for k in range(22)}
-        want = sum(code_chunks(walk_dim(len(S)), Luse[len(S)]) for S in sets if len(S))
+        want = sum(served_chunks(len(S), Luse[len(S)]) for S in sets)
         assert st["chunks_walked"] + st["chunks_skipped"] == want, short
 
 
