@@ -38,7 +38,7 @@ VALU_PER_CODEWORD = 174 / 16
 LANES_PER_CU_CLK = 64       # 4 SIMD x 16 lanes: one int32 VALU op per lane per clock
 
 
-PROFILE = ROOT / "profiles" / "md_main_mixed4096.md"
+PROFILE = ROOT / "profiles" / "md_fused_mixed4096.md"
 CANON_PROFILE = ROOT / "profiles" / "canon_kernel_1M.md"
 WALK_PROBE_FORM = 6          # gt_issue_probe: the walk's 16-codeword block
 WALK_BLOCK_CODEWORDS = 16

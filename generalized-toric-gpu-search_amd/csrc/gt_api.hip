@@ -544,6 +544,15 @@ int gt_set_md_level(gt_ctx* c, int32_t k, int32_t level) {
   return c->upload_params(c->stream);
 }
 
+int gt_md_level_info(int32_t m, int32_t s, uint32_t* rep_mask, int32_t* nsub, int32_t* cert) {
+  if (m < 1 || m > kMaxLevel) return fail("level %d outside 1..%d", m, kMaxLevel);
+  if (s < 0 || s >= level_nsub(m)) return fail("level %d has %d representatives", m, level_nsub(m));
+  if (rep_mask) *rep_mask = level_rep(m, s);
+  if (nsub) *nsub = level_nsub(m);
+  if (cert) *cert = level_cert(m);
+  return 0;
+}
+
 int gt_set_md_fused_max_k(gt_ctx* c, int32_t k) {
   if (!c) return fail("null context");
   if (k < 1 || k > kFusedMaxK) return fail("fused bound %d outside 1..%d", k, kFusedMaxK);

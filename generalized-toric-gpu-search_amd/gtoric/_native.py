@@ -36,6 +36,7 @@ _SIGS = {
     "gt_set_md_params": ([vp, i32, i32, i32], ctypes.c_int),
     "gt_get_md_params": ([vp, i32, P(i32), P(i32)], ctypes.c_int),
     "gt_set_md_level": ([vp, i32, i32], ctypes.c_int),
+    "gt_md_level_info": ([i32, i32, P(ctypes.c_uint32), P(i32), P(i32)], ctypes.c_int),
     "gt_get_md_level": ([vp, i32, P(i32)], ctypes.c_int),
     "gt_md_batch": ([vp, vp, vp, i32, vp, i32, i64, u64, vp], ctypes.c_int),
     "gt_md_batch_dev": ([vp, vp, i64, vp, i32, vp, i32, i32, i64, u64, vp, vp], ctypes.c_int),

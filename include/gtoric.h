@@ -74,6 +74,12 @@ int gt_get_md_params(gt_ctx* ctx, int32_t k, int32_t* L, int32_t* check_log2);
 int gt_set_md_level(gt_ctx* ctx, int32_t k, int32_t level);
 int gt_get_md_level(gt_ctx* ctx, int32_t k, int32_t* level);
 
+/* The level table itself (host only, no device): representative s of level
+ * m as a bit mask over r (the points r d of a line, bit r set <=> r in the
+ * set), the number of representatives, and the certificate 7m - 6.  No
+ * counterpart in the reference; for checking the table. */
+int gt_md_level_info(int32_t m, int32_t s, uint32_t* rep_mask, int32_t* nsub, int32_t* cert);
+
 /* Batched minimum distance, mixed k (replaces DistanceOracle.max_zeros_batch,
  * kernel.py:276-323, and DistanceOracleCUDABP.max_zeros_batch,
  * kernel_cuda_bp.py:290-323; kernel eval_min_distance_batch_bp,
