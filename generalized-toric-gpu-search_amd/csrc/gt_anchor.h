@@ -156,7 +156,7 @@ __device__ __forceinline__ bool md_rows_level(int g, int k, uint32_t lane,
 // balance it bought.)
 constexpr int kFusedRows = 45;   // 3 subcodes x 3 (kFusedMaxK - 3) rows
 static_assert(kFusedRows == 3 * 3 * (kFusedMaxK - 3), "the level-3 rows of the largest fused code");
-constexpr int kFusedWaves = 4;   // codes (and waves) per workgroup
+constexpr int kFusedWaves = 4;   // waves per workgroup of the one-wave search
 
 // Rows of level m into `base` (md_rows_level); false: no full-rank direction.
 __device__ __forceinline__ bool md_rows_switch(int m, int g, int k, uint32_t lane,

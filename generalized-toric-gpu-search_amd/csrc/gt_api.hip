@@ -143,8 +143,9 @@ struct ProfRec {
 constexpr size_t kMdClaimBytes = (size_t)kClaimShards * kClaimStride * 4;
 constexpr size_t kMdPlanAreaBytes = kMdClaimBytes + 512;
 constexpr size_t kMdFbCountOff = kMdPlanAreaBytes - 4;
+constexpr size_t kMdFusedCountOff = kMdPlanAreaBytes - 8;   // codes md_fused takes
 constexpr size_t kMdPlanBytesPerCode = 20;
-static_assert(sizeof(MdPlan) <= 508, "MdPlan and the list count fit the slot");
+static_assert(sizeof(MdPlan) <= 504, "MdPlan and the two counts fit the slot");
 
 // ---- scratch of the allocation-free exact pass (gt_md_batch_dev_noalloc) --
 // One file-scope device array per GPU, held until the process ends: the plan
@@ -370,7 +371,9 @@ struct gt_ctx {
     if (timed)
       for (hipEvent_t& e : ev) GT_HIP(hipEventCreate(&e));
     GT_HIP(launch_md_fused(d_idx, n_idx, d_offsets, n, mplanes.as<uint32_t>(), dp, d_tmax, tmax_s,
-                           d_out, ax, fused_k, kmax, ctrl_p, s, ev[0], ev[1]));
+                           d_out, ax, d_order,
+                           reinterpret_cast<uint32_t*>(d_plan_area + kMdFusedCountOff), fused_k,
+                           kmax, ctrl_p, s, ev[0], ev[1]));
     if (timed) prof_rec["md_fused"].ev.emplace_back(ev[0], ev[1]);
     if (kmax <= fused_k) {
       if (timed)

@@ -33,8 +33,10 @@ struct MdAux {
   uint32_t* big_count;
   uint32_t* big_list;
 };
-// Exact (or thresholded) search of one batch: md_fused (one wave per code of
-// k <= kfused, rows in LDS, results written; larger codes listed), then,
+// Exact (or thresholded) search of one batch: md_order (sizes checked,
+// results of codes without a walk, larger codes listed, the others sorted
+// by size into d_order) and md_fused (a wave per code of k <= kfused, rows
+// in LDS, results written), then,
 // unless the caller knows that no code has more than kfused generators, the
 // list search: md_prepl and md_main at the codes' levels, md_prepl and
 // md_main at second_level(k) for the codes the first could not certify.
@@ -47,8 +49,9 @@ struct MdAux {
 hipError_t launch_md_fused(const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets,
                            int32_t n, const uint32_t* d_mplanes, const MdParams* d_params,
                            const int32_t* d_tmax, int32_t tmax, int32_t* d_out, const MdAux& ax,
-                           int kfused, int kmax, uint32_t* d_ctrl, hipStream_t s,
-                           hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
+                           uint32_t* d_order, uint32_t* d_fused_count, int kfused, int kmax,
+                           uint32_t* d_ctrl, hipStream_t s, hipEvent_t ev_start = nullptr,
+                           hipEvent_t ev_stop = nullptr);
 hipError_t launch_md_prepl(bool second, const int32_t* d_idx, int64_t n_idx,
                            const uint32_t* d_mplanes, uint32_t* d_rows, const int32_t* d_offsets,
                            int32_t n, const MdParams* d_params, MdPlan* d_plan, uint32_t* d_order,

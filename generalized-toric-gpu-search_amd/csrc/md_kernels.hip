@@ -147,53 +147,108 @@ __global__ __launch_bounds__(256) void md_rows_kernel(const int32_t* __restrict_
   md_rows_one((int64_t)blockIdx.x * blockDim.x + threadIdx.x, idx, n_idx, mplanes, rows, ctrl);
 }
 
-// Codes with kfused < k <= kmax go to the list; k > kmax is an input error
-// (a caller's bound that skips the list search).
-__global__ __launch_bounds__(kFusedWaves * kWave) void md_fused_kernel(
-    const int32_t* __restrict__ idx, int64_t n_idx, const int32_t* __restrict__ offsets,
-    int32_t n, const uint32_t* __restrict__ mplanes, const MdParams* __restrict__ p,
+// The order the one-wave search takes the codes in: one workgroup checks
+// every code's size, writes the results of codes without a walk (k <= 1, and
+// 0 for bad sizes, flagged), lists the codes with kfused < k <= kmax for the
+// planned walk (k > kmax is an input error: a caller's bound that skips the
+// planned walk), and counting-sorts the others by k, largest first, into
+// order[0 .. *fused_count) (the order inside a size does not matter).
+constexpr int kOrderThreads = 1024;
+constexpr int kFusedCodesPerWave = 1;   // md_fused launches a wave per code,
+constexpr int kFusedMaxWaves = 8192;    // at most this many (4096 codes of k in
+                                        // 3..8: 1 per wave 15 us, 2: 17, 4: 22, 8: 39)
+
+__global__ __launch_bounds__(kOrderThreads) void md_order_kernel(
+    const int32_t* __restrict__ offsets, int32_t n, int64_t n_idx,
     const int32_t* __restrict__ tmax_arr, int32_t tmax_s, int32_t* __restrict__ out,
-    uint32_t* big_count, uint32_t* big_list, int32_t kfused, int32_t kmax, uint32_t* ctrl) {
+    uint32_t* big_count, uint32_t* big_list, uint32_t* __restrict__ order,
+    uint32_t* fused_count, int32_t kfused, int32_t kmax, uint32_t* ctrl) {
+  __shared__ uint32_t cnt[kFusedMaxK + 1];
+  const int tid = (int)threadIdx.x;
+  if (tid <= kFusedMaxK) cnt[tid] = 0u;
+  __syncthreads();
+  uint32_t err = 0u;
+  for (int i = tid; i < n; i += kOrderThreads) {
+    const int o = offsets[i];
+    const int k = offsets[i + 1] - o;
+    const bool bad_k = k < 0 || k > kMaxKExact || k > kmax || o < 0 || (int64_t)o + k > n_idx;
+    if (bad_k || k <= 1) {
+      if (bad_k) err |= kErrBadK;
+      // No walk: k = 0 (and bad input) gives 0, k = 1 a monomial (weight 49).
+      out[i] = (!bad_k && k == 1) ? md_result(49, tmax_arr ? tmax_arr[i] : tmax_s) : 0;
+    } else if (k > kfused) {
+      const uint32_t j = atomicAdd(big_count, 1u);
+      if (j < (uint32_t)n) big_list[j] = (uint32_t)i;
+      else err |= kErrBadK;   // (a count left by an interrupted call)
+    } else {
+      atomicAdd(&cnt[k], 1u);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t base = 0u;
+    for (int k = kFusedMaxK; k >= 2; --k) {
+      const uint32_t t = cnt[k];
+      cnt[k] = base;
+      base += t;
+    }
+    *fused_count = base;
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += kOrderThreads) {
+    const int o = offsets[i];
+    const int k = offsets[i + 1] - o;
+    const bool bad_k = k < 0 || k > kMaxKExact || k > kmax || o < 0 || (int64_t)o + k > n_idx;
+    if (!bad_k && k >= 2 && k <= kfused) order[atomicAdd(&cnt[k], 1u)] = (uint32_t)i;
+  }
+  if (err) atomicOr(&ctrl[2], err);
+}
+
+// The one-wave search over order[0 .. *fused_count): W waves; in round r
+// wave w takes position r W + w (r even) or (r + 1) W - 1 - w (r odd), a
+// snake over the size-sorted codes.  With a wave per code the waves are
+// dispatched largest code first, so the long searches start early and the
+// short ones fill in behind them (one wave per code in input order left the
+// SIMDs that drew several k = 8 codes 3x the mean); with more codes than
+// waves every wave gets codes of every size class in turn.  At most 96
+// VGPRs (five waves per SIMD; 120 unbounded, four): a call issued on another
+// stream gets waves onto the SIMDs while this call's last waves finish.
+__global__ __launch_bounds__(kFusedWaves * kWave, 5) void md_fused_kernel(
+    const int32_t* __restrict__ idx, const int32_t* __restrict__ offsets, int32_t nwaves,
+    const uint32_t* __restrict__ order, const uint32_t* fused_count,
+    const uint32_t* __restrict__ mplanes, const MdParams* __restrict__ p,
+    const int32_t* __restrict__ tmax_arr, int32_t tmax_s, int32_t* __restrict__ out,
+    uint32_t* ctrl) {
   __shared__ __align__(16) uint32_t lrows[kFusedWaves][kFusedRows * kRowWords];
   const int wv = (int)(threadIdx.x / kWave);
-  const int code = (int)blockIdx.x * kFusedWaves + wv;
-  if (code >= n) return;
+  const uint32_t w = blockIdx.x * kFusedWaves + wv;
+  if (w >= (uint32_t)nwaves) return;
   const uint32_t lane = lane_id();
-  const int o = offsets[code];
-  const int k = offsets[code + 1] - o;
-  const int tmax = tmax_arr ? tmax_arr[code] : tmax_s;
-  const bool bad_k = k < 0 || k > kMaxKExact || k > kmax || o < 0 || (int64_t)o + k > n_idx;
-  if (bad_k || k <= 1) {
-    if (lane == 0) {
-      if (bad_k) atomicOr(&ctrl[2], kErrBadK);
-      // No walk: k = 0 (and bad input) gives 0, k = 1 a monomial (weight 49).
-      out[code] = (!bad_k && k == 1) ? md_result(49, tmax) : 0;
+  const uint32_t m = __builtin_amdgcn_readfirstlane(*fused_count);
+  const uint32_t W = (uint32_t)nwaves;
+  for (uint32_t r = 0;; ++r) {
+    const uint32_t pos = (r & 1u) ? (r + 1u) * W - 1u - w : r * W + w;
+    if (pos >= m) break;   // later rounds start above m too
+    const int code = (int)__builtin_amdgcn_readfirstlane(order[pos]);
+    const int o = offsets[code];
+    const int k = offsets[code + 1] - o;
+    const int tmax = tmax_arr ? tmax_arr[code] : tmax_s;
+    int g = 0;
+    bool bad = false;
+    if (lane < (uint32_t)k) {
+      g = idx[o + (int)lane];
+      bad = g < 0 || g >= kNpts;
     }
-    return;
-  }
-  if (k > kfused) {
-    if (lane == 0) {
-      const uint32_t i = atomicAdd(big_count, 1u);
-      if (i < (uint32_t)n) big_list[i] = (uint32_t)code;
-      else atomicOr(&ctrl[2], kErrBadK);   // (a count left by an interrupted call)
+    if (__ballot(bad) != 0ull) {
+      if (lane == 0) {
+        atomicOr(&ctrl[2], kErrBadIndex);
+        out[code] = 0;
+      }
+      continue;
     }
-    return;
+    const uint32_t best = fused_search(g, k, tmax, lane, mplanes, p->level, lrows[wv]);
+    if (lane == 0) out[code] = md_result((int)best, tmax);
   }
-  int g = 0;
-  bool bad = false;
-  if (lane < (uint32_t)k) {
-    g = idx[o + (int)lane];
-    bad = g < 0 || g >= kNpts;
-  }
-  if (__ballot(bad) != 0ull) {
-    if (lane == 0) {
-      atomicOr(&ctrl[2], kErrBadIndex);
-      out[code] = 0;
-    }
-    return;
-  }
-  const uint32_t best = fused_search(g, k, tmax, lane, mplanes, p->level, lrows[wv]);
-  if (lane == 0) out[code] = md_result((int)best, tmax);
 }
 
 // ---- plan and rows of a listed set of codes ----------------------------------
@@ -870,13 +925,20 @@ hipError_t launch_md_rows(const int32_t* d_idx, int64_t n_idx, const uint32_t* d
 hipError_t launch_md_fused(const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets,
                            int32_t n, const uint32_t* d_mplanes, const MdParams* d_params,
                            const int32_t* d_tmax, int32_t tmax, int32_t* d_out, const MdAux& ax,
-                           int kfused, int kmax, uint32_t* d_ctrl, hipStream_t s,
-                           hipEvent_t ev_start, hipEvent_t ev_stop) {
+                           uint32_t* d_order, uint32_t* d_fused_count, int kfused, int kmax,
+                           uint32_t* d_ctrl, hipStream_t s, hipEvent_t ev_start,
+                           hipEvent_t ev_stop) {
   if (n <= 0) return hipSuccess;
-  const unsigned blocks = (unsigned)(((int64_t)n + kFusedWaves - 1) / kFusedWaves);
+  hipLaunchKernelGGL(md_order_kernel, dim3(1), dim3(kOrderThreads), 0, s, d_offsets, n, n_idx,
+                     d_tmax, tmax, d_out, ax.big_count, ax.big_list, d_order, d_fused_count,
+                     kfused, kmax, d_ctrl);
+  // About kFusedCodesPerWave codes per wave, at most kFusedMaxWaves waves.
+  const int64_t waves = std::min<int64_t>(((int64_t)n + kFusedCodesPerWave - 1) /
+                                          kFusedCodesPerWave, kFusedMaxWaves);
+  const unsigned blocks = (unsigned)((waves + kFusedWaves - 1) / kFusedWaves);
   hipExtLaunchKernelGGL(md_fused_kernel, dim3(blocks), dim3(kFusedWaves * kWave), 0, s, ev_start,
-                        ev_stop, 0, d_idx, n_idx, d_offsets, n, d_mplanes, d_params, d_tmax, tmax,
-                        d_out, ax.big_count, ax.big_list, kfused, kmax, d_ctrl);
+                        ev_stop, 0, d_idx, d_offsets, (int32_t)waves, d_order, d_fused_count,
+                        d_mplanes, d_params, d_tmax, tmax, d_out, d_ctrl);
   return hipGetLastError();
 }
 
