@@ -121,29 +121,19 @@ def proj(k: int) -> int:
     return (8 ** w - 1) // 7
 
 
-def level_walk(k: int, m: int) -> int:
-    """Projective codewords of a level-m pass over a k-point code: 3 subcodes
-    for m = 3, 4, one otherwise, of dimension k - m (csrc/gt_common.h)."""
-    return (3 if m in (3, 4) else 1) * ((8 ** (k - m) - 1) // 7)
-
-
-def walked(k: int, z: int, levels: dict, fused_k: int = 8) -> int:
-    """Codewords the exact search walks for a k-point code of exact maximum z:
-    the first pass at its level; when z is below that level's certificate
-    (7m - 6), the one-wave search (k <= fused_k) goes on at the highest lower
-    level that max(z_m, k - 1) certifies - counted here from k - 1, which can
-    only overstate the work - and the planned walk at level 1."""
-    if k <= 1:
-        return 0
-    m = 1 if k <= 2 else levels[k]
-    n = level_walk(k, m)
-    if m > 1 and z < 7 * m - 6:
-        if k <= fused_k:
-            m2 = max([q for q in range(1, m) if 7 * q - 6 <= k - 1] or [1])
-            n += level_walk(k, m2)
-        else:
-            n += proj(k)
-    return n
+def walked_codewords(ctx, sets) -> tuple[int, np.ndarray]:
+    """Projective codewords the exact search of `sets` walks, from the
+    library's per-code walk record of one untimed call (the same passes and
+    levels as the timed calls; results checked against them by the caller)."""
+    from gtoric.distance import flatten
+    flat, offs = flatten(sets)
+    ctx.set_md_walk_record(True)
+    try:
+        got = ctx.md_batch(flat, offs, 48)
+        rec = ctx.md_walk_record(len(sets))
+    finally:
+        ctx.set_md_walk_record(False)
+    return int(rec.sum()), got
 
 
 def ref_polys(k: int) -> int:
@@ -647,9 +637,10 @@ def main() -> None:
         ctxs, sets, args.steps, args.warmup, expect,
         before_warmup=lambda: ceil.update(walk_ceiling()), kmax=max(len(s) for s in sets),
         kernel="md_fused")
-    # Codewords the search walked: from each code's exact result (the run's
-    # own, bit-exact against the fixture).
-    codewords = sum(walked(len(s), int(z), levels) for s, z in zip(sets, got))
+    # Codewords the search walked, recorded by the library on one more call.
+    codewords, rgot = walked_codewords(ctx, sets)
+    if not np.array_equal(rgot, got):
+        raise SystemExit("the recorded call's results differ from the timed calls'")
     level1_codewords = sum(proj(len(s)) for s in sets)
     value_one = n * args.steps / dt
     dt_value = dt_pipe if dt_pipe is not None else dt
@@ -743,8 +734,11 @@ def main() -> None:
     if "wide" not in skip:
         wsets = workloads.mixed_k(4096, 3, 10, seed=12)
         wdt, wnk, wkms, wgot, _, _ = bench_md([ctx], wsets, 3, 1, None, kmax=10,
-                                              kernel=("md_fused", "md_main", "md_main2"))
-        wcw = sum(walked(len(s), int(z), levels) for s, z in zip(wsets, wgot))
+                                              kernel=("md_fused", "md_main", "md_main_mid",
+                                                      "md_main2"))
+        wcw, rgot = walked_codewords(ctx, wsets)
+        if not np.array_equal(rgot, wgot):
+            raise SystemExit("the recorded call's results differ from the timed calls'")
         wk = wkms / max(wnk, 1)
         wgold = ROOT / "tests" / "golden" / "md_wide_golden.npz"
         wparity = "unchecked"
@@ -762,8 +756,9 @@ def main() -> None:
             "parity": wparity,
             "walked_codewords": wcw, "level1_codewords": sum(proj(len(s)) for s in wsets),
             "kernel_ms": round(wk, 3),
-            "kernels": "md_fused (k <= 8, one wave per code) + md_main (k = 9, 10, first "
-                       "pass) + md_main (second pass), device time per call",
+            "kernels": "md_fused (k <= 8, one wave per code) + md_main (k = 9, 10: first "
+                       "pass, middle pass, final pass), device time per call",
+            "levels": {str(k): levels[k] for k in (9, 10)},
             "achieved_TOPs": round(OPS_PER_CODEWORD * wcw / (wk * 1e-3) / 1e12, 3),
             "frac_of_attainable": round(OPS_PER_CODEWORD * wcw / (wk * 1e-3) / 1e12 / peak, 4),
             "note": "achieved counts 11 int32 ops per codeword; the attainable peak is the "

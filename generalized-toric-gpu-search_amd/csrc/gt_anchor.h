@@ -143,13 +143,14 @@ __device__ __forceinline__ bool md_rows_level(int g, int k, uint32_t lane,
 
 // ---- one wave per code (k <= kFusedMaxK) -----------------------------------
 // A code of at most 8 generators walks at most 45 rows (level 3: 3 subcodes
-// of 3 x 5 rows) and at most 3 (8^5 - 1) / 7 codewords at its default level,
-// so one wave builds its rows in LDS and walks them itself: no plan, no
-// cross-wave fold, one launch.  A first-level maximum below the level's
-// certificate (or a level without a direction of full rank) is followed, in
-// the same wave, by the highest lower level whose certificate the bound
-// max(z, k - 1) already meets (level 1 at worst: 21 rows, 299,593 codewords
-// for k = 8).  Codes with more generators are appended to the list of the
+// of 3 x 5 rows; any other level fewer) and at most 3 (8^5 - 1) / 7
+// codewords at its default level, so one wave builds its rows in LDS and
+// walks them itself: no plan, no cross-wave fold, one launch.  A maximum z
+// found below the level's certificate for found maxima, 7m - 7 (or a level
+// without a direction of full rank), is followed, in the same wave, by the
+// highest lower level whose certificate the bound max(z, k - 1) already
+// meets or whose found certificate z meets (level 1 at worst: 21 rows,
+// 299,593 codewords for k = 8).  Codes with more generators are appended to the list of the
 // planned search (md_prepl, md_main).  (Dealing the walks of a workgroup's
 // four codes over its four waves measured slower: 26.9 against 23.6 us on
 // the 4096-code mixed batch, the dealing's scalar work outweighing the
@@ -191,14 +192,15 @@ __device__ __forceinline__ void wave_lds_sync() {
 // The whole exact search of one code of 2 <= k <= kFusedMaxK generators by
 // one wave (lane i < k holds the code's index g of point i): the first
 // level's rows built in `base` (kFusedRows rows of the wave's LDS), walked,
-// certified; below the certificate (or without a full-rank direction) the
-// highest lower level that the bound max(z, k - 1) certifies, in the same
-// wave.  Returns the minimum weight: exact, or <= 48 - tmax once the
-// threshold is exceeded (the result is md_result of it).
+// certified (z >= 7m - 7); below that (or without a full-rank direction) a
+// lower level, in the same wave, until a level certifies its maximum.
+// Returns the minimum weight: exact, or <= 48 - tmax once the threshold is
+// exceeded (the result is md_result of it); adds the projective codewords of
+// the subcodes it walked to `walked`.
 __device__ __forceinline__ uint32_t fused_search(int g, int k, int tmax, uint32_t lane,
                                                  const uint32_t* __restrict__ mplanes,
                                                  const uint8_t* __restrict__ level,
-                                                 uint32_t* __restrict__ base) {
+                                                 uint32_t* __restrict__ base, uint64_t& walked) {
   const int thr = 48 - tmax;
   int m = __builtin_amdgcn_readfirstlane(code_level(level, k, false));
   for (;;) {
@@ -208,17 +210,22 @@ __device__ __forceinline__ uint32_t fused_search(int g, int k, int tmax, uint32_
     uint32_t best = 64u;
     if (ok) {
       const int w = k - m;
-      for (int s = 0; s < level_nsub(m) && (int)best > thr; ++s)
+      for (int s = 0; s < level_nsub(m) && (int)best > thr; ++s) {
         best = wave_min(min(best, walk_all(LdsRows{base + (size_t)s * 3 * w * kRowWords}, w,
                                            lane)));
+        walked += proj_codewords(w);
+      }
     }
     const int z = kNpts - (int)best;
-    if (m == 1 || (ok && (z > tmax || z >= level_cert(m)))) return best;
-    // Every lower level m' with 7m' - 6 <= max(z, k - 1) is certified.
+    if (m == 1 || (ok && (z > tmax || z >= found_cert(m)))) return best;
+    // The highest lower level q that max(z, k - 1) certifies beforehand
+    // (7q - 6), or whose found certificate 7q - 7 the walk's z already meets
+    // (the level-q walk in the same direction finds at least z; its own
+    // maximum is checked again).
     const int lb = max(ok ? z : 0, k - 1);
     int m2 = 1;
     for (int q = m - 1; q >= 2; --q)
-      if (level_cert(q) <= lb) { m2 = q; break; }
+      if (level_cert(q) <= lb || (ok && found_cert(q) <= z)) { m2 = q; break; }
     m = m2;
   }
 }

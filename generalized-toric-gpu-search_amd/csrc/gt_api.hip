@@ -135,24 +135,27 @@ struct ProfRec {
 };
 
 // Plan area of one exact search: claim counters (kClaimShards x 4 KB), then
-// a 512-byte slot holding MdPlan and, in its last word, the second-pass list
-// count, then five arrays of 4 bytes per code: order, row offsets, aux
-// words, the second-pass list and the list of codes md_fused leaves.  The
-// count of that last list lives apart (it must read 0 before a call's first
+// a 512-byte slot holding MdPlan and, in its last words, the final- and
+// middle-pass list counts and md_fused's code count, then six arrays of 4
+// bytes per code: order, row offsets, aux words, the final-pass list, the
+// list of codes md_fused leaves and the middle-pass list.  The count of the
+// list md_fused leaves lives apart (it must read 0 before a call's first
 // kernel, and the plan area is reallocated as batches grow).
 constexpr size_t kMdClaimBytes = (size_t)kClaimShards * kClaimStride * 4;
 constexpr size_t kMdPlanAreaBytes = kMdClaimBytes + 512;
 constexpr size_t kMdFbCountOff = kMdPlanAreaBytes - 4;
 constexpr size_t kMdFusedCountOff = kMdPlanAreaBytes - 8;   // codes md_fused takes
-constexpr size_t kMdPlanBytesPerCode = 20;
-static_assert(sizeof(MdPlan) <= 504, "MdPlan and the two counts fit the slot");
+constexpr size_t kMdMidCountOff = kMdPlanAreaBytes - 12;
+constexpr size_t kMdPlanBytesPerCode = 24;
+static_assert(sizeof(MdPlan) <= 500, "MdPlan and the three counts fit the slot");
 
 // ---- scratch of the allocation-free exact pass (gt_md_batch_dev_noalloc) --
 // One file-scope device array per GPU, held until the process ends: the plan
 // area, the per-code state words, the bit rows of at most kNaMaxCodes codes
-// with kNaMaxIdx indices in total, and the list count (19.4 MB).  Host code never passes the
-// array's name to a kernel: it uses the address hipGetSymbolAddress returns
-// for the current device (g_scratch_ptr), taken once per device by gt_open.
+// with kNaMaxIdx indices in total, and the list count (19.5 MB).  Host code
+// never passes the array's name to a kernel: it uses the address
+// hipGetSymbolAddress returns for the current device (g_scratch_ptr), taken
+// once per device by gt_open.
 constexpr int32_t kNaMaxCodes = 16384;
 constexpr int64_t kNaMaxIdx = 65536;
 constexpr size_t kNaPlanOff = 0;
@@ -214,6 +217,12 @@ struct gt_ctx {
   uint32_t* ctrl_p = nullptr;
   DevBuf idx, offs, tmax, rows, plan, best, out, zmax, stopf;  // min distance
   DevBuf mdcnt;   // count of the list md_fused leaves (zeroed at open)
+  // Per-code walk record (gt_set_md_walk_record): on for the exact calls
+  // that follow; walk_n = batch size of the last exact call, -1 when that
+  // call was not recorded.
+  bool walk_on = false;
+  int32_t walk_n = -1;
+  DevBuf walkbuf;
   // One-code call: its device state {minimum, waves done} ({64, 0} between
   // launches), the pinned host word the kernel writes (tag, result) to, and
   // the last tag.
@@ -236,7 +245,7 @@ struct gt_ctx {
   std::vector<DevBuf*> all_bufs() {
     return {&mplanes, &params_buf, &ctrl, &idx, &offs, &tmax, &rows, &plan, &best, &mdcnt,
             &out, &zmax, &stopf, &sbuf, &masks, &klo, &khi, &canon, &stab,
-            &children, &bflag, &bsum, &bidx, &bcnt, &bmasks, &brows, &boffs};
+            &children, &bflag, &bsum, &bidx, &bcnt, &bmasks, &brows, &boffs, &walkbuf};
   }
 
   int activate() {
@@ -339,35 +348,50 @@ struct gt_ctx {
     GT_OK(rows.ensure((size_t)std::max<int64_t>(n_idx, 1) * kRowsPerIdx * kRowWords * 4));
     GT_OK(plan.ensure(kMdPlanAreaBytes + (size_t)n * kMdPlanBytesPerCode));
     GT_OK(best.ensure((size_t)n * 8));
+    uint64_t* d_walked = nullptr;
+    walk_n = -1;
+    if (walk_on) {
+      GT_OK(walkbuf.ensure((size_t)n * 8));
+      d_walked = walkbuf.as<uint64_t>();
+      walk_n = n;
+    }
     return md_exact_launch(d_idx, n_idx, d_offsets, n, d_tmax, tmax_s, thresholded, d_out,
                            rows.as<uint32_t>(), plan.as<uint8_t>(), best.as<uint64_t>(),
-                           mdcnt.as<uint32_t>(), s, true, kmax);
+                           mdcnt.as<uint32_t>(), d_walked, s, true, kmax);
   }
 
   // Exact or thresholded search with caller-given scratch: rows
-  // (kRowsPerIdx * n_idx rows), the plan area (kMdPlanAreaBytes + 20 n bytes:
-  // claim counters, MdPlan and the second-pass count, order, row offsets,
-  // aux words, the two lists), the per-code state (n 64-bit words) and the
-  // list count (one word, 0 between calls).  md_fused searches the codes of
-  // k <= fused_k and lists the others; when kmax allows larger codes the
-  // list search follows (md_prepl and md_main at the codes' levels, then at
-  // second_level(k) for the codes the first pass could not certify).  Every word a
-  // kernel reads is written before by an earlier kernel of the call, so no
-  // clearing is needed.
+  // (kRowsPerIdx * n_idx rows), the plan area (kMdPlanAreaBytes + 24 n bytes:
+  // claim counters, MdPlan and the list counts, order, row offsets, aux
+  // words, the three lists), the per-code state (n 64-bit words) and the
+  // count of md_fused's list (one word, 0 between calls).  md_fused searches
+  // the codes of k <= fused_k and lists the others; when kmax allows larger
+  // codes the list search follows: md_prepl and md_main at the codes'
+  // levels, then, for the codes the first pass could not certify, one level
+  // lower (only when some k in (fused_k, kmax] has a middle level) and at
+  // second_level(k).  Every word a kernel reads is written before by an
+  // earlier kernel of the call, so no clearing is needed.
   int md_exact_launch(const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets, int32_t n,
                       const int32_t* d_tmax, int32_t tmax_s, bool thresholded, int32_t* d_out,
                       uint32_t* d_rows, uint8_t* d_plan_area, uint64_t* d_state,
-                      uint32_t* d_big_count, hipStream_t s, bool profile, int kmax) {
+                      uint32_t* d_big_count, uint64_t* d_walked, hipStream_t s, bool profile,
+                      int kmax) {
     uint32_t* d_claims = reinterpret_cast<uint32_t*>(d_plan_area);
     MdPlan* d_plan = reinterpret_cast<MdPlan*>(d_plan_area + kMdClaimBytes);
     uint32_t* d_order = reinterpret_cast<uint32_t*>(d_plan_area + kMdPlanAreaBytes);
     uint32_t* d_roff = d_order + n;
-    const MdAux ax{d_roff + n, reinterpret_cast<uint32_t*>(d_plan_area + kMdFbCountOff),
-                   d_roff + 2 * (size_t)n, d_big_count, d_roff + 3 * (size_t)n};
+    const MdAux ax{d_roff + n,
+                   reinterpret_cast<uint32_t*>(d_plan_area + kMdFbCountOff),
+                   d_roff + 2 * (size_t)n,
+                   d_big_count,
+                   d_roff + 3 * (size_t)n,
+                   reinterpret_cast<uint32_t*>(d_plan_area + kMdMidCountOff),
+                   d_roff + 4 * (size_t)n,
+                   d_walked};
     const MdParams* dp = params_buf.as<MdParams>();
     const int grid = md_grid[thresholded ? 1 : 0];
     const bool timed = profile && prof && !bstate.frozen;
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[8] = {};
     if (timed)
       for (hipEvent_t& e : ev) GT_HIP(hipEventCreate(&e));
     GT_HIP(launch_md_fused(d_idx, n_idx, d_offsets, n, mplanes.as<uint32_t>(), dp, d_tmax, tmax_s,
@@ -375,23 +399,24 @@ struct gt_ctx {
                            reinterpret_cast<uint32_t*>(d_plan_area + kMdFusedCountOff), fused_k,
                            kmax, ctrl_p, s, ev[0], ev[1]));
     if (timed) prof_rec["md_fused"].ev.emplace_back(ev[0], ev[1]);
-    if (kmax <= fused_k) {
-      if (timed)
-        for (int i = 2; i < 6; ++i) GT_HIP(hipEventDestroy(ev[i]));
-      return 0;
+    bool mid = false;
+    for (int k = fused_k + 1; k <= std::min(kmax, kMaxKExact); ++k)
+      mid = mid || mid_level(params.level, k) > 0;
+    if (timed)   // events of the launches that do not follow
+      for (int i = 2; i < 8; ++i)
+        if (kmax <= fused_k || (!mid && (i == 4 || i == 5))) GT_HIP(hipEventDestroy(ev[i]));
+    if (kmax <= fused_k) return 0;
+    static const char* const kPassKernel[kPasses] = {"md_main", "md_main_mid", "md_main2"};
+    for (int pass = kPassFirst; pass < kPasses; ++pass) {
+      if (pass == kPassMid && !mid) continue;
+      GT_HIP(launch_md_prepl(pass, d_idx, n_idx, mplanes.as<uint32_t>(), d_rows, d_offsets, n,
+                             dp, d_plan, d_order, d_roff, d_claims, d_state, d_out, d_tmax,
+                             tmax_s, ax, ctrl_p, s));
+      hipEvent_t e0 = ev[2 + 2 * pass], e1 = ev[3 + 2 * pass];
+      GT_HIP(launch_md_main(thresholded, grid, d_rows, d_plan, d_order, d_roff, d_claims, d_state,
+                            d_tmax, tmax_s, d_out, dp, ax, pass, s, e0, e1));
+      if (timed) prof_rec[kPassKernel[pass]].ev.emplace_back(e0, e1);
     }
-    GT_HIP(launch_md_prepl(false, d_idx, n_idx, mplanes.as<uint32_t>(), d_rows, d_offsets, n, dp,
-                           d_plan, d_order, d_roff, d_claims, d_state, d_out, d_tmax, tmax_s, ax,
-                           ctrl_p, s));
-    GT_HIP(launch_md_main(thresholded, grid, d_rows, d_plan, d_order, d_roff, d_claims, d_state,
-                          d_tmax, tmax_s, d_out, dp, ax, false, s, ev[2], ev[3]));
-    if (timed) prof_rec["md_main"].ev.emplace_back(ev[2], ev[3]);
-    GT_HIP(launch_md_prepl(true, d_idx, n_idx, mplanes.as<uint32_t>(), d_rows, d_offsets, n, dp,
-                           d_plan, d_order, d_roff, d_claims, d_state, d_out, d_tmax, tmax_s, ax,
-                           ctrl_p, s));
-    GT_HIP(launch_md_main(thresholded, grid, d_rows, d_plan, d_order, d_roff, d_claims, d_state,
-                          d_tmax, tmax_s, d_out, dp, ax, true, s, ev[4], ev[5]));
-    if (timed) prof_rec["md_main2"].ev.emplace_back(ev[4], ev[5]);
     return 0;
   }
 };
@@ -571,6 +596,28 @@ int gt_get_md_fused_max_k(gt_ctx* c, int32_t* k) {
   return 0;
 }
 
+int gt_set_md_walk_record(gt_ctx* c, int32_t on) {
+  if (!c) return fail("null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (c->bstate.frozen) return fail("walk record switched during graph capture");
+  c->walk_on = on != 0;
+  return 0;
+}
+
+int gt_get_md_walk_record(gt_ctx* c, uint64_t* out, int32_t n) {
+  if (!c) return fail("null context");
+  if (n < 0 || (n > 0 && !out)) return fail("null out");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (c->walk_n < 0) return fail("the last exact call on this context was not recorded");
+  if (n != c->walk_n) return fail("the last recorded call had %d codes, not %d", c->walk_n, n);
+  if (n == 0) return 0;
+  GT_OK(c->activate());
+  hipStream_t s = c->last_s ? c->last_s : c->stream;
+  GT_HIP(hipMemcpyAsync(out, c->walkbuf.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+  GT_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 int gt_get_md_level(gt_ctx* c, int32_t k, int32_t* level) {
   if (!c) return fail("null context");
   if (k < 0 || k > kMaxKExact) return fail("k=%d outside 0..%d", k, kMaxKExact);
@@ -668,11 +715,12 @@ int gt_md_batch_dev_noalloc(gt_ctx* c, const int32_t* d_idx, int64_t n_idx,
   StreamUse su(c, c->pick(stream));
   GT_OK(su.rc);
   uint8_t* base = c->scratch_dev;
+  c->walk_n = -1;   // not recorded (the record's buffer is allocated)
   return c->md_exact_launch(d_idx, n_idx, d_offsets, n, d_tmax, tmax, thresholded != 0, d_out,
                             reinterpret_cast<uint32_t*>(base + kNaRowsOff), base + kNaPlanOff,
                             reinterpret_cast<uint64_t*>(base + kNaBestOff),
-                            reinterpret_cast<uint32_t*>(base + kNaCountOff), c->pick(stream),
-                            false, kMaxKExact);
+                            reinterpret_cast<uint32_t*>(base + kNaCountOff), nullptr,
+                            c->pick(stream), false, kMaxKExact);
 }
 
 int gt_md_batch_dev_noalloc_limits(int32_t* max_codes, int64_t* max_idx) {

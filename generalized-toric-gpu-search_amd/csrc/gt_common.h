@@ -52,9 +52,12 @@
 // the maximum over the subcodes C_R = {c : c(r d) = 0 for r in R}, R over
 // the level's representatives (1 for m = 1, 2, 5, 6; 3 for m = 3, 4), and
 // every codeword walked is a codeword of C(S), so the maximum z_m found is
-// at most z* in any case.  Hence z_m >= level_cert(m) proves z_m = z*
-// (otherwise z* < level_cert(m), and z_m <= z*); a code below that is
-// walked again at a level whose certificate a known bound meets.  Any k - 1
+// at most z* in any case.  A found maximum certifies itself one zero
+// earlier: a codeword with m zeros in some coset is carried into one of the
+// subcodes, so it has at most z_m zeros, and any other codeword has at most
+// m - 1 zeros in each of the 7 cosets; so z* <= max(z_m, 7 (m - 1)), and
+// z_m >= found_cert(m) = 7 (m - 1) proves z_m = z*.  A code below that is
+// walked again at a level whose certificate a known bound on z* meets.  Any k - 1
 // points are the zeros of some nonzero codeword (k coefficients, k - 1
 // conditions), so z* >= k - 1: level 2 is certified for every code of k >= 9
 // and level 1 (the subcode through (1,1) above, certified whenever z* >= 1)
@@ -87,21 +90,25 @@ constexpr int kLaneBits = 6;     // 64 lanes own 6 Gray index bits
 // Dimension of the exact walk of a code with k generators (see above).
 GT_HD int walk_dim(int k) { return (k <= 1) ? 0 : k - 1; }
 
+// Passes of the list search over the codes of one call (see the anchored
+// subcodes above): the first at the code's level, the middle one level lower
+// for the codes whose first-pass maximum certifies that level, the final one
+// at second_level(k) for the rest.
+constexpr int kPassFirst = 0, kPassMid = 1, kPassFinal = 2, kPasses = 3;
+
 // Per-k launch parameters chosen by the autotuner, indexed by the code's
 // generator count k (the walk has dimension k - 1).
 struct MdParams {
   uint8_t L[kMaxKExact + 1];      // Gray walk bits per lane for full chunks
   uint8_t check_log2[kMaxKExact + 1];  // early-exit check every 2^x blocks
   uint8_t level[kMaxKExact + 1];  // anchor level m of the first pass
-  // Largest k whose first-pass subcodes have no piece above the tiny part
-  // (walk dimension k - level[k] <= kSmallW): the small codes of that pass.
-  uint8_t small_k;
-  // Chunks of a code of k generators at L[k] in the first pass (level[k]),
-  // in the second pass (second_level(k)) and at level 1 (the limit a call is
-  // refused at: 2^31 chunks in total), saturated at 2^31; kept with L and
-  // level by md_params_set / md_level_set.
-  uint32_t chunks[kMaxKExact + 1];
-  uint32_t chunks2[kMaxKExact + 1];
+  // Per pass, the largest k whose subcodes in that pass have no piece above
+  // the tiny part (walk dimension <= kSmallW): the small codes of the pass.
+  uint8_t small_k[kPasses];
+  // Chunks of a code of k generators at L[k] in each pass (code_level) and
+  // at level 1 (the limit a call is refused at: 2^31 chunks in total),
+  // saturated at 2^31; kept with L and level by md_params_set / md_level_set.
+  uint32_t chunks[kPasses][kMaxKExact + 1];
   uint32_t chunks1[kMaxKExact + 1];
 };
 
@@ -174,16 +181,29 @@ constexpr GT_HD int level_last(int m, int s) {
 constexpr GT_HD uint32_t level_rep(int m, int s) {
   return ((1u << (m - 1)) - 1u) | (1u << level_last(m, s));
 }
-// Zeros that certify a level-m maximum (7 (m - 1) + 1).
+// A lower bound on z* that makes level m exact (7 (m - 1) + 1), and a
+// maximum found by a level-m walk that is exact (7 (m - 1); see the top of
+// this file).
 GT_HD int level_cert(int m) { return 7 * m - 6; }
-// Level of the second pass over a code of k generators: 2 when k - 1 meets
+GT_HD int found_cert(int m) { return 7 * m - 7; }
+// Level of the final pass over a code of k generators: 2 when k - 1 meets
 // its certificate (k >= 9), else 1.
 GT_HD int second_level(int k) { return (k >= 9) ? 2 : 1; }
-// Level of a code of k generators in the first or the second pass (codes
-// with k <= 2: level 1; k above the exact range, which no plan holds: 1).
-GT_HD int code_level(const uint8_t* level, int k, bool second) {
+// Level of the middle pass: one below the first pass's when that is above
+// the final pass's, else 0 (no middle pass for k).
+GT_HD int mid_level(const uint8_t* level, int k) {
+  if (k <= 2 || k > kMaxKExact) return 0;
+  const int m = (int)level[k] - 1;
+  return (m > second_level(k)) ? m : 0;
+}
+// Level of a code of k generators in a pass (codes with k <= 2: level 1; k
+// above the exact range, which no plan holds: 1; the middle pass of a k
+// without one, which no code enters: the final pass's level).
+GT_HD int code_level(const uint8_t* level, int k, int pass) {
   if (k <= 2 || k > kMaxKExact) return 1;
-  return second ? second_level(k) : (int)level[k];
+  if (pass == kPassFirst) return (int)level[k];
+  if (pass == kPassMid && mid_level(level, k) > 0) return mid_level(level, k);
+  return second_level(k);
 }
 // Walk dimension of one subcode of a code of k generators at level m.
 GT_HD int anchor_dim(int k, int m) { return (k <= 1) ? 0 : k - m; }
@@ -197,22 +217,27 @@ constexpr int kFusedMaxK = 8;
 // dimension k - m and certifies a maximum of at least 7m - 6 zeros.  On the
 // 4096-code mixed batch (k = 3..8) these levels certify every code: k = 4..6
 // have at least 9 zeros (level 2 needs 8), k = 7, 8 at least 15 (level 3);
-// k = 3 codes often have 7 (level 1).  For k >= 9 level 3 leaves to the
-// second pass (level 2) only codes of distance 35 or more.
-GT_HD int default_level(int k) { return (k <= 3) ? 1 : (k <= 6) ? 2 : 3; }
+// k = 3 codes often have 7 (level 1).  Random codes of k = 9..12 have 19 to
+// 28 zeros, mostly 25 or 28: level 4 (22) certifies about three in four
+// (k = 10: 3 x 37,449 codewords against 3 x 299,593 at level 3), and the
+// others meet level 3's 15 in the first pass, which sends them to the
+// middle pass at level 3.
+GT_HD int default_level(int k) { return (k <= 3) ? 1 : (k <= 6) ? 2 : (k <= 8) ? 3 : 4; }
 
 // L and check interval of k, with the chunk counts the plan reads.
 inline uint32_t sat31(uint64_t c) { return (c >= (1ull << 31)) ? (1u << 31) : (uint32_t)c; }
 inline void md_params_refresh(MdParams& p, int k) {
-  const int m = code_level(p.level, k, false);
-  p.chunks[k] = sat31((uint64_t)level_nsub(m) * code_chunks(anchor_dim(k, m), p.L[k]));
-  const int m2 = code_level(p.level, k, true);
-  p.chunks2[k] = sat31((uint64_t)level_nsub(m2) * code_chunks(anchor_dim(k, m2), p.L[k]));
+  for (int pass = 0; pass < kPasses; ++pass) {
+    const int m = code_level(p.level, k, pass);
+    p.chunks[pass][k] = sat31((uint64_t)level_nsub(m) * code_chunks(anchor_dim(k, m), p.L[k]));
+    int sk = 1;
+    for (int q = 2; q <= kMaxKExact; ++q) {
+      if (anchor_dim(q, code_level(p.level, q, pass)) > kTinyTop + 1) break;
+      sk = q;
+    }
+    p.small_k[pass] = (uint8_t)sk;
+  }
   p.chunks1[k] = sat31(code_chunks(walk_dim(k), p.L[k]));
-  int sk = 1;
-  for (int q = 2; q <= kMaxKExact; ++q)
-    if (anchor_dim(q, code_level(p.level, q, false)) <= kTinyTop + 1) sk = q;
-  p.small_k = (uint8_t)sk;
 }
 inline void md_params_set(MdParams& p, int k, int L, int check_log2) {
   p.L[k] = (uint8_t)L;

@@ -23,36 +23,44 @@ hipError_t launch_md_rows(const int32_t* d_idx, int64_t n_idx, const uint32_t* d
                           uint32_t* d_rows, uint32_t* d_ctrl, hipStream_t s);
 // Per-call words of the exact search.  big_count / big_list: the codes
 // md_fused leaves to the list search (k > kfused), reset by the first-pass
-// md_main; aux[code] = 1 when a list code's first pass found no direction of
-// full rank; fb_count / fb_list: the codes the first pass hands to the
-// second pass, reset by the first-pass md_prepl.
+// md_main; aux[code] = 1 when a list code's pass found no direction of
+// full rank; mid_count / mid_list and fb_count / fb_list: the codes handed
+// to the middle and the final pass, both reset by the first-pass md_prepl;
+// walked (may be null): per code, the projective codewords of the subcodes
+// its passes walked (zeroed by md_order, written by md_fused, added to by
+// each md_main pass that finishes the code).
 struct MdAux {
   uint32_t* aux;
   uint32_t* fb_count;
   uint32_t* fb_list;
   uint32_t* big_count;
   uint32_t* big_list;
+  uint32_t* mid_count;
+  uint32_t* mid_list;
+  uint64_t* walked;
 };
 // Exact (or thresholded) search of one batch: md_order (sizes checked,
 // results of codes without a walk, larger codes listed, the others sorted
 // by size into d_order) and md_fused (a wave per code of k <= kfused, rows
 // in LDS, results written), then,
 // unless the caller knows that no code has more than kfused generators, the
-// list search: md_prepl and md_main at the codes' levels, md_prepl and
-// md_main at second_level(k) for the codes the first could not certify.
+// list search: md_prepl and md_main at the codes' levels (kPassFirst), then
+// over the codes the first could not certify, one level lower (kPassMid,
+// when some k of the batch has a middle level) and at second_level(k)
+// (kPassFinal).
 // d_rows: kRowsPerIdx row slots per index.
 // d_claims: kClaimShards counters, kClaimStride words apart (reset by the plan).
 // d_order, d_roff: n entries each (code index and first index per plan position).
 // d_state: one word per code (minimum weight, chunks finished), initialised
 // by md_prepl for its listed codes; md_main writes a listed code's result
-// when its last chunk is folded, or lists it for the second pass.
+// when its last chunk is folded, or lists it for a later pass.
 hipError_t launch_md_fused(const int32_t* d_idx, int64_t n_idx, const int32_t* d_offsets,
                            int32_t n, const uint32_t* d_mplanes, const MdParams* d_params,
                            const int32_t* d_tmax, int32_t tmax, int32_t* d_out, const MdAux& ax,
                            uint32_t* d_order, uint32_t* d_fused_count, int kfused, int kmax,
                            uint32_t* d_ctrl, hipStream_t s, hipEvent_t ev_start = nullptr,
                            hipEvent_t ev_stop = nullptr);
-hipError_t launch_md_prepl(bool second, const int32_t* d_idx, int64_t n_idx,
+hipError_t launch_md_prepl(int pass, const int32_t* d_idx, int64_t n_idx,
                            const uint32_t* d_mplanes, uint32_t* d_rows, const int32_t* d_offsets,
                            int32_t n, const MdParams* d_params, MdPlan* d_plan, uint32_t* d_order,
                            uint32_t* d_roff, uint32_t* d_claims, uint64_t* d_state,
@@ -62,7 +70,7 @@ hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
                           const MdPlan* d_plan, const uint32_t* d_order, const uint32_t* d_roff,
                           uint32_t* d_claims, uint64_t* d_state, const int32_t* d_tmax,
                           int32_t tmax, int32_t* d_out, const MdParams* d_params, const MdAux& ax,
-                          bool second, hipStream_t s, hipEvent_t ev_start = nullptr,
+                          int pass, hipStream_t s, hipEvent_t ev_start = nullptr,
                           hipEvent_t ev_stop = nullptr);
 // One code, one launch: grid = code_chunks(w, L) single-wave workgroups, the
 // 3w rows (w <= 20) in the kernel arguments.  d_state: {64, 0} before the

@@ -29,9 +29,11 @@
 //               chunk folded at once (other waves stop on the folded value).
 //               Finishing a code writes max_zeros clamped at tmax + 1, or,
 //               below the level's certificate, lists the code for
-//   md_prepl,   the same two launches over that list (the second pass), at
-//   md_main     the level the bound z* >= k - 1 certifies (2 for k >= 9,
-//               else 1), which always finish.
+//   md_prepl,   the same two launches over that list: the middle pass, one
+//   md_main     level lower, for the codes whose first-pass maximum already
+//               certifies that level; then the final pass, at the level the
+//               bound z* >= k - 1 certifies (2 for k >= 9, else 1), for the
+//               rest.  The middle and final passes always finish.
 //   The sampled path keeps the full rows x^b * g_i of C(S) (md_rows).
 
 // Inner step (per projective codeword, all VALU; the block's rows are
@@ -71,18 +73,19 @@ __device__ __forceinline__ void md_rows_one(int64_t t, const int32_t* __restrict
   }
 }
 
-// Rows of one code for a pass at level m = code_level(k) (md_rows_level),
-// starting at slot kRowsPerIdx * offsets[code].  The wave also resets the
-// code's state word, writes the result of a code without a walk (k <= 1),
-// and sets aux[code] = 1 when no direction has full rank (the code then goes
-// to the second pass; its rows are not used for a result).  Codes with
+// Rows of one code for a pass at level m = code_level(k, pass)
+// (md_rows_level), starting at slot kRowsPerIdx * offsets[code].  The wave
+// also resets the code's state word, writes the result of a code without a
+// walk (k <= 1), and, in the first and middle passes, sets aux[code] = 1
+// when no direction has full rank (the code then goes to the final pass;
+// its rows are not used for a result).  Codes with
 // k > 21 or offsets outside [0, n_idx] are flagged by the plan (no row is
 // written for them).
 __device__ __forceinline__ void md_rows_anchor(int code, int32_t n, int64_t n_idx,
                                                const int32_t* __restrict__ idx,
                                                const int32_t* __restrict__ offsets,
                                                const uint32_t* __restrict__ mplanes,
-                                               const uint8_t* __restrict__ level, bool second,
+                                               const uint8_t* __restrict__ level, int pass,
                                                uint32_t* __restrict__ rows, uint64_t* state,
                                                uint32_t* aux, int32_t* out,
                                                const int32_t* tmax_arr, int32_t tmax_s,
@@ -112,7 +115,7 @@ __device__ __forceinline__ void md_rows_anchor(int code, int32_t n, int64_t n_id
   }
   if (k < 2) return;
   uint32_t* base = rows + (size_t)o * kRowsPerIdx * kRowWords;
-  const int m = __builtin_amdgcn_readfirstlane(code_level(level, k, second));
+  const int m = __builtin_amdgcn_readfirstlane(code_level(level, k, pass));
   bool ok;
   switch (m) {
     case 1: ok = md_rows_level<1>(g, k, lane, mplanes, base); break;
@@ -123,8 +126,8 @@ __device__ __forceinline__ void md_rows_anchor(int code, int32_t n, int64_t n_id
     default: ok = md_rows_level<6>(g, k, lane, mplanes, base); break;   // kMaxLevel
   }
   if (!ok) {
-    if (second) {
-      // Second pass, level 2 without a full-rank direction: every index is
+    if (pass == kPassFinal) {
+      // Final pass, level 2 without a full-rank direction: every index is
       // the same point, the set spans the zero codeword, and zero rows give
       // its 49 zeros.
       const int w = k - m;
@@ -162,7 +165,8 @@ __global__ __launch_bounds__(kOrderThreads) void md_order_kernel(
     const int32_t* __restrict__ offsets, int32_t n, int64_t n_idx,
     const int32_t* __restrict__ tmax_arr, int32_t tmax_s, int32_t* __restrict__ out,
     uint32_t* big_count, uint32_t* big_list, uint32_t* __restrict__ order,
-    uint32_t* fused_count, int32_t kfused, int32_t kmax, uint32_t* ctrl) {
+    uint32_t* fused_count, int32_t kfused, int32_t kmax, uint64_t* __restrict__ walked,
+    uint32_t* ctrl) {
   __shared__ uint32_t cnt[kFusedMaxK + 1];
   const int tid = (int)threadIdx.x;
   if (tid <= kFusedMaxK) cnt[tid] = 0u;
@@ -172,6 +176,7 @@ __global__ __launch_bounds__(kOrderThreads) void md_order_kernel(
     const int o = offsets[i];
     const int k = offsets[i + 1] - o;
     const bool bad_k = k < 0 || k > kMaxKExact || k > kmax || o < 0 || (int64_t)o + k > n_idx;
+    if (walked) walked[i] = 0ull;
     if (bad_k || k <= 1) {
       if (bad_k) err |= kErrBadK;
       // No walk: k = 0 (and bad input) gives 0, k = 1 a monomial (weight 49).
@@ -218,7 +223,7 @@ __global__ __launch_bounds__(kFusedWaves * kWave, 5) void md_fused_kernel(
     const uint32_t* __restrict__ order, const uint32_t* fused_count,
     const uint32_t* __restrict__ mplanes, const MdParams* __restrict__ p,
     const int32_t* __restrict__ tmax_arr, int32_t tmax_s, int32_t* __restrict__ out,
-    uint32_t* ctrl) {
+    uint64_t* __restrict__ walked, uint32_t* ctrl) {
   __shared__ __align__(16) uint32_t lrows[kFusedWaves][kFusedRows * kRowWords];
   const int wv = (int)(threadIdx.x / kWave);
   const uint32_t w = blockIdx.x * kFusedWaves + wv;
@@ -246,15 +251,20 @@ __global__ __launch_bounds__(kFusedWaves * kWave, 5) void md_fused_kernel(
       }
       continue;
     }
-    const uint32_t best = fused_search(g, k, tmax, lane, mplanes, p->level, lrows[wv]);
-    if (lane == 0) out[code] = md_result((int)best, tmax);
+    uint64_t cw = 0;
+    const uint32_t best = fused_search(g, k, tmax, lane, mplanes, p->level, lrows[wv], cw);
+    if (lane == 0) {
+      out[code] = md_result((int)best, tmax);
+      if (walked) walked[code] = cw;
+    }
   }
 }
 
 // ---- plan and rows of a listed set of codes ----------------------------------
 // The list search: the first pass over the codes md_fused left (k > kfused,
-// at their levels), then the second pass (second_level(k)) over the codes
-// the first pass could not certify.  Workgroup 0 plans the listed codes (positions by
+// at their levels), then the middle pass (one level lower) and the final
+// pass (second_level(k)) over the codes the first pass could not certify,
+// each over its own list.  Workgroup 0 plans the listed codes (positions by
 // bucket from LDS counters; the order inside a bucket does not matter); the
 // other workgroups build their rows and reset their state words, a wave per
 // listed code in turn.  With an empty list the plan is empty and the walk
@@ -268,14 +278,17 @@ __device__ __forceinline__ void md_list_plan(const int32_t* __restrict__ offsets
                                              uint32_t* __restrict__ order,
                                              uint32_t* __restrict__ roff,
                                              uint32_t* __restrict__ claims, uint32_t nl,
-                                             const uint32_t* __restrict__ list, bool second,
-                                             uint32_t* fb_count, uint32_t* ctrl) {
+                                             const uint32_t* __restrict__ list, int pass,
+                                             const MdAux& ax, uint32_t* ctrl) {
   constexpr int kB = kMaxKExact;
   __shared__ uint32_t cnt[kB + 1];
   const int tid = threadIdx.x;
   if (tid <= kB) cnt[tid] = 0u;
   if (tid < kClaimShards) claims[tid * kClaimStride] = 0u;
-  if (tid == 0 && !second) *fb_count = 0u;   // the second-pass list of this call
+  if (tid == 0 && pass == kPassFirst) {   // the later lists of this call
+    *ax.mid_count = 0u;
+    *ax.fb_count = 0u;
+  }
   __syncthreads();
   for (uint32_t i = tid; i < nl; i += blockDim.x) {
     const int code = (int)list[i];
@@ -291,13 +304,13 @@ __device__ __forceinline__ void md_list_plan(const int32_t* __restrict__ offsets
     for (int b = 0; b < kB; ++b) {
       const int k = min(kB + 1 - b, kB);
       const uint32_t t = cnt[b];
-      const uint64_t cc = second ? p->chunks2[k] : p->chunks[k];
+      const uint64_t cc = p->chunks[pass][k];
       plan->ub[b] = eu;
       plan->cb[b] = (uint32_t)min(ec, 1ull << 31);
       plan->pb[b] = ep;
       plan->cc[b] = (uint32_t)cc;
       cnt[b] = ep;
-      eu += (uint64_t)t * anchor_units(k, code_level(p->level, k, second));
+      eu += (uint64_t)t * anchor_units(k, code_level(p->level, k, pass));
       ec = min(ec + (uint64_t)t * cc, 1ull << 31);
       e1 = min(e1 + (uint64_t)t * p->chunks1[k], 1ull << 31);
       ep += t;
@@ -331,14 +344,15 @@ __global__ __launch_bounds__(kPlanThreads) void md_prepl_kernel(
     const MdParams* __restrict__ p, MdPlan* __restrict__ plan, uint32_t* __restrict__ order,
     uint32_t* __restrict__ roff, uint32_t* __restrict__ claims, uint64_t* __restrict__ state,
     int32_t* __restrict__ out, const int32_t* __restrict__ tmax_arr, int32_t tmax_s, MdAux ax,
-    int32_t second, uint32_t* ctrl) {
-  const uint32_t* cnt = second ? ax.fb_count : ax.big_count;
-  const uint32_t* list = second ? ax.fb_list : ax.big_list;
+    int32_t pass, uint32_t* ctrl) {
+  const uint32_t* cnt = (pass == kPassFirst) ? ax.big_count
+                        : (pass == kPassMid) ? ax.mid_count : ax.fb_count;
+  const uint32_t* list = (pass == kPassFirst) ? ax.big_list
+                         : (pass == kPassMid) ? ax.mid_list : ax.fb_list;
   uint32_t nl = __builtin_amdgcn_readfirstlane(*cnt);
   nl = min(nl, (uint32_t)max(n, 0));
   if (blockIdx.x == 0) {
-    md_list_plan(offsets, n, p, plan, order, roff, claims, nl, list, second != 0, ax.fb_count,
-                 ctrl);
+    md_list_plan(offsets, n, p, plan, order, roff, claims, nl, list, pass, ax, ctrl);
     return;
   }
   constexpr uint32_t kWaves = kPrepRowBlocks * (kPlanThreads / kWave);
@@ -346,8 +360,8 @@ __global__ __launch_bounds__(kPlanThreads) void md_prepl_kernel(
        i += kWaves) {
     const uint32_t code = __builtin_amdgcn_readfirstlane(list[i]);
     if (code < (uint32_t)n)
-      md_rows_anchor((int)code, n, n_idx, idx, offsets, mplanes, p->level, second != 0, rows,
-                     state, ax.aux, out, tmax_arr, tmax_s, ctrl);
+      md_rows_anchor((int)code, n, n_idx, idx, offsets, mplanes, p->level, pass, rows, state,
+                     ax.aux, out, tmax_arr, tmax_s, ctrl);
   }
 }
 
@@ -367,7 +381,7 @@ struct MdShared {
   uint8_t L[kMaxKExact + 1];
   uint8_t check_log2[kMaxKExact + 1];
   uint8_t level[kMaxKExact + 1];
-  uint8_t small_k;
+  uint8_t small_k[kPasses];
 };
 
 __device__ __forceinline__ void md_load_shared(MdShared& sh, const MdPlan* __restrict__ plan,
@@ -380,7 +394,7 @@ __device__ __forceinline__ void md_load_shared(MdShared& sh, const MdPlan* __res
     sh.check_log2[threadIdx.x] = p->check_log2[threadIdx.x];
     sh.level[threadIdx.x] = p->level[threadIdx.x];
   }
-  if (threadIdx.x == 0) sh.small_k = p->small_k;
+  if (threadIdx.x < kPasses) sh.small_k[threadIdx.x] = p->small_k[threadIdx.x];
   __syncthreads();
 }
 
@@ -401,7 +415,7 @@ struct MdChunk {
 // (chunk, subcode) pairs, then codes: id = ((c * nsub + s) * codes + q).
 __device__ __forceinline__ MdChunk md_decode(const MdShared& sh, uint32_t g,
                                              const uint32_t* __restrict__ order,
-                                             const uint32_t* __restrict__ roff, bool second) {
+                                             const uint32_t* __restrict__ roff, int pass) {
   const uint32_t* cb = sh.plan + offsetof(MdPlan, cb) / 4;
   const uint32_t* pb = sh.plan + offsetof(MdPlan, pb) / 4;
   const uint32_t* ccv = sh.plan + offsetof(MdPlan, cc) / 4;
@@ -419,7 +433,7 @@ __device__ __forceinline__ MdChunk md_decode(const MdShared& sh, uint32_t g,
   MdChunk r;
   r.nch = cc;
   const int k = kMaxKExact + 1 - lo;
-  r.m = __builtin_amdgcn_readfirstlane(code_level(sh.level, k, second));
+  r.m = __builtin_amdgcn_readfirstlane(code_level(sh.level, k, pass));
   const uint32_t nsub = (uint32_t)level_nsub(r.m);
   r.w = k - r.m;
   r.L = __builtin_amdgcn_readfirstlane(sh.L[k]);
@@ -461,25 +475,41 @@ __device__ __forceinline__ uint32_t md_walk(const MdChunk& ck, const uint32_t* R
 // performed before the minimum; both halves of one word share a coherence
 // point.  The wave whose count completes the code (all its nch chunks) reads
 // the final minimum there and finishes the code (md_finish).
-// A pass at level m > 1 certifies its maximum z only when z >= level_cert(m)
-// (or z > tmax: the threshold is exceeded by a real codeword) and the code
-// had a direction of full rank; any other code is appended to the
-// second-pass list instead of written (gt_common.h).  Lane 0 only.
-__device__ __forceinline__ void md_finish(int code, uint32_t bw, int m, int tmax,
-                                          const MdAux& ax, int32_t* out, bool second) {
+// A pass at level m > 1 certifies its maximum z only when z >= found_cert(m)
+// = 7m - 7 (or z > tmax: the threshold is exceeded by a real codeword) and
+// the code had a direction of full rank.  Any other code of the first pass
+// is appended to the middle-pass list when the middle level is certified
+// beforehand by the bound max(z, k - 1) on z* (7q - 6) or its found
+// certificate is met by z (the middle walk, in the same direction, finds at
+// least z), else to the final-pass list; a middle-pass code that its walk
+// does not certify goes to the final-pass list too.  The final pass's level
+// is certified by z* >= k - 1.  The rest are written.  Lane 0 only.
+__device__ __forceinline__ void md_finish(int code, uint32_t bw, int k, int m, int mid, int tmax,
+                                          const MdAux& ax, int32_t* out, int pass) {
+  if (ax.walked) ax.walked[code] += (uint64_t)level_nsub(m) * proj_codewords(k - m);
   const int z = kNpts - (int)bw;
-  // The second pass's level is certified by the bound z* >= k - 1.
-  if (!second && m > 1 && (ax.aux[code] != 0u || (z <= tmax && z < level_cert(m)))) {
-    const uint32_t i = atomicAdd(ax.fb_count, 1u);
-    ax.fb_list[i] = (uint32_t)code;
-  } else {
+  bool done = true;
+  bool to_mid = false;
+  if (pass != kPassFinal && m > 1) {
+    const bool norank = ax.aux[code] != 0u;
+    if (norank || (z <= tmax && z < found_cert(m))) {
+      done = false;
+      to_mid = pass == kPassFirst && mid > 0 &&
+               (level_cert(mid) <= k - 1 || (!norank && found_cert(mid) <= z));
+    }
+  }
+  if (done) {
     out[code] = md_result((int)bw, tmax);
+  } else if (to_mid) {
+    ax.mid_list[atomicAdd(ax.mid_count, 1u)] = (uint32_t)code;
+  } else {
+    ax.fb_list[atomicAdd(ax.fb_count, 1u)] = (uint32_t)code;
   }
 }
 
 __device__ __forceinline__ void md_fold(uint64_t* state, uint32_t b, uint32_t n, uint32_t nch,
-                                        int code, int m, int tmax, const MdAux& ax,
-                                        int32_t* out, bool second) {
+                                        int code, int k, int m, int mid, int tmax,
+                                        const MdAux& ax, int32_t* out, int pass) {
   uint32_t* st = reinterpret_cast<uint32_t*>(state);
   const uint32_t prev = __hip_atomic_fetch_min(&st[0], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __builtin_amdgcn_s_waitcnt(0);
@@ -487,7 +517,7 @@ __device__ __forceinline__ void md_fold(uint64_t* state, uint32_t b, uint32_t n,
       __hip_atomic_fetch_add(&st[1], n + (prev >> 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + n;
   if (done == nch) {
     const uint32_t bw = __hip_atomic_load(&st[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    md_finish(code, bw, m, tmax, ax, out, second);
+    md_finish(code, bw, k, m, mid, tmax, ax, out, pass);
   }
 }
 
@@ -573,17 +603,19 @@ __global__ __launch_bounds__(256) void md_main_exact_kernel(
     const uint32_t* __restrict__ rows, const MdPlan* __restrict__ plan,
     const uint32_t* __restrict__ order, const uint32_t* __restrict__ roff, uint64_t* state,
     const int32_t* __restrict__ tmax_arr, int32_t tmax_s, int32_t* __restrict__ out,
-    const MdParams* __restrict__ p, MdAux ax, int32_t second) {
+    const MdParams* __restrict__ p, MdAux ax, int32_t pass) {
   __shared__ MdShared sh;
   // The first-pass list was read by md_prepl: empty for the next call.
-  if (!second && blockIdx.x == 0 && threadIdx.x == 0) *ax.big_count = 0u;
+  if (pass == kPassFirst && blockIdx.x == 0 && threadIdx.x == 0) *ax.big_count = 0u;
   md_load_shared(sh, plan, p);
-  // First bucket of small codes (k <= small_k: subcodes without a piece
-  // above the tiny part; in the second pass k <= 5, at level 1).
-  const int kSmallB = kMaxKExact + 1 -
-                      (second ? kSmallW + 1 : (int)__builtin_amdgcn_readfirstlane(sh.small_k));
+  // First bucket of small codes (k <= small_k of the pass: subcodes without
+  // a piece above the tiny part).
+  const int kSmallB = kMaxKExact + 1 - (int)__builtin_amdgcn_readfirstlane(sh.small_k[pass]);
   auto lvl = [&](int k) {
-    return (int)__builtin_amdgcn_readfirstlane(code_level(sh.level, k, second != 0));
+    return (int)__builtin_amdgcn_readfirstlane(code_level(sh.level, k, pass));
+  };
+  auto midl = [&](int k) {
+    return (int)__builtin_amdgcn_readfirstlane(mid_level(sh.level, k));
   };
   const uint32_t lane = lane_id();
   const uint64_t* ub = reinterpret_cast<const uint64_t*>(sh.plan + offsetof(MdPlan, ub) / 4);
@@ -620,7 +652,7 @@ __global__ __launch_bounds__(256) void md_main_exact_kernel(
         b = min(b, tiny_chunk(ConstRows{R + (size_t)s * 3 * w * kRowWords}, tiny_top(w), lane));
       b = wave_min(b);
       if (lane == 0)
-        md_finish(code, b, m, tmax_arr ? tmax_arr[code] : tmax_s, ax, out, second != 0);
+        md_finish(code, b, k, m, midl(k), tmax_arr ? tmax_arr[code] : tmax_s, ax, out, pass);
     }
   }
 
@@ -653,14 +685,15 @@ __global__ __launch_bounds__(256) void md_main_exact_kernel(
     const uint64_t e = min(a + cu, u1);   // end of this wave's part of it
     const uint32_t b = wave_min(walk_code_units(R, w, level_nsub(m), off, e - a, lane));
     const int tmax = tmax_arr ? tmax_arr[code] : tmax_s;
+    const int k = kMaxKExact + 1 - lo;
     if (a >= u0 && a + cu <= u1) {
-      if (lane == 0) md_finish(code, b, m, tmax, ax, out, second != 0);
+      if (lane == 0) md_finish(code, b, k, m, midl(k), tmax, ax, out, pass);
     } else {
       const uint64_t first = (a < u0) ? deal.wave_of(a) : wid;
       const uint64_t last = (a + cu > u1) ? deal.wave_of(a + cu - 1) : wid;
       if (lane == 0)
-        md_fold(&state[code], b, 1u, (uint32_t)(last - first + 1), code, m, tmax, ax, out,
-                second != 0);
+        md_fold(&state[code], b, 1u, (uint32_t)(last - first + 1), code, k, m, midl(k), tmax,
+                ax, out, pass);
     }
     u = e;
     if (u >= u1) break;
@@ -696,9 +729,9 @@ __global__ __launch_bounds__(256) void md_main_thr_kernel(
     const uint32_t* __restrict__ rows, const MdPlan* __restrict__ plan,
     const uint32_t* __restrict__ order, const uint32_t* __restrict__ roff, uint32_t* claims,
     uint64_t* state, const int32_t* __restrict__ tmax_arr, int32_t tmax_s,
-    int32_t* __restrict__ out, const MdParams* __restrict__ p, MdAux ax, int32_t second) {
+    int32_t* __restrict__ out, const MdParams* __restrict__ p, MdAux ax, int32_t pass) {
   __shared__ MdShared sh;
-  if (!second && blockIdx.x == 0 && threadIdx.x == 0) *ax.big_count = 0u;
+  if (pass == kPassFirst && blockIdx.x == 0 && threadIdx.x == 0) *ax.big_count = 0u;
   md_load_shared(sh, plan, p);
   const uint32_t lane = lane_id();
   const uint32_t total = __builtin_amdgcn_readfirstlane(sh.plan[offsetof(MdPlan, cb) / 4 + kMaxKExact]);
@@ -709,14 +742,15 @@ __global__ __launch_bounds__(256) void md_main_thr_kernel(
   uint32_t g = __builtin_amdgcn_readfirstlane(first) * kClaimShards + shard;
   for (;;) {
     const bool have = g < total;
-    int code = 0, tmax = 0, m = 1;
+    int code = 0, tmax = 0, m = 1, k = 0;
     bool fold = false;
     uint32_t b = 64u, nch = 0;
     if (have) {
-      const MdChunk ck = md_decode(sh, g, order, roff, second != 0);
+      const MdChunk ck = md_decode(sh, g, order, roff, pass);
       code = ck.code;
       nch = ck.nch;
       m = ck.m;
+      k = ck.w + ck.m;
       fold = true;   // walked or skipped, the chunk is counted
       const uint32_t* R =
           rows + ((size_t)ck.roff * kRowsPerIdx + (size_t)ck.s * 3 * ck.w) * kRowWords;
@@ -743,7 +777,9 @@ __global__ __launch_bounds__(256) void md_main_thr_kernel(
     }
     uint32_t next = 0;
     if (lane == 0) {
-      if (fold) md_fold(&state[code], b, 1u, nch, code, m, tmax, ax, out, second != 0);
+      if (fold)
+        md_fold(&state[code], b, 1u, nch, code, k, m, mid_level(sh.level, k), tmax, ax, out,
+                pass);
       next = atomicAdd(&claims[shard * kClaimStride], 1u);
     }
     g = __builtin_amdgcn_readfirstlane(next) * kClaimShards + shard;
@@ -931,18 +967,18 @@ hipError_t launch_md_fused(const int32_t* d_idx, int64_t n_idx, const int32_t* d
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(md_order_kernel, dim3(1), dim3(kOrderThreads), 0, s, d_offsets, n, n_idx,
                      d_tmax, tmax, d_out, ax.big_count, ax.big_list, d_order, d_fused_count,
-                     kfused, kmax, d_ctrl);
+                     kfused, kmax, ax.walked, d_ctrl);
   // About kFusedCodesPerWave codes per wave, at most kFusedMaxWaves waves.
   const int64_t waves = std::min<int64_t>(((int64_t)n + kFusedCodesPerWave - 1) /
                                           kFusedCodesPerWave, kFusedMaxWaves);
   const unsigned blocks = (unsigned)((waves + kFusedWaves - 1) / kFusedWaves);
   hipExtLaunchKernelGGL(md_fused_kernel, dim3(blocks), dim3(kFusedWaves * kWave), 0, s, ev_start,
                         ev_stop, 0, d_idx, d_offsets, (int32_t)waves, d_order, d_fused_count,
-                        d_mplanes, d_params, d_tmax, tmax, d_out, d_ctrl);
+                        d_mplanes, d_params, d_tmax, tmax, d_out, ax.walked, d_ctrl);
   return hipGetLastError();
 }
 
-hipError_t launch_md_prepl(bool second, const int32_t* d_idx, int64_t n_idx,
+hipError_t launch_md_prepl(int pass, const int32_t* d_idx, int64_t n_idx,
                            const uint32_t* d_mplanes, uint32_t* d_rows, const int32_t* d_offsets,
                            int32_t n, const MdParams* d_params, MdPlan* d_plan, uint32_t* d_order,
                            uint32_t* d_roff, uint32_t* d_claims, uint64_t* d_state,
@@ -950,7 +986,7 @@ hipError_t launch_md_prepl(bool second, const int32_t* d_idx, int64_t n_idx,
                            uint32_t* d_ctrl, hipStream_t s) {
   hipLaunchKernelGGL(md_prepl_kernel, dim3(1 + kPrepRowBlocks), dim3(kPlanThreads), 0, s, d_idx,
                      n_idx, d_mplanes, d_rows, d_offsets, n, d_params, d_plan, d_order, d_roff,
-                     d_claims, d_state, d_out, d_tmax, tmax, ax, second ? 1 : 0, d_ctrl);
+                     d_claims, d_state, d_out, d_tmax, tmax, ax, (int32_t)pass, d_ctrl);
   return hipGetLastError();
 }
 
@@ -967,10 +1003,10 @@ hipError_t launch_md_main(bool thresholded, int grid, const uint32_t* d_rows,
                           const MdPlan* d_plan, const uint32_t* d_order, const uint32_t* d_roff,
                           uint32_t* d_claims, uint64_t* d_state, const int32_t* d_tmax,
                           int32_t tmax, int32_t* d_out, const MdParams* d_params, const MdAux& ax,
-                          bool second, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
+                          int pass, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
   // With events, the launch records them from its own dispatch (start and
   // end of the kernel), without marker packets of their own on the stream.
-  const int32_t l1 = second ? 1 : 0;
+  const int32_t l1 = (int32_t)pass;
   if (thresholded)
     hipExtLaunchKernelGGL(md_main_thr_kernel, dim3(grid), dim3(256), 0, s, ev_start, ev_stop, 0,
                           d_rows, d_plan, d_order, d_roff, d_claims, d_state, d_tmax, tmax, d_out,

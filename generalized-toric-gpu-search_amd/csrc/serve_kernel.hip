@@ -256,7 +256,8 @@ __device__ void worker(ServeDev* d, ServeHost* h, const uint32_t* __restrict__ m
       if (k <= kFusedMaxK) {
         // The whole search of a small request (rows built in the LDS area).
         t2 = wall_clock64();
-        b = fused_search(gi, k, tmax, lane, mplanes, d->level, lrows);
+        uint64_t walked = 0;
+        b = fused_search(gi, k, tmax, lane, mplanes, d->level, lrows, walked);
         staged_slot = 0xFFFFFFFFu;
         ++st_staged;
       } else {

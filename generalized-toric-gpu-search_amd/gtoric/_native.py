@@ -44,6 +44,8 @@ _SIGS = {
                          ctypes.c_int),
     "gt_set_md_fused_max_k": ([vp, i32], ctypes.c_int),
     "gt_get_md_fused_max_k": ([vp, P(i32)], ctypes.c_int),
+    "gt_set_md_walk_record": ([vp, i32], ctypes.c_int),
+    "gt_get_md_walk_record": ([vp, vp, i32], ctypes.c_int),
     "gt_md_batch_dev_noalloc": ([vp, vp, i64, vp, i32, vp, i32, i32, vp, vp], ctypes.c_int),
     "gt_md_batch_dev_noalloc_limits": ([P(i32), P(i64)], ctypes.c_int),
     "gt_md_single": ([vp, vp, i32, i32, P(i32)], ctypes.c_int),
@@ -230,6 +232,16 @@ class Context:
         k = ctypes.c_int32()
         check(lib().gt_get_md_fused_max_k(self.handle, ctypes.byref(k)))
         return k.value
+
+    def set_md_walk_record(self, on: bool) -> None:
+        """Record, per code, the projective codewords each exact call walks."""
+        check(lib().gt_set_md_walk_record(self.handle, int(bool(on))))
+
+    def md_walk_record(self, n: int) -> np.ndarray:
+        """The record of the last exact call (n = its batch size)."""
+        out = np.zeros(n, dtype=np.uint64)
+        check(lib().gt_get_md_walk_record(self.handle, out.ctypes.data_as(vp), int(n)))
+        return out
 
     def md_batch_dev_noalloc(self, d_idx: int, n_idx: int, d_offsets: int, n: int, d_out: int,
                              tmax: int = 48, d_tmax: int | None = None,

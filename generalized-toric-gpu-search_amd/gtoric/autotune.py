@@ -69,12 +69,12 @@ def _batch(k: int, n: int, seed: int) -> list[list[int]]:
     return [sorted(rng.choice(49, k, replace=False).tolist()) for _ in range(n)]
 
 
-SEARCH_KERNELS = ("md_fused", "md_main", "md_main2")
+SEARCH_KERNELS = ("md_fused", "md_main", "md_main_mid", "md_main2")
 
 
 def _time_md(ctx: _native.Context, flat, offs, tmax, repeats: int) -> float:
     """Device time of one exact (or thresholded) search: the one-wave kernel
-    plus both planned passes, per call."""
+    plus the planned passes, per call."""
     ctx.md_batch(flat, offs, tmax)              # warm
     for name in SEARCH_KERNELS:
         ctx.prof_read(name)

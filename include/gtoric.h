@@ -66,18 +66,24 @@ int gt_get_md_params(gt_ctx* ctx, int32_t k, int32_t* L, int32_t* check_log2);
  * parameters above, with the same effect on results (none).  A level-m pass
  * walks the subcodes that vanish at m points of a line through the torus
  * point (1,1) (1 subcode for m = 1, 2, 5, 6, 3 for m = 3, 4, of dimension
- * k - m) and certifies a maximum of at least 7m - 6 zeros; codes below that
- * are searched again at level 1 (the subcode through (1,1), which the
+ * k - m); a maximum it finds of at least 7m - 7 zeros is exact, and so is
+ * every level-m maximum of a code known to have at least 7m - 6.  Codes
+ * below that are searched again at a lower level: in the same wavefront for
+ * k <= 8, else at m - 1 when that level is certified by what the first pass
+ * found, and at level 2 (k >= 9, certified by the k - 1 zeros every code
+ * has) or level 1 otherwise (the subcode through (1,1), which the
  * reference's enumeration of all 8^k - 1 coefficient vectors, kernel.py:
  * 128-139, amounts to).  1 <= level <= min(k - 1, 6), and k - level must not
- * decrease with k.  Defaults: 1 for k <= 3, 2 for k = 4..6, 3 above. */
+ * decrease with k.  Defaults: 1 for k <= 3, 2 for k = 4..6, 3 for k = 7, 8,
+ * 4 above. */
 int gt_set_md_level(gt_ctx* ctx, int32_t k, int32_t level);
 int gt_get_md_level(gt_ctx* ctx, int32_t k, int32_t* level);
 
 /* The level table itself (host only, no device): representative s of level
  * m as a bit mask over r (the points r d of a line, bit r set <=> r in the
- * set), the number of representatives, and the certificate 7m - 6.  No
- * counterpart in the reference; for checking the table. */
+ * set), the number of representatives, and the certificate 7m - 6 (the
+ * zeros that make level m exact beforehand; a found maximum is exact one
+ * zero earlier).  No counterpart in the reference; for checking the table. */
 int gt_md_level_info(int32_t m, int32_t s, uint32_t* rep_mask, int32_t* nsub, int32_t* cert);
 
 /* Batched minimum distance, mixed k (replaces DistanceOracle.max_zeros_batch,
@@ -109,6 +115,17 @@ int gt_md_batch_dev2(gt_ctx* ctx, const int32_t* d_idx, int64_t n_idx,
                      const int32_t* d_offsets, int32_t n, const int32_t* d_tmax_per_set,
                      int32_t tmax, int32_t thresholded, int32_t kmax, int64_t sample_count,
                      uint64_t sample_seed, int32_t* d_out, void* stream);
+
+/* Per-code record of the work of exact calls: while on (default off), each
+ * exact or thresholded call on the context's own scratch (not the
+ * allocation-free call) writes, for each code, the projective codewords of
+ * the subcodes its passes walked (exact mode: the codewords evaluated;
+ * thresholded: an upper bound, chunks stop early); the getter copies the
+ * record of the last such call, whose batch size n must be, after waiting
+ * for it.  No counterpart in the reference (whose enumeration walks 8^k - 1
+ * coefficient vectors per code); for measuring the search. */
+int gt_set_md_walk_record(gt_ctx* ctx, int32_t on);
+int gt_get_md_walk_record(gt_ctx* ctx, uint64_t* out, int32_t n);
 
 /* Largest set size searched one wavefront per code (1..8, default 8); larger
  * codes go through the planned walk over all CUs.  Results never depend on

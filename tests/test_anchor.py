@@ -6,8 +6,11 @@
 * The certification argument on random codes, with a numpy model of the
   device's elimination: the maximum over a level's subcodes never exceeds the
   oracle's exact maximum (reference algorithm, all 8^k - 1 coefficient
-  vectors), and equals it whenever it reaches the certificate.  A mistake in
-  the representatives, the points of a line or the elimination fails these.
+  vectors), equals it whenever the exact maximum reaches the certificate
+  7m - 6, and equals it whenever the level's own maximum reaches 7m - 7 (a
+  codeword outside the subcodes has at most m - 1 zeros in each of the 7
+  cosets).  A mistake in the representatives, the points of a line or the
+  elimination fails these.
 """
 
 from __future__ import annotations
@@ -56,7 +59,7 @@ def test_representatives_cover_each_orbit_once(m):
 def test_certified_levels_equal_the_exact_maximum():
     rng = random.Random(11)
     tables = {m: _level(m)[0] for m in range(1, 7)}
-    checked = 0
+    checked = edge = 0
     for trial in range(60):
         k = rng.randint(2, 5)
         if trial % 10 == 9:   # repeated indices: the zero codeword (49 zeros)
@@ -71,9 +74,10 @@ def test_certified_levels_equal_the_exact_maximum():
             assert all(len(b) == k - m for b in bases)
             zm = max(max_zeros_span(b) for b in bases)
             assert zm <= z, (S, m, zm, z)
-            if zm >= 7 * m - 6:
+            if zm >= 7 * m - 7:
                 assert zm == z, (S, m, zm, z)
+                edge += m > 1 and zm == 7 * m - 7
             if z >= 7 * m - 6:
                 assert zm == z, (S, m, zm, z)
             checked += 1
-    assert checked > 100
+    assert checked > 100 and edge >= 5   # the found certificate met exactly

@@ -94,31 +94,86 @@ def test_mixed_k3_10_batch_against_fixture(dist):
     assert dist.max_zeros_batch(subsets, (50 - mz).astype(np.int32)) == mz.tolist()
 
 
-def test_second_pass_of_planned_codes(dist):
-    """Codes above the one-wave bound (k = 9, 10) at forced high first-pass
-    levels (4 and 5: certificates 22 and 29 zeros) mostly fall below their
-    certificate and are searched again by the planned walk's second pass, at
-    level 2 (certified by max_zeros >= k - 1 >= 8): the fixture's k = 9, 10
-    codes still equal the oracle, exact and thresholded.  Sets whose indices
-    are all one point have no direction of full rank at any level; they span
-    the zero codeword (49 zeros), as the reference's enumeration finds."""
+@pytest.mark.parametrize("table", [{}, {9: 3, 10: 3}, {9: 4, 10: 5}, {8: 4, 9: 5, 10: 6}])
+def test_later_passes_of_planned_codes(dist, table):
+    """Codes above the one-wave bound (k = 9, 10) that fall below their
+    first-pass level's certificate (7m - 6 zeros: 22 at the default level 4)
+    are searched again by the planned walk: at the middle level m - 1 when
+    the first pass's maximum (or k - 1) meets that level's certificate, else
+    at the final level 2 (certified by max_zeros >= k - 1 >= 8).  Level 3
+    has no middle pass (every failing code goes to level 2); levels 5 and 6
+    send the codes of 22 or more zeros to levels 4 and 5 and the others to
+    level 2.  The fixture's k = 9, 10 codes equal the oracle at each table,
+    exact and thresholded.  Sets whose indices are all one point have no
+    direction of full rank at any level; they span the zero codeword (49
+    zeros), as the reference's enumeration finds."""
     ctx = dist.context
     g = np.load(GOLD / "md_wide_golden.npz")
     sets = _rows(g["flat"], g["offsets"])
     big = [int(i) for i in g["pos"] if len(sets[int(i)]) >= 9]
     bsets = [sets[i] for i in big]
     exp = [int(z) for i, z in zip(g["pos"], g["mz"]) if len(sets[int(i)]) >= 9]
+    # Both kinds of code are in the fixture: certified by the first pass at
+    # level 4 (>= 22 zeros) and not (< 22), the latter with >= 15 (level 3).
+    assert min(exp) >= 15 and min(exp) < 22 <= max(exp)
     same = [[5] * 9, [3] * 10, [48] * 12]
-    saved = _force_levels(ctx, {9: 4, 10: 5})
+    saved = _force_levels(ctx, table)
     try:
         assert dist.max_zeros_batch(bsets, 1) == exp
         assert dist.max_zeros_batch(same, 1) == [49, 49, 49]
         mz = np.array(exp)
         assert dist.max_zeros_batch(bsets, (49 - mz).astype(np.int32)) == exp   # exact
         assert dist.max_zeros_batch(bsets, (50 - mz).astype(np.int32)) == exp   # clamp tmax + 1
+        # Mixed with the small codes of the batch (one call, every pass).
+        assert dist.max_zeros_batch(sets[:64] + bsets, 1)[64:] == exp
     finally:
         _restore_levels(ctx, saved)
     assert dist.max_zeros_batch(same, 1) == [49, 49, 49]
+
+
+def _level_walk(k, m):
+    """Projective codewords of a level-m pass over a k-point code."""
+    return (3 if m in (3, 4) else 1) * ((8 ** (k - m) - 1) // 7)
+
+
+def test_walk_record(dist, md_gold):
+    """gt_set_md_walk_record: per code, the codewords its passes walked.  At
+    the default levels every config-2 code certifies at its first level (the
+    record is that level's walk); each k = 9, 10 fixture code walks level 4
+    and then at most one of level 3 (middle pass) and level 2 (final pass),
+    and some take each route.  Recording changes no result; with it off the
+    getter fails."""
+    from gtoric._native import GtError
+    from gtoric.distance import flatten
+    ctx = dist.context
+    sets = _rows(md_gold["c2_flat"], md_gold["c2_offsets"])
+    g = np.load(GOLD / "md_wide_golden.npz")
+    wsets = _rows(g["flat"], g["offsets"])
+    big = [int(i) for i in g["pos"] if len(wsets[int(i)]) >= 9]
+    bsets = [wsets[i] for i in big]
+    bexp = [int(z) for i, z in zip(g["pos"], g["mz"]) if len(wsets[int(i)]) >= 9]
+    ctx.set_md_walk_record(True)
+    try:
+        got = ctx.md_batch(*flatten(sets), 48)
+        assert got.tolist() == md_gold["c2_mz"].tolist()
+        rec = ctx.md_walk_record(len(sets))
+        assert rec.tolist() == [_level_walk(len(s), ctx.get_md_level(len(s))) for s in sets]
+        mixed = sets[:32] + bsets
+        got = ctx.md_batch(*flatten(mixed), 48)
+        assert got[32:].tolist() == bexp
+        rec = ctx.md_walk_record(len(mixed))[32:].tolist()
+        routes = []
+        for s, r in zip(bsets, rec):
+            k = len(s)
+            first = _level_walk(k, 4)
+            routes.append({first: 0, first + _level_walk(k, 3): 1,
+                           first + _level_walk(k, 2): 2}[r])
+        assert 0 in routes and 1 in routes, routes
+    finally:
+        ctx.set_md_walk_record(False)
+    ctx.md_batch(*flatten(sets[:4]), 48)
+    with pytest.raises(GtError):
+        ctx.md_walk_record(4)
 
 
 def test_dev2_kmax_bound(dist):
@@ -279,13 +334,13 @@ def test_levels_do_not_change_results(dist, md_gold):
     """First-pass levels and the one-wave bound change the work, never the
     result.  With the highest level per k (k - 1 up to 6) most codes fall
     below their level's certificate, or have no direction of full rank, and
-    are searched again at a lower level (in the same wave) or at level 1
-    (the planned walk's second pass): every exact and thresholded result
+    are searched again at a lower level (in the same wave, or by the planned
+    walk's middle and final passes): every exact and thresholded result
     still equals the oracle fixture (config 2, and the per-set thresholded
     batch), with codes of k <= 8 searched one wave per code and with every
     code through the planned walk."""
     ctx = dist.context
-    assert [ctx.get_md_level(k) for k in (2, 3, 4, 6, 7, 8, 12, 21)] == [1, 1, 2, 2, 3, 3, 3, 3]
+    assert [ctx.get_md_level(k) for k in (2, 3, 4, 6, 7, 8, 9, 12, 21)] == [1, 1, 2, 2, 3, 3, 4, 4, 4]
     sets = _rows(md_gold["c2_flat"], md_gold["c2_offsets"])
     exp = md_gold["c2_mz"].tolist()
     tsets = _rows(md_gold["thr_flat"], md_gold["thr_offsets"])
@@ -318,7 +373,9 @@ def test_level_settings_are_checked(dist):
         ctx.set_md_level(9, 7)      # above the largest level
     with pytest.raises(GtError):
         ctx.set_md_level(9, 5)      # k - level would drop from 8 - 3 to 9 - 5
-    assert ctx.get_md_level(9) == 3
+    with pytest.raises(GtError):
+        ctx.set_md_level(10, 6)     # k - level would drop from 9 - 4 to 10 - 6
+    assert ctx.get_md_level(9) == 4 and ctx.get_md_level(10) == 4
 
 
 @pytest.mark.parametrize("k", [3, 4])
