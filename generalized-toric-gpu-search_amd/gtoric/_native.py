@@ -118,6 +118,8 @@ def exported_symbols() -> list[str]:
 def lib():
     """Load libgtoric.so once (raises if it was not built)."""
     global _lib
+    if _lib is not None:   # (the common case: no lock)
+        return _lib
     with _lock:
         if _lib is None:
             if not LIB_PATH.exists():
@@ -213,17 +215,19 @@ class Context:
                      kmax: int | None = None):
         """gt_md_batch_dev, or gt_md_batch_dev2 when the caller bounds the set
         sizes by kmax (a larger set is then an input error)."""
-        seed = ctypes.c_uint64(int(sample_seed) & 0xFFFF_FFFF_FFFF_FFFF)
-        st = vp(stream) if stream else None
-        dt = vp(d_tmax) if d_tmax else None
+        # Plain ints and None: the declared argument types convert them (this
+        # is a per-call path; wrapper objects would cost about a microsecond).
+        seed = int(sample_seed) & 0xFFFF_FFFF_FFFF_FFFF
+        dt, st = int(d_tmax) if d_tmax else None, int(stream) if stream else None
         if kmax is None:
-            check(lib().gt_md_batch_dev(self.handle, vp(d_idx), int(n_idx), vp(d_offsets), int(n),
-                                        dt, int(tmax), int(bool(thresholded)), int(sample_count),
-                                        seed, vp(d_out), st))
+            rc = lib().gt_md_batch_dev(self.handle, int(d_idx), n_idx, int(d_offsets), n, dt, tmax,
+                                       1 if thresholded else 0, sample_count, seed, int(d_out), st)
         else:
-            check(lib().gt_md_batch_dev2(self.handle, vp(d_idx), int(n_idx), vp(d_offsets),
-                                         int(n), dt, int(tmax), int(bool(thresholded)),
-                                         int(kmax), int(sample_count), seed, vp(d_out), st))
+            rc = lib().gt_md_batch_dev2(self.handle, int(d_idx), n_idx, int(d_offsets), n, dt,
+                                        tmax, 1 if thresholded else 0, kmax, sample_count, seed,
+                                        int(d_out), st)
+        if rc:
+            check(rc)
 
     def set_md_fused_max_k(self, k: int) -> None:
         check(lib().gt_set_md_fused_max_k(self.handle, int(k)))
