@@ -575,7 +575,9 @@ def main() -> None:
     # 200 untimed steps (about 0.1 s) first: the device's clocks take tens of
     # milliseconds to settle after it goes busy, and a few steps would time
     # part of that ramp.
-    ap.add_argument("--steps", type=int, default=200)
+    # 1000 steps: the three-stream region then lasts about 10 ms (200 steps,
+    # 2 ms, read 10-30 % low: the first steps fill the streams).
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=200)
     # Contexts (streams) the headline steps are issued on, round-robin: three
     # measured best on MI355X (one step's rows, plan and walk start overlap
