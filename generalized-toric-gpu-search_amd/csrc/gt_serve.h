@@ -12,9 +12,40 @@ namespace gt {
 constexpr int kServeMaxK = kMaxKExact;
 // Bit rows of one request's walk in a worker's LDS: a request of k <= 8
 // points is searched whole by one wave (at most kFusedRows = 45 rows); a
-// larger one is walked chunk by chunk at level 2, 3 (k - 2) rows.
-constexpr int kServeRowWords = 3 * (kServeMaxK - 2) * kRowWords;
-static_assert(3 * (kServeMaxK - 2) >= 45, "the rows of a whole small request fit");
+// larger one is walked chunk by chunk in up to three phases (serve_phases),
+// at most the 3 x 3 (k - 3) rows of level 3's three subcodes.
+constexpr int kServeRowWords = 3 * 3 * (kServeMaxK - 3) * kRowWords;
+static_assert(3 * 3 * (kServeMaxK - 3) >= 45, "the rows of a whole small request fit");
+
+// Phases of a request of k > kFusedMaxK points: its first level m[0], the
+// level below it m[1] (0: none, mid_level), and level 2 (second_level),
+// with their chunks at walk length L (c[1] = 0 without a middle level, and
+// only the first phase when the first level is already level 2).  A later
+// phase's chunks are walked only when the earlier phases did not certify
+// the request (serve_kernel.hip).
+// A subcode of walk dimension at most kServeWholeW (at most 37,449
+// codewords) is one chunk, walked whole; larger ones are chunked as in the
+// batch walk (a chunk also pays a claim, a lookup, the rows and a fold).
+constexpr int kServeWholeW = 6;
+__host__ __device__ __forceinline__ uint32_t serve_sub_chunks(int w, int L) {
+  return (w <= kServeWholeW) ? 1u : (uint32_t)code_chunks(w, L);
+}
+__host__ __device__ __forceinline__ void serve_phases(const uint8_t* level, int k, int L,
+                                                      int m[3], uint32_t c[3]) {
+  m[0] = (int)level[k];
+  m[1] = mid_level(level, k);
+  m[2] = second_level(k);
+  c[0] = (uint32_t)level_nsub(m[0]) * serve_sub_chunks(k - m[0], L);
+  c[1] = (m[1] > 0) ? (uint32_t)level_nsub(m[1]) * serve_sub_chunks(k - m[1], L) : 0u;
+  c[2] = (m[0] > m[2]) ? serve_sub_chunks(k - m[2], L) : 0u;
+}
+// Bits of a request's phase flags.
+enum : uint32_t {
+  kPhNoRank0 = 1u,    // the first level had no direction of full rank
+  kPhNoRank1 = 2u,    // the middle level had none
+  kPhMidOff = 4u,     // the middle phase was not certified beforehand: skipped
+  kPhTimeout = 8u,    // a phase wait ran out: the last phase is walked
+};
 
 // One request descriptor (fine-grained host memory, written by the host):
 // 48 bytes, read by the dispatcher with three 16-byte loads.
@@ -63,7 +94,7 @@ enum ServeStat : int {
   kStDispRequests,     // requests dispatched
   kStDispTicks,        // dispatcher time inside those passes
   kStChunks,           // chunks walked
-  kStSkipped,          // chunks skipped (their request had already stopped)
+  kStSkipped,          // chunks skipped (their request had stopped, or was certified)
   kStWaitTicks,        // worker time waiting for a chunk to be published
   kStLookupTicks,      // worker time finding the request, reading its state, staging rows
   kStWalkTicks,        // worker time in the Gray walk
@@ -86,7 +117,7 @@ struct ServeDev {
   uint64_t first_req;   // first request index this launch dispatches (set by the init kernel)
   uint8_t L[kServeMaxK + 1];
   uint8_t check_log2[kServeMaxK + 1];
-  uint8_t level[kServeMaxK + 1];   // first-pass level of a request searched whole
+  uint8_t level[kServeMaxK + 1];   // first level of a request (its search or first phase)
   int32_t* s_k;         // per slot: k | (L << 8), 0 when the request has no chunks
   int32_t* s_tmax;
   uint64_t* s_state;    // per slot: bits 0..31 minimum weight, 32..63 chunks finished
@@ -94,6 +125,8 @@ struct ServeDev {
   uint64_t* s_seq;
   uint64_t* ord_start;  // 4 * slots tagged entries (ord_entry), request r at r % (4 * slots)
   uint4* s_idx;         // per slot: the request's exponent indices, 6 bits each (unpack_idx6)
+  uint64_t* s_ph;       // per slot: chunks of the first phase | chunks of the middle phase << 32
+  uint32_t* s_flags;    // per slot: phase flags (kPh*)
   uint64_t stats[kServeStats];
 };
 

@@ -47,14 +47,16 @@ double now_s() {
 
 // Projective codewords the search of a k-point request walks: its first
 // level's subcodes (k <= 8, searched whole by one wave; more only when a
-// lower level follows), or the level-2 subcode (larger k).
+// lower level follows), or, for larger k, its first phase and its middle
+// phase (serve_phases; the level-2 phase is walked only when neither
+// certifies the request).
 uint64_t walk_cw(int k) {
   if (k <= 1) return 0;
-  if (k <= kFusedMaxK) {
-    const int m = (k <= 2) ? 1 : default_level(k);
-    return (uint64_t)level_nsub(m) * proj_codewords(k - m);
-  }
-  return proj_codewords(k - second_level(k));
+  const int m = (k <= 2) ? 1 : default_level(k);
+  uint64_t w = (uint64_t)level_nsub(m) * proj_codewords(k - m);
+  if (k > kFusedMaxK && m - 1 > second_level(k))
+    w += (uint64_t)level_nsub(m - 1) * proj_codewords(k - m + 1);
+  return w;
 }
 
 }  // namespace
@@ -282,7 +284,8 @@ int gt_serve_start(int32_t device, int32_t slots, int64_t work_budget, const int
       sv->dalloc(&dh.s_state, slots) ||
       sv->dalloc(&dh.s_nch, slots) || sv->dalloc(&dh.s_seq, slots) ||
       sv->dalloc(&dh.ord_start, (size_t)4 * slots) ||
-      sv->dalloc(&dh.s_idx, (size_t)slots) || sv->dalloc(&sv->d, 1) ||
+      sv->dalloc(&dh.s_idx, (size_t)slots) || sv->dalloc(&dh.s_ph, (size_t)slots) ||
+      sv->dalloc(&dh.s_flags, (size_t)slots) || sv->dalloc(&sv->d, 1) ||
       sv->dalloc(&sv->mplanes, (size_t)kNpts * kRowWords))
     return bail(-1);
   // Evaluation-matrix bit planes (same construction as the batch path).

@@ -12,13 +12,24 @@
 // A request of k <= 8 points is one chunk: the wave runs the request's
 // whole search (the anchored subcodes of its first level, rows in the
 // wave's LDS, lower levels when the first does not certify: fused_search,
-// gt_anchor.h).  A larger request is chunked at level 2 (certified for
-// k >= 9 by max_zeros >= k - 1; gt_common.h): the wave builds the rows of
-// the subcode vanishing at two points of a line in its LDS area and walks
-// the chunk (the Gray walk of the batch kernel).  Building the rows in the
-// workers keeps the dispatcher, one wave for the whole device, to a few
-// stores per request.  The wave that finishes the last chunk of a request
-// writes its result and sequence number to host memory.
+// gt_anchor.h).  A larger request's chunks cover up to three phases
+// (serve_phases, gt_serve.h): its first level (4 by default), the level
+// below it, and level 2 (certified for k >= 9 by max_zeros >= k - 1;
+// gt_common.h).  A wave builds the rows of its chunk's level in its LDS area
+// and walks the chunk (the Gray walk of the batch kernel).  A chunk of a
+// later phase first waits until every chunk of the earlier phases is
+// counted (those chunks have lower ids, so running workers already hold
+// them: the wait ends; it is also bounded), then is skipped when the
+// request's minimum so far certifies it: a first-level maximum of 7m - 7 or
+// more zeros (gt_common.h; codewords of later phases in the minimum only
+// raise it, and z* <= max(first-level maximum, 7(m - 1)) still holds), or,
+// before the last phase, a middle level that ran whole with full rank and
+// reached its own 7m' - 7.  The middle phase runs only when max(z, k - 1)
+// certifies its level beforehand or z meets its found certificate; the last
+// phase is exact.  Building the rows in the workers keeps the dispatcher,
+// one wave for the whole device, to a few stores per request.  The wave
+// that finishes the last chunk of a request writes its result and sequence
+// number to host memory.
 //
 // Requests that arrive while others run are admitted as soon as a slot and
 // the host-side work budget allow; their chunks queue behind the chunks
@@ -103,12 +114,17 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, uint32_t lane) {
       const int Lq = (int)((v0.x >> 8) & 0xFFu);
       const int32_t q_tmax = (int32_t)v0.y;
       // k <= 8: one chunk, the whole search in one wave; larger k: the
-      // chunks of the level-2 walk.
+      // chunks of its phases.
       int L = 0;
-      if (k > kFusedMaxK && k <= kServeMaxK) L = (Lq != 0) ? Lq : (int)d->L[k];
+      uint32_t pc[3] = {0u, 0u, 0u};
+      if (k > kFusedMaxK && k <= kServeMaxK) {
+        L = (Lq != 0) ? Lq : (int)d->L[k];
+        int pm[3];
+        serve_phases(d->level, k, L, pm, pc);
+      }
       nch = (k < 2 || k > kServeMaxK) ? 0u
             : (k <= kFusedMaxK)       ? 1u
-                                      : (uint32_t)code_chunks(k - second_level(k), L);
+                                      : pc[0] + pc[1] + pc[2];
       const uint32_t ib[8] = {v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, v2.z, v2.w};
       uint64_t lo = 0, hi = 0;
 #pragma unroll
@@ -131,6 +147,8 @@ __device__ void dispatcher(ServeDev* d, ServeHost* h, uint32_t lane) {
       d->s_tmax[slot] = q_tmax;
       d->s_state[slot] = 64ull;
       d->s_nch[slot] = nch;
+      d->s_ph[slot] = (uint64_t)pc[0] | ((uint64_t)pc[1] << 32);
+      d->s_flags[slot] = 0u;
       d->s_seq[slot] = r + 1;
       // k <= 1 needs no walk: k = 0 gives 0, k = 1 (a monomial, weight 49)
       // gives 0 or, below tmax = -1, the clamp tmax + 1; unsupported k: -1.
@@ -185,6 +203,7 @@ __device__ void worker(ServeDev* d, ServeHost* h, const uint32_t* __restrict__ m
   uint64_t av_known = 0;
   uint32_t staged_slot = 0xFFFFFFFFu;
   uint64_t staged_seq = 0;
+  int staged_m = 0;
   uint64_t st_chunks = 0, st_skipped = 0, st_staged = 0;
   uint64_t st_wait = 0, st_lookup = 0, st_walk = 0, st_fold = 0;
   uint64_t first = 0;
@@ -249,6 +268,7 @@ __device__ void worker(ServeDev* d, ServeHost* h, const uint32_t* __restrict__ m
     const bool skip = (int)__builtin_amdgcn_readfirstlane(ld_rlx(best_g)) <= thr;
     uint32_t b = 64u;
     uint64_t t2 = t1;
+    bool walked_chunk = !skip;   // (a later phase's chunk may be skipped below)
     if (!skip) {
       // Lane i < k: the request's point i.
       const uint4 iv = d->s_idx[slot];
@@ -261,27 +281,83 @@ __device__ void worker(ServeDev* d, ServeHost* h, const uint32_t* __restrict__ m
         staged_slot = 0xFFFFFFFFu;
         ++st_staged;
       } else {
-        if (slot != staged_slot || seq != staged_seq) {
-          // Level-2 rows; without a direction of full rank every index is
-          // the same point, the request spans the zero codeword, and zero
-          // rows give its 49 zeros.
-          wave_lds_sync();
-          if (!md_rows_level<2>(gi, k, lane, mplanes, lrows))
-            for (int r = (int)lane; r < 3 * (k - 2); r += kWave)
-#pragma unroll
-              for (int t = 0; t < 6; ++t) lrows[r * kRowWords + t] = 0u;
-          wave_lds_sync();
-          staged_slot = slot;
-          staged_seq = seq;
-          ++st_staged;
+        // The chunk's phase and level.
+        const uint64_t ph = uniform64(d->s_ph[slot]);
+        const uint32_t c0 = (uint32_t)ph, c1 = (uint32_t)(ph >> 32);
+        const int phase = (local < c0) ? 0 : (local < (uint64_t)c0 + c1) ? 1 : 2;
+        const uint64_t lp = local - ((phase >= 1) ? c0 : 0u) - ((phase == 2) ? c1 : 0u);
+        const int m0 = (int)d->level[k], m1 = mid_level(d->level, k);
+        const int m = (phase == 0) ? m0 : (phase == 1) ? m1 : second_level(k);
+        uint32_t* flags = &d->s_flags[slot];
+        bool walk = true;
+        if (phase > 0) {
+          // Wait until the earlier phases' chunks are all counted.
+          const uint32_t need = (phase == 1) ? c0 : c0 + c1;
+          const uint32_t* cnt = reinterpret_cast<const uint32_t*>(state) + 1;
+          bool ready = false;
+          for (uint32_t it = 0; it < (1u << 22); ++it) {
+            if (__builtin_amdgcn_readfirstlane(ld_rlx(cnt)) >= need) { ready = true; break; }
+            __builtin_amdgcn_s_sleep(2);
+          }
+          uint32_t f = __builtin_amdgcn_readfirstlane(ld_rlx(flags));
+          if (!ready) {
+            if (lane == 0) atomicOr(flags, kPhTimeout);
+            f |= kPhTimeout;
+          }
+          const int z = kNpts - (int)__builtin_amdgcn_readfirstlane(ld_rlx(best_g));
+          const bool first_ok = !(f & kPhNoRank0) && z >= found_cert(m0);
+          if (phase == 1) {
+            if (ready && (z > tmax || first_ok)) {
+              walk = false;   // certified
+            } else if (!(level_cert(m1) <= k - 1 || (!(f & kPhNoRank0) && found_cert(m1) <= z))) {
+              if (lane == 0) atomicOr(flags, kPhMidOff);   // the middle level would not certify
+              walk = false;
+            }
+          } else {
+            const bool mid_ok = c1 > 0u && !(f & (kPhNoRank1 | kPhMidOff)) && z >= found_cert(m1);
+            if (!(f & kPhTimeout) && (z > tmax || first_ok || mid_ok)) walk = false;
+          }
+          walked_chunk = walk;
         }
-        t2 = wall_clock64();
-        const uint32_t cmask = (1u << d->check_log2[k]) - 1u;
-        b = run_chunk<true>(LdsRows{lrows}, k - second_level(k), L, local, lane, thr, cmask,
-                            best_g);
-        b = wave_min(b);
+        if (walk) {
+          const int w = k - m;
+          if (slot != staged_slot || seq != staged_seq || m != staged_m) {
+            // Rows of the phase's level.  Without a direction of full rank:
+            // at level 2 every index is the same point, the request spans
+            // the zero codeword, and zero rows give its 49 zeros; at a
+            // higher level the phase is flagged and its chunks walk nothing.
+            wave_lds_sync();
+            const bool ok = md_rows_switch(m, gi, k, lane, mplanes, lrows);
+            if (!ok && phase == 2)
+              for (int r = (int)lane; r < 3 * w; r += kWave)
+#pragma unroll
+                for (int t = 0; t < 6; ++t) lrows[r * kRowWords + t] = 0u;
+            if (!ok && phase < 2 && lane == 0)
+              atomicOr(flags, (phase == 0) ? kPhNoRank0 : kPhNoRank1);
+            wave_lds_sync();
+            staged_slot = slot;
+            staged_seq = seq;
+            staged_m = (ok || phase == 2) ? m : -1;   // (a failed level is built again)
+            ++st_staged;
+          }
+          if (staged_m == m) {
+            t2 = wall_clock64();
+            // Subcode s of the phase's level, chunk lp of it (or all of it).
+            const uint64_t cc = serve_sub_chunks(w, L);
+            const uint32_t sub = (uint32_t)(lp / cc);
+            const LdsRows rows{lrows + (size_t)sub * 3 * w * kRowWords};
+            if (w <= kServeWholeW) {
+              b = walk_all(rows, w, lane);
+            } else {
+              const uint32_t cmask = (1u << d->check_log2[k]) - 1u;
+              b = run_chunk<true>(rows, w, L, lp - (uint64_t)sub * cc, lane, thr, cmask, best_g);
+            }
+            b = wave_min(b);
+          }
+        }
       }
-      ++st_chunks;
+      if (walked_chunk) ++st_chunks;
+      else ++st_skipped;
     } else {
       t2 = wall_clock64();
       ++st_skipped;

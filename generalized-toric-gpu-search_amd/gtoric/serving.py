@@ -24,16 +24,25 @@ MAX_K = 21
 FUSED_MAX_K = 8      # requests of at most 8 points: one chunk, searched whole by one wave
 
 
+SERVE_WHOLE_W = 6    # a phase's subcode of at most this dimension is one chunk
+
+
 def served_chunks(k: int, L: int) -> int:
-    """Chunks the server walks for a k-point request at walk length L: none
+    """Chunks the server counts for a k-point request at walk length L: none
     for k <= 1, one for k <= 8 (the whole search in one wave), else the
-    chunks of the level-2 walk of dimension k - 2 (csrc/serve_kernel.hip)."""
+    chunks of its three phases, walked or skipped (csrc/gt_serve.h,
+    serve_phases): the default first level 4 (three subcodes of dimension
+    k - 4), level 3 (three of k - 3) and level 2 (one of k - 2), a subcode of
+    dimension <= SERVE_WHOLE_W one chunk."""
     from .autotune import code_chunks
     if k <= 1:
         return 0
     if k <= FUSED_MAX_K:
         return 1
-    return code_chunks(k - 2, L)
+
+    def sub(w):
+        return 1 if w <= SERVE_WHOLE_W else code_chunks(w, L)
+    return 3 * sub(k - 4) + 3 * sub(k - 3) + sub(k - 2)
 
 
 # Served walk length per k (chunks of 64 x 2^L codewords).  A served chunk also

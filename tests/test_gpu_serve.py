@@ -182,6 +182,37 @@ def test_large_k_requests_match_the_batch_path(ctx):
     assert res.tolist() == exp
 
 
+def test_phases_of_large_requests_against_fixture(ctx):
+    """Requests of 9 and 10 points walk level 4, then level 3 when level 4
+    does not certify them, then level 2 when neither does (chunks of a later
+    phase wait for the earlier ones and are skipped once the request is
+    certified).  The k = 9, 10 codes of the k = 3..10 fixture (19 to 28 zeros:
+    every route) and sets whose indices are all one point (no direction of
+    full rank at levels 4 and 3: the level-2 phase, 49 zeros), exact and
+    thresholded, mixed with small requests, equal the oracle's values."""
+    from pathlib import Path
+    from gtoric.serving import Server
+    g = np.load(Path(__file__).resolve().parent / "golden" / "md_wide_golden.npz")
+    flat, offs = g["flat"], g["offsets"]
+    allsets = [flat[offs[i]:offs[i + 1]].tolist() for i in range(len(offs) - 1)]
+    big = [(allsets[int(i)], int(z)) for i, z in zip(g["pos"], g["mz"]) if len(allsets[int(i)]) >= 9]
+    small = [(allsets[int(i)], int(z)) for i, z in zip(g["pos"][:200], g["mz"][:200])]
+    same = [([5] * 9, 49), ([3] * 10, 49), ([48] * 12, 49)]
+    items = big + small + same
+    sets = [S for S, _ in items]
+    mz = np.array([z for _, z in items], dtype=np.int32)
+    with Server(0, slots=256, life_s=60) as srv:
+        exact, _, _ = srv.run_stream(sets, np.full(len(sets), 48, np.int32), np.zeros(len(sets)),
+                                     timeout_s=90)
+        # tmax = mz - 1: the walk stops at mz zeros; the clamp tmax + 1 is mz.
+        clamp, _, _ = srv.run_stream(sets, (mz - 1).astype(np.int32), np.zeros(len(sets)),
+                                     timeout_s=90)
+        st = srv.stats()
+    assert exact.tolist() == mz.tolist()
+    assert clamp.tolist() == mz.tolist()
+    assert st["chunks_skipped"] > 0
+
+
 def test_requests_outlive_a_kernel_that_stops_itself(ctx):
     """A kernel that reaches its lifetime limit leaves requests published
     after its last dispatcher pass; the next launch takes them, so every
