@@ -257,7 +257,7 @@ inline void md_level_set(MdParams& p, int k, int m) {
 // lane-major: lane l owns the Gray indices l * 2^(3j-6) + [0, 2^(3j-6)), and
 // the piece is 2^(3j-10) blocks, one unit each; a wave may walk any run of
 // them.  The tiny part (pieces 0 .. tiny_top(w): 585 codewords for
-// w > kTinyTop, about 300 VALU instructions in one wave) counts as
+// w > kTinyTop, about 160 VALU instructions in one wave) counts as
 // kTinyUnits units, which also prices the start of a code (two dependent
 // loads before its first row) and its fold: measured on a 4096-code mix,
 // weights 6 to 8 balance best, 2 leaves the waves that hold many k = 6 or 7

@@ -178,38 +178,45 @@ __device__ __forceinline__ uint32_t gray_blocks(const Rows& row, uint32_t j, uin
   return best;
 }
 
-// Pieces 0..jt (jt <= kTinyTop = 3): pieces j <= 2 one codeword per lane
-// (lane a < 8^j: h_j plus rows 0 .. 3j-1 by the bits of a); piece 3, 512
-// codewords, lane-major as in gray_blocks with P = 3: lane l starts at
-// gray(8 l) and takes the 7 Gray steps over rows 0..2.
+// Pieces 0..jt (jt <= kTinyTop = 3): piece j is row 3j plus the rows below
+// it by its index bits.  One partial sum per lane serves every piece: P3 =
+// rows 0..2 by the lane's bits 0..2, P = P3 + rows 3..5 by bits 3..5.  Piece
+// 0 is row 0; piece 1 (lanes < 8) row 3 + P3; piece 2 row 6 + P; piece 3
+// (512 codewords) row 9 + P, then 7 Gray steps over rows 6..8.  (Six masked
+// row additions in all, against one set per piece: the tiny part of a
+// dimension-4 subcode costs about 160 VALU instead of 250.)
 template <class Rows>
 __device__ __forceinline__ uint32_t tiny_chunk(const Rows& row, int jt, uint32_t lane) {
-  uint32_t best = 64;
+  uint32_t best = weight(row(0));
+  if (jt == 0) return best;
+  RowV p = {{0u, 0u, 0u, 0u, 0u, 0u}};
 #pragma unroll
-  for (int j = 0; j <= 2; ++j) {
-    if (j > jt) break;
-    RowV v = row(3 * j);
-#pragma unroll
-    for (int r = 0; r < 3 * j; ++r) xor_row_masked(v, row(r), 0u - ((lane >> r) & 1u));
+  for (int r = 0; r < 3; ++r) xor_row_masked(p, row(r), 0u - ((lane >> r) & 1u));
+  {
+    RowV v = row(3);
+    xor_row(v, p);
     const uint32_t wt = weight(v);
-    if (lane < (1u << (3 * j))) best = min(best, wt);
+    if (lane < 8u) best = min(best, wt);
   }
-  if (jt >= 3) {
-    RowV v = row(9);
-    const uint32_t lp = lane ^ (lane >> 1);
+  if (jt == 1) return best;
 #pragma unroll
-    for (int t = 0; t < kLaneBits; ++t) xor_row_masked(v, row(3 + t), 0u - ((lp >> t) & 1u));
-    xor_row_masked(v, row(2), 0u - (lane & 1u));
-    const RowV r0 = row(0), r1 = row(1), r2 = row(2);
-    best = min(best, weight(v));
-    xor_row(v, r0); best = min(best, weight(v));
-    xor_row(v, r1); best = min(best, weight(v));
-    xor_row(v, r0); best = min(best, weight(v));
-    xor_row(v, r2); best = min(best, weight(v));
-    xor_row(v, r0); best = min(best, weight(v));
-    xor_row(v, r1); best = min(best, weight(v));
-    xor_row(v, r0); best = min(best, weight(v));
-  }
+  for (int r = 3; r < 6; ++r) xor_row_masked(p, row(r), 0u - ((lane >> r) & 1u));
+  const RowV r6 = row(6);
+  RowV v = r6;
+  xor_row(v, p);
+  best = min(best, weight(v));
+  if (jt == 2) return best;
+  const RowV r7 = row(7), r8 = row(8);
+  v = row(9);
+  xor_row(v, p);
+  best = min(best, weight(v));
+  xor_row(v, r6); best = min(best, weight(v));
+  xor_row(v, r7); best = min(best, weight(v));
+  xor_row(v, r6); best = min(best, weight(v));
+  xor_row(v, r8); best = min(best, weight(v));
+  xor_row(v, r6); best = min(best, weight(v));
+  xor_row(v, r7); best = min(best, weight(v));
+  xor_row(v, r6); best = min(best, weight(v));
   return best;
 }
 static_assert(kTinyTop == 3, "tiny_chunk enumerates pieces 0..3");
