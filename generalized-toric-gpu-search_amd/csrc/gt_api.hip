@@ -729,6 +729,8 @@ int gt_md_batch_dev_noalloc_limits(int32_t* max_codes, int64_t* max_idx) {
   return 0;
 }
 
+constexpr int kSingleBatchK = 12;   // one-code calls of at least this k use the batch path
+
 int gt_md_single(gt_ctx* c, const int32_t* idx, int32_t k, int32_t tmax, int32_t* out) {
   if (!c) return fail("null context");
   if (!out) return fail("null out");
@@ -741,6 +743,14 @@ int gt_md_single(gt_ctx* c, const int32_t* idx, int32_t k, int32_t tmax, int32_t
   if (w == 0) {   // k = 1: a monomial has weight 49
     *out = (49 <= thr) ? tmax + 1 : 0;
     return 0;
+  }
+  // From k = kSingleBatchK on, the level-1 walk of one launch (8^(k-1) / 7
+  // codewords: 0.33 ms at k = 12, 21 ms at k = 14) costs more than the batch
+  // path's passes over anchored subcodes (level 4, then 3, then 2), launches
+  // included: the code goes through gt_md_batch as a batch of one.
+  if (k >= kSingleBatchK) {
+    const int32_t offsets[2] = {0, k};
+    return gt_md_batch(c, idx, offsets, 1, nullptr, tmax, 0, 0, out);
   }
   std::lock_guard<std::mutex> lk(c->mu);
   GT_OK(c->activate());

@@ -150,9 +150,11 @@ int gt_md_batch_dev_noalloc(gt_ctx* ctx, const int32_t* d_idx, int64_t n_idx,
                             int32_t tmax, int32_t thresholded, int32_t* d_out, void* stream);
 int gt_md_batch_dev_noalloc_limits(int32_t* max_codes, int64_t* max_idx);
 
-/* One code, one kernel launch (replaces DistanceOracle.max_zeros,
- * kernel.py:241-274, and DistanceOracleCUDABP.max_zeros,
- * kernel_cuda_bp.py:266-288).  k = 0 gives 0; k > 21 is an error. */
+/* One code (replaces DistanceOracle.max_zeros, kernel.py:241-274, and
+ * DistanceOracleCUDABP.max_zeros, kernel_cuda_bp.py:266-288): one kernel
+ * launch over the subcode through (1,1) for k <= 11; from k = 12 on the
+ * code is searched as a batch of one (gt_md_batch: anchored subcodes, a few
+ * launches, far fewer codewords).  k = 0 gives 0; k > 21 is an error. */
 int gt_md_single(gt_ctx* ctx, const int32_t* idx, int32_t k, int32_t tmax, int32_t* out);
 
 /* Canonical form under AGL2(F7) of n sets of size k given as sorted index rows

@@ -291,6 +291,27 @@ def test_single_launch_matches_batch(dist):
             assert dist.max_zeros(S, td) == oracle.max_zeros(S, 49 - td)
 
 
+def test_single_calls_of_large_codes(dist):
+    """One-code calls of k >= 12 go through the batch path (anchored
+    subcodes, level 4 first); k = 11 stays one launch over the subcode
+    through (1,1).  Both equal the batch path with every level forced to 1
+    (the whole subcode through (1,1), no anchoring), exact and
+    thresholded."""
+    from gtoric.distance import flatten
+    ctx = dist.context
+    rng = np.random.default_rng(103)
+    sets = [sorted(rng.choice(49, k, replace=False).tolist()) for k in (11, 12, 12, 13)]
+    saved = _force_levels(ctx, {k: 1 for k in range(3, 22)})
+    try:
+        exp = ctx.md_batch(*flatten(sets), 48).tolist()
+    finally:
+        _restore_levels(ctx, saved)
+    assert [dist.max_zeros(S, 1) for S in sets] == exp
+    for S, z in zip(sets, exp):
+        assert dist.max_zeros(S, 49 - z) == z          # tmax = z: the exact value
+        assert dist.max_zeros(S, 50 - z) == z          # tmax = z - 1: the clamp z
+
+
 def test_dependent_generators(dist):
     # Repeated rows give the zero codeword: 49 zeros (reference enumerates it too).
     assert dist.max_zeros_batch([[3, 3], [0, 5, 5, 9]], 1) == [49, 49]
