@@ -24,11 +24,17 @@ constexpr uint32_t kF8Inv = 0u | 1u << 3 | 5u << 6 | 6u << 9 | 7u << 12 | 2u << 
                             4u << 21;
 
 // Product in GF(8) = GF(2)[x]/(x^3 + x + 1) of 3-bit polynomials.
-__device__ __forceinline__ uint32_t f8_mul(uint32_t a, uint32_t b) {
+GT_HD constexpr uint32_t f8_mul(uint32_t a, uint32_t b) {
   uint32_t p = ((b & 1u) ? a : 0u) ^ ((b & 2u) ? a << 1 : 0u) ^ ((b & 4u) ? a << 2 : 0u);
   if (p & 16u) p ^= 0x16u;   // x^4 = x^2 + x
   if (p & 8u) p ^= 0x0Bu;    // x^3 = x + 1
   return p;
+}
+// The products v * b for v = 0..7, three bits each (bits 3v .. 3v + 2).
+GT_HD constexpr uint32_t f8_mul_row(uint32_t b) {
+  uint32_t t = 0u;
+  for (uint32_t v = 1; v < 8; ++v) t |= f8_mul(v, b) << (3 * v);
+  return t;
 }
 
 // Evaluation-matrix column (point index) of the exponent vector r d on the
@@ -42,12 +48,18 @@ __device__ __forceinline__ int line_point(int d, int r) {
   return (x - 1) * 7 + (y - 1);
 }
 
-// Value of the codeword h (bit planes) at point p (wave-uniform).
+// Value of the codeword h (bit planes) at point p (wave-uniform: a uniform
+// branch picks the words, three bit-field extracts read the planes).
 __device__ __forceinline__ uint32_t cw_at(const uint32_t h[6], int p) {
-  const bool hi = p >= 32;
-  const int bit = p & 31;
-  return (((hi ? h[1] : h[0]) >> bit) & 1u) | ((((hi ? h[3] : h[2]) >> bit) & 1u) << 1) |
-         ((((hi ? h[5] : h[4]) >> bit) & 1u) << 2);
+  const uint32_t bit = (uint32_t)p & 31u;
+  uint32_t a, b, c;
+  if (p >= 32) {
+    a = h[1]; b = h[3]; c = h[5];
+  } else {
+    a = h[0]; b = h[2]; c = h[4];
+  }
+  return __builtin_amdgcn_ubfe(a, bit, 1u) | (__builtin_amdgcn_ubfe(b, bit, 1u) << 1) |
+         (__builtin_amdgcn_ubfe(c, bit, 1u) << 2);
 }
 
 // One elimination step of the wave's basis (lane t holds h_t, act: still in
@@ -62,7 +74,13 @@ __device__ __forceinline__ bool cw_eliminate(uint32_t h[6], bool& act, int p, ui
   if (cand == 0ull) return false;
   const int t0 = __builtin_ctzll(cand);
   const uint32_t e0 = (uint32_t)__builtin_amdgcn_readlane((int)e, t0);
-  const uint32_t c = f8_mul(e, (kF8Inv >> (3 * e0)) & 7u);   // h_t(p) / h_t0(p)
+  // h_t(p) / h_t0(p): row 1 / e0 of the multiplication table (wave-uniform),
+  // read at e.
+  static constexpr uint32_t kRows[8] = {f8_mul_row(0), f8_mul_row(1), f8_mul_row(2),
+                                        f8_mul_row(3), f8_mul_row(4), f8_mul_row(5),
+                                        f8_mul_row(6), f8_mul_row(7)};
+  const uint32_t row = kRows[(kF8Inv >> (3 * e0)) & 7u];
+  const uint32_t c = __builtin_amdgcn_ubfe(row, 3u * e, 3u);
   const bool upd = act && lane != (uint32_t)t0;
   uint32_t H[6];
 #pragma unroll
