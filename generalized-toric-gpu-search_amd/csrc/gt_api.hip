@@ -612,9 +612,11 @@ int gt_get_md_walk_record(gt_ctx* c, uint64_t* out, int32_t n) {
   if (n != c->walk_n) return fail("the last recorded call had %d codes, not %d", c->walk_n, n);
   if (n == 0) return 0;
   GT_OK(c->activate());
-  hipStream_t s = c->last_s ? c->last_s : c->stream;
-  GT_HIP(hipMemcpyAsync(out, c->walkbuf.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
-  GT_HIP(hipStreamSynchronize(s));
+  // The recorded call may have run on a caller's stream that no longer
+  // exists: wait for the device, then copy (a diagnostic call, not a hot
+  // path).
+  GT_HIP(hipDeviceSynchronize());
+  GT_HIP(hipMemcpy(out, c->walkbuf.p, (size_t)n * 8, hipMemcpyDeviceToHost));
   return 0;
 }
 
