@@ -23,10 +23,15 @@ static_assert(3 * 3 * (kServeMaxK - 3) >= 45, "the rows of a whole small request
 // only the first phase when the first level is already level 2).  A later
 // phase's chunks are walked only when the earlier phases did not certify
 // the request (serve_kernel.hip).
-// A subcode of walk dimension at most kServeWholeW (at most 37,449
-// codewords) is one chunk, walked whole; larger ones are chunked as in the
-// batch walk (a chunk also pays a claim, a lookup, the rows and a fold).
-constexpr int kServeWholeW = 6;
+// A subcode of walk dimension at most kServeWholeW (at most 299,593
+// codewords, about 90 us for one wave) is one chunk, walked whole; larger
+// ones are chunked as in the batch walk (a chunk also pays a claim, a
+// lookup, the rows and a fold, and a later phase's chunk waits for the
+// earlier phases: fewer chunks, fewer waiting workers).  On the 20,000
+// request k = 3..10 stream: 6 -> 7 took the burst capacity from 2.95 to
+// 3.3-3.4 M requests/s and p99 at 0.7x from 0.065 to 0.057-0.059 ms; 8 made
+// a k = 12 request's first phase three 2.4 M-codeword single-wave chunks.
+constexpr int kServeWholeW = 7;
 __host__ __device__ __forceinline__ uint32_t serve_sub_chunks(int w, int L) {
   return (w <= kServeWholeW) ? 1u : (uint32_t)code_chunks(w, L);
 }

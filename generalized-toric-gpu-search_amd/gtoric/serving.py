@@ -24,7 +24,7 @@ MAX_K = 21
 FUSED_MAX_K = 8      # requests of at most 8 points: one chunk, searched whole by one wave
 
 
-SERVE_WHOLE_W = 6    # a phase's subcode of at most this dimension is one chunk
+SERVE_WHOLE_W = 7    # a phase's subcode of at most this dimension is one chunk
 
 
 def served_chunks(k: int, L: int) -> int:
