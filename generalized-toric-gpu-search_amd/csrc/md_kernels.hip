@@ -282,8 +282,18 @@ __device__ __forceinline__ void md_list_plan(const int32_t* __restrict__ offsets
                                              const MdAux& ax, uint32_t* ctrl) {
   constexpr int kB = kMaxKExact;
   __shared__ uint32_t cnt[kB + 1];
+  // Per-bucket chunk counts and units, staged by one thread each so that the
+  // serial scan below reads LDS only (not a chain of dependent global loads).
+  __shared__ uint32_t cc_s[kB], c1_s[kB];
+  __shared__ uint64_t un_s[kB];
   const int tid = threadIdx.x;
   if (tid <= kB) cnt[tid] = 0u;
+  if (tid < kB) {
+    const int k = min(kB + 1 - tid, kB);
+    cc_s[tid] = p->chunks[pass][k];
+    c1_s[tid] = p->chunks1[k];
+    un_s[tid] = anchor_units(k, code_level(p->level, k, pass));
+  }
   if (tid < kClaimShards) claims[tid * kClaimStride] = 0u;
   if (tid == 0 && pass == kPassFirst) {   // the later lists of this call
     *ax.mid_count = 0u;
@@ -302,17 +312,16 @@ __device__ __forceinline__ void md_list_plan(const int32_t* __restrict__ offsets
     uint64_t eu = 0, ec = 0, e1 = 0;
     uint32_t ep = 0;
     for (int b = 0; b < kB; ++b) {
-      const int k = min(kB + 1 - b, kB);
       const uint32_t t = cnt[b];
-      const uint64_t cc = p->chunks[pass][k];
+      const uint64_t cc = cc_s[b];
       plan->ub[b] = eu;
       plan->cb[b] = (uint32_t)min(ec, 1ull << 31);
       plan->pb[b] = ep;
       plan->cc[b] = (uint32_t)cc;
       cnt[b] = ep;
-      eu += (uint64_t)t * anchor_units(k, code_level(p->level, k, pass));
+      eu += (uint64_t)t * un_s[b];
       ec = min(ec + (uint64_t)t * cc, 1ull << 31);
-      e1 = min(e1 + (uint64_t)t * p->chunks1[k], 1ull << 31);
+      e1 = min(e1 + (uint64_t)t * c1_s[b], 1ull << 31);
       ep += t;
     }
     const bool refused = ec >= (1ull << 31) || e1 >= (1ull << 31);
