@@ -616,6 +616,8 @@ __global__ __launch_bounds__(256) void md_main_exact_kernel(
   __shared__ MdShared sh;
   // The first-pass list was read by md_prepl: empty for the next call.
   if (pass == kPassFirst && blockIdx.x == 0 && threadIdx.x == 0) *ax.big_count = 0u;
+  // No listed code (the usual final pass): leave before staging the plan.
+  if (__builtin_amdgcn_readfirstlane(plan->pb[kMaxKExact]) == 0u) return;
   md_load_shared(sh, plan, p);
   // First bucket of small codes (k <= small_k of the pass: subcodes without
   // a piece above the tiny part).
@@ -741,6 +743,8 @@ __global__ __launch_bounds__(256) void md_main_thr_kernel(
     int32_t* __restrict__ out, const MdParams* __restrict__ p, MdAux ax, int32_t pass) {
   __shared__ MdShared sh;
   if (pass == kPassFirst && blockIdx.x == 0 && threadIdx.x == 0) *ax.big_count = 0u;
+  // No listed code (the usual final pass): leave before staging the plan.
+  if (__builtin_amdgcn_readfirstlane(plan->pb[kMaxKExact]) == 0u) return;
   md_load_shared(sh, plan, p);
   const uint32_t lane = lane_id();
   const uint32_t total = __builtin_amdgcn_readfirstlane(sh.plan[offsetof(MdPlan, cb) / 4 + kMaxKExact]);
